@@ -1,0 +1,156 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/ba_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from nav24_amd.abi import NavBaOptions, NavBaProblem, Options, Problem, ResultBuffers, Result
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libba_oracle.so")
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P, O = C.POINTER(NavBaProblem), C.POINTER(NavBaOptions)
+        L.oracle_ba_solve.argtypes = [P, O, C.c_void_p, _dp, _i32p]
+        L.oracle_ba_solve.restype = C.c_int32
+        L.oracle_cost.argtypes = [P, O, _dp, _dp, _dp, _u8p]
+        L.oracle_cost.restype = C.c_double
+        L.oracle_linear_blocks.argtypes = [P, O, _dp, _dp, _dp, _dp]
+        L.oracle_linear_blocks.restype = C.c_double
+        L.oracle_reduced_system.argtypes = [P, O, C.c_double, _dp, _dp, C.c_int32]
+        L.oracle_reduced_system.restype = C.c_int32
+        L.oracle_schur_step.argtypes = [P, O, C.c_double, _dp, C.c_int32]
+        L.oracle_schur_step.restype = C.c_int32
+        L.oracle_dense_step.argtypes = [P, O, C.c_double, _dp, C.c_int32]
+        L.oracle_dense_step.restype = C.c_int32
+        L.oracle_quat_from_R.argtypes = [_dp, _dp]
+        L.oracle_R_from_quat.argtypes = [_dp, _dp]
+        L.oracle_se3_exp.argtypes = [_dp, _dp, _dp]
+        L.oracle_se3_oplus.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.oracle_huber.argtypes = [C.c_double, C.c_double, _dp]
+        L.oracle_inv3.argtypes = [_dp, _dp]
+        L.oracle_edge.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  _dp, _dp, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def solve(prob: Problem, opts: Options, with_history: bool = False):
+    ps, os_ = prob.c_struct(), opts.c_struct()
+    rb = ResultBuffers(prob, opts.n_iter)
+    lam = np.zeros(max(opts.n_iter, 1), np.float64)
+    tri = np.zeros(max(opts.n_iter, 1), np.int32)
+    st = lib().oracle_ba_solve(C.byref(ps), C.byref(os_), C.addressof(rb.c), _d(lam),
+                               tri.ctypes.data_as(_i32p))
+    res = rb.result(st)
+    if with_history:
+        return res, lam[: res.iterations], tri[: res.iterations]
+    return res
+
+
+def cost(prob: Problem, opts: Options, T_cw: np.ndarray, X: np.ndarray):
+    T = np.ascontiguousarray(T_cw, np.float64)
+    Xa = np.ascontiguousarray(X, np.float64)
+    c2 = np.zeros(prob.n_obs, np.float64)
+    fl = np.zeros(prob.n_obs, np.uint8)
+    ps, os_ = prob.c_struct(), opts.c_struct()
+    total = lib().oracle_cost(C.byref(ps), C.byref(os_), _d(T), _d(Xa), _d(c2), fl.ctypes.data_as(_u8p))
+    return total, c2, fl
+
+
+def linear_blocks(prob: Problem, opts: Options):
+    Hpp = np.zeros((prob.n_cam, 6, 6)); bp = np.zeros((prob.n_cam, 6))
+    Hll = np.zeros((prob.n_pt, 3, 3)); bl = np.zeros((prob.n_pt, 3))
+    ps, os_ = prob.c_struct(), opts.c_struct()
+    chi = lib().oracle_linear_blocks(C.byref(ps), C.byref(os_), _d(Hpp), _d(bp), _d(Hll), _d(bl))
+    return chi, Hpp, bp, Hll, bl
+
+
+def reduced_system(prob: Problem, opts: Options, lam: float):
+    ps, os_ = prob.c_struct(), opts.c_struct()
+    n = lib().oracle_reduced_system(C.byref(ps), C.byref(os_), lam, None, None, 0)
+    S = np.zeros((max(n, 1), max(n, 1))); g = np.zeros(max(n, 1))
+    lib().oracle_reduced_system(C.byref(ps), C.byref(os_), lam, _d(S), _d(g), n)
+    return S[:n, :n], g[:n]
+
+
+def schur_step(prob: Problem, opts: Options, lam: float):
+    ps, os_ = prob.c_struct(), opts.c_struct()
+    nmax = 6 * prob.n_cam + 3 * prob.n_pt
+    dx = np.zeros(max(nmax, 1))
+    n = lib().oracle_schur_step(C.byref(ps), C.byref(os_), lam, _d(dx), nmax)
+    return None if n < 0 else dx[:n]
+
+
+def dense_step(prob: Problem, opts: Options, lam: float):
+    ps, os_ = prob.c_struct(), opts.c_struct()
+    nmax = 6 * prob.n_cam + 3 * prob.n_pt
+    dx = np.zeros(max(nmax, 1))
+    n = lib().oracle_dense_step(C.byref(ps), C.byref(os_), lam, _d(dx), nmax)
+    return None if n < 0 else dx[:n]
+
+
+def quat_from_R(R):
+    R = np.ascontiguousarray(R, np.float64).reshape(9); q = np.zeros(4)
+    lib().oracle_quat_from_R(_d(R), _d(q)); return q
+
+
+def R_from_quat(q):
+    q = np.ascontiguousarray(q, np.float64).reshape(4); R = np.zeros(9)
+    lib().oracle_R_from_quat(_d(q), _d(R)); return R.reshape(3, 3)
+
+
+def se3_exp(xi):
+    xi = np.ascontiguousarray(xi, np.float64).reshape(6); q = np.zeros(4); t = np.zeros(3)
+    lib().oracle_se3_exp(_d(xi), _d(q), _d(t)); return q, t
+
+
+def se3_oplus(xi, q, t):
+    xi = np.ascontiguousarray(xi, np.float64).reshape(6)
+    q = np.ascontiguousarray(q, np.float64).reshape(4); t = np.ascontiguousarray(t, np.float64).reshape(3)
+    qo = np.zeros(4); to = np.zeros(3)
+    lib().oracle_se3_oplus(_d(xi), _d(q), _d(t), _d(qo), _d(to)); return qo, to
+
+
+def huber(s, delta):
+    rho = np.zeros(3); lib().oracle_huber(float(s), float(delta), _d(rho)); return rho
+
+
+def inv3(A):
+    A = np.ascontiguousarray(A, np.float64).reshape(9); B = np.zeros(9)
+    lib().oracle_inv3(_d(A), _d(B)); return B.reshape(3, 3)
+
+
+def edge(q, t, X, K, uv):
+    q = np.ascontiguousarray(q, np.float64); t = np.ascontiguousarray(t, np.float64)
+    X = np.ascontiguousarray(X, np.float64); uv = np.ascontiguousarray(uv, np.float64)
+    e = np.zeros(2); JX = np.zeros(6); Jxi = np.zeros(12); Xc = np.zeros(3)
+    lib().oracle_edge(_d(q), _d(t), _d(X), K[0], K[1], K[2], K[3], _d(uv), _d(e), _d(JX), _d(Jxi), _d(Xc))
+    return e, JX.reshape(2, 3), Jxi.reshape(2, 6), Xc
