@@ -1,0 +1,139 @@
+"""Python binding of the MI355X BA back-end (libnav24_ba.so, include/nav_ba.h).
+
+`GraphOptimHip` mirrors the reference back-end `NAV24::BE::GraphOptim`
+(core/backEnd/BE_GraphOptim.hpp:19-34): `solve(problem)` refines the poses
+and points of a problem in one call.  The problem is the flat structure of
+arrays of `nav24_amd.abi.Problem` instead of PR_VBA's object graph
+(core/backEnd/Problem.hpp:41-47).  Errors raise `BAError` carrying the C
+status; there is no CPU fallback — without the built library or a gfx950
+device the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from ._loader import load_library
+from .abi import (NAV_BA_EMPTY, NAV_BA_OK, STATUS_NAMES, NavBaOptions, NavBaProblem, NavBaResult,
+                  Options, Problem, Result, ResultBuffers)
+
+_dp = C.POINTER(C.c_double)
+
+# Every entry point declared in include/nav_ba.h and include/nav_ba_debug.h.
+EXPORTED = [
+    "nav_ba_version", "nav_ba_create", "nav_ba_destroy", "nav_ba_last_error",
+    "nav_ba_default_options", "nav_ba_solve", "nav_ba_prepare", "nav_ba_run", "nav_ba_fetch",
+    "nav_ba_debug_reduced_system",
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        L = load_library("libnav24_ba.so")
+        H = C.c_void_p
+        L.nav_ba_version.restype = C.c_char_p
+        L.nav_ba_version.argtypes = []
+        L.nav_ba_create.argtypes = [C.POINTER(H), C.c_int32]
+        L.nav_ba_create.restype = C.c_int32
+        L.nav_ba_destroy.argtypes = [H]
+        L.nav_ba_destroy.restype = None
+        L.nav_ba_last_error.argtypes = [H]
+        L.nav_ba_last_error.restype = C.c_char_p
+        L.nav_ba_default_options.argtypes = [C.POINTER(NavBaOptions)]
+        L.nav_ba_default_options.restype = None
+        L.nav_ba_solve.argtypes = [H, C.POINTER(NavBaProblem), C.POINTER(NavBaOptions), C.POINTER(NavBaResult)]
+        L.nav_ba_solve.restype = C.c_int32
+        L.nav_ba_prepare.argtypes = [H, C.POINTER(NavBaProblem), C.POINTER(NavBaOptions)]
+        L.nav_ba_prepare.restype = C.c_int32
+        L.nav_ba_run.argtypes = [H]
+        L.nav_ba_run.restype = C.c_int32
+        L.nav_ba_fetch.argtypes = [H, C.POINTER(NavBaResult)]
+        L.nav_ba_fetch.restype = C.c_int32
+        L.nav_ba_debug_reduced_system.argtypes = [H, C.c_double, _dp, _dp, C.c_int32, C.POINTER(C.c_int32)]
+        L.nav_ba_debug_reduced_system.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+class BAError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def default_options() -> Options:
+    o = NavBaOptions()
+    lib().nav_ba_default_options(C.byref(o))
+    return Options(n_iter=o.n_iter, robust=bool(o.robust), huber_delta=o.huber_delta,
+                   lambda_init=o.lambda_init, precision=o.precision, compat_fp32_io=bool(o.compat_fp32_io),
+                   outlier_chi2=o.outlier_chi2, max_trials=o.max_trials)
+
+
+class GraphOptimHip:
+    """GPU bundle-adjustment back-end on one gfx950 device (one HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        st = lib().nav_ba_create(C.byref(self._h), device)
+        if st != NAV_BA_OK:
+            raise BAError(st, f"nav_ba_create(device={device}) failed")
+        self._keep = None
+        self._prob: Optional[Problem] = None
+        self._opts: Optional[Options] = None
+
+    def close(self):
+        if self._h:
+            lib().nav_ba_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int, allow_empty: bool = False) -> int:
+        if st == NAV_BA_OK or (allow_empty and st == NAV_BA_EMPTY):
+            return st
+        raise BAError(st, lib().nav_ba_last_error(self._h).decode())
+
+    def solve(self, prob: Problem, opts: Options) -> Result:
+        """GraphOptim::solve (BE_GraphOptim.cpp:31-77) in one call."""
+        ps, os_ = prob.c_struct(), opts.c_struct()
+        rb = ResultBuffers(prob, opts.n_iter)
+        st = lib().nav_ba_solve(self._h, C.byref(ps), C.byref(os_), C.byref(rb.c))
+        self._check(st, allow_empty=True)
+        return rb.result(st)
+
+    def prepare(self, prob: Problem, opts: Options) -> None:
+        ps, os_ = prob.c_struct(), opts.c_struct()
+        self._keep = (ps, os_)
+        self._prob, self._opts = prob, opts
+        self._check(lib().nav_ba_prepare(self._h, C.byref(ps), C.byref(os_)))
+
+    def run(self) -> int:
+        return self._check(lib().nav_ba_run(self._h), allow_empty=True)
+
+    def fetch(self) -> Result:
+        rb = ResultBuffers(self._prob, self._opts.n_iter)
+        st = lib().nav_ba_fetch(self._h, C.byref(rb.c))
+        self._check(st, allow_empty=True)
+        return rb.result(st)
+
+    def debug_reduced_system(self, lam: float):
+        n = C.c_int32(0)
+        self._check(lib().nav_ba_debug_reduced_system(self._h, lam, None, None, 0, C.byref(n)), allow_empty=True)
+        S = np.zeros((max(n.value, 1), max(n.value, 1)))
+        g = np.zeros(max(n.value, 1))
+        self._check(lib().nav_ba_debug_reduced_system(self._h, lam, S.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
+                                                      n.value, C.byref(n)), allow_empty=True)
+        return S[: n.value, : n.value], g[: n.value]
+
+
+def version() -> str:
+    return lib().nav_ba_version().decode()

@@ -1,0 +1,248 @@
+// ba_device.hpp — device-side math of the BA kernels (gfx950).
+//
+// Everything here is templated on the compute type T (double for the plain
+// FP64 path, float for the mixed path); master state stays FP64.
+//
+// Reference semantics restated (file:line):
+//   edge error     e = z - pi(T X)          OptimizableTypes.hpp:102-107, Pinhole.cpp:30-36
+//   Jacobians      P = -Jpi(Xc); J_X = P R; J_xi = P [-[Xc]x | I]
+//                                           OptimizableTypes.cpp:140-161, Pinhole.cpp:70-80
+//   Huber          rho(s) = s | 2 d sqrt(s) - d^2,  rho'(s) = 1 | d / sqrt(s)   (g2o kernel the
+//                                           reference enables at BE_GraphOptim.cpp:191-195)
+//   SE3 update     T <- exp(xi) T, quaternion w >= 0 and unit norm (g2o VertexSE3Expmap,
+//                                           the vertex of OptimizableTypes.hpp:221-227)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace navba {
+
+constexpr int kWave = 64;
+
+// Camera as used by the kernels: R row-major and t, 12 values (FP64 in HBM).
+struct CamRT { double R[9]; double t[3]; };
+
+// Pose state: unit quaternion (x, y, z, w) and translation; 8 doubles.
+struct PoseQT { double q[4]; double t[3]; double pad; };
+
+struct Intrinsics { double fx, fy, cx, cy; };
+
+// Linearisation of one edge in compute type T.
+template <typename T>
+struct EdgeLin {
+    T e0, e1;      // residual
+    T z;           // camera-frame depth
+    T JX[6];       // 2x3, d e / d X
+    T Jxi[12];     // 2x6, d e / d xi (omega, upsilon)
+};
+
+template <typename T>
+__device__ __forceinline__ void transform(const T R[9], const T t[3], T x, T y, T z, T& xc, T& yc, T& zc) {
+    xc = R[0] * x + R[1] * y + R[2] * z + t[0];
+    yc = R[3] * x + R[4] * y + R[5] * z + t[1];
+    zc = R[6] * x + R[7] * y + R[8] * z + t[2];
+}
+
+// Residual only.  Returns chi2 = e'e; depth in zc.
+template <typename T>
+__device__ __forceinline__ T edge_residual(const T R[9], const T t[3], T x, T y, T z,
+                                           T fx, T fy, T cx, T cy, T u, T v, T& e0, T& e1, T& zc) {
+    T xc, yc;
+    transform(R, t, x, y, z, xc, yc, zc);
+    e0 = u - (fx * xc / zc + cx);
+    e1 = v - (fy * yc / zc + cy);
+    return e0 * e0 + e1 * e1;
+}
+
+// Full linearisation: residual and both Jacobians.  The constant zeros of
+// P = -Jpi and of [-[Xc]x | I] are folded out.
+template <typename T>
+__device__ __forceinline__ T edge_linearize(const T R[9], const T t[3], T x, T y, T z,
+                                            T fx, T fy, T cx, T cy, T u, T v, EdgeLin<T>& L) {
+    T xc, yc, zc;
+    transform(R, t, x, y, z, xc, yc, zc);
+    L.e0 = u - (fx * xc / zc + cx);
+    L.e1 = v - (fy * yc / zc + cy);
+    L.z = zc;
+    const T iz = T(1) / zc;
+    const T P00 = -(fx * iz);
+    const T P02 = fx * xc * iz * iz;
+    const T P11 = -(fy * iz);
+    const T P12 = fy * yc * iz * iz;
+    // J_X = P R
+    L.JX[0] = P00 * R[0] + P02 * R[6];
+    L.JX[1] = P00 * R[1] + P02 * R[7];
+    L.JX[2] = P00 * R[2] + P02 * R[8];
+    L.JX[3] = P11 * R[3] + P12 * R[6];
+    L.JX[4] = P11 * R[4] + P12 * R[7];
+    L.JX[5] = P11 * R[5] + P12 * R[8];
+    // J_xi = P [[0, z, -y, 1, 0, 0], [-z, 0, x, 0, 1, 0], [y, -x, 0, 0, 0, 1]]
+    L.Jxi[0] = P02 * yc;
+    L.Jxi[1] = P00 * zc - P02 * xc;
+    L.Jxi[2] = -(P00 * yc);
+    L.Jxi[3] = P00;
+    L.Jxi[4] = T(0);
+    L.Jxi[5] = P02;
+    L.Jxi[6] = P12 * yc - P11 * zc;
+    L.Jxi[7] = -(P12 * xc);
+    L.Jxi[8] = P11 * xc;
+    L.Jxi[9] = T(0);
+    L.Jxi[10] = P11;
+    L.Jxi[11] = P12;
+    return L.e0 * L.e0 + L.e1 * L.e1;
+}
+
+// Huber: returns rho(s); w = rho'(s).  Without the kernel rho = s, w = 1.
+template <typename T>
+__device__ __forceinline__ T robustify(T s, int robust, T delta, T& w) {
+    if (!robust) { w = T(1); return s; }
+    const T d2 = delta * delta;
+    if (s <= d2) { w = T(1); return s; }
+    const T r = sqrt(s);
+    w = delta / r;
+    return T(2) * r * delta - d2;
+}
+
+// 3x3 symmetric inverse by the adjugate; A given as (a00 a01 a02 a11 a12 a22),
+// result in the same packing.
+__device__ __forceinline__ void inv3_sym(const double a[6], double o[6]) {
+    const double a00 = a[0], a01 = a[1], a02 = a[2], a11 = a[3], a12 = a[4], a22 = a[5];
+    const double c00 = a11 * a22 - a12 * a12;
+    const double c01 = a12 * a02 - a01 * a22;
+    const double c02 = a01 * a12 - a11 * a02;
+    const double det = a00 * c00 + a01 * c01 + a02 * c02;
+    const double id = 1.0 / det;
+    o[0] = c00 * id;
+    o[1] = c01 * id;
+    o[2] = c02 * id;
+    o[3] = (a00 * a22 - a02 * a02) * id;
+    o[4] = (a02 * a01 - a00 * a12) * id;
+    o[5] = (a00 * a11 - a01 * a01) * id;
+}
+
+// ---- quaternions (FP64) ----
+__device__ __forceinline__ void quat_to_R(const double q[4], double R[9]) {
+    const double tx = 2 * q[0], ty = 2 * q[1], tz = 2 * q[2];
+    const double twx = tx * q[3], twy = ty * q[3], twz = tz * q[3];
+    const double txx = tx * q[0], txy = ty * q[0], txz = tz * q[0];
+    const double tyy = ty * q[1], tyz = tz * q[1], tzz = tz * q[2];
+    R[0] = 1 - (tyy + tzz); R[1] = txy - twz;       R[2] = txz + twy;
+    R[3] = txy + twz;       R[4] = 1 - (txx + tzz); R[5] = tyz - twx;
+    R[6] = txz - twy;       R[7] = tyz + twx;       R[8] = 1 - (txx + tyy);
+}
+
+// Rotation matrix -> quaternion (Shoemake 1987, trace / largest-diagonal branches).
+__device__ __forceinline__ void R_to_quat(const double R[9], double q[4]) {
+    const double tr = R[0] + R[4] + R[8];
+    if (tr > 0.0) {
+        double s = sqrt(tr + 1.0);
+        q[3] = 0.5 * s;
+        s = 0.5 / s;
+        q[0] = (R[7] - R[5]) * s;
+        q[1] = (R[2] - R[6]) * s;
+        q[2] = (R[3] - R[1]) * s;
+    } else {
+        int i = 0;
+        if (R[4] > R[0]) i = 1;
+        if (R[8] > R[3 * i + i]) i = 2;
+        const int j = (i + 1) % 3, k = (j + 1) % 3;
+        double s = sqrt(R[3 * i + i] - R[3 * j + j] - R[3 * k + k] + 1.0);
+        double c[3];
+        c[i] = 0.5 * s;
+        s = 0.5 / s;
+        q[3] = (R[3 * k + j] - R[3 * j + k]) * s;
+        c[j] = (R[3 * j + i] + R[3 * i + j]) * s;
+        c[k] = (R[3 * k + i] + R[3 * i + k]) * s;
+        q[0] = c[0]; q[1] = c[1]; q[2] = c[2];
+    }
+}
+
+__device__ __forceinline__ void quat_normalize_pos(double q[4]) {
+    if (q[3] < 0) { q[0] = -q[0]; q[1] = -q[1]; q[2] = -q[2]; q[3] = -q[3]; }
+    const double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    q[0] /= n; q[1] /= n; q[2] /= n; q[3] /= n;
+}
+
+__device__ __forceinline__ void quat_rotate(const double q[4], const double v[3], double o[3]) {
+    double ux = q[1] * v[2] - q[2] * v[1];
+    double uy = q[2] * v[0] - q[0] * v[2];
+    double uz = q[0] * v[1] - q[1] * v[0];
+    ux += ux; uy += uy; uz += uz;
+    o[0] = v[0] + q[3] * ux + (q[1] * uz - q[2] * uy);
+    o[1] = v[1] + q[3] * uy + (q[2] * ux - q[0] * uz);
+    o[2] = v[2] + q[3] * uz + (q[0] * uy - q[1] * ux);
+}
+
+// T_out = exp(xi) * T_in.  exp: theta < 1e-5 -> R = I + W + W^2, V = R;
+// otherwise Rodrigues R and left Jacobian V.
+__device__ inline void se3_oplus(const double xi[6], const PoseQT& in, PoseQT& out) {
+    const double wx = xi[0], wy = xi[1], wz = xi[2];
+    const double theta = sqrt(wx * wx + wy * wy + wz * wz);
+    const double W[9] = {0, -wz, wy, wz, 0, -wx, -wy, wx, 0};
+    double W2[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            W2[3 * i + j] = W[3 * i] * W[j] + W[3 * i + 1] * W[3 + j] + W[3 * i + 2] * W[6 + j];
+    double R[9], V[9];
+    if (theta < 0.00001) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { R[k] = ((k % 4 == 0) ? 1.0 : 0.0) + W[k] + W2[k]; V[k] = R[k]; }
+    } else {
+        double s, c;
+        sincos(theta, &s, &c);
+        const double a = s / theta;
+        const double b = (1 - c) / (theta * theta);
+        const double cc = (theta - s) / (theta * theta * theta);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const double I = (k % 4 == 0) ? 1.0 : 0.0;
+            R[k] = I + a * W[k] + b * W2[k];
+            V[k] = I + b * W[k] + cc * W2[k];
+        }
+    }
+    double qi[4];
+    R_to_quat(R, qi);
+    quat_normalize_pos(qi);
+    double ti[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ti[i] = V[3 * i] * xi[3] + V[3 * i + 1] * xi[4] + V[3 * i + 2] * xi[5];
+    double rt[3];
+    quat_rotate(qi, in.t, rt);
+    out.t[0] = ti[0] + rt[0]; out.t[1] = ti[1] + rt[1]; out.t[2] = ti[2] + rt[2];
+    const double* b = in.q;
+    out.q[3] = qi[3] * b[3] - qi[0] * b[0] - qi[1] * b[1] - qi[2] * b[2];
+    out.q[0] = qi[3] * b[0] + qi[0] * b[3] + qi[1] * b[2] - qi[2] * b[1];
+    out.q[1] = qi[3] * b[1] + qi[1] * b[3] + qi[2] * b[0] - qi[0] * b[2];
+    out.q[2] = qi[3] * b[2] + qi[2] * b[3] + qi[0] * b[1] - qi[1] * b[0];
+    quat_normalize_pos(out.q);
+    out.pad = 0.0;
+}
+
+// ---- reductions (fixed order: deterministic) ----
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum over a block of NT threads; result valid in thread 0.  `sh` needs NT/64 slots.
+template <typename T, int NT>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    T r = T(0);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NT / kWave; ++i) r += sh[i];
+    }
+    return r;
+}
+
+}  // namespace navba
