@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""bench.py — bundle-adjustment LM throughput on one MI355X.
+
+One step = one full LM solve (nav_ba_run) of the workload, with the problem
+already resident in HBM (nav_ba_prepare runs before the timed region).  The
+headline is the plain-FP64 GPU path on the random-walk workload (60 keyframes,
+20 000 points, ~565 k observations, 20 LM iterations); the JSON line also
+carries observations/s, the mixed-precision path on the same problem (final
+cost relative to FP64), the per-stage roofline of the dominant kernel, and
+the FP64 CPU oracle timed on the host as the baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BA LM iters/sec + obs/sec at equal final cost; mixed/batched vs plain FP64"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 matrix
+
+WORKLOAD_NAMES = {
+    0: "circle-10kf-1500pt", 1: "randomwalk-60kf-20000pt", 2: "loop-300kf-150000pt",
+}
+
+
+def problem_stats(prob):
+    """Sizes that the algorithmic byte / flop counts use."""
+    kp = np.bincount(prob.obs_pt, minlength=prob.n_pt)
+    active = kp > 0
+    n_pt = int(active.sum())
+    free_cam = np.ones(prob.n_cam, bool) if prob.cam_fixed is None else prob.cam_fixed == 0
+    # observations on free cameras per point (Schur pair structure)
+    kf = np.bincount(prob.obs_pt[free_cam[prob.obs_cam]], minlength=prob.n_pt).astype(np.float64)
+    return dict(n_obs=prob.n_obs, n_pt=n_pt, n_cam=prob.n_cam, n_fc=int(free_cam.sum()),
+                sum_k=float(kf.sum()), sum_pairs=float((kf * (kf + 1) / 2).sum()))
+
+
+def stage_work(st, fp64=True):
+    """Algorithmic bytes (HBM-bound stages) or flops (Schur rows) per launch."""
+    no, npt, nc = st["n_obs"], st["n_pt"], st["n_cam"]
+    if fp64:
+        lin_pt = 32 * no + 113 * npt + 96 * nc     # o_cam, o_pt, uv, w out | X, Hll+bl out, ptr, free | cams
+        lin_cam = 32 * no + 32 * npt + 96 * nc     # index, o_pt, uv, w | X | cams
+        cost = 24 * no + 32 * npt + 96 * nc        # o_cam, o_pt, uv | X | cams
+    else:
+        lin_pt = 28 * no + 77 * npt + 96 * nc
+        lin_cam = 28 * no + 32 * npt + 96 * nc
+        cost = 24 * no + 32 * npt + 96 * nc
+    schur_flops = 108 * st["sum_k"] + 36 * st["sum_k"] + 216 * st["sum_pairs"]
+    return {
+        "lin_pt": ("hbm", lin_pt), "lin_cam": ("hbm", lin_cam), "cost": ("hbm", cost),
+        "schur_row": ("mfma", schur_flops),
+    }
+
+
+def roofline_entry(timing, st, fp64):
+    work = stage_work(st, fp64)
+    total = {k: v[0] for k, v in timing.items() if v[1] > 0}
+    dominant = max(total, key=total.get)
+    cand = dominant if dominant in work else max((k for k in work if k in total), key=lambda k: total[k])
+    ms, n = timing[cand]
+    avg_s = ms / n / 1e3
+    bound, amount = work[cand]
+    if bound == "hbm":
+        achieved = amount / avg_s / 1e9
+        peak, unit = HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved = amount / avg_s / 1e12
+        peak, unit = (FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS), "TFLOP/s"
+    return {
+        "kernel": cand, "dominant_stage": dominant, "bound": bound, "achieved": achieved, "peak": peak,
+        "unit": unit, "frac": achieved / peak, "traffic": None, "avg_launch_us": avg_s * 1e6,
+        "algorithmic_per_launch": amount,
+        "stages_ms_per_step": {k: round(v[0], 4) for k, v in timing.items() if v[1] > 0},
+    }
+
+
+def cpu_baseline(prob, opts):
+    from oracle import oracle  # checker and CPU baseline only
+    t0 = time.perf_counter()
+    r = oracle.solve(prob, opts)
+    dt = time.perf_counter() - t0
+    return {
+        "value": r.iterations / dt, "unit": "LM iterations/s", "cores": 1, "kind": "port",
+        "sample": f"one full solve of the same workload ({r.iterations} LM iterations, "
+                  f"{prob.n_obs} observations), FP64 oracle, 1 thread, {dt:.2f} s",
+        "obs_per_s": prob.n_obs * r.iterations / dt, "final_cost": r.final_cost,
+        "cpu": _cpu_model(), "host_cpus": os.cpu_count(),
+    }, r
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device synchronisation around the timed region
+    from nav24_amd import problems
+    from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
+    from nav24_amd.ba import GraphOptimHip
+
+    prob = problems.workload(args.workload, seed=0)
+    opts = workload_options(args.workload)
+    opts.precision = NAV_BA_FP64
+    st = problem_stats(prob)
+    solver = GraphOptimHip(0)
+    solver.prepare(prob, opts)
+    for _ in range(args.warmup):
+        solver.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = solver.fetch()
+    iters = res.iterations
+    value = iters * args.steps / dt
+
+    solver.set_timing(True)
+    for _ in range(args.steps):
+        solver.run()
+    timing = solver.timing()
+    solver.set_timing(False)
+    per_step = {k: (v[0] / args.steps, v[1] / args.steps) for k, v in timing.items()}
+    roof = roofline_entry(per_step, st, True)
+    roof["achieved"] = round(roof["achieved"], 3)
+    roof["frac"] = round(roof["frac"], 5)
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "LM iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD_NAMES.get(args.workload, str(args.workload)),
+                   "n_cam": prob.n_cam, "n_pt": prob.n_pt, "n_obs": prob.n_obs,
+                   "lm_iterations": iters, "precision": "fp64", "huber_delta": "sqrt(5.991)",
+                   "lambda_init": 1e-4, "fixed_cameras": 2},
+        "obs_per_s": round(prob.n_obs * value, 1),
+        "final_cost": res.final_cost, "trials": res.trials,
+        "roofline": roof,
+    }
+
+    if not args.no_mixed:
+        mopts = workload_options(args.workload)
+        mopts.precision = NAV_BA_MIXED
+        ms = GraphOptimHip(0)
+        ms.prepare(prob, mopts)
+        for _ in range(args.warmup):
+            ms.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ms.run()
+        torch.cuda.synchronize()
+        mdt = time.perf_counter() - t0
+        mres = ms.fetch()
+        line["mixed"] = {
+            "value": round(mres.iterations * args.steps / mdt, 3), "unit": "LM iterations/s",
+            "ms_per_step": round(mdt / args.steps * 1e3, 3), "lm_iterations": mres.iterations,
+            "final_cost": mres.final_cost,
+            "final_cost_rel_vs_fp64": abs(mres.final_cost - res.final_cost) / res.final_cost,
+            "speedup_vs_fp64": round((mres.iterations / mdt) / (iters / dt), 3),
+        }
+        ms.close()
+
+    if not args.no_cpu:
+        cb, ref = cpu_baseline(prob, opts)
+        line["cpu_baseline"] = cb
+        line["final_cost_rel_vs_oracle"] = abs(res.final_cost - ref.final_cost) / ref.final_cost
+        line["outlier_flags_identical"] = bool(np.array_equal(res.outlier, ref.outlier))
+    solver.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

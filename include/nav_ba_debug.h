@@ -22,6 +22,35 @@ extern "C" {
 int32_t nav_ba_debug_reduced_system(nav_ba_handle* h, double lambda, double* S, double* g,
                                     int32_t max_n, int32_t* n_out);
 
+/* Stages of the LM solve, for per-stage device timing. */
+enum nav_ba_stage {
+    NAV_BA_ST_LIN_PT = 0,      /* per-observation residual/Jacobian, Hll/bl per point */
+    NAV_BA_ST_LIN_CAM,         /* Hpp/bp per camera (+ chunk reduction) */
+    NAV_BA_ST_LAMBDA,          /* lambda initialisation */
+    NAV_BA_ST_HINV,            /* (Hll + lambda I)^-1 per point */
+    NAV_BA_ST_SCHUR_ROW,       /* reduced camera system rows */
+    NAV_BA_ST_SCHUR_REDUCE,    /* partial sums -> dense S, g */
+    NAV_BA_ST_CHOLESKY,        /* dense factorisation and substitutions */
+    NAV_BA_ST_BACKSUB,         /* point back-substitution and candidate points */
+    NAV_BA_ST_UPDATE_CAM,      /* SE3 exp-map update */
+    NAV_BA_ST_COST,            /* robust chi2 */
+    NAV_BA_ST_ACCEPT,          /* LM decision, commit, status */
+    NAV_BA_ST_FLAGS,           /* final outlier flags and FP64 cost */
+    NAV_BA_N_STAGES
+};
+
+typedef struct nav_ba_timing {
+    double  ms[16];        /* accumulated device time per stage */
+    int32_t launches[16];  /* timed launches per stage */
+} nav_ba_timing;
+
+/* enable != 0: later nav_ba_run calls launch the stages one by one (no
+ * graphs) with HIP events on the handle's stream around each stage, and
+ * accumulate the times; enable == 0 returns to graph replay.  Enabling
+ * clears the accumulators. */
+int32_t nav_ba_set_timing(nav_ba_handle* h, int32_t enable);
+int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,8 +25,15 @@ _dp = C.POINTER(C.c_double)
 EXPORTED = [
     "nav_ba_version", "nav_ba_create", "nav_ba_destroy", "nav_ba_last_error",
     "nav_ba_default_options", "nav_ba_solve", "nav_ba_prepare", "nav_ba_run", "nav_ba_fetch",
-    "nav_ba_debug_reduced_system",
+    "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing",
 ]
+
+STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
+          "backsub", "update_cam", "cost", "accept", "flags"]
+
+
+class NavBaTiming(C.Structure):
+    _fields_ = [("ms", C.c_double * 16), ("launches", C.c_int32 * 16)]
 
 _lib = None
 
@@ -56,6 +63,10 @@ def lib() -> C.CDLL:
         L.nav_ba_fetch.restype = C.c_int32
         L.nav_ba_debug_reduced_system.argtypes = [H, C.c_double, _dp, _dp, C.c_int32, C.POINTER(C.c_int32)]
         L.nav_ba_debug_reduced_system.restype = C.c_int32
+        L.nav_ba_set_timing.argtypes = [H, C.c_int32]
+        L.nav_ba_set_timing.restype = C.c_int32
+        L.nav_ba_get_timing.argtypes = [H, C.POINTER(NavBaTiming)]
+        L.nav_ba_get_timing.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -124,6 +135,16 @@ class GraphOptimHip:
         st = lib().nav_ba_fetch(self._h, C.byref(rb.c))
         self._check(st, allow_empty=True)
         return rb.result(st)
+
+    def set_timing(self, enable: bool) -> None:
+        """Stage-by-stage launches with HIP events around each stage (no graphs)."""
+        self._check(lib().nav_ba_set_timing(self._h, int(bool(enable))))
+
+    def timing(self) -> dict:
+        """{stage: (total_ms, launches)} accumulated since set_timing(True)."""
+        t = NavBaTiming()
+        self._check(lib().nav_ba_get_timing(self._h, C.byref(t)))
+        return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(STAGES)}
 
     def debug_reduced_system(self, lam: float):
         n = C.c_int32(0)

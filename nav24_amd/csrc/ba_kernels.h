@@ -14,7 +14,8 @@ constexpr int kChunk = 1024;       // camera-major observations per Schur / Hpp 
 constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget:
                                    // 160*36*8 B + 2*kMaxFreeCams B < 64 KiB)
 constexpr int kNB = 64;            // dense Cholesky tile
-constexpr int kMaxFreeCams = 8192; // slot map in LDS is int16 [n_fc]
+constexpr int kMaxFreeCams = 8192; // slot map in LDS is int16
+constexpr int kCamLdsMax = 512;    // Schur rows keep the camera table in LDS up to this many cameras
 
 struct alignas(32) D4 { double x, y, z, w; };
 
@@ -42,6 +43,7 @@ struct DevProblem {
     // camera-major lists of free cameras: observation indices ascending
     const int* cam_ptr;      // [n_fc + 1]
     const int* cam_obs;
+    const int* cam_obs_b0;   // per cam_obs entry: first observation of its point on a camera >= this one
     // linearisation tiles (whole points, <= kTile observations or one point)
     const int* tile_obs;     // [n_tiles + 1]
     const int* tile_pt;      // [n_tiles + 1]
@@ -83,12 +85,12 @@ struct DevWork {
     void* hpart;        // [n_hitems * 27] T
     double* Hc;         // [n_fc * 28]: Hpp packed (21) + bp (6)
     double* ptsys;      // [n_pt * 12]: Dinv packed (6) + Dinv*bl (3)
-    void* spart;        // Schur row partials, T
+    double* spart;      // Schur row partials (FP64 for both compute types)
     double* S;          // [n_pad * n_pad]
     double* g;          // [n_pad]
     double* y;          // [n_pad]
     double* dp;         // [n_pad]
-    double* Linv;       // [nt * 64 * 64]
+    double* Lm;         // [n_pad * n_pad] Cholesky factor (lower tiles)
     double* red_cost;   // [n_cost_blocks]
     double* red_scale;  // [1 + n_pt_blocks]
     double* red_aux;    // [n_cost_blocks] outlier counts
@@ -98,7 +100,22 @@ struct DevWork {
     int n_pt_blocks;
 };
 
+__device__ __forceinline__ int sym6(int i, int j) {
+    if (i > j) { const int t = i; i = j; j = t; }
+    return i * 6 - (i * (i - 1)) / 2 + (j - i);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_cam(const CamRT* cam, int c, T R[9], T t[3]) {
+    const double* p = reinterpret_cast<const double*>(cam + c);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = (T)p[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] = (T)p[9 + k];
+}
+
 // Launchers.  T = double (FP64 path) or float (mixed path).
+template <typename T> int prepare_schur_row(const DevProblem& P);  // LDS attribute; < 0 if too large
 void launch_cam_rt(const DevProblem& P, const DevState& s, hipStream_t st);
 void launch_lm_reset(LMDev* lm, double lambda0, hipStream_t st);
 template <typename T> void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);

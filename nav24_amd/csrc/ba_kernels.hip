@@ -24,20 +24,6 @@
 
 namespace navba {
 
-__device__ __forceinline__ int sym6(int i, int j) {
-    if (i > j) { const int t = i; i = j; j = t; }
-    return i * 6 - (i * (i - 1)) / 2 + (j - i);
-}
-
-template <typename T>
-__device__ __forceinline__ void load_cam(const CamRT* cam, int c, T R[9], T t[3]) {
-    const double* p = reinterpret_cast<const double*>(cam + c);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) R[k] = (T)p[k];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) t[k] = (T)p[9 + k];
-}
-
 // ---------------------------------------------------------------- cameras
 __global__ void k_cam_rt(int n, const PoseQT* __restrict__ pose, CamRT* __restrict__ cam) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,136 +255,7 @@ void launch_hinv(const DevProblem& P, const DevWork& W, hipStream_t st) {
     hipLaunchKernelGGL(k_hinv<T>, dim3((P.n_pt + 255) / 256), dim3(256), 0, st, P, (const T*)W.Hl, W.lm, W.ptsys);
 }
 
-// ---------------------------------------------------------------- schur_row
-// One workgroup per (free camera f, column group, chunk of f's observations).
-// Lane = observation a of camera f on free point p:
-//   Ha = Hpl_a (6x3), W = Ha Dinv_p;  coef += Ha Dinv bl;
-//   for every observation b of p on a free camera g >= f:  W Hpl_b^T
-// into the diagonal (lane registers, block-reduced) or the LDS S-row slot of g.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT* __restrict__ cam,
-                                                      const D4* __restrict__ X, const T* __restrict__ w,
-                                                      const double* __restrict__ ptsys, T* __restrict__ spart) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int it = blockIdx.x;
-    const int f = P.sitem_f[it], a0 = P.sitem_a0[it], a1 = P.sitem_a1[it];
-    const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
-    T* acc = reinterpret_cast<T*>(smem);
-    short* slot = reinterpret_cast<short*>(smem + sizeof(T) * 36 * (size_t)ns);
-    const int col0 = P.colptr[f] + g * kMaxSlots;
-    for (int i = threadIdx.x; i < P.n_fc; i += kBlock) slot[i] = -1;
-    for (int i = threadIdx.x; i < ns * 36; i += kBlock) acc[i] = T(0);
-    __syncthreads();
-    for (int s = threadIdx.x; s < ns; s += kBlock) slot[P.colcam[col0 + s]] = (short)s;
-    __syncthreads();
-
-    const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy;
-    T Rf[9], tf[3];
-    load_cam(cam, P.free_cam[f], Rf, tf);
-    const bool g0 = (g == 0);
-    T diag[36], coef[6];
-#pragma unroll
-    for (int k = 0; k < 36; ++k) diag[k] = T(0);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) coef[k] = T(0);
-
-    for (int k = a0 + (int)threadIdx.x; k < a1; k += kBlock) {
-        const int a = P.cam_obs[k];
-        const int p = P.o_pt[a];
-        if (!P.pt_free[p]) continue;
-        const D4 xp = X[p];
-        const T x = (T)xp.x, y = (T)xp.y, z = (T)xp.z;
-        EdgeLin<T> La;
-        edge_linearize(Rf, tf, x, y, z, fx, fy, cx, cy, T(0), T(0), La);
-        const T wa = w[a];
-        T Ha[18];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const T A0 = La.Jxi[i] * wa, A1 = La.Jxi[6 + i] * wa;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) Ha[3 * i + j] = A0 * La.JX[j] + A1 * La.JX[3 + j];
-        }
-        const double* ps = ptsys + 12 * (size_t)p;
-        const T d00 = (T)ps[0], d01 = (T)ps[1], d02 = (T)ps[2], d11 = (T)ps[3], d12 = (T)ps[4], d22 = (T)ps[5];
-        T Wm[18];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const T h0 = Ha[3 * i], h1 = Ha[3 * i + 1], h2 = Ha[3 * i + 2];
-            Wm[3 * i + 0] = h0 * d00 + h1 * d01 + h2 * d02;
-            Wm[3 * i + 1] = h0 * d01 + h1 * d11 + h2 * d12;
-            Wm[3 * i + 2] = h0 * d02 + h1 * d12 + h2 * d22;
-        }
-        if (g0) {
-            const T e0 = (T)ps[6], e1 = (T)ps[7], e2 = (T)ps[8];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) coef[i] += Ha[3 * i] * e0 + Ha[3 * i + 1] * e1 + Ha[3 * i + 2] * e2;
-        }
-        const int b1 = P.pt_ptr[p + 1];
-        for (int b = P.pt_ptr[p]; b < b1; ++b) {
-            const int cb = P.o_cam[b];
-            const int fb = P.cam_free[cb];
-            if (fb < f) continue;
-            int s = 0;
-            if (fb == f) {
-                if (!g0) continue;
-            } else {
-                s = slot[fb];
-                if (s < 0) continue;
-            }
-            T Rb[9], tb[3];
-            load_cam(cam, cb, Rb, tb);
-            EdgeLin<T> Lb;
-            edge_linearize(Rb, tb, x, y, z, fx, fy, cx, cy, T(0), T(0), Lb);
-            const T wb = w[b];
-            T M0[6], M1[6];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                M0[i] = Wm[3 * i] * Lb.JX[0] + Wm[3 * i + 1] * Lb.JX[1] + Wm[3 * i + 2] * Lb.JX[2];
-                M1[i] = Wm[3 * i] * Lb.JX[3] + Wm[3 * i + 1] * Lb.JX[4] + Wm[3 * i + 2] * Lb.JX[5];
-            }
-            T B0[6], B1[6];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) { B0[c] = wb * Lb.Jxi[c]; B1[c] = wb * Lb.Jxi[6 + c]; }
-            if (fb == f) {
-#pragma unroll
-                for (int i = 0; i < 6; ++i)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) diag[6 * i + c] += M0[i] * B0[c] + M1[i] * B1[c];
-            } else {
-                T* dst = acc + 36 * s;
-#pragma unroll
-                for (int i = 0; i < 6; ++i)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) atomicAdd(dst + 6 * i + c, M0[i] * B0[c] + M1[i] * B1[c]);
-            }
-        }
-    }
-    __shared__ T red[kBlock / kWave][42];
-    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-    if (g0) {
-#pragma unroll
-        for (int k = 0; k < 36; ++k) {
-            const T v = wave_sum(diag[k]);
-            if (lane == 0) red[wid][k] = v;
-        }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const T v = wave_sum(coef[k]);
-            if (lane == 0) red[wid][36 + k] = v;
-        }
-    }
-    __syncthreads();
-    T* out = spart + P.sitem_pofs[it];
-    if (g0 && threadIdx.x < 42) {
-        T s = T(0);
-        for (int i = 0; i < kBlock / kWave; ++i) s += red[i][threadIdx.x];
-        if (threadIdx.x < 36) acc[threadIdx.x] = s;
-        else out[36 * ns + threadIdx.x - 36] = s;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < ns * 36; i += kBlock) out[i] = acc[i];
-}
-
+// ---------------------------------------------------------------- schur reduce
 __global__ void k_fill_system(int n, int n_pad, double* __restrict__ S, double* __restrict__ g) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t tot = (size_t)n_pad * n_pad;
@@ -412,7 +269,7 @@ __global__ void k_fill_system(int n, int n_pad, double* __restrict__ S, double* 
 // One workgroup per S row block f: sum the chunk partials (FP64, fixed order),
 // form S_ff = Hpp + lambda I - sum, S_gf = -sum^T (lower triangle), g = bp - coef.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const T* __restrict__ spart,
+__global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const double* __restrict__ spart,
                                                          const double* __restrict__ Hc, const LMDev* lm,
                                                          double* __restrict__ S, double* __restrict__ gv) {
     const int f = blockIdx.x;
@@ -449,209 +306,11 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const T* 
 }
 
 template <typename T>
-void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st) {
-    if (P.n_sitems == 0) return;
-    const size_t smem = sizeof(T) * 36 * (size_t)P.max_ns + 2 * (size_t)P.n_fc + 16;
-    hipLaunchKernelGGL(k_schur_row<T>, dim3(P.n_sitems), dim3(kBlock), smem, st, P, s.cam, s.X,
-                       (const T*)W.w, W.ptsys, (T*)W.spart);
-}
-
-template <typename T>
 void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) {
     const size_t tot = (size_t)P.n_pad * P.n_pad;
     hipLaunchKernelGGL(k_fill_system, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P.n, P.n_pad, W.S, W.g);
     if (P.n_fc == 0) return;
-    hipLaunchKernelGGL(k_schur_reduce<T>, dim3(P.n_fc), dim3(kBlock), 0, st, P, (const T*)W.spart, W.Hc, W.lm, W.S, W.g);
-}
-
-// ---------------------------------------------------------------- dense Cholesky (FP64)
-// Right-looking blocked LL^T with 64x64 tiles on the padded matrix (padding is
-// the identity, so tiles need no bounds checks).  potrf factors the diagonal
-// tile in LDS and stores its inverse; trsm is then a tile product with the
-// inverse; syrk updates the trailing lower tiles.
-__global__ __launch_bounds__(256) void k_potrf(double* __restrict__ S, int lda, int k,
-                                               double* __restrict__ Linv, LMDev* lm) {
-    __shared__ double A[64][65];
-    __shared__ double L[64][65];
-    __shared__ double Xs[64][65];
-    __shared__ double part[4][64];
-    const int tid = threadIdx.x;
-    const size_t base = (size_t)(64 * k) * lda + 64 * k;
-    for (int idx = tid; idx < 4096; idx += 256) {
-        const int i = idx >> 6, j = idx & 63;
-        A[i][j] = (j <= i) ? S[base + (size_t)i * lda + j] : 0.0;
-        L[i][j] = 0.0;
-    }
-    __syncthreads();
-    // Step j: every thread reads the updated column j, writes the scaled
-    // column j to L and applies the rank-1 update to its own row entries of A.
-    const int ri = tid >> 2, cq = tid & 3;
-    bool bad = false;
-    for (int j = 0; j < 64; ++j) {
-        double d = A[j][j];
-        if (!(d > 0.0) || !isfinite(d)) { bad = true; d = 1.0; }
-        const double piv = sqrt(d);
-        if (ri >= j && cq == 0) L[ri][j] = (ri == j) ? piv : A[ri][j] / piv;
-        if (ri > j) {
-            const double lij = A[ri][j] / piv;
-            for (int c = j + 1 + ((cq - (j + 1)) & 3); c <= ri; c += 4) A[ri][c] -= lij * (A[c][j] / piv);
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < 4096; idx += 256) {
-        const int i = idx >> 6, j = idx & 63;
-        if (j <= i) S[base + (size_t)i * lda + j] = L[i][j];
-    }
-    // Inverse of the lower triangular tile, row by row: for row i every column
-    // c <= i is a dot product split over 4 threads.
-    const int c = tid & 63, q = tid >> 6;
-    for (int i = 0; i < 64; ++i) {
-        double s = 0.0;
-        if (c < i)
-            for (int m = c + q; m < i; m += 4) s -= L[i][m] * Xs[m][c];
-        part[q][c] = s;
-        __syncthreads();
-        if (q == 0) {
-            double v = 0.0;
-            if (c <= i) v = ((c == i ? 1.0 : 0.0) + part[0][c] + part[1][c] + part[2][c] + part[3][c]) / L[i][i];
-            Xs[i][c] = v;
-        }
-        __syncthreads();
-    }
-    double* o = Linv + (size_t)k * 4096;
-    for (int idx = tid; idx < 4096; idx += 256) o[idx] = Xs[idx >> 6][idx & 63];
-    if (bad) atomicOr(&lm->fail, 1);
-}
-
-// A_ik <- A_ik * Linv_kk^T for tile rows i > k.
-__global__ __launch_bounds__(256) void k_trsm(double* __restrict__ S, int lda, int k, const double* __restrict__ Linv) {
-    __shared__ double A[64][65];
-    __shared__ double B[64][65];
-    const int i = k + 1 + blockIdx.x;
-    const size_t base = (size_t)(64 * i) * lda + 64 * k;
-    const double* Li = Linv + (size_t)k * 4096;
-    for (int idx = threadIdx.x; idx < 4096; idx += 256) {
-        const int r = idx >> 6, c = idx & 63;
-        A[r][c] = S[base + (size_t)r * lda + c];
-        B[r][c] = Li[idx];
-    }
-    __syncthreads();
-    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-    double acc[4][4] = {};
-    for (int m = 0; m < 64; ++m) {
-        double a[4], b[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { a[u] = A[4 * tr + u][m]; b[u] = B[tc + 16 * u][m]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) S[base + (size_t)(4 * tr + u) * lda + tc + 16 * v] = acc[u][v];
-}
-
-// A_ij -= A_ik A_jk^T for k < j <= i.
-__global__ __launch_bounds__(256) void k_syrk(double* __restrict__ S, int lda, int k, int nt) {
-    __shared__ double A[64][65];
-    __shared__ double B[64][65];
-    const int idx0 = blockIdx.x;
-    int ii = (int)((sqrt(8.0 * idx0 + 1.0) - 1.0) * 0.5);
-    while ((ii + 1) * (ii + 2) / 2 <= idx0) ++ii;
-    while (ii * (ii + 1) / 2 > idx0) --ii;
-    const int jj = idx0 - ii * (ii + 1) / 2;
-    const int i = k + 1 + ii, j = k + 1 + jj;
-    const size_t bi = (size_t)(64 * i) * lda + 64 * k;
-    const size_t bj = (size_t)(64 * j) * lda + 64 * k;
-    for (int idx = threadIdx.x; idx < 4096; idx += 256) {
-        const int r = idx >> 6, c = idx & 63;
-        A[r][c] = S[bi + (size_t)r * lda + c];
-        B[r][c] = S[bj + (size_t)r * lda + c];
-    }
-    __syncthreads();
-    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-    double acc[4][4] = {};
-    for (int m = 0; m < 64; ++m) {
-        double a[4], b[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { a[u] = A[4 * tr + u][m]; b[u] = B[tc + 16 * u][m]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
-    }
-    const size_t bo = (size_t)(64 * i) * lda + 64 * j;
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) S[bo + (size_t)(4 * tr + u) * lda + tc + 16 * v] -= acc[u][v];
-}
-
-// Forward and backward substitution with the factor's tiles and the inverse
-// diagonal tiles; one workgroup.  dp = S^-1 g, or 0 if the factorisation failed.
-__global__ __launch_bounds__(256) void k_trsv(const double* __restrict__ S, int lda, int nt, int n,
-                                              const double* __restrict__ Linv, const double* __restrict__ g,
-                                              double* __restrict__ y, double* __restrict__ dp, const LMDev* lm) {
-    __shared__ double part[4][64];
-    __shared__ double rv[64];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (lm->fail) {
-        for (int i = tid; i < 64 * nt; i += 256) dp[i] = 0.0;
-        return;
-    }
-    // forward: y_k = Linv_k (g_k - sum_{j<k} L_kj y_j)
-    for (int k = 0; k < nt; ++k) {
-        for (int rr = 0; rr < 16; ++rr) {
-            const int r = 16 * wid + rr;
-            const double* row = S + (size_t)(64 * k + r) * lda;
-            double s = 0.0;
-            for (int m = lane; m < 64 * k; m += 64) s += row[m] * y[m];
-            s = wave_sum(s);
-            if (lane == 0) rv[r] = g[64 * k + r] - s;
-        }
-        __syncthreads();
-        const double* Li = Linv + (size_t)k * 4096;
-        double s = 0.0;
-        for (int m = wid; m < 64; m += 4) s += Li[lane * 64 + m] * rv[m];
-        part[wid][lane] = s;
-        __syncthreads();
-        if (wid == 0) y[64 * k + lane] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-        __syncthreads();
-    }
-    // backward: x_k = Linv_k^T (y_k - sum_{j>k} L_jk^T x_j)
-    for (int k = nt - 1; k >= 0; --k) {
-        double s = 0.0;
-        for (int m = 64 * (k + 1) + wid; m < 64 * nt; m += 4) s += S[(size_t)m * lda + 64 * k + lane] * dp[m];
-        part[wid][lane] = s;
-        __syncthreads();
-        if (wid == 0) rv[lane] = y[64 * k + lane] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-        __syncthreads();
-        const double* Li = Linv + (size_t)k * 4096;
-        s = 0.0;
-        for (int m = wid; m < 64; m += 4) s += Li[m * 64 + lane] * rv[m];
-        part[wid][lane] = s;
-        __syncthreads();
-        if (wid == 0) dp[64 * k + lane] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-        __syncthreads();
-    }
-    (void)n;
-}
-
-void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st) {
-    if (P.n_fc == 0) return;
-    const int nt = P.nt, lda = P.n_pad;
-    for (int k = 0; k < nt; ++k) {
-        hipLaunchKernelGGL(k_potrf, dim3(1), dim3(256), 0, st, W.S, lda, k, W.Linv, W.lm);
-        const int rem = nt - k - 1;
-        if (rem > 0) {
-            hipLaunchKernelGGL(k_trsm, dim3(rem), dim3(256), 0, st, W.S, lda, k, (const double*)W.Linv);
-            hipLaunchKernelGGL(k_syrk, dim3(rem * (rem + 1) / 2), dim3(256), 0, st, W.S, lda, k, nt);
-        }
-    }
-    hipLaunchKernelGGL(k_trsv, dim3(1), dim3(256), 0, st, (const double*)W.S, lda, nt, P.n,
-                       (const double*)W.Linv, (const double*)W.g, W.y, W.dp, (const LMDev*)W.lm);
+    hipLaunchKernelGGL(k_schur_reduce<T>, dim3(P.n_fc), dim3(kBlock), 0, st, P, (const double*)W.spart, W.Hc, W.lm, W.S, W.g);
 }
 
 // ---------------------------------------------------------------- back-substitution
@@ -911,7 +570,6 @@ void launch_final(const DevWork& W, double* out2, hipStream_t st) {
     template void launch_lin_cam<T>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);        \
     template void launch_lambda_init<T>(const DevProblem&, const DevWork&, hipStream_t);                     \
     template void launch_hinv<T>(const DevProblem&, const DevWork&, hipStream_t);                            \
-    template void launch_schur_row<T>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);      \
     template void launch_schur_reduce<T>(const DevProblem&, const DevWork&, hipStream_t);                    \
     template void launch_backsub<T>(const DevProblem&, const DevState&, const DevState&, const DevWork&,     \
                                     hipStream_t);                                                            \

@@ -37,6 +37,45 @@ struct Graphs {
     hipGraphExec_t iter0 = nullptr, iter = nullptr, trial = nullptr;
 };
 
+// HIP event pairs around stage launches; read back after a stream sync.
+struct StageTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cat;
+    size_t used = 0;
+    double ms[16] = {};
+    int launches[16] = {};
+    void begin(int c, hipStream_t s) {
+        if (ev.size() < 2 * (used + 1)) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            ev.push_back(a);
+            ev.push_back(b);
+            cat.push_back(0);
+        }
+        cat[used] = c;
+        (void)hipEventRecord(ev[2 * used], s);
+    }
+    void end(hipStream_t s) { (void)hipEventRecord(ev[2 * used + 1], s); ++used; }
+    void collect() {
+        for (size_t i = 0; i < used; ++i) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) == hipSuccess) {
+                ms[cat[i]] += t;
+                launches[cat[i]] += 1;
+            }
+        }
+        used = 0;
+    }
+    void clear() {
+        used = 0;
+        for (int i = 0; i < 16; ++i) { ms[i] = 0; launches[i] = 0; }
+    }
+    ~StageTimer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 }  // namespace
 
 struct nav_ba_handle {
@@ -58,6 +97,9 @@ struct nav_ba_handle {
     std::vector<DevBuf> bufs;
     int nbuf = 0;
     Graphs graphs;
+    bool timing = false;               // stage-by-stage launches with events
+    bool timing_active = false;
+    StageTimer timer;
     LMDev* hstatus = nullptr;          // pinned
     double* hfinal = nullptr;          // pinned [2]
     double* dfinal = nullptr;
@@ -167,27 +209,55 @@ void destroy_graphs(nav_ba_handle* h) {
 
 bool mixed(const nav_ba_handle* h) { return h->opt.precision == NAV_BA_MIXED; }
 
+// Enqueue one stage; with timing active it is bracketed by HIP events.
+template <class F>
+void stage(nav_ba_handle* h, int cat, F&& f) {
+    if (h->timing_active) h->timer.begin(cat, h->stream);
+    f();
+    if (h->timing_active) h->timer.end(h->stream);
+}
+
 template <typename T>
 void enqueue_lin(nav_ba_handle* h, bool lambda_init) {
-    launch_lin_pt<T>(h->P, h->cur, h->W, h->stream);
-    launch_lin_cam<T>(h->P, h->cur, h->W, h->stream);
-    if (lambda_init) launch_lambda_init<T>(h->P, h->W, h->stream);
+    stage(h, NAV_BA_ST_LIN_PT, [&] { launch_lin_pt<T>(h->P, h->cur, h->W, h->stream); });
+    stage(h, NAV_BA_ST_LIN_CAM, [&] { launch_lin_cam<T>(h->P, h->cur, h->W, h->stream); });
+    if (lambda_init) stage(h, NAV_BA_ST_LAMBDA, [&] { launch_lambda_init<T>(h->P, h->W, h->stream); });
 }
 
 template <typename T>
 void enqueue_trial(nav_ba_handle* h) {
-    launch_hinv<T>(h->P, h->W, h->stream);
+    stage(h, NAV_BA_ST_HINV, [&] { launch_hinv<T>(h->P, h->W, h->stream); });
     if (h->P.n_fc > 0) {
-        launch_schur_row<T>(h->P, h->cur, h->W, h->stream);
-        launch_schur_reduce<T>(h->P, h->W, h->stream);
-        launch_cholesky_solve(h->P, h->W, h->stream);
+        stage(h, NAV_BA_ST_SCHUR_ROW, [&] { launch_schur_row<T>(h->P, h->cur, h->W, h->stream); });
+        stage(h, NAV_BA_ST_SCHUR_REDUCE, [&] { launch_schur_reduce<T>(h->P, h->W, h->stream); });
+        stage(h, NAV_BA_ST_CHOLESKY, [&] { launch_cholesky_solve(h->P, h->W, h->stream); });
     }
-    launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream);
-    launch_update_cam(h->P, h->cur, h->cand, h->W, h->stream);
-    launch_cost<T>(h->P, h->cand, h->W, h->stream);
-    launch_lm_accept(h->W, h->stream);
-    launch_commit(h->P, h->cur, h->cand, h->W, h->stream);
+    stage(h, NAV_BA_ST_BACKSUB, [&] { launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream); });
+    stage(h, NAV_BA_ST_UPDATE_CAM, [&] { launch_update_cam(h->P, h->cur, h->cand, h->W, h->stream); });
+    stage(h, NAV_BA_ST_COST, [&] { launch_cost<T>(h->P, h->cand, h->W, h->stream); });
+    stage(h, NAV_BA_ST_ACCEPT, [&] {
+        launch_lm_accept(h->W, h->stream);
+        launch_commit(h->P, h->cur, h->cand, h->W, h->stream);
+    });
     (void)hipMemcpyAsync(h->hstatus, h->W.lm, sizeof(LMDev), hipMemcpyDeviceToHost, h->stream);
+}
+
+// One outer iteration (which = 0: with lambda initialisation, 1: without) or
+// one more trial (which = 2): graph replay, or stage-by-stage when timing.
+int launch_step(nav_ba_handle* h, int which) {
+    if (!h->timing) {
+        hipGraphExec_t g = which == 0 ? h->graphs.iter0 : which == 1 ? h->graphs.iter : h->graphs.trial;
+        HIPTRY(h, hipGraphLaunch(g, h->stream));
+        return NAV_BA_OK;
+    }
+    h->timing_active = true;
+    const bool m = mixed(h);
+    if (which == 0 || which == 1) {
+        if (m) enqueue_lin<float>(h, which == 0); else enqueue_lin<double>(h, which == 0);
+    }
+    if (m) enqueue_trial<float>(h); else enqueue_trial<double>(h);
+    h->timing_active = false;
+    return NAV_BA_OK;
 }
 
 int capture(nav_ba_handle* h, hipGraphExec_t* out, int which) {
@@ -345,7 +415,7 @@ int32_t nav_ba_prepare(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_o
     if (nfc > kMaxFreeCams) return fail(h, NAV_BA_TOO_LARGE, "more than 8192 free cameras");
 
     // ---- camera-major observation lists of free cameras (ascending observation index)
-    std::vector<int> cam_ptr(nfc + 1, 0), cam_obs;
+    std::vector<int> cam_ptr(nfc + 1, 0), cam_obs, cam_obs_b0;
     for (int k = 0; k < no; ++k) if (cam_free[o_cam[k]] >= 0) cam_ptr[cam_free[o_cam[k]] + 1]++;
     for (int f = 0; f < nfc; ++f) cam_ptr[f + 1] += cam_ptr[f];
     cam_obs.resize(cam_ptr[nfc]);
@@ -355,6 +425,15 @@ int32_t nav_ba_prepare(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_o
             const int f = cam_free[o_cam[k]];
             if (f >= 0) cam_obs[pos[f]++] = k;
         }
+    }
+    // first observation of the same point on the same camera (start of the
+    // point's observations on cameras >= this one; duplicates are adjacent)
+    cam_obs_b0.resize(cam_obs.size());
+    for (size_t k = 0; k < cam_obs.size(); ++k) {
+        int b = cam_obs[k];
+        const int q = o_pt[b];
+        while (b > pt_ptr[q] && o_cam[b - 1] == o_cam[cam_obs[k]]) --b;
+        cam_obs_b0[k] = b;
     }
 
     // ---- point-aligned linearisation tiles
@@ -458,7 +537,7 @@ int32_t nav_ba_prepare(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_o
 #define UP(dst, vec) if ((st = upload(h, &(dst), (vec))) != NAV_BA_OK) return st
     UP(P.o_cam, o_cam); UP(P.o_pt, o_pt); UP(P.o_uv, o_uv); UP(P.o_orig, o_orig);
     UP(P.pt_ptr, pt_ptr); UP(P.pt_free, pt_free); UP(P.cam_free, cam_free); UP(P.free_cam, free_cam);
-    UP(P.cam_ptr, cam_ptr); UP(P.cam_obs, cam_obs);
+    UP(P.cam_ptr, cam_ptr); UP(P.cam_obs, cam_obs); UP(P.cam_obs_b0, cam_obs_b0);
     UP(P.tile_obs, tile_obs); UP(P.tile_pt, tile_pt);
     UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
     UP(P.sitem_f, sf); UP(P.sitem_a0, sa0); UP(P.sitem_a1, sa1); UP(P.sitem_g, sg); UP(P.sitem_nslots, sns);
@@ -488,12 +567,12 @@ int32_t nav_ba_prepare(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_o
     AL(W.hpart, tsz * 27 * std::max<size_t>(hf.size(), 1));
     AL(W.Hc, sizeof(double) * 28 * std::max(nfc, 1));
     AL(W.ptsys, sizeof(double) * 12 * (size_t)np);
-    AL(W.spart, tsz * std::max<long long>(pofs, 1));
+    AL(W.spart, sizeof(double) * std::max<long long>(pofs, 1));
     AL(W.S, sizeof(double) * std::max<size_t>((size_t)P.n_pad * P.n_pad, 1));
     AL(W.g, sizeof(double) * std::max(P.n_pad, 1));
     AL(W.y, sizeof(double) * std::max(P.n_pad, 1));
     AL(W.dp, sizeof(double) * std::max(P.n_pad, 1));
-    AL(W.Linv, sizeof(double) * 4096 * std::max(P.nt, 1));
+    AL(W.Lm, sizeof(double) * std::max<size_t>((size_t)P.n_pad * P.n_pad, 1));
     AL(W.red_cost, sizeof(double) * W.n_cost_blocks);
     AL(W.red_scale, sizeof(double) * (1 + W.n_pt_blocks));
     AL(W.red_aux, sizeof(double) * W.n_cost_blocks);
@@ -504,7 +583,9 @@ int32_t nav_ba_prepare(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_o
     HIPTRY(h, hipMemcpyAsync(h->init.pose, pose0.data(), sizeof(PoseQT) * nc, hipMemcpyHostToDevice, h->stream));
     HIPTRY(h, hipMemcpyAsync(h->init.X, X0.data(), sizeof(D4) * np, hipMemcpyHostToDevice, h->stream));
     HIPTRY(h, hipStreamSynchronize(h->stream));
-    // Schur LDS above the default 64 KiB (never the case with the budget above, kept explicit)
+    // Schur rows may use more than the default 64 KiB of LDS
+    if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
+        return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
     const bool auto_lambda = !(o->lambda_init > 0);
     if ((st = capture(h, &h->graphs.iter, 1)) != NAV_BA_OK) return st;
     if (auto_lambda && (st = capture(h, &h->graphs.iter0, 0)) != NAV_BA_OK) return st;
@@ -543,13 +624,14 @@ int32_t nav_ba_run(nav_ba_handle* h) {
     h->initial_cost = h->hstatus->chi;
     h->cost_history[0] = h->initial_cost;
     const bool auto_lambda = !(h->opt.lambda_init > 0);
+    int st;
     for (int it = 0; it < h->opt.n_iter; ++it) {
-        HIPTRY(h, hipGraphLaunch((it == 0 && auto_lambda) ? h->graphs.iter0 : h->graphs.iter, s));
+        if ((st = launch_step(h, (it == 0 && auto_lambda) ? 0 : 1)) != NAV_BA_OK) return st;
         HIPTRY(h, hipStreamSynchronize(s));
         int q = 1;
         double rho = h->hstatus->rho;
         while (rho < 0 && q < h->opt.max_trials) {
-            HIPTRY(h, hipGraphLaunch(h->graphs.trial, s));
+            if ((st = launch_step(h, 2)) != NAV_BA_OK) return st;
             HIPTRY(h, hipStreamSynchronize(s));
             rho = h->hstatus->rho;
             ++q;
@@ -564,14 +646,19 @@ int32_t nav_ba_run(nav_ba_handle* h) {
     }
     h->final_lambda = h->hstatus->lambda;
     // outlier flags and the FP64 cost at the final state
-    launch_flags(h->P, h->cur, h->W, h->opt.outlier_chi2, s);
-    launch_final(h->W, h->dfinal, s);
+    h->timing_active = h->timing;
+    stage(h, NAV_BA_ST_FLAGS, [&] {
+        launch_flags(h->P, h->cur, h->W, h->opt.outlier_chi2, s);
+        launch_final(h->W, h->dfinal, s);
+    });
+    h->timing_active = false;
     HIPTRY(h, hipMemcpyAsync(h->hfinal, h->dfinal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPTRY(h, hipEventRecord(h->ev1, s));
     HIPTRY(h, hipStreamSynchronize(s));
     float ms = 0.f;
     HIPTRY(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->solve_ms = ms;
+    if (h->timing) h->timer.collect();
     h->final_cost = h->hfinal[0];
     h->n_outliers = (int)h->hfinal[1];
     h->ran = true;
@@ -617,6 +704,19 @@ int32_t nav_ba_fetch(nav_ba_handle* h, nav_ba_result* r) {
             o[2] = compat ? rf32(X[q].z) : X[q].z;
         }
     }
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_set_timing(nav_ba_handle* h, int32_t enable) {
+    if (!h) return NAV_BA_BAD_ARG;
+    h->timing = enable != 0;
+    h->timer.clear();
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out) {
+    if (!h || !out) return NAV_BA_BAD_ARG;
+    for (int i = 0; i < 16; ++i) { out->ms[i] = h->timer.ms[i]; out->launches[i] = h->timer.launches[i]; }
     return NAV_BA_OK;
 }
 
