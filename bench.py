@@ -100,6 +100,68 @@ def cpu_baseline(prob, opts):
     }, r
 
 
+def batched_leg(n_windows, steps, warmup):
+    """Uneven local windows (5-30 keyframes, 300-5000 points, 2-8 views per
+    point, seed = window index): all windows in one batched solve, mixed and
+    FP64, against the same windows solved one at a time by the plain FP64
+    path (each window prepared before its timed run)."""
+    import torch
+    from nav24_amd import problems
+    from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
+    from nav24_amd.ba import GraphOptimHip
+
+    wins = [problems.workload(3, seed=w) for w in range(n_windows)]
+    n_obs = np.array([w.n_obs for w in wins], np.float64)
+    out = {"workload": f"{n_windows} uneven windows (5-30 kf, 300-5000 pts)", "n_windows": n_windows,
+           "n_obs_total": int(n_obs.sum()), "n_cam_total": int(sum(w.n_cam for w in wins))}
+    res = {}
+    for name, prec in (("mixed", NAV_BA_MIXED), ("fp64", NAV_BA_FP64)):
+        o = workload_options(3)
+        o.precision = prec
+        s = GraphOptimHip(0)
+        s.prepare_batched(wins, o)
+        for _ in range(warmup):
+            s.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        r = s.fetch_batched()
+        s.close()
+        it = np.array([x.iterations for x in r], np.float64)
+        res[name] = r
+        out[name] = {"window_lm_iterations_per_s": round(float(it.sum()) / dt, 1),
+                     "obs_per_s": round(float((it * n_obs).sum()) / dt, 1),
+                     "ms_per_batch": round(dt * 1e3, 3), "lm_iterations_per_window": int(it.max())}
+    rel = np.array([abs(a.final_cost - b.final_cost) / b.final_cost for a, b in zip(res["mixed"], res["fp64"])])
+    out["mixed"]["final_cost_rel_vs_fp64"] = {"max": float(rel.max()), "median": float(np.median(rel)),
+                                              "p99": float(np.quantile(rel, 0.99)),
+                                              "windows_within_1e-5": int((rel <= 1e-5).sum())}
+    # plain FP64 path, one window at a time
+    o = workload_options(3)
+    o.precision = NAV_BA_FP64
+    s = GraphOptimHip(0)
+    tot = 0.0
+    iters = 0
+    for w in wins:
+        s.prepare(w, o)
+        s.run()  # warm-up of this window's graph
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run()
+        torch.cuda.synchronize()
+        tot += time.perf_counter() - t0
+        iters += s.fetch().iterations
+    s.close()
+    out["plain_fp64_one_window_at_a_time"] = {"window_lm_iterations_per_s": round(iters / tot, 1),
+                                              "ms_total": round(tot * 1e3, 3)}
+    out["mixed_batched_speedup_vs_plain_fp64"] = round(
+        out["mixed"]["window_lm_iterations_per_s"] / out["plain_fp64_one_window_at_a_time"]["window_lm_iterations_per_s"], 2)
+    return out
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -119,6 +181,7 @@ def main():
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--windows", type=int, default=512, help="batched leg: number of windows (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -194,6 +257,9 @@ def main():
             "speedup_vs_fp64": round((mres.iterations / mdt) / (iters / dt), 3),
         }
         ms.close()
+
+    if args.windows > 0:
+        line["batched"] = batched_leg(args.windows, args.steps, args.warmup)
 
     if not args.no_cpu:
         cb, ref = cpu_baseline(prob, opts)

@@ -134,6 +134,23 @@ int32_t nav_ba_prepare(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_o
 int32_t nav_ba_run(nav_ba_handle* h);
 int32_t nav_ba_fetch(nav_ba_handle* h, nav_ba_result* r);
 
+/* A batch of independent local windows solved together: the loop of
+ * per-window GraphOptim::solve calls the front end makes on each local map
+ * (FE_SlamMonoV.cpp:161-165, BE_GraphOptim.cpp:31-77 once per window).  Every
+ * window gets its own LM schedule (lambda, nu, trials, termination) exactly as
+ * if it were solved alone; all windows share the options and the intrinsics.
+ * A window may hold at most 30 free cameras (NAV_BA_TOO_LARGE otherwise) and
+ * at most 8192 cameras.  `results[w]` receives window w's results, with the
+ * same meaning as for nav_ba_fetch; solve_ms is the time of the whole batch.
+ * nav_ba_run runs a prepared batch; an empty window reports NAV_BA_STOP_EMPTY
+ * and fetch returns NAV_BA_EMPTY if any window was empty (the others are
+ * still written). */
+int32_t nav_ba_prepare_batched(nav_ba_handle* h, const nav_ba_problem* windows, int32_t n_windows,
+                               const nav_ba_options* o);
+int32_t nav_ba_fetch_batched(nav_ba_handle* h, nav_ba_result* results, int32_t n_windows);
+int32_t nav_ba_solve_batched(nav_ba_handle* h, const nav_ba_problem* windows, int32_t n_windows,
+                             const nav_ba_options* o, nav_ba_result* results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,10 @@ NAV_BA_NOT_PREPARED = 7
 NAV_BA_FP64 = 0
 NAV_BA_MIXED = 1
 
+NAV_BA_STOP_MAX_ITER = 0
+NAV_BA_STOP_TERMINATE = 1
+NAV_BA_STOP_EMPTY = 2
+
 STATUS_NAMES = {
     0: "OK", 1: "EMPTY", 2: "BAD_ARG", 3: "NOT_PD_TERMINATED", 4: "HIP_ERROR",
     5: "NO_DEVICE", 6: "TOO_LARGE", 7: "NOT_PREPARED",

@@ -11,12 +11,12 @@ device the calls fail loudly.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional
+from typing import List, Optional, Sequence
 
 import numpy as np
 
 from ._loader import load_library
-from .abi import (NAV_BA_EMPTY, NAV_BA_OK, STATUS_NAMES, NavBaOptions, NavBaProblem, NavBaResult,
+from .abi import (NAV_BA_EMPTY, NAV_BA_OK, NAV_BA_STOP_EMPTY, STATUS_NAMES, NavBaOptions, NavBaProblem, NavBaResult,
                   Options, Problem, Result, ResultBuffers)
 
 _dp = C.POINTER(C.c_double)
@@ -25,6 +25,7 @@ _dp = C.POINTER(C.c_double)
 EXPORTED = [
     "nav_ba_version", "nav_ba_create", "nav_ba_destroy", "nav_ba_last_error",
     "nav_ba_default_options", "nav_ba_solve", "nav_ba_prepare", "nav_ba_run", "nav_ba_fetch",
+    "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing",
 ]
 
@@ -61,6 +62,13 @@ def lib() -> C.CDLL:
         L.nav_ba_run.restype = C.c_int32
         L.nav_ba_fetch.argtypes = [H, C.POINTER(NavBaResult)]
         L.nav_ba_fetch.restype = C.c_int32
+        L.nav_ba_prepare_batched.argtypes = [H, C.POINTER(NavBaProblem), C.c_int32, C.POINTER(NavBaOptions)]
+        L.nav_ba_prepare_batched.restype = C.c_int32
+        L.nav_ba_fetch_batched.argtypes = [H, C.POINTER(NavBaResult), C.c_int32]
+        L.nav_ba_fetch_batched.restype = C.c_int32
+        L.nav_ba_solve_batched.argtypes = [H, C.POINTER(NavBaProblem), C.c_int32, C.POINTER(NavBaOptions),
+                                           C.POINTER(NavBaResult)]
+        L.nav_ba_solve_batched.restype = C.c_int32
         L.nav_ba_debug_reduced_system.argtypes = [H, C.c_double, _dp, _dp, C.c_int32, C.POINTER(C.c_int32)]
         L.nav_ba_debug_reduced_system.restype = C.c_int32
         L.nav_ba_set_timing.argtypes = [H, C.c_int32]
@@ -135,6 +143,43 @@ class GraphOptimHip:
         st = lib().nav_ba_fetch(self._h, C.byref(rb.c))
         self._check(st, allow_empty=True)
         return rb.result(st)
+
+    # -- batches of independent windows (FE_SlamMonoV.cpp:161-165, one solve per window)
+
+    def _batch(self, windows: Sequence[Problem], opts: Options):
+        arr = (NavBaProblem * len(windows))(*[w.c_struct() for w in windows])
+        return arr, opts.c_struct()
+
+    def _results(self, windows: Sequence[Problem], n_iter: int):
+        bufs = [ResultBuffers(w, n_iter) for w in windows]
+        return bufs, (NavBaResult * len(windows))(*[b.c for b in bufs])
+
+    def solve_batched(self, windows: Sequence[Problem], opts: Options) -> List[Result]:
+        """One GraphOptim::solve per window, all windows in one device solve."""
+        arr, os_ = self._batch(windows, opts)
+        bufs, res = self._results(windows, opts.n_iter)
+        st = lib().nav_ba_solve_batched(self._h, arr, len(windows), C.byref(os_), res)
+        self._check(st, allow_empty=True)
+        return [self._window_result(b, res[i]) for i, b in enumerate(bufs)]
+
+    def prepare_batched(self, windows: Sequence[Problem], opts: Options) -> None:
+        arr, os_ = self._batch(windows, opts)
+        self._keep = (arr, os_)
+        self._prob, self._opts = list(windows), opts
+        self._check(lib().nav_ba_prepare_batched(self._h, arr, len(windows), C.byref(os_)))
+
+    def fetch_batched(self) -> List[Result]:
+        windows = self._prob
+        bufs, res = self._results(windows, self._opts.n_iter)
+        st = lib().nav_ba_fetch_batched(self._h, res, len(windows))
+        self._check(st, allow_empty=True)
+        return [self._window_result(b, res[i]) for i, b in enumerate(bufs)]
+
+    @staticmethod
+    def _window_result(b: ResultBuffers, c: NavBaResult) -> Result:
+        b.c = c
+        st = NAV_BA_EMPTY if c.stop_reason == NAV_BA_STOP_EMPTY else NAV_BA_OK
+        return b.result(st)
 
     def set_timing(self, enable: bool) -> None:
         """Stage-by-stage launches with HIP events around each stage (no graphs)."""

@@ -1,4 +1,11 @@
 // ba_kernels.h — launch interface between the host runtime and the gfx950 kernels.
+//
+// A prepared problem is one or more independent BA windows laid end to end:
+// cameras, points and observations of window w occupy contiguous ranges, and
+// every kernel covers all windows in one launch.  A single problem is the
+// one-window case.  Each window carries its own LM state on the device, and
+// the accept kernel runs g2o's trial loop per window, so one replayed graph
+// (linearise if needed, one trial, decide) advances every window at once.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,31 +18,39 @@ namespace navba {
 constexpr int kBlock = 256;        // threads per workgroup for the per-observation kernels
 constexpr int kTile = 256;         // observations per point-aligned linearisation tile
 constexpr int kChunk = 1024;       // camera-major observations per Schur / Hpp work item
-constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget:
-                                   // 160*36*8 B + 2*kMaxFreeCams B < 64 KiB)
-constexpr int kNB = 64;            // dense Cholesky tile
-constexpr int kMaxFreeCams = 8192; // slot map in LDS is int16
-constexpr int kCamLdsMax = 512;    // Schur rows keep the camera table in LDS up to this many cameras
+constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget)
+constexpr int kNB = 64;            // tiled dense Cholesky tile
+constexpr int kMaxWinCams = 8192;  // cameras per window: Schur slot map in LDS is int16
+constexpr int kCamLdsMax = 512;    // Schur rows keep the window's camera table in LDS up to this size
+constexpr int kSmallMaxN = 180;    // per-window in-LDS Cholesky: at most 30 free cameras
+constexpr int kCostObs = 1024;     // observations per cost / flag block
+constexpr int kPtBlock = 256;      // points per back-substitution block
+
+// LM phase of one window
+enum { kPhaseLin = 0, kPhaseTrial = 1, kPhaseDone = 2 };
 
 struct alignas(32) D4 { double x, y, z, w; };
 
-// Scalars of the LM schedule, resident on the device.
+// Scalars of the LM schedule of one window, resident on the device.
 struct LMDev {
     double lambda, nu, chi, temp_chi, scale, rho;
-    int fail, accepted, trials, pad;
+    int fail, accepted, trials, phase;
+    int iter, q, stop, pad;
 };
 
-// Problem structure built once per prepare (device pointers).
 struct DevProblem {
     int n_cam, n_pt, n_obs, n_fc;
     int robust;
     double delta;
     double fx, fy, cx, cy;
+    // LM settings (same for every window)
+    int n_iter, max_trials, auto_lambda;
+    double lambda0;
     // observations, point-major (by point, then camera, then input order)
     const int* o_cam;
     const int* o_pt;
     const double* o_uv;      // 2 per observation
-    const int* o_orig;       // caller index
+    const int* o_orig;       // caller index (batched: index in the concatenated caller order)
     const int* pt_ptr;       // [n_pt + 1]
     const uint8_t* pt_free;  // [n_pt]
     const int* cam_free;     // [n_cam] -> free index or -1
@@ -44,9 +59,10 @@ struct DevProblem {
     const int* cam_ptr;      // [n_fc + 1]
     const int* cam_obs;
     const int* cam_obs_b0;   // per cam_obs entry: first observation of its point on a camera >= this one
-    // linearisation tiles (whole points, <= kTile observations or one point)
+    // linearisation tiles (whole points of one window, <= kTile observations or one point)
     const int* tile_obs;     // [n_tiles + 1]
     const int* tile_pt;      // [n_tiles + 1]
+    const int* tile_win;     // [n_tiles]
     int n_tiles;
     // Hpp work items (free camera, chunk)
     const int* hitem_f;
@@ -65,10 +81,29 @@ struct DevProblem {
     const int* row_nchunks;    // [n_fc]
     int n_sitems;
     int max_ns;                // largest slot count of any Schur work item
-    // column blocks of each S row: free indices >= row, first is the row itself
+    // column blocks of each S row: free indices >= row (same window), first is the row itself
     const int* colptr;         // [n_fc + 1]
     const int* colcam;
-    // dense reduced system
+    // windows
+    int n_win;
+    int max_win_cams;
+    int batched;               // 1: per-window dense systems factored in LDS
+    const int* cam_win;        // [n_cam]
+    const int* pt_win;         // [n_pt]
+    const int* win_cam0;       // [n_win + 1]
+    const int* win_fc0;        // [n_win + 1]
+    const int* win_pt0;        // [n_win + 1]
+    const long long* win_S;    // [n_win] offset of the window's dense system in S
+    const int* win_lda;        // [n_win] leading dimension of the window's dense system
+    const int* cblk_o0;        // [n_cblk + 1] cost blocks: window-aligned observation ranges
+    const int* cblk_win;       // [n_cblk]
+    const int* win_cblk0;      // [n_win + 1]
+    int n_cblk;
+    const int* pblk_p0;        // [n_pblk + 1] point blocks, window-aligned
+    const int* pblk_win;       // [n_pblk]
+    const int* win_pblk0;      // [n_win + 1]
+    int n_pblk;
+    // dense reduced system of the single-window tiled path
     int n, n_pad, nt;
 };
 
@@ -78,27 +113,31 @@ struct DevState {
     D4* X;         // [n_pt]
 };
 
-// Per-iteration / per-trial work buffers.
+// Work buffers.
 struct DevWork {
     void* w;            // [n_obs] T: robust weights
-    void* Hl;           // [n_pt * 9] T: Hll (6 packed) + bl (3)
-    void* hpart;        // [n_hitems * 27] T
+    double* Hl;         // [n_pt * 9]: Hll (6 packed) + bl (3)
+    double* hpart;      // [n_hitems * 27]: Hpp (21 packed) + bp (6) per chunk
     double* Hc;         // [n_fc * 28]: Hpp packed (21) + bp (6)
     double* ptsys;      // [n_pt * 12]: Dinv packed (6) + Dinv*bl (3)
     double* spart;      // Schur row partials (FP64 for both compute types)
-    double* S;          // [n_pad * n_pad]
-    double* g;          // [n_pad]
-    double* y;          // [n_pad]
-    double* dp;         // [n_pad]
-    double* Lm;         // [n_pad * n_pad] Cholesky factor (lower tiles)
-    double* red_cost;   // [n_cost_blocks]
-    double* red_scale;  // [1 + n_pt_blocks]
-    double* red_aux;    // [n_cost_blocks] outlier counts
+    double* S;          // dense reduced systems
+    double* g;          // reduced right-hand sides (free-camera order, padded for the tiled path)
+    double* y;
+    double* dp;
+    double* Lm;         // tiled path: Cholesky factor
+    double* red_cost;   // [n_cblk]
+    double* red_scale;  // [n_win + n_pblk]: per-window camera part, then point blocks
+    double* red_aux;    // [n_cblk] outlier counts
     uint8_t* outlier;   // [n_obs] caller order
-    LMDev* lm;
-    int n_cost_blocks;
-    int n_pt_blocks;
+    double* hist;       // [n_win * (n_iter + 1)] chi2 before / after each iteration
+    double* final;      // [n_win * 2] FP64 final cost, outlier count
+    int* done_count;    // windows finished
+    LMDev* lm;          // [n_win]
 };
+
+__device__ __forceinline__ bool needs_lin(const LMDev* lm, int w) { return lm[w].phase == kPhaseLin; }
+__device__ __forceinline__ bool running(const LMDev* lm, int w) { return lm[w].phase != kPhaseDone; }
 
 __device__ __forceinline__ int sym6(int i, int j) {
     if (i > j) { const int t = i; i = j; j = t; }
@@ -117,7 +156,7 @@ __device__ __forceinline__ void load_cam(const CamRT* cam, int c, T R[9], T t[3]
 // Launchers.  T = double (FP64 path) or float (mixed path).
 template <typename T> int prepare_schur_row(const DevProblem& P);  // LDS attribute; < 0 if too large
 void launch_cam_rt(const DevProblem& P, const DevState& s, hipStream_t st);
-void launch_lm_reset(LMDev* lm, double lambda0, hipStream_t st);
+void launch_lm_reset(const DevProblem& P, const DevWork& W, hipStream_t st);
 template <typename T> void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 template <typename T> void launch_lin_cam(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 template <typename T> void launch_lambda_init(const DevProblem& P, const DevWork& W, hipStream_t st);
@@ -125,14 +164,18 @@ template <typename T> void launch_hinv(const DevProblem& P, const DevWork& W, hi
 template <typename T> void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st);
 void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st);
+void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st);
+int prepare_cholesky_small();  // LDS attribute of the per-window factorisation
 template <typename T> void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand,
                                           const DevWork& W, hipStream_t st);
 void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);
-template <typename T> void launch_cost(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
-void launch_lm_accept(const DevWork& W, hipStream_t st);
-void launch_lm_set_chi(const DevWork& W, hipStream_t st);
+// trial = 1: candidate state of running windows; trial = 0: current state, all windows
+template <typename T> void launch_cost(const DevProblem& P, const DevState& s, const DevWork& W, int trial,
+                                       hipStream_t st);
+void launch_lm_accept(const DevProblem& P, const DevWork& W, hipStream_t st);
+void launch_lm_set_chi(const DevProblem& P, const DevWork& W, hipStream_t st);
 void launch_commit(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);
 void launch_flags(const DevProblem& P, const DevState& s, const DevWork& W, double thr, hipStream_t st);
-void launch_final(const DevWork& W, double* out2, hipStream_t st);
+void launch_final(const DevProblem& P, const DevWork& W, hipStream_t st);
 
 }  // namespace navba

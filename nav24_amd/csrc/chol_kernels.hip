@@ -273,6 +273,96 @@ __global__ __launch_bounds__(256) void k_chol_solve(const double* __restrict__ L
     }
 }
 
+// ---------------------------------------------------------------- per-window systems
+// One workgroup per running window: the window's reduced system (n <= 180)
+// is copied to LDS as a packed lower triangle (row i starts at i(i+1)/2),
+// factored column by column (2 barriers per column: scale the column, then
+// the rank-1 update by waves over rows and lanes over columns), and solved by
+// forward and backward substitution in wave 0 with readlane broadcasts (lane
+// m keeps rows m, m+64, m+128 of the right-hand side).
+__device__ __forceinline__ int pk(int i, int j) { return i * (i + 1) / 2 + j; }
+
+__device__ __forceinline__ double sel3(double a, double b, double c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+
+__global__ __launch_bounds__(256) void k_chol_small(DevProblem P, const double* __restrict__ S,
+                                                    const double* __restrict__ g, double* __restrict__ dp, LMDev* lm) {
+    extern __shared__ double Lp[];
+    __shared__ int bad;
+    const int win = blockIdx.x;
+    if (!running(lm, win)) return;
+    const int n = 6 * (P.win_fc0[win + 1] - P.win_fc0[win]);
+    if (n == 0) return;
+    const double* Sw = S + P.win_S[win];
+    const int lda = P.win_lda[win];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int i = wid; i < n; i += 4)
+        for (int j = lane; j <= i; j += 64) Lp[pk(i, j)] = Sw[(size_t)i * lda + j];
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        double d = Lp[pk(k, k)];
+        const bool ok = (d > 0.0) && isfinite(d);
+        d = ok ? d : 1.0;
+        const double piv = sqrt(d);
+        const double inv = 1.0 / piv;
+        for (int i = k + 1 + tid; i < n; i += 256) Lp[pk(i, k)] *= inv;
+        if (tid == 0) { if (!ok) bad = 1; }
+        __syncthreads();
+        for (int i = k + 1 + wid; i < n; i += 4) {
+            const double lik = Lp[pk(i, k)];
+            for (int j = k + 1 + lane; j <= i; j += 64) Lp[pk(i, j)] -= lik * Lp[pk(j, k)];
+        }
+        if (tid == 0) Lp[pk(k, k)] = piv;
+        __syncthreads();
+    }
+    const int g0 = 6 * P.win_fc0[win];
+    if (bad) {
+        for (int i = tid; i < n; i += 256) dp[g0 + i] = 0.0;
+        if (tid == 0) lm[win].fail = 1;
+        return;
+    }
+    if (wid == 0) {
+        // forward: L y = g
+        double r0 = lane < n ? g[g0 + lane] : 0.0;
+        double r1 = lane + 64 < n ? g[g0 + lane + 64] : 0.0;
+        double r2 = lane + 128 < n ? g[g0 + lane + 128] : 0.0;
+        for (int c = 0; c < n; ++c) {
+            const int cs = c >> 6, cl = c & 63;
+            const double yc = readlane(sel3(r0, r1, r2, cs), cl) / Lp[pk(c, c)];
+            if (lane == cl) { if (cs == 0) r0 = yc; else if (cs == 1) r1 = yc; else r2 = yc; }
+            const int m0 = lane, m1 = lane + 64, m2 = lane + 128;
+            if (m0 > c && m0 < n) r0 -= Lp[pk(m0, c)] * yc;
+            if (m1 > c && m1 < n) r1 -= Lp[pk(m1, c)] * yc;
+            if (m2 > c && m2 < n) r2 -= Lp[pk(m2, c)] * yc;
+        }
+        // backward: L^T x = y
+        for (int c = n - 1; c >= 0; --c) {
+            const int cs = c >> 6, cl = c & 63;
+            const double xc = readlane(sel3(r0, r1, r2, cs), cl) / Lp[pk(c, c)];
+            if (lane == cl) { if (cs == 0) r0 = xc; else if (cs == 1) r1 = xc; else r2 = xc; }
+            const int m0 = lane, m1 = lane + 64, m2 = lane + 128;
+            if (m0 < c) r0 -= Lp[pk(c, m0)] * xc;
+            if (m1 < c) r1 -= Lp[pk(c, m1)] * xc;
+            if (m2 < c) r2 -= Lp[pk(c, m2)] * xc;
+        }
+        if (lane < n) dp[g0 + lane] = r0;
+        if (lane + 64 < n) dp[g0 + lane + 64] = r1;
+        if (lane + 128 < n) dp[g0 + lane + 128] = r2;
+    }
+}
+
+void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st) {
+    const size_t smem = sizeof(double) * ((size_t)kSmallMaxN * (kSmallMaxN + 1) / 2);
+    hipLaunchKernelGGL(k_chol_small, dim3(P.n_win), dim3(256), smem, st, P, (const double*)W.S, (const double*)W.g,
+                       W.dp, W.lm);
+}
+
+int prepare_cholesky_small() {
+    const size_t smem = sizeof(double) * ((size_t)kSmallMaxN * (kSmallMaxN + 1) / 2);
+    return hipFuncSetAttribute((const void*)k_chol_small, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) ==
+                   hipSuccess ? 0 : -1;
+}
+
 void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st) {
     if (P.n_fc == 0) return;
     const int nt = P.nt, lda = P.n_pad;

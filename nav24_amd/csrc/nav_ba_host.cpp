@@ -5,10 +5,13 @@
 //              as flat arrays: points with observations compacted in ascending id,
 //              observations ordered by (point, camera, input order), camera-major
 //              index lists, linearisation tiles, Schur work items, co-visibility.
-//   run      = optimizer.optimize(nIterations) (:66-68) on the device; the LM
-//              iteration and each trial are replayed hipGraphs, the host only
-//              reads a 64-byte status per trial to decide on another trial.
+//   run      = optimizer.optimize(nIterations) (:66-68) on the device.  One
+//              replayed hipGraph advances every window by one LM trial (gated
+//              linearisation, trial, accept/reject, next phase); the host only
+//              watches a small status.
 //   fetch    = recoverPose / recoverMapPoints (:213-244) plus the outlier flags.
+// A batch of independent windows (the per-window calls nav24's front end
+// makes, FE_SlamMonoV.cpp:161-165) is prepared as windows laid end to end.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,15 +29,12 @@ using namespace navba;
 
 namespace {
 
-constexpr const char* kVersion = "nav24_amd-ba 0.1 (gfx950)";
+constexpr const char* kVersion = "nav24_amd-ba 0.2 (gfx950)";
+constexpr int kMaxDenseN = 16384;  // tiled dense reduced system of one window (2 GiB FP64)
 
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
-};
-
-struct Graphs {
-    hipGraphExec_t iter0 = nullptr, iter = nullptr, trial = nullptr;
 };
 
 // HIP event pairs around stage launches; read back after a stream sync.
@@ -76,6 +76,15 @@ struct StageTimer {
     }
 };
 
+// Host copy of what fetch needs for one window.
+struct WinHost {
+    int n_cam = 0, n_pt = 0, n_obs = 0;
+    int cam0 = 0, pt_local0 = 0, pt_local1 = 0, obs0 = 0;  // ranges in the concatenated problem
+    std::vector<int> pt_orig;      // local point -> caller point index of this window
+    std::vector<double> X_in, T_in;
+    bool empty = false;
+};
+
 }  // namespace
 
 struct nav_ba_handle {
@@ -84,31 +93,26 @@ struct nav_ba_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     bool prepared = false;
+    bool batched = false;
     nav_ba_options opt{};
-    // host copy of the input shape and what fetch needs
-    int n_cam = 0, n_pt_all = 0, n_obs = 0;
-    bool empty = false;
-    std::vector<int> pt_orig;          // local -> caller point index
-    std::vector<double> X_in;          // caller points (for inactive points)
-    std::vector<double> T_in;
+    std::vector<WinHost> win;
+    bool all_empty = false;
     DevProblem P{};
     DevState cur{}, cand{}, init{};
     DevWork W{};
     std::vector<DevBuf> bufs;
     int nbuf = 0;
-    Graphs graphs;
-    bool timing = false;               // stage-by-stage launches with events
+    hipGraphExec_t step = nullptr;
+    bool timing = false;
     bool timing_active = false;
     StageTimer timer;
-    LMDev* hstatus = nullptr;          // pinned
-    double* hfinal = nullptr;          // pinned [2]
-    double* dfinal = nullptr;
+    LMDev* hstatus = nullptr;  // pinned: LM state of window 0
+    int* hdone = nullptr;      // pinned: windows finished
     // results of the last run
     bool ran = false;
-    int iterations = 0, trials = 0, stop_reason = 0;
-    double initial_cost = 0, final_cost = 0, final_lambda = 0, solve_ms = 0;
-    int n_outliers = 0;
-    std::vector<double> cost_history;
+    std::vector<LMDev> lm_out;
+    std::vector<double> hist_out, final_out;
+    double solve_ms = 0;
 };
 
 namespace {
@@ -127,7 +131,7 @@ int fail(nav_ba_handle* h, int code, const std::string& msg) {
     return code;
 }
 
-// Device allocation slot `i` of at least `bytes` (grow on demand, reused across prepares).
+// Device allocation slot of at least `bytes` (grow on demand, reused across prepares).
 int alloc(nav_ba_handle* h, void** out, size_t bytes) {
     if ((int)h->bufs.size() <= h->nbuf) h->bufs.resize(h->nbuf + 1);
     DevBuf& b = h->bufs[h->nbuf++];
@@ -200,13 +204,6 @@ void host_R_from_quat(const double q[4], double R[9]) {
 
 inline double rf32(double v) { return (double)(float)v; }
 
-void destroy_graphs(nav_ba_handle* h) {
-    if (h->graphs.iter0) (void)hipGraphExecDestroy(h->graphs.iter0);
-    if (h->graphs.iter) (void)hipGraphExecDestroy(h->graphs.iter);
-    if (h->graphs.trial) (void)hipGraphExecDestroy(h->graphs.trial);
-    h->graphs = Graphs{};
-}
-
 bool mixed(const nav_ba_handle* h) { return h->opt.precision == NAV_BA_MIXED; }
 
 // Enqueue one stage; with timing active it is bracketed by HIP events.
@@ -218,58 +215,59 @@ void stage(nav_ba_handle* h, int cat, F&& f) {
 }
 
 template <typename T>
-void enqueue_lin(nav_ba_handle* h, bool lambda_init) {
+void enqueue_lin(nav_ba_handle* h) {
     stage(h, NAV_BA_ST_LIN_PT, [&] { launch_lin_pt<T>(h->P, h->cur, h->W, h->stream); });
     stage(h, NAV_BA_ST_LIN_CAM, [&] { launch_lin_cam<T>(h->P, h->cur, h->W, h->stream); });
-    if (lambda_init) stage(h, NAV_BA_ST_LAMBDA, [&] { launch_lambda_init<T>(h->P, h->W, h->stream); });
+    if (h->P.auto_lambda) stage(h, NAV_BA_ST_LAMBDA, [&] { launch_lambda_init<T>(h->P, h->W, h->stream); });
 }
 
 template <typename T>
-void enqueue_trial(nav_ba_handle* h) {
+void enqueue_reduced_system(nav_ba_handle* h) {
     stage(h, NAV_BA_ST_HINV, [&] { launch_hinv<T>(h->P, h->W, h->stream); });
     if (h->P.n_fc > 0) {
         stage(h, NAV_BA_ST_SCHUR_ROW, [&] { launch_schur_row<T>(h->P, h->cur, h->W, h->stream); });
         stage(h, NAV_BA_ST_SCHUR_REDUCE, [&] { launch_schur_reduce<T>(h->P, h->W, h->stream); });
-        stage(h, NAV_BA_ST_CHOLESKY, [&] { launch_cholesky_solve(h->P, h->W, h->stream); });
     }
+}
+
+// One LM step of every window: gated linearisation, one trial, decision.
+template <typename T>
+void enqueue_step(nav_ba_handle* h) {
+    enqueue_lin<T>(h);
+    enqueue_reduced_system<T>(h);
+    if (h->P.n_fc > 0)
+        stage(h, NAV_BA_ST_CHOLESKY, [&] {
+            if (h->P.batched) launch_cholesky_small(h->P, h->W, h->stream);
+            else launch_cholesky_solve(h->P, h->W, h->stream);
+        });
     stage(h, NAV_BA_ST_BACKSUB, [&] { launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream); });
     stage(h, NAV_BA_ST_UPDATE_CAM, [&] { launch_update_cam(h->P, h->cur, h->cand, h->W, h->stream); });
-    stage(h, NAV_BA_ST_COST, [&] { launch_cost<T>(h->P, h->cand, h->W, h->stream); });
+    stage(h, NAV_BA_ST_COST, [&] { launch_cost<T>(h->P, h->cand, h->W, 1, h->stream); });
     stage(h, NAV_BA_ST_ACCEPT, [&] {
-        launch_lm_accept(h->W, h->stream);
+        launch_lm_accept(h->P, h->W, h->stream);
         launch_commit(h->P, h->cur, h->cand, h->W, h->stream);
     });
     (void)hipMemcpyAsync(h->hstatus, h->W.lm, sizeof(LMDev), hipMemcpyDeviceToHost, h->stream);
+    (void)hipMemcpyAsync(h->hdone, h->W.done_count, sizeof(int), hipMemcpyDeviceToHost, h->stream);
 }
 
-// One outer iteration (which = 0: with lambda initialisation, 1: without) or
-// one more trial (which = 2): graph replay, or stage-by-stage when timing.
-int launch_step(nav_ba_handle* h, int which) {
+int launch_step(nav_ba_handle* h) {
     if (!h->timing) {
-        hipGraphExec_t g = which == 0 ? h->graphs.iter0 : which == 1 ? h->graphs.iter : h->graphs.trial;
-        HIPTRY(h, hipGraphLaunch(g, h->stream));
+        HIPTRY(h, hipGraphLaunch(h->step, h->stream));
         return NAV_BA_OK;
     }
     h->timing_active = true;
-    const bool m = mixed(h);
-    if (which == 0 || which == 1) {
-        if (m) enqueue_lin<float>(h, which == 0); else enqueue_lin<double>(h, which == 0);
-    }
-    if (m) enqueue_trial<float>(h); else enqueue_trial<double>(h);
+    if (mixed(h)) enqueue_step<float>(h); else enqueue_step<double>(h);
     h->timing_active = false;
     return NAV_BA_OK;
 }
 
-int capture(nav_ba_handle* h, hipGraphExec_t* out, int which) {
+int capture_step(nav_ba_handle* h) {
     hipGraph_t g = nullptr;
     HIPTRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    const bool m = mixed(h);
-    if (which == 0 || which == 1) {
-        if (m) enqueue_lin<float>(h, which == 0); else enqueue_lin<double>(h, which == 0);
-    }
-    if (m) enqueue_trial<float>(h); else enqueue_trial<double>(h);
+    if (mixed(h)) enqueue_step<float>(h); else enqueue_step<double>(h);
     HIPTRY(h, hipStreamEndCapture(h->stream, &g));
-    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&h->step, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(h, NAV_BA_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     return NAV_BA_OK;
@@ -283,6 +281,440 @@ int validate(const nav_ba_problem* p) {
         if (p->obs_cam[i] < 0 || p->obs_cam[i] >= p->n_cam || p->obs_pt[i] < 0 || p->obs_pt[i] >= p->n_pt)
             return NAV_BA_BAD_ARG;
     return NAV_BA_OK;
+}
+
+// Build the device structure of n_win windows laid end to end (see ba_kernels.h).
+int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, const nav_ba_options* o, bool batched) {
+    if (!h) return NAV_BA_BAD_ARG;
+    h->err.clear();
+    h->prepared = false;
+    h->ran = false;
+    if (!o || !probs || n_win <= 0) return fail(h, NAV_BA_BAD_ARG, "null options or problems");
+    for (int w = 0; w < n_win; ++w)
+        if (validate(&probs[w]) != NAV_BA_OK)
+            return fail(h, NAV_BA_BAD_ARG, "invalid problem " + std::to_string(w) + " (null array or index out of range)");
+    if (o->precision != NAV_BA_FP64 && o->precision != NAV_BA_MIXED) return fail(h, NAV_BA_BAD_ARG, "unknown precision");
+    if (o->n_iter < 0) return fail(h, NAV_BA_BAD_ARG, "negative iteration count");
+    for (int w = 1; w < n_win; ++w)
+        if (probs[w].fx != probs[0].fx || probs[w].fy != probs[0].fy || probs[w].cx != probs[0].cx || probs[w].cy != probs[0].cy)
+            return fail(h, NAV_BA_BAD_ARG, "batched windows must share the camera intrinsics");
+    HIPTRY(h, hipSetDevice(h->device));
+    HIPTRY(h, hipStreamSynchronize(h->stream));
+    if (h->step) { (void)hipGraphExecDestroy(h->step); h->step = nullptr; }
+    h->opt = *o;
+    if (h->opt.max_trials <= 0) h->opt.max_trials = 10;
+    h->batched = batched;
+    const bool compat = o->compat_fp32_io != 0;
+
+    // ---- concatenate the windows; keep only points with observations
+    h->win.assign(n_win, WinHost());
+    std::vector<int> cam_off(n_win + 1, 0), obs_off(n_win + 1, 0);
+    for (int w = 0; w < n_win; ++w) {
+        cam_off[w + 1] = cam_off[w] + probs[w].n_cam;
+        obs_off[w + 1] = obs_off[w] + probs[w].n_obs;
+    }
+    const int nc = cam_off[n_win];
+    const int no = obs_off[n_win];
+    std::vector<int> g_cam(no), g_pt(no), pt_first(n_win + 1, 0);
+    std::vector<int> pt_win_all;  // local point -> window
+    std::vector<int> cam_win(nc);
+    int np = 0;
+    bool any = false;
+    for (int w = 0; w < n_win; ++w) {
+        const nav_ba_problem& p = probs[w];
+        WinHost& wh = h->win[w];
+        wh.n_cam = p.n_cam; wh.n_pt = p.n_pt; wh.n_obs = p.n_obs;
+        wh.cam0 = cam_off[w]; wh.obs0 = obs_off[w];
+        wh.T_in.assign(p.T_cw, p.T_cw + 12 * (size_t)p.n_cam);
+        wh.X_in.assign(p.X, p.X + 3 * (size_t)p.n_pt);
+        for (int c = 0; c < p.n_cam; ++c) cam_win[cam_off[w] + c] = w;
+        std::vector<int> cnt(p.n_pt, 0), loc(p.n_pt, -1);
+        for (int i = 0; i < p.n_obs; ++i) cnt[p.obs_pt[i]]++;
+        wh.pt_local0 = np;
+        for (int i = 0; i < p.n_pt; ++i)
+            if (cnt[i] > 0) { loc[i] = np++; wh.pt_orig.push_back(i); pt_win_all.push_back(w); }
+        wh.pt_local1 = np;
+        wh.empty = (p.n_cam == 0 || wh.pt_orig.empty());
+        any = any || !wh.empty;
+        for (int i = 0; i < p.n_obs; ++i) {
+            g_cam[obs_off[w] + i] = cam_off[w] + p.obs_cam[i];
+            g_pt[obs_off[w] + i] = loc[p.obs_pt[i]];
+        }
+        pt_first[w + 1] = np;
+    }
+    h->all_empty = !any;
+    if (!batched && n_win != 1) return fail(h, NAV_BA_BAD_ARG, "single-problem prepare takes one problem");
+    if (h->all_empty) {
+        h->prepared = true;
+        return NAV_BA_OK;
+    }
+
+    // ---- observations ordered by (point, camera, input order): two stable counting sorts
+    std::vector<int> by_cam(no), ord(no);
+    {
+        std::vector<int> off(nc + 1, 0);
+        for (int i = 0; i < no; ++i) off[g_cam[i] + 1]++;
+        for (int c = 0; c < nc; ++c) off[c + 1] += off[c];
+        for (int i = 0; i < no; ++i) by_cam[off[g_cam[i]]++] = i;
+        std::vector<int> poff(np + 1, 0);
+        for (int i = 0; i < no; ++i) poff[g_pt[i] + 1]++;
+        for (int q = 0; q < np; ++q) poff[q + 1] += poff[q];
+        for (int k = 0; k < no; ++k) {
+            const int i = by_cam[k];
+            ord[poff[g_pt[i]]++] = i;
+        }
+    }
+    std::vector<int> o_cam(no), o_pt(no), o_orig(no), pt_ptr(np + 1, 0);
+    std::vector<double> o_uv(2 * (size_t)no);
+    for (int k = 0; k < no; ++k) {
+        const int i = ord[k];
+        const int w = (int)(std::upper_bound(obs_off.begin(), obs_off.end(), i) - obs_off.begin()) - 1;
+        const double* uv = probs[w].obs_uv + 2 * (size_t)(i - obs_off[w]);
+        o_cam[k] = g_cam[i];
+        o_pt[k] = g_pt[i];
+        o_orig[k] = i;
+        o_uv[2 * (size_t)k] = compat ? rf32(uv[0]) : uv[0];
+        o_uv[2 * (size_t)k + 1] = compat ? rf32(uv[1]) : uv[1];
+        pt_ptr[o_pt[k] + 1]++;
+    }
+    for (int q = 0; q < np; ++q) pt_ptr[q + 1] += pt_ptr[q];
+    std::vector<uint8_t> pt_free(np);
+    for (int q = 0; q < np; ++q) {
+        const int w = pt_win_all[q];
+        const int i = h->win[w].pt_orig[q - h->win[w].pt_local0];
+        pt_free[q] = (probs[w].pt_fixed && probs[w].pt_fixed[i]) ? 0 : 1;
+    }
+
+    // ---- free cameras: not fixed and with at least one (therefore active) edge
+    std::vector<int> cnt_cam(nc, 0);
+    for (int i = 0; i < no; ++i) cnt_cam[g_cam[i]]++;
+    std::vector<int> cam_free(nc, -1), free_cam, win_fc0(n_win + 1, 0), win_cam0(n_win + 1, 0);
+    for (int w = 0; w < n_win; ++w) {
+        win_cam0[w] = cam_off[w];
+        win_fc0[w] = (int)free_cam.size();
+        for (int c = cam_off[w]; c < cam_off[w + 1]; ++c) {
+            const int lc = c - cam_off[w];
+            if (!(probs[w].cam_fixed && probs[w].cam_fixed[lc]) && cnt_cam[c] > 0) {
+                cam_free[c] = (int)free_cam.size();
+                free_cam.push_back(c);
+            }
+        }
+    }
+    win_cam0[n_win] = nc;
+    win_fc0[n_win] = (int)free_cam.size();
+    const int nfc = (int)free_cam.size();
+    int max_win_cams = 1;
+    for (int w = 0; w < n_win; ++w) max_win_cams = std::max(max_win_cams, probs[w].n_cam);
+    if (max_win_cams > kMaxWinCams) return fail(h, NAV_BA_TOO_LARGE, "more than 8192 cameras in a window");
+
+    // ---- camera-major observation lists of free cameras (ascending observation index)
+    std::vector<int> cam_ptr(nfc + 1, 0), cam_obs, cam_obs_b0;
+    for (int k = 0; k < no; ++k) if (cam_free[o_cam[k]] >= 0) cam_ptr[cam_free[o_cam[k]] + 1]++;
+    for (int f = 0; f < nfc; ++f) cam_ptr[f + 1] += cam_ptr[f];
+    cam_obs.resize(cam_ptr[nfc]);
+    {
+        std::vector<int> pos(cam_ptr.begin(), cam_ptr.end() - 1);
+        for (int k = 0; k < no; ++k) {
+            const int f = cam_free[o_cam[k]];
+            if (f >= 0) cam_obs[pos[f]++] = k;
+        }
+    }
+    // first observation of the same point on the same camera (start of the
+    // point's observations on cameras >= this one; duplicates are adjacent)
+    cam_obs_b0.resize(cam_obs.size());
+    for (size_t k = 0; k < cam_obs.size(); ++k) {
+        int b = cam_obs[k];
+        const int q = o_pt[b];
+        while (b > pt_ptr[q] && o_cam[b - 1] == o_cam[cam_obs[k]]) --b;
+        cam_obs_b0[k] = b;
+    }
+
+    // ---- point-aligned linearisation tiles, never spanning two windows
+    std::vector<int> tile_obs{0}, tile_pt{0}, tile_win;
+    for (int w = 0; w < n_win; ++w) {
+        for (int q = pt_first[w]; q < pt_first[w + 1];) {
+            int q1 = q;
+            while (q1 < pt_first[w + 1] && pt_ptr[q1 + 1] - pt_ptr[q] <= kTile) ++q1;
+            if (q1 == q) q1 = q + 1;  // one point with more than kTile observations
+            tile_obs.push_back(pt_ptr[q1]);
+            tile_pt.push_back(q1);
+            tile_win.push_back(w);
+            q = q1;
+        }
+    }
+    // ---- window-aligned cost blocks (observations) and point blocks; windows
+    // are contiguous, so each block starts where the previous one ends
+    std::vector<int> cblk_o0{0}, cblk_win, win_cblk0{0}, pblk_p0{0}, pblk_win, win_pblk0{0};
+    std::vector<int> win_pt0(n_win + 1);
+    for (int w = 0; w < n_win; ++w) {
+        const int o0 = pt_ptr[pt_first[w]], o1 = pt_ptr[pt_first[w + 1]];
+        for (int a = o0; a < o1; a += kCostObs) { cblk_o0.push_back(std::min(a + kCostObs, o1)); cblk_win.push_back(w); }
+        win_cblk0.push_back((int)cblk_win.size());
+        for (int q = pt_first[w]; q < pt_first[w + 1]; q += kPtBlock) {
+            pblk_p0.push_back(std::min(q + kPtBlock, pt_first[w + 1]));
+            pblk_win.push_back(w);
+        }
+        win_pblk0.push_back((int)pblk_win.size());
+        win_pt0[w] = pt_first[w];
+    }
+    win_pt0[n_win] = np;
+
+    // ---- Hpp work items: (free camera, chunk)
+    std::vector<int> hf, ha0, ha1, cam_hitem_ptr{0};
+    for (int f = 0; f < nfc; ++f) {
+        for (int a = cam_ptr[f]; a < cam_ptr[f + 1]; a += kChunk) {
+            hf.push_back(f); ha0.push_back(a); ha1.push_back(std::min(a + kChunk, cam_ptr[f + 1]));
+        }
+        cam_hitem_ptr.push_back((int)hf.size());
+    }
+
+    // ---- co-visibility through free points, per window: column blocks g >= f of each S row
+    std::vector<int> colptr{0}, colcam;
+    {
+        std::vector<uint64_t> bits;
+        std::vector<int> fl;
+        for (int w = 0; w < n_win; ++w) {
+            const int f0 = win_fc0[w], nw = win_fc0[w + 1] - f0;
+            bits.assign(((size_t)nw * nw + 63) / 64, 0);
+            for (int q = pt_first[w]; q < pt_first[w + 1]; ++q) {
+                if (!pt_free[q]) continue;
+                fl.clear();
+                for (int k = pt_ptr[q]; k < pt_ptr[q + 1]; ++k) {
+                    const int f = cam_free[o_cam[k]];
+                    if (f >= 0 && (fl.empty() || fl.back() != f - f0)) fl.push_back(f - f0);
+                }
+                for (size_t a = 0; a < fl.size(); ++a)
+                    for (size_t b = a + 1; b < fl.size(); ++b) {
+                        const size_t bit = (size_t)fl[a] * nw + fl[b];
+                        bits[bit >> 6] |= 1ull << (bit & 63);
+                    }
+            }
+            for (int lf = 0; lf < nw; ++lf) {
+                colcam.push_back(f0 + lf);
+                for (int lg = lf + 1; lg < nw; ++lg) {
+                    const size_t bit = (size_t)lf * nw + lg;
+                    if (bits[bit >> 6] >> (bit & 63) & 1ull) colcam.push_back(f0 + lg);
+                }
+                colptr.push_back((int)colcam.size());
+            }
+        }
+    }
+
+    // ---- Schur work items: per row, column groups of kMaxSlots, chunks of kChunk
+    std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr{0}, row_nch;
+    std::vector<long long> spofs;
+    long long pofs = 0;
+    int max_ns = 1;
+    for (int f = 0; f < nfc; ++f) {
+        const int ncols = colptr[f + 1] - colptr[f];
+        const int ngroups = (ncols + kMaxSlots - 1) / kMaxSlots;
+        const int nobs = cam_ptr[f + 1] - cam_ptr[f];
+        const int nch = std::max(1, (nobs + kChunk - 1) / kChunk);
+        row_nch.push_back(nch);
+        for (int gi = 0; gi < ngroups; ++gi) {
+            const int ns = std::min(kMaxSlots, ncols - gi * kMaxSlots);
+            max_ns = std::max(max_ns, ns);
+            for (int ch = 0; ch < nch; ++ch) {
+                sf.push_back(f);
+                sa0.push_back(cam_ptr[f] + ch * kChunk);
+                sa1.push_back(std::min(cam_ptr[f] + (ch + 1) * kChunk, cam_ptr[f + 1]));
+                sg.push_back(gi);
+                sns.push_back(ns);
+                spofs.push_back(pofs);
+                pofs += 36LL * ns + 6;
+            }
+        }
+        row_sitem_ptr.push_back((int)sf.size());
+    }
+
+    // ---- dense reduced systems: one padded tiled system, or one small system per window
+    std::vector<long long> win_S(n_win, 0);
+    std::vector<int> win_lda(n_win, 1);
+    long long S_size = 1;
+    const int n = 6 * nfc;
+    int n_pad = ((n + kNB - 1) / kNB) * kNB;
+    if (!batched) {
+        if (n > kMaxDenseN) return fail(h, NAV_BA_TOO_LARGE, "reduced camera system larger than 16384");
+        win_lda[0] = std::max(n_pad, 1);
+        S_size = std::max<long long>((long long)n_pad * n_pad, 1);
+    } else {
+        S_size = 0;
+        for (int w = 0; w < n_win; ++w) {
+            const int nw = 6 * (win_fc0[w + 1] - win_fc0[w]);
+            if (nw > kSmallMaxN) return fail(h, NAV_BA_TOO_LARGE, "a batched window has more than 30 free cameras");
+            win_S[w] = S_size;
+            win_lda[w] = std::max(nw, 1);
+            S_size += (long long)win_lda[w] * win_lda[w];
+        }
+        n_pad = 0;
+    }
+
+    // ---- initial state
+    std::vector<PoseQT> pose0(nc);
+    for (int w = 0; w < n_win; ++w)
+        for (int c = 0; c < probs[w].n_cam; ++c) {
+            const double* T = probs[w].T_cw + 12 * (size_t)c;
+            PoseQT& P0 = pose0[cam_off[w] + c];
+            host_quat_from_R(T, P0.q);
+            P0.t[0] = T[9]; P0.t[1] = T[10]; P0.t[2] = T[11];
+            P0.pad = 0;
+        }
+    std::vector<D4> X0(np);
+    for (int q = 0; q < np; ++q) {
+        const int w = pt_win_all[q];
+        const double* x = probs[w].X + 3 * (size_t)h->win[w].pt_orig[q - h->win[w].pt_local0];
+        X0[q] = compat ? D4{rf32(x[0]), rf32(x[1]), rf32(x[2]), 0.0} : D4{x[0], x[1], x[2], 0.0};
+    }
+
+    // ---- device upload
+    h->nbuf = 0;
+    DevProblem& P = h->P;
+    P = DevProblem{};
+    P.n_cam = nc; P.n_pt = np; P.n_obs = no; P.n_fc = nfc;
+    P.robust = o->robust ? 1 : 0;
+    P.delta = o->huber_delta;
+    P.fx = compat ? rf32(probs[0].fx) : probs[0].fx; P.fy = compat ? rf32(probs[0].fy) : probs[0].fy;
+    P.cx = compat ? rf32(probs[0].cx) : probs[0].cx; P.cy = compat ? rf32(probs[0].cy) : probs[0].cy;
+    P.n_iter = o->n_iter;
+    P.max_trials = h->opt.max_trials;
+    P.auto_lambda = !(o->lambda_init > 0);
+    P.lambda0 = o->lambda_init;
+    int st;
+#define UP(dst, vec) if ((st = upload(h, &(dst), (vec))) != NAV_BA_OK) return st
+    UP(P.o_cam, o_cam); UP(P.o_pt, o_pt); UP(P.o_uv, o_uv); UP(P.o_orig, o_orig);
+    UP(P.pt_ptr, pt_ptr); UP(P.pt_free, pt_free); UP(P.cam_free, cam_free); UP(P.free_cam, free_cam);
+    UP(P.cam_ptr, cam_ptr); UP(P.cam_obs, cam_obs); UP(P.cam_obs_b0, cam_obs_b0);
+    UP(P.tile_obs, tile_obs); UP(P.tile_pt, tile_pt); UP(P.tile_win, tile_win);
+    UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
+    UP(P.sitem_f, sf); UP(P.sitem_a0, sa0); UP(P.sitem_a1, sa1); UP(P.sitem_g, sg); UP(P.sitem_nslots, sns);
+    UP(P.sitem_pofs, spofs); UP(P.row_sitem_ptr, row_sitem_ptr); UP(P.row_nchunks, row_nch);
+    UP(P.colptr, colptr); UP(P.colcam, colcam);
+    UP(P.cam_win, cam_win); UP(P.pt_win, pt_win_all); UP(P.win_cam0, win_cam0); UP(P.win_fc0, win_fc0);
+    UP(P.win_pt0, win_pt0); UP(P.win_S, win_S); UP(P.win_lda, win_lda);
+    UP(P.cblk_o0, cblk_o0); UP(P.cblk_win, cblk_win); UP(P.win_cblk0, win_cblk0);
+    UP(P.pblk_p0, pblk_p0); UP(P.pblk_win, pblk_win); UP(P.win_pblk0, win_pblk0);
+#undef UP
+    P.n_tiles = (int)tile_win.size();
+    P.n_hitems = (int)hf.size();
+    P.n_sitems = (int)sf.size();
+    P.max_ns = max_ns;
+    P.n_win = n_win;
+    P.max_win_cams = max_win_cams;
+    P.batched = batched ? 1 : 0;
+    P.n_cblk = (int)cblk_win.size();
+    P.n_pblk = (int)pblk_win.size();
+    P.n = n;
+    P.n_pad = n_pad;
+    P.nt = n_pad / kNB;
+
+    const size_t tsz = mixed(h) ? sizeof(float) : sizeof(double);
+    const size_t nvec = batched ? std::max<size_t>(6 * (size_t)nfc, 1) : std::max<size_t>(n_pad, 1);
+    auto A = [&](void** dst, size_t bytes) { return alloc(h, dst, bytes); };
+#define AL(dst, bytes) if ((st = A((void**)&(dst), (bytes))) != NAV_BA_OK) return st
+    AL(h->init.pose, sizeof(PoseQT) * nc); AL(h->init.X, sizeof(D4) * np); h->init.cam = nullptr;
+    AL(h->cur.pose, sizeof(PoseQT) * nc); AL(h->cur.cam, sizeof(CamRT) * nc); AL(h->cur.X, sizeof(D4) * np);
+    AL(h->cand.pose, sizeof(PoseQT) * nc); AL(h->cand.cam, sizeof(CamRT) * nc); AL(h->cand.X, sizeof(D4) * np);
+    DevWork& W = h->W;
+    AL(W.w, tsz * no);
+    AL(W.Hl, sizeof(double) * 9 * (size_t)np);
+    AL(W.hpart, sizeof(double) * 27 * std::max<size_t>(hf.size(), 1));
+    AL(W.Hc, sizeof(double) * 28 * std::max(nfc, 1));
+    AL(W.ptsys, sizeof(double) * 12 * (size_t)np);
+    AL(W.spart, sizeof(double) * std::max<long long>(pofs, 1));
+    AL(W.S, sizeof(double) * (size_t)std::max<long long>(S_size, 1));
+    AL(W.g, sizeof(double) * nvec);
+    AL(W.y, sizeof(double) * nvec);
+    AL(W.dp, sizeof(double) * nvec);
+    AL(W.Lm, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(S_size, 1)));
+    AL(W.red_cost, sizeof(double) * std::max(P.n_cblk, 1));
+    AL(W.red_scale, sizeof(double) * (n_win + std::max(P.n_pblk, 1)));
+    AL(W.red_aux, sizeof(double) * std::max(P.n_cblk, 1));
+    AL(W.outlier, std::max<size_t>(no, 1));
+    AL(W.hist, sizeof(double) * (size_t)n_win * (o->n_iter + 1));
+    AL(W.final, sizeof(double) * 2 * (size_t)n_win);
+    AL(W.done_count, sizeof(int));
+    AL(W.lm, sizeof(LMDev) * n_win);
+#undef AL
+    HIPTRY(h, hipMemcpyAsync(h->init.pose, pose0.data(), sizeof(PoseQT) * nc, hipMemcpyHostToDevice, h->stream));
+    HIPTRY(h, hipMemcpyAsync(h->init.X, X0.data(), sizeof(D4) * np, hipMemcpyHostToDevice, h->stream));
+    HIPTRY(h, hipStreamSynchronize(h->stream));
+    // Schur rows and the per-window factorisation may use more than 64 KiB of LDS
+    if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
+        return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
+    if (batched && prepare_cholesky_small() != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
+    if ((st = capture_step(h)) != NAV_BA_OK) return st;
+    h->prepared = true;
+    return NAV_BA_OK;
+}
+
+int fetch_window(nav_ba_handle* h, int w, const std::vector<PoseQT>& pose, const std::vector<D4>& X,
+                 const std::vector<uint8_t>& flags, nav_ba_result* r) {
+    const WinHost& wh = h->win[w];
+    r->solve_ms = h->solve_ms;
+    if (wh.empty || h->all_empty) {
+        r->iterations = 0; r->trials = 0; r->stop_reason = NAV_BA_STOP_EMPTY; r->n_outliers = 0;
+        r->initial_cost = r->final_cost = r->final_lambda = 0.0;
+        if (r->T_cw) std::memcpy(r->T_cw, wh.T_in.data(), sizeof(double) * 12 * (size_t)wh.n_cam);
+        if (r->X) std::memcpy(r->X, wh.X_in.data(), sizeof(double) * 3 * (size_t)wh.n_pt);
+        if (r->outlier) std::memset(r->outlier, 0, (size_t)wh.n_obs);
+        return NAV_BA_EMPTY;
+    }
+    const LMDev& L = h->lm_out[w];
+    const double* hist = h->hist_out.data() + (size_t)w * (h->P.n_iter + 1);
+    r->iterations = L.iter;
+    r->trials = L.trials;
+    r->stop_reason = L.stop == 1 ? NAV_BA_STOP_TERMINATE : NAV_BA_STOP_MAX_ITER;
+    r->initial_cost = hist[0];
+    r->final_cost = h->final_out[2 * (size_t)w];
+    r->n_outliers = (int)h->final_out[2 * (size_t)w + 1];
+    r->final_lambda = L.lambda;
+    if (r->cost_history)
+        for (int i = 0; i <= L.iter; ++i) r->cost_history[i] = hist[i];
+    if (r->T_cw)
+        for (int c = 0; c < wh.n_cam; ++c) {
+            const PoseQT& q = pose[wh.cam0 + c];
+            double* T = r->T_cw + 12 * (size_t)c;
+            host_R_from_quat(q.q, T);  // BE_GraphOptim.cpp:220-223
+            T[9] = q.t[0]; T[10] = q.t[1]; T[11] = q.t[2];
+        }
+    if (r->X) {
+        std::memcpy(r->X, wh.X_in.data(), sizeof(double) * 3 * (size_t)wh.n_pt);
+        const bool compat = h->opt.compat_fp32_io != 0;
+        for (int q = wh.pt_local0; q < wh.pt_local1; ++q) {
+            double* o = r->X + 3 * (size_t)wh.pt_orig[q - wh.pt_local0];
+            o[0] = compat ? rf32(X[q].x) : X[q].x;  // estimate().cast<float>() (BE_GraphOptim.cpp:239)
+            o[1] = compat ? rf32(X[q].y) : X[q].y;
+            o[2] = compat ? rf32(X[q].z) : X[q].z;
+        }
+    }
+    if (r->outlier && wh.n_obs > 0) std::memcpy(r->outlier, flags.data() + wh.obs0, (size_t)wh.n_obs);
+    return NAV_BA_OK;
+}
+
+int fetch_all(nav_ba_handle* h, nav_ba_result* results, int n) {
+    if (!h || !results) return NAV_BA_BAD_ARG;
+    if (!h->prepared || !h->ran) return fail(h, NAV_BA_NOT_PREPARED, "fetch before nav_ba_run");
+    if (n != (int)h->win.size()) return fail(h, NAV_BA_BAD_ARG, "result count differs from the prepared window count");
+    std::vector<PoseQT> pose;
+    std::vector<D4> X;
+    std::vector<uint8_t> flags;
+    if (!h->all_empty) {
+        HIPTRY(h, hipSetDevice(h->device));
+        pose.resize(h->P.n_cam);
+        X.resize(h->P.n_pt);
+        flags.resize(h->P.n_obs);
+        HIPTRY(h, hipMemcpyAsync(pose.data(), h->cur.pose, sizeof(PoseQT) * pose.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPTRY(h, hipMemcpyAsync(X.data(), h->cur.X, sizeof(D4) * X.size(), hipMemcpyDeviceToHost, h->stream));
+        if (!flags.empty())
+            HIPTRY(h, hipMemcpyAsync(flags.data(), h->W.outlier, flags.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPTRY(h, hipStreamSynchronize(h->stream));
+    }
+    int worst = NAV_BA_OK;
+    for (int w = 0; w < n; ++w) {
+        const int st = fetch_window(h, w, pose, X, flags, &results[w]);
+        if (st != NAV_BA_OK) worst = st;
+    }
+    return worst;
 }
 
 }  // namespace
@@ -316,7 +748,7 @@ int32_t nav_ba_create(nav_ba_handle** out, int32_t device) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
         hipHostMalloc((void**)&h->hstatus, sizeof(LMDev), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h->hfinal, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&h->hdone, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         nav_ba_destroy(h);
         return NAV_BA_HIP_ERROR;
     }
@@ -328,11 +760,11 @@ void nav_ba_destroy(nav_ba_handle* h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    destroy_graphs(h);
+    if (h->step) (void)hipGraphExecDestroy(h->step);
     for (auto& b : h->bufs)
         if (b.p) (void)hipFree(b.p);
     if (h->hstatus) (void)hipHostFree(h->hstatus);
-    if (h->hfinal) (void)hipHostFree(h->hfinal);
+    if (h->hdone) (void)hipHostFree(h->hdone);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -342,256 +774,12 @@ void nav_ba_destroy(nav_ba_handle* h) {
 const char* nav_ba_last_error(const nav_ba_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
 int32_t nav_ba_prepare(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_options* o) {
-    if (!h) return NAV_BA_BAD_ARG;
-    h->err.clear();
-    h->prepared = false;
-    h->ran = false;
-    if (!o) return fail(h, NAV_BA_BAD_ARG, "null options");
-    if (validate(p) != NAV_BA_OK) return fail(h, NAV_BA_BAD_ARG, "invalid problem (null array or index out of range)");
-    if (o->precision != NAV_BA_FP64 && o->precision != NAV_BA_MIXED) return fail(h, NAV_BA_BAD_ARG, "unknown precision");
-    if (o->n_iter < 0) return fail(h, NAV_BA_BAD_ARG, "negative iteration count");
-    HIPTRY(h, hipSetDevice(h->device));
-    HIPTRY(h, hipStreamSynchronize(h->stream));
-    destroy_graphs(h);
-    h->opt = *o;
-    if (h->opt.max_trials <= 0) h->opt.max_trials = 10;
-    const bool compat = o->compat_fp32_io != 0;
-    const int nc = p->n_cam, npa = p->n_pt, no = p->n_obs;
-    h->n_cam = nc; h->n_pt_all = npa; h->n_obs = no;
-    h->T_in.assign(p->T_cw, p->T_cw + 12 * (size_t)nc);
-    h->X_in.assign(p->X, p->X + 3 * (size_t)npa);
+    return prepare_windows(h, p, 1, o, false);
+}
 
-    // ---- points with observations, compacted in ascending caller index
-    std::vector<int> cnt_pt(npa, 0);
-    for (int i = 0; i < no; ++i) cnt_pt[p->obs_pt[i]]++;
-    std::vector<int> pt_local(npa, -1);
-    h->pt_orig.clear();
-    for (int i = 0; i < npa; ++i)
-        if (cnt_pt[i] > 0) { pt_local[i] = (int)h->pt_orig.size(); h->pt_orig.push_back(i); }
-    const int np = (int)h->pt_orig.size();
-    h->empty = (nc == 0 || np == 0);
-    if (h->empty) {
-        h->prepared = true;
-        return NAV_BA_OK;
-    }
-
-    // ---- observations ordered by (point, camera, input order): two stable counting sorts
-    std::vector<int> by_cam(no), ord(no);
-    {
-        std::vector<int> off(nc + 1, 0);
-        for (int i = 0; i < no; ++i) off[p->obs_cam[i] + 1]++;
-        for (int c = 0; c < nc; ++c) off[c + 1] += off[c];
-        for (int i = 0; i < no; ++i) by_cam[off[p->obs_cam[i]]++] = i;
-        std::vector<int> poff(np + 1, 0);
-        for (int i = 0; i < no; ++i) poff[pt_local[p->obs_pt[i]] + 1]++;
-        for (int q = 0; q < np; ++q) poff[q + 1] += poff[q];
-        for (int k = 0; k < no; ++k) {
-            const int i = by_cam[k];
-            ord[poff[pt_local[p->obs_pt[i]]]++] = i;
-        }
-    }
-    std::vector<int> o_cam(no), o_pt(no), o_orig(no), pt_ptr(np + 1, 0);
-    std::vector<double> o_uv(2 * (size_t)no);
-    for (int k = 0; k < no; ++k) {
-        const int i = ord[k];
-        o_cam[k] = p->obs_cam[i];
-        o_pt[k] = pt_local[p->obs_pt[i]];
-        o_orig[k] = i;
-        o_uv[2 * (size_t)k] = compat ? rf32(p->obs_uv[2 * (size_t)i]) : p->obs_uv[2 * (size_t)i];
-        o_uv[2 * (size_t)k + 1] = compat ? rf32(p->obs_uv[2 * (size_t)i + 1]) : p->obs_uv[2 * (size_t)i + 1];
-        pt_ptr[o_pt[k] + 1]++;
-    }
-    for (int q = 0; q < np; ++q) pt_ptr[q + 1] += pt_ptr[q];
-    std::vector<uint8_t> pt_free(np);
-    for (int q = 0; q < np; ++q) pt_free[q] = (p->pt_fixed && p->pt_fixed[h->pt_orig[q]]) ? 0 : 1;
-
-    // ---- free cameras: not fixed and with at least one (therefore active) edge
-    std::vector<int> cnt_cam(nc, 0);
-    for (int i = 0; i < no; ++i) cnt_cam[p->obs_cam[i]]++;
-    std::vector<int> cam_free(nc, -1), free_cam;
-    for (int c = 0; c < nc; ++c)
-        if (!(p->cam_fixed && p->cam_fixed[c]) && cnt_cam[c] > 0) { cam_free[c] = (int)free_cam.size(); free_cam.push_back(c); }
-    const int nfc = (int)free_cam.size();
-    if (nfc > kMaxFreeCams) return fail(h, NAV_BA_TOO_LARGE, "more than 8192 free cameras");
-
-    // ---- camera-major observation lists of free cameras (ascending observation index)
-    std::vector<int> cam_ptr(nfc + 1, 0), cam_obs, cam_obs_b0;
-    for (int k = 0; k < no; ++k) if (cam_free[o_cam[k]] >= 0) cam_ptr[cam_free[o_cam[k]] + 1]++;
-    for (int f = 0; f < nfc; ++f) cam_ptr[f + 1] += cam_ptr[f];
-    cam_obs.resize(cam_ptr[nfc]);
-    {
-        std::vector<int> pos(cam_ptr.begin(), cam_ptr.end() - 1);
-        for (int k = 0; k < no; ++k) {
-            const int f = cam_free[o_cam[k]];
-            if (f >= 0) cam_obs[pos[f]++] = k;
-        }
-    }
-    // first observation of the same point on the same camera (start of the
-    // point's observations on cameras >= this one; duplicates are adjacent)
-    cam_obs_b0.resize(cam_obs.size());
-    for (size_t k = 0; k < cam_obs.size(); ++k) {
-        int b = cam_obs[k];
-        const int q = o_pt[b];
-        while (b > pt_ptr[q] && o_cam[b - 1] == o_cam[cam_obs[k]]) --b;
-        cam_obs_b0[k] = b;
-    }
-
-    // ---- point-aligned linearisation tiles
-    std::vector<int> tile_obs{0}, tile_pt{0};
-    for (int q = 0; q < np;) {
-        int q1 = q;
-        while (q1 < np && pt_ptr[q1 + 1] - pt_ptr[q] <= kTile) ++q1;
-        if (q1 == q) q1 = q + 1;  // one point with more than kTile observations
-        tile_obs.push_back(pt_ptr[q1]);
-        tile_pt.push_back(q1);
-        q = q1;
-    }
-    const int n_tiles = (int)tile_obs.size() - 1;
-
-    // ---- Hpp work items: (free camera, chunk)
-    std::vector<int> hf, ha0, ha1, cam_hitem_ptr{0};
-    for (int f = 0; f < nfc; ++f) {
-        for (int a = cam_ptr[f]; a < cam_ptr[f + 1]; a += kChunk) {
-            hf.push_back(f); ha0.push_back(a); ha1.push_back(std::min(a + kChunk, cam_ptr[f + 1]));
-        }
-        cam_hitem_ptr.push_back((int)hf.size());
-    }
-
-    // ---- co-visibility through free points: column blocks g >= f of each S row
-    std::vector<uint64_t> bits(((size_t)nfc * nfc + 63) / 64, 0);
-    std::vector<int> fl;
-    for (int q = 0; q < np; ++q) {
-        if (!pt_free[q]) continue;
-        fl.clear();
-        for (int k = pt_ptr[q]; k < pt_ptr[q + 1]; ++k) {
-            const int f = cam_free[o_cam[k]];
-            if (f >= 0 && (fl.empty() || fl.back() != f)) fl.push_back(f);
-        }
-        for (size_t a = 0; a < fl.size(); ++a)
-            for (size_t b = a + 1; b < fl.size(); ++b) {
-                const size_t bit = (size_t)fl[a] * nfc + fl[b];
-                bits[bit >> 6] |= 1ull << (bit & 63);
-            }
-    }
-    std::vector<int> colptr{0}, colcam;
-    for (int f = 0; f < nfc; ++f) {
-        colcam.push_back(f);
-        for (int g2 = f + 1; g2 < nfc; ++g2) {
-            const size_t bit = (size_t)f * nfc + g2;
-            if (bits[bit >> 6] >> (bit & 63) & 1ull) colcam.push_back(g2);
-        }
-        colptr.push_back((int)colcam.size());
-    }
-
-    // ---- Schur work items: per row, column groups of kMaxSlots, chunks of kChunk
-    std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr{0}, row_nch;
-    std::vector<long long> spofs;
-    long long pofs = 0;
-    int max_ns = 1;
-    for (int f = 0; f < nfc; ++f) {
-        const int ncols = colptr[f + 1] - colptr[f];
-        const int ngroups = (ncols + kMaxSlots - 1) / kMaxSlots;
-        const int nobs = cam_ptr[f + 1] - cam_ptr[f];
-        const int nch = std::max(1, (nobs + kChunk - 1) / kChunk);
-        row_nch.push_back(nch);
-        for (int gi = 0; gi < ngroups; ++gi) {
-            const int ns = std::min(kMaxSlots, ncols - gi * kMaxSlots);
-            max_ns = std::max(max_ns, ns);
-            for (int ch = 0; ch < nch; ++ch) {
-                sf.push_back(f);
-                sa0.push_back(cam_ptr[f] + ch * kChunk);
-                sa1.push_back(std::min(cam_ptr[f] + (ch + 1) * kChunk, cam_ptr[f + 1]));
-                sg.push_back(gi);
-                sns.push_back(ns);
-                spofs.push_back(pofs);
-                pofs += 36LL * ns + 6;
-            }
-        }
-        row_sitem_ptr.push_back((int)sf.size());
-    }
-
-    // ---- initial state
-    std::vector<PoseQT> pose0(nc);
-    for (int c = 0; c < nc; ++c) {
-        const double* T = p->T_cw + 12 * (size_t)c;
-        host_quat_from_R(T, pose0[c].q);
-        pose0[c].t[0] = T[9]; pose0[c].t[1] = T[10]; pose0[c].t[2] = T[11];
-        pose0[c].pad = 0;
-    }
-    std::vector<D4> X0(np);
-    for (int q = 0; q < np; ++q) {
-        const double* x = p->X + 3 * (size_t)h->pt_orig[q];
-        X0[q] = compat ? D4{rf32(x[0]), rf32(x[1]), rf32(x[2]), 0.0} : D4{x[0], x[1], x[2], 0.0};
-    }
-
-    // ---- device upload
-    h->nbuf = 0;
-    DevProblem& P = h->P;
-    P = DevProblem{};
-    P.n_cam = nc; P.n_pt = np; P.n_obs = no; P.n_fc = nfc;
-    P.robust = o->robust ? 1 : 0;
-    P.delta = o->huber_delta;
-    P.fx = compat ? rf32(p->fx) : p->fx; P.fy = compat ? rf32(p->fy) : p->fy;
-    P.cx = compat ? rf32(p->cx) : p->cx; P.cy = compat ? rf32(p->cy) : p->cy;
-    int st;
-#define UP(dst, vec) if ((st = upload(h, &(dst), (vec))) != NAV_BA_OK) return st
-    UP(P.o_cam, o_cam); UP(P.o_pt, o_pt); UP(P.o_uv, o_uv); UP(P.o_orig, o_orig);
-    UP(P.pt_ptr, pt_ptr); UP(P.pt_free, pt_free); UP(P.cam_free, cam_free); UP(P.free_cam, free_cam);
-    UP(P.cam_ptr, cam_ptr); UP(P.cam_obs, cam_obs); UP(P.cam_obs_b0, cam_obs_b0);
-    UP(P.tile_obs, tile_obs); UP(P.tile_pt, tile_pt);
-    UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
-    UP(P.sitem_f, sf); UP(P.sitem_a0, sa0); UP(P.sitem_a1, sa1); UP(P.sitem_g, sg); UP(P.sitem_nslots, sns);
-    UP(P.sitem_pofs, spofs); UP(P.row_sitem_ptr, row_sitem_ptr); UP(P.row_nchunks, row_nch);
-    UP(P.colptr, colptr); UP(P.colcam, colcam);
-#undef UP
-    P.n_tiles = n_tiles;
-    P.n_hitems = (int)hf.size();
-    P.n_sitems = (int)sf.size();
-    P.max_ns = max_ns;
-    P.n = 6 * nfc;
-    P.n_pad = ((P.n + kNB - 1) / kNB) * kNB;
-    P.nt = P.n_pad / kNB;
-
-    const size_t tsz = mixed(h) ? sizeof(float) : sizeof(double);
-    auto A = [&](void** dst, size_t bytes) { return alloc(h, dst, bytes); };
-#define AL(dst, bytes) if ((st = A((void**)&(dst), (bytes))) != NAV_BA_OK) return st
-    AL(h->init.pose, sizeof(PoseQT) * nc); AL(h->init.X, sizeof(D4) * np); h->init.cam = nullptr;
-    AL(h->cur.pose, sizeof(PoseQT) * nc); AL(h->cur.cam, sizeof(CamRT) * nc); AL(h->cur.X, sizeof(D4) * np);
-    AL(h->cand.pose, sizeof(PoseQT) * nc); AL(h->cand.cam, sizeof(CamRT) * nc); AL(h->cand.X, sizeof(D4) * np);
-    DevWork& W = h->W;
-    W.n_cost_blocks = (no + 256 * 4 - 1) / (256 * 4);
-    if (W.n_cost_blocks == 0) W.n_cost_blocks = 1;
-    W.n_pt_blocks = (np + 255) / 256;
-    AL(W.w, tsz * no);
-    AL(W.Hl, tsz * 9 * (size_t)np);
-    AL(W.hpart, tsz * 27 * std::max<size_t>(hf.size(), 1));
-    AL(W.Hc, sizeof(double) * 28 * std::max(nfc, 1));
-    AL(W.ptsys, sizeof(double) * 12 * (size_t)np);
-    AL(W.spart, sizeof(double) * std::max<long long>(pofs, 1));
-    AL(W.S, sizeof(double) * std::max<size_t>((size_t)P.n_pad * P.n_pad, 1));
-    AL(W.g, sizeof(double) * std::max(P.n_pad, 1));
-    AL(W.y, sizeof(double) * std::max(P.n_pad, 1));
-    AL(W.dp, sizeof(double) * std::max(P.n_pad, 1));
-    AL(W.Lm, sizeof(double) * std::max<size_t>((size_t)P.n_pad * P.n_pad, 1));
-    AL(W.red_cost, sizeof(double) * W.n_cost_blocks);
-    AL(W.red_scale, sizeof(double) * (1 + W.n_pt_blocks));
-    AL(W.red_aux, sizeof(double) * W.n_cost_blocks);
-    AL(W.outlier, (size_t)no);
-    AL(W.lm, sizeof(LMDev));
-    AL(h->dfinal, 2 * sizeof(double));
-#undef AL
-    HIPTRY(h, hipMemcpyAsync(h->init.pose, pose0.data(), sizeof(PoseQT) * nc, hipMemcpyHostToDevice, h->stream));
-    HIPTRY(h, hipMemcpyAsync(h->init.X, X0.data(), sizeof(D4) * np, hipMemcpyHostToDevice, h->stream));
-    HIPTRY(h, hipStreamSynchronize(h->stream));
-    // Schur rows may use more than the default 64 KiB of LDS
-    if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
-        return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
-    const bool auto_lambda = !(o->lambda_init > 0);
-    if ((st = capture(h, &h->graphs.iter, 1)) != NAV_BA_OK) return st;
-    if (auto_lambda && (st = capture(h, &h->graphs.iter0, 0)) != NAV_BA_OK) return st;
-    if ((st = capture(h, &h->graphs.trial, 2)) != NAV_BA_OK) return st;
-    h->prepared = true;
-    return NAV_BA_OK;
+int32_t nav_ba_prepare_batched(nav_ba_handle* h, const nav_ba_problem* windows, int32_t n_windows,
+                               const nav_ba_options* o) {
+    return prepare_windows(h, windows, n_windows, o, true);
 }
 
 int32_t nav_ba_run(nav_ba_handle* h) {
@@ -599,112 +787,87 @@ int32_t nav_ba_run(nav_ba_handle* h) {
     if (!h->prepared) return fail(h, NAV_BA_NOT_PREPARED, "nav_ba_run before nav_ba_prepare");
     h->err.clear();
     h->ran = false;
-    h->iterations = 0; h->trials = 0; h->stop_reason = NAV_BA_STOP_MAX_ITER;
-    h->cost_history.assign(1, 0.0);
-    if (h->empty) {
-        h->stop_reason = NAV_BA_STOP_EMPTY;
-        h->initial_cost = h->final_cost = 0.0;
-        h->n_outliers = 0;
+    if (h->all_empty) {
         h->ran = true;
         return NAV_BA_EMPTY;
     }
     HIPTRY(h, hipSetDevice(h->device));
     hipStream_t s = h->stream;
-    const int nc = h->P.n_cam, np = h->P.n_pt;
+    const DevProblem& P = h->P;
     const bool m = mixed(h);
     HIPTRY(h, hipEventRecord(h->ev0, s));
-    HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * nc, hipMemcpyDeviceToDevice, s));
-    HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * np, hipMemcpyDeviceToDevice, s));
-    launch_lm_reset(h->W.lm, h->opt.lambda_init > 0 ? h->opt.lambda_init : 0.0, s);
-    launch_cam_rt(h->P, h->cur, s);
-    if (m) launch_cost<float>(h->P, h->cur, h->W, s); else launch_cost<double>(h->P, h->cur, h->W, s);
-    launch_lm_set_chi(h->W, s);
-    HIPTRY(h, hipMemcpyAsync(h->hstatus, h->W.lm, sizeof(LMDev), hipMemcpyDeviceToHost, s));
-    HIPTRY(h, hipStreamSynchronize(s));
-    h->initial_cost = h->hstatus->chi;
-    h->cost_history[0] = h->initial_cost;
-    const bool auto_lambda = !(h->opt.lambda_init > 0);
+    HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
+    HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
+    launch_lm_reset(P, h->W, s);
+    launch_cam_rt(P, h->cur, s);
+    if (m) launch_cost<float>(P, h->cur, h->W, 0, s); else launch_cost<double>(P, h->cur, h->W, 0, s);
+    launch_lm_set_chi(P, h->W, s);
+    // every window needs at most n_iter * max_trials steps; the host stops when all are done
+    const long long max_steps = (long long)P.n_iter * P.max_trials;
+    long long steps = 0;
     int st;
-    for (int it = 0; it < h->opt.n_iter; ++it) {
-        if ((st = launch_step(h, (it == 0 && auto_lambda) ? 0 : 1)) != NAV_BA_OK) return st;
-        HIPTRY(h, hipStreamSynchronize(s));
-        int q = 1;
-        double rho = h->hstatus->rho;
-        while (rho < 0 && q < h->opt.max_trials) {
-            if ((st = launch_step(h, 2)) != NAV_BA_OK) return st;
+    if (!h->batched) {
+        while (steps < max_steps) {
+            if ((st = launch_step(h)) != NAV_BA_OK) return st;
             HIPTRY(h, hipStreamSynchronize(s));
-            rho = h->hstatus->rho;
-            ++q;
+            ++steps;
+            if (h->hstatus->phase == kPhaseDone) break;
         }
-        h->trials += q;
-        h->iterations = it + 1;
-        h->cost_history.push_back(h->hstatus->chi);
-        if (q == h->opt.max_trials || rho == 0) {
-            h->stop_reason = NAV_BA_STOP_TERMINATE;
-            break;
+    } else {
+        long long burst = std::max(1, P.n_iter);
+        while (steps < max_steps) {
+            for (long long i = 0; i < burst && steps < max_steps; ++i, ++steps)
+                if ((st = launch_step(h)) != NAV_BA_OK) return st;
+            HIPTRY(h, hipStreamSynchronize(s));
+            if (*h->hdone >= P.n_win) break;
+            burst = 2;
         }
     }
-    h->final_lambda = h->hstatus->lambda;
     // outlier flags and the FP64 cost at the final state
     h->timing_active = h->timing;
     stage(h, NAV_BA_ST_FLAGS, [&] {
-        launch_flags(h->P, h->cur, h->W, h->opt.outlier_chi2, s);
-        launch_final(h->W, h->dfinal, s);
+        launch_flags(P, h->cur, h->W, h->opt.outlier_chi2, s);
+        launch_final(P, h->W, s);
     });
     h->timing_active = false;
-    HIPTRY(h, hipMemcpyAsync(h->hfinal, h->dfinal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPTRY(h, hipEventRecord(h->ev1, s));
+    h->lm_out.resize(P.n_win);
+    h->hist_out.resize((size_t)P.n_win * (P.n_iter + 1));
+    h->final_out.resize(2 * (size_t)P.n_win);
+    HIPTRY(h, hipMemcpyAsync(h->lm_out.data(), h->W.lm, sizeof(LMDev) * P.n_win, hipMemcpyDeviceToHost, s));
+    HIPTRY(h, hipMemcpyAsync(h->hist_out.data(), h->W.hist, sizeof(double) * h->hist_out.size(), hipMemcpyDeviceToHost, s));
+    HIPTRY(h, hipMemcpyAsync(h->final_out.data(), h->W.final, sizeof(double) * h->final_out.size(), hipMemcpyDeviceToHost, s));
     HIPTRY(h, hipStreamSynchronize(s));
     float ms = 0.f;
     HIPTRY(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->solve_ms = ms;
     if (h->timing) h->timer.collect();
-    h->final_cost = h->hfinal[0];
-    h->n_outliers = (int)h->hfinal[1];
     h->ran = true;
     return NAV_BA_OK;
 }
 
-int32_t nav_ba_fetch(nav_ba_handle* h, nav_ba_result* r) {
-    if (!h || !r) return NAV_BA_BAD_ARG;
-    if (!h->prepared || !h->ran) return fail(h, NAV_BA_NOT_PREPARED, "nav_ba_fetch before nav_ba_run");
-    const int nc = h->n_cam, npa = h->n_pt_all;
-    r->iterations = h->iterations; r->trials = h->trials; r->stop_reason = h->stop_reason;
-    r->initial_cost = h->initial_cost; r->final_cost = h->final_cost; r->final_lambda = h->final_lambda;
-    r->n_outliers = h->n_outliers; r->solve_ms = h->solve_ms;
-    if (r->cost_history)
-        for (size_t i = 0; i < h->cost_history.size(); ++i) r->cost_history[i] = h->cost_history[i];
-    if (h->empty) {
-        if (r->T_cw) std::memcpy(r->T_cw, h->T_in.data(), sizeof(double) * 12 * (size_t)nc);
-        if (r->X) std::memcpy(r->X, h->X_in.data(), sizeof(double) * 3 * (size_t)npa);
-        if (r->outlier) std::memset(r->outlier, 0, (size_t)h->n_obs);
-        return NAV_BA_EMPTY;
-    }
-    HIPTRY(h, hipSetDevice(h->device));
-    std::vector<PoseQT> pose(nc);
-    std::vector<D4> X(h->P.n_pt);
-    HIPTRY(h, hipMemcpyAsync(pose.data(), h->cur.pose, sizeof(PoseQT) * nc, hipMemcpyDeviceToHost, h->stream));
-    HIPTRY(h, hipMemcpyAsync(X.data(), h->cur.X, sizeof(D4) * X.size(), hipMemcpyDeviceToHost, h->stream));
-    if (r->outlier && h->n_obs > 0)
-        HIPTRY(h, hipMemcpyAsync(r->outlier, h->W.outlier, (size_t)h->n_obs, hipMemcpyDeviceToHost, h->stream));
-    HIPTRY(h, hipStreamSynchronize(h->stream));
-    if (r->T_cw)
-        for (int c = 0; c < nc; ++c) {
-            double* T = r->T_cw + 12 * (size_t)c;
-            host_R_from_quat(pose[c].q, T);  // BE_GraphOptim.cpp:220-223
-            T[9] = pose[c].t[0]; T[10] = pose[c].t[1]; T[11] = pose[c].t[2];
-        }
-    if (r->X) {
-        std::memcpy(r->X, h->X_in.data(), sizeof(double) * 3 * (size_t)npa);
-        const bool compat = h->opt.compat_fp32_io != 0;
-        for (size_t q = 0; q < X.size(); ++q) {
-            double* o = r->X + 3 * (size_t)h->pt_orig[q];
-            o[0] = compat ? rf32(X[q].x) : X[q].x;  // estimate().cast<float>() (BE_GraphOptim.cpp:239)
-            o[1] = compat ? rf32(X[q].y) : X[q].y;
-            o[2] = compat ? rf32(X[q].z) : X[q].z;
-        }
-    }
-    return NAV_BA_OK;
+int32_t nav_ba_fetch(nav_ba_handle* h, nav_ba_result* r) { return fetch_all(h, r, 1); }
+
+int32_t nav_ba_fetch_batched(nav_ba_handle* h, nav_ba_result* results, int32_t n_windows) {
+    return fetch_all(h, results, n_windows);
+}
+
+int32_t nav_ba_solve(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r) {
+    int st = nav_ba_prepare(h, p, o);
+    if (st != NAV_BA_OK) return st;
+    st = nav_ba_run(h);
+    if (st != NAV_BA_OK && st != NAV_BA_EMPTY) return st;
+    const int st2 = nav_ba_fetch(h, r);
+    return st2 != NAV_BA_OK ? st2 : st;
+}
+
+int32_t nav_ba_solve_batched(nav_ba_handle* h, const nav_ba_problem* windows, int32_t n_windows,
+                             const nav_ba_options* o, nav_ba_result* results) {
+    int st = nav_ba_prepare_batched(h, windows, n_windows, o);
+    if (st != NAV_BA_OK) return st;
+    st = nav_ba_run(h);
+    if (st != NAV_BA_OK && st != NAV_BA_EMPTY) return st;
+    return nav_ba_fetch_batched(h, results, n_windows);
 }
 
 int32_t nav_ba_set_timing(nav_ba_handle* h, int32_t enable) {
@@ -724,24 +887,23 @@ int32_t nav_ba_debug_reduced_system(nav_ba_handle* h, double lambda, double* S, 
                                     int32_t max_n, int32_t* n_out) {
     if (!h || !n_out) return NAV_BA_BAD_ARG;
     if (!h->prepared) return fail(h, NAV_BA_NOT_PREPARED, "debug call before nav_ba_prepare");
-    if (h->empty) { *n_out = 0; return NAV_BA_EMPTY; }
+    if (h->batched) return fail(h, NAV_BA_BAD_ARG, "debug reduced system is for single problems");
+    if (h->all_empty) { *n_out = 0; return NAV_BA_EMPTY; }
     HIPTRY(h, hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const DevProblem& P = h->P;
     *n_out = P.n;
     HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
     HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
-    launch_lm_reset(h->W.lm, lambda, s);
+    LMDev L{};
+    L.lambda = lambda;
+    L.nu = 2.0;
+    L.phase = kPhaseLin;
+    L.iter = 1;  // skips the lambda initialisation
+    HIPTRY(h, hipMemcpyAsync(h->W.lm, &L, sizeof(LMDev), hipMemcpyHostToDevice, s));
     launch_cam_rt(P, h->cur, s);
-    if (mixed(h)) {
-        enqueue_lin<float>(h, false);
-        launch_hinv<float>(P, h->W, s);
-        if (P.n_fc > 0) { launch_schur_row<float>(P, h->cur, h->W, s); launch_schur_reduce<float>(P, h->W, s); }
-    } else {
-        enqueue_lin<double>(h, false);
-        launch_hinv<double>(P, h->W, s);
-        if (P.n_fc > 0) { launch_schur_row<double>(P, h->cur, h->W, s); launch_schur_reduce<double>(P, h->W, s); }
-    }
+    if (mixed(h)) { enqueue_lin<float>(h); enqueue_reduced_system<float>(h); }
+    else { enqueue_lin<double>(h); enqueue_reduced_system<double>(h); }
     HIPTRY(h, hipStreamSynchronize(s));
     if (P.n > max_n) return NAV_BA_OK;
     std::vector<double> Sd((size_t)P.n_pad * P.n_pad), gd(P.n_pad);
@@ -755,15 +917,6 @@ int32_t nav_ba_debug_reduced_system(nav_ba_handle* h, double lambda, double* S, 
         }
     }
     return NAV_BA_OK;
-}
-
-int32_t nav_ba_solve(nav_ba_handle* h, const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r) {
-    int st = nav_ba_prepare(h, p, o);
-    if (st != NAV_BA_OK) return st;
-    st = nav_ba_run(h);
-    if (st != NAV_BA_OK && st != NAV_BA_EMPTY) return st;
-    const int st2 = nav_ba_fetch(h, r);
-    return st2 != NAV_BA_OK ? st2 : st;
 }
 
 }  // extern "C"

@@ -7,16 +7,20 @@
 // Here one workgroup owns one block row f of S (restricted to the cameras
 // co-visible with f, and to one observation chunk of camera f):
 //   phase A  one lane per observation a of camera f on a free point p:
-//            W_a = Hpl_a Dinv_p, the point's coordinates and the range of its
-//            observations on cameras >= f go to a per-wave LDS buffer;
+//            W_a = Hpl_a Dinv_p (FP64), the point's coordinates and the range
+//            of its observations on cameras >= f go to a per-wave LDS buffer;
 //   phase B  the wave's (point, later observation b) pairs are packed densely
 //            over the 64 lanes (prefix sum + binary search), so the lanes of
 //            one point hit distinct S-row slots; each lane forms
 //            W_a Hpl_b^T = (W_a J_X,b^T)(w_b J_xi,b) from recomputed Jacobians
 //            and adds it into the LDS slot of camera(b) (slot 0 = camera f).
-// The S-row slots accumulate in FP64 for both compute types (the reduced
-// camera system is FP64; on this GPU ds_add_f64 is also the fast LDS atomic,
-// ds_add_f32 measured 8x slower on the same access pattern).
+// T is the type of the residuals and Jacobians only.  W, the pair products and
+// the S-row slots are FP64 for both compute types: the point elimination
+// cancels Hpp against Hpl Dinv Hpl^T, and with a few-view point's Dinv
+// rounded to FP32 the final cost of an uneven window moved by 1e-3 relative
+// in a CPU emulation, against 1e-6 for FP32 Jacobians alone.  (On this GPU
+// ds_add_f64 is also the fast LDS atomic; ds_add_f32 measured 8x slower on
+// the same access pattern.)
 // LDS layouts avoid bank conflicts: the S-row accumulator is entry-major
 // (acc[e][slot], lanes on different slots hit consecutive words) and the
 // camera table uses an odd stride of 13 values.
@@ -34,13 +38,13 @@ constexpr int kCamStride = 13;  // odd: consecutive cameras fall in different LD
 
 template <typename T>
 struct SchurLayout {
-    static constexpr int kWbuf = 21;  // W (18) + point (3)
+    static constexpr int kWbuf = 21;  // W (18) + point (3), FP64
     size_t acc, cam, wbuf, lo, pre, slot, total;
-    __host__ __device__ SchurLayout(int ns, int n_cam, bool cam_lds) {
+    __host__ __device__ SchurLayout(int ns, int n_cam, bool cam_lds) {  // n_cam: cameras of the largest window
         size_t o = 0;
         acc = o;  o += sizeof(double) * 36 * (size_t)ns;
         cam = o;  o += cam_lds ? sizeof(T) * kCamStride * (size_t)n_cam : 0;
-        wbuf = o; o += sizeof(T) * kWbuf * kBlock;
+        wbuf = o; o += sizeof(double) * kWbuf * kBlock;
         lo = o;   o += sizeof(int) * kBlock;
         pre = o;  o += sizeof(int) * (kBlock / kWave) * (kWave + 1);
         slot = o; o += sizeof(short) * (size_t)n_cam;
@@ -48,8 +52,9 @@ struct SchurLayout {
     }
 };
 
+// Camera c of the window (local index; cam0 = the window's first camera).
 template <typename T, bool kCamLds>
-__device__ __forceinline__ void cam_get(const T* camt, const CamRT* cam, int c, T R[9], T t[3]) {
+__device__ __forceinline__ void cam_get(const T* camt, const CamRT* cam, int c, int cam0, T R[9], T t[3]) {
     if (kCamLds) {
         const T* p = camt + kCamStride * c;
 #pragma unroll
@@ -57,49 +62,54 @@ __device__ __forceinline__ void cam_get(const T* camt, const CamRT* cam, int c, 
 #pragma unroll
         for (int k = 0; k < 3; ++k) t[k] = p[9 + k];
     } else {
-        load_cam(cam, c, R, t);
+        load_cam(cam, cam0 + c, R, t);
     }
 }
 
 template <typename T, bool kCamLds>
 __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT* __restrict__ cam,
                                                       const D4* __restrict__ X, const T* __restrict__ w,
-                                                      const double* __restrict__ ptsys, double* __restrict__ spart) {
+                                                      const double* __restrict__ ptsys, const LMDev* lm,
+                                                      double* __restrict__ spart) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int it = blockIdx.x;
     const int f = P.sitem_f[it], a0 = P.sitem_a0[it], a1 = P.sitem_a1[it];
     const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
     const int cf = P.free_cam[f];
+    const int win = P.cam_win[cf];
+    if (!running(lm, win)) return;
+    // cameras are addressed relative to the window's first camera
+    const int cam0 = P.win_cam0[win], ncw = P.win_cam0[win + 1] - cam0;
     const bool g0 = (g == 0);
-    const SchurLayout<T> L(ns, P.n_cam, kCamLds);
+    const SchurLayout<T> L(ns, P.max_win_cams, kCamLds);
     double* acc = reinterpret_cast<double*>(smem + L.acc);
     T* camt = reinterpret_cast<T*>(smem + L.cam);
-    T* wbuf_all = reinterpret_cast<T*>(smem + L.wbuf);
+    double* wbuf_all = reinterpret_cast<double*>(smem + L.wbuf);
     int* lo_all = reinterpret_cast<int*>(smem + L.lo);
     int* pre_all = reinterpret_cast<int*>(smem + L.pre);
     short* slot = reinterpret_cast<short*>(smem + L.slot);
 
     const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
     for (int i = tid; i < ns * 36; i += kBlock) acc[i] = 0.0;
-    for (int i = tid; i < P.n_cam; i += kBlock) slot[i] = -1;
+    for (int i = tid; i < ncw; i += kBlock) slot[i] = -1;
     if (kCamLds)
-        for (int i = tid; i < P.n_cam * 12; i += kBlock)
-            camt[kCamStride * (i / 12) + i % 12] = (T)reinterpret_cast<const double*>(cam)[i];
+        for (int i = tid; i < ncw * 12; i += kBlock)
+            camt[kCamStride * (i / 12) + i % 12] = (T)reinterpret_cast<const double*>(cam + cam0)[i];
     __syncthreads();
     const int col0 = P.colptr[f] + g * kMaxSlots;
-    for (int s = tid; s < ns; s += kBlock) slot[P.free_cam[P.colcam[col0 + s]]] = (short)s;
+    for (int s = tid; s < ns; s += kBlock) slot[P.free_cam[P.colcam[col0 + s]] - cam0] = (short)s;
     __syncthreads();
 
     const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy;
     T Rf[9], tf[3];
-    cam_get<T, kCamLds>(camt, cam, cf, Rf, tf);
-    T* wbuf = wbuf_all + (size_t)wid * kWave * SchurLayout<T>::kWbuf;
+    cam_get<T, kCamLds>(camt, cam, cf - cam0, cam0, Rf, tf);
+    double* wbuf = wbuf_all + (size_t)wid * kWave * SchurLayout<T>::kWbuf;
     int* lo_w = lo_all + wid * kWave;
     int* pre = pre_all + wid * (kWave + 1);
 
-    T coef[6];
+    double coef[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) coef[k] = T(0);
+    for (int k = 0; k < 6; ++k) coef[k] = 0.0;
 
     for (int kb = a0 + wid * kWave; kb < a1; kb += kBlock) {
         wave_fence();
@@ -114,27 +124,30 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 const T x = (T)xp.x, y = (T)xp.y, z = (T)xp.z;
                 EdgeLin<T> La;
                 edge_linearize(Rf, tf, x, y, z, fx, fy, cx, cy, T(0), T(0), La);
-                const T wa = w[a];
-                T Ha[18];
+                const double wa = (double)w[a];
+                double JXa[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) JXa[j] = (double)La.JX[j];
+                double Ha[18];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    const T A0 = La.Jxi[i] * wa, A1 = La.Jxi[6 + i] * wa;
+                    const double A0 = (double)La.Jxi[i] * wa, A1 = (double)La.Jxi[6 + i] * wa;
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) Ha[3 * i + j] = A0 * La.JX[j] + A1 * La.JX[3 + j];
+                    for (int j = 0; j < 3; ++j) Ha[3 * i + j] = A0 * JXa[j] + A1 * JXa[3 + j];
                 }
                 const double* ps = ptsys + 12 * (size_t)p;
-                const T d00 = (T)ps[0], d01 = (T)ps[1], d02 = (T)ps[2], d11 = (T)ps[3], d12 = (T)ps[4], d22 = (T)ps[5];
-                T* wb = wbuf + lane * SchurLayout<T>::kWbuf;
+                const double d00 = ps[0], d01 = ps[1], d02 = ps[2], d11 = ps[3], d12 = ps[4], d22 = ps[5];
+                double* wb = wbuf + lane * SchurLayout<T>::kWbuf;
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    const T h0 = Ha[3 * i], h1 = Ha[3 * i + 1], h2 = Ha[3 * i + 2];
+                    const double h0 = Ha[3 * i], h1 = Ha[3 * i + 1], h2 = Ha[3 * i + 2];
                     wb[3 * i + 0] = h0 * d00 + h1 * d01 + h2 * d02;
                     wb[3 * i + 1] = h0 * d01 + h1 * d11 + h2 * d12;
                     wb[3 * i + 2] = h0 * d02 + h1 * d12 + h2 * d22;
                 }
-                wb[18] = x; wb[19] = y; wb[20] = z;
+                wb[18] = xp.x; wb[19] = xp.y; wb[20] = xp.z;
                 if (g0) {
-                    const T e0 = (T)ps[6], e1 = (T)ps[7], e2 = (T)ps[8];
+                    const double e0 = ps[6], e1 = ps[7], e2 = ps[8];
 #pragma unroll
                     for (int i = 0; i < 6; ++i) coef[i] += Ha[3 * i] * e0 + Ha[3 * i + 1] * e1 + Ha[3 * i + 2] * e2;
                 }
@@ -164,38 +177,41 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
             }
             const int i = lo_i;
             const int b = lo_w[i] + (t - pre[i]);
-            const int cb = P.o_cam[b];
+            const int cb = P.o_cam[b] - cam0;
             const int s = slot[cb];  // camera f itself is slot 0 of group 0
             if (s < 0) continue;
-            const T* wb = wbuf + i * SchurLayout<T>::kWbuf;
+            const double* wb = wbuf + i * SchurLayout<T>::kWbuf;
             T Rb[9], tb[3];
-            cam_get<T, kCamLds>(camt, cam, cb, Rb, tb);
+            cam_get<T, kCamLds>(camt, cam, cb, cam0, Rb, tb);
             EdgeLin<T> Lb;
-            edge_linearize(Rb, tb, wb[18], wb[19], wb[20], fx, fy, cx, cy, T(0), T(0), Lb);
-            const T wgt = w[b];
-            T M0[6], M1[6];
+            edge_linearize(Rb, tb, (T)wb[18], (T)wb[19], (T)wb[20], fx, fy, cx, cy, T(0), T(0), Lb);
+            const double wgt = (double)w[b];
+            double JXb[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) JXb[j] = (double)Lb.JX[j];
+            double M0[6], M1[6];
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
-                const T w0 = wb[3 * r], w1 = wb[3 * r + 1], w2 = wb[3 * r + 2];
-                M0[r] = w0 * Lb.JX[0] + w1 * Lb.JX[1] + w2 * Lb.JX[2];
-                M1[r] = w0 * Lb.JX[3] + w1 * Lb.JX[4] + w2 * Lb.JX[5];
+                const double w0 = wb[3 * r], w1 = wb[3 * r + 1], w2 = wb[3 * r + 2];
+                M0[r] = w0 * JXb[0] + w1 * JXb[1] + w2 * JXb[2];
+                M1[r] = w0 * JXb[3] + w1 * JXb[4] + w2 * JXb[5];
             }
-            T B0[6], B1[6];
+            double B0[6], B1[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) { B0[c] = wgt * Lb.Jxi[c]; B1[c] = wgt * Lb.Jxi[6 + c]; }
+            for (int c = 0; c < 6; ++c) { B0[c] = wgt * (double)Lb.Jxi[c]; B1[c] = wgt * (double)Lb.Jxi[6 + c]; }
 #pragma unroll
             for (int r = 0; r < 6; ++r)
 #pragma unroll
                 for (int c = 0; c < 6; ++c)
-                    atomicAdd(acc + (6 * r + c) * ns + s, (double)(M0[r] * B0[c] + M1[r] * B1[c]));
+                    atomicAdd(acc + (6 * r + c) * ns + s, M0[r] * B0[c] + M1[r] * B1[c]);
         }
     }
     // ---- coefficient: fixed-order block reduction; S-row slots out (slot-major)
-    __shared__ T red[kBlock / kWave][6];
+    __shared__ double red[kBlock / kWave][6];
     if (g0) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            const T v = wave_sum(coef[k]);
+            const double v = wave_sum(coef[k]);
             if (lane == 0) red[wid][k] = v;
         }
     }
@@ -203,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     double* out = spart + P.sitem_pofs[it];
     if (g0 && tid < 6) {
         double s = 0.0;
-        for (int i = 0; i < kBlock / kWave; ++i) s += (double)red[i][tid];
+        for (int i = 0; i < kBlock / kWave; ++i) s += red[i][tid];
         out[36 * ns + tid] = s;
     }
     for (int i = tid; i < ns * 36; i += kBlock) out[i] = acc[(i % 36) * ns + i / 36];
@@ -211,14 +227,14 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
 
 template <typename T>
 size_t schur_row_smem(const DevProblem& P) {
-    return SchurLayout<T>(P.max_ns, P.n_cam, P.n_cam <= kCamLdsMax).total;
+    return SchurLayout<T>(P.max_ns, P.max_win_cams, P.max_win_cams <= kCamLdsMax).total;
 }
 
 template <typename T>
 int prepare_schur_row(const DevProblem& P) {
     const size_t bytes = schur_row_smem<T>(P);
     if (bytes > 160 * 1024) return -1;
-    hipError_t e = P.n_cam <= kCamLdsMax
+    hipError_t e = P.max_win_cams <= kCamLdsMax
         ? hipFuncSetAttribute((const void*)k_schur_row<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)
         : hipFuncSetAttribute((const void*)k_schur_row<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     return e == hipSuccess ? 0 : -2;
@@ -228,12 +244,12 @@ template <typename T>
 void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st) {
     if (P.n_sitems == 0) return;
     const size_t smem = schur_row_smem<T>(P);
-    if (P.n_cam <= kCamLdsMax)
+    if (P.max_win_cams <= kCamLdsMax)
         hipLaunchKernelGGL((k_schur_row<T, true>), dim3(P.n_sitems), dim3(kBlock), smem, st, P, s.cam, s.X,
-                           (const T*)W.w, W.ptsys, (double*)W.spart);
+                           (const T*)W.w, W.ptsys, (const LMDev*)W.lm, W.spart);
     else
         hipLaunchKernelGGL((k_schur_row<T, false>), dim3(P.n_sitems), dim3(kBlock), smem, st, P, s.cam, s.X,
-                           (const T*)W.w, W.ptsys, (double*)W.spart);
+                           (const T*)W.w, W.ptsys, (const LMDev*)W.lm, W.spart);
 }
 
 template void launch_schur_row<double>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);
