@@ -162,6 +162,49 @@ def batched_leg(n_windows, steps, warmup):
     return out
 
 
+def copy_calibration(mb, steps):
+    """A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
+    scales the same counters on the solver kernels."""
+    import torch
+    n = mb * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"copy_mib": mb, "bytes_read": n * 4, "bytes_written": n * 4,
+                      "ms": round(dt * 1e3, 4), "GBps_read_plus_write": round(2 * n * 4 / dt / 1e9, 1)}), flush=True)
+
+
+def batched_profile(n_windows, steps, warmup):
+    import torch
+    from nav24_amd import problems
+    from nav24_amd.abi import NAV_BA_MIXED, workload_options
+    from nav24_amd.ba import GraphOptimHip
+    wins = [problems.workload(3, seed=w) for w in range(n_windows)]
+    o = workload_options(3)
+    o.precision = NAV_BA_MIXED
+    s = GraphOptimHip(0)
+    s.prepare_batched(wins, o)
+    for _ in range(warmup):
+        s.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    s.set_timing(True)
+    s.run()
+    tm = s.timing()
+    s.close()
+    print(json.dumps({"batched_windows": n_windows, "ms_per_batch": round(dt * 1e3, 3),
+                      "stages_ms": {k: round(v[0], 4) for k, v in tm.items() if v[1] > 0}}), flush=True)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -182,6 +225,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--windows", type=int, default=512, help="batched leg: number of windows (0 = skip)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="profiling: run only the batched mixed solve (K timed runs) and print its timing")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="profiling: K device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -189,6 +236,12 @@ def main():
         sys.exit(2)
 
     import torch  # device synchronisation around the timed region
+    if args.copy_mb > 0:
+        copy_calibration(args.copy_mb, args.steps)
+        return
+    if args.batched_only:
+        batched_profile(args.windows, args.steps, args.warmup)
+        return
     from nav24_amd import problems
     from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
     from nav24_amd.ba import GraphOptimHip

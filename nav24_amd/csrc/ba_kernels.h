@@ -87,6 +87,7 @@ struct DevProblem {
     // windows
     int n_win;
     int max_win_cams;
+    int max_win_fc;            // free cameras of the largest window
     int batched;               // 1: per-window dense systems factored in LDS
     const int* cam_win;        // [n_cam]
     const int* pt_win;         // [n_pt]
@@ -165,7 +166,7 @@ template <typename T> void launch_schur_row(const DevProblem& P, const DevState&
 template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st);
 void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st);
 void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st);
-int prepare_cholesky_small();  // LDS attribute of the per-window factorisation
+int prepare_cholesky_small(const DevProblem& P);  // LDS attribute of the per-window factorisation
 template <typename T> void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand,
                                           const DevWork& W, hipStream_t st);
 void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);

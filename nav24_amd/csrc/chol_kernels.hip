@@ -274,91 +274,233 @@ __global__ __launch_bounds__(256) void k_chol_solve(const double* __restrict__ L
 }
 
 // ---------------------------------------------------------------- per-window systems
-// One workgroup per running window: the window's reduced system (n <= 180)
-// is copied to LDS as a packed lower triangle (row i starts at i(i+1)/2),
-// factored column by column (2 barriers per column: scale the column, then
-// the rank-1 update by waves over rows and lanes over columns), and solved by
-// forward and backward substitution in wave 0 with readlane broadcasts (lane
-// m keeps rows m, m+64, m+128 of the right-hand side).
-__device__ __forceinline__ int pk(int i, int j) { return i * (i + 1) / 2 + j; }
+// One workgroup (512 threads) per running window.  The window's reduced
+// system (n = 6 nb unknowns, nb <= 30 free cameras) is held in LDS as a packed
+// lower triangle of 6x6 camera blocks (block (I, J), J <= I, at
+// (I(I+1)/2 + J) * kBS doubles, row-major inside; the odd stride kBS = 37
+// spreads neighbouring blocks over LDS banks).  Right-looking block Cholesky,
+// per block column K:
+//   diag   L_KK = chol(A_KK), M_K = L_KK^-1 and y_K = M_K y_K, in registers of
+//          one thread (1/sqrt by rsq + two Newton steps) - done by the thread
+//          that applied the last update to A_KK, right after it;
+//   panel  L_IK = A_IK M_K^T and y_I -= L_IK y_K, one thread per row of a block;
+//   update A_IJ -= L_IK L_JK^T for K < J <= I, one thread per 6x6 block (36
+//          register accumulators).
+// Two barriers per block column.  Backward substitution x_K = M_K^T (y_K -
+// sum_{I>K} L_IK^T x_I) runs block by block.  A pivot that is not positive and
+// finite marks the window failed (dp = 0, lm.fail = 1), as the tiled path does.
+constexpr int kBS = 37;
+constexpr int kSmallThreads = 512;
 
-__device__ __forceinline__ double sel3(double a, double b, double c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+__device__ __forceinline__ int blk_off(int I, int J) { return (I * (I + 1) / 2 + J) * kBS; }
 
-__global__ __launch_bounds__(256) void k_chol_small(DevProblem P, const double* __restrict__ S,
-                                                    const double* __restrict__ g, double* __restrict__ dp, LMDev* lm) {
-    extern __shared__ double Lp[];
+// Lower-triangular 6x6 Cholesky and inverse of the block at a (row-major 6x6,
+// lower part read), result L written back to a (lower part) and M = L^-1 to m
+// (full 6x6, upper part zero); y <- M y.  Returns false on a bad pivot.
+__device__ __forceinline__ bool diag_factor(double* a, double* m, double* y) {
+    double l[6][6], inv[6];
+    bool ok_all = true;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double d = a[6 * j + j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= l[j][k] * l[j][k];
+        const bool ok = (d > 0.0) && isfinite(d);
+        ok_all = ok_all && ok;
+        d = ok ? d : 1.0;
+        double r = __builtin_amdgcn_rsq(d);
+        r = r * (1.5 - 0.5 * d * r * r);
+        r = r * (1.5 - 0.5 * d * r * r);
+        l[j][j] = d * r;
+        inv[j] = r;
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double v = a[6 * i + j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) v -= l[i][k] * l[j][k];
+            l[i][j] = v * r;
+        }
+    }
+    double M[6][6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) M[i][j] = 0.0;
+        M[j][j] = inv[j];
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = j; k < i; ++k) v += l[i][k] * M[k][j];
+            M[i][j] = -v * inv[i];
+        }
+    }
+    double yn[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k <= i; ++k) v += M[i][k] * y[k];
+        yn[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        y[i] = yn[i];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (j <= i) a[6 * i + j] = l[i][j];
+            m[6 * i + j] = M[i][j];
+        }
+    }
+    return ok_all;
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, const double* __restrict__ S,
+                                                              const double* __restrict__ g,
+                                                              double* __restrict__ dp, LMDev* lm) {
+    extern __shared__ double sm[];
+    __shared__ double xk[6];
     __shared__ int bad;
     const int win = blockIdx.x;
     if (!running(lm, win)) return;
-    const int n = 6 * (P.win_fc0[win + 1] - P.win_fc0[win]);
-    if (n == 0) return;
+    const int nb = P.win_fc0[win + 1] - P.win_fc0[win];
+    if (nb == 0) return;
+    const int n = 6 * nb;
+    const int nblk = nb * (nb + 1) / 2;
+    double* Lb = sm;                      // nblk * kBS
+    double* Mb = sm + (size_t)nblk * kBS;  // nb * kBS: M_K = L_KK^-1
+    double* yv = Mb + (size_t)nb * kBS;    // n
     const double* Sw = S + P.win_S[win];
     const int lda = P.win_lda[win];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    for (int i = wid; i < n; i += 4)
-        for (int j = lane; j <= i; j += 64) Lp[pk(i, j)] = Sw[(size_t)i * lda + j];
+    const int g0 = 6 * P.win_fc0[win];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < nblk * 36; idx += kSmallThreads) {
+        const int b = idx / 36, e = idx - 36 * (idx / 36);
+        int I = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+        while ((I + 1) * (I + 2) / 2 <= b) ++I;
+        while (I * (I + 1) / 2 > b) --I;
+        const int J = b - I * (I + 1) / 2;
+        const int r = e / 6, c = e - 6 * (e / 6);
+        Lb[b * kBS + e] = Sw[(size_t)(6 * I + r) * lda + 6 * J + c];
+    }
+    for (int i = tid; i < n; i += kSmallThreads) yv[i] = g[g0 + i];
     if (tid == 0) bad = 0;
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        double d = Lp[pk(k, k)];
-        const bool ok = (d > 0.0) && isfinite(d);
-        d = ok ? d : 1.0;
-        const double piv = sqrt(d);
-        const double inv = 1.0 / piv;
-        for (int i = k + 1 + tid; i < n; i += 256) Lp[pk(i, k)] *= inv;
-        if (tid == 0) { if (!ok) bad = 1; }
-        __syncthreads();
-        for (int i = k + 1 + wid; i < n; i += 4) {
-            const double lik = Lp[pk(i, k)];
-            for (int j = k + 1 + lane; j <= i; j += 64) Lp[pk(i, j)] -= lik * Lp[pk(j, k)];
+    if (tid == 0) {
+        if (!diag_factor(Lb + blk_off(0, 0), Mb, yv)) bad = 1;
+    }
+    __syncthreads();
+    for (int K = 0; K < nb; ++K) {
+        // panel: L_IK = A_IK M_K^T, y_I -= L_IK y_K
+        const int rows = 6 * (nb - K - 1);
+        const double* M = Mb + K * kBS;
+        for (int t = tid; t < rows; t += kSmallThreads) {
+            const int I = K + 1 + t / 6, r = t - 6 * (t / 6);
+            double* row = Lb + blk_off(I, K) + 6 * r;
+            double a[6], o[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a[c] = row[c];
+            double yr = 0.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                double v = 0.0;
+#pragma unroll
+                for (int m = 0; m <= c; ++m) v += a[m] * M[6 * c + m];
+                o[c] = v;
+                yr += v * yv[6 * K + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) row[c] = o[c];
+            yv[6 * I + r] -= yr;
         }
-        if (tid == 0) Lp[pk(k, k)] = piv;
+        __syncthreads();
+        // trailing update A_IJ -= L_IK L_JK^T; block (K+1, K+1) is item 0 and its
+        // thread factors it right away
+        const int rem = nb - K - 1;
+        const int nupd = rem * (rem + 1) / 2;
+        for (int q = tid; q < nupd; q += kSmallThreads) {
+            // q -> (I, J) with I, J in K+1.., J <= I, column-major from the diagonal
+            int jj = 0, qq = q, len = rem;
+            while (qq >= len) { qq -= len; ++jj; --len; }
+            const int J = K + 1 + jj, I = J + qq;
+            const double* A = Lb + blk_off(I, K);
+            const double* B = Lb + blk_off(J, K);
+            double* C = Lb + blk_off(I, J);
+            double acc[36];
+#pragma unroll
+            for (int e = 0; e < 36; ++e) acc[e] = C[e];
+            double bm[36];
+#pragma unroll
+            for (int e = 0; e < 36; ++e) bm[e] = B[e];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                double ar[6];
+#pragma unroll
+                for (int m = 0; m < 6; ++m) ar[m] = A[6 * r + m];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    double v = acc[6 * r + c];
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) v -= ar[m] * bm[6 * c + m];
+                    acc[6 * r + c] = v;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 36; ++e) C[e] = acc[e];
+            if (q == 0) {  // block (K+1, K+1): this thread's own writes are visible to it
+                double yk[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) yk[i] = yv[6 * J + i];
+                if (!diag_factor(C, Mb + J * kBS, yk)) bad = 1;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) yv[6 * J + i] = yk[i];
+            }
+        }
         __syncthreads();
     }
-    const int g0 = 6 * P.win_fc0[win];
     if (bad) {
-        for (int i = tid; i < n; i += 256) dp[g0 + i] = 0.0;
+        for (int i = tid; i < n; i += kSmallThreads) dp[g0 + i] = 0.0;
         if (tid == 0) lm[win].fail = 1;
         return;
     }
-    if (wid == 0) {
-        // forward: L y = g
-        double r0 = lane < n ? g[g0 + lane] : 0.0;
-        double r1 = lane + 64 < n ? g[g0 + lane + 64] : 0.0;
-        double r2 = lane + 128 < n ? g[g0 + lane + 128] : 0.0;
-        for (int c = 0; c < n; ++c) {
-            const int cs = c >> 6, cl = c & 63;
-            const double yc = readlane(sel3(r0, r1, r2, cs), cl) / Lp[pk(c, c)];
-            if (lane == cl) { if (cs == 0) r0 = yc; else if (cs == 1) r1 = yc; else r2 = yc; }
-            const int m0 = lane, m1 = lane + 64, m2 = lane + 128;
-            if (m0 > c && m0 < n) r0 -= Lp[pk(m0, c)] * yc;
-            if (m1 > c && m1 < n) r1 -= Lp[pk(m1, c)] * yc;
-            if (m2 > c && m2 < n) r2 -= Lp[pk(m2, c)] * yc;
+    // backward: x_K = M_K^T (y_K - sum_{I>K} L_IK^T x_I); y holds the updated right-hand sides
+    for (int K = nb - 1; K >= 0; --K) {
+        if (tid < 6) {
+            const double* M = Mb + K * kBS;
+            double v = 0.0;
+            for (int m = tid; m < 6; ++m) v += M[6 * m + tid] * yv[6 * K + m];
+            xk[tid] = v;
         }
-        // backward: L^T x = y
-        for (int c = n - 1; c >= 0; --c) {
-            const int cs = c >> 6, cl = c & 63;
-            const double xc = readlane(sel3(r0, r1, r2, cs), cl) / Lp[pk(c, c)];
-            if (lane == cl) { if (cs == 0) r0 = xc; else if (cs == 1) r1 = xc; else r2 = xc; }
-            const int m0 = lane, m1 = lane + 64, m2 = lane + 128;
-            if (m0 < c) r0 -= Lp[pk(c, m0)] * xc;
-            if (m1 < c) r1 -= Lp[pk(c, m1)] * xc;
-            if (m2 < c) r2 -= Lp[pk(c, m2)] * xc;
+        __syncthreads();
+        if (tid < 6) {
+            yv[6 * K + tid] = xk[tid];
+            dp[g0 + 6 * K + tid] = xk[tid];
         }
-        if (lane < n) dp[g0 + lane] = r0;
-        if (lane + 64 < n) dp[g0 + lane + 64] = r1;
-        if (lane + 128 < n) dp[g0 + lane + 128] = r2;
+        for (int t = tid; t < 6 * K; t += kSmallThreads) {
+            const int J = t / 6, c = t - 6 * (t / 6);
+            const double* L = Lb + blk_off(K, J);
+            double v = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) v += L[6 * r + c] * xk[r];
+            yv[t] -= v;
+        }
+        __syncthreads();
     }
 }
 
-void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st) {
-    const size_t smem = sizeof(double) * ((size_t)kSmallMaxN * (kSmallMaxN + 1) / 2);
-    hipLaunchKernelGGL(k_chol_small, dim3(P.n_win), dim3(256), smem, st, P, (const double*)W.S, (const double*)W.g,
-                       W.dp, W.lm);
+size_t cholesky_small_smem(const DevProblem& P) {
+    const size_t nb = (size_t)P.max_win_fc;
+    return sizeof(double) * (nb * (nb + 1) / 2 * kBS + nb * kBS + 6 * nb);
 }
 
-int prepare_cholesky_small() {
-    const size_t smem = sizeof(double) * ((size_t)kSmallMaxN * (kSmallMaxN + 1) / 2);
+void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st) {
+    hipLaunchKernelGGL(k_chol_small, dim3(P.n_win), dim3(kSmallThreads), cholesky_small_smem(P), st, P,
+                       (const double*)W.S, (const double*)W.g, W.dp, W.lm);
+}
+
+int prepare_cholesky_small(const DevProblem& P) {
+    const size_t smem = cholesky_small_smem(P);
+    if (smem > 160 * 1024) return -1;
     return hipFuncSetAttribute((const void*)k_chol_small, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) ==
                    hipSuccess ? 0 : -1;
 }

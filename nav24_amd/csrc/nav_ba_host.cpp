@@ -600,6 +600,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     P.max_ns = max_ns;
     P.n_win = n_win;
     P.max_win_cams = max_win_cams;
+    P.max_win_fc = 0;
+    for (int w = 0; w < n_win; ++w) P.max_win_fc = std::max(P.max_win_fc, win_fc0[w + 1] - win_fc0[w]);
     P.batched = batched ? 1 : 0;
     P.n_cblk = (int)cblk_win.size();
     P.n_pblk = (int)pblk_win.size();
@@ -641,7 +643,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     // Schur rows and the per-window factorisation may use more than 64 KiB of LDS
     if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
         return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
-    if (batched && prepare_cholesky_small() != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
+    if (batched && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
     if ((st = capture_step(h)) != NAV_BA_OK) return st;
     h->prepared = true;
     return NAV_BA_OK;
