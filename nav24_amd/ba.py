@@ -26,7 +26,7 @@ EXPORTED = [
     "nav_ba_version", "nav_ba_create", "nav_ba_destroy", "nav_ba_last_error",
     "nav_ba_default_options", "nav_ba_solve", "nav_ba_prepare", "nav_ba_run", "nav_ba_fetch",
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
-    "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing",
+    "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
 ]
 
 STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
         L.nav_ba_set_timing.restype = C.c_int32
         L.nav_ba_get_timing.argtypes = [H, C.POINTER(NavBaTiming)]
         L.nav_ba_get_timing.restype = C.c_int32
+        L.nav_ba_debug_set_solver.argtypes = [H, C.c_int32]
+        L.nav_ba_debug_set_solver.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -190,6 +192,10 @@ class GraphOptimHip:
         t = NavBaTiming()
         self._check(lib().nav_ba_get_timing(self._h, C.byref(t)))
         return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(STAGES)}
+
+    def set_solver(self, mode: int) -> None:
+        """Factorisation for the next prepare: 0 auto, 1 64x64 tiles, 2 6x6 camera blocks."""
+        self._check(lib().nav_ba_debug_set_solver(self._h, int(mode)))
 
     def debug_reduced_system(self, lam: float):
         n = C.c_int32(0)

@@ -17,12 +17,16 @@ namespace navba {
 
 constexpr int kBlock = 256;        // threads per workgroup for the per-observation kernels
 constexpr int kTile = 256;         // observations per point-aligned linearisation tile
-constexpr int kChunk = 1024;       // camera-major observations per Schur / Hpp work item
+constexpr int kChunk = 1024;       // camera-major observations per Hpp work item
+constexpr int kSchurChunk = 256;   // camera-major observations per Schur work item
 constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget)
 constexpr int kNB = 64;            // tiled dense Cholesky tile
 constexpr int kMaxWinCams = 8192;  // cameras per window: Schur slot map in LDS is int16
 constexpr int kCamLdsMax = 512;    // Schur rows keep the window's camera table in LDS up to this size
 constexpr int kSmallMaxN = 180;    // per-window in-LDS Cholesky: at most 30 free cameras
+constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Cholesky is possible up to this size
+constexpr int kBlocksAutoFc = 16;  // ... and chosen automatically up to this size (one CU's latency chain
+                                   // grows with the block count; 64x64 tiles over many CUs win above)
 constexpr int kCostObs = 1024;     // observations per cost / flag block
 constexpr int kPtBlock = 256;      // points per back-substitution block
 
@@ -79,6 +83,7 @@ struct DevProblem {
     const long long* sitem_pofs;
     const int* row_sitem_ptr;  // [n_fc + 1]
     const int* row_nchunks;    // [n_fc]
+    const int* sitem_order;    // [n_sitems] dispatch order: estimated work, heaviest first
     int n_sitems;
     int max_ns;                // largest slot count of any Schur work item
     // column blocks of each S row: free indices >= row (same window), first is the row itself
@@ -104,8 +109,9 @@ struct DevProblem {
     const int* pblk_win;       // [n_pblk]
     const int* win_pblk0;      // [n_win + 1]
     int n_pblk;
-    // dense reduced system of the single-window tiled path
+    // dense reduced system of one problem: tiled (n_pad = multiple of 64) or 6x6 blocks
     int n, n_pad, nt;
+    int use_blocks;            // 1: single-workgroup block Cholesky
 };
 
 struct DevState {
@@ -126,7 +132,8 @@ struct DevWork {
     double* g;          // reduced right-hand sides (free-camera order, padded for the tiled path)
     double* y;
     double* dp;
-    double* Lm;         // tiled path: Cholesky factor
+    double* Lm;         // one problem: Cholesky factor (tiled: n_pad^2; blocks: packed 6x6 blocks)
+    double* Li;         // tiled path: inverses of the diagonal factor tiles [nt][64][64]
     double* red_cost;   // [n_cblk]
     double* red_scale;  // [n_win + n_pblk]: per-window camera part, then point blocks
     double* red_aux;    // [n_cblk] outlier counts
@@ -167,6 +174,7 @@ template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWor
 void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st);
 void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st);
 int prepare_cholesky_small(const DevProblem& P);  // LDS attribute of the per-window factorisation
+int prepare_cholesky_blocks(const DevProblem& P);  // LDS attribute of the one-problem block factorisation
 template <typename T> void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand,
                                           const DevWork& W, hipStream_t st);
 void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);

@@ -298,11 +298,16 @@ __global__ void k_fill_system(DevProblem P, const LMDev* lm, double* __restrict_
     }
 }
 
-// One workgroup per S row block f: sum the chunk partials (FP64, fixed order),
-// form S_ff = Hpp + lambda I - sum, S_gf = -sum^T (lower triangle), g = bp - coef.
+// Workgroups (f, part) for S row block f, each taking a quarter of the row's
+// entries: sum the chunk partials (FP64, fixed order), form
+// S_ff = Hpp + lambda I - sum, S_gf = -sum^T (lower triangle); part 0 also
+// forms g = bp - coef.
+constexpr int kReduceParts = 4;
+
 __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const double* __restrict__ spart,
                                                          const double* __restrict__ Hc, const LMDev* lm,
                                                          double* __restrict__ S, double* __restrict__ gv) {
+    __shared__ double cpart[4][6];
     const int f = blockIdx.x;
     const int win = P.cam_win[P.free_cam[f]];
     if (!running(lm, win)) return;
@@ -315,14 +320,24 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
     const int nch = P.row_nchunks[f];
     const int base = P.row_sitem_ptr[f];
     const double lambda = lm[win].lambda;
-    for (int idx = threadIdx.x; idx < ncols * 36; idx += kBlock) {
+    const int nent = ncols * 36;
+    const int per = (nent + kReduceParts - 1) / kReduceParts;
+    const int e0 = blockIdx.y * per, e1 = min(nent, e0 + per);
+    for (int idx = e0 + threadIdx.x; idx < e1; idx += kBlock) {
         const int s = idx / 36, e = idx % 36;
         const int gr = s / kMaxSlots, sl = s % kMaxSlots;
-        double sum = 0.0;
-        for (int ch = 0; ch < nch; ++ch) {
-            const int it = base + gr * nch + ch;
-            sum += spart[P.sitem_pofs[it] + 36 * sl + e];
+        // chunks summed in a fixed order by four interleaved partial sums (loads in flight)
+        const int it0 = base + gr * nch;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int ch = 0;
+        for (; ch + 4 <= nch; ch += 4) {
+            s0 += spart[P.sitem_pofs[it0 + ch] + 36 * sl + e];
+            s1 += spart[P.sitem_pofs[it0 + ch + 1] + 36 * sl + e];
+            s2 += spart[P.sitem_pofs[it0 + ch + 2] + 36 * sl + e];
+            s3 += spart[P.sitem_pofs[it0 + ch + 3] + 36 * sl + e];
         }
+        for (; ch < nch; ++ch) s0 += spart[P.sitem_pofs[it0 + ch] + 36 * sl + e];
+        const double sum = (s0 + s1) + (s2 + s3);
         const int i = e / 6, c = e % 6;
         if (s == 0) {
             Sw[(size_t)(6 * lf + i) * lda + 6 * lf + c] = Hc[28 * (size_t)f + sym6(i, c)] + (i == c ? lambda : 0.0) - sum;
@@ -331,13 +346,23 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
             Sw[(size_t)(6 * lg + c) * lda + 6 * lf + i] = -sum;
         }
     }
-    if (threadIdx.x < 6) {
-        double coef = 0.0;
-        for (int ch = 0; ch < nch; ++ch) {
-            const int it = base + ch;
-            coef += spart[P.sitem_pofs[it] + 36 * P.sitem_nslots[it] + threadIdx.x];
+    if (blockIdx.y == 0) {
+        // coefficient of group 0's chunks: 24 threads, entry t % 6, chunks = t / 6 (mod 4)
+        if (threadIdx.x < 24) {
+            const int e = threadIdx.x % 6, q = threadIdx.x / 6;
+            double c = 0.0;
+            for (int ch = q; ch < nch; ch += 4) {
+                const int it = base + ch;
+                c += spart[P.sitem_pofs[it] + 36 * P.sitem_nslots[it] + e];
+            }
+            cpart[q][e] = c;
         }
-        gv[6 * f + threadIdx.x] = Hc[28 * (size_t)f + 21 + threadIdx.x] - coef;
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const double coef = (cpart[0][threadIdx.x] + cpart[1][threadIdx.x]) +
+                                (cpart[2][threadIdx.x] + cpart[3][threadIdx.x]);
+            gv[6 * f + threadIdx.x] = Hc[28 * (size_t)f + 21 + threadIdx.x] - coef;
+        }
     }
 }
 
@@ -347,8 +372,8 @@ void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) 
     const int chunks = P.batched ? 4 : (int)(tiles < 65535 ? (tiles > 0 ? tiles : 1) : 65535);
     hipLaunchKernelGGL(k_fill_system, dim3(chunks, P.n_win), dim3(256), 0, st, P, (const LMDev*)W.lm, W.S, W.g);
     if (P.n_fc == 0) return;
-    hipLaunchKernelGGL(k_schur_reduce, dim3(P.n_fc), dim3(kBlock), 0, st, P, (const double*)W.spart, W.Hc,
-                       (const LMDev*)W.lm, W.S, W.g);
+    hipLaunchKernelGGL(k_schur_reduce, dim3(P.n_fc, kReduceParts), dim3(kBlock), 0, st, P, (const double*)W.spart,
+                       W.Hc, (const LMDev*)W.lm, W.S, W.g);
 }
 
 // ---------------------------------------------------------------- back-substitution

@@ -105,6 +105,7 @@ struct nav_ba_handle {
     hipGraphExec_t step = nullptr;
     bool timing = false;
     bool timing_active = false;
+    int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks
     StageTimer timer;
     LMDev* hstatus = nullptr;  // pinned: LM state of window 0
     int* hdone = nullptr;      // pinned: windows finished
@@ -500,32 +501,41 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
         }
     }
 
-    // ---- Schur work items: per row, column groups of kMaxSlots, chunks of kChunk
+    // ---- Schur work items: per row, column groups of kMaxSlots, chunks of kSchurChunk;
+    // dispatched heaviest first (estimated pairs + observations)
     std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr{0}, row_nch;
-    std::vector<long long> spofs;
+    std::vector<long long> spofs, scost;
     long long pofs = 0;
     int max_ns = 1;
     for (int f = 0; f < nfc; ++f) {
         const int ncols = colptr[f + 1] - colptr[f];
         const int ngroups = (ncols + kMaxSlots - 1) / kMaxSlots;
         const int nobs = cam_ptr[f + 1] - cam_ptr[f];
-        const int nch = std::max(1, (nobs + kChunk - 1) / kChunk);
+        const int nch = std::max(1, (nobs + kSchurChunk - 1) / kSchurChunk);
         row_nch.push_back(nch);
         for (int gi = 0; gi < ngroups; ++gi) {
             const int ns = std::min(kMaxSlots, ncols - gi * kMaxSlots);
             max_ns = std::max(max_ns, ns);
             for (int ch = 0; ch < nch; ++ch) {
+                const int a0 = cam_ptr[f] + ch * kSchurChunk;
+                const int a1 = std::min(cam_ptr[f] + (ch + 1) * kSchurChunk, cam_ptr[f + 1]);
+                long long cost = 0;
+                for (int k = a0; k < a1; ++k) cost += 4 + (pt_ptr[o_pt[cam_obs[k]] + 1] - cam_obs_b0[k]);
                 sf.push_back(f);
-                sa0.push_back(cam_ptr[f] + ch * kChunk);
-                sa1.push_back(std::min(cam_ptr[f] + (ch + 1) * kChunk, cam_ptr[f + 1]));
+                sa0.push_back(a0);
+                sa1.push_back(a1);
                 sg.push_back(gi);
                 sns.push_back(ns);
                 spofs.push_back(pofs);
+                scost.push_back(cost);
                 pofs += 36LL * ns + 6;
             }
         }
         row_sitem_ptr.push_back((int)sf.size());
     }
+    std::vector<int> sorder(sf.size());
+    for (size_t i = 0; i < sorder.size(); ++i) sorder[i] = (int)i;
+    std::stable_sort(sorder.begin(), sorder.end(), [&](int x, int y) { return scost[x] > scost[y]; });
 
     // ---- dense reduced systems: one padded tiled system, or one small system per window
     std::vector<long long> win_S(n_win, 0);
@@ -588,6 +598,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
     UP(P.sitem_f, sf); UP(P.sitem_a0, sa0); UP(P.sitem_a1, sa1); UP(P.sitem_g, sg); UP(P.sitem_nslots, sns);
     UP(P.sitem_pofs, spofs); UP(P.row_sitem_ptr, row_sitem_ptr); UP(P.row_nchunks, row_nch);
+    UP(P.sitem_order, sorder);
     UP(P.colptr, colptr); UP(P.colcam, colcam);
     UP(P.cam_win, cam_win); UP(P.pt_win, pt_win_all); UP(P.win_cam0, win_cam0); UP(P.win_fc0, win_fc0);
     UP(P.win_pt0, win_pt0); UP(P.win_S, win_S); UP(P.win_lda, win_lda);
@@ -608,6 +619,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     P.n = n;
     P.n_pad = n_pad;
     P.nt = n_pad / kNB;
+    P.use_blocks = (!batched && nfc > 0 && nfc <= kBlocksMaxFc &&
+                    (h->solver_mode == 2 || (h->solver_mode == 0 && nfc <= kBlocksAutoFc))) ? 1 : 0;
 
     const size_t tsz = mixed(h) ? sizeof(float) : sizeof(double);
     const size_t nvec = batched ? std::max<size_t>(6 * (size_t)nfc, 1) : std::max<size_t>(n_pad, 1);
@@ -628,6 +641,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.y, sizeof(double) * nvec);
     AL(W.dp, sizeof(double) * nvec);
     AL(W.Lm, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(S_size, 1)));
+    AL(W.Li, sizeof(double) * 4096 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1)));
     AL(W.red_cost, sizeof(double) * std::max(P.n_cblk, 1));
     AL(W.red_scale, sizeof(double) * (n_win + std::max(P.n_pblk, 1)));
     AL(W.red_aux, sizeof(double) * std::max(P.n_cblk, 1));
@@ -643,6 +657,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     // Schur rows and the per-window factorisation may use more than 64 KiB of LDS
     if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
         return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
+    if (prepare_cholesky_blocks(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the block factorisation");
     if (batched && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
     if ((st = capture_step(h)) != NAV_BA_OK) return st;
     h->prepared = true;
@@ -882,6 +897,12 @@ int32_t nav_ba_set_timing(nav_ba_handle* h, int32_t enable) {
 int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out) {
     if (!h || !out) return NAV_BA_BAD_ARG;
     for (int i = 0; i < 16; ++i) { out->ms[i] = h->timer.ms[i]; out->launches[i] = h->timer.launches[i]; }
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode) {
+    if (!h || mode < 0 || mode > 2) return NAV_BA_BAD_ARG;
+    h->solver_mode = mode;
     return NAV_BA_OK;
 }
 

@@ -9,11 +9,15 @@
 //   phase A  one lane per observation a of camera f on a free point p:
 //            W_a = Hpl_a Dinv_p (FP64), the point's coordinates and the range
 //            of its observations on cameras >= f go to a per-wave LDS buffer;
-//   phase B  the wave's (point, later observation b) pairs are packed densely
-//            over the 64 lanes (prefix sum + binary search), so the lanes of
-//            one point hit distinct S-row slots; each lane forms
+//            the diagonal pair (a, a) adds W_a Hpl_a^T (symmetric, 21 values)
+//            to registers of its own lane: every point has one, and in LDS
+//            they would all hit slot 0 at once;
+//   phase B  the wave's other (point, later observation b) pairs are packed
+//            densely over the 64 lanes (prefix sum + binary search), so the
+//            lanes of one point hit distinct S-row slots; each lane forms
 //            W_a Hpl_b^T = (W_a J_X,b^T)(w_b J_xi,b) from recomputed Jacobians
 //            and adds it into the LDS slot of camera(b) (slot 0 = camera f).
+// Work items are dispatched heaviest first (a host-side permutation).
 // T is the type of the residuals and Jacobians only.  W, the pair products and
 // the S-row slots are FP64 for both compute types: the point elimination
 // cancels Hpp against Hpl Dinv Hpl^T, and with a few-view point's Dinv
@@ -39,13 +43,14 @@ constexpr int kCamStride = 13;  // odd: consecutive cameras fall in different LD
 template <typename T>
 struct SchurLayout {
     static constexpr int kWbuf = 21;  // W (18) + point (3), FP64
-    size_t acc, cam, wbuf, lo, pre, slot, total;
+    size_t acc, cam, wbuf, lo, aw, pre, slot, total;
     __host__ __device__ SchurLayout(int ns, int n_cam, bool cam_lds) {  // n_cam: cameras of the largest window
         size_t o = 0;
         acc = o;  o += sizeof(double) * 36 * (size_t)ns;
         cam = o;  o += cam_lds ? sizeof(T) * kCamStride * (size_t)n_cam : 0;
         wbuf = o; o += sizeof(double) * kWbuf * kBlock;
         lo = o;   o += sizeof(int) * kBlock;
+        aw = o;   o += sizeof(int) * kBlock;
         pre = o;  o += sizeof(int) * (kBlock / kWave) * (kWave + 1);
         slot = o; o += sizeof(short) * (size_t)n_cam;
         total = (o + 15) & ~(size_t)15;
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                                                       const double* __restrict__ ptsys, const LMDev* lm,
                                                       double* __restrict__ spart) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int it = blockIdx.x;
+    const int it = P.sitem_order[blockIdx.x];
     const int f = P.sitem_f[it], a0 = P.sitem_a0[it], a1 = P.sitem_a1[it];
     const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
     const int cf = P.free_cam[f];
@@ -86,6 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     T* camt = reinterpret_cast<T*>(smem + L.cam);
     double* wbuf_all = reinterpret_cast<double*>(smem + L.wbuf);
     int* lo_all = reinterpret_cast<int*>(smem + L.lo);
+    int* aw_all = reinterpret_cast<int*>(smem + L.aw);
     int* pre_all = reinterpret_cast<int*>(smem + L.pre);
     short* slot = reinterpret_cast<short*>(smem + L.slot);
 
@@ -105,11 +111,14 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     cam_get<T, kCamLds>(camt, cam, cf - cam0, cam0, Rf, tf);
     double* wbuf = wbuf_all + (size_t)wid * kWave * SchurLayout<T>::kWbuf;
     int* lo_w = lo_all + wid * kWave;
+    int* aw_w = aw_all + wid * kWave;
     int* pre = pre_all + wid * (kWave + 1);
 
-    double coef[6];
+    double coef[6], dsum[21];
 #pragma unroll
     for (int k = 0; k < 6; ++k) coef[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 21; ++k) dsum[k] = 0.0;
 
     for (int kb = a0 + wid * kWave; kb < a1; kb += kBlock) {
         wave_fence();
@@ -138,22 +147,32 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 const double* ps = ptsys + 12 * (size_t)p;
                 const double d00 = ps[0], d01 = ps[1], d02 = ps[2], d11 = ps[3], d12 = ps[4], d22 = ps[5];
                 double* wb = wbuf + lane * SchurLayout<T>::kWbuf;
+                double W[18];
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
                     const double h0 = Ha[3 * i], h1 = Ha[3 * i + 1], h2 = Ha[3 * i + 2];
-                    wb[3 * i + 0] = h0 * d00 + h1 * d01 + h2 * d02;
-                    wb[3 * i + 1] = h0 * d01 + h1 * d11 + h2 * d12;
-                    wb[3 * i + 2] = h0 * d02 + h1 * d12 + h2 * d22;
+                    W[3 * i + 0] = h0 * d00 + h1 * d01 + h2 * d02;
+                    W[3 * i + 1] = h0 * d01 + h1 * d11 + h2 * d12;
+                    W[3 * i + 2] = h0 * d02 + h1 * d12 + h2 * d22;
                 }
+#pragma unroll
+                for (int e = 0; e < 18; ++e) wb[e] = W[e];
                 wb[18] = xp.x; wb[19] = xp.y; wb[20] = xp.z;
                 if (g0) {
                     const double e0 = ps[6], e1 = ps[7], e2 = ps[8];
 #pragma unroll
                     for (int i = 0; i < 6; ++i) coef[i] += Ha[3 * i] * e0 + Ha[3 * i + 1] * e1 + Ha[3 * i + 2] * e2;
+                    int q = 0;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i)
+#pragma unroll
+                        for (int c = i; c < 6; ++c, ++q)
+                            dsum[q] += W[3 * i] * Ha[3 * c] + W[3 * i + 1] * Ha[3 * c + 1] + W[3 * i + 2] * Ha[3 * c + 2];
                 }
                 const int lo = P.cam_obs_b0[k];
                 lo_w[lane] = lo;
-                cnt = P.pt_ptr[p + 1] - lo;
+                aw_w[lane] = a - lo;
+                cnt = P.pt_ptr[p + 1] - lo - 1;  // all later observations but a itself
             }
         }
         // wave-exclusive prefix sum of the pair counts
@@ -176,7 +195,8 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 if (pre[mid] <= t) lo_i = mid; else hi_i = mid - 1;
             }
             const int i = lo_i;
-            const int b = lo_w[i] + (t - pre[i]);
+            const int tt = t - pre[i];
+            const int b = lo_w[i] + tt + (tt >= aw_w[i] ? 1 : 0);  // skips a itself
             const int cb = P.o_cam[b] - cam0;
             const int s = slot[cb];  // camera f itself is slot 0 of group 0
             if (s < 0) continue;
@@ -206,7 +226,8 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                     atomicAdd(acc + (6 * r + c) * ns + s, M0[r] * B0[c] + M1[r] * B1[c]);
         }
     }
-    // ---- coefficient: fixed-order block reduction; S-row slots out (slot-major)
+    // ---- diagonal block (slot 0 of group 0): wave sums, one LDS add per wave;
+    // coefficient: fixed-order block reduction; S-row slots out (slot-major)
     __shared__ double red[kBlock / kWave][6];
     if (g0) {
 #pragma unroll
@@ -214,6 +235,17 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
             const double v = wave_sum(coef[k]);
             if (lane == 0) red[wid][k] = v;
         }
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int c = i; c < 6; ++c, ++q) {
+                const double v = wave_sum(dsum[q]);
+                if (lane == 0) {
+                    atomicAdd(acc + (6 * i + c) * ns, v);
+                    if (c != i) atomicAdd(acc + (6 * c + i) * ns, v);
+                }
+            }
     }
     __syncthreads();
     double* out = spart + P.sitem_pofs[it];

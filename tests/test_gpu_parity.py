@@ -34,6 +34,22 @@ def test_fp64_solve_matches_oracle(solver, oracle_mod, workload, seed, over):
     check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fp64_both_factorisations_match_oracle(solver, oracle_mod, mode):
+    _say(f"fp64 solve workload=1 seed=2 factorisation mode {mode} (1 tiles, 2 blocks)")
+    prob = problems.workload(1, seed=2, n_pt=6000)
+    opts = workload_options(1)
+    opts.precision = NAV_BA_FP64
+    opts.n_iter = 8
+    ref = oracle_mod.solve(prob, opts)
+    solver.set_solver(mode)
+    try:
+        got = solver.solve(prob, opts)
+    finally:
+        solver.set_solver(0)
+    check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
+
+
 def test_mixed_solve_close_to_oracle(solver, oracle_mod):
     _say("mixed solve workload0")
     prob = problems.workload(0, seed=1)
