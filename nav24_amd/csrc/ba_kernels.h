@@ -28,7 +28,6 @@ constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Ch
 constexpr int kBlocksAutoFc = 16;  // ... and chosen automatically up to this size (one CU's latency chain
                                    // grows with the block count; 64x64 tiles over many CUs win above)
 constexpr int kCostObs = 1024;     // observations per cost / flag block
-constexpr int kPtBlock = 256;      // points per back-substitution block
 
 // LM phase of one window
 enum { kPhaseLin = 0, kPhaseTrial = 1, kPhaseDone = 2 };
@@ -63,6 +62,8 @@ struct DevProblem {
     const int* cam_ptr;      // [n_fc + 1]
     const int* cam_obs;
     const int* cam_obs_b0;   // per cam_obs entry: first observation of its point on a camera >= this one
+    const int* c_pt;         // per cam_obs entry: its point (camera-major copy of o_pt)
+    const double* c_uv;      // per cam_obs entry: its measurement, 2 values (camera-major copy of o_uv)
     // linearisation tiles (whole points of one window, <= kTile observations or one point)
     const int* tile_obs;     // [n_tiles + 1]
     const int* tile_pt;      // [n_tiles + 1]
@@ -105,10 +106,7 @@ struct DevProblem {
     const int* cblk_win;       // [n_cblk]
     const int* win_cblk0;      // [n_win + 1]
     int n_cblk;
-    const int* pblk_p0;        // [n_pblk + 1] point blocks, window-aligned
-    const int* pblk_win;       // [n_pblk]
-    const int* win_pblk0;      // [n_win + 1]
-    int n_pblk;
+    const int* win_tile0;      // [n_win + 1] first linearisation tile of each window
     // dense reduced system of one problem: tiled (n_pad = multiple of 64) or 6x6 blocks
     int n, n_pad, nt;
     int use_blocks;            // 1: single-workgroup block Cholesky
@@ -135,7 +133,7 @@ struct DevWork {
     double* Lm;         // one problem: Cholesky factor (tiled: n_pad^2; blocks: packed 6x6 blocks)
     double* Li;         // tiled path: inverses of the diagonal factor tiles [nt][64][64]
     double* red_cost;   // [n_cblk]
-    double* red_scale;  // [n_win + n_pblk]: per-window camera part, then point blocks
+    double* red_scale;  // [n_win + n_tiles]: per-window camera part, then per-tile point parts
     double* red_aux;    // [n_cblk] outlier counts
     uint8_t* outlier;   // [n_obs] caller order
     double* hist;       // [n_win * (n_iter + 1)] chi2 before / after each iteration

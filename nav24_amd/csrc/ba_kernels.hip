@@ -141,7 +141,8 @@ void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hip
 // ---------------------------------------------------------------- lin_cam
 // One workgroup per (free camera, chunk of its observations): Hpp (21 packed)
 // and bp (6) from T Jacobians, formed and block-reduced in FP64; then one
-// workgroup per camera sums the chunks.
+// workgroup per camera sums the chunks.  Inputs are read in camera-major
+// order (c_pt, c_uv), so the per-observation reads are contiguous.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* __restrict__ cam,
                                                     const D4* __restrict__ X, const T* __restrict__ w,
@@ -157,13 +158,17 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
     double acc[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) acc[k] = 0.0;
+    const T delta = (T)P.delta;
     for (int k = a0 + (int)threadIdx.x; k < a1; k += kBlock) {
-        const int a = P.cam_obs[k];
-        const D4 xp = X[P.o_pt[a]];
+        // camera-major copies of the point index and measurement: contiguous reads;
+        // the Huber weight is recomputed from the residual (the same arithmetic as k_lin_pt)
+        const D4 xp = X[P.c_pt[k]];
         EdgeLin<T> L;
-        edge_linearize(R, tt, (T)xp.x, (T)xp.y, (T)xp.z, fx, fy, cx, cy,
-                       (T)P.o_uv[2 * (size_t)a], (T)P.o_uv[2 * (size_t)a + 1], L);
-        const double wa = (double)w[a];
+        const T s = edge_linearize(R, tt, (T)xp.x, (T)xp.y, (T)xp.z, fx, fy, cx, cy,
+                                   (T)P.c_uv[2 * (size_t)k], (T)P.c_uv[2 * (size_t)k + 1], L);
+        T wt;
+        robustify(s, P.robust, delta, wt);
+        const double wa = (double)wt;
         const double r0 = -(double)L.e0 * wa, r1 = -(double)L.e1 * wa;
         double J[12];
 #pragma unroll
@@ -377,65 +382,128 @@ void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) 
 }
 
 // ---------------------------------------------------------------- back-substitution
-// Per point: cl = bl - sum_b Hpl_b^T dxi_b, dX = Dinv cl, X' = X + dX and the
-// point part of x'(lambda x + b); point blocks never span two windows.
+// One workgroup per linearisation tile (whole points of one window).  Per
+// observation b on a free camera: c_b = J_X,b^T w_b J_xi,b dxi_b (one lane per
+// observation); per point, in fixed order from LDS: cl = bl - sum_b c_b,
+// dX = Dinv cl, X' = X + dX, and the point part of x'(lambda x + b), summed
+// per tile.  A point with more than kTile observations has a tile of its own
+// and is summed by a block reduction.
 template <typename T>
-__global__ __launch_bounds__(kPtBlock) void k_backsub(DevProblem P, const CamRT* __restrict__ cam,
-                                                      const D4* __restrict__ X, const T* __restrict__ w,
-                                                      const double* __restrict__ Hl, const double* __restrict__ ptsys,
-                                                      const double* __restrict__ dp, const LMDev* lm,
-                                                      D4* __restrict__ Xn, double* __restrict__ red_scale) {
-    __shared__ double sh[kPtBlock / kWave];
-    const int b = blockIdx.x;
-    const int win = P.pblk_win[b];
-    if (!running(lm, win)) return;
-    const int p = P.pblk_p0[b] + threadIdx.x;
-    double sc = 0.0;
-    if (p < P.pblk_p0[b + 1]) {
-        const D4 xp = X[p];
-        D4 o = xp;
-        if (P.pt_free[p] && !lm[win].fail) {
-            const double lambda = lm[win].lambda;
-            const double* h = Hl + 9 * (size_t)p;
-            const double bl0 = h[6], bl1 = h[7], bl2 = h[8];
-            double c0 = bl0, c1 = bl1, c2 = bl2;
-            const T x = (T)xp.x, y = (T)xp.y, z = (T)xp.z;
-            const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy;
-            for (int ob = P.pt_ptr[p]; ob < P.pt_ptr[p + 1]; ++ob) {
-                const int cb = P.o_cam[ob];
-                const int fb = P.cam_free[cb];
-                if (fb < 0) continue;
-                T R[9], tt[3];
-                load_cam(cam, cb, R, tt);
-                EdgeLin<T> L;
-                edge_linearize(R, tt, x, y, z, fx, fy, cx, cy, T(0), T(0), L);
-                const double wb = (double)w[ob];
-                const double* d = dp + 6 * (size_t)fb;
-                double t0 = 0.0, t1 = 0.0;
+__device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __restrict__ cam, const D4& xp,
+                                            int ob, const T* __restrict__ w, const double* __restrict__ dp,
+                                            double c[3]) {
+    c[0] = c[1] = c[2] = 0.0;
+    const int cb = P.o_cam[ob];
+    const int fb = P.cam_free[cb];
+    if (fb < 0) return;
+    T R[9], tt[3];
+    load_cam(cam, cb, R, tt);
+    EdgeLin<T> L;
+    edge_linearize(R, tt, (T)xp.x, (T)xp.y, (T)xp.z, (T)P.fx, (T)P.fy, (T)P.cx, (T)P.cy, T(0), T(0), L);
+    const double wb = (double)w[ob];
+    const double* d = dp + 6 * (size_t)fb;
+    double t0 = 0.0, t1 = 0.0;
 #pragma unroll
-                for (int i = 0; i < 6; ++i) { t0 += (double)L.Jxi[i] * d[i]; t1 += (double)L.Jxi[6 + i] * d[i]; }
-                t0 *= wb; t1 *= wb;
-                c0 -= (double)L.JX[0] * t0 + (double)L.JX[3] * t1;
-                c1 -= (double)L.JX[1] * t0 + (double)L.JX[4] * t1;
-                c2 -= (double)L.JX[2] * t0 + (double)L.JX[5] * t1;
-            }
-            const double* D = ptsys + 12 * (size_t)p;
-            const double d0 = D[0] * c0 + D[1] * c1 + D[2] * c2;
-            const double d1 = D[1] * c0 + D[3] * c1 + D[4] * c2;
-            const double d2 = D[2] * c0 + D[4] * c1 + D[5] * c2;
-            o.x += d0; o.y += d1; o.z += d2;
-            sc = d0 * (lambda * d0 + bl0) + d1 * (lambda * d1 + bl1) + d2 * (lambda * d2 + bl2);
+    for (int i = 0; i < 6; ++i) { t0 += (double)L.Jxi[i] * d[i]; t1 += (double)L.Jxi[6 + i] * d[i]; }
+    t0 *= wb; t1 *= wb;
+    c[0] = (double)L.JX[0] * t0 + (double)L.JX[3] * t1;
+    c[1] = (double)L.JX[1] * t0 + (double)L.JX[4] * t1;
+    c[2] = (double)L.JX[2] * t0 + (double)L.JX[5] * t1;
+}
+
+__device__ __forceinline__ double backsub_point(const DevProblem& P, int p, double lambda, const double* __restrict__ Hl,
+                                                const double* __restrict__ ptsys, const double s[3],
+                                                const D4& xp, D4* __restrict__ Xn) {
+    const double* h = Hl + 9 * (size_t)p;
+    const double bl0 = h[6], bl1 = h[7], bl2 = h[8];
+    const double c0 = bl0 - s[0], c1 = bl1 - s[1], c2 = bl2 - s[2];
+    const double* D = ptsys + 12 * (size_t)p;
+    const double d0 = D[0] * c0 + D[1] * c1 + D[2] * c2;
+    const double d1 = D[1] * c0 + D[3] * c1 + D[4] * c2;
+    const double d2 = D[2] * c0 + D[4] * c1 + D[5] * c2;
+    Xn[p] = D4{xp.x + d0, xp.y + d1, xp.z + d2, xp.w};
+    return d0 * (lambda * d0 + bl0) + d1 * (lambda * d1 + bl1) + d2 * (lambda * d2 + bl2);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* __restrict__ cam,
+                                                    const D4* __restrict__ X, const T* __restrict__ w,
+                                                    const double* __restrict__ Hl, const double* __restrict__ ptsys,
+                                                    const double* __restrict__ dp, const LMDev* lm,
+                                                    D4* __restrict__ Xn, double* __restrict__ red_scale) {
+    __shared__ double sh[3][kTile];
+    __shared__ double red[kBlock / kWave][3];
+    const int t = blockIdx.x;
+    const int win = P.tile_win[t];
+    if (!running(lm, win)) return;
+    const int o0 = P.tile_obs[t], o1 = P.tile_obs[t + 1];
+    const int p0 = P.tile_pt[t], p1 = P.tile_pt[t + 1];
+    const bool fail = lm[win].fail != 0;
+    const double lambda = lm[win].lambda;
+    double sc = 0.0;
+    if (o1 - o0 <= kTile) {
+        const int a = o0 + (int)threadIdx.x;
+        double c[3] = {0.0, 0.0, 0.0};
+        if (a < o1 && !fail) {
+            const int p = P.o_pt[a];
+            if (P.pt_free[p]) backsub_obs<T>(P, cam, X[p], a, w, dp, c);
         }
-        Xn[p] = o;
+        sh[0][threadIdx.x] = c[0]; sh[1][threadIdx.x] = c[1]; sh[2][threadIdx.x] = c[2];
+        __syncthreads();
+        for (int p = p0 + (int)threadIdx.x; p < p1; p += kBlock) {
+            const D4 xp = X[p];
+            if (P.pt_free[p] && !fail) {
+                double s[3] = {0.0, 0.0, 0.0};
+                for (int b = P.pt_ptr[p]; b < P.pt_ptr[p + 1]; ++b) {
+                    s[0] += sh[0][b - o0]; s[1] += sh[1][b - o0]; s[2] += sh[2][b - o0];
+                }
+                sc += backsub_point(P, p, lambda, Hl, ptsys, s, xp, Xn);
+            } else {
+                Xn[p] = xp;
+            }
+        }
+    } else {
+        // one point (p0) with more than kTile observations
+        const D4 xp = X[p0];
+        const bool active = P.pt_free[p0] && !fail;
+        double c[3] = {0.0, 0.0, 0.0};
+        if (active)
+            for (int a = o0 + (int)threadIdx.x; a < o1; a += kBlock) {
+                double ca[3];
+                backsub_obs<T>(P, cam, xp, a, w, dp, ca);
+                c[0] += ca[0]; c[1] += ca[1]; c[2] += ca[2];
+            }
+        const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double v = wave_sum(c[k]);
+            if (lane == 0) red[wid][k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (active) {
+                double s[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    s[k] = 0.0;
+                    for (int i = 0; i < kBlock / kWave; ++i) s[k] += red[i][k];
+                }
+                sc = backsub_point(P, p0, lambda, Hl, ptsys, s, xp, Xn);
+            } else {
+                Xn[p0] = xp;
+            }
+        }
+        __syncthreads();
     }
-    const double s = block_sum<double, kPtBlock>(sc, sh);
-    if (threadIdx.x == 0) red_scale[P.n_win + b] = s;
+    __shared__ double shs[kBlock / kWave];
+    const double s = block_sum<double, kBlock>(sc, shs);
+    if (threadIdx.x == 0) red_scale[P.n_win + t] = s;
 }
 
 template <typename T>
 void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st) {
-    if (P.n_pblk == 0) return;
-    hipLaunchKernelGGL(k_backsub<T>, dim3(P.n_pblk), dim3(kPtBlock), 0, st, P, cur.cam, cur.X, (const T*)W.w,
+    if (P.n_tiles == 0) return;
+    hipLaunchKernelGGL(k_backsub<T>, dim3(P.n_tiles), dim3(kBlock), 0, st, P, cur.cam, cur.X, (const T*)W.w,
                        (const double*)W.Hl, W.ptsys, W.dp, (const LMDev*)W.lm, cand.X, W.red_scale);
 }
 
@@ -539,7 +607,7 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
     const double cost = sum_range(red_cost, P.win_cblk0[win], P.win_cblk0[win + 1], sh);
     __syncthreads();
     const double scale = red_scale[win] +
-                         sum_range(red_scale, P.n_win + P.win_pblk0[win], P.n_win + P.win_pblk0[win + 1], sh);
+                         sum_range(red_scale, P.n_win + P.win_tile0[win], P.n_win + P.win_tile0[win + 1], sh);
     if (threadIdx.x == 0) {
         LMDev L = lm[win];
         const double temp = L.fail ? DBL_MAX : cost;

@@ -420,6 +420,13 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
             if (f >= 0) cam_obs[pos[f]++] = k;
         }
     }
+    std::vector<int> c_pt(cam_obs.size());
+    std::vector<double> c_uv(2 * cam_obs.size());
+    for (size_t k = 0; k < cam_obs.size(); ++k) {
+        c_pt[k] = o_pt[cam_obs[k]];
+        c_uv[2 * k] = o_uv[2 * (size_t)cam_obs[k]];
+        c_uv[2 * k + 1] = o_uv[2 * (size_t)cam_obs[k] + 1];
+    }
     // first observation of the same point on the same camera (start of the
     // point's observations on cameras >= this one; duplicates are adjacent)
     cam_obs_b0.resize(cam_obs.size());
@@ -431,7 +438,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     }
 
     // ---- point-aligned linearisation tiles, never spanning two windows
-    std::vector<int> tile_obs{0}, tile_pt{0}, tile_win;
+    std::vector<int> tile_obs{0}, tile_pt{0}, tile_win, win_tile0{0};
     for (int w = 0; w < n_win; ++w) {
         for (int q = pt_first[w]; q < pt_first[w + 1];) {
             int q1 = q;
@@ -442,20 +449,16 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
             tile_win.push_back(w);
             q = q1;
         }
+        win_tile0.push_back((int)tile_win.size());
     }
     // ---- window-aligned cost blocks (observations) and point blocks; windows
     // are contiguous, so each block starts where the previous one ends
-    std::vector<int> cblk_o0{0}, cblk_win, win_cblk0{0}, pblk_p0{0}, pblk_win, win_pblk0{0};
+    std::vector<int> cblk_o0{0}, cblk_win, win_cblk0{0};
     std::vector<int> win_pt0(n_win + 1);
     for (int w = 0; w < n_win; ++w) {
         const int o0 = pt_ptr[pt_first[w]], o1 = pt_ptr[pt_first[w + 1]];
         for (int a = o0; a < o1; a += kCostObs) { cblk_o0.push_back(std::min(a + kCostObs, o1)); cblk_win.push_back(w); }
         win_cblk0.push_back((int)cblk_win.size());
-        for (int q = pt_first[w]; q < pt_first[w + 1]; q += kPtBlock) {
-            pblk_p0.push_back(std::min(q + kPtBlock, pt_first[w + 1]));
-            pblk_win.push_back(w);
-        }
-        win_pblk0.push_back((int)pblk_win.size());
         win_pt0[w] = pt_first[w];
     }
     win_pt0[n_win] = np;
@@ -594,6 +597,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.o_cam, o_cam); UP(P.o_pt, o_pt); UP(P.o_uv, o_uv); UP(P.o_orig, o_orig);
     UP(P.pt_ptr, pt_ptr); UP(P.pt_free, pt_free); UP(P.cam_free, cam_free); UP(P.free_cam, free_cam);
     UP(P.cam_ptr, cam_ptr); UP(P.cam_obs, cam_obs); UP(P.cam_obs_b0, cam_obs_b0);
+    UP(P.c_pt, c_pt); UP(P.c_uv, c_uv);
     UP(P.tile_obs, tile_obs); UP(P.tile_pt, tile_pt); UP(P.tile_win, tile_win);
     UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
     UP(P.sitem_f, sf); UP(P.sitem_a0, sa0); UP(P.sitem_a1, sa1); UP(P.sitem_g, sg); UP(P.sitem_nslots, sns);
@@ -603,7 +607,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.cam_win, cam_win); UP(P.pt_win, pt_win_all); UP(P.win_cam0, win_cam0); UP(P.win_fc0, win_fc0);
     UP(P.win_pt0, win_pt0); UP(P.win_S, win_S); UP(P.win_lda, win_lda);
     UP(P.cblk_o0, cblk_o0); UP(P.cblk_win, cblk_win); UP(P.win_cblk0, win_cblk0);
-    UP(P.pblk_p0, pblk_p0); UP(P.pblk_win, pblk_win); UP(P.win_pblk0, win_pblk0);
+    UP(P.win_tile0, win_tile0);
 #undef UP
     P.n_tiles = (int)tile_win.size();
     P.n_hitems = (int)hf.size();
@@ -615,7 +619,6 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     for (int w = 0; w < n_win; ++w) P.max_win_fc = std::max(P.max_win_fc, win_fc0[w + 1] - win_fc0[w]);
     P.batched = batched ? 1 : 0;
     P.n_cblk = (int)cblk_win.size();
-    P.n_pblk = (int)pblk_win.size();
     P.n = n;
     P.n_pad = n_pad;
     P.nt = n_pad / kNB;
@@ -643,7 +646,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.Lm, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(S_size, 1)));
     AL(W.Li, sizeof(double) * 4096 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1)));
     AL(W.red_cost, sizeof(double) * std::max(P.n_cblk, 1));
-    AL(W.red_scale, sizeof(double) * (n_win + std::max(P.n_pblk, 1)));
+    AL(W.red_scale, sizeof(double) * (n_win + std::max(P.n_tiles, 1)));
     AL(W.red_aux, sizeof(double) * std::max(P.n_cblk, 1));
     AL(W.outlier, std::max<size_t>(no, 1));
     AL(W.hist, sizeof(double) * (size_t)n_win * (o->n_iter + 1));

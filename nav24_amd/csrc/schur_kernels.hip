@@ -127,13 +127,16 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
         int cnt = 0;
         if (k < a1) {
             const int a = P.cam_obs[k];
-            const int p = P.o_pt[a];
+            const int p = P.c_pt[k];
             if (P.pt_free[p]) {
                 const D4 xp = X[p];
                 const T x = (T)xp.x, y = (T)xp.y, z = (T)xp.z;
                 EdgeLin<T> La;
-                edge_linearize(Rf, tf, x, y, z, fx, fy, cx, cy, T(0), T(0), La);
-                const double wa = (double)w[a];
+                const T sa = edge_linearize(Rf, tf, x, y, z, fx, fy, cx, cy, (T)P.c_uv[2 * (size_t)k],
+                                            (T)P.c_uv[2 * (size_t)k + 1], La);
+                T wat;
+                robustify(sa, P.robust, (T)P.delta, wat);  // as k_lin_pt, without a gather of w
+                const double wa = (double)wat;
                 double JXa[6];
 #pragma unroll
                 for (int j = 0; j < 6; ++j) JXa[j] = (double)La.JX[j];
