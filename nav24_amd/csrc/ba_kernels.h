@@ -85,6 +85,9 @@ struct DevProblem {
     const int* row_sitem_ptr;  // [n_fc + 1]
     const int* row_nchunks;    // [n_fc]
     const int* sitem_order;    // [n_sitems] dispatch order: estimated work, heaviest first
+    const int* pair_seg;       // [n_sitems * schur_segs + 1] pair range of each (item, round, wave)
+    const int2* pairs;         // {observation b, owner lane | slot << 6 | local camera << 14}
+    int schur_segs;
     int n_sitems;
     int max_ns;                // largest slot count of any Schur work item
     // column blocks of each S row: free indices >= row (same window), first is the row itself

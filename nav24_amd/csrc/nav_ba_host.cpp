@@ -522,20 +522,69 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
             for (int ch = 0; ch < nch; ++ch) {
                 const int a0 = cam_ptr[f] + ch * kSchurChunk;
                 const int a1 = std::min(cam_ptr[f] + (ch + 1) * kSchurChunk, cam_ptr[f + 1]);
-                long long cost = 0;
-                for (int k = a0; k < a1; ++k) cost += 4 + (pt_ptr[o_pt[cam_obs[k]] + 1] - cam_obs_b0[k]);
                 sf.push_back(f);
                 sa0.push_back(a0);
                 sa1.push_back(a1);
                 sg.push_back(gi);
                 sns.push_back(ns);
                 spofs.push_back(pofs);
-                scost.push_back(cost);
+                scost.push_back(0);
                 pofs += 36LL * ns + 6;
             }
         }
         row_sitem_ptr.push_back((int)sf.size());
     }
+    // ---- Schur pair lists: per item, per (round, wave) segment, the pairs (a, b) of
+    // each lane's observation a with the later observations b of its point on free
+    // cameras of the item's column group (a itself excluded: the diagonal pair is
+    // summed in registers).  Entry: {b, lane | slot << 6 | local camera << 14}.
+    // Inside a segment the pairs are dealt round-robin over the slots.
+    const int rounds = (kSchurChunk + kBlock - 1) / kBlock;
+    const int segs = rounds * (kBlock / kWave);
+    std::vector<int> pair_seg{0};
+    std::vector<int2> pairs;
+    {
+        std::vector<int> slotmap(nc, -1);
+        std::vector<std::vector<int2>> bucket;
+        std::vector<int2> seg;
+        for (size_t it = 0; it < sf.size(); ++it) {
+            const int f = sf[it], gi = sg[it], ns = sns[it];
+            const int col0 = colptr[f] + gi * kMaxSlots;
+            const int w = cam_win[free_cam[f]];
+            for (int sl = 0; sl < ns; ++sl) slotmap[free_cam[colcam[col0 + sl]]] = sl;
+            long long npairs = 0;
+            for (int r = 0; r < rounds; ++r)
+                for (int wv = 0; wv < kBlock / kWave; ++wv) {
+                    seg.clear();
+                    const int kb = sa0[it] + r * kBlock + wv * kWave;
+                    for (int k = kb; k < std::min(kb + kWave, sa1[it]); ++k) {
+                        const int a = cam_obs[k];
+                        const int q = o_pt[a];
+                        if (!pt_free[q]) continue;
+                        for (int b = cam_obs_b0[k]; b < pt_ptr[q + 1]; ++b) {
+                            if (b == a) continue;
+                            const int sl = slotmap[o_cam[b]];
+                            if (sl < 0) continue;
+                            seg.push_back(int2{b, (k - kb) | (sl << 6) | ((o_cam[b] - cam_off[w]) << 14)});
+                        }
+                    }
+                    if (bucket.size() < (size_t)ns) bucket.resize(ns);
+                    for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
+                    for (const int2& e : seg) bucket[(e.y >> 6) & 255].push_back(e);
+                    size_t left = seg.size(), pos = 0;
+                    std::vector<size_t> cur(ns, 0);
+                    while (left > 0)
+                        for (int sl = 0; sl < ns; ++sl)
+                            if (cur[sl] < bucket[sl].size()) { seg[pos++] = bucket[sl][cur[sl]++]; --left; }
+                    pairs.insert(pairs.end(), seg.begin(), seg.end());
+                    pair_seg.push_back((int)pairs.size());
+                    npairs += (long long)seg.size();
+                }
+            for (int sl = 0; sl < ns; ++sl) slotmap[free_cam[colcam[col0 + sl]]] = -1;
+            scost[it] = npairs + 4LL * (sa1[it] - sa0[it]);
+        }
+    }
+    if (pairs.size() > (size_t)0x7fffffff) return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pairs");
     std::vector<int> sorder(sf.size());
     for (size_t i = 0; i < sorder.size(); ++i) sorder[i] = (int)i;
     std::stable_sort(sorder.begin(), sorder.end(), [&](int x, int y) { return scost[x] > scost[y]; });
@@ -603,6 +652,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.sitem_f, sf); UP(P.sitem_a0, sa0); UP(P.sitem_a1, sa1); UP(P.sitem_g, sg); UP(P.sitem_nslots, sns);
     UP(P.sitem_pofs, spofs); UP(P.row_sitem_ptr, row_sitem_ptr); UP(P.row_nchunks, row_nch);
     UP(P.sitem_order, sorder);
+    UP(P.pair_seg, pair_seg); UP(P.pairs, pairs);
+    P.schur_segs = segs;
     UP(P.colptr, colptr); UP(P.colcam, colcam);
     UP(P.cam_win, cam_win); UP(P.pt_win, pt_win_all); UP(P.win_cam0, win_cam0); UP(P.win_fc0, win_fc0);
     UP(P.win_pt0, win_pt0); UP(P.win_S, win_S); UP(P.win_lda, win_lda);

@@ -5,18 +5,19 @@
 // Hpl_ip (6x3) and D_p = Hll_p + lambda I:
 //     S_ij -= Hpl_ip D_p^-1 Hpl_jp^T  (j >= i),   coef_i += Hpl_ip D_p^-1 bl_p.
 // Here one workgroup owns one block row f of S (restricted to the cameras
-// co-visible with f, and to one observation chunk of camera f):
+// co-visible with f, and to one chunk of camera f's observations):
 //   phase A  one lane per observation a of camera f on a free point p:
-//            W_a = Hpl_a Dinv_p (FP64), the point's coordinates and the range
-//            of its observations on cameras >= f go to a per-wave LDS buffer;
-//            the diagonal pair (a, a) adds W_a Hpl_a^T (symmetric, 21 values)
-//            to registers of its own lane: every point has one, and in LDS
-//            they would all hit slot 0 at once;
-//   phase B  the wave's other (point, later observation b) pairs are packed
-//            densely over the 64 lanes (prefix sum + binary search), so the
-//            lanes of one point hit distinct S-row slots; each lane forms
+//            W_a = Hpl_a Dinv_p (FP64) and the point's coordinates go to the
+//            lane's slot of a per-wave LDS buffer; the diagonal pair (a, a)
+//            adds W_a Hpl_a^T (symmetric, 21 values) to the lane's registers;
+//   phase B  the wave's other pairs (a, b), b a later observation of the same
+//            point on a free camera of this column group, come from a pair
+//            list built once at prepare time (observation b, owner lane,
+//            column slot, camera: 8 bytes, read coalesced).  Each lane forms
 //            W_a Hpl_b^T = (W_a J_X,b^T)(w_b J_xi,b) from recomputed Jacobians
-//            and adds it into the LDS slot of camera(b) (slot 0 = camera f).
+//            and adds it into the LDS slot of camera(b).  Inside each wave's
+//            list the pairs are dealt round-robin over the slots, so the 64
+//            lanes of one LDS atomic mostly hit distinct slots.
 // Work items are dispatched heaviest first (a host-side permutation).
 // T is the type of the residuals and Jacobians only.  W, the pair products and
 // the S-row slots are FP64 for both compute types: the point elimination
@@ -43,16 +44,12 @@ constexpr int kCamStride = 13;  // odd: consecutive cameras fall in different LD
 template <typename T>
 struct SchurLayout {
     static constexpr int kWbuf = 21;  // W (18) + point (3), FP64
-    size_t acc, cam, wbuf, lo, aw, pre, slot, total;
+    size_t acc, cam, wbuf, total;
     __host__ __device__ SchurLayout(int ns, int n_cam, bool cam_lds) {  // n_cam: cameras of the largest window
         size_t o = 0;
         acc = o;  o += sizeof(double) * 36 * (size_t)ns;
         cam = o;  o += cam_lds ? sizeof(T) * kCamStride * (size_t)n_cam : 0;
         wbuf = o; o += sizeof(double) * kWbuf * kBlock;
-        lo = o;   o += sizeof(int) * kBlock;
-        aw = o;   o += sizeof(int) * kBlock;
-        pre = o;  o += sizeof(int) * (kBlock / kWave) * (kWave + 1);
-        slot = o; o += sizeof(short) * (size_t)n_cam;
         total = (o + 15) & ~(size_t)15;
     }
 };
@@ -90,29 +87,18 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     double* acc = reinterpret_cast<double*>(smem + L.acc);
     T* camt = reinterpret_cast<T*>(smem + L.cam);
     double* wbuf_all = reinterpret_cast<double*>(smem + L.wbuf);
-    int* lo_all = reinterpret_cast<int*>(smem + L.lo);
-    int* aw_all = reinterpret_cast<int*>(smem + L.aw);
-    int* pre_all = reinterpret_cast<int*>(smem + L.pre);
-    short* slot = reinterpret_cast<short*>(smem + L.slot);
 
     const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
     for (int i = tid; i < ns * 36; i += kBlock) acc[i] = 0.0;
-    for (int i = tid; i < ncw; i += kBlock) slot[i] = -1;
     if (kCamLds)
         for (int i = tid; i < ncw * 12; i += kBlock)
             camt[kCamStride * (i / 12) + i % 12] = (T)reinterpret_cast<const double*>(cam + cam0)[i];
-    __syncthreads();
-    const int col0 = P.colptr[f] + g * kMaxSlots;
-    for (int s = tid; s < ns; s += kBlock) slot[P.free_cam[P.colcam[col0 + s]] - cam0] = (short)s;
     __syncthreads();
 
     const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy;
     T Rf[9], tf[3];
     cam_get<T, kCamLds>(camt, cam, cf - cam0, cam0, Rf, tf);
     double* wbuf = wbuf_all + (size_t)wid * kWave * SchurLayout<T>::kWbuf;
-    int* lo_w = lo_all + wid * kWave;
-    int* aw_w = aw_all + wid * kWave;
-    int* pre = pre_all + wid * (kWave + 1);
 
     double coef[6], dsum[21];
 #pragma unroll
@@ -120,13 +106,12 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
 #pragma unroll
     for (int k = 0; k < 21; ++k) dsum[k] = 0.0;
 
-    for (int kb = a0 + wid * kWave; kb < a1; kb += kBlock) {
+    int round = 0;
+    for (int kb = a0 + wid * kWave; kb < a1; kb += kBlock, ++round) {
         wave_fence();
         // ---- phase A: one lane per observation of camera f
         const int k = kb + lane;
-        int cnt = 0;
         if (k < a1) {
-            const int a = P.cam_obs[k];
             const int p = P.c_pt[k];
             if (P.pt_free[p]) {
                 const D4 xp = X[p];
@@ -172,37 +157,18 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                         for (int c = i; c < 6; ++c, ++q)
                             dsum[q] += W[3 * i] * Ha[3 * c] + W[3 * i + 1] * Ha[3 * c + 1] + W[3 * i + 2] * Ha[3 * c + 2];
                 }
-                const int lo = P.cam_obs_b0[k];
-                lo_w[lane] = lo;
-                aw_w[lane] = a - lo;
-                cnt = P.pt_ptr[p + 1] - lo - 1;  // all later observations but a itself
             }
         }
-        // wave-exclusive prefix sum of the pair counts
-        int incl = cnt;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            const int v = __shfl_up(incl, off, kWave);
-            if (lane >= off) incl += v;
-        }
-        pre[lane] = incl - cnt;
-        const int total = __shfl(incl, kWave - 1, kWave);
         wave_fence();
-        // ---- phase B: the wave's pairs, 64 at a time
-        for (int t0 = 0; t0 < total; t0 += kWave) {
-            const int t = t0 + lane;
-            if (t >= total) continue;
-            int lo_i = 0, hi_i = kWave - 1;
-            while (lo_i < hi_i) {
-                const int mid = (lo_i + hi_i + 1) >> 1;
-                if (pre[mid] <= t) lo_i = mid; else hi_i = mid - 1;
-            }
-            const int i = lo_i;
-            const int tt = t - pre[i];
-            const int b = lo_w[i] + tt + (tt >= aw_w[i] ? 1 : 0);  // skips a itself
-            const int cb = P.o_cam[b] - cam0;
-            const int s = slot[cb];  // camera f itself is slot 0 of group 0
-            if (s < 0) continue;
+        // ---- phase B: this wave's pairs of this round, 64 at a time
+        const int seg = it * P.schur_segs + round * (kBlock / kWave) + wid;
+        const int q0 = P.pair_seg[seg], q1 = P.pair_seg[seg + 1];
+        for (int qb = q0; qb < q1; qb += kWave) {
+            const int q = qb + lane;
+            if (q >= q1) continue;
+            const int2 pr = P.pairs[q];
+            const int b = pr.x;
+            const int i = pr.y & 63, s = (pr.y >> 6) & 255, cb = pr.y >> 14;
             const double* wb = wbuf + i * SchurLayout<T>::kWbuf;
             T Rb[9], tb[3];
             cam_get<T, kCamLds>(camt, cam, cb, cam0, Rb, tb);
