@@ -162,6 +162,68 @@ def batched_leg(n_windows, steps, warmup):
     return out
 
 
+def _pose_point_rmse(a, b):
+    """RMS rotation angle (rad), translation (m) and point (m) differences."""
+    Ra = a.T_cw[:, :9].reshape(-1, 3, 3)
+    Rb = b.T_cw[:, :9].reshape(-1, 3, 3)
+    tr = np.einsum("nij,nij->n", Ra, Rb)  # trace(Ra Rb^T)
+    ang = np.arccos(np.clip((tr - 1.0) / 2.0, -1.0, 1.0))
+    dt = np.linalg.norm(a.T_cw[:, 9:] - b.T_cw[:, 9:], axis=1)
+    dx = np.linalg.norm(a.X - b.X, axis=1)
+    return float(np.sqrt(np.mean(ang ** 2))), float(np.sqrt(np.mean(dt ** 2))), float(np.sqrt(np.mean(dx ** 2)))
+
+
+def loop_leg(steps, warmup):
+    """300 keyframes on a 30 m loop, 150 000 points, 30 LM iterations: mixed
+    against FP64 on the GPU (final cost, pose and point RMSE)."""
+    import torch
+    from nav24_amd import problems
+    from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
+    from nav24_amd.ba import GraphOptimHip
+
+    prob = problems.workload(2, seed=0)
+    out = {"workload": WORKLOAD_NAMES[2], "n_cam": prob.n_cam, "n_pt": prob.n_pt, "n_obs": prob.n_obs}
+    res = {}
+    for name, prec in (("mixed", NAV_BA_MIXED), ("fp64", NAV_BA_FP64)):
+        o = workload_options(2)
+        o.precision = prec
+        s = GraphOptimHip(0)
+        t0 = time.perf_counter()
+        s.prepare(prob, o)
+        prep = time.perf_counter() - t0
+        for _ in range(warmup):
+            s.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        r = s.fetch()
+        s.close()
+        res[name] = r
+        out[name] = {"lm_iterations_per_s": round(r.iterations / dt, 2), "obs_per_s": round(prob.n_obs * r.iterations / dt, 1),
+                     "ms_per_solve": round(dt * 1e3, 2), "lm_iterations": r.iterations, "trials": r.trials,
+                     "final_cost": r.final_cost, "prepare_s": round(prep, 3)}
+    # the problem's own sensitivity: FP64 from points moved by one ulp
+    from nav24_amd.abi import Problem
+    q = Problem(**{k: getattr(prob, k) for k in ("T_cw", "X", "obs_cam", "obs_pt", "obs_uv", "fx", "fy", "cx", "cy",
+                                                 "cam_fixed", "pt_fixed")})
+    q.X = np.nextafter(prob.X, np.inf)
+    o = workload_options(2)
+    o.precision = NAV_BA_FP64
+    s = GraphOptimHip(0)
+    rq = s.solve(q, o)
+    s.close()
+    m, f = res["mixed"], res["fp64"]
+    out["fp64"]["one_ulp_point_change_rel"] = abs(rq.final_cost - f.final_cost) / f.final_cost
+    rot, trans, pts = _pose_point_rmse(m, f)
+    out["mixed"]["final_cost_rel_vs_fp64"] = abs(m.final_cost - f.final_cost) / f.final_cost
+    out["mixed"]["rmse_vs_fp64"] = {"rotation_rad": rot, "translation_m": trans, "point_m": pts}
+    out["mixed"]["speedup_vs_fp64"] = round(out["mixed"]["lm_iterations_per_s"] / out["fp64"]["lm_iterations_per_s"], 3)
+    return out
+
+
 def copy_calibration(mb, steps):
     """A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
     scales the same counters on the solver kernels."""
@@ -225,6 +287,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--windows", type=int, default=512, help="batched leg: number of windows (0 = skip)")
+    ap.add_argument("--loop", type=int, default=1, help="loop leg (300 keyframes, 150 000 points): 0 = skip")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling: run only the batched mixed solve (K timed runs) and print its timing")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -313,6 +376,8 @@ def main():
 
     if args.windows > 0:
         line["batched"] = batched_leg(args.windows, args.steps, args.warmup)
+    if args.loop:
+        line["loop"] = loop_leg(max(1, min(args.steps, 3)), 1)
 
     if not args.no_cpu:
         cb, ref = cpu_baseline(prob, opts)

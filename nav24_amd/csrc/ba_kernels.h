@@ -153,6 +153,15 @@ __device__ __forceinline__ int sym6(int i, int j) {
     return i * 6 - (i * (i - 1)) / 2 + (j - i);
 }
 
+// Huber weight rho'(s) computed in FP64 and rounded to T: the value stored per
+// observation and every recomputation of it agree bit for bit.
+template <typename T>
+__device__ __forceinline__ T huber_weight(double s, const DevProblem& P) {
+    double w;
+    robustify(s, P.robust, P.delta, w);
+    return (T)w;
+}
+
 template <typename T>
 __device__ __forceinline__ void load_cam(const CamRT* cam, int c, T R[9], T t[3]) {
     const double* p = reinterpret_cast<const double*>(cam + c);

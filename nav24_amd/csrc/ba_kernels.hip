@@ -72,24 +72,22 @@ __global__ __launch_bounds__(kBlock) void k_lin_pt(DevProblem P, const CamRT* __
     if (!needs_lin(lm, P.tile_win[t])) return;
     const int o0 = P.tile_obs[t], o1 = P.tile_obs[t + 1];
     const int p0 = P.tile_pt[t], p1 = P.tile_pt[t + 1];
-    const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy, delta = (T)P.delta;
     double c[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) c[k] = 0.0;
     for (int a = o0 + (int)threadIdx.x; a < o1; a += kBlock) {
         const int p = P.o_pt[a], ci = P.o_cam[a];
-        T R[9], tt[3];
+        double R[9], tt[3];
         load_cam(cam, ci, R, tt);
         const D4 xp = X[p];
         EdgeLin<T> L;
-        const T s = edge_linearize(R, tt, (T)xp.x, (T)xp.y, (T)xp.z, fx, fy, cx, cy,
-                                   (T)P.o_uv[2 * (size_t)a], (T)P.o_uv[2 * (size_t)a + 1], L);
-        T wt;
-        robustify(s, P.robust, delta, wt);
+        const double s = edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
+                                        P.o_uv[2 * (size_t)a], P.o_uv[2 * (size_t)a + 1], L);
+        const T wt = huber_weight<T>(s, P);
         w_out[a] = wt;
         if (P.pt_free[p]) {
             const double w = (double)wt;
-            const double r0 = -(double)L.e0 * w, r1 = -(double)L.e1 * w;
+            const double r0 = -L.e0 * w, r1 = -L.e1 * w;
             double A[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) A[k] = (double)L.JX[k];
@@ -152,24 +150,20 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
     const int f = P.hitem_f[it], a0 = P.hitem_a0[it], a1 = P.hitem_a1[it];
     const int cf = P.free_cam[f];
     if (!needs_lin(lm, P.cam_win[cf])) return;
-    T R[9], tt[3];
+    double R[9], tt[3];
     load_cam(cam, cf, R, tt);
-    const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy;
     double acc[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) acc[k] = 0.0;
-    const T delta = (T)P.delta;
     for (int k = a0 + (int)threadIdx.x; k < a1; k += kBlock) {
         // camera-major copies of the point index and measurement: contiguous reads;
         // the Huber weight is recomputed from the residual (the same arithmetic as k_lin_pt)
         const D4 xp = X[P.c_pt[k]];
         EdgeLin<T> L;
-        const T s = edge_linearize(R, tt, (T)xp.x, (T)xp.y, (T)xp.z, fx, fy, cx, cy,
-                                   (T)P.c_uv[2 * (size_t)k], (T)P.c_uv[2 * (size_t)k + 1], L);
-        T wt;
-        robustify(s, P.robust, delta, wt);
-        const double wa = (double)wt;
-        const double r0 = -(double)L.e0 * wa, r1 = -(double)L.e1 * wa;
+        const double s = edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
+                                        P.c_uv[2 * (size_t)k], P.c_uv[2 * (size_t)k + 1], L);
+        const double wa = (double)huber_weight<T>(s, P);
+        const double r0 = -L.e0 * wa, r1 = -L.e1 * wa;
         double J[12];
 #pragma unroll
         for (int i = 0; i < 12; ++i) J[i] = (double)L.Jxi[i];
@@ -396,10 +390,10 @@ __device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __
     const int cb = P.o_cam[ob];
     const int fb = P.cam_free[cb];
     if (fb < 0) return;
-    T R[9], tt[3];
+    double R[9], tt[3];
     load_cam(cam, cb, R, tt);
     EdgeLin<T> L;
-    edge_linearize(R, tt, (T)xp.x, (T)xp.y, (T)xp.z, (T)P.fx, (T)P.fy, (T)P.cx, (T)P.cy, T(0), T(0), L);
+    edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, L);
     const double wb = (double)w[ob];
     const double* d = dp + 6 * (size_t)fb;
     double t0 = 0.0, t1 = 0.0;
@@ -560,15 +554,14 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restr
         return;
     }
     double acc = 0.0;
-    const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy, delta = (T)P.delta;
     for (int a = P.cblk_o0[b] + threadIdx.x; a < P.cblk_o0[b + 1]; a += 256) {
-        T R[9], tt[3];
+        double R[9], tt[3];
         load_cam(cam, P.o_cam[a], R, tt);
         const D4 xp = X[P.o_pt[a]];
-        T e0, e1, zc, w;
-        const T s = edge_residual(R, tt, (T)xp.x, (T)xp.y, (T)xp.z, fx, fy, cx, cy,
-                                  (T)P.o_uv[2 * (size_t)a], (T)P.o_uv[2 * (size_t)a + 1], e0, e1, zc);
-        acc += (double)robustify(s, P.robust, delta, w);
+        double e0, e1, zc, w;
+        const double s = edge_residual(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
+                                       P.o_uv[2 * (size_t)a], P.o_uv[2 * (size_t)a + 1], e0, e1, zc);
+        acc += robustify(s, P.robust, P.delta, w);
     }
     const double s = block_sum<double, 256>(acc, sh);
     if (threadIdx.x == 0) red_cost[b] = s;

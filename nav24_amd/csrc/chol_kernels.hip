@@ -348,47 +348,60 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     }
 }
 
-// Backward step x = L^-T y, one workgroup, tile by tile from the last:
-// r_k = y_k - sum_{m>k} L_mk^T x_m (lanes over the tile's columns, waves over
-// the rows below), then x_k = M_k^T r_k.  dp = 0 when the factorisation failed.
-__global__ __launch_bounds__(256) void k_chol_solve(const double* __restrict__ Lm, const double* __restrict__ Li,
-                                                    int lda, int nt, const double* __restrict__ y,
-                                                    double* __restrict__ dp, const LMDev* lm) {
+// Backward step x = L^-T y, one launch per tile k from the last.  The
+// right-hand sides r of the tiles still to solve live in y (the forward
+// result) and are updated in place: every workgroup of launch k forms
+// x_k = M_k^T r_k (a 64x64 product, recomputed by each workgroup instead of
+// waiting for one); workgroup k writes it to dp, workgroup i < k subtracts
+// L_ki^T x_k from r_i.  r_k is only read during launch k, and final when it
+// starts (launches j > k subtracted their terms).  dp = 0 when the
+// factorisation failed.
+__global__ __launch_bounds__(256) void k_chol_back(const double* __restrict__ Lm, const double* __restrict__ Li,
+                                                   int lda, int nt, int k, double* __restrict__ y,
+                                                   double* __restrict__ dp, const LMDev* lm) {
     __shared__ double part[4][64];
-    __shared__ double rv[64];
+    __shared__ double xk[64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int i = blockIdx.x;  // 0..k: i == k writes x_k, i < k updates r_i
     if (lm->fail) {
-        for (int i = tid; i < 64 * nt; i += 256) dp[i] = 0.0;
+        if (k == nt - 1 && tid < 64) dp[64 * i + tid] = 0.0;  // launch nt-1 has one workgroup per tile
         return;
     }
-    for (int k = nt - 1; k >= 0; --k) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        const double* col = Lm + 64 * k + lane;
-        for (int m = 64 * (k + 1) + wid; m < 64 * nt; m += 16) {
-            s0 += col[(size_t)m * lda] * dp[m];
-            s1 += col[(size_t)(m + 4) * lda] * dp[m + 4];
-            s2 += col[(size_t)(m + 8) * lda] * dp[m + 8];
-            s3 += col[(size_t)(m + 12) * lda] * dp[m + 12];
-        }
-        part[wid][lane] = (s0 + s1) + (s2 + s3);
-        __syncthreads();
-        if (wid == 0) rv[lane] = y[64 * k + lane] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-        __syncthreads();
-        // x_k[c] = sum_{r >= c} M_k[r][c] rv[r]: lanes over c, waves over r
+    // x_k[c] = sum_{r >= c} M_k[r][c] r_k[r]: lanes over c, waves over r
+    {
         const double* Mk = Li + (size_t)4096 * k;
+        const double* rk = y + 64 * k;
         double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
 #pragma unroll
         for (int r = wid; r < 64; r += 16) {  // M_k[r][c] = 0 for r < c
-            t0 += Mk[r * 64 + lane] * rv[r];
-            t1 += Mk[(r + 4) * 64 + lane] * rv[r + 4];
-            t2 += Mk[(r + 8) * 64 + lane] * rv[r + 8];
-            t3 += Mk[(r + 12) * 64 + lane] * rv[r + 12];
+            t0 += Mk[r * 64 + lane] * rk[r];
+            t1 += Mk[(r + 4) * 64 + lane] * rk[r + 4];
+            t2 += Mk[(r + 8) * 64 + lane] * rk[r + 8];
+            t3 += Mk[(r + 12) * 64 + lane] * rk[r + 12];
         }
         part[wid][lane] = (t0 + t1) + (t2 + t3);
-        __syncthreads();
-        if (wid == 0) dp[64 * k + lane] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-        __syncthreads();
     }
+    __syncthreads();
+    if (wid == 0) xk[lane] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    __syncthreads();
+    if (i == k) {
+        if (tid < 64) dp[64 * k + tid] = xk[tid];
+        return;
+    }
+    // r_i[c] -= sum_m L[64k + m][64i + c] x_k[m]: lanes over c, waves over m
+    const double* L = Lm + (size_t)(64 * k) * lda + 64 * i + lane;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int m = wid; m < 64; m += 16) {
+        s0 += L[(size_t)m * lda] * xk[m];
+        s1 += L[(size_t)(m + 4) * lda] * xk[m + 4];
+        s2 += L[(size_t)(m + 8) * lda] * xk[m + 8];
+        s3 += L[(size_t)(m + 12) * lda] * xk[m + 12];
+    }
+    __syncthreads();
+    part[wid][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (wid == 0) y[64 * i + lane] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 // ---------------------------------------------------------------- per-window systems
@@ -813,8 +826,10 @@ void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st
         hipLaunchKernelGGL(k_chol_step, dim3(rem * (rem + 1) / 2), dim3(256), 0, st, W.S, W.Lm, W.Li, lda, k,
                            (const double*)W.g, W.y, W.lm);
     }
-    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(256), 0, st, (const double*)W.Lm, (const double*)W.Li, lda, nt,
-                       (const double*)W.y, W.dp, (const LMDev*)W.lm);
+    // backward: one launch per tile from the last
+    for (int k = nt - 1; k >= 0; --k)
+        hipLaunchKernelGGL(k_chol_back, dim3(k + 1), dim3(256), 0, st, (const double*)W.Lm, (const double*)W.Li, lda,
+                           nt, k, W.y, W.dp, (const LMDev*)W.lm);
 }
 
 }  // namespace navba

@@ -19,8 +19,8 @@
 //            list the pairs are dealt round-robin over the slots, so the 64
 //            lanes of one LDS atomic mostly hit distinct slots.
 // Work items are dispatched heaviest first (a host-side permutation).
-// T is the type of the residuals and Jacobians only.  W, the pair products and
-// the S-row slots are FP64 for both compute types: the point elimination
+// T is the type of the Jacobian entries only.  Cameras, points, W, the pair
+// products and the S-row slots are FP64 for both compute types: the point elimination
 // cancels Hpp against Hpl Dinv Hpl^T, and with a few-view point's Dinv
 // rounded to FP32 the final cost of an uneven window moved by 1e-3 relative
 // in a CPU emulation, against 1e-6 for FP32 Jacobians alone.  (On this GPU
@@ -48,17 +48,18 @@ struct SchurLayout {
     __host__ __device__ SchurLayout(int ns, int n_cam, bool cam_lds) {  // n_cam: cameras of the largest window
         size_t o = 0;
         acc = o;  o += sizeof(double) * 36 * (size_t)ns;
-        cam = o;  o += cam_lds ? sizeof(T) * kCamStride * (size_t)n_cam : 0;
+        cam = o;  o += cam_lds ? sizeof(double) * kCamStride * (size_t)n_cam : 0;
         wbuf = o; o += sizeof(double) * kWbuf * kBlock;
         total = (o + 15) & ~(size_t)15;
     }
 };
 
 // Camera c of the window (local index; cam0 = the window's first camera).
-template <typename T, bool kCamLds>
-__device__ __forceinline__ void cam_get(const T* camt, const CamRT* cam, int c, int cam0, T R[9], T t[3]) {
+template <bool kCamLds>
+__device__ __forceinline__ void cam_get(const double* camt, const CamRT* cam, int c, int cam0, double R[9],
+                                        double t[3]) {
     if (kCamLds) {
-        const T* p = camt + kCamStride * c;
+        const double* p = camt + kCamStride * c;
 #pragma unroll
         for (int k = 0; k < 9; ++k) R[k] = p[k];
 #pragma unroll
@@ -85,19 +86,18 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     const bool g0 = (g == 0);
     const SchurLayout<T> L(ns, P.max_win_cams, kCamLds);
     double* acc = reinterpret_cast<double*>(smem + L.acc);
-    T* camt = reinterpret_cast<T*>(smem + L.cam);
+    double* camt = reinterpret_cast<double*>(smem + L.cam);
     double* wbuf_all = reinterpret_cast<double*>(smem + L.wbuf);
 
     const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
     for (int i = tid; i < ns * 36; i += kBlock) acc[i] = 0.0;
     if (kCamLds)
         for (int i = tid; i < ncw * 12; i += kBlock)
-            camt[kCamStride * (i / 12) + i % 12] = (T)reinterpret_cast<const double*>(cam + cam0)[i];
+            camt[kCamStride * (i / 12) + i % 12] = reinterpret_cast<const double*>(cam + cam0)[i];
     __syncthreads();
 
-    const T fx = (T)P.fx, fy = (T)P.fy, cx = (T)P.cx, cy = (T)P.cy;
-    T Rf[9], tf[3];
-    cam_get<T, kCamLds>(camt, cam, cf - cam0, cam0, Rf, tf);
+    double Rf[9], tf[3];
+    cam_get<kCamLds>(camt, cam, cf - cam0, cam0, Rf, tf);
     double* wbuf = wbuf_all + (size_t)wid * kWave * SchurLayout<T>::kWbuf;
 
     double coef[6], dsum[21];
@@ -115,13 +115,10 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
             const int p = P.c_pt[k];
             if (P.pt_free[p]) {
                 const D4 xp = X[p];
-                const T x = (T)xp.x, y = (T)xp.y, z = (T)xp.z;
                 EdgeLin<T> La;
-                const T sa = edge_linearize(Rf, tf, x, y, z, fx, fy, cx, cy, (T)P.c_uv[2 * (size_t)k],
-                                            (T)P.c_uv[2 * (size_t)k + 1], La);
-                T wat;
-                robustify(sa, P.robust, (T)P.delta, wat);  // as k_lin_pt, without a gather of w
-                const double wa = (double)wat;
+                const double sa = edge_linearize(Rf, tf, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
+                                                 P.c_uv[2 * (size_t)k], P.c_uv[2 * (size_t)k + 1], La);
+                const double wa = (double)huber_weight<T>(sa, P);  // as k_lin_pt, without a gather of w
                 double JXa[6];
 #pragma unroll
                 for (int j = 0; j < 6; ++j) JXa[j] = (double)La.JX[j];
@@ -170,10 +167,10 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
             const int b = pr.x;
             const int i = pr.y & 63, s = (pr.y >> 6) & 255, cb = pr.y >> 14;
             const double* wb = wbuf + i * SchurLayout<T>::kWbuf;
-            T Rb[9], tb[3];
-            cam_get<T, kCamLds>(camt, cam, cb, cam0, Rb, tb);
+            double Rb[9], tb[3];
+            cam_get<kCamLds>(camt, cam, cb, cam0, Rb, tb);
             EdgeLin<T> Lb;
-            edge_linearize(Rb, tb, (T)wb[18], (T)wb[19], (T)wb[20], fx, fy, cx, cy, T(0), T(0), Lb);
+            edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
             const double wgt = (double)w[b];
             double JXb[6];
 #pragma unroll

@@ -50,6 +50,23 @@ def test_fp64_both_factorisations_match_oracle(solver, oracle_mod, mode):
     check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
 
 
+def test_loop_tiled_factorisation_fp64_and_mixed(solver, oracle_mod):
+    """300-keyframe loop (298 free cameras, 1788 pose unknowns: the tiled
+    multi-workgroup factorisation), fewer points so the oracle stays quick."""
+    _say("loop workload=2 n_pt=12000 fp64 and mixed, 4 iterations")
+    prob = problems.workload(2, seed=0, n_pt=12000)
+    opts = workload_options(2)
+    opts.precision = NAV_BA_FP64
+    opts.n_iter = 4
+    ref = oracle_mod.solve(prob, opts)
+    got = solver.solve(prob, opts)
+    check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
+    opts.precision = NAV_BA_MIXED
+    mix = solver.solve(prob, opts)
+    _say(f"  mixed {mix.final_cost!r} oracle {ref.final_cost!r} rel {_rel(mix.final_cost, ref.final_cost):.2e}")
+    assert _rel(mix.final_cost, ref.final_cost) <= 1e-5
+
+
 def test_mixed_solve_close_to_oracle(solver, oracle_mod):
     _say("mixed solve workload0")
     prob = problems.workload(0, seed=1)
