@@ -224,6 +224,45 @@ def loop_leg(steps, warmup):
     return out
 
 
+def diffcost_leg(n_problems, steps):
+    """Robust reprojection cost with radial-tangential distortion and its
+    hand-written adjoint (poses, points, intrinsics, distortion) over a batch of
+    20-keyframe, 5 000-point problems resident in HBM: forward (cost only) and
+    forward + backward throughput, FP32 against FP64."""
+    import torch
+    from nav24_amd import problems
+    from nav24_amd.diffcost import CostBatch
+
+    probs = [problems.workload(4, seed=s) for s in range(n_problems)]
+    T = np.concatenate([p.T_cw for p in probs])
+    X = np.concatenate([p.X for p in probs])
+    intr = np.array([probs[0].fx, probs[0].fy, probs[0].cx, probs[0].cy, *probs[0].dist])
+    b = CostBatch(probs)
+    out = {"workload": f"{n_problems} x (20 kf, 5000 pts), rad-tan distortion", "n_obs": b.n_obs,
+           "n_cam": b.n_cam, "n_pt": b.n_pt}
+    res = {}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        args = [torch.tensor(a, device="cuda", dtype=dt) for a in (T, X, intr)]
+        for grad, key in ((False, "forward"), (True, "forward_backward")):
+            b.eval(*args, grad=grad)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                r = b.eval(*args, grad=grad)
+            torch.cuda.synchronize()
+            dt_s = (time.perf_counter() - t0) / steps
+            out.setdefault(name, {})[key] = {"ms": round(dt_s * 1e3, 4), "obs_per_s": round(b.n_obs / dt_s, 1)}
+            if grad:
+                res[name] = r
+    rel = lambda a, r: float((a.double() - r).abs().max() / r.abs().max())
+    out["fp32"]["rel_err_vs_fp64"] = {"cost": rel(res["fp32"].cost, res["fp64"].cost),
+                                      "pose": rel(res["fp32"].g_pose, res["fp64"].g_pose),
+                                      "point": rel(res["fp32"].g_X, res["fp64"].g_X),
+                                      "intrinsics": rel(res["fp32"].g_intr, res["fp64"].g_intr)}
+    b.close()
+    return out
+
+
 def copy_calibration(mb, steps):
     """A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
     scales the same counters on the solver kernels."""
@@ -288,6 +327,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--windows", type=int, default=512, help="batched leg: number of windows (0 = skip)")
     ap.add_argument("--loop", type=int, default=1, help="loop leg (300 keyframes, 150 000 points): 0 = skip")
+    ap.add_argument("--diffcost", type=int, default=64,
+                    help="differentiable-cost leg: number of distorted-camera problems (0 = skip)")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling: run only the batched mixed solve (K timed runs) and print its timing")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -378,6 +419,8 @@ def main():
         line["batched"] = batched_leg(args.windows, args.steps, args.warmup)
     if args.loop:
         line["loop"] = loop_leg(max(1, min(args.steps, 3)), 1)
+    if args.diffcost > 0:
+        line["diffcost"] = diffcost_leg(args.diffcost, max(args.steps, 5))
 
     if not args.no_cpu:
         cb, ref = cpu_baseline(prob, opts)

@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 
-HIP_SOURCES = ["ba_kernels.hip", "schur_kernels.hip", "chol_kernels.hip", "nav_ba_host.cpp"]
+HIP_SOURCES = ["ba_kernels.hip", "schur_kernels.hip", "chol_kernels.hip", "nav_ba_host.cpp", "cost_kernels.hip"]
 
 
 def _run(cmd):

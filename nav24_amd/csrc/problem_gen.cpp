@@ -70,6 +70,7 @@ void rodrigues(const double w[3], double R[9]) {
 struct Cam { double fx, fy, cx, cy, k1, k2, p1, p2; int w, h; bool dist; };
 
 constexpr double kLoopMaxDepth = 25.0;  // metres: about 3 M observations for 150 000 points
+constexpr double kLoopMinDepth = 1.0;   // metres: no feature tracked closer than this
 
 // Pinhole (optionally radial-tangential / Brown-Conrady) projection.
 bool project(const Cam& K, const double T[12], const double X[3], double uv[2], double min_depth,
@@ -180,6 +181,7 @@ int32_t nav_gen_make(const nav_gen_params* p, nav_gen_problem** out) {
     // loop: features are tracked up to a range, so a point is seen from the
     // part of the loop near it (about 20 views per point)
     const double max_depth = p->geometry == NAV_GEN_LOOP ? kLoopMaxDepth : 1e300;
+    const double min_depth = p->geometry == NAV_GEN_LOOP ? kLoopMinDepth : 0.1;
     for (int i = 0; i < np; ++i) {
         double* X = &Xgt[3 * (size_t)i];
         int m = 0;
@@ -187,7 +189,7 @@ int32_t nav_gen_make(const nav_gen_params* p, nav_gen_problem** out) {
             sample_point(r, p->geometry, X);
             vis.clear();
             for (int c = 0; c < nc; ++c)
-                if (project(K, &Tgt[12 * (size_t)c], X, uv, 0.1, max_depth)) vis.push_back(c);
+                if (project(K, &Tgt[12 * (size_t)c], X, uv, min_depth, max_depth)) vis.push_back(c);
             if ((int)vis.size() >= 2 || attempt >= 1000) break;
         }
         if (p->obs_min > 0) {

@@ -14,9 +14,11 @@ from nav24_amd.abi import NAV_BA_NO_DEVICE, NavBaOptions, NavBaProblem, NavBaRes
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
+def declared(headers=None):
     names = []
-    for h in ("nav_ba.h", "nav_ba_debug.h"):
+    if headers is None:
+        headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    for h in headers:
         src = open(os.path.join(ROOT, "include", h)).read()
         names += re.findall(r"^\s*(?:const char\*|int32_t|void)\s+(nav_ba_\w+)\s*\(", src, re.M)
     return names
@@ -25,9 +27,20 @@ def declared():
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(lib_path("libnav24_ba.so"))
     names = declared()
-    assert sorted(names) == sorted(ba.EXPORTED)
+    assert sorted(declared(["nav_ba.h", "nav_ba_debug.h"])) == sorted(ba.EXPORTED)
+    assert "nav_ba_cost_eval" in names
     for n in names:
         assert hasattr(lib, n), n
+
+
+def test_cost_handle_reports_no_device_here():
+    from nav24_amd import diffcost
+    h = ctypes.c_void_p()
+    st = diffcost.lib().nav_ba_cost_create(ctypes.byref(h), 0)
+    if st == NAV_BA_NO_DEVICE:
+        assert not h.value
+    else:  # on a GPU box the handle is real
+        diffcost.lib().nav_ba_cost_destroy(h)
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
