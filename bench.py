@@ -51,7 +51,7 @@ def stage_work(st, fp64=True):
     no, npt, nc = st["n_obs"], st["n_pt"], st["n_cam"]
     if fp64:
         lin_pt = 32 * no + 113 * npt + 96 * nc     # o_cam, o_pt, uv, w out | X, Hll+bl out, ptr, free | cams
-        lin_cam = 32 * no + 32 * npt + 96 * nc     # index, o_pt, uv, w | X | cams
+        lin_cam = 20 * st["sum_k"] + 32 * npt + 96 * nc  # camera-major point index + uv | X | cams
         cost = 24 * no + 32 * npt + 96 * nc        # o_cam, o_pt, uv | X | cams
     else:
         lin_pt = 28 * no + 77 * npt + 96 * nc
@@ -62,6 +62,21 @@ def stage_work(st, fp64=True):
         "lin_pt": ("hbm", lin_pt), "lin_cam": ("hbm", lin_cam), "cost": ("hbm", cost),
         "schur_row": ("mfma", schur_flops),
     }
+
+
+STAGE_KERNEL = {"schur_row": "k_schur_row<double, true>", "lin_pt": "k_lin_pt<double>",
+                "lin_cam": "k_lin_cam<double>", "cost": "k_cost<double>"}
+
+
+def measured_traffic(workload_name, stage):
+    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload_name}_fp64.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"].get(STAGE_KERNEL.get(stage, ""))
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def roofline_entry(timing, st, fp64):
@@ -377,6 +392,16 @@ def main():
     roof = roofline_entry(per_step, st, True)
     roof["achieved"] = round(roof["achieved"], 3)
     roof["frac"] = round(roof["frac"], 5)
+    roof["traffic"] = measured_traffic(WORKLOAD_NAMES.get(args.workload, ""), roof["kernel"])
+    # the factorisation stage (latency-bound chain of tile launches), priced on FP64 flops
+    n = 6 * st["n_fc"]
+    trials = res.trials
+    chol_ms = timing.get("cholesky", (0.0, 0))[0] / args.steps
+    if chol_ms > 0 and trials > 0:
+        flops = n ** 3 / 3.0 + 2.0 * n * n
+        ach = flops / (chol_ms / trials / 1e3) / 1e12
+        roof["cholesky_stage"] = {"n": n, "flops_per_trial": flops, "us_per_trial": round(chol_ms / trials * 1e3, 2),
+                                  "achieved": round(ach, 4), "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 6)}
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "LM iterations/s", "n_gpus": 1,
