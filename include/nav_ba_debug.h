@@ -58,6 +58,18 @@ int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out);
  * to rounding; the switch lets tests cover both on the same problem. */
 int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode);
 
+/* Observations per Schur work item for the next nav_ba_prepare on this
+ * handle: 0 = automatic (problems of at least 131072 observations time the
+ * candidates 128, 256, 512, 1024 and 2048 on their initial linearisation at prepare
+ * and keep the fastest, remembered per device, precision and problem shape for
+ * the life of the process; smaller problems use 256), or a multiple of 64 up
+ * to 4096 to force it.  Any size gives the same reduced system up to the
+ * order of FP64 additions. */
+int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk);
+/* The chunk size the last nav_ba_prepare used (0 before a prepare, or when
+ * the problem had nothing to optimise). */
+int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk);
+
 #ifdef __cplusplus
 }
 #endif

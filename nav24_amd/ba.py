@@ -27,6 +27,7 @@ EXPORTED = [
     "nav_ba_default_options", "nav_ba_solve", "nav_ba_prepare", "nav_ba_run", "nav_ba_fetch",
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
+    "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk",
 ]
 
 STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
@@ -77,6 +78,10 @@ def lib() -> C.CDLL:
         L.nav_ba_get_timing.restype = C.c_int32
         L.nav_ba_debug_set_solver.argtypes = [H, C.c_int32]
         L.nav_ba_debug_set_solver.restype = C.c_int32
+        L.nav_ba_debug_set_schur_chunk.argtypes = [H, C.c_int32]
+        L.nav_ba_debug_set_schur_chunk.restype = C.c_int32
+        L.nav_ba_debug_get_schur_chunk.argtypes = [H, C.POINTER(C.c_int32)]
+        L.nav_ba_debug_get_schur_chunk.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -196,6 +201,18 @@ class GraphOptimHip:
     def set_solver(self, mode: int) -> None:
         """Factorisation for the next prepare: 0 auto, 1 64x64 tiles, 2 6x6 camera blocks."""
         self._check(lib().nav_ba_debug_set_solver(self._h, int(mode)))
+
+    def set_schur_chunk(self, chunk: int) -> None:
+        """Observations per Schur work item for the next prepare: 0 automatic
+        (tuned at prepare on large problems), or a multiple of 64 up to 4096."""
+        self._check(lib().nav_ba_debug_set_schur_chunk(self._h, int(chunk)))
+
+    @property
+    def schur_chunk(self) -> int:
+        """Observations per Schur work item chosen by the last prepare."""
+        v = C.c_int32(0)
+        self._check(lib().nav_ba_debug_get_schur_chunk(self._h, C.byref(v)))
+        return v.value
 
     def debug_reduced_system(self, lam: float):
         n = C.c_int32(0)

@@ -18,7 +18,7 @@ namespace navba {
 constexpr int kBlock = 256;        // threads per workgroup for the per-observation kernels
 constexpr int kTile = 256;         // observations per point-aligned linearisation tile
 constexpr int kChunk = 1024;       // camera-major observations per Hpp work item
-constexpr int kSchurChunk = 256;   // camera-major observations per Schur work item
+constexpr int kSchurChunk = 256;   // camera-major observations per Schur work item (default; tuned per problem)
 constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget)
 constexpr int kNB = 64;            // tiled dense Cholesky tile
 constexpr int kMaxWinCams = 8192;  // cameras per window: Schur slot map in LDS is int16

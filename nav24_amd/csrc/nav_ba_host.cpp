@@ -18,7 +18,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/nav_ba.h"
@@ -31,6 +34,13 @@ namespace {
 
 constexpr const char* kVersion = "nav24_amd-ba 0.2 (gfx950)";
 constexpr int kMaxDenseN = 16384;  // tiled dense reduced system of one window (2 GiB FP64)
+// Schur chunk tuner: problems with at least kTuneMinObs observations time each
+// candidate chunk size once at prepare (best of kTuneReps launches) and keep a
+// candidate only if it beats the current best by the margin.
+constexpr int kTuneMinObs = 1 << 17;
+constexpr int kTuneChunks[] = {128, 256, 512, 1024, 2048};
+constexpr int kTuneReps = 3;
+constexpr float kTuneMargin = 0.97f;
 
 struct DevBuf {
     void* p = nullptr;
@@ -106,6 +116,8 @@ struct nav_ba_handle {
     bool timing = false;
     bool timing_active = false;
     int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks
+    int schur_chunk = 0;  // observations per Schur work item: 0 = default or tuned
+    int schur_chunk_used = 0;
     StageTimer timer;
     LMDev* hstatus = nullptr;  // pinned: LM state of window 0
     int* hdone = nullptr;      // pinned: windows finished
@@ -284,12 +296,222 @@ int validate(const nav_ba_problem* p) {
     return NAV_BA_OK;
 }
 
+// ---- Schur work items -------------------------------------------------------------------
+// Host structure the items are built from (camera-major lists of free cameras,
+// per-row co-visible column blocks, point-major observations).
+struct SchurGraph {
+    int nfc, nc;
+    const int* colptr;
+    const int* colcam;
+    const int* cam_ptr;
+    const int* cam_obs;
+    const int* cam_obs_b0;
+    const int* o_pt;
+    const int* o_cam;
+    const int* pt_ptr;
+    const uint8_t* pt_free;
+    const int* free_cam;
+    const int* cam_win;
+    const int* cam_off;
+};
+
+// Work items of one chunk size (camera-major observations per item) and their pair lists.
+struct SchurItems {
+    int chunk = 0, segs = 0, max_ns = 1;
+    long long pofs = 0;
+    std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr, row_nch, pair_seg, sorder;
+    std::vector<long long> spofs;
+    std::vector<int2> pairs;
+};
+
+// Items: per S row, column groups of kMaxSlots, chunks of `chunk` observations,
+// dispatched heaviest first (pairs + observations).  Pair lists: per item, per
+// (round, wave) segment, the pairs (a, b) of each lane's observation a with the
+// later observations b of its point on free cameras of the item's column group
+// (a itself excluded: the diagonal pair is summed in registers).  Entry:
+// {b, lane | slot << 6 | local camera << 14}.  Inside a segment the pairs are
+// dealt round-robin over the slots.  False when the pair count exceeds int32.
+bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
+    S = SchurItems{};
+    S.chunk = chunk;
+    S.row_sitem_ptr.push_back(0);
+    for (int f = 0; f < G.nfc; ++f) {
+        const int ncols = G.colptr[f + 1] - G.colptr[f];
+        const int ngroups = (ncols + kMaxSlots - 1) / kMaxSlots;
+        const int nobs = G.cam_ptr[f + 1] - G.cam_ptr[f];
+        const int nch = std::max(1, (nobs + chunk - 1) / chunk);
+        S.row_nch.push_back(nch);
+        for (int gi = 0; gi < ngroups; ++gi) {
+            const int ns = std::min(kMaxSlots, ncols - gi * kMaxSlots);
+            S.max_ns = std::max(S.max_ns, ns);
+            for (int ch = 0; ch < nch; ++ch) {
+                S.sf.push_back(f);
+                S.sa0.push_back(G.cam_ptr[f] + ch * chunk);
+                S.sa1.push_back(std::min(G.cam_ptr[f] + (ch + 1) * chunk, G.cam_ptr[f + 1]));
+                S.sg.push_back(gi);
+                S.sns.push_back(ns);
+                S.spofs.push_back(S.pofs);
+                S.pofs += 36LL * ns + 6;
+            }
+        }
+        S.row_sitem_ptr.push_back((int)S.sf.size());
+    }
+    const int rounds = (chunk + kBlock - 1) / kBlock;
+    S.segs = rounds * (kBlock / kWave);
+    S.pair_seg.assign(1, 0);
+    std::vector<long long> scost(S.sf.size(), 0);
+    std::vector<int> slotmap(G.nc, -1);
+    std::vector<std::vector<int2>> bucket;
+    std::vector<int2> seg;
+    std::vector<size_t> cur;
+    for (size_t it = 0; it < S.sf.size(); ++it) {
+        const int f = S.sf[it], gi = S.sg[it], ns = S.sns[it];
+        const int col0 = G.colptr[f] + gi * kMaxSlots;
+        const int w = G.cam_win[G.free_cam[f]];
+        for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = sl;
+        long long npairs = 0;
+        for (int r = 0; r < rounds; ++r)
+            for (int wv = 0; wv < kBlock / kWave; ++wv) {
+                seg.clear();
+                const int kb = S.sa0[it] + r * kBlock + wv * kWave;
+                for (int k = kb; k < std::min(kb + kWave, S.sa1[it]); ++k) {
+                    const int a = G.cam_obs[k];
+                    const int q = G.o_pt[a];
+                    if (!G.pt_free[q]) continue;
+                    for (int b = G.cam_obs_b0[k]; b < G.pt_ptr[q + 1]; ++b) {
+                        if (b == a) continue;
+                        const int sl = slotmap[G.o_cam[b]];
+                        if (sl < 0) continue;
+                        seg.push_back(int2{b, (k - kb) | (sl << 6) | ((G.o_cam[b] - G.cam_off[w]) << 14)});
+                    }
+                }
+                if (bucket.size() < (size_t)ns) bucket.resize(ns);
+                for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
+                for (const int2& e : seg) bucket[(e.y >> 6) & 255].push_back(e);
+                size_t left = seg.size(), pos = 0;
+                cur.assign(ns, 0);
+                while (left > 0)
+                    for (int sl = 0; sl < ns; ++sl)
+                        if (cur[sl] < bucket[sl].size()) { seg[pos++] = bucket[sl][cur[sl]++]; --left; }
+                S.pairs.insert(S.pairs.end(), seg.begin(), seg.end());
+                if (S.pairs.size() > (size_t)0x7fffffff) return false;
+                S.pair_seg.push_back((int)S.pairs.size());
+                npairs += (long long)seg.size();
+            }
+        for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = -1;
+        scost[it] = npairs + 4LL * (S.sa1[it] - S.sa0[it]);
+    }
+    S.sorder.resize(S.sf.size());
+    for (size_t i = 0; i < S.sorder.size(); ++i) S.sorder[i] = (int)i;
+    std::stable_sort(S.sorder.begin(), S.sorder.end(), [&](int x, int y) { return scost[x] > scost[y]; });
+    return true;
+}
+
+// Upload the items and size the partial-sum buffer; these are the last device
+// allocations of a prepare, so candidates of the tuner reuse the same slots.
+int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
+    DevProblem& P = h->P;
+    int st;
+#define UP(dst, vec) if ((st = upload(h, &(dst), (vec))) != NAV_BA_OK) return st
+    UP(P.sitem_f, S.sf); UP(P.sitem_a0, S.sa0); UP(P.sitem_a1, S.sa1); UP(P.sitem_g, S.sg); UP(P.sitem_nslots, S.sns);
+    UP(P.sitem_pofs, S.spofs); UP(P.row_sitem_ptr, S.row_sitem_ptr); UP(P.row_nchunks, S.row_nch);
+    UP(P.sitem_order, S.sorder);
+    UP(P.pair_seg, S.pair_seg); UP(P.pairs, S.pairs);
+#undef UP
+    if ((st = alloc(h, (void**)&h->W.spart, sizeof(double) * std::max<long long>(S.pofs, 1))) != NAV_BA_OK) return st;
+    P.schur_segs = S.segs;
+    P.n_sitems = (int)S.sf.size();
+    P.max_ns = S.max_ns;
+    h->schur_chunk_used = S.chunk;
+    return NAV_BA_OK;
+}
+
+// Process-wide memory of tuned chunk sizes, per device, precision and problem shape.
+struct TuneKey {
+    int device, mixed, n_win, nc, np, no, nfc;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(device, mixed, n_win, nc, np, no, nfc) < std::tie(o.device, o.mixed, o.n_win, o.nc, o.np, o.no, o.nfc);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tuned;
+
+bool cached_chunk(const TuneKey& k, int* chunk) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(k);
+    if (it == g_tuned.end()) return false;
+    *chunk = it->second;
+    return true;
+}
+
+void remember_chunk(const TuneKey& k, int chunk) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[k] = chunk;
+}
+
+// Device time of the Schur stage (rows + reduction) of the uploaded items, best of kTuneReps.
+int time_schur(nav_ba_handle* h, float* best_ms) {
+    *best_ms = 1e30f;
+    for (int r = 0; r <= kTuneReps; ++r) {
+        HIPTRY(h, hipEventRecord(h->ev0, h->stream));
+        if (mixed(h)) {
+            launch_schur_row<float>(h->P, h->cur, h->W, h->stream);
+            launch_schur_reduce<float>(h->P, h->W, h->stream);
+        } else {
+            launch_schur_row<double>(h->P, h->cur, h->W, h->stream);
+            launch_schur_reduce<double>(h->P, h->W, h->stream);
+        }
+        HIPTRY(h, hipEventRecord(h->ev1, h->stream));
+        HIPTRY(h, hipEventSynchronize(h->ev1));
+        float ms = 0.f;
+        HIPTRY(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        if (r > 0) *best_ms = std::min(*best_ms, ms);  // the first launch is a warm-up
+    }
+    return NAV_BA_OK;
+}
+
+// Pick the chunk size of the Schur items by timing each candidate on the
+// problem itself, linearised at its initial state.  `S` holds the items of
+// the default size, uploaded from allocation slot `mark` on; it returns
+// holding (and with the device holding) the fastest candidate's items.
+int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems& S) {
+    hipStream_t s = h->stream;
+    const DevProblem& P = h->P;
+    HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
+    HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
+    std::vector<LMDev> L(P.n_win);
+    for (auto& l : L) { l = LMDev{}; l.lambda = 1e-4; l.nu = 2.0; l.phase = kPhaseLin; l.iter = 1; }
+    HIPTRY(h, hipMemcpyAsync(h->W.lm, L.data(), sizeof(LMDev) * L.size(), hipMemcpyHostToDevice, s));
+    launch_cam_rt(P, h->cur, s);
+    if (mixed(h)) { enqueue_lin<float>(h); launch_hinv<float>(P, h->W, s); }
+    else { enqueue_lin<double>(h); launch_hinv<double>(P, h->W, s); }
+    int st;
+    float best_ms;
+    if ((st = time_schur(h, &best_ms)) != NAV_BA_OK) return st;
+    const int def = S.chunk;
+    SchurItems C;
+    for (int chunk : kTuneChunks) {
+        if (chunk == def) continue;
+        if (!build_schur_items(G, chunk, C)) continue;
+        h->nbuf = mark;
+        if ((st = upload_schur_items(h, C)) != NAV_BA_OK) return st;
+        float ms;
+        if ((st = time_schur(h, &ms)) != NAV_BA_OK) return st;
+        if (ms < kTuneMargin * best_ms) { best_ms = ms; std::swap(S, C); }
+    }
+    h->nbuf = mark;
+    if ((st = upload_schur_items(h, S)) != NAV_BA_OK) return st;
+    HIPTRY(h, hipStreamSynchronize(s));
+    return NAV_BA_OK;
+}
+
 // Build the device structure of n_win windows laid end to end (see ba_kernels.h).
 int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, const nav_ba_options* o, bool batched) {
     if (!h) return NAV_BA_BAD_ARG;
     h->err.clear();
     h->prepared = false;
     h->ran = false;
+    h->schur_chunk_used = 0;
     if (!o || !probs || n_win <= 0) return fail(h, NAV_BA_BAD_ARG, "null options or problems");
     for (int w = 0; w < n_win; ++w)
         if (validate(&probs[w]) != NAV_BA_OK)
@@ -504,90 +726,16 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
         }
     }
 
-    // ---- Schur work items: per row, column groups of kMaxSlots, chunks of kSchurChunk;
-    // dispatched heaviest first (estimated pairs + observations)
-    std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr{0}, row_nch;
-    std::vector<long long> spofs, scost;
-    long long pofs = 0;
-    int max_ns = 1;
-    for (int f = 0; f < nfc; ++f) {
-        const int ncols = colptr[f + 1] - colptr[f];
-        const int ngroups = (ncols + kMaxSlots - 1) / kMaxSlots;
-        const int nobs = cam_ptr[f + 1] - cam_ptr[f];
-        const int nch = std::max(1, (nobs + kSchurChunk - 1) / kSchurChunk);
-        row_nch.push_back(nch);
-        for (int gi = 0; gi < ngroups; ++gi) {
-            const int ns = std::min(kMaxSlots, ncols - gi * kMaxSlots);
-            max_ns = std::max(max_ns, ns);
-            for (int ch = 0; ch < nch; ++ch) {
-                const int a0 = cam_ptr[f] + ch * kSchurChunk;
-                const int a1 = std::min(cam_ptr[f] + (ch + 1) * kSchurChunk, cam_ptr[f + 1]);
-                sf.push_back(f);
-                sa0.push_back(a0);
-                sa1.push_back(a1);
-                sg.push_back(gi);
-                sns.push_back(ns);
-                spofs.push_back(pofs);
-                scost.push_back(0);
-                pofs += 36LL * ns + 6;
-            }
-        }
-        row_sitem_ptr.push_back((int)sf.size());
-    }
-    // ---- Schur pair lists: per item, per (round, wave) segment, the pairs (a, b) of
-    // each lane's observation a with the later observations b of its point on free
-    // cameras of the item's column group (a itself excluded: the diagonal pair is
-    // summed in registers).  Entry: {b, lane | slot << 6 | local camera << 14}.
-    // Inside a segment the pairs are dealt round-robin over the slots.
-    const int rounds = (kSchurChunk + kBlock - 1) / kBlock;
-    const int segs = rounds * (kBlock / kWave);
-    std::vector<int> pair_seg{0};
-    std::vector<int2> pairs;
-    {
-        std::vector<int> slotmap(nc, -1);
-        std::vector<std::vector<int2>> bucket;
-        std::vector<int2> seg;
-        for (size_t it = 0; it < sf.size(); ++it) {
-            const int f = sf[it], gi = sg[it], ns = sns[it];
-            const int col0 = colptr[f] + gi * kMaxSlots;
-            const int w = cam_win[free_cam[f]];
-            for (int sl = 0; sl < ns; ++sl) slotmap[free_cam[colcam[col0 + sl]]] = sl;
-            long long npairs = 0;
-            for (int r = 0; r < rounds; ++r)
-                for (int wv = 0; wv < kBlock / kWave; ++wv) {
-                    seg.clear();
-                    const int kb = sa0[it] + r * kBlock + wv * kWave;
-                    for (int k = kb; k < std::min(kb + kWave, sa1[it]); ++k) {
-                        const int a = cam_obs[k];
-                        const int q = o_pt[a];
-                        if (!pt_free[q]) continue;
-                        for (int b = cam_obs_b0[k]; b < pt_ptr[q + 1]; ++b) {
-                            if (b == a) continue;
-                            const int sl = slotmap[o_cam[b]];
-                            if (sl < 0) continue;
-                            seg.push_back(int2{b, (k - kb) | (sl << 6) | ((o_cam[b] - cam_off[w]) << 14)});
-                        }
-                    }
-                    if (bucket.size() < (size_t)ns) bucket.resize(ns);
-                    for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
-                    for (const int2& e : seg) bucket[(e.y >> 6) & 255].push_back(e);
-                    size_t left = seg.size(), pos = 0;
-                    std::vector<size_t> cur(ns, 0);
-                    while (left > 0)
-                        for (int sl = 0; sl < ns; ++sl)
-                            if (cur[sl] < bucket[sl].size()) { seg[pos++] = bucket[sl][cur[sl]++]; --left; }
-                    pairs.insert(pairs.end(), seg.begin(), seg.end());
-                    pair_seg.push_back((int)pairs.size());
-                    npairs += (long long)seg.size();
-                }
-            for (int sl = 0; sl < ns; ++sl) slotmap[free_cam[colcam[col0 + sl]]] = -1;
-            scost[it] = npairs + 4LL * (sa1[it] - sa0[it]);
-        }
-    }
-    if (pairs.size() > (size_t)0x7fffffff) return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pairs");
-    std::vector<int> sorder(sf.size());
-    for (size_t i = 0; i < sorder.size(); ++i) sorder[i] = (int)i;
-    std::stable_sort(sorder.begin(), sorder.end(), [&](int x, int y) { return scost[x] > scost[y]; });
+    // ---- Schur work items and pair lists at the default chunk size (tuned below)
+    const SchurGraph G{nfc, nc, colptr.data(), colcam.data(), cam_ptr.data(), cam_obs.data(), cam_obs_b0.data(),
+                       o_pt.data(), o_cam.data(), pt_ptr.data(), pt_free.data(), free_cam.data(), cam_win.data(),
+                       cam_off.data()};
+    const TuneKey key{h->device, mixed(h) ? 1 : 0, n_win, nc, np, no, nfc};
+    int chunk0 = h->schur_chunk > 0 ? h->schur_chunk : kSchurChunk;
+    bool tune = h->schur_chunk == 0 && nfc > 0 && no >= kTuneMinObs;
+    if (tune && cached_chunk(key, &chunk0)) tune = false;  // this shape was tuned before on this device
+    SchurItems SI;
+    if (!build_schur_items(G, chunk0, SI)) return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pairs");
 
     // ---- dense reduced systems: one padded tiled system, or one small system per window
     std::vector<long long> win_S(n_win, 0);
@@ -649,11 +797,6 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.c_pt, c_pt); UP(P.c_uv, c_uv);
     UP(P.tile_obs, tile_obs); UP(P.tile_pt, tile_pt); UP(P.tile_win, tile_win);
     UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
-    UP(P.sitem_f, sf); UP(P.sitem_a0, sa0); UP(P.sitem_a1, sa1); UP(P.sitem_g, sg); UP(P.sitem_nslots, sns);
-    UP(P.sitem_pofs, spofs); UP(P.row_sitem_ptr, row_sitem_ptr); UP(P.row_nchunks, row_nch);
-    UP(P.sitem_order, sorder);
-    UP(P.pair_seg, pair_seg); UP(P.pairs, pairs);
-    P.schur_segs = segs;
     UP(P.colptr, colptr); UP(P.colcam, colcam);
     UP(P.cam_win, cam_win); UP(P.pt_win, pt_win_all); UP(P.win_cam0, win_cam0); UP(P.win_fc0, win_fc0);
     UP(P.win_pt0, win_pt0); UP(P.win_S, win_S); UP(P.win_lda, win_lda);
@@ -662,8 +805,6 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
 #undef UP
     P.n_tiles = (int)tile_win.size();
     P.n_hitems = (int)hf.size();
-    P.n_sitems = (int)sf.size();
-    P.max_ns = max_ns;
     P.n_win = n_win;
     P.max_win_cams = max_win_cams;
     P.max_win_fc = 0;
@@ -689,7 +830,6 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.hpart, sizeof(double) * 27 * std::max<size_t>(hf.size(), 1));
     AL(W.Hc, sizeof(double) * 28 * std::max(nfc, 1));
     AL(W.ptsys, sizeof(double) * 12 * (size_t)np);
-    AL(W.spart, sizeof(double) * std::max<long long>(pofs, 1));
     AL(W.S, sizeof(double) * (size_t)std::max<long long>(S_size, 1));
     AL(W.g, sizeof(double) * nvec);
     AL(W.y, sizeof(double) * nvec);
@@ -705,6 +845,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.done_count, sizeof(int));
     AL(W.lm, sizeof(LMDev) * n_win);
 #undef AL
+    const int schur_mark = h->nbuf;
+    if ((st = upload_schur_items(h, SI)) != NAV_BA_OK) return st;
     HIPTRY(h, hipMemcpyAsync(h->init.pose, pose0.data(), sizeof(PoseQT) * nc, hipMemcpyHostToDevice, h->stream));
     HIPTRY(h, hipMemcpyAsync(h->init.X, X0.data(), sizeof(D4) * np, hipMemcpyHostToDevice, h->stream));
     HIPTRY(h, hipStreamSynchronize(h->stream));
@@ -713,6 +855,10 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
         return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
     if (prepare_cholesky_blocks(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the block factorisation");
     if (batched && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
+    if (tune) {
+        if ((st = tune_schur_chunk(h, G, schur_mark, SI)) != NAV_BA_OK) return st;
+        remember_chunk(key, SI.chunk);
+    }
     if ((st = capture_step(h)) != NAV_BA_OK) return st;
     h->prepared = true;
     return NAV_BA_OK;
@@ -957,6 +1103,18 @@ int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out) {
 int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode) {
     if (!h || mode < 0 || mode > 2) return NAV_BA_BAD_ARG;
     h->solver_mode = mode;
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk) {
+    if (!h || chunk < 0 || chunk > 4096 || chunk % kWave != 0) return NAV_BA_BAD_ARG;
+    h->schur_chunk = chunk;
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk) {
+    if (!h || !chunk) return NAV_BA_BAD_ARG;
+    *chunk = h->prepared ? h->schur_chunk_used : 0;
     return NAV_BA_OK;
 }
 

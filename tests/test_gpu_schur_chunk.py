@@ -1,0 +1,59 @@
+"""Schur work-item size: forced sizes and the prepare-time tuner against the oracle.
+
+The chunk size only changes how observations of one camera row are split over
+workgroups (and so the order of FP64 additions), never what is summed: the
+reduced system must match the oracle's at every size, and the tuned solve must
+match the oracle like the default one.
+"""
+import numpy as np
+import pytest
+
+from nav24_amd import problems
+from nav24_amd.abi import NAV_BA_FP64, workload_options
+from parity_util import check_fp64_against_oracle, say as _say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("chunk", [64, 128, 512, 2048])
+def test_reduced_system_any_chunk(solver, oracle_mod, chunk):
+    _say(f"reduced system workload1 n_pt=3000 schur chunk {chunk}")
+    prob = problems.workload(1, seed=4, n_pt=3000)
+    opts = workload_options(1)
+    solver.set_schur_chunk(chunk)
+    try:
+        solver.prepare(prob, opts)
+        assert solver.schur_chunk == chunk
+        S, g = solver.debug_reduced_system(1e-4)
+    finally:
+        solver.set_schur_chunk(0)
+    So, go = oracle_mod.reduced_system(prob, opts, 1e-4)
+    assert S.shape == So.shape
+    assert np.abs(S - So).max() <= 1e-11 * np.abs(So).max()
+    assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
+
+
+def test_tuned_chunk_solve_matches_oracle(solver, oracle_mod):
+    prob = problems.workload(1, seed=5)
+    assert prob.n_obs >= 1 << 17, "the tuner runs from 131072 observations on"
+    _say(f"tuned schur chunk workload1 seed=5 n_obs={prob.n_obs}")
+    opts = workload_options(1)
+    opts.precision = NAV_BA_FP64
+    opts.n_iter = 6
+    solver.set_schur_chunk(0)
+    got = solver.solve(prob, opts)
+    chunk = solver.schur_chunk
+    _say(f"  tuned chunk {chunk}")
+    assert chunk in (128, 256, 512, 1024, 2048)
+    ref = oracle_mod.solve(prob, opts)
+    check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
+    # a second prepare of the same shape reuses the tuned size
+    solver.prepare(prob, opts)
+    assert solver.schur_chunk == chunk
+
+
+def test_chunk_argument_checked(solver):
+    from nav24_amd.ba import BAError
+    for bad in (-64, 100, 8192):
+        with pytest.raises(BAError):
+            solver.set_schur_chunk(bad)
