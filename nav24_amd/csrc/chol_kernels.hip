@@ -52,192 +52,236 @@ __device__ __forceinline__ void tile_store(const TileLds& T, double* __restrict_
     }
 }
 
-// In-LDS Cholesky of the lower triangle of A (64x64, 256 threads): A <- L with
-// a zero upper triangle.  Eight 8-column panels; each panel is factored by
-// wave 0, lane r holding row r of the panel in registers (pivot and column
-// entries broadcast with readlane, 1/sqrt by rsq + two Newton steps), then all
-// four waves apply the panel to the trailing lower triangle in 4x4 register
-// blocks.  Returns true (in every thread) if a pivot was not positive and
-// finite (it is then replaced by 1 so the values stay finite).
-__device__ __forceinline__ bool tile_potrf(TileLds& A, int* bad_flag) {
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// One FP64 16x16x4 matrix FMA of a wave, D = A B + C, with A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k and D[i][j] in element i / 4 of lane
+// j + 16 (i % 4) (checked on gfx950 against a host product).
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// In-LDS Cholesky of the SPD tile A (64x64, 256 threads): its lower triangle
+// becomes L and rpiv[j] = 1 / L_jj; the upper triangle is neither read nor
+// written.  Eight 8-column panels:
+//   diagonal block: thread 0 factors it in registers (1/sqrt by rsq + two
+//     Newton steps), the shortest chain through the eight pivots;
+//   panel rows: one thread per row below solves its row against the block;
+//   trailing lower triangle -= panel panel^T: FP64 MFMA on 16x16 blocks
+//     (K = 8), entries left of the trailing part masked.
+// Returns true in thread 0 if a pivot was not positive and finite (it is then
+// replaced by 1 so that the values stay finite).
+__device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) *bad_flag = 0;
-    __syncthreads();
-#pragma unroll
+    const int r16 = lane & 15, kq = lane >> 4;
+    bool bad = false;
     for (int p = 0; p < 8; ++p) {
         const int c0 = 8 * p;
-        if (wid == 0) {
-            double a[8];
+        if (tid == 0) {
+            double a[8][8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) a[j] = A[lane][c0 + j];
-            bool bad = false;
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j <= i; ++j) a[i][j] = A[c0 + i][c0 + j];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                double d = readlane(a[j], c0 + j);
+                double d = a[j][j];
                 const bool ok = (d > 0.0) && isfinite(d);
                 bad |= !ok;
                 d = ok ? d : 1.0;
                 double r = __builtin_amdgcn_rsq(d);
                 r = r * (1.5 - 0.5 * d * r * r);
                 r = r * (1.5 - 0.5 * d * r * r);
-                const double l = a[j] * r;
-                a[j] = (lane > c0 + j) ? l : ((lane == c0 + j) ? d * r : a[j]);
+                a[j][j] = d * r;
+                rpiv[c0 + j] = r;
 #pragma unroll
-                for (int c = j + 1; c < 8; ++c) {
-                    const double lc = readlane(a[j], c0 + c);
-                    a[c] = (lane >= c0 + c) ? a[c] - a[j] * lc : a[c];
-                }
-            }
-            if (lane >= c0) {
+                for (int i = j + 1; i < 8; ++i) a[i][j] *= r;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) A[lane][c0 + j] = a[j];
+                for (int i = j + 1; i < 8; ++i)
+#pragma unroll
+                    for (int c = j + 1; c <= i; ++c) a[i][c] -= a[i][j] * a[c][j];
             }
-            if (bad && lane == 0) *bad_flag = 1;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j <= i; ++j) A[c0 + i][c0 + j] = a[i][j];
         }
         __syncthreads();
-        if (p < 7) {
-            // A[r][c] -= sum_j L[r][c0+j] L[c][c0+j] for c0+8 <= c <= r
-            const int r0 = 4 * (tid >> 4), cc0 = 4 * (tid & 15);
-            if (cc0 + 3 >= c0 + 8 && r0 + 3 >= cc0) {
-                double acc[4][4] = {};
+        if (p == 7) break;
+        const int s0 = c0 + 8;  // first row and column of the trailing part
+        if (tid < 64 - s0) {
+            const int row = s0 + tid;
+            double x[8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    double x[4], y[4];
+            for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) { x[u] = A[r0 + u][c0 + j]; y[u] = A[cc0 + u][c0 + j]; }
+            for (int c = 0; c < 8; ++c) {
+                x[c] *= rpiv[c0 + c];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * y[v];
-                }
-                double cur[4][4];  // all 16 reads before the writes
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) cur[u][v] = A[r0 + u][cc0 + v];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        if (cc0 + v >= c0 + 8 && r0 + u >= cc0 + v) A[r0 + u][cc0 + v] = cur[u][v] - acc[u][v];
+                for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * A[c0 + e][c0 + c];
             }
-            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
         }
+        __syncthreads();
+        // 16x16 blocks (bi, bj), b0 <= bj <= bi <= 3, dealt over the waves
+        const int b0 = s0 >> 4, nb = 4 - b0;
+        for (int t = wid; t < nb * (nb + 1) / 2; t += 4) {
+            int i = 0;
+            while ((i + 1) * (i + 2) / 2 <= t) ++i;
+            const int bi = b0 + i, bj = b0 + (t - i * (i + 1) / 2);
+            const int ra = 16 * bi + r16, rb = 16 * bj + r16;
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 8; kk += 4) {
+                const double a = ra >= s0 ? A[ra][c0 + kk + kq] : 0.0;
+                const double b = rb >= s0 ? A[rb][c0 + kk + kq] : 0.0;
+                acc = mfma16x16x4(a, b, acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * bi + 4 * v + kq, col = rb;
+                if (col >= s0 && row >= col) A[row][col] -= acc[v];
+            }
+        }
+        __syncthreads();
     }
-    for (int idx = tid; idx < 4096; idx += 256)
-        if ((idx & 63) > (idx >> 6)) A[idx >> 6][idx & 63] = 0.0;
-    __syncthreads();
-    return *bad_flag != 0;
+    return bad;
 }
 
-// M = L^-1 for the lower-triangular 64x64 L (both in LDS, M full with a zero
-// upper triangle), in 8x8 blocks: eight threads invert the diagonal blocks,
-// then block row I = 1..7 is M_IJ = -M_II sum_{K=J}^{I-1} L_IK M_KJ for all
-// J < I at once (one thread per entry).
-__device__ __forceinline__ void tile_trinv(const TileLds& L, TileLds& M) {
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < 4096; idx += 256) M[idx >> 6][idx & 63] = 0.0;
-    __syncthreads();
-    if (tid < 8) {
-        const int b = 8 * tid;
-        double inv[8], m[8][8];
+// M = L^-1 for the factor in the lower triangle of A, rpiv = 1 / diag(L), by
+// recursive doubling.  8x8 diagonal blocks by substitution (one thread per
+// block column); then, for h = 8, 16, 32, the lower-left block of every
+// 2h x 2h diagonal block, M21 = -M22 (L21 M11): two h x h products each,
+// on FP64 MFMA for h = 32.  M's upper triangle is set to zero; A's upper
+// triangle is not read.
+__device__ __forceinline__ void tile_trinv(const TileLds& A, const double* rpiv, TileLds& M) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    for (int idx = tid; idx < 4096; idx += 256) {
+        const int i = idx >> 6, j = idx & 63;
+        if ((j >> 3) > (i >> 3)) M[i][j] = 0.0;
+    }
+    if (tid < 64) {
+        const int b = 8 * (tid >> 3), j = tid & 7;
+        double d[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) inv[j] = 1.0 / L[b + j][b + j];
+        for (int i = 0; i < 8; ++i) d[i] = (i == j) ? rpiv[b + j] : 0.0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int i = 1; i < 8; ++i) {
+            double s = 0.0;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) m[i][j] = 0.0;
-            m[j][j] = inv[j];
-#pragma unroll
-            for (int i = j + 1; i < 8; ++i) {
-                double v = 0.0;
-#pragma unroll
-                for (int k = j; k < i; ++k) v += L[b + i][b + k] * m[k][j];
-                m[i][j] = -v * inv[i];
-            }
+            for (int k = 0; k < i; ++k) s += A[b + i][b + k] * d[k];  // d[k] = 0 for k < j
+            if (i > j) d[i] = -s * rpiv[b + i];
         }
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-#pragma unroll
-            for (int j = 0; j <= i; ++j) M[b + i][b + j] = m[i][j];
+        for (int i = 0; i < 8; ++i) M[b + i][b + j] = d[i];
     }
     __syncthreads();
-    for (int I = 1; I < 8; ++I) {
-        const int ne = 64 * I;  // entries (i, jc): rows 8I..8I+7, columns 0..8I-1
-        double t[2];
+    // h = 8: four 16x16 diagonal blocks, one entry of M21 per thread
+    {
+        const int R = 16 * wid + 8, C = 16 * wid, i = lane >> 3, j = lane & 7;
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) t += A[R + i][C + b] * M[C + b][C + j];
+        M[R + i][C + j] = t;  // L21 M11, read back below
+        __syncthreads();
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) v += M[R + i][R + a] * M[R + a][C + j];
+        __syncthreads();
+        M[R + i][C + j] = -v;
+        __syncthreads();
+    }
+    // h = 16: two 32x32 diagonal blocks, two entries of M21 per thread
+    {
+        double v[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s;
-            t[s] = 0.0;
-            if (e < ne) {
-                const int i = 8 * I + e / (8 * I), jc = e % (8 * I);
-                // M[k][jc] = 0 for k < jc: start at a multiple of 4, four accumulators
-                double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-                for (int k = jc & ~3; k < 8 * I; k += 4) {
-                    v0 += L[i][k] * M[k][jc];
-                    v1 += L[i][k + 1] * M[k + 1][jc];
-                    v2 += L[i][k + 2] * M[k + 2][jc];
-                    v3 += L[i][k + 3] * M[k + 3][jc];
-                }
-                t[s] = (v0 + v1) + (v2 + v3);
+            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
+            const int R = 32 * g + 16, C = 32 * g;
+            double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+            for (int b = 0; b < 16; b += 2) {
+                t0 += A[R + i][C + b] * M[C + b][C + j];
+                t1 += A[R + i][C + b + 1] * M[C + b + 1][C + j];
             }
+            v[s] = t0 + t1;
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
+            M[32 * g + 16 + i][32 * g + j] = v[s];
         }
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s;
-            if (e < ne) M[8 * I + e / (8 * I)][e % (8 * I)] = t[s];
-        }
-        __syncthreads();
+            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
+            const int R = 32 * g + 16, C = 32 * g;
+            double t0 = 0.0, t1 = 0.0;
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s;
-            t[s] = 0.0;
-            if (e < ne) {
-                const int ii = e / (8 * I), jc = e % (8 * I);
-                double v = 0.0;
-                for (int k = 0; k <= ii; ++k) v += M[8 * I + ii][8 * I + k] * M[8 * I + k][jc];
-                t[s] = -v;
+            for (int a = 0; a < 16; a += 2) {
+                t0 += M[R + i][R + a] * M[R + a][C + j];
+                t1 += M[R + i][R + a + 1] * M[R + a + 1][C + j];
             }
+            v[s] = -(t0 + t1);
         }
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s;
-            if (e < ne) M[8 * I + e / (8 * I)][e % (8 * I)] = t[s];
+            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
+            M[32 * g + 16 + i][32 * g + j] = v[s];
         }
+        __syncthreads();
+    }
+    // h = 32: the 32x32 block M21 (rows 32.., columns 0..31), one 16x16 block per wave
+    {
+        const int bi = wid >> 1, bj = wid & 1;
+        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m0 = 0; m0 < 32; m0 += 4)  // M11[m][16bj + j] = 0 for m < 16bj
+            if (m0 >= 16 * bj) acc = mfma16x16x4(A[32 + 16 * bi + r16][m0 + kq], M[m0 + kq][16 * bj + r16], acc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) M[32 + 16 * bi + 4 * v + kq][16 * bj + r16] = acc[v];
+        __syncthreads();
+        acc = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m0 = 0; m0 < 32; m0 += 4)  // M22[i][m] = 0 for m > i
+            if (m0 < 16 * bi + 16)
+                acc = mfma16x16x4(M[32 + 16 * bi + r16][32 + m0 + kq], M[32 + m0 + kq][16 * bj + r16], acc);
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 4; ++v) M[32 + 16 * bi + 4 * v + kq][16 * bj + r16] = -acc[v];
         __syncthreads();
     }
 }
 
-// acc (4x4 per thread) = A B^T, rows 4*(tid>>4)+u, columns (tid&15)+16v.
-__device__ __forceinline__ void tile_gemm_nt(const TileLds& A, const TileLds& B, double acc[4][4]) {
-    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+// acc[b] = block (wave, b) of X Y^T in 16x16 blocks; with kTriY, Y is lower
+// triangular and the zero part of the sum is skipped.
+template <bool kTriY>
+__device__ __forceinline__ void tile_mfma_nt(const TileLds& X, const TileLds& Y, dbl4 acc[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int b = 0; b < 4; ++b) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-    for (int m = 0; m < 64; ++m) {
-        double a[4], b[4];
+    for (int m0 = 0; m0 < 64; m0 += 4) {
+        const double a = X[16 * w + r16][m0 + kq];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) { a[u] = A[4 * tr + u][m]; b[u] = B[tc + 16 * u][m]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+        for (int b = 0; b < 4; ++b)
+            if (!kTriY || m0 < 16 * b + 16) acc[b] = mfma16x16x4(a, Y[16 * b + r16][m0 + kq], acc[b]);
     }
 }
 
-// B <- B M^T (the panel solve B L^-T with M = L^-1 in LDS).
+// B <- B M^T (the panel solve B L^-T with M = L^-1 lower triangular, in LDS).
 __device__ __forceinline__ void tile_right_mul_t(TileLds& B, const TileLds& M) {
-    double acc[4][4];
-    tile_gemm_nt(B, M, acc);
-    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    dbl4 acc[4];
+    tile_mfma_nt<true>(B, M, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) B[4 * tr + u][tc + 16 * v] = acc[u][v];
+        for (int v = 0; v < 4; ++v) B[16 * w + 4 * v + kq][16 * b + r16] = acc[b][v];
     __syncthreads();
 }
 
@@ -281,12 +325,11 @@ __global__ __launch_bounds__(256) void k_chol_first(const double* __restrict__ S
     __shared__ TileLds A;
     __shared__ TileLds M;
     __shared__ double rv[64];
-    __shared__ int bad;
+    __shared__ double rpiv[64];
     tile_load(A, S, lda);
     __syncthreads();
-    if (tile_potrf(A, &bad) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
-    tile_trinv(A, M);
-    tile_store(A, Lm, lda);
+    if (tile_potrf(A, rpiv) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
+    tile_trinv(A, rpiv, M);
     tile_store(M, Li, 64);
     tile_forward(M, Lm, lda, 0, g, y, rv);
 }
@@ -298,7 +341,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     __shared__ TileLds Bi;
     __shared__ TileLds Bj;
     __shared__ double rv[64];
-    __shared__ int bad;
+    __shared__ double rpiv[64];
     const int idx = blockIdx.x;
     int ii = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
     while ((ii + 1) * (ii + 2) / 2 <= idx) ++ii;
@@ -313,38 +356,33 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     tile_right_mul_t(Bi, Mk);  // L_ik
     if (!same) tile_right_mul_t(Bj, Mk);  // L_jk
     if (j == k + 1) tile_store(Bi, Lm + (size_t)(64 * i) * lda + 64 * k, lda);
-    double acc[4][4];
-    tile_gemm_nt(Bi, same ? Bi : Bj, acc);
-    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    dbl4 acc[4];
+    tile_mfma_nt<false>(Bi, same ? Bi : Bj, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
     double* Aij = S + (size_t)(64 * i) * lda + 64 * j;
+    double cur[4][4];  // all 16 reads in flight before the writes
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cur[b][v] = Aij[(size_t)(16 * w + 4 * v + kq) * lda + 16 * b + r16];
     if (same && i == k + 1) {
-        double cur[4][4];
+        // Mk is free (the last product that read it ended with a barrier): it
+        // takes the updated tile to factor, Bj its inverse
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) cur[u][v] = Aij[(size_t)(4 * tr + u) * lda + tc + 16 * v];
-        __syncthreads();  // Mk is free again: it takes the tile to factor, Bj its inverse
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Mk[4 * tr + u][tc + 16 * v] = cur[u][v] - acc[u][v];
+            for (int v = 0; v < 4; ++v) Mk[16 * w + 4 * v + kq][16 * b + r16] = cur[b][v] - acc[b][v];
         __syncthreads();
-        if (tile_potrf(Mk, &bad) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
-        tile_trinv(Mk, Bj);
-        tile_store(Mk, Lm + (size_t)(64 * i) * lda + 64 * i, lda);
+        if (tile_potrf(Mk, rpiv) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
+        tile_trinv(Mk, rpiv, Bj);
         tile_store(Bj, Li + (size_t)4096 * i, 64);
         __syncthreads();  // L_i,k (stored above by this workgroup) is read back by the forward step
         tile_forward(Bj, Lm, lda, i, g, y, rv);
     } else {
-        double cur[4][4];  // all 16 reads in flight before the writes
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) cur[u][v] = Aij[(size_t)(4 * tr + u) * lda + tc + 16 * v];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Aij[(size_t)(4 * tr + u) * lda + tc + 16 * v] = cur[u][v] - acc[u][v];
+            for (int v = 0; v < 4; ++v) Aij[(size_t)(16 * w + 4 * v + kq) * lda + 16 * b + r16] = cur[b][v] - acc[b][v];
     }
 }
 
@@ -363,21 +401,38 @@ __global__ __launch_bounds__(256) void k_chol_back(const double* __restrict__ Lm
     __shared__ double xk[64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int i = blockIdx.x;  // 0..k: i == k writes x_k, i < k updates r_i
-    if (lm->fail) {
-        if (k == nt - 1 && tid < 64) dp[64 * i + tid] = 0.0;  // launch nt-1 has one workgroup per tile
-        return;
+    // every global load of the launch is issued before the first use: the
+    // L_ki column slice (independent of x_k), M_k, r_k and the failure flag
+    double lv[16];
+    if (i < k) {
+        const double* L = Lm + (size_t)(64 * k) * lda + 64 * i + lane;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) lv[t] = L[(size_t)(wid + 4 * t) * lda];
     }
-    // x_k[c] = sum_{r >= c} M_k[r][c] r_k[r]: lanes over c, waves over r
+    double mv[16], rv[16];
     {
         const double* Mk = Li + (size_t)4096 * k;
         const double* rk = y + 64 * k;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {  // rows r = wid + 4t; M_k[r][c] = 0 for r < c
+            mv[t] = Mk[(wid + 4 * t) * 64 + lane];
+            rv[t] = rk[wid + 4 * t];
+        }
+    }
+    const int fail = lm->fail;
+    if (fail) {
+        if (k == nt - 1 && tid < 64) dp[64 * i + tid] = 0.0;  // launch nt-1 has one workgroup per tile
+        return;
+    }
+    // x_k[c] = sum_r M_k[r][c] r_k[r]: lanes over c, waves over r
+    {
         double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
 #pragma unroll
-        for (int r = wid; r < 64; r += 16) {  // M_k[r][c] = 0 for r < c
-            t0 += Mk[r * 64 + lane] * rk[r];
-            t1 += Mk[(r + 4) * 64 + lane] * rk[r + 4];
-            t2 += Mk[(r + 8) * 64 + lane] * rk[r + 8];
-            t3 += Mk[(r + 12) * 64 + lane] * rk[r + 12];
+        for (int t = 0; t < 16; t += 4) {
+            t0 += mv[t] * rv[t];
+            t1 += mv[t + 1] * rv[t + 1];
+            t2 += mv[t + 2] * rv[t + 2];
+            t3 += mv[t + 3] * rv[t + 3];
         }
         part[wid][lane] = (t0 + t1) + (t2 + t3);
     }
@@ -388,15 +443,14 @@ __global__ __launch_bounds__(256) void k_chol_back(const double* __restrict__ Lm
         if (tid < 64) dp[64 * k + tid] = xk[tid];
         return;
     }
-    // r_i[c] -= sum_m L[64k + m][64i + c] x_k[m]: lanes over c, waves over m
-    const double* L = Lm + (size_t)(64 * k) * lda + 64 * i + lane;
+    // r_i[c] -= sum_m L[64k + m][64i + c] x_k[m]: lanes over c, waves over m = wid + 4t
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-    for (int m = wid; m < 64; m += 16) {
-        s0 += L[(size_t)m * lda] * xk[m];
-        s1 += L[(size_t)(m + 4) * lda] * xk[m + 4];
-        s2 += L[(size_t)(m + 8) * lda] * xk[m + 8];
-        s3 += L[(size_t)(m + 12) * lda] * xk[m + 12];
+    for (int t = 0; t < 16; t += 4) {
+        s0 += lv[t] * xk[wid + 4 * t];
+        s1 += lv[t + 1] * xk[wid + 4 * t + 4];
+        s2 += lv[t + 2] * xk[wid + 4 * t + 8];
+        s3 += lv[t + 3] * xk[wid + 4 * t + 12];
     }
     __syncthreads();
     part[wid][lane] = (s0 + s1) + (s2 + s3);
