@@ -184,7 +184,7 @@ template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWor
 void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st);
 void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st);
 int prepare_cholesky_small(const DevProblem& P);  // LDS attribute of the per-window factorisation
-int prepare_cholesky_blocks(const DevProblem& P);  // LDS attribute of the one-problem block factorisation
+int prepare_cholesky(const DevProblem& P);  // LDS attributes of the one-problem factorisation kernels
 template <typename T> void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand,
                                           const DevWork& W, hipStream_t st);
 void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);

@@ -16,26 +16,39 @@
 // updated A_ik, A_jk and M_k, then updates A_ij -= L_ik L_jk^T; workgroups
 // (i, k+1) store L_ik, and workgroup (k+1, k+1) factors its freshly updated
 // tile, inverts it and runs that tile's forward step.  The factor goes to a
-// separate buffer, so no workgroup overwrites a tile another one still reads.
+// separate buffer, so no workgroup overwrites a tile another one still reads;
+// the diagonal factors L_kk themselves are never stored (only M_k is used).
+// The tile products run on FP64 MFMA (v_mfma_f64_16x16x4); the chain through
+// the pivots runs in registers.  The backward step is a single launch.
 #include "ba_kernels.h"
 
 namespace navba {
 
 typedef double TileLds[64][65];
 
-// 16 elements per thread; all loads are issued before the first store
-__device__ __forceinline__ void tile_load(TileLds& T, const double* __restrict__ src, int lda) {
-    double v[16];
+// A 64x64 tile through registers, 16 elements per thread: fetch issues the
+// global loads, put writes LDS (several fetches can be in flight before the
+// first put).
+__device__ __forceinline__ void tile_fetch(double v[16], const double* __restrict__ src, int lda) {
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         const int idx = threadIdx.x + 256 * s;
         v[s] = src[(size_t)(idx >> 6) * lda + (idx & 63)];
     }
+}
+
+__device__ __forceinline__ void tile_put(TileLds& T, const double v[16]) {
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         const int idx = threadIdx.x + 256 * s;
         T[idx >> 6][idx & 63] = v[s];
     }
+}
+
+__device__ __forceinline__ void tile_load(TileLds& T, const double* __restrict__ src, int lda) {
+    double v[16];
+    tile_fetch(v, src, lda);
+    tile_put(T, v);
 }
 
 __device__ __forceinline__ void tile_store(const TileLds& T, double* __restrict__ dst, int lda) {
@@ -63,12 +76,13 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
 
 // In-LDS Cholesky of the SPD tile A (64x64, 256 threads): its lower triangle
 // becomes L and rpiv[j] = 1 / L_jj; the upper triangle is neither read nor
-// written.  Eight 8-column panels:
-//   diagonal block: thread 0 factors it in registers (1/sqrt by rsq + two
-//     Newton steps), the shortest chain through the eight pivots;
-//   panel rows: one thread per row below solves its row against the block;
-//   trailing lower triangle -= panel panel^T: FP64 MFMA on 16x16 blocks
-//     (K = 8), entries left of the trailing part masked.
+// written.  Eight 8-column panels, two barriers each:
+//   wave 0: every lane factors the panel's 8x8 diagonal block in registers
+//     (the same values in all lanes, from broadcast LDS reads; 1/sqrt by rsq +
+//     two Newton steps), then lane l solves panel row c0 + 8 + l against it;
+//   all waves: trailing lower triangle -= panel panel^T on FP64 MFMA, 16x16
+//     blocks (K = 8), a wave's blocks loaded and multiplied together, entries
+//     left of the trailing part masked.
 // Returns true in thread 0 if a pivot was not positive and finite (it is then
 // replaced by 1 so that the values stay finite).
 __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
@@ -76,13 +90,20 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
     const int r16 = lane & 15, kq = lane >> 4;
     bool bad = false;
     for (int p = 0; p < 8; ++p) {
-        const int c0 = 8 * p;
-        if (tid == 0) {
-            double a[8][8];
+        const int c0 = 8 * p, s0 = c0 + 8;  // s0: first row and column of the trailing part
+        if (wid == 0) {
+            double a[8][8], rp[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i)
 #pragma unroll
                 for (int j = 0; j <= i; ++j) a[i][j] = A[c0 + i][c0 + j];
+            double x[8];
+            const int row = s0 + lane;
+            const bool has_row = row < 64;
+            if (has_row) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
+            }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 double d = a[j][j];
@@ -93,7 +114,7 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
                 r = r * (1.5 - 0.5 * d * r * r);
                 r = r * (1.5 - 0.5 * d * r * r);
                 a[j][j] = d * r;
-                rpiv[c0 + j] = r;
+                rp[j] = r;
 #pragma unroll
                 for (int i = j + 1; i < 8; ++i) a[i][j] *= r;
 #pragma unroll
@@ -101,47 +122,69 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
 #pragma unroll
                     for (int c = j + 1; c <= i; ++c) a[i][c] -= a[i][j] * a[c][j];
             }
+            if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+                for (int i = 0; i < 8; ++i) {
+                    rpiv[c0 + i] = rp[i];
 #pragma unroll
-                for (int j = 0; j <= i; ++j) A[c0 + i][c0 + j] = a[i][j];
+                    for (int j = 0; j <= i; ++j) A[c0 + i][c0 + j] = a[i][j];
+                }
+            }
+            if (has_row) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    x[c] *= rp[c];
+#pragma unroll
+                    for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * a[e][c];
+                }
+#pragma unroll
+                for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
+            }
         }
         __syncthreads();
         if (p == 7) break;
-        const int s0 = c0 + 8;  // first row and column of the trailing part
-        if (tid < 64 - s0) {
-            const int row = s0 + tid;
-            double x[8];
+        // 16x16 blocks (bi, bj), b0 <= bj <= bi <= 3, numbered row by row;
+        // wave w takes blocks w, w + 4, w + 8
+        const int b0 = s0 >> 4, nb = 4 - b0, nblk = nb * (nb + 1) / 2;
+        int bi[3], rb[3];
+        double av[3][2], bv[3][2];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                x[c] *= rpiv[c0 + c];
-#pragma unroll
-                for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * A[c0 + e][c0 + c];
-            }
-#pragma unroll
-            for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
-        }
-        __syncthreads();
-        // 16x16 blocks (bi, bj), b0 <= bj <= bi <= 3, dealt over the waves
-        const int b0 = s0 >> 4, nb = 4 - b0;
-        for (int t = wid; t < nb * (nb + 1) / 2; t += 4) {
+        for (int u = 0; u < 3; ++u) {
+            const int t = wid + 4 * u;
             int i = 0;
             while ((i + 1) * (i + 2) / 2 <= t) ++i;
-            const int bi = b0 + i, bj = b0 + (t - i * (i + 1) / 2);
-            const int ra = 16 * bi + r16, rb = 16 * bj + r16;
-            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+            bi[u] = b0 + i;
+            rb[u] = 16 * (b0 + (t - i * (i + 1) / 2)) + r16;
+            if (t < nblk) {
+                const int ra = 16 * bi[u] + r16;
 #pragma unroll
-            for (int kk = 0; kk < 8; kk += 4) {
-                const double a = ra >= s0 ? A[ra][c0 + kk + kq] : 0.0;
-                const double b = rb >= s0 ? A[rb][c0 + kk + kq] : 0.0;
-                acc = mfma16x16x4(a, b, acc);
+                for (int h = 0; h < 2; ++h) {
+                    av[u][h] = ra >= s0 ? A[ra][c0 + 4 * h + kq] : 0.0;
+                    bv[u][h] = rb[u] >= s0 ? A[rb[u]][c0 + 4 * h + kq] : 0.0;
+                }
+            }
+        }
+        dbl4 acc[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+                if (wid + 4 * u < nblk) acc[u] = mfma16x16x4(av[u][h], bv[u][h], acc[u]);
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (wid + 4 * u >= nblk) continue;
+            double cur[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * bi[u] + 4 * v + kq, col = rb[u];
+                cur[v] = (col >= s0 && row >= col) ? A[row][col] : 0.0;
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = 16 * bi + 4 * v + kq, col = rb;
-                if (col >= s0 && row >= col) A[row][col] -= acc[v];
+                const int row = 16 * bi[u] + 4 * v + kq, col = rb[u];
+                if (col >= s0 && row >= col) A[row][col] = cur[v] - acc[u][v];
             }
         }
         __syncthreads();
@@ -149,111 +192,107 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
     return bad;
 }
 
+// Orders the LDS accesses of one wave around a hand-off between its lanes
+// (the wave's LDS instructions execute in program order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // M = L^-1 for the factor in the lower triangle of A, rpiv = 1 / diag(L), by
-// recursive doubling.  8x8 diagonal blocks by substitution (one thread per
-// block column); then, for h = 8, 16, 32, the lower-left block of every
-// 2h x 2h diagonal block, M21 = -M22 (L21 M11): two h x h products each,
-// on FP64 MFMA for h = 32.  M's upper triangle is set to zero; A's upper
-// triangle is not read.
-__device__ __forceinline__ void tile_trinv(const TileLds& A, const double* rpiv, TileLds& M) {
+// recursive doubling; M's upper triangle is set to zero.  The strict upper
+// triangle of A is used as scratch (its rows 0..31, columns 32..63).
+//   wave q, on its 16x16 diagonal block: the two 8x8 diagonal blocks by
+//     substitution (one lane per column), then M21 = -M22 (L21 M11) of the
+//     block with 8x8 products on FP64 MFMA (zero padded to 16x16);
+//   waves 0 and 1: the same for the two 32x32 diagonal blocks (16x16 products);
+//   all waves: the same for the whole tile (32x32 products, one 16x16 block
+//     per wave).
+// Four barriers; the hand-offs inside one wave need none.
+__device__ __forceinline__ void tile_trinv(TileLds& A, const double* rpiv, TileLds& M) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int r16 = lane & 15, kq = lane >> 4;
     for (int idx = tid; idx < 4096; idx += 256) {
         const int i = idx >> 6, j = idx & 63;
         if ((j >> 3) > (i >> 3)) M[i][j] = 0.0;
     }
-    if (tid < 64) {
-        const int b = 8 * (tid >> 3), j = tid & 7;
-        double d[8];
+    {
+        const int C = 16 * wid, R = C + 8;
+        if (lane < 16) {
+            const int b = C + 8 * (lane >> 3), j = lane & 7;
+            double d[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) d[i] = (i == j) ? rpiv[b + j] : 0.0;
+            for (int i = 0; i < 8; ++i) d[i] = (i == j) ? rpiv[b + j] : 0.0;
 #pragma unroll
-        for (int i = 1; i < 8; ++i) {
-            double s = 0.0;
+            for (int i = 1; i < 8; ++i) {
+                double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < i; ++k) s += A[b + i][b + k] * d[k];  // d[k] = 0 for k < j
-            if (i > j) d[i] = -s * rpiv[b + i];
+                for (int k = 0; k < i; ++k) s += A[b + i][b + k] * d[k];  // d[k] = 0 for k < j
+                if (i > j) d[i] = -s * rpiv[b + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) M[b + i][b + j] = d[i];
         }
+        wave_sync();
+        // T = L21 M11 (8x8): rows i = 4v + kq < 8, columns r16 < 8 of the 16x16 product
+        const bool in = r16 < 8;
+        dbl4 t = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int i = 0; i < 8; ++i) M[b + i][b + j] = d[i];
+        for (int m0 = 0; m0 < 8; m0 += 4)
+            t = mfma16x16x4(in ? A[R + r16][C + m0 + kq] : 0.0, in ? M[C + m0 + kq][C + r16] : 0.0, t);
+        if (in) {
+            M[R + kq][C + r16] = t[0];
+            M[R + 4 + kq][C + r16] = t[1];
+        }
+        wave_sync();
+        dbl4 u = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m0 = 0; m0 < 8; m0 += 4)
+            u = mfma16x16x4(in ? M[R + r16][R + m0 + kq] : 0.0, in ? M[R + m0 + kq][C + r16] : 0.0, u);
+        wave_sync();
+        if (in) {
+            M[R + kq][C + r16] = -u[0];
+            M[R + 4 + kq][C + r16] = -u[1];
+        }
     }
     __syncthreads();
-    // h = 8: four 16x16 diagonal blocks, one entry of M21 per thread
-    {
-        const int R = 16 * wid + 8, C = 16 * wid, i = lane >> 3, j = lane & 7;
-        double t = 0.0;
+    if (wid < 2) {
+        const int C = 32 * wid, R = C + 16;
+        dbl4 t = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int b = 0; b < 8; ++b) t += A[R + i][C + b] * M[C + b][C + j];
-        M[R + i][C + j] = t;  // L21 M11, read back below
-        __syncthreads();
-        double v = 0.0;
+        for (int m0 = 0; m0 < 16; m0 += 4) t = mfma16x16x4(A[R + r16][C + m0 + kq], M[C + m0 + kq][C + r16], t);
 #pragma unroll
-        for (int a = 0; a < 8; ++a) v += M[R + i][R + a] * M[R + a][C + j];
-        __syncthreads();
-        M[R + i][C + j] = -v;
-        __syncthreads();
+        for (int v = 0; v < 4; ++v) M[R + 4 * v + kq][C + r16] = t[v];
+        wave_sync();
+        dbl4 u = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m0 = 0; m0 < 16; m0 += 4) u = mfma16x16x4(M[R + r16][R + m0 + kq], M[R + m0 + kq][C + r16], u);
+        wave_sync();
+#pragma unroll
+        for (int v = 0; v < 4; ++v) M[R + 4 * v + kq][C + r16] = -u[v];
     }
-    // h = 16: two 32x32 diagonal blocks, two entries of M21 per thread
+    __syncthreads();
+    const int bi = wid >> 1, bj = wid & 1;
     {
-        double v[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
-            const int R = 32 * g + 16, C = 32 * g;
-            double t0 = 0.0, t1 = 0.0;
-#pragma unroll
-            for (int b = 0; b < 16; b += 2) {
-                t0 += A[R + i][C + b] * M[C + b][C + j];
-                t1 += A[R + i][C + b + 1] * M[C + b + 1][C + j];
-            }
-            v[s] = t0 + t1;
-        }
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
-            M[32 * g + 16 + i][32 * g + j] = v[s];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
-            const int R = 32 * g + 16, C = 32 * g;
-            double t0 = 0.0, t1 = 0.0;
-#pragma unroll
-            for (int a = 0; a < 16; a += 2) {
-                t0 += M[R + i][R + a] * M[R + a][C + j];
-                t1 += M[R + i][R + a + 1] * M[R + a + 1][C + j];
-            }
-            v[s] = -(t0 + t1);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int e = tid + 256 * s, g = e >> 8, i = (e >> 4) & 15, j = e & 15;
-            M[32 * g + 16 + i][32 * g + j] = v[s];
-        }
-        __syncthreads();
-    }
-    // h = 32: the 32x32 block M21 (rows 32.., columns 0..31), one 16x16 block per wave
-    {
-        const int bi = wid >> 1, bj = wid & 1;
-        dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        // T = L21 M11 (32x32) into A's strict upper triangle: T[i][j] at A[i][32 + j]
+        dbl4 t = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int m0 = 0; m0 < 32; m0 += 4)  // M11[m][16bj + j] = 0 for m < 16bj
-            if (m0 >= 16 * bj) acc = mfma16x16x4(A[32 + 16 * bi + r16][m0 + kq], M[m0 + kq][16 * bj + r16], acc);
+            if (m0 >= 16 * bj) t = mfma16x16x4(A[32 + 16 * bi + r16][m0 + kq], M[m0 + kq][16 * bj + r16], t);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) M[32 + 16 * bi + 4 * v + kq][16 * bj + r16] = acc[v];
-        __syncthreads();
-        acc = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int v = 0; v < 4; ++v) A[16 * bi + 4 * v + kq][32 + 16 * bj + r16] = t[v];
+    }
+    __syncthreads();
+    {
+        dbl4 u = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int m0 = 0; m0 < 32; m0 += 4)  // M22[i][m] = 0 for m > i
             if (m0 < 16 * bi + 16)
-                acc = mfma16x16x4(M[32 + 16 * bi + r16][32 + m0 + kq], M[32 + m0 + kq][16 * bj + r16], acc);
-        __syncthreads();
+                u = mfma16x16x4(M[32 + 16 * bi + r16][32 + m0 + kq], A[m0 + kq][32 + 16 * bj + r16], u);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) M[32 + 16 * bi + 4 * v + kq][16 * bj + r16] = -acc[v];
-        __syncthreads();
+        for (int v = 0; v < 4; ++v) M[32 + 16 * bi + 4 * v + kq][16 * bj + r16] = -u[v];
     }
+    __syncthreads();
 }
 
 // acc[b] = block (wave, b) of X Y^T in 16x16 blocks; with kTriY, Y is lower
@@ -349,22 +388,27 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     const int jj = idx - ii * (ii + 1) / 2;
     const int i = k + 1 + ii, j = k + 1 + jj;
     const bool same = (i == j);
-    tile_load(Mk, Li + (size_t)4096 * k, 64);
-    tile_load(Bi, S + (size_t)(64 * i) * lda + 64 * k, lda);
-    if (!same) tile_load(Bj, S + (size_t)(64 * j) * lda + 64 * k, lda);
+    // every global load of the launch in flight at once: M_k, A_ik, A_jk and
+    // the tile A_ij to update (no other workgroup writes any of them)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
+    double* Aij = S + (size_t)(64 * i) * lda + 64 * j;
+    double vm[16], vi[16], vj[16], cur[4][4];
+    tile_fetch(vm, Li + (size_t)4096 * k, 64);
+    tile_fetch(vi, S + (size_t)(64 * i) * lda + 64 * k, lda);
+    if (!same) tile_fetch(vj, S + (size_t)(64 * j) * lda + 64 * k, lda);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cur[b][v] = Aij[(size_t)(16 * w + 4 * v + kq) * lda + 16 * b + r16];
+    tile_put(Mk, vm);
+    tile_put(Bi, vi);
+    if (!same) tile_put(Bj, vj);
     __syncthreads();
     tile_right_mul_t(Bi, Mk);  // L_ik
     if (!same) tile_right_mul_t(Bj, Mk);  // L_jk
     if (j == k + 1) tile_store(Bi, Lm + (size_t)(64 * i) * lda + 64 * k, lda);
     dbl4 acc[4];
     tile_mfma_nt<false>(Bi, same ? Bi : Bj, acc);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
-    double* Aij = S + (size_t)(64 * i) * lda + 64 * j;
-    double cur[4][4];  // all 16 reads in flight before the writes
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) cur[b][v] = Aij[(size_t)(16 * w + 4 * v + kq) * lda + 16 * b + r16];
     if (same && i == k + 1) {
         // Mk is free (the last product that read it ended with a barrier): it
         // takes the updated tile to factor, Bj its inverse
@@ -386,76 +430,87 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     }
 }
 
-// Backward step x = L^-T y, one launch per tile k from the last.  The
-// right-hand sides r of the tiles still to solve live in y (the forward
-// result) and are updated in place: every workgroup of launch k forms
-// x_k = M_k^T r_k (a 64x64 product, recomputed by each workgroup instead of
-// waiting for one); workgroup k writes it to dp, workgroup i < k subtracts
-// L_ki^T x_k from r_i.  r_k is only read during launch k, and final when it
-// starts (launches j > k subtracted their terms).  dp = 0 when the
-// factorisation failed.
-__global__ __launch_bounds__(256) void k_chol_back(const double* __restrict__ Lm, const double* __restrict__ Li,
-                                                   int lda, int nt, int k, double* __restrict__ y,
-                                                   double* __restrict__ dp, const LMDev* lm) {
-    __shared__ double part[4][64];
+// Backward step x = L^-T y in one workgroup of 512 threads, tile by tile
+// from the last.  The right-hand sides r (the forward result) are held in LDS
+// and updated in place:
+//   x_k  = M_k^T r_k      (64x64 product, 8 partial sums per entry),
+//   r_i -= L_ki^T x_k     (all columns left of tile k: 128 per pass, 4 partial
+//                          sums per column);
+// r_k is final when step k starts (every later step has subtracted its
+// term).  The L_ki loads of a step's first pass are issued before x_k is
+// formed.  One workgroup keeps the tile chain inside one launch.  dp = 0 when
+// the factorisation failed.
+constexpr int kBackThreads = 512;
+
+__global__ __launch_bounds__(kBackThreads) void k_chol_back(const double* __restrict__ Lm,
+                                                            const double* __restrict__ Li, int lda, int nt,
+                                                            const double* __restrict__ y, double* __restrict__ dp,
+                                                            const LMDev* lm) {
+    extern __shared__ double rs[];  // [64 nt]
+    __shared__ double part[8][64];
+    __shared__ double red[4][128];
     __shared__ double xk[64];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int i = blockIdx.x;  // 0..k: i == k writes x_k, i < k updates r_i
-    // every global load of the launch is issued before the first use: the
-    // L_ki column slice (independent of x_k), M_k, r_k and the failure flag
-    double lv[16];
-    if (i < k) {
-        const double* L = Lm + (size_t)(64 * k) * lda + 64 * i + lane;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) lv[t] = L[(size_t)(wid + 4 * t) * lda];
-    }
-    double mv[16], rv[16];
-    {
-        const double* Mk = Li + (size_t)4096 * k;
-        const double* rk = y + 64 * k;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {  // rows r = wid + 4t; M_k[r][c] = 0 for r < c
-            mv[t] = Mk[(wid + 4 * t) * 64 + lane];
-            rv[t] = rk[wid + 4 * t];
-        }
-    }
-    const int fail = lm->fail;
-    if (fail) {
-        if (k == nt - 1 && tid < 64) dp[64 * i + tid] = 0.0;  // launch nt-1 has one workgroup per tile
+    const int tid = threadIdx.x, n = 64 * nt;
+    if (lm->fail) {
+        for (int i = tid; i < n; i += kBackThreads) dp[i] = 0.0;
         return;
     }
-    // x_k[c] = sum_r M_k[r][c] r_k[r]: lanes over c, waves over r
-    {
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    for (int i = tid; i < n; i += kBackThreads) rs[i] = y[i];
+    __syncthreads();
+    const int c = tid & 63, grp = tid >> 6;   // x_k: entry c, rows grp + 8u
+    const int pc = tid & 127, mq = tid >> 7;  // updates: column pc of a pass, rows 16 mq + u
+    for (int k = nt - 1; k >= 0; --k) {
+        const int ncols = 64 * k;
+        const double* Lk = Lm + (size_t)(64 * k) * lda;
+        double lv[16];
+        if (pc < ncols) {
 #pragma unroll
-        for (int t = 0; t < 16; t += 4) {
-            t0 += mv[t] * rv[t];
-            t1 += mv[t + 1] * rv[t + 1];
-            t2 += mv[t + 2] * rv[t + 2];
-            t3 += mv[t + 3] * rv[t + 3];
+            for (int u = 0; u < 16; ++u) lv[u] = Lk[(size_t)(16 * mq + u) * lda + pc];
         }
-        part[wid][lane] = (t0 + t1) + (t2 + t3);
-    }
-    __syncthreads();
-    if (wid == 0) xk[lane] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
-    __syncthreads();
-    if (i == k) {
-        if (tid < 64) dp[64 * k + tid] = xk[tid];
-        return;
-    }
-    // r_i[c] -= sum_m L[64k + m][64i + c] x_k[m]: lanes over c, waves over m = wid + 4t
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        {
+            const double* Mk = Li + (size_t)4096 * k;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-    for (int t = 0; t < 16; t += 4) {
-        s0 += lv[t] * xk[wid + 4 * t];
-        s1 += lv[t + 1] * xk[wid + 4 * t + 4];
-        s2 += lv[t + 2] * xk[wid + 4 * t + 8];
-        s3 += lv[t + 3] * xk[wid + 4 * t + 12];
+            for (int u = 0; u < 8; u += 4) {
+                s0 += Mk[(grp + 8 * u) * 64 + c] * rs[64 * k + grp + 8 * u];
+                s1 += Mk[(grp + 8 * u + 8) * 64 + c] * rs[64 * k + grp + 8 * u + 8];
+                s2 += Mk[(grp + 8 * u + 16) * 64 + c] * rs[64 * k + grp + 8 * u + 16];
+                s3 += Mk[(grp + 8 * u + 24) * 64 + c] * rs[64 * k + grp + 8 * u + 24];
+            }
+            part[grp][c] = (s0 + s1) + (s2 + s3);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const double x = ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid])) +
+                             ((part[4][tid] + part[5][tid]) + (part[6][tid] + part[7][tid]));
+            xk[tid] = x;
+            dp[64 * k + tid] = x;
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < ncols; c0 += 128) {
+            const int col = c0 + pc;
+            double s = 0.0;
+            if (col < ncols) {
+                if (c0 > 0) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) lv[u] = Lk[(size_t)(16 * mq + u) * lda + col];
+                }
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; u += 4) {
+                    s0 += lv[u] * xk[16 * mq + u];
+                    s1 += lv[u + 1] * xk[16 * mq + u + 1];
+                    s2 += lv[u + 2] * xk[16 * mq + u + 2];
+                    s3 += lv[u + 3] * xk[16 * mq + u + 3];
+                }
+                s = (s0 + s1) + (s2 + s3);
+            }
+            red[mq][pc] = s;
+            __syncthreads();
+            if (tid < 128 && col < ncols) rs[col] -= (red[0][pc] + red[1][pc]) + (red[2][pc] + red[3][pc]);
+            __syncthreads();
+        }
     }
-    __syncthreads();
-    part[wid][lane] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    if (wid == 0) y[64 * i + lane] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 // ---------------------------------------------------------------- per-window systems
@@ -857,8 +912,14 @@ size_t cholesky_blocks_smem(int nb) {
     return sizeof(double) * ((size_t)(nb > 1 ? nb - 1 : 1) * kBS + (size_t)nb * kBS + 6 * (size_t)nb + kBS);
 }
 
-int prepare_cholesky_blocks(const DevProblem& P) {
-    if (!P.use_blocks) return 0;
+int prepare_cholesky(const DevProblem& P) {
+    if (!P.use_blocks) {
+        // the tiled path's backward step holds the right-hand sides in LDS
+        const size_t smem = sizeof(double) * (size_t)std::max(P.n_pad, 64);
+        if (smem + 9 * 1024 > 160 * 1024) return -1;
+        return hipFuncSetAttribute((const void*)k_chol_back, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) ==
+                       hipSuccess ? 0 : -1;
+    }
     const size_t smem = cholesky_blocks_smem(P.n_fc);
     if (smem > 160 * 1024) return -1;
     return hipFuncSetAttribute((const void*)k_chol_blocks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) ==
@@ -880,10 +941,9 @@ void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st
         hipLaunchKernelGGL(k_chol_step, dim3(rem * (rem + 1) / 2), dim3(256), 0, st, W.S, W.Lm, W.Li, lda, k,
                            (const double*)W.g, W.y, W.lm);
     }
-    // backward: one launch per tile from the last
-    for (int k = nt - 1; k >= 0; --k)
-        hipLaunchKernelGGL(k_chol_back, dim3(k + 1), dim3(256), 0, st, (const double*)W.Lm, (const double*)W.Li, lda,
-                           nt, k, W.y, W.dp, (const LMDev*)W.lm);
+    hipLaunchKernelGGL(k_chol_back, dim3(1), dim3(kBackThreads), sizeof(double) * (size_t)lda, st,
+                       (const double*)W.Lm, (const double*)W.Li, lda, nt, (const double*)W.y, W.dp,
+                       (const LMDev*)W.lm);
 }
 
 }  // namespace navba

@@ -853,7 +853,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     // Schur rows and the per-window factorisation may use more than 64 KiB of LDS
     if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
         return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
-    if (prepare_cholesky_blocks(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the block factorisation");
+    if (prepare_cholesky(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the factorisation kernels");
     if (batched && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
     if (tune) {
         if ((st = tune_schur_chunk(h, G, schur_mark, SI)) != NAV_BA_OK) return st;
