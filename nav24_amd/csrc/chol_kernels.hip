@@ -74,36 +74,38 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Index of the calling wave in its workgroup, as a scalar (wave-uniform
+// branches and block indices stay on the scalar unit).
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // In-LDS Cholesky of the SPD tile A (64x64, 256 threads): its lower triangle
-// becomes L and rpiv[j] = 1 / L_jj; the upper triangle is neither read nor
-// written.  Eight 8-column panels, two barriers each:
+// becomes L and rpiv[j] = 1 / L_jj; the upper triangle is not read.  Eight
+// 8-column panels, two barriers each:
 //   wave 0: every lane factors the panel's 8x8 diagonal block in registers
 //     (the same values in all lanes, from broadcast LDS reads; 1/sqrt by rsq +
 //     two Newton steps), then lane l solves panel row c0 + 8 + l against it;
 //   all waves: trailing lower triangle -= panel panel^T on FP64 MFMA, 16x16
-//     blocks (K = 8), a wave's blocks loaded and multiplied together, entries
-//     left of the trailing part masked.
+//     blocks (K = 8), a wave's blocks loaded and multiplied together.  Entries
+//     of a block outside the trailing lower triangle get their own value
+//     back (no other wave writes them, and the panel entries among them are
+//     final), so no access is predicated.
 // Returns true in thread 0 if a pivot was not positive and finite (it is then
 // replaced by 1 so that the values stay finite).
 __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int lane = threadIdx.x & 63, wid = wave_id();
     const int r16 = lane & 15, kq = lane >> 4;
     bool bad = false;
     for (int p = 0; p < 8; ++p) {
         const int c0 = 8 * p, s0 = c0 + 8;  // s0: first row and column of the trailing part
         if (wid == 0) {
-            double a[8][8], rp[8];
+            double a[8][8], rp[8], x[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i)
 #pragma unroll
                 for (int j = 0; j <= i; ++j) a[i][j] = A[c0 + i][c0 + j];
-            double x[8];
-            const int row = s0 + lane;
-            const bool has_row = row < 64;
-            if (has_row) {
+            const int row = min(s0 + lane, 63);
 #pragma unroll
-                for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
-            }
+            for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 double d = a[j][j];
@@ -122,6 +124,12 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
 #pragma unroll
                     for (int c = j + 1; c <= i; ++c) a[i][c] -= a[i][j] * a[c][j];
             }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                x[c] *= rp[c];
+#pragma unroll
+                for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * a[e][c];
+            }
             if (lane == 0) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
@@ -130,13 +138,7 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
                     for (int j = 0; j <= i; ++j) A[c0 + i][c0 + j] = a[i][j];
                 }
             }
-            if (has_row) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    x[c] *= rp[c];
-#pragma unroll
-                    for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * a[e][c];
-                }
+            if (s0 + lane < 64) {
 #pragma unroll
                 for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
             }
@@ -146,7 +148,7 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
         // 16x16 blocks (bi, bj), b0 <= bj <= bi <= 3, numbered row by row;
         // wave w takes blocks w, w + 4, w + 8
         const int b0 = s0 >> 4, nb = 4 - b0, nblk = nb * (nb + 1) / 2;
-        int bi[3], rb[3];
+        int bi[3], bj[3];
         double av[3][2], bv[3][2];
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
@@ -154,13 +156,14 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
             int i = 0;
             while ((i + 1) * (i + 2) / 2 <= t) ++i;
             bi[u] = b0 + i;
-            rb[u] = 16 * (b0 + (t - i * (i + 1) / 2)) + r16;
+            bj[u] = b0 + (t - i * (i + 1) / 2);
             if (t < nblk) {
-                const int ra = 16 * bi[u] + r16;
+                const int ra = 16 * bi[u] + r16, rb = 16 * bj[u] + r16;
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    av[u][h] = ra >= s0 ? A[ra][c0 + 4 * h + kq] : 0.0;
-                    bv[u][h] = rb[u] >= s0 ? A[rb[u]][c0 + 4 * h + kq] : 0.0;
+                    const double va = A[ra][c0 + 4 * h + kq], vb = A[rb][c0 + 4 * h + kq];
+                    av[u][h] = ra >= s0 ? va : 0.0;
+                    bv[u][h] = rb >= s0 ? vb : 0.0;
                 }
             }
         }
@@ -174,17 +177,16 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
                 if (wid + 4 * u < nblk) acc[u] = mfma16x16x4(av[u][h], bv[u][h], acc[u]);
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
-            if (wid + 4 * u >= nblk) continue;
-            double cur[4];
+            if (wid + 4 * u < nblk) {
+                const int col = 16 * bj[u] + r16;
+                double cur[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = 16 * bi[u] + 4 * v + kq, col = rb[u];
-                cur[v] = (col >= s0 && row >= col) ? A[row][col] : 0.0;
-            }
+                for (int v = 0; v < 4; ++v) cur[v] = A[16 * bi[u] + 4 * v + kq][col];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = 16 * bi[u] + 4 * v + kq, col = rb[u];
-                if (col >= s0 && row >= col) A[row][col] = cur[v] - acc[u][v];
+                for (int v = 0; v < 4; ++v) {
+                    const int row = 16 * bi[u] + 4 * v + kq;
+                    A[row][col] = (col >= s0 && row >= col) ? cur[v] - acc[u][v] : cur[v];
+                }
             }
         }
         __syncthreads();
@@ -210,8 +212,8 @@ __device__ __forceinline__ void wave_sync() {
 //     per wave).
 // Four barriers; the hand-offs inside one wave need none.
 __device__ __forceinline__ void tile_trinv(TileLds& A, const double* rpiv, TileLds& M) {
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int r16 = lane & 15, kq = lane >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wid = wave_id();
+    const int r16 = lane & 15, kq = lane >> 4, r8 = lane & 7;
     for (int idx = tid; idx < 4096; idx += 256) {
         const int i = idx >> 6, j = idx & 63;
         if ((j >> 3) > (i >> 3)) M[i][j] = 0.0;
@@ -234,12 +236,15 @@ __device__ __forceinline__ void tile_trinv(TileLds& A, const double* rpiv, TileL
             for (int i = 0; i < 8; ++i) M[b + i][b + j] = d[i];
         }
         wave_sync();
-        // T = L21 M11 (8x8): rows i = 4v + kq < 8, columns r16 < 8 of the 16x16 product
+        // T = L21 M11 (8x8): rows i = 4v + kq < 8, columns r16 < 8 of the 16x16
+        // product (operand rows and columns 8..15 zero; row 8 + r8 read instead)
         const bool in = r16 < 8;
         dbl4 t = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int m0 = 0; m0 < 8; m0 += 4)
-            t = mfma16x16x4(in ? A[R + r16][C + m0 + kq] : 0.0, in ? M[C + m0 + kq][C + r16] : 0.0, t);
+        for (int m0 = 0; m0 < 8; m0 += 4) {
+            const double va = A[R + r8][C + m0 + kq], vb = M[C + m0 + kq][C + r16];
+            t = mfma16x16x4(in ? va : 0.0, in ? vb : 0.0, t);
+        }
         if (in) {
             M[R + kq][C + r16] = t[0];
             M[R + 4 + kq][C + r16] = t[1];
@@ -247,8 +252,10 @@ __device__ __forceinline__ void tile_trinv(TileLds& A, const double* rpiv, TileL
         wave_sync();
         dbl4 u = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int m0 = 0; m0 < 8; m0 += 4)
-            u = mfma16x16x4(in ? M[R + r16][R + m0 + kq] : 0.0, in ? M[R + m0 + kq][C + r16] : 0.0, u);
+        for (int m0 = 0; m0 < 8; m0 += 4) {
+            const double va = M[R + r8][R + m0 + kq], vb = M[R + m0 + kq][C + r16];
+            u = mfma16x16x4(in ? va : 0.0, in ? vb : 0.0, u);
+        }
         wave_sync();
         if (in) {
             M[R + kq][C + r16] = -u[0];
@@ -299,7 +306,7 @@ __device__ __forceinline__ void tile_trinv(TileLds& A, const double* rpiv, TileL
 // triangular and the zero part of the sum is skipped.
 template <bool kTriY>
 __device__ __forceinline__ void tile_mfma_nt(const TileLds& X, const TileLds& Y, dbl4 acc[4]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
+    const int lane = threadIdx.x & 63, w = wave_id(), r16 = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[b] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -315,7 +322,7 @@ __device__ __forceinline__ void tile_mfma_nt(const TileLds& X, const TileLds& Y,
 __device__ __forceinline__ void tile_right_mul_t(TileLds& B, const TileLds& M) {
     dbl4 acc[4];
     tile_mfma_nt<true>(B, M, acc);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
+    const int lane = threadIdx.x & 63, w = wave_id(), r16 = lane & 15, kq = lane >> 4;
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < 4; ++b)
@@ -390,7 +397,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     const bool same = (i == j);
     // every global load of the launch in flight at once: M_k, A_ik, A_jk and
     // the tile A_ij to update (no other workgroup writes any of them)
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
+    const int lane = threadIdx.x & 63, w = wave_id(), r16 = lane & 15, kq = lane >> 4;
     double* Aij = S + (size_t)(64 * i) * lda + 64 * j;
     double vm[16], vi[16], vj[16], cur[4][4];
     tile_fetch(vm, Li + (size_t)4096 * k, 64);
