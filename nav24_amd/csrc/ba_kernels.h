@@ -20,6 +20,7 @@ constexpr int kTile = 256;         // observations per point-aligned linearisati
 constexpr int kChunk = 1024;       // camera-major observations per Hpp work item
 constexpr int kSchurChunk = 256;   // camera-major observations per Schur work item (default; tuned per problem)
 constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget)
+constexpr int kRunLen = 8;         // Schur pairs of one slot a lane sums in registers before one LDS add
 constexpr int kNB = 64;            // tiled dense Cholesky tile
 constexpr int kMaxWinCams = 8192;  // cameras per window: Schur slot map in LDS is int16
 constexpr int kCamLdsMax = 512;    // Schur rows keep the window's camera table in LDS up to this size
@@ -85,8 +86,9 @@ struct DevProblem {
     const int* row_sitem_ptr;  // [n_fc + 1]
     const int* row_nchunks;    // [n_fc]
     const int* sitem_order;    // [n_sitems] dispatch order: estimated work, heaviest first
-    const int* pair_seg;       // [n_sitems * schur_segs + 1] pair range of each (item, round, wave)
-    const int2* pairs;         // {observation b, owner lane | slot << 6 | local camera << 14}
+    const int* seg_batch;      // [n_sitems * schur_segs + 1] batch range of each (item, round, wave)
+    const int* batch_ptr;      // [batches + 1] pair range of each batch (64 runs, R rows: 64 R entries)
+    const int2* pairs;         // {observation b, owner lane | slot << 6 | local camera << 14}; {-1, 0} padding
     int schur_segs;
     int n_sitems;
     int max_ns;                // largest slot count of any Schur work item

@@ -319,18 +319,23 @@ struct SchurGraph {
 struct SchurItems {
     int chunk = 0, segs = 0, max_ns = 1;
     long long pofs = 0;
-    std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr, row_nch, pair_seg, sorder;
+    std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr, row_nch, seg_batch, batch_ptr, sorder;
     std::vector<long long> spofs;
     std::vector<int2> pairs;
 };
 
 // Items: per S row, column groups of kMaxSlots, chunks of `chunk` observations,
-// dispatched heaviest first (pairs + observations).  Pair lists: per item, per
-// (round, wave) segment, the pairs (a, b) of each lane's observation a with the
-// later observations b of its point on free cameras of the item's column group
-// (a itself excluded: the diagonal pair is summed in registers).  Entry:
-// {b, lane | slot << 6 | local camera << 14}.  Inside a segment the pairs are
-// dealt round-robin over the slots.  False when the pair count exceeds int32.
+// dispatched heaviest first (pair iterations + observations).  Pairs: per
+// item, per (round, wave) segment, the pairs (a, b) of each lane's
+// observation a with the later observations b of its point on free cameras of
+// the item's column group (a itself excluded: the diagonal pair is summed in
+// registers).  Entry: {b, lane | slot << 6 | local camera << 14}; {-1, 0} is
+// padding.  A segment's pairs are grouped by slot into runs of at most
+// kRunLen pairs (a slot's pairs split evenly), the runs sorted longest first
+// and packed 64 to a batch: pair j of the batch's run l at batch offset
+// 64 j + l, R = the batch's longest run rows.  A lane then works through one
+// run (one camera, one slot) and adds its 6x6 block to LDS once per batch.
+// False when the pair count exceeds int32.
 bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
     S = SchurItems{};
     S.chunk = chunk;
@@ -358,21 +363,23 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
     }
     const int rounds = (chunk + kBlock - 1) / kBlock;
     S.segs = rounds * (kBlock / kWave);
-    S.pair_seg.assign(1, 0);
+    S.seg_batch.assign(1, 0);
+    S.batch_ptr.assign(1, 0);
     std::vector<long long> scost(S.sf.size(), 0);
     std::vector<int> slotmap(G.nc, -1);
     std::vector<std::vector<int2>> bucket;
-    std::vector<int2> seg;
-    std::vector<size_t> cur;
+    struct Run { int slot, start, len; };
+    std::vector<Run> runs;
     for (size_t it = 0; it < S.sf.size(); ++it) {
         const int f = S.sf[it], gi = S.sg[it], ns = S.sns[it];
         const int col0 = G.colptr[f] + gi * kMaxSlots;
         const int w = G.cam_win[G.free_cam[f]];
         for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = sl;
-        long long npairs = 0;
+        if (bucket.size() < (size_t)ns) bucket.resize(ns);
+        long long iters = 0;
         for (int r = 0; r < rounds; ++r)
             for (int wv = 0; wv < kBlock / kWave; ++wv) {
-                seg.clear();
+                for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
                 const int kb = S.sa0[it] + r * kBlock + wv * kWave;
                 for (int k = kb; k < std::min(kb + kWave, S.sa1[it]); ++k) {
                     const int a = G.cam_obs[k];
@@ -382,24 +389,37 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
                         if (b == a) continue;
                         const int sl = slotmap[G.o_cam[b]];
                         if (sl < 0) continue;
-                        seg.push_back(int2{b, (k - kb) | (sl << 6) | ((G.o_cam[b] - G.cam_off[w]) << 14)});
+                        bucket[sl].push_back(int2{b, (k - kb) | (sl << 6) | ((G.o_cam[b] - G.cam_off[w]) << 14)});
                     }
                 }
-                if (bucket.size() < (size_t)ns) bucket.resize(ns);
-                for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
-                for (const int2& e : seg) bucket[(e.y >> 6) & 255].push_back(e);
-                size_t left = seg.size(), pos = 0;
-                cur.assign(ns, 0);
-                while (left > 0)
-                    for (int sl = 0; sl < ns; ++sl)
-                        if (cur[sl] < bucket[sl].size()) { seg[pos++] = bucket[sl][cur[sl]++]; --left; }
-                S.pairs.insert(S.pairs.end(), seg.begin(), seg.end());
-                if (S.pairs.size() > (size_t)0x7fffffff) return false;
-                S.pair_seg.push_back((int)S.pairs.size());
-                npairs += (long long)seg.size();
+                runs.clear();
+                for (int sl = 0; sl < ns; ++sl) {
+                    const int c = (int)bucket[sl].size();
+                    if (c == 0) continue;
+                    const int nr = (c + kRunLen - 1) / kRunLen, base = c / nr, extra = c % nr;
+                    for (int u = 0, st = 0; u < nr; ++u) {
+                        const int len = base + (u < extra ? 1 : 0);
+                        runs.push_back(Run{sl, st, len});
+                        st += len;
+                    }
+                }
+                std::stable_sort(runs.begin(), runs.end(), [](const Run& x, const Run& y) { return x.len > y.len; });
+                for (size_t r0 = 0; r0 < runs.size(); r0 += kWave) {
+                    const int R = runs[r0].len;
+                    const size_t base = S.pairs.size();
+                    if (base + (size_t)kWave * R > (size_t)0x7fffffff) return false;
+                    S.pairs.resize(base + (size_t)kWave * R, int2{-1, 0});
+                    for (size_t l = 0; l < (size_t)kWave && r0 + l < runs.size(); ++l) {
+                        const Run& rn = runs[r0 + l];
+                        for (int j = 0; j < rn.len; ++j) S.pairs[base + (size_t)kWave * j + l] = bucket[rn.slot][rn.start + j];
+                    }
+                    S.batch_ptr.push_back((int)S.pairs.size());
+                    iters += R;
+                }
+                S.seg_batch.push_back((int)S.batch_ptr.size() - 1);
             }
         for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = -1;
-        scost[it] = npairs + 4LL * (S.sa1[it] - S.sa0[it]);
+        scost[it] = (long long)kWave * iters + 4LL * (S.sa1[it] - S.sa0[it]);
     }
     S.sorder.resize(S.sf.size());
     for (size_t i = 0; i < S.sorder.size(); ++i) S.sorder[i] = (int)i;
@@ -416,7 +436,7 @@ int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
     UP(P.sitem_f, S.sf); UP(P.sitem_a0, S.sa0); UP(P.sitem_a1, S.sa1); UP(P.sitem_g, S.sg); UP(P.sitem_nslots, S.sns);
     UP(P.sitem_pofs, S.spofs); UP(P.row_sitem_ptr, S.row_sitem_ptr); UP(P.row_nchunks, S.row_nch);
     UP(P.sitem_order, S.sorder);
-    UP(P.pair_seg, S.pair_seg); UP(P.pairs, S.pairs);
+    UP(P.seg_batch, S.seg_batch); UP(P.batch_ptr, S.batch_ptr); UP(P.pairs, S.pairs);
 #undef UP
     if ((st = alloc(h, (void**)&h->W.spart, sizeof(double) * std::max<long long>(S.pofs, 1))) != NAV_BA_OK) return st;
     P.schur_segs = S.segs;

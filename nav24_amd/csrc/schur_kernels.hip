@@ -13,11 +13,13 @@
 //   phase B  the wave's other pairs (a, b), b a later observation of the same
 //            point on a free camera of this column group, come from a pair
 //            list built once at prepare time (observation b, owner lane,
-//            column slot, camera: 8 bytes, read coalesced).  Each lane forms
-//            W_a Hpl_b^T = (W_a J_X,b^T)(w_b J_xi,b) from recomputed Jacobians
-//            and adds it into the LDS slot of camera(b).  Inside each wave's
-//            list the pairs are dealt round-robin over the slots, so the 64
-//            lanes of one LDS atomic mostly hit distinct slots.
+//            column slot, camera: 8 bytes, read coalesced), packed as batches
+//            of 64 runs: each run holds up to 8 pairs of one slot.  A lane
+//            takes one run, reads its camera once, forms each
+//            W_a Hpl_b^T = (W_a J_X,b^T)(w_b J_xi,b) from recomputed
+//            Jacobians, sums the 6x6 blocks in registers and adds the sum to
+//            the slot in LDS once per batch (the lanes of one batch hold
+//            different runs, mostly of different slots).
 // Work items are dispatched heaviest first (a host-side permutation).
 // T is the type of the Jacobian entries only.  Cameras, points, W, the pair
 // products and the S-row slots are FP64 for both compute types: the point elimination
@@ -89,15 +91,14 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     double* camt = reinterpret_cast<double*>(smem + L.cam);
     double* wbuf_all = reinterpret_cast<double*>(smem + L.wbuf);
 
-    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+    const int tid = threadIdx.x, lane = tid % kWave;
+    const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);  // scalar: rounds and batches are wave-uniform
     for (int i = tid; i < ns * 36; i += kBlock) acc[i] = 0.0;
     if (kCamLds)
         for (int i = tid; i < ncw * 12; i += kBlock)
             camt[kCamStride * (i / 12) + i % 12] = reinterpret_cast<const double*>(cam + cam0)[i];
     __syncthreads();
 
-    double Rf[9], tf[3];
-    cam_get<kCamLds>(camt, cam, cf - cam0, cam0, Rf, tf);
     double* wbuf = wbuf_all + (size_t)wid * kWave * SchurLayout<T>::kWbuf;
 
     double coef[6], dsum[21];
@@ -114,6 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
         if (k < a1) {
             const int p = P.c_pt[k];
             if (P.pt_free[p]) {
+                double Rf[9], tf[3];  // camera f, reloaded per round (fewer live registers in phase B)
+                cam_get<kCamLds>(camt, cam, cf - cam0, cam0, Rf, tf);
                 const D4 xp = X[p];
                 EdgeLin<T> La;
                 const double sa = edge_linearize(Rf, tf, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
@@ -157,39 +160,48 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
             }
         }
         wave_fence();
-        // ---- phase B: this wave's pairs of this round, 64 at a time
+        // ---- phase B: this wave's pair batches of this round
         const int seg = it * P.schur_segs + round * (kBlock / kWave) + wid;
-        const int q0 = P.pair_seg[seg], q1 = P.pair_seg[seg + 1];
-        for (int qb = q0; qb < q1; qb += kWave) {
-            const int q = qb + lane;
-            if (q >= q1) continue;
-            const int2 pr = P.pairs[q];
-            const int b = pr.x;
-            const int i = pr.y & 63, s = (pr.y >> 6) & 255, cb = pr.y >> 14;
-            const double* wb = wbuf + i * SchurLayout<T>::kWbuf;
+        const int bt1 = P.seg_batch[seg + 1];
+        for (int bt = P.seg_batch[seg]; bt < bt1; ++bt) {
+            const int e0 = P.batch_ptr[bt], nrow = (P.batch_ptr[bt + 1] - e0) / kWave;
+            const int2 p0 = P.pairs[e0 + lane];
+            const bool has = p0.x >= 0;
+            const int s = (p0.y >> 6) & 255, cb = has ? p0.y >> 14 : 0;
             double Rb[9], tb[3];
             cam_get<kCamLds>(camt, cam, cb, cam0, Rb, tb);
-            EdgeLin<T> Lb;
-            edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
-            const double wgt = (double)w[b];
-            double JXb[6];
+            double sacc[36];
 #pragma unroll
-            for (int j = 0; j < 6; ++j) JXb[j] = (double)Lb.JX[j];
-            double M0[6], M1[6];
+            for (int e = 0; e < 36; ++e) sacc[e] = 0.0;
+            for (int j = 0; j < nrow; ++j) {
+                const int2 pr = j == 0 ? p0 : P.pairs[e0 + j * kWave + lane];
+                if (pr.x < 0) continue;
+                const double* wb = wbuf + (pr.y & 63) * SchurLayout<T>::kWbuf;
+                EdgeLin<T> Lb;
+                edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
+                const double wgt = (double)w[pr.x];
+                double JXb[6];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                const double w0 = wb[3 * r], w1 = wb[3 * r + 1], w2 = wb[3 * r + 2];
-                M0[r] = w0 * JXb[0] + w1 * JXb[1] + w2 * JXb[2];
-                M1[r] = w0 * JXb[3] + w1 * JXb[4] + w2 * JXb[5];
+                for (int q = 0; q < 6; ++q) JXb[q] = (double)Lb.JX[q];
+                double M0[6], M1[6];
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    const double w0 = wb[3 * r], w1 = wb[3 * r + 1], w2 = wb[3 * r + 2];
+                    M0[r] = w0 * JXb[0] + w1 * JXb[1] + w2 * JXb[2];
+                    M1[r] = w0 * JXb[3] + w1 * JXb[4] + w2 * JXb[5];
+                }
+                double B0[6], B1[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) { B0[c] = wgt * (double)Lb.Jxi[c]; B1[c] = wgt * (double)Lb.Jxi[6 + c]; }
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) sacc[6 * r + c] += M0[r] * B0[c] + M1[r] * B1[c];
             }
-            double B0[6], B1[6];
+            if (has) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) { B0[c] = wgt * (double)Lb.Jxi[c]; B1[c] = wgt * (double)Lb.Jxi[6 + c]; }
-#pragma unroll
-            for (int r = 0; r < 6; ++r)
-#pragma unroll
-                for (int c = 0; c < 6; ++c)
-                    atomicAdd(acc + (6 * r + c) * ns + s, M0[r] * B0[c] + M1[r] * B1[c]);
+                for (int e = 0; e < 36; ++e) atomicAdd(acc + e * ns + s, sacc[e]);
+            }
         }
     }
     // ---- diagonal block (slot 0 of group 0): wave sums, one LDS add per wave;
