@@ -332,7 +332,7 @@ struct SchurItems {
 // registers).  Entry: {b, lane | slot << 6 | local camera << 14}; {-1, 0} is
 // padding.  A segment's pairs are grouped by slot into runs of at most
 // kRunLen pairs (a slot's pairs split evenly), the runs sorted longest first
-// and packed 64 to a batch: pair j of the batch's run l at batch offset
+// (the runs of one slot spread apart) and packed 64 to a batch: pair j of the batch's run l at batch offset
 // 64 j + l, R = the batch's longest run rows.  A lane then works through one
 // run (one camera, one slot) and adds its 6x6 block to LDS once per batch.
 // False when the pair count exceeds int32.
@@ -368,7 +368,7 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
     std::vector<long long> scost(S.sf.size(), 0);
     std::vector<int> slotmap(G.nc, -1);
     std::vector<std::vector<int2>> bucket;
-    struct Run { int slot, start, len; };
+    struct Run { int slot, start, len, u; };
     std::vector<Run> runs;
     for (size_t it = 0; it < S.sf.size(); ++it) {
         const int f = S.sf[it], gi = S.sg[it], ns = S.sns[it];
@@ -399,11 +399,16 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
                     const int nr = (c + kRunLen - 1) / kRunLen, base = c / nr, extra = c % nr;
                     for (int u = 0, st = 0; u < nr; ++u) {
                         const int len = base + (u < extra ? 1 : 0);
-                        runs.push_back(Run{sl, st, len});
+                        runs.push_back(Run{sl, st, len, u});
                         st += len;
                     }
                 }
-                std::stable_sort(runs.begin(), runs.end(), [](const Run& x, const Run& y) { return x.len > y.len; });
+                // longest first; among equal lengths the slots' first runs, then
+                // their second runs, ...: lanes next to each other hold
+                // different slots, so their LDS adds hit different addresses
+                std::stable_sort(runs.begin(), runs.end(), [](const Run& x, const Run& y) {
+                    return x.len != y.len ? x.len > y.len : x.u < y.u;
+                });
                 for (size_t r0 = 0; r0 < runs.size(); r0 += kWave) {
                     const int R = runs[r0].len;
                     const size_t base = S.pairs.size();
