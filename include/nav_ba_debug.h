@@ -60,7 +60,7 @@ int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode);
 
 /* Observations per Schur work item for the next nav_ba_prepare on this
  * handle: 0 = automatic (problems of at least 131072 observations time the
- * candidates 128, 256, 512, 1024 and 2048 on their initial linearisation at prepare
+ * candidates 128, 256, ..., 4096 on their initial linearisation at prepare
  * and keep the fastest, remembered per device, precision and problem shape for
  * the life of the process; smaller problems use 256), or a multiple of 64 up
  * to 4096 to force it.  Any size gives the same reduced system up to the
@@ -69,6 +69,11 @@ int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk);
 /* The chunk size the last nav_ba_prepare used (0 before a prepare, or when
  * the problem had nothing to optimise). */
 int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk);
+
+/* Schur pairs of one column slot that a lane sums in registers before one
+ * LDS add, for the next nav_ba_prepare: 0 = default (4), or 1..64.  Any value
+ * gives the same reduced system up to the order of FP64 additions. */
+int32_t nav_ba_debug_set_schur_run(nav_ba_handle* h, int32_t run);
 
 #ifdef __cplusplus
 }

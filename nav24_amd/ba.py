@@ -27,7 +27,7 @@ EXPORTED = [
     "nav_ba_default_options", "nav_ba_solve", "nav_ba_prepare", "nav_ba_run", "nav_ba_fetch",
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
-    "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk",
+    "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_set_schur_run",
 ]
 
 STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
         L.nav_ba_debug_set_schur_chunk.restype = C.c_int32
         L.nav_ba_debug_get_schur_chunk.argtypes = [H, C.POINTER(C.c_int32)]
         L.nav_ba_debug_get_schur_chunk.restype = C.c_int32
+        L.nav_ba_debug_set_schur_run.argtypes = [H, C.c_int32]
+        L.nav_ba_debug_set_schur_run.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -206,6 +208,10 @@ class GraphOptimHip:
         """Observations per Schur work item for the next prepare: 0 automatic
         (tuned at prepare on large problems), or a multiple of 64 up to 4096."""
         self._check(lib().nav_ba_debug_set_schur_chunk(self._h, int(chunk)))
+
+    def set_schur_run(self, run: int) -> None:
+        """Schur pairs per register run for the next prepare: 0 default (4), or 1..64."""
+        self._check(lib().nav_ba_debug_set_schur_run(self._h, int(run)))
 
     @property
     def schur_chunk(self) -> int:

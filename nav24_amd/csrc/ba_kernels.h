@@ -20,7 +20,7 @@ constexpr int kTile = 256;         // observations per point-aligned linearisati
 constexpr int kChunk = 1024;       // camera-major observations per Hpp work item
 constexpr int kSchurChunk = 256;   // camera-major observations per Schur work item (default; tuned per problem)
 constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget)
-constexpr int kRunLen = 8;         // Schur pairs of one slot a lane sums in registers before one LDS add
+constexpr int kRunLen = 4;         // Schur pairs of one slot a lane sums in registers before one LDS add
 constexpr int kNB = 64;            // tiled dense Cholesky tile
 constexpr int kMaxWinCams = 8192;  // cameras per window: Schur slot map in LDS is int16
 constexpr int kCamLdsMax = 512;    // Schur rows keep the window's camera table in LDS up to this size

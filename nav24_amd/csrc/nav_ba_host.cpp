@@ -38,7 +38,7 @@ constexpr int kMaxDenseN = 16384;  // tiled dense reduced system of one window (
 // candidate chunk size once at prepare (best of kTuneReps launches) and keep a
 // candidate only if it beats the current best by the margin.
 constexpr int kTuneMinObs = 1 << 17;
-constexpr int kTuneChunks[] = {128, 256, 512, 1024, 2048};
+constexpr int kTuneChunks[] = {128, 256, 512, 1024, 2048, 4096};
 constexpr int kTuneReps = 3;
 constexpr float kTuneMargin = 0.97f;
 
@@ -118,6 +118,7 @@ struct nav_ba_handle {
     int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks
     int schur_chunk = 0;  // observations per Schur work item: 0 = default or tuned
     int schur_chunk_used = 0;
+    int schur_run = kRunLen;  // Schur pairs per register run
     StageTimer timer;
     LMDev* hstatus = nullptr;  // pinned: LM state of window 0
     int* hdone = nullptr;      // pinned: windows finished
@@ -331,12 +332,12 @@ struct SchurItems {
 // the item's column group (a itself excluded: the diagonal pair is summed in
 // registers).  Entry: {b, lane | slot << 6 | local camera << 14}; {-1, 0} is
 // padding.  A segment's pairs are grouped by slot into runs of at most
-// kRunLen pairs (a slot's pairs split evenly), the runs sorted longest first
+// `run` pairs (a slot's pairs split evenly), the runs sorted longest first
 // (the runs of one slot spread apart) and packed 64 to a batch: pair j of the batch's run l at batch offset
 // 64 j + l, R = the batch's longest run rows.  A lane then works through one
 // run (one camera, one slot) and adds its 6x6 block to LDS once per batch.
 // False when the pair count exceeds int32.
-bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
+bool build_schur_items(const SchurGraph& G, int chunk, int run, SchurItems& S) {
     S = SchurItems{};
     S.chunk = chunk;
     S.row_sitem_ptr.push_back(0);
@@ -396,7 +397,7 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
                 for (int sl = 0; sl < ns; ++sl) {
                     const int c = (int)bucket[sl].size();
                     if (c == 0) continue;
-                    const int nr = (c + kRunLen - 1) / kRunLen, base = c / nr, extra = c % nr;
+                    const int nr = (c + run - 1) / run, base = c / nr, extra = c % nr;
                     for (int u = 0, st = 0; u < nr; ++u) {
                         const int len = base + (u < extra ? 1 : 0);
                         runs.push_back(Run{sl, st, len, u});
@@ -453,9 +454,10 @@ int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
 
 // Process-wide memory of tuned chunk sizes, per device, precision and problem shape.
 struct TuneKey {
-    int device, mixed, n_win, nc, np, no, nfc;
+    int device, mixed, run, n_win, nc, np, no, nfc;
     bool operator<(const TuneKey& o) const {
-        return std::tie(device, mixed, n_win, nc, np, no, nfc) < std::tie(o.device, o.mixed, o.n_win, o.nc, o.np, o.no, o.nfc);
+        return std::tie(device, mixed, run, n_win, nc, np, no, nfc) <
+               std::tie(o.device, o.mixed, o.run, o.n_win, o.nc, o.np, o.no, o.nfc);
     }
 };
 std::mutex g_tune_mu;
@@ -517,7 +519,7 @@ int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems
     SchurItems C;
     for (int chunk : kTuneChunks) {
         if (chunk == def) continue;
-        if (!build_schur_items(G, chunk, C)) continue;
+        if (!build_schur_items(G, chunk, h->schur_run, C)) continue;
         h->nbuf = mark;
         if ((st = upload_schur_items(h, C)) != NAV_BA_OK) return st;
         float ms;
@@ -755,12 +757,12 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     const SchurGraph G{nfc, nc, colptr.data(), colcam.data(), cam_ptr.data(), cam_obs.data(), cam_obs_b0.data(),
                        o_pt.data(), o_cam.data(), pt_ptr.data(), pt_free.data(), free_cam.data(), cam_win.data(),
                        cam_off.data()};
-    const TuneKey key{h->device, mixed(h) ? 1 : 0, n_win, nc, np, no, nfc};
+    const TuneKey key{h->device, mixed(h) ? 1 : 0, h->schur_run, n_win, nc, np, no, nfc};
     int chunk0 = h->schur_chunk > 0 ? h->schur_chunk : kSchurChunk;
     bool tune = h->schur_chunk == 0 && nfc > 0 && no >= kTuneMinObs;
     if (tune && cached_chunk(key, &chunk0)) tune = false;  // this shape was tuned before on this device
     SchurItems SI;
-    if (!build_schur_items(G, chunk0, SI)) return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pairs");
+    if (!build_schur_items(G, chunk0, h->schur_run, SI)) return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pairs");
 
     // ---- dense reduced systems: one padded tiled system, or one small system per window
     std::vector<long long> win_S(n_win, 0);
@@ -1134,6 +1136,12 @@ int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode) {
 int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk) {
     if (!h || chunk < 0 || chunk > 4096 || chunk % kWave != 0) return NAV_BA_BAD_ARG;
     h->schur_chunk = chunk;
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_set_schur_run(nav_ba_handle* h, int32_t run) {
+    if (!h || run < 0 || run > 64) return NAV_BA_BAD_ARG;
+    h->schur_run = run == 0 ? kRunLen : run;
     return NAV_BA_OK;
 }
 

@@ -14,7 +14,7 @@
 //            point on a free camera of this column group, come from a pair
 //            list built once at prepare time (observation b, owner lane,
 //            column slot, camera: 8 bytes, read coalesced), packed as batches
-//            of 64 runs: each run holds up to 8 pairs of one slot.  A lane
+//            of 64 runs: each run holds a few pairs of one slot.  A lane
 //            takes one run, reads its camera once, forms each
 //            W_a Hpl_b^T = (W_a J_X,b^T)(w_b J_xi,b) from recomputed
 //            Jacobians, sums the 6x6 blocks in registers and adds the sum to

@@ -1,4 +1,4 @@
-"""Schur work-item size: forced sizes and the prepare-time tuner against the oracle.
+"""Schur work items: forced sizes, run lengths and the prepare-time tuner against the oracle.
 
 The chunk size only changes how observations of one camera row are split over
 workgroups (and so the order of FP64 additions), never what is summed: the
@@ -33,6 +33,24 @@ def test_reduced_system_any_chunk(solver, oracle_mod, chunk):
     assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
 
 
+@pytest.mark.parametrize("run", [1, 3, 16])
+def test_reduced_system_any_run_length(solver, oracle_mod, run):
+    """Pairs of one slot summed in registers in runs of 1, 3 or 16 before the
+    LDS add: the same reduced system up to the order of additions."""
+    _say(f"reduced system workload1 n_pt=3000 schur run length {run}")
+    prob = problems.workload(1, seed=6, n_pt=3000)
+    opts = workload_options(1)
+    solver.set_schur_run(run)
+    try:
+        solver.prepare(prob, opts)
+        S, g = solver.debug_reduced_system(1e-4)
+    finally:
+        solver.set_schur_run(0)
+    So, go = oracle_mod.reduced_system(prob, opts, 1e-4)
+    assert np.abs(S - So).max() <= 1e-11 * np.abs(So).max()
+    assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
+
+
 def test_tuned_chunk_solve_matches_oracle(solver, oracle_mod):
     prob = problems.workload(1, seed=5)
     assert prob.n_obs >= 1 << 17, "the tuner runs from 131072 observations on"
@@ -44,7 +62,7 @@ def test_tuned_chunk_solve_matches_oracle(solver, oracle_mod):
     got = solver.solve(prob, opts)
     chunk = solver.schur_chunk
     _say(f"  tuned chunk {chunk}")
-    assert chunk in (128, 256, 512, 1024, 2048)
+    assert chunk in (128, 256, 512, 1024, 2048, 4096)
     ref = oracle_mod.solve(prob, opts)
     check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
     # a second prepare of the same shape reuses the tuned size
@@ -57,3 +75,6 @@ def test_chunk_argument_checked(solver):
     for bad in (-64, 100, 8192):
         with pytest.raises(BAError):
             solver.set_schur_chunk(bad)
+    for bad in (-1, 65):
+        with pytest.raises(BAError):
+            solver.set_schur_run(bad)
