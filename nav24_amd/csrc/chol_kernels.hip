@@ -78,17 +78,24 @@ __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
 // branches and block indices stay on the scalar unit).
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
+// Orders the LDS accesses of one wave around a hand-off between its lanes
+// (the wave's LDS instructions execute in program order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // In-LDS Cholesky of the SPD tile A (64x64, 256 threads): its lower triangle
-// becomes L and rpiv[j] = 1 / L_jj; the upper triangle is not read.  Eight
-// 8-column panels, two barriers each:
-//   wave 0: every lane factors the panel's 8x8 diagonal block in registers
-//     (the same values in all lanes, from broadcast LDS reads; 1/sqrt by rsq +
-//     two Newton steps), then lane l solves panel row c0 + 8 + l against it;
-//   all waves: trailing lower triangle -= panel panel^T on FP64 MFMA, 16x16
-//     blocks (K = 8), a wave's blocks loaded and multiplied together.  Entries
-//     of a block outside the trailing lower triangle get their own value
-//     back (no other wave writes them, and the panel entries among them are
-//     final), so no access is predicated.
+// becomes L and rpiv[j] = 1 / L_jj; the upper triangle holds no data.  Eight
+// 8-column panels with one step of look-ahead, one barrier each.  In step p:
+//   wave 0: lane l holds row c0 + l of panel p (c0 = 8p) and applies panel
+//     p-1 to it (8x8 product with that panel's rows c0..c0+7); the updated
+//     diagonal block goes through LDS to all lanes, which factor it in
+//     registers (the same values in every lane; 1/sqrt by rsq + two Newton
+//     steps); lanes 8.. then solve their rows against it;
+//   waves 1-3 meanwhile apply panel p-1 to the trailing lower triangle right
+//     of panel p on FP64 MFMA (16x16 blocks, K = 8, predicated writes: wave 0
+//     is writing panel p's columns in the same step).
 // Returns true in thread 0 if a pivot was not positive and finite (it is then
 // replaced by 1 so that the values stay finite).
 __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
@@ -96,16 +103,37 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
     const int r16 = lane & 15, kq = lane >> 4;
     bool bad = false;
     for (int p = 0; p < 8; ++p) {
-        const int c0 = 8 * p, s0 = c0 + 8;  // s0: first row and column of the trailing part
+        const int c0 = 8 * p, s0 = c0 + 8;  // s0: first row and column right of / below panel p
         if (wid == 0) {
-            double a[8][8], rp[8], x[8];
+            const int row = min(c0 + lane, 63);
+            double x[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
+            if (p > 0) {
+                double pv[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) pv[m] = A[row][c0 - 8 + m];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 8; m += 2) {
+                        t0 += pv[m] * A[c0 + c][c0 - 8 + m];
+                        t1 += pv[m + 1] * A[c0 + c][c0 - 7 + m];
+                    }
+                    x[c] -= t0 + t1;
+                }
+                if (lane < 8) {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
+                }
+                wave_sync();
+            }
+            double a[8][8], rp[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i)
 #pragma unroll
                 for (int j = 0; j <= i; ++j) a[i][j] = A[c0 + i][c0 + j];
-            const int row = min(s0 + lane, 63);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 double d = a[j][j];
@@ -124,12 +152,7 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
 #pragma unroll
                     for (int c = j + 1; c <= i; ++c) a[i][c] -= a[i][j] * a[c][j];
             }
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                x[c] *= rp[c];
-#pragma unroll
-                for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * a[e][c];
-            }
+            wave_sync();  // every lane has read the diagonal block before lane 0 overwrites it
             if (lane == 0) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
@@ -138,54 +161,59 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
                     for (int j = 0; j <= i; ++j) A[c0 + i][c0 + j] = a[i][j];
                 }
             }
-            if (s0 + lane < 64) {
+            if (lane >= 8 && c0 + lane < 64) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    x[c] *= rp[c];
+#pragma unroll
+                    for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * a[e][c];
+                }
 #pragma unroll
                 for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
             }
-        }
-        __syncthreads();
-        if (p == 7) break;
-        // 16x16 blocks (bi, bj), b0 <= bj <= bi <= 3, numbered row by row;
-        // wave w takes blocks w, w + 4, w + 8
-        const int b0 = s0 >> 4, nb = 4 - b0, nblk = nb * (nb + 1) / 2;
-        int bi[3], bj[3];
-        double av[3][2], bv[3][2];
+        } else if (p > 0) {
+            // 16x16 blocks (bi, bj), b0 <= bj <= bi <= 3, numbered row by row;
+            // wave w (1..3) takes blocks w - 1 and w + 2
+            const int q0 = c0 - 8, b0 = s0 >> 4, nb = 4 - b0, nblk = nb * (nb + 1) / 2;
+            int bi[2], bj[2];
+            double av[2][2], bv[2][2];
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            const int t = wid + 4 * u;
-            int i = 0;
-            while ((i + 1) * (i + 2) / 2 <= t) ++i;
-            bi[u] = b0 + i;
-            bj[u] = b0 + (t - i * (i + 1) / 2);
-            if (t < nblk) {
-                const int ra = 16 * bi[u] + r16, rb = 16 * bj[u] + r16;
+            for (int u = 0; u < 2; ++u) {
+                const int t = wid - 1 + 3 * u;
+                int i = 0;
+                while ((i + 1) * (i + 2) / 2 <= t) ++i;
+                bi[u] = b0 + i;
+                bj[u] = b0 + (t - i * (i + 1) / 2);
+                if (t < nblk) {
+                    const int ra = 16 * bi[u] + r16, rb = 16 * bj[u] + r16;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const double va = A[ra][c0 + 4 * h + kq], vb = A[rb][c0 + 4 * h + kq];
-                    av[u][h] = ra >= s0 ? va : 0.0;
-                    bv[u][h] = rb >= s0 ? vb : 0.0;
+                    for (int h = 0; h < 2; ++h) {
+                        const double va = A[ra][q0 + 4 * h + kq], vb = A[rb][q0 + 4 * h + kq];
+                        av[u][h] = ra >= s0 ? va : 0.0;
+                        bv[u][h] = rb >= s0 ? vb : 0.0;
+                    }
                 }
             }
-        }
-        dbl4 acc[3];
+            dbl4 acc[2];
 #pragma unroll
-        for (int u = 0; u < 3; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+            for (int u = 0; u < 2; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int u = 0; u < 3; ++u)
-                if (wid + 4 * u < nblk) acc[u] = mfma16x16x4(av[u][h], bv[u][h], acc[u]);
+                for (int u = 0; u < 2; ++u)
+                    if (wid - 1 + 3 * u < nblk) acc[u] = mfma16x16x4(av[u][h], bv[u][h], acc[u]);
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            if (wid + 4 * u < nblk) {
-                const int col = 16 * bj[u] + r16;
-                double cur[4];
+            for (int u = 0; u < 2; ++u) {
+                if (wid - 1 + 3 * u < nblk) {
+                    const int col = 16 * bj[u] + r16;
+                    double cur[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) cur[v] = A[16 * bi[u] + 4 * v + kq][col];
+                    for (int v = 0; v < 4; ++v) cur[v] = A[16 * bi[u] + 4 * v + kq][col];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = 16 * bi[u] + 4 * v + kq;
-                    A[row][col] = (col >= s0 && row >= col) ? cur[v] - acc[u][v] : cur[v];
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = 16 * bi[u] + 4 * v + kq;
+                        if (col >= s0 && row >= col) A[row][col] = cur[v] - acc[u][v];
+                    }
                 }
             }
         }
@@ -194,12 +222,6 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
     return bad;
 }
 
-// Orders the LDS accesses of one wave around a hand-off between its lanes
-// (the wave's LDS instructions execute in program order).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // M = L^-1 for the factor in the lower triangle of A, rpiv = 1 / diag(L), by
 // recursive doubling; M's upper triangle is set to zero.  The strict upper
