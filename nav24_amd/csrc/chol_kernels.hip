@@ -395,6 +395,7 @@ __global__ __launch_bounds__(256) void k_chol_first(const double* __restrict__ S
     __shared__ double rv[64];
     __shared__ double rpiv[64];
     tile_load(A, S, lda);
+    if (lm->phase == kPhaseDone) return;  // a step launched after the solve has finished
     __syncthreads();
     if (tile_potrf(A, rpiv) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
     tile_trinv(A, rpiv, M);
@@ -429,6 +430,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     for (int b = 0; b < 4; ++b)
 #pragma unroll
         for (int v = 0; v < 4; ++v) cur[b][v] = Aij[(size_t)(16 * w + 4 * v + kq) * lda + 16 * b + r16];
+    if (lm->phase == kPhaseDone) return;  // a step launched after the solve has finished (loads in flight meanwhile)
     tile_put(Mk, vm);
     tile_put(Bi, vi);
     if (!same) tile_put(Bj, vj);
@@ -480,6 +482,7 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(const double* __rest
     __shared__ double red[4][128];
     __shared__ double xk[64];
     const int tid = threadIdx.x, n = 64 * nt;
+    if (lm->phase == kPhaseDone) return;
     if (lm->fail) {
         for (int i = tid; i < n; i += kBackThreads) dp[i] = 0.0;
         return;
@@ -805,6 +808,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_blocks(const double* __r
     extern __shared__ double sm[];
     __shared__ double xk[6];
     __shared__ int bad;
+    if (lm->phase == kPhaseDone) return;
     const int n = 6 * nb;
     const int nblk = nb * (nb + 1) / 2;
     double* Pn = sm;                                      // (nb - 1) * kBS: panel column L_IK, I > K

@@ -120,7 +120,6 @@ struct nav_ba_handle {
     int schur_chunk_used = 0;
     int schur_run = kRunLen;  // Schur pairs per register run
     StageTimer timer;
-    LMDev* hstatus = nullptr;  // pinned: LM state of window 0
     int* hdone = nullptr;      // pinned: windows finished
     // results of the last run
     bool ran = false;
@@ -261,8 +260,6 @@ void enqueue_step(nav_ba_handle* h) {
         launch_lm_accept(h->P, h->W, h->stream);
         launch_commit(h->P, h->cur, h->cand, h->W, h->stream);
     });
-    (void)hipMemcpyAsync(h->hstatus, h->W.lm, sizeof(LMDev), hipMemcpyDeviceToHost, h->stream);
-    (void)hipMemcpyAsync(h->hdone, h->W.done_count, sizeof(int), hipMemcpyDeviceToHost, h->stream);
 }
 
 int launch_step(nav_ba_handle* h) {
@@ -991,7 +988,6 @@ int32_t nav_ba_create(nav_ba_handle** out, int32_t device) {
     h->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
-        hipHostMalloc((void**)&h->hstatus, sizeof(LMDev), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h->hdone, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         nav_ba_destroy(h);
         return NAV_BA_HIP_ERROR;
@@ -1007,7 +1003,6 @@ void nav_ba_destroy(nav_ba_handle* h) {
     if (h->step) (void)hipGraphExecDestroy(h->step);
     for (auto& b : h->bufs)
         if (b.p) (void)hipFree(b.p);
-    if (h->hstatus) (void)hipHostFree(h->hstatus);
     if (h->hdone) (void)hipHostFree(h->hdone);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1046,26 +1041,21 @@ int32_t nav_ba_run(nav_ba_handle* h) {
     launch_cam_rt(P, h->cur, s);
     if (m) launch_cost<float>(P, h->cur, h->W, 0, s); else launch_cost<double>(P, h->cur, h->W, 0, s);
     launch_lm_set_chi(P, h->W, s);
-    // every window needs at most n_iter * max_trials steps; the host stops when all are done
+    // every window needs at most n_iter * max_trials steps.  Steps are
+    // launched in bursts (n_iter first: one per iteration when every trial is
+    // accepted, then 2), the host checks the count of finished windows after
+    // each burst; steps past a window's end return at once from every kernel.
     const long long max_steps = (long long)P.n_iter * P.max_trials;
     long long steps = 0;
     int st;
-    if (!h->batched) {
-        while (steps < max_steps) {
+    long long burst = std::max(1, P.n_iter);
+    while (steps < max_steps) {
+        for (long long i = 0; i < burst && steps < max_steps; ++i, ++steps)
             if ((st = launch_step(h)) != NAV_BA_OK) return st;
-            HIPTRY(h, hipStreamSynchronize(s));
-            ++steps;
-            if (h->hstatus->phase == kPhaseDone) break;
-        }
-    } else {
-        long long burst = std::max(1, P.n_iter);
-        while (steps < max_steps) {
-            for (long long i = 0; i < burst && steps < max_steps; ++i, ++steps)
-                if ((st = launch_step(h)) != NAV_BA_OK) return st;
-            HIPTRY(h, hipStreamSynchronize(s));
-            if (*h->hdone >= P.n_win) break;
-            burst = 2;
-        }
+        HIPTRY(h, hipMemcpyAsync(h->hdone, h->W.done_count, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPTRY(h, hipStreamSynchronize(s));
+        if (*h->hdone >= P.n_win) break;
+        burst = 2;
     }
     // outlier flags and the FP64 cost at the final state
     h->timing_active = h->timing;
