@@ -101,6 +101,23 @@ def roofline_entry(timing, st, fp64):
     }
 
 
+def hbm_stage_rates(timing, st, fp64=True):
+    """Achieved HBM rate of the residual / Jacobian / accumulation stages on
+    their algorithmic bytes (stage-timed launches; the lin_cam stage includes
+    its per-camera chunk reduction)."""
+    work = stage_work(st, fp64)
+    out = {}
+    for k in ("lin_pt", "lin_cam", "cost"):
+        ms, n = timing.get(k, (0.0, 0))
+        if n <= 0:
+            continue
+        us = ms / n * 1e3
+        gbs = work[k][1] / (us * 1e-6) / 1e9
+        out[k] = {"avg_launch_us": round(us, 2), "algorithmic_bytes": int(work[k][1]), "achieved_GBs": round(gbs, 1),
+                  "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def cpu_baseline(prob, opts):
     from oracle import oracle  # checker and CPU baseline only
     t0 = time.perf_counter()
@@ -253,6 +270,15 @@ def loop_leg(steps, warmup):
     s.close()
     m, f = res["mixed"], res["fp64"]
     out["fp64"]["one_ulp_point_change_rel"] = abs(rq.final_cost - f.final_cost) / f.final_cost
+    # stage-timed FP64 solve: HBM rate of the residual / Jacobian / accumulation kernels at 3 M observations
+    o = workload_options(2)
+    o.precision = NAV_BA_FP64
+    s = GraphOptimHip(0)
+    s.prepare(prob, o)
+    s.set_timing(True)
+    s.run()
+    out["fp64"]["hbm_kernels"] = hbm_stage_rates(s.timing(), problem_stats(prob), True)
+    s.close()
     rot, trans, pts = _pose_point_rmse(m, f)
     out["mixed"]["final_cost_rel_vs_fp64"] = abs(m.final_cost - f.final_cost) / f.final_cost
     out["mixed"]["rmse_vs_fp64"] = {"rotation_rad": rot, "translation_m": trans, "point_m": pts}
@@ -414,6 +440,7 @@ def main():
     roof["achieved"] = round(roof["achieved"], 3)
     roof["frac"] = round(roof["frac"], 5)
     roof["traffic"] = measured_traffic(WORKLOAD_NAMES.get(args.workload, ""), roof["kernel"])
+    roof["hbm_kernels"] = hbm_stage_rates(per_step, st, True)
     # the factorisation stage (latency-bound chain of tile launches), priced on FP64 flops
     n = 6 * st["n_fc"]
     trials = res.trials

@@ -155,13 +155,25 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
     double acc[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) acc[k] = 0.0;
-    for (int k = a0 + (int)threadIdx.x; k < a1; k += kBlock) {
-        // camera-major copies of the point index and measurement: contiguous reads;
-        // the Huber weight is recomputed from the residual (the same arithmetic as k_lin_pt)
-        const D4 xp = X[P.c_pt[k]];
+    // camera-major copies of the point index and measurement: contiguous
+    // reads, all issued before the first point gather; the Huber weight is
+    // recomputed from the residual (the same arithmetic as k_lin_pt)
+    constexpr int kU = kChunk / kBlock;
+    int pk[kU];
+    double uk[kU], vk[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const int k = min(a0 + (int)threadIdx.x + kBlock * u, a1 - 1);  // a1 > a0: items are never empty
+        pk[u] = P.c_pt[k];
+        uk[u] = P.c_uv[2 * (size_t)k];
+        vk[u] = P.c_uv[2 * (size_t)k + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        if (a0 + (int)threadIdx.x + kBlock * u >= a1) break;
+        const D4 xp = X[pk[u]];
         EdgeLin<T> L;
-        const double s = edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
-                                        P.c_uv[2 * (size_t)k], P.c_uv[2 * (size_t)k + 1], L);
+        const double s = edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, uk[u], vk[u], L);
         const double wa = (double)huber_weight<T>(s, P);
         const double r0 = -L.e0 * wa, r1 = -L.e1 * wa;
         double J[12];
@@ -553,15 +565,31 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restr
         if (threadIdx.x == 0) red_cost[b] = 0.0;
         return;
     }
+    // kCostObs / 256 observations per thread; every index and measurement
+    // load of the block is issued before the first gather (the gathers of one
+    // observation depend on its indices)
+    constexpr int kU = kCostObs / 256;
+    const int a0 = P.cblk_o0[b], a1 = P.cblk_o0[b + 1];
+    int ci[kU], pi[kU];
+    double uu[kU], vv[kU];
+#pragma unroll
+    for (int q = 0; q < kU; ++q) {
+        const int a = min(a0 + (int)threadIdx.x + 256 * q, a1 - 1);  // a1 > a0: blocks are never empty
+        ci[q] = P.o_cam[a];
+        pi[q] = P.o_pt[a];
+        uu[q] = P.o_uv[2 * (size_t)a];
+        vv[q] = P.o_uv[2 * (size_t)a + 1];
+    }
     double acc = 0.0;
-    for (int a = P.cblk_o0[b] + threadIdx.x; a < P.cblk_o0[b + 1]; a += 256) {
+#pragma unroll
+    for (int q = 0; q < kU; ++q) {
         double R[9], tt[3];
-        load_cam(cam, P.o_cam[a], R, tt);
-        const D4 xp = X[P.o_pt[a]];
+        load_cam(cam, ci[q], R, tt);
+        const D4 xp = X[pi[q]];
         double e0, e1, zc, w;
-        const double s = edge_residual(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
-                                       P.o_uv[2 * (size_t)a], P.o_uv[2 * (size_t)a + 1], e0, e1, zc);
-        acc += robustify(s, P.robust, P.delta, w);
+        const double s = edge_residual(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, uu[q], vv[q], e0, e1, zc);
+        const double rho = robustify(s, P.robust, P.delta, w);
+        if (a0 + (int)threadIdx.x + 256 * q < a1) acc += rho;
     }
     const double s = block_sum<double, 256>(acc, sh);
     if (threadIdx.x == 0) red_cost[b] = s;

@@ -52,7 +52,8 @@ def check_fp64_against_oracle(oracle_mod, prob, opts, got, ref, label=""):
     1e-6.  When the problem itself is that ill-conditioned (its 1-ulp
     sensitivity above the bar), cost and history bars widen to four times the
     sensitivity; a trial count may then differ only if one ulp of input
-    already changes the trajectory (sensitivity above 1e-6)."""
+    already changes the trajectory (sensitivity above 1e-6), and the flags are
+    then checked against the oracle's flags at the GPU's final state."""
     say(f"  {label} cost gpu {got.final_cost!r} oracle {ref.final_cost!r} iters {got.iterations}/{ref.iterations} "
         f"trials {got.trials}/{ref.trials} rel {rel(got.final_cost, ref.final_cost):.3e} "
         f"hist {hist_rel(got.cost_history, ref.cost_history):.3e} "
@@ -71,5 +72,14 @@ def check_fp64_against_oracle(oracle_mod, prob, opts, got, ref, label=""):
     assert got.stop_reason == ref.stop_reason
     assert rel(got.final_cost, ref.final_cost) <= tol
     assert hist_rel(got.cost_history, ref.cost_history) <= tol_hist
-    assert np.array_equal(got.outlier, ref.outlier)
+    if same_schedule:
+        assert np.array_equal(got.outlier, ref.outlier)
+    else:
+        # a chaotic problem on another LM path ends in another state: its flags
+        # must be the oracle's flags at the GPU's own final state (observations
+        # within 1e-6 of the threshold excepted: the returned state is rounded)
+        _, c2, fl = oracle_mod.cost(prob, opts, got.T_cw, got.X)
+        near = np.abs(c2 - opts.outlier_chi2) <= 1e-6 * opts.outlier_chi2
+        say(f"  flags at the GPU state: {int((got.outlier != fl).sum())} differ, {int(near.sum())} near the threshold")
+        assert np.array_equal(got.outlier[~near], fl[~near])
     np.testing.assert_allclose(got.T_cw, ref.T_cw, atol=tol_pose)
