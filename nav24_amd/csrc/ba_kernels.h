@@ -95,6 +95,8 @@ struct DevProblem {
     // column blocks of each S row: free indices >= row (same window), first is the row itself
     const int* colptr;         // [n_fc + 1]
     const int* colcam;
+    const int* zcolptr;        // [n_fc + 1] per S row: the blocks g > f with no co-visibility
+    const int* zcolcam;        //   (written as zeros by the reduction every trial)
     // windows
     int n_win;
     int max_win_cams;
@@ -173,6 +175,34 @@ __device__ __forceinline__ void load_cam(const CamRT* cam, int c, T R[9], T t[3]
     for (int k = 0; k < 3; ++k) t[k] = (T)p[9 + k];
 }
 
+// Candidate pose of camera c, T <- exp(xi) T with xi = step(f, 0..5) for a
+// free camera f (a copy for a fixed camera or a failed factorisation), written
+// to pn / cn; returns the camera's term of x'(lambda x + b).
+template <class Step>
+__device__ __forceinline__ double camera_candidate(const DevProblem& P, const PoseQT* __restrict__ pose,
+                                                   const double* __restrict__ Hc, double lambda, int fail, int c,
+                                                   Step step, PoseQT* __restrict__ pn, CamRT* __restrict__ cn) {
+    const int f = P.cam_free[c];
+    const PoseQT in = pose[c];
+    PoseQT o = in;
+    double sc = 0.0;
+    if (f >= 0 && !fail) {
+        double xi[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) xi[i] = step(f, i);
+        se3_oplus(xi, in, o);
+        const double* bp = Hc + 28 * (size_t)f + 21;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sc += xi[i] * (lambda * xi[i] + bp[i]);
+    }
+    pn[c] = o;
+    CamRT r;
+    quat_to_R(o.q, r.R);
+    r.t[0] = o.t[0]; r.t[1] = o.t[1]; r.t[2] = o.t[2];
+    cn[c] = r;
+    return sc;
+}
+
 // Launchers.  T = double (FP64 path) or float (mixed path).
 template <typename T> int prepare_schur_row(const DevProblem& P);  // LDS attribute; < 0 if too large
 void launch_cam_rt(const DevProblem& P, const DevState& s, hipStream_t st);
@@ -182,8 +212,13 @@ template <typename T> void launch_lin_cam(const DevProblem& P, const DevState& s
 template <typename T> void launch_lambda_init(const DevProblem& P, const DevWork& W, hipStream_t st);
 template <typename T> void launch_hinv(const DevProblem& P, const DevWork& W, hipStream_t st);
 template <typename T> void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
+void launch_fill_system(const DevProblem& P, const DevWork& W, hipStream_t st);  // once per prepare
 template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st);
-void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st);
+// One problem: the tiled path's backward launch also forms the candidate
+// poses (cholesky_updates_cameras); the block path leaves that to launch_update_cam.
+void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
+                           hipStream_t st);
+inline bool cholesky_updates_cameras(const DevProblem& P) { return !P.batched && !P.use_blocks && P.n_fc > 0; }
 void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st);
 int prepare_cholesky_small(const DevProblem& P);  // LDS attribute of the per-window factorisation
 int prepare_cholesky(const DevProblem& P);  // LDS attributes of the one-problem factorisation kernels

@@ -289,11 +289,12 @@ void launch_hinv(const DevProblem& P, const DevWork& W, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- schur reduce
-// Clear the dense system of every running window (padding rows of the tiled
-// path get a unit diagonal) and its right-hand side.
-__global__ void k_fill_system(DevProblem P, const LMDev* lm, double* __restrict__ S, double* __restrict__ g) {
+// Clear the dense systems (padding rows of the tiled path get a unit
+// diagonal) and their right-hand sides; run once per prepare.  The reduction
+// rewrites every lower-triangle block of a window each trial (zeros where no
+// point links two cameras), and the padding is never changed.
+__global__ void k_fill_system(DevProblem P, double* __restrict__ S, double* __restrict__ g) {
     const int win = blockIdx.y;
-    if (!running(lm, win)) return;
     const int lda = P.win_lda[win];
     const int n = 6 * (P.win_fc0[win + 1] - P.win_fc0[win]);
     double* Sw = S + P.win_S[win];
@@ -311,8 +312,9 @@ __global__ void k_fill_system(DevProblem P, const LMDev* lm, double* __restrict_
 
 // Workgroups (f, part) for S row block f, each taking a quarter of the row's
 // entries: sum the chunk partials (FP64, fixed order), form
-// S_ff = Hpp + lambda I - sum, S_gf = -sum^T (lower triangle); part 0 also
-// forms g = bp - coef.
+// S_ff = Hpp + lambda I - sum, S_gf = -sum^T (lower triangle), and write
+// zeros to the row's blocks without co-visibility (the factorisation of the
+// previous trial left values there); part 0 also forms g = bp - coef.
 constexpr int kReduceParts = 4;
 
 __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const double* __restrict__ spart,
@@ -357,6 +359,15 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
             Sw[(size_t)(6 * lg + c) * lda + 6 * lf + i] = -sum;
         }
     }
+    {
+        const int z0 = P.zcolptr[f], nz = (P.zcolptr[f + 1] - z0) * 36;
+        const int zper = (nz + kReduceParts - 1) / kReduceParts;
+        const int q0 = blockIdx.y * zper, q1 = min(nz, q0 + zper);
+        for (int idx = q0 + threadIdx.x; idx < q1; idx += kBlock) {
+            const int lg = P.zcolcam[z0 + idx / 36] - fc0, e = idx % 36;
+            Sw[(size_t)(6 * lg + e % 6) * lda + 6 * lf + e / 6] = 0.0;
+        }
+    }
     if (blockIdx.y == 0) {
         // coefficient of group 0's chunks: 24 threads, entry t % 6, chunks = t / 6 (mod 4)
         if (threadIdx.x < 24) {
@@ -377,11 +388,14 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
     }
 }
 
-template <typename T>
-void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) {
+void launch_fill_system(const DevProblem& P, const DevWork& W, hipStream_t st) {
     const size_t tiles = ((size_t)P.n_pad * P.n_pad + 255) / 256;
     const int chunks = P.batched ? 4 : (int)(tiles < 65535 ? (tiles > 0 ? tiles : 1) : 65535);
-    hipLaunchKernelGGL(k_fill_system, dim3(chunks, P.n_win), dim3(256), 0, st, P, (const LMDev*)W.lm, W.S, W.g);
+    hipLaunchKernelGGL(k_fill_system, dim3(chunks, P.n_win), dim3(256), 0, st, P, W.S, W.g);
+}
+
+template <typename T>
+void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) {
     if (P.n_fc == 0) return;
     hipLaunchKernelGGL(k_schur_reduce, dim3(P.n_fc, kReduceParts), dim3(kBlock), 0, st, P, (const double*)W.spart,
                        W.Hc, (const LMDev*)W.lm, W.S, W.g);
@@ -526,25 +540,8 @@ __global__ __launch_bounds__(256) void k_update_cam(DevProblem P, const PoseQT* 
     const double lambda = lm[win].lambda;
     const int fail = lm[win].fail;
     double sc = 0.0;
-    for (int c = P.win_cam0[win] + threadIdx.x; c < P.win_cam0[win + 1]; c += 256) {
-        const int f = P.cam_free[c];
-        const PoseQT in = pose[c];
-        PoseQT o = in;
-        if (f >= 0 && !fail) {
-            double xi[6];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) xi[i] = dp[6 * (size_t)f + i];
-            se3_oplus(xi, in, o);
-            const double* bp = Hc + 28 * (size_t)f + 21;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) sc += xi[i] * (lambda * xi[i] + bp[i]);
-        }
-        pn[c] = o;
-        CamRT r;
-        quat_to_R(o.q, r.R);
-        r.t[0] = o.t[0]; r.t[1] = o.t[1]; r.t[2] = o.t[2];
-        cn[c] = r;
-    }
+    for (int c = P.win_cam0[win] + threadIdx.x; c < P.win_cam0[win + 1]; c += 256)
+        sc += camera_candidate(P, pose, Hc, lambda, fail, c, [&](int f, int i) { return dp[6 * (size_t)f + i]; }, pn, cn);
     const double s = block_sum<double, 256>(sc, sh);
     if (threadIdx.x == 0) red_scale[win] = s;
 }

@@ -454,10 +454,14 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
         __syncthreads();  // L_i,k (stored above by this workgroup) is read back by the forward step
         tile_forward(Bj, Lm, lda, i, g, y, rv);
     } else {
+        // a diagonal tile keeps its upper triangle (never read; left unchanged
+        // so that it does not drift over the trials of a solve)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Aij[(size_t)(16 * w + 4 * v + kq) * lda + 16 * b + r16] = cur[b][v] - acc[b][v];
+            for (int v = 0; v < 4; ++v)
+                if (!same || 16 * w + 4 * v + kq >= 16 * b + r16)
+                    Aij[(size_t)(16 * w + 4 * v + kq) * lda + 16 * b + r16] = cur[b][v] - acc[b][v];
     }
 }
 
@@ -468,25 +472,30 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
 //   r_i -= L_ki^T x_k     (all columns left of tile k: 128 per pass, 4 partial
 //                          sums per column);
 // r_k is final when step k starts (every later step has subtracted its
-// term).  The L_ki loads of a step's first pass are issued before x_k is
-// formed.  One workgroup keeps the tile chain inside one launch.  dp = 0 when
-// the factorisation failed.
+// term), and x_k takes its place in LDS.  The L_ki loads of a step's first
+// pass are issued before x_k is formed.  One workgroup keeps the tile chain
+// inside one launch.  dp = 0 when the factorisation failed.  The same
+// workgroup then forms the candidate poses from x (the camera update of the
+// problem's one window) and the camera part of x'(lambda x + b).
 constexpr int kBackThreads = 512;
 
-__global__ __launch_bounds__(kBackThreads) void k_chol_back(const double* __restrict__ Lm,
+__global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const double* __restrict__ Lm,
                                                             const double* __restrict__ Li, int lda, int nt,
                                                             const double* __restrict__ y, double* __restrict__ dp,
-                                                            const LMDev* lm) {
+                                                            const LMDev* lm, const PoseQT* __restrict__ pose,
+                                                            const double* __restrict__ Hc, PoseQT* __restrict__ pn,
+                                                            CamRT* __restrict__ cn, double* __restrict__ red_scale) {
     extern __shared__ double rs[];  // [64 nt]
     __shared__ double part[8][64];
     __shared__ double red[4][128];
     __shared__ double xk[64];
+    __shared__ double shs[kBackThreads / kWave];
     const int tid = threadIdx.x, n = 64 * nt;
     if (lm->phase == kPhaseDone) return;
-    if (lm->fail) {
+    const int fail = lm->fail;
+    if (fail) {
         for (int i = tid; i < n; i += kBackThreads) dp[i] = 0.0;
-        return;
-    }
+    } else {
     for (int i = tid; i < n; i += kBackThreads) rs[i] = y[i];
     __syncthreads();
     const int c = tid & 63, grp = tid >> 6;   // x_k: entry c, rows grp + 8u
@@ -516,6 +525,7 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(const double* __rest
             const double x = ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid])) +
                              ((part[4][tid] + part[5][tid]) + (part[6][tid] + part[7][tid]));
             xk[tid] = x;
+            rs[64 * k + tid] = x;  // r_k is not read again
             dp[64 * k + tid] = x;
         }
         __syncthreads();
@@ -543,6 +553,13 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(const double* __rest
             __syncthreads();
         }
     }
+    }
+    const double lambda = lm->lambda;
+    double sc = 0.0;
+    for (int c = tid; c < P.n_cam; c += kBackThreads)
+        sc += camera_candidate(P, pose, Hc, lambda, fail, c, [&](int f, int i) { return rs[6 * f + i]; }, pn, cn);
+    const double s = block_sum<double, kBackThreads>(sc, shs);
+    if (tid == 0) red_scale[0] = s;
 }
 
 // ---------------------------------------------------------------- per-window systems
@@ -959,7 +976,8 @@ int prepare_cholesky(const DevProblem& P) {
                    hipSuccess ? 0 : -1;
 }
 
-void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st) {
+void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
+                           hipStream_t st) {
     if (P.n_fc == 0) return;
     const int nt = P.nt, lda = P.n_pad;
     if (P.use_blocks) {
@@ -974,9 +992,10 @@ void launch_cholesky_solve(const DevProblem& P, const DevWork& W, hipStream_t st
         hipLaunchKernelGGL(k_chol_step, dim3(rem * (rem + 1) / 2), dim3(256), 0, st, W.S, W.Lm, W.Li, lda, k,
                            (const double*)W.g, W.y, W.lm);
     }
-    hipLaunchKernelGGL(k_chol_back, dim3(1), dim3(kBackThreads), sizeof(double) * (size_t)lda, st,
+    hipLaunchKernelGGL(k_chol_back, dim3(1), dim3(kBackThreads), sizeof(double) * (size_t)lda, st, P,
                        (const double*)W.Lm, (const double*)W.Li, lda, nt, (const double*)W.y, W.dp,
-                       (const LMDev*)W.lm);
+                       (const LMDev*)W.lm, (const PoseQT*)cur.pose, (const double*)W.Hc, cand.pose, cand.cam,
+                       W.red_scale);
 }
 
 }  // namespace navba

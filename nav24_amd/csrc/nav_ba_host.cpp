@@ -251,10 +251,11 @@ void enqueue_step(nav_ba_handle* h) {
     if (h->P.n_fc > 0)
         stage(h, NAV_BA_ST_CHOLESKY, [&] {
             if (h->P.batched) launch_cholesky_small(h->P, h->W, h->stream);
-            else launch_cholesky_solve(h->P, h->W, h->stream);
+            else launch_cholesky_solve(h->P, h->cur, h->cand, h->W, h->stream);
         });
     stage(h, NAV_BA_ST_BACKSUB, [&] { launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream); });
-    stage(h, NAV_BA_ST_UPDATE_CAM, [&] { launch_update_cam(h->P, h->cur, h->cand, h->W, h->stream); });
+    if (!cholesky_updates_cameras(h->P))
+        stage(h, NAV_BA_ST_UPDATE_CAM, [&] { launch_update_cam(h->P, h->cur, h->cand, h->W, h->stream); });
     stage(h, NAV_BA_ST_COST, [&] { launch_cost<T>(h->P, h->cand, h->W, 1, h->stream); });
     stage(h, NAV_BA_ST_ACCEPT, [&] {
         launch_lm_accept(h->P, h->W, h->stream);
@@ -719,7 +720,9 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     }
 
     // ---- co-visibility through free points, per window: column blocks g >= f of each S row
-    std::vector<int> colptr{0}, colcam;
+    // ... and the other blocks g > f of each row (zero in S, rewritten every trial
+    // because the factorisation works in place)
+    std::vector<int> colptr{0}, colcam, zcolptr{0}, zcolcam;
     {
         std::vector<uint64_t> bits;
         std::vector<int> fl;
@@ -744,8 +747,10 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
                 for (int lg = lf + 1; lg < nw; ++lg) {
                     const size_t bit = (size_t)lf * nw + lg;
                     if (bits[bit >> 6] >> (bit & 63) & 1ull) colcam.push_back(f0 + lg);
+                    else zcolcam.push_back(f0 + lg);
                 }
                 colptr.push_back((int)colcam.size());
+                zcolptr.push_back((int)zcolcam.size());
             }
         }
     }
@@ -821,7 +826,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.c_pt, c_pt); UP(P.c_uv, c_uv);
     UP(P.tile_obs, tile_obs); UP(P.tile_pt, tile_pt); UP(P.tile_win, tile_win);
     UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
-    UP(P.colptr, colptr); UP(P.colcam, colcam);
+    UP(P.colptr, colptr); UP(P.colcam, colcam); UP(P.zcolptr, zcolptr); UP(P.zcolcam, zcolcam);
     UP(P.cam_win, cam_win); UP(P.pt_win, pt_win_all); UP(P.win_cam0, win_cam0); UP(P.win_fc0, win_fc0);
     UP(P.win_pt0, win_pt0); UP(P.win_S, win_S); UP(P.win_lda, win_lda);
     UP(P.cblk_o0, cblk_o0); UP(P.cblk_win, cblk_win); UP(P.win_cblk0, win_cblk0);
@@ -879,6 +884,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
         return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
     if (prepare_cholesky(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the factorisation kernels");
     if (batched && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
+    launch_fill_system(P, h->W, h->stream);
     if (tune) {
         if ((st = tune_schur_chunk(h, G, schur_mark, SI)) != NAV_BA_OK) return st;
         remember_chunk(key, SI.chunk);
