@@ -91,15 +91,17 @@ __global__ __launch_bounds__(kBlock) void k_lin_pt(DevProblem P, const CamRT* __
             double A[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) A[k] = (double)L.JX[k];
-            c[0] += (A[0] * w) * A[0] + (A[3] * w) * A[3];
-            c[1] += (A[0] * w) * A[1] + (A[3] * w) * A[4];
-            c[2] += (A[0] * w) * A[2] + (A[3] * w) * A[5];
-            c[3] += (A[1] * w) * A[1] + (A[4] * w) * A[4];
-            c[4] += (A[1] * w) * A[2] + (A[4] * w) * A[5];
-            c[5] += (A[2] * w) * A[2] + (A[5] * w) * A[5];
-            c[6] += A[0] * r0 + A[3] * r1;
-            c[7] += A[1] * r0 + A[4] * r1;
-            c[8] += A[2] * r0 + A[5] * r1;
+            const double w0 = A[0] * w, w1 = A[1] * w, w2 = A[2] * w;
+            const double w3 = A[3] * w, w4 = A[4] * w, w5 = A[5] * w;
+            c[0] = fma(w3, A[3], fma(w0, A[0], c[0]));
+            c[1] = fma(w3, A[4], fma(w0, A[1], c[1]));
+            c[2] = fma(w3, A[5], fma(w0, A[2], c[2]));
+            c[3] = fma(w4, A[4], fma(w1, A[1], c[3]));
+            c[4] = fma(w4, A[5], fma(w1, A[2], c[4]));
+            c[5] = fma(w5, A[5], fma(w2, A[2], c[5]));
+            c[6] = fma(A[3], r1, fma(A[0], r0, c[6]));
+            c[7] = fma(A[4], r1, fma(A[1], r0, c[7]));
+            c[8] = fma(A[5], r1, fma(A[2], r0, c[8]));
         }
     }
     if (o1 - o0 <= kTile) {
@@ -184,10 +186,10 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
         for (int i = 0; i < 6; ++i) {
             const double A0 = J[i] * wa, A1 = J[6 + i] * wa;
 #pragma unroll
-            for (int j = i; j < 6; ++j) acc[q++] += A0 * J[j] + A1 * J[6 + j];
+            for (int j = i; j < 6; ++j, ++q) acc[q] = fma(A1, J[6 + j], fma(A0, J[j], acc[q]));
         }
 #pragma unroll
-        for (int i = 0; i < 6; ++i) acc[21 + i] += J[i] * r0 + J[6 + i] * r1;
+        for (int i = 0; i < 6; ++i) acc[21 + i] = fma(J[6 + i], r1, fma(J[i], r0, acc[21 + i]));
     }
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll

@@ -149,13 +149,15 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 if (g0) {
                     const double e0 = ps[6], e1 = ps[7], e2 = ps[8];
 #pragma unroll
-                    for (int i = 0; i < 6; ++i) coef[i] += Ha[3 * i] * e0 + Ha[3 * i + 1] * e1 + Ha[3 * i + 2] * e2;
+                    for (int i = 0; i < 6; ++i)
+                        coef[i] = fma(Ha[3 * i + 2], e2, fma(Ha[3 * i + 1], e1, fma(Ha[3 * i], e0, coef[i])));
                     int q = 0;
 #pragma unroll
                     for (int i = 0; i < 6; ++i)
 #pragma unroll
                         for (int c = i; c < 6; ++c, ++q)
-                            dsum[q] += W[3 * i] * Ha[3 * c] + W[3 * i + 1] * Ha[3 * c + 1] + W[3 * i + 2] * Ha[3 * c + 2];
+                            dsum[q] = fma(W[3 * i + 2], Ha[3 * c + 2],
+                                          fma(W[3 * i + 1], Ha[3 * c + 1], fma(W[3 * i], Ha[3 * c], dsum[q])));
                 }
             }
         }
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
 #pragma unroll
                 for (int r = 0; r < 6; ++r)
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) sacc[6 * r + c] += M0[r] * B0[c] + M1[r] * B1[c];
+                    for (int c = 0; c < 6; ++c) sacc[6 * r + c] = fma(M1[r], B1[c], fma(M0[r], B0[c], sacc[6 * r + c]));
             }
             if (has) {
 #pragma unroll
