@@ -181,10 +181,11 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 const double* wb = wbuf + (pr.y & 63) * SchurLayout<T>::kWbuf;
                 EdgeLin<T> Lb;
                 edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
+                // the Huber weight of b folded into J_X,b: W_a (w_b J_X,b)^T J_xi,b
                 const double wgt = (double)w[pr.x];
                 double JXb[6];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) JXb[q] = (double)Lb.JX[q];
+                for (int q = 0; q < 6; ++q) JXb[q] = wgt * (double)Lb.JX[q];
                 double M0[6], M1[6];
 #pragma unroll
                 for (int r = 0; r < 6; ++r) {
@@ -194,11 +195,18 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 }
                 double B0[6], B1[6];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) { B0[c] = wgt * (double)Lb.Jxi[c]; B1[c] = wgt * (double)Lb.Jxi[6 + c]; }
+                for (int c = 0; c < 6; ++c) { B0[c] = (double)Lb.Jxi[c]; B1[c] = (double)Lb.Jxi[6 + c]; }
+                // J_xi,b is zero at (0, 4) and (1, 3) (edge_linearize: d e_u / d upsilon_y and
+                // d e_v / d upsilon_x of a pinhole); those products are skipped
 #pragma unroll
                 for (int r = 0; r < 6; ++r)
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) sacc[6 * r + c] = fma(M1[r], B1[c], fma(M0[r], B0[c], sacc[6 * r + c]));
+                    for (int c = 0; c < 6; ++c) {
+                        double& a = sacc[6 * r + c];
+                        if (c == 4) a = fma(M1[r], B1[c], a);
+                        else if (c == 3) a = fma(M0[r], B0[c], a);
+                        else a = fma(M1[r], B1[c], fma(M0[r], B0[c], a));
+                    }
             }
             if (has) {
 #pragma unroll
