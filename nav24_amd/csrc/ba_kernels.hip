@@ -186,10 +186,17 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
         for (int i = 0; i < 6; ++i) {
             const double A0 = J[i] * wa, A1 = J[6 + i] * wa;
 #pragma unroll
-            for (int j = i; j < 6; ++j, ++q) acc[q] = fma(A1, J[6 + j], fma(A0, J[j], acc[q]));
+            for (int j = i; j < 6; ++j, ++q) {  // J_xi is zero at (0, 4) and (1, 3)
+                const bool z0 = (i == 4 || j == 4), z1 = (i == 3 || j == 3);
+                if (!z0 && !z1) acc[q] = fma(A1, J[6 + j], fma(A0, J[j], acc[q]));
+                else if (!z0) acc[q] = fma(A0, J[j], acc[q]);
+                else if (!z1) acc[q] = fma(A1, J[6 + j], acc[q]);
+            }
         }
 #pragma unroll
-        for (int i = 0; i < 6; ++i) acc[21 + i] = fma(J[6 + i], r1, fma(J[i], r0, acc[21 + i]));
+        for (int i = 0; i < 6; ++i)
+            acc[21 + i] = i == 4 ? fma(J[6 + i], r1, acc[21 + i])
+                                 : i == 3 ? fma(J[i], r0, acc[21 + i]) : fma(J[6 + i], r1, fma(J[i], r0, acc[21 + i]));
     }
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
@@ -426,7 +433,10 @@ __device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __
     const double* d = dp + 6 * (size_t)fb;
     double t0 = 0.0, t1 = 0.0;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) { t0 += (double)L.Jxi[i] * d[i]; t1 += (double)L.Jxi[6 + i] * d[i]; }
+    for (int i = 0; i < 6; ++i) {  // J_xi is zero at (0, 4) and (1, 3)
+        if (i != 4) t0 += (double)L.Jxi[i] * d[i];
+        if (i != 3) t1 += (double)L.Jxi[6 + i] * d[i];
+    }
     t0 *= wb; t1 *= wb;
     c[0] = (double)L.JX[0] * t0 + (double)L.JX[3] * t1;
     c[1] = (double)L.JX[1] * t0 + (double)L.JX[4] * t1;

@@ -130,7 +130,8 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 for (int i = 0; i < 6; ++i) {
                     const double A0 = (double)La.Jxi[i] * wa, A1 = (double)La.Jxi[6 + i] * wa;
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) Ha[3 * i + j] = A0 * JXa[j] + A1 * JXa[3 + j];
+                    for (int j = 0; j < 3; ++j)  // J_xi is zero at (0, 4) and (1, 3)
+                        Ha[3 * i + j] = i == 4 ? A1 * JXa[3 + j] : i == 3 ? A0 * JXa[j] : A0 * JXa[j] + A1 * JXa[3 + j];
                 }
                 const double* ps = ptsys + 12 * (size_t)p;
                 const double d00 = ps[0], d01 = ps[1], d02 = ps[2], d11 = ps[3], d12 = ps[4], d22 = ps[5];
