@@ -139,6 +139,7 @@ struct DevWork {
     double* dp;
     double* Lm;         // one problem: Cholesky factor (tiled: n_pad^2; blocks: packed 6x6 blocks)
     double* Li;         // tiled path: inverses of the diagonal factor tiles [nt][64][64]
+    double* fpart;      // tiled path: forward-step products L_ij y_j [nt][nt][64]
     double* red_cost;   // [n_cblk]
     double* red_scale;  // [n_win + n_tiles]: per-window camera part, then per-tile point parts
     double* red_aux;    // [n_cblk] outlier counts
@@ -214,11 +215,14 @@ template <typename T> void launch_hinv(const DevProblem& P, const DevWork& W, hi
 template <typename T> void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 void launch_fill_system(const DevProblem& P, const DevWork& W, hipStream_t st);  // once per prepare
 template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st);
-// One problem: the tiled path's backward launch also forms the candidate
-// poses (cholesky_updates_cameras); the block path leaves that to launch_update_cam.
+// One problem: the tiled path's one-workgroup backward launch also forms the
+// candidate poses (cholesky_updates_cameras); otherwise launch_update_cam does.
 void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
                            hipStream_t st);
-inline bool cholesky_updates_cameras(const DevProblem& P) { return !P.batched && !P.use_blocks && P.n_fc > 0; }
+constexpr int kBackSingleMax = 10;  // tiles up to which the backward step runs in one workgroup
+inline bool cholesky_updates_cameras(const DevProblem& P) {
+    return !P.batched && !P.use_blocks && P.n_fc > 0 && P.nt <= kBackSingleMax;
+}
 void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st);
 int prepare_cholesky_small(const DevProblem& P);  // LDS attribute of the per-window factorisation
 int prepare_cholesky(const DevProblem& P);  // LDS attributes of the one-problem factorisation kernels

@@ -353,26 +353,45 @@ __device__ __forceinline__ void tile_right_mul_t(TileLds& B, const TileLds& M) {
     __syncthreads();
 }
 
-// y_k = M_k (g_k - sum_{j<k} L_kj y_j): the rows of the factor come from Lm
-// (L_kj already stored), y_j (j < k) from global memory, M_k from LDS.
-__device__ __forceinline__ void tile_forward(const TileLds& M, const double* __restrict__ Lm, int lda, int k,
+// out = B v for a 64x64 tile B in LDS and a 64-vector v in global memory
+// (four threads per row, summed by DPP in a fixed order).
+__device__ __forceinline__ void tile_gemv(const TileLds& B, const double* __restrict__ v, double* __restrict__ out) {
+    const int tid = threadIdx.x;
+    const int r = tid >> 2, q = tid & 3;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int m = q; m < 64; m += 16) {
+        s0 += B[r][m] * v[m];
+        s1 += B[r][m + 4] * v[m + 4];
+        s2 += B[r][m + 8] * v[m + 8];
+        s3 += B[r][m + 12] * v[m + 12];
+    }
+    double s = (s0 + s1) + (s2 + s3);
+    s += dpp_move<0xB1>(s);
+    s += dpp_move<0x4E>(s);
+    if (q == 0) out[r] = s;
+}
+
+// y_k = M_k (g_k - sum_{j<=k-1} L_kj y_j).  The products L_kj y_j were formed
+// by the workgroups that computed L_kj (tile_gemv into fpart[k][j], 64 values
+// each); the nterms of them are summed here in a fixed order, so the row block
+// of the factor is not read again.  M_k is in LDS.
+__device__ __forceinline__ void tile_forward(const TileLds& M, const double* __restrict__ fpart, int nterms,
                                              const double* __restrict__ g, double* __restrict__ y, double* rv) {
     const int tid = threadIdx.x;
     const int r = tid >> 2, q = tid & 3;
     {
-        // four independent accumulators keep 4 loads per thread in flight
-        const double* row = Lm + (size_t)(64 * k + r) * lda;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        for (int m = q; m < 64 * k; m += 16) {
-            s0 += row[m] * y[m];
-            s1 += row[m + 4] * y[m + 4];
-            s2 += row[m + 8] * y[m + 8];
-            s3 += row[m + 12] * y[m + 12];
+        double s0 = 0.0, s1 = 0.0;
+        int j = q;
+        for (; j + 4 < nterms; j += 8) {
+            s0 += fpart[64 * j + r];
+            s1 += fpart[64 * (j + 4) + r];
         }
-        double s = (s0 + s1) + (s2 + s3);
+        if (j < nterms) s0 += fpart[64 * j + r];
+        double s = s0 + s1;
         s += dpp_move<0xB1>(s);
         s += dpp_move<0x4E>(s);
-        if (q == 0) rv[r] = g[64 * k + r] - s;
+        if (q == 0) rv[r] = g[r] - s;
     }
     __syncthreads();
     double s0 = 0.0, s1 = 0.0;
@@ -383,7 +402,7 @@ __device__ __forceinline__ void tile_forward(const TileLds& M, const double* __r
     double s = s0 + s1;
     s += dpp_move<0xB1>(s);
     s += dpp_move<0x4E>(s);
-    if (q == 0) y[64 * k + r] = s;
+    if (q == 0) y[r] = s;
     __syncthreads();
 }
 
@@ -400,12 +419,13 @@ __global__ __launch_bounds__(256) void k_chol_first(const double* __restrict__ S
     if (tile_potrf(A, rpiv) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
     tile_trinv(A, rpiv, M);
     tile_store(M, Li, 64);
-    tile_forward(M, Lm, lda, 0, g, y, rv);
+    tile_forward(M, nullptr, 0, g, y, rv);
 }
 
 __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, double* __restrict__ Lm,
                                                    double* __restrict__ Li, int lda, int k,
-                                                   const double* __restrict__ g, double* __restrict__ y, LMDev* lm) {
+                                                   const double* __restrict__ g, double* __restrict__ y,
+                                                   double* __restrict__ fpart, int nt, LMDev* lm) {
     __shared__ TileLds Mk;
     __shared__ TileLds Bi;
     __shared__ TileLds Bj;
@@ -437,7 +457,10 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     __syncthreads();
     tile_right_mul_t(Bi, Mk);  // L_ik
     if (!same) tile_right_mul_t(Bj, Mk);  // L_jk
-    if (j == k + 1) tile_store(Bi, Lm + (size_t)(64 * i) * lda + 64 * k, lda);
+    if (j == k + 1) {
+        tile_store(Bi, Lm + (size_t)(64 * i) * lda + 64 * k, lda);
+        tile_gemv(Bi, y + 64 * k, fpart + ((size_t)i * nt + k) * 64);  // L_ik y_k for the forward step of tile i
+    }
     dbl4 acc[4];
     tile_mfma_nt<false>(Bi, same ? Bi : Bj, acc);
     if (same && i == k + 1) {
@@ -451,8 +474,8 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
         if (tile_potrf(Mk, rpiv) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
         tile_trinv(Mk, rpiv, Bj);
         tile_store(Bj, Li + (size_t)4096 * i, 64);
-        __syncthreads();  // L_i,k (stored above by this workgroup) is read back by the forward step
-        tile_forward(Bj, Lm, lda, i, g, y, rv);
+        __syncthreads();  // fpart[i][k] (written above by this workgroup) is read back by the forward step
+        tile_forward(Bj, fpart + (size_t)i * nt * 64, k + 1, g + 64 * i, y + 64 * i, rv);
     } else {
         // a diagonal tile keeps its upper triangle (never read; left unchanged
         // so that it does not drift over the trials of a solve)
@@ -465,7 +488,8 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
     }
 }
 
-// Backward step x = L^-T y in one workgroup of 512 threads, tile by tile
+// Backward step x = L^-T y for up to kBackSingleMax tiles in one workgroup of
+// 512 threads, tile by tile
 // from the last.  The right-hand sides r (the forward result) are held in LDS
 // and updated in place:
 //   x_k  = M_k^T r_k      (64x64 product, 8 partial sums per entry),
@@ -560,6 +584,80 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const 
         sc += camera_candidate(P, pose, Hc, lambda, fail, c, [&](int f, int i) { return rs[6 * f + i]; }, pn, cn);
     const double s = block_sum<double, kBackThreads>(sc, shs);
     if (tid == 0) red_scale[0] = s;
+}
+
+// Backward step for many tiles (more than kBackSingleMax): one launch per tile
+// k from the last, so the reads of the factor spread over k + 1 CUs.  The
+// right-hand sides r of the tiles still to solve live in y (the forward
+// result) and are updated in place: every workgroup of launch k forms
+// x_k = M_k^T r_k (a 64x64 product, recomputed by each workgroup instead of
+// waiting for one); workgroup k writes it to dp, workgroup i < k subtracts
+// L_ki^T x_k from r_i.  r_k is only read during launch k, and final when it
+// starts (launches j > k subtracted their terms).  dp = 0 when the
+// factorisation failed.
+__global__ __launch_bounds__(256) void k_chol_back_tile(const double* __restrict__ Lm, const double* __restrict__ Li,
+                                                   int lda, int nt, int k, double* __restrict__ y,
+                                                   double* __restrict__ dp, const LMDev* lm) {
+    __shared__ double part[4][64];
+    __shared__ double xk[64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int i = blockIdx.x;  // 0..k: i == k writes x_k, i < k updates r_i
+    // every global load of the launch is issued before the first use: the
+    // L_ki column slice (independent of x_k), M_k, r_k and the failure flag
+    double lv[16];
+    if (i < k) {
+        const double* L = Lm + (size_t)(64 * k) * lda + 64 * i + lane;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) lv[t] = L[(size_t)(wid + 4 * t) * lda];
+    }
+    double mv[16], rv[16];
+    {
+        const double* Mk = Li + (size_t)4096 * k;
+        const double* rk = y + 64 * k;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {  // rows r = wid + 4t; M_k[r][c] = 0 for r < c
+            mv[t] = Mk[(wid + 4 * t) * 64 + lane];
+            rv[t] = rk[wid + 4 * t];
+        }
+    }
+    if (lm->phase == kPhaseDone) return;  // a step launched after the solve has finished
+    const int fail = lm->fail;
+    if (fail) {
+        if (k == nt - 1 && tid < 64) dp[64 * i + tid] = 0.0;  // launch nt-1 has one workgroup per tile
+        return;
+    }
+    // x_k[c] = sum_r M_k[r][c] r_k[r]: lanes over c, waves over r
+    {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+#pragma unroll
+        for (int t = 0; t < 16; t += 4) {
+            t0 += mv[t] * rv[t];
+            t1 += mv[t + 1] * rv[t + 1];
+            t2 += mv[t + 2] * rv[t + 2];
+            t3 += mv[t + 3] * rv[t + 3];
+        }
+        part[wid][lane] = (t0 + t1) + (t2 + t3);
+    }
+    __syncthreads();
+    if (wid == 0) xk[lane] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    __syncthreads();
+    if (i == k) {
+        if (tid < 64) dp[64 * k + tid] = xk[tid];
+        return;
+    }
+    // r_i[c] -= sum_m L[64k + m][64i + c] x_k[m]: lanes over c, waves over m = wid + 4t
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; t += 4) {
+        s0 += lv[t] * xk[wid + 4 * t];
+        s1 += lv[t + 1] * xk[wid + 4 * t + 4];
+        s2 += lv[t + 2] * xk[wid + 4 * t + 8];
+        s3 += lv[t + 3] * xk[wid + 4 * t + 12];
+    }
+    __syncthreads();
+    part[wid][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (wid == 0) y[64 * i + lane] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 // ---------------------------------------------------------------- per-window systems
@@ -990,12 +1088,18 @@ void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevSt
     for (int k = 0; k + 1 < nt; ++k) {
         const int rem = nt - k - 1;
         hipLaunchKernelGGL(k_chol_step, dim3(rem * (rem + 1) / 2), dim3(256), 0, st, W.S, W.Lm, W.Li, lda, k,
-                           (const double*)W.g, W.y, W.lm);
+                           (const double*)W.g, W.y, W.fpart, nt, W.lm);
     }
-    hipLaunchKernelGGL(k_chol_back, dim3(1), dim3(kBackThreads), sizeof(double) * (size_t)lda, st, P,
-                       (const double*)W.Lm, (const double*)W.Li, lda, nt, (const double*)W.y, W.dp,
-                       (const LMDev*)W.lm, (const PoseQT*)cur.pose, (const double*)W.Hc, cand.pose, cand.cam,
-                       W.red_scale);
+    if (nt <= kBackSingleMax) {
+        hipLaunchKernelGGL(k_chol_back, dim3(1), dim3(kBackThreads), sizeof(double) * (size_t)lda, st, P,
+                           (const double*)W.Lm, (const double*)W.Li, lda, nt, (const double*)W.y, W.dp,
+                           (const LMDev*)W.lm, (const PoseQT*)cur.pose, (const double*)W.Hc, cand.pose, cand.cam,
+                           W.red_scale);
+    } else {
+        for (int k = nt - 1; k >= 0; --k)
+            hipLaunchKernelGGL(k_chol_back_tile, dim3(k + 1), dim3(256), 0, st, (const double*)W.Lm,
+                               (const double*)W.Li, lda, nt, k, W.y, W.dp, (const LMDev*)W.lm);
+    }
 }
 
 }  // namespace navba

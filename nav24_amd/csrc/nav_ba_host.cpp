@@ -865,6 +865,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.dp, sizeof(double) * nvec);
     AL(W.Lm, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(S_size, 1)));
     AL(W.Li, sizeof(double) * 4096 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1)));
+    AL(W.fpart, sizeof(double) * 64 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1) * std::max(P.nt, 1)));
     AL(W.red_cost, sizeof(double) * std::max(P.n_cblk, 1));
     AL(W.red_scale, sizeof(double) * (n_win + std::max(P.n_tiles, 1)));
     AL(W.red_aux, sizeof(double) * std::max(P.n_cblk, 1));
