@@ -235,39 +235,61 @@ void launch_lin_cam(const DevProblem& P, const DevState& s, const DevWork& W, hi
 
 // ---------------------------------------------------------------- lambda init
 // g2o computeLambdaInit with no user lambda: tau * max |H_jj| over the free
-// vertices of the window, tau = 1e-5.  One workgroup per window, first
-// iteration only.
-__global__ __launch_bounds__(256) void k_lambda_init(DevProblem P, const double* __restrict__ Hc,
-                                                     const double* __restrict__ Hl, LMDev* lm) {
-    __shared__ double sh[4];
-    const int win = blockIdx.x;
-    if (!needs_lin(lm, win) || lm[win].iter != 0) return;
+// vertices of the window, tau = 1e-5.  Launched in the first step only, where
+// every window is at iteration 0 and its lambda slot holds 0 (k_lm_reset).
+// k_diag_max: one thread per free vertex (free cameras, then points); the
+// maximum goes into the window's lambda slot by an integer atomic max on the
+// bit pattern, which orders non-negative doubles as their values (NaN
+// diagonals are skipped, as fmax skips them).  One atomic per wave when the
+// wave's vertices share a window.
+__global__ __launch_bounds__(256) void k_diag_max(DevProblem P, const double* __restrict__ Hc,
+                                                  const double* __restrict__ Hl, LMDev* lm) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
     double m = 0.0;
-    const int f0 = P.win_fc0[win], f1 = P.win_fc0[win + 1];
-    for (int i = f0 * 6 + threadIdx.x; i < f1 * 6; i += 256) {
-        const int d = i % 6;
-        m = fmax(m, fabs(Hc[28 * (size_t)(i / 6) + sym6(d, d)]));
-    }
-    const int q0 = P.win_pt0[win], q1 = P.win_pt0[win + 1];
-    for (int i = q0 * 3 + threadIdx.x; i < q1 * 3; i += 256) {
-        const int d = i % 3;  // packed (00 01 02 11 12 22): diagonal at 0, 3, 5
-        if (P.pt_free[i / 3]) m = fmax(m, fabs(Hl[9 * (size_t)(i / 3) + d * (7 - d) / 2]));
-    }
+    int win = -1;
+    if (i < P.n_fc) {
+        win = P.cam_win[P.free_cam[i]];
+        const double* h = Hc + 28 * (size_t)i;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
-        lm[win].lambda = 1e-5 * m;
-        lm[win].nu = 2.0;
+        for (int d = 0; d < 6; ++d) m = fmax(m, fabs(h[sym6(d, d)]));
+    } else if (i < P.n_fc + P.n_pt) {
+        const int q = i - P.n_fc;
+        if (P.pt_free[q]) {
+            win = P.pt_win[q];
+            const double* h = Hl + 9 * (size_t)q;  // packed (00 01 02 11 12 22)
+            m = fmax(fmax(fabs(h[0]), fabs(h[3])), fabs(h[5]));
+        }
     }
+    if (win >= 0 && (!needs_lin(lm, win) || lm[win].iter != 0)) win = -1;
+    const int w0 = __builtin_amdgcn_readfirstlane(win);
+    if (__all(win == w0 || win < 0)) {
+        int wl = win;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            m = fmax(m, __shfl_xor(m, off, 64));
+            wl = max(wl, __shfl_xor(wl, off, 64));
+        }
+        if ((threadIdx.x & 63) == 0 && wl >= 0)
+            atomicMax((unsigned long long*)&lm[wl].lambda, (unsigned long long)__double_as_longlong(m));
+    } else if (win >= 0) {
+        atomicMax((unsigned long long*)&lm[win].lambda, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void k_lambda_scale(int n_win, LMDev* lm) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_win || !needs_lin(lm, w) || lm[w].iter != 0) return;
+    lm[w].lambda *= 1e-5;
+    lm[w].nu = 2.0;
 }
 
 template <typename T>
 void launch_lambda_init(const DevProblem& P, const DevWork& W, hipStream_t st) {
     if (!P.auto_lambda) return;
-    hipLaunchKernelGGL(k_lambda_init, dim3(P.n_win), dim3(256), 0, st, P, W.Hc, (const double*)W.Hl, W.lm);
+    const int n = P.n_fc + P.n_pt;
+    if (n > 0)
+        hipLaunchKernelGGL(k_diag_max, dim3((n + 255) / 256), dim3(256), 0, st, P, W.Hc, (const double*)W.Hl, W.lm);
+    hipLaunchKernelGGL(k_lambda_scale, dim3((P.n_win + 255) / 256), dim3(256), 0, st, P.n_win, W.lm);
 }
 
 // ---------------------------------------------------------------- hinv

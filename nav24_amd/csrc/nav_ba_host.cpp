@@ -112,7 +112,8 @@ struct nav_ba_handle {
     DevWork W{};
     std::vector<DevBuf> bufs;
     int nbuf = 0;
-    hipGraphExec_t step = nullptr;
+    hipGraphExec_t step = nullptr;   // one LM step
+    hipGraphExec_t step0 = nullptr;  // the first step of a solve
     bool timing = false;
     bool timing_active = false;
     int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks
@@ -227,11 +228,12 @@ void stage(nav_ba_handle* h, int cat, F&& f) {
     if (h->timing_active) h->timer.end(h->stream);
 }
 
+// The first step of a solve (every window at iteration 0) also sets lambda.
 template <typename T>
-void enqueue_lin(nav_ba_handle* h) {
+void enqueue_lin(nav_ba_handle* h, bool first) {
     stage(h, NAV_BA_ST_LIN_PT, [&] { launch_lin_pt<T>(h->P, h->cur, h->W, h->stream); });
     stage(h, NAV_BA_ST_LIN_CAM, [&] { launch_lin_cam<T>(h->P, h->cur, h->W, h->stream); });
-    if (h->P.auto_lambda) stage(h, NAV_BA_ST_LAMBDA, [&] { launch_lambda_init<T>(h->P, h->W, h->stream); });
+    if (first && h->P.auto_lambda) stage(h, NAV_BA_ST_LAMBDA, [&] { launch_lambda_init<T>(h->P, h->W, h->stream); });
 }
 
 template <typename T>
@@ -245,8 +247,8 @@ void enqueue_reduced_system(nav_ba_handle* h) {
 
 // One LM step of every window: gated linearisation, one trial, decision.
 template <typename T>
-void enqueue_step(nav_ba_handle* h) {
-    enqueue_lin<T>(h);
+void enqueue_step(nav_ba_handle* h, bool first) {
+    enqueue_lin<T>(h, first);
     enqueue_reduced_system<T>(h);
     if (h->P.n_fc > 0)
         stage(h, NAV_BA_ST_CHOLESKY, [&] {
@@ -263,25 +265,29 @@ void enqueue_step(nav_ba_handle* h) {
     });
 }
 
-int launch_step(nav_ba_handle* h) {
+int launch_step(nav_ba_handle* h, bool first) {
     if (!h->timing) {
-        HIPTRY(h, hipGraphLaunch(h->step, h->stream));
+        HIPTRY(h, hipGraphLaunch(first ? h->step0 : h->step, h->stream));
         return NAV_BA_OK;
     }
     h->timing_active = true;
-    if (mixed(h)) enqueue_step<float>(h); else enqueue_step<double>(h);
+    if (mixed(h)) enqueue_step<float>(h, first); else enqueue_step<double>(h, first);
     h->timing_active = false;
     return NAV_BA_OK;
 }
 
+// Two graphs: the first step of a solve (with the lambda initialisation) and every later one.
 int capture_step(nav_ba_handle* h) {
-    hipGraph_t g = nullptr;
-    HIPTRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    if (mixed(h)) enqueue_step<float>(h); else enqueue_step<double>(h);
-    HIPTRY(h, hipStreamEndCapture(h->stream, &g));
-    hipError_t e = hipGraphInstantiate(&h->step, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(h, NAV_BA_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    for (int first = 0; first < 2; ++first) {
+        hipGraph_t g = nullptr;
+        HIPTRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        if (mixed(h)) enqueue_step<float>(h, first); else enqueue_step<double>(h, first);
+        HIPTRY(h, hipStreamEndCapture(h->stream, &g));
+        hipError_t e = hipGraphInstantiate(first ? &h->step0 : &h->step, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess)
+            return fail(h, NAV_BA_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
     return NAV_BA_OK;
 }
 
@@ -508,8 +514,8 @@ int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems
     for (auto& l : L) { l = LMDev{}; l.lambda = 1e-4; l.nu = 2.0; l.phase = kPhaseLin; l.iter = 1; }
     HIPTRY(h, hipMemcpyAsync(h->W.lm, L.data(), sizeof(LMDev) * L.size(), hipMemcpyHostToDevice, s));
     launch_cam_rt(P, h->cur, s);
-    if (mixed(h)) { enqueue_lin<float>(h); launch_hinv<float>(P, h->W, s); }
-    else { enqueue_lin<double>(h); launch_hinv<double>(P, h->W, s); }
+    if (mixed(h)) { enqueue_lin<float>(h, false); launch_hinv<float>(P, h->W, s); }
+    else { enqueue_lin<double>(h, false); launch_hinv<double>(P, h->W, s); }
     int st;
     float best_ms;
     if ((st = time_schur(h, &best_ms)) != NAV_BA_OK) return st;
@@ -549,6 +555,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     HIPTRY(h, hipSetDevice(h->device));
     HIPTRY(h, hipStreamSynchronize(h->stream));
     if (h->step) { (void)hipGraphExecDestroy(h->step); h->step = nullptr; }
+    if (h->step0) { (void)hipGraphExecDestroy(h->step0); h->step0 = nullptr; }
     h->opt = *o;
     if (h->opt.max_trials <= 0) h->opt.max_trials = 10;
     h->batched = batched;
@@ -1008,6 +1015,7 @@ void nav_ba_destroy(nav_ba_handle* h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->step) (void)hipGraphExecDestroy(h->step);
+    if (h->step0) (void)hipGraphExecDestroy(h->step0);
     for (auto& b : h->bufs)
         if (b.p) (void)hipFree(b.p);
     if (h->hdone) (void)hipHostFree(h->hdone);
@@ -1058,7 +1066,7 @@ int32_t nav_ba_run(nav_ba_handle* h) {
     long long burst = std::max(1, P.n_iter);
     while (steps < max_steps) {
         for (long long i = 0; i < burst && steps < max_steps; ++i, ++steps)
-            if ((st = launch_step(h)) != NAV_BA_OK) return st;
+            if ((st = launch_step(h, steps == 0)) != NAV_BA_OK) return st;
         HIPTRY(h, hipMemcpyAsync(h->hdone, h->W.done_count, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPTRY(h, hipStreamSynchronize(s));
         if (*h->hdone >= P.n_win) break;
@@ -1164,11 +1172,11 @@ int32_t nav_ba_debug_reduced_system(nav_ba_handle* h, double lambda, double* S, 
     L.lambda = lambda;
     L.nu = 2.0;
     L.phase = kPhaseLin;
-    L.iter = 1;  // skips the lambda initialisation
+    L.iter = 1;
     HIPTRY(h, hipMemcpyAsync(h->W.lm, &L, sizeof(LMDev), hipMemcpyHostToDevice, s));
     launch_cam_rt(P, h->cur, s);
-    if (mixed(h)) { enqueue_lin<float>(h); enqueue_reduced_system<float>(h); }
-    else { enqueue_lin<double>(h); enqueue_reduced_system<double>(h); }
+    if (mixed(h)) { enqueue_lin<float>(h, false); enqueue_reduced_system<float>(h); }
+    else { enqueue_lin<double>(h, false); enqueue_reduced_system<double>(h); }
     HIPTRY(h, hipStreamSynchronize(s));
     if (P.n > max_n) return NAV_BA_OK;
     std::vector<double> Sd((size_t)P.n_pad * P.n_pad), gd(P.n_pad);
