@@ -85,143 +85,112 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// The value of v in lane l (l wave-uniform), as a uniform double.
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // In-LDS Cholesky of the SPD tile A (64x64, 256 threads): its lower triangle
-// becomes L and rpiv[j] = 1 / L_jj; the upper triangle holds no data.  Eight
-// 8-column panels with one step of look-ahead, one barrier each.  In step p:
-//   wave 0: lane l holds row c0 + l of panel p (c0 = 8p) and applies panel
-//     p-1 to it (8x8 product with that panel's rows c0..c0+7); the updated
-//     diagonal block goes through LDS to all lanes, which factor it in
-//     registers (the same values in every lane; 1/sqrt by rsq + two Newton
-//     steps); lanes 8.. then solve their rows against it;
-//   waves 1-3 meanwhile apply panel p-1 to the trailing lower triangle right
-//     of panel p on FP64 MFMA (16x16 blocks, K = 8, predicated writes: wave 0
-//     is writing panel p's columns in the same step).
-// Returns true in thread 0 if a pivot was not positive and finite (it is then
-// replaced by 1 so that the values stay finite).
-__device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv) {
-    const int lane = threadIdx.x & 63, wid = wave_id();
+// becomes L and rpiv[j] = 1 / L_jj; the strict upper triangle of the four
+// 16x16 diagonal blocks is left with scratch values, the rest of the upper
+// triangle is unchanged.  Wave w owns columns 16w..16w+15, lane l row l.
+// Four stages, one barrier between stages; in stage s:
+//   waves w >= s apply panel s-1 (final since the last barrier) to their
+//     columns, rows 16w.., on FP64 MFMA (16x16 blocks, K = 16, in place);
+//   wave s then factors its 16 columns in registers.  Pivot j: 1/sqrt(d_j)
+//     by rsq and two Goldschmidt steps; the next pivot is formed from
+//     wave-uniform values read before this pivot's column update,
+//     d_{j+1} = a_{j+1,j+1} - (a_{j+1,j} / sqrt(d_j))^2, so that the chain
+//     through the pivots holds no cross-lane step; columns j+1 and j+2 are
+//     updated with uniform multipliers, the others with the scaled column
+//     broadcast through `scratch` (16 x 64 doubles of LDS).
+// A pivot that is not positive and finite is replaced by 1 (the values stay
+// finite); the return value, the same in every thread, says whether one was.
+__device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv, double* scratch) {
+    const int lane = threadIdx.x & 63, w = wave_id();
     const int r16 = lane & 15, kq = lane >> 4;
+    const int c0 = 16 * w;
+    double (*cb)[64] = reinterpret_cast<double (*)[64]>(scratch);
     bool bad = false;
-    for (int p = 0; p < 8; ++p) {
-        const int c0 = 8 * p, s0 = c0 + 8;  // s0: first row and column right of / below panel p
-        if (wid == 0) {
-            const int row = min(c0 + lane, 63);
-            double x[8];
+    for (int s = 0; s < 4; ++s) {
+        if (s > 0 && w >= s) {
+            const int p = 16 * (s - 1);
+            double bv[4], av[4][4];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) x[c] = A[row][c0 + c];
-            if (p > 0) {
-                double pv[8];
+            for (int kk = 0; kk < 4; ++kk) bv[kk] = A[c0 + r16][p + 4 * kk + kq];
 #pragma unroll
-                for (int m = 0; m < 8; ++m) pv[m] = A[row][c0 - 8 + m];
+            for (int bi = 0; bi < 4; ++bi)
+                if (bi >= w)
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    double t0 = 0.0, t1 = 0.0;
+                    for (int kk = 0; kk < 4; ++kk) av[bi][kk] = A[16 * bi + r16][p + 4 * kk + kq];
 #pragma unroll
-                    for (int m = 0; m < 8; m += 2) {
-                        t0 += pv[m] * A[c0 + c][c0 - 8 + m];
-                        t1 += pv[m + 1] * A[c0 + c][c0 - 7 + m];
-                    }
-                    x[c] -= t0 + t1;
-                }
-                if (lane < 8) {
+            for (int bi = 0; bi < 4; ++bi) {
+                if (bi >= w) {
+                    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
-                }
-                wave_sync();
-            }
-            double a[8][8], rp[8];
+                    for (int kk = 0; kk < 4; ++kk) acc = mfma16x16x4(av[bi][kk], bv[kk], acc);
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int j = 0; j <= i; ++j) a[i][j] = A[c0 + i][c0 + j];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                double d = a[j][j];
-                const bool ok = (d > 0.0) && isfinite(d);
-                bad |= !ok;
-                d = ok ? d : 1.0;
-                double r = __builtin_amdgcn_rsq(d);
-                r = r * (1.5 - 0.5 * d * r * r);
-                r = r * (1.5 - 0.5 * d * r * r);
-                a[j][j] = d * r;
-                rp[j] = r;
-#pragma unroll
-                for (int i = j + 1; i < 8; ++i) a[i][j] *= r;
-#pragma unroll
-                for (int i = j + 1; i < 8; ++i)
-#pragma unroll
-                    for (int c = j + 1; c <= i; ++c) a[i][c] -= a[i][j] * a[c][j];
-            }
-            wave_sync();  // every lane has read the diagonal block before lane 0 overwrites it
-            if (lane == 0) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    rpiv[c0 + i] = rp[i];
-#pragma unroll
-                    for (int j = 0; j <= i; ++j) A[c0 + i][c0 + j] = a[i][j];
-                }
-            }
-            if (lane >= 8 && c0 + lane < 64) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    x[c] *= rp[c];
-#pragma unroll
-                    for (int e = c + 1; e < 8; ++e) x[e] -= x[c] * a[e][c];
-                }
-#pragma unroll
-                for (int c = 0; c < 8; ++c) A[row][c0 + c] = x[c];
-            }
-        } else if (p > 0) {
-            // 16x16 blocks (bi, bj), b0 <= bj <= bi <= 3, numbered row by row;
-            // wave w (1..3) takes blocks w - 1 and w + 2
-            const int q0 = c0 - 8, b0 = s0 >> 4, nb = 4 - b0, nblk = nb * (nb + 1) / 2;
-            int bi[2], bj[2];
-            double av[2][2], bv[2][2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int t = wid - 1 + 3 * u;
-                int i = 0;
-                while ((i + 1) * (i + 2) / 2 <= t) ++i;
-                bi[u] = b0 + i;
-                bj[u] = b0 + (t - i * (i + 1) / 2);
-                if (t < nblk) {
-                    const int ra = 16 * bi[u] + r16, rb = 16 * bj[u] + r16;
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const double va = A[ra][q0 + 4 * h + kq], vb = A[rb][q0 + 4 * h + kq];
-                        av[u][h] = ra >= s0 ? va : 0.0;
-                        bv[u][h] = rb >= s0 ? vb : 0.0;
-                    }
-                }
-            }
-            dbl4 acc[2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int u = 0; u < 2; ++u)
-                    if (wid - 1 + 3 * u < nblk) acc[u] = mfma16x16x4(av[u][h], bv[u][h], acc[u]);
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                if (wid - 1 + 3 * u < nblk) {
-                    const int col = 16 * bj[u] + r16;
-                    double cur[4];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) cur[v] = A[16 * bi[u] + 4 * v + kq][col];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int row = 16 * bi[u] + 4 * v + kq;
-                        if (col >= s0 && row >= col) A[row][col] = cur[v] - acc[u][v];
-                    }
+                    for (int v = 0; v < 4; ++v) A[16 * bi + 4 * v + kq][c0 + r16] -= acc[v];
                 }
             }
         }
-        __syncthreads();
+        if (w == s) {
+            wave_sync();
+            double x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = A[lane][c0 + c];
+            double rp[16];
+            double d = lane_value(x[0], c0);
+            double a1 = lane_value(x[0], c0 + 1), a2 = lane_value(x[0], c0 + 2), ad = lane_value(x[1], c0 + 1);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const bool ok = (d > 0.0) && isfinite(d);
+                bad |= !ok;
+                d = ok ? d : 1.0;
+                double H = __builtin_amdgcn_rsq(d);
+                double g = d * H;
+#pragma unroll
+                for (int it = 0; it < 2; ++it) {
+                    const double e = fma(-g, H, 1.0);
+                    const double gh = 0.5 * g, Hh = 0.5 * H;
+                    g = fma(gh, e, g);
+                    H = fma(Hh, e, H);
+                }
+                rp[j] = H;
+                const double l1 = a1 * H, l2 = a2 * H;
+                const double dn = fma(-l1, l1, ad);
+                x[j] = (lane == c0 + j) ? g : x[j] * H;
+                if (j + 3 < 16) cb[j][lane] = x[j];
+                if (j + 1 < 16) x[j + 1] = fma(-x[j], l1, x[j + 1]);
+                if (j + 2 < 16) x[j + 2] = fma(-x[j], l2, x[j + 2]);
+                if (j + 1 < 16) {
+                    d = dn;
+                    a1 = (j + 2 < 16) ? lane_value(x[j + 1], c0 + j + 2) : 0.0;
+                    a2 = (j + 3 < 16) ? lane_value(x[j + 1], c0 + j + 3) : 0.0;
+                    ad = (j + 2 < 16) ? lane_value(x[j + 2], c0 + j + 2) : 0.0;
+                }
+                if (j + 3 < 16) {
+                    wave_sync();
+#pragma unroll
+                    for (int c = j + 3; c < 16; ++c) x[c] = fma(-x[j], cb[j][c0 + c], x[c]);
+                }
+            }
+            if (lane >= c0) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c) A[lane][c0 + c] = x[c];
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) rpiv[c0 + j] = rp[j];
+            }
+        }
+        if (s < 3) __syncthreads();
     }
-    return bad;
+    return __syncthreads_or(bad);
 }
-
 
 // M = L^-1 for the factor in the lower triangle of A, rpiv = 1 / diag(L), by
 // recursive doubling; M's upper triangle is set to zero.  The strict upper
@@ -416,7 +385,7 @@ __global__ __launch_bounds__(256) void k_chol_first(const double* __restrict__ S
     tile_load(A, S, lda);
     if (lm->phase == kPhaseDone) return;  // a step launched after the solve has finished
     __syncthreads();
-    if (tile_potrf(A, rpiv) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
+    if (tile_potrf(A, rpiv, &M[0][0]) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
     tile_trinv(A, rpiv, M);
     tile_store(M, Li, 64);
     tile_forward(M, nullptr, 0, g, y, rv);
@@ -471,7 +440,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
 #pragma unroll
             for (int v = 0; v < 4; ++v) Mk[16 * w + 4 * v + kq][16 * b + r16] = cur[b][v] - acc[b][v];
         __syncthreads();
-        if (tile_potrf(Mk, rpiv) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
+        if (tile_potrf(Mk, rpiv, &Bi[0][0]) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
         tile_trinv(Mk, rpiv, Bj);
         tile_store(Bj, Li + (size_t)4096 * i, 64);
         __syncthreads();  // fpart[i][k] (written above by this workgroup) is read back by the forward step
