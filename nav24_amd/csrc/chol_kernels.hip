@@ -85,6 +85,103 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// M[C..C+15][C..C+15] = L^-1 of the 16x16 diagonal block of the factor in A
+// (lower triangle, rpiv = 1 / diag), by one wave: its two 8x8 diagonal blocks
+// by substitution (one lane per column), then M21 = -M22 (L21 M11) on FP64
+// MFMA (8x8 products, zero padded to 16x16).  The block's upper right 8x8
+// quarter is not written (it must already hold zeros).
+__device__ __forceinline__ void wave_inv16(const TileLds& A, const double* rpiv, TileLds& M, int C) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, kq = lane >> 4, r8 = lane & 7;
+    const int R = C + 8;
+    if (lane < 16) {
+        const int b = C + 8 * (lane >> 3), j = lane & 7;
+        double d[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) d[i] = (i == j) ? rpiv[b + j] : 0.0;
+#pragma unroll
+        for (int i = 1; i < 8; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < i; ++k) s += A[b + i][b + k] * d[k];  // d[k] = 0 for k < j
+            if (i > j) d[i] = -s * rpiv[b + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) M[b + i][b + j] = d[i];
+    }
+    wave_sync();
+    // T = L21 M11 (8x8): rows i = 4v + kq < 8, columns r16 < 8 of the 16x16
+    // product (operand rows and columns 8..15 zero; row 8 + r8 read instead)
+    const bool in = r16 < 8;
+    dbl4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m0 = 0; m0 < 8; m0 += 4) {
+        const double va = A[R + r8][C + m0 + kq], vb = M[C + m0 + kq][C + r16];
+        t = mfma16x16x4(in ? va : 0.0, in ? vb : 0.0, t);
+    }
+    if (in) {
+        M[R + kq][C + r16] = t[0];
+        M[R + 4 + kq][C + r16] = t[1];
+    }
+    wave_sync();
+    dbl4 u = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m0 = 0; m0 < 8; m0 += 4) {
+        const double va = M[R + r8][R + m0 + kq], vb = M[R + m0 + kq][C + r16];
+        u = mfma16x16x4(in ? va : 0.0, in ? vb : 0.0, u);
+    }
+    wave_sync();
+    if (in) {
+        M[R + kq][C + r16] = -u[0];
+        M[R + 4 + kq][C + r16] = -u[1];
+    }
+    wave_sync();
+}
+
+// 16x16 block product of one wave, acc += X[xr.., xc..] Y[yr.., yc..]; the
+// result's element v in lane (r16, kq) is row 4v + kq, column r16.
+__device__ __forceinline__ dbl4 blk_mul(const TileLds& X, int xr, int xc, const TileLds& Y, int yr, int yc, dbl4 acc) {
+    const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+    double a[4], b[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        a[kk] = X[xr + r16][xc + 4 * kk + kq];
+        b[kk] = Y[yr + 4 * kk + kq][yc + r16];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) acc = mfma16x16x4(a[kk], b[kk], acc);
+    return acc;
+}
+
+// Block (r, c) of M (16x16) <- sign * acc, in the layout of blk_mul.
+__device__ __forceinline__ void blk_put(TileLds& M, int r, int c, dbl4 acc, double sign) {
+    const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) M[16 * r + 4 * v + kq][16 * c + r16] = sign * acc[v];
+}
+
+// Block (i, j) of the tile inverse, M_ij = -M_ii T_ij, where block (i, j) of M
+// holds T_ij = sum_{j<=k<i} L_ik M_kj (one wave; in place).
+__device__ __forceinline__ void blk_finish(TileLds& M, int i, int j) {
+    const dbl4 z = {0.0, 0.0, 0.0, 0.0};
+    const dbl4 t = blk_mul(M, 16 * i, 16 * i, M, 16 * i, 16 * j, z);
+    wave_sync();
+    blk_put(M, i, j, t, -1.0);
+}
+
+// Block (i, j) of M += L_ik M_kj (or = when first), one wave.
+__device__ __forceinline__ void blk_term(const TileLds& A, TileLds& M, int i, int k, int j, bool first) {
+    const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    if (!first) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = M[16 * i + 4 * v + kq][16 * j + r16];
+    }
+    acc = blk_mul(A, 16 * i, 16 * k, M, 16 * k, 16 * j, acc);
+    wave_sync();
+    blk_put(M, i, j, acc, 1.0);
+}
+
 // The value of v in lane l (l wave-uniform), as a uniform double.
 __device__ __forceinline__ double lane_value(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -93,9 +190,10 @@ __device__ __forceinline__ double lane_value(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// In-LDS Cholesky of the SPD tile A (64x64, 256 threads): its lower triangle
-// becomes L and rpiv[j] = 1 / L_jj; the strict upper triangle of the four
-// 16x16 diagonal blocks is left with scratch values, the rest of the upper
+// In-LDS Cholesky of the SPD tile A (64x64, 256 threads) and the inverse of
+// its factor: the lower triangle of A becomes L, rpiv[j] = 1 / L_jj and
+// M = L^-1 (upper triangle zero).  The strict upper triangle of A's four
+// 16x16 diagonal blocks is left with scratch values, the rest of A's upper
 // triangle is unchanged.  Wave w owns columns 16w..16w+15, lane l row l.
 // Four stages, one barrier between stages; in stage s:
 //   waves w >= s apply panel s-1 (final since the last barrier) to their
@@ -106,16 +204,34 @@ __device__ __forceinline__ double lane_value(double v, int l) {
 //     d_{j+1} = a_{j+1,j+1} - (a_{j+1,j} / sqrt(d_j))^2, so that the chain
 //     through the pivots holds no cross-lane step; columns j+1 and j+2 are
 //     updated with uniform multipliers, the others with the scaled column
-//     broadcast through `scratch` (16 x 64 doubles of LDS).
-// A pivot that is not positive and finite is replaced by 1 (the values stay
-// finite); the return value, the same in every thread, says whether one was.
-__device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv, double* scratch) {
+//     broadcast through `scratch` (16 x 64 doubles of LDS, not M);
+//   wave s-1, whose panel is final, forms the inverse's block row s-1 in 16x16
+//     blocks, M_ii = L_ii^-1 and M_ij = -M_ii T_ij, and adds its terms to
+//     the sums T_mj = sum_{j<=k<m} L_mk M_kj of the rows below (kept in M's
+//     blocks (m, j)); stage 0's idle waves clear M's upper triangle.
+// After the last stage wave 3 forms block row 3.  A pivot that is not
+// positive and finite is replaced by 1 (the values stay finite); the return
+// value, the same in every thread, says whether one was.
+__device__ __forceinline__ bool tile_potrf_inv(TileLds& A, double* rpiv, double* scratch, TileLds& M) {
     const int lane = threadIdx.x & 63, w = wave_id();
     const int r16 = lane & 15, kq = lane >> 4;
     const int c0 = 16 * w;
     double (*cb)[64] = reinterpret_cast<double (*)[64]>(scratch);
     bool bad = false;
     for (int s = 0; s < 4; ++s) {
+        if (s == 0 && w > 0) {
+            for (int idx = (int)threadIdx.x - 64; idx < 4096; idx += 192) {
+                const int i = idx >> 6, j = idx & 63;
+                if ((j >> 3) > (i >> 3)) M[i][j] = 0.0;
+            }
+        }
+        if (s > 0 && w == s - 1) {
+            // block row s-1 of the inverse and its terms of the rows below
+            wave_inv16(A, rpiv, M, 16 * w);
+            for (int j = 0; j < w; ++j) blk_finish(M, w, j);
+            for (int m = w + 1; m < 4; ++m)
+                for (int j = 0; j <= w; ++j) blk_term(A, M, m, w, j, j == w);
+        }
         if (s > 0 && w >= s) {
             const int p = 16 * (s - 1);
             double bv[4], av[4][4];
@@ -189,108 +305,13 @@ __device__ __forceinline__ bool tile_potrf(TileLds& A, double* rpiv, double* scr
         }
         if (s < 3) __syncthreads();
     }
-    return __syncthreads_or(bad);
-}
-
-// M = L^-1 for the factor in the lower triangle of A, rpiv = 1 / diag(L), by
-// recursive doubling; M's upper triangle is set to zero.  The strict upper
-// triangle of A is used as scratch (its rows 0..31, columns 32..63).
-//   wave q, on its 16x16 diagonal block: the two 8x8 diagonal blocks by
-//     substitution (one lane per column), then M21 = -M22 (L21 M11) of the
-//     block with 8x8 products on FP64 MFMA (zero padded to 16x16);
-//   waves 0 and 1: the same for the two 32x32 diagonal blocks (16x16 products);
-//   all waves: the same for the whole tile (32x32 products, one 16x16 block
-//     per wave).
-// Four barriers; the hand-offs inside one wave need none.
-__device__ __forceinline__ void tile_trinv(TileLds& A, const double* rpiv, TileLds& M) {
-    const int tid = threadIdx.x, lane = tid & 63, wid = wave_id();
-    const int r16 = lane & 15, kq = lane >> 4, r8 = lane & 7;
-    for (int idx = tid; idx < 4096; idx += 256) {
-        const int i = idx >> 6, j = idx & 63;
-        if ((j >> 3) > (i >> 3)) M[i][j] = 0.0;
-    }
-    {
-        const int C = 16 * wid, R = C + 8;
-        if (lane < 16) {
-            const int b = C + 8 * (lane >> 3), j = lane & 7;
-            double d[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) d[i] = (i == j) ? rpiv[b + j] : 0.0;
-#pragma unroll
-            for (int i = 1; i < 8; ++i) {
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < i; ++k) s += A[b + i][b + k] * d[k];  // d[k] = 0 for k < j
-                if (i > j) d[i] = -s * rpiv[b + i];
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i) M[b + i][b + j] = d[i];
-        }
-        wave_sync();
-        // T = L21 M11 (8x8): rows i = 4v + kq < 8, columns r16 < 8 of the 16x16
-        // product (operand rows and columns 8..15 zero; row 8 + r8 read instead)
-        const bool in = r16 < 8;
-        dbl4 t = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m0 = 0; m0 < 8; m0 += 4) {
-            const double va = A[R + r8][C + m0 + kq], vb = M[C + m0 + kq][C + r16];
-            t = mfma16x16x4(in ? va : 0.0, in ? vb : 0.0, t);
-        }
-        if (in) {
-            M[R + kq][C + r16] = t[0];
-            M[R + 4 + kq][C + r16] = t[1];
-        }
-        wave_sync();
-        dbl4 u = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m0 = 0; m0 < 8; m0 += 4) {
-            const double va = M[R + r8][R + m0 + kq], vb = M[R + m0 + kq][C + r16];
-            u = mfma16x16x4(in ? va : 0.0, in ? vb : 0.0, u);
-        }
-        wave_sync();
-        if (in) {
-            M[R + kq][C + r16] = -u[0];
-            M[R + 4 + kq][C + r16] = -u[1];
-        }
+    bad = __syncthreads_or(bad);
+    if (w == 3) {
+        wave_inv16(A, rpiv, M, 48);
+        for (int j = 0; j < 3; ++j) blk_finish(M, 3, j);
     }
     __syncthreads();
-    if (wid < 2) {
-        const int C = 32 * wid, R = C + 16;
-        dbl4 t = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m0 = 0; m0 < 16; m0 += 4) t = mfma16x16x4(A[R + r16][C + m0 + kq], M[C + m0 + kq][C + r16], t);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) M[R + 4 * v + kq][C + r16] = t[v];
-        wave_sync();
-        dbl4 u = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m0 = 0; m0 < 16; m0 += 4) u = mfma16x16x4(M[R + r16][R + m0 + kq], M[R + m0 + kq][C + r16], u);
-        wave_sync();
-#pragma unroll
-        for (int v = 0; v < 4; ++v) M[R + 4 * v + kq][C + r16] = -u[v];
-    }
-    __syncthreads();
-    const int bi = wid >> 1, bj = wid & 1;
-    {
-        // T = L21 M11 (32x32) into A's strict upper triangle: T[i][j] at A[i][32 + j]
-        dbl4 t = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m0 = 0; m0 < 32; m0 += 4)  // M11[m][16bj + j] = 0 for m < 16bj
-            if (m0 >= 16 * bj) t = mfma16x16x4(A[32 + 16 * bi + r16][m0 + kq], M[m0 + kq][16 * bj + r16], t);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) A[16 * bi + 4 * v + kq][32 + 16 * bj + r16] = t[v];
-    }
-    __syncthreads();
-    {
-        dbl4 u = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m0 = 0; m0 < 32; m0 += 4)  // M22[i][m] = 0 for m > i
-            if (m0 < 16 * bi + 16)
-                u = mfma16x16x4(M[32 + 16 * bi + r16][32 + m0 + kq], A[m0 + kq][32 + 16 * bj + r16], u);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) M[32 + 16 * bi + 4 * v + kq][16 * bj + r16] = -u[v];
-    }
-    __syncthreads();
+    return bad;
 }
 
 // acc[b] = block (wave, b) of X Y^T in 16x16 blocks; with kTriY, Y is lower
@@ -380,13 +401,13 @@ __global__ __launch_bounds__(256) void k_chol_first(const double* __restrict__ S
                                                     double* __restrict__ y, LMDev* lm) {
     __shared__ TileLds A;
     __shared__ TileLds M;
+    __shared__ double scratch[16 * 64];
     __shared__ double rv[64];
     __shared__ double rpiv[64];
     tile_load(A, S, lda);
     if (lm->phase == kPhaseDone) return;  // a step launched after the solve has finished
     __syncthreads();
-    if (tile_potrf(A, rpiv, &M[0][0]) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
-    tile_trinv(A, rpiv, M);
+    if (tile_potrf_inv(A, rpiv, scratch, M) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
     tile_store(M, Li, 64);
     tile_forward(M, nullptr, 0, g, y, rv);
 }
@@ -440,8 +461,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
 #pragma unroll
             for (int v = 0; v < 4; ++v) Mk[16 * w + 4 * v + kq][16 * b + r16] = cur[b][v] - acc[b][v];
         __syncthreads();
-        if (tile_potrf(Mk, rpiv, &Bi[0][0]) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
-        tile_trinv(Mk, rpiv, Bj);
+        if (tile_potrf_inv(Mk, rpiv, &Bi[0][0], Bj) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
         tile_store(Bj, Li + (size_t)4096 * i, 64);
         __syncthreads();  // fpart[i][k] (written above by this workgroup) is read back by the forward step
         tile_forward(Bj, fpart + (size_t)i * nt * 64, k + 1, g + 64 * i, y + 64 * i, rv);
