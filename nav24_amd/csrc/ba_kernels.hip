@@ -355,6 +355,18 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
     const int f = blockIdx.x;
     const int win = P.cam_win[P.free_cam[f]];
     if (!running(lm, win)) return;
+    if (!P.batched && !P.use_blocks) {
+        // the padding rows n..n_pad-1 of the tiled system (identity): the
+        // factorisation reads them, and a failed trial can leave non-finite
+        // values there (a zero row times an overflowed factor entry)
+        const int pr = P.n_pad - P.n;
+        const long long tot = (long long)pr * P.n_pad;
+        for (long long q = ((long long)blockIdx.x * gridDim.y + blockIdx.y) * kBlock + threadIdx.x; q < tot;
+             q += (long long)gridDim.x * gridDim.y * kBlock) {
+            const int r = P.n + (int)(q / P.n_pad), c = (int)(q % P.n_pad);
+            if (c <= r) S[(size_t)r * P.n_pad + c] = (r == c) ? 1.0 : 0.0;
+        }
+    }
     const int fc0 = P.win_fc0[win];
     const int lf = f - fc0;
     double* Sw = S + P.win_S[win];
