@@ -168,22 +168,39 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
         const int bt1 = P.seg_batch[seg + 1];
         for (int bt = P.seg_batch[seg]; bt < bt1; ++bt) {
             const int e0 = P.batch_ptr[bt], nrow = (P.batch_ptr[bt + 1] - e0) / kWave;
-            const int2 p0 = P.pairs[e0 + lane];
+            // the pair entries of a group of kRunLen rows, then their weights, each
+            // issued together (one round trip each per group, not per pair)
+            int2 pe[kRunLen];
+            double pw[kRunLen];
+#pragma unroll
+            for (int j = 0; j < kRunLen; ++j) pe[j] = j < nrow ? P.pairs[e0 + j * kWave + lane] : int2{-1, 0};
+            const int2 p0 = pe[0];
             const bool has = p0.x >= 0;
             const int s = (p0.y >> 6) & 255, cb = has ? p0.y >> 14 : 0;
+#pragma unroll
+            for (int j = 0; j < kRunLen; ++j) pw[j] = pe[j].x >= 0 ? (double)w[pe[j].x] : 0.0;
             double Rb[9], tb[3];
             cam_get<kCamLds>(camt, cam, cb, cam0, Rb, tb);
             double sacc[36];
 #pragma unroll
             for (int e = 0; e < 36; ++e) sacc[e] = 0.0;
-            for (int j = 0; j < nrow; ++j) {
-                const int2 pr = j == 0 ? p0 : P.pairs[e0 + j * kWave + lane];
+            for (int j0 = 0; j0 < nrow; j0 += kRunLen) {
+                if (j0 > 0) {  // runs longer than kRunLen (a forced run length)
+#pragma unroll
+                    for (int j = 0; j < kRunLen; ++j)
+                        pe[j] = j0 + j < nrow ? P.pairs[e0 + (j0 + j) * kWave + lane] : int2{-1, 0};
+#pragma unroll
+                    for (int j = 0; j < kRunLen; ++j) pw[j] = pe[j].x >= 0 ? (double)w[pe[j].x] : 0.0;
+                }
+#pragma unroll
+                for (int jj = 0; jj < kRunLen; ++jj) {
+                const int2 pr = pe[jj];
                 if (pr.x < 0) continue;
                 const double* wb = wbuf + (pr.y & 63) * SchurLayout<T>::kWbuf;
                 EdgeLin<T> Lb;
                 edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
                 // the Huber weight of b folded into J_X,b: W_a (w_b J_X,b)^T J_xi,b
-                const double wgt = (double)w[pr.x];
+                const double wgt = pw[jj];
                 double JXb[6];
 #pragma unroll
                 for (int q = 0; q < 6; ++q) JXb[q] = wgt * (double)Lb.JX[q];
@@ -208,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                         else if (c == 3) a = fma(M0[r], B0[c], a);
                         else a = fma(M1[r], B1[c], fma(M0[r], B0[c], a));
                     }
+                }
             }
             if (has) {
 #pragma unroll
