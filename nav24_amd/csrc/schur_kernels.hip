@@ -165,15 +165,25 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
         wave_fence();
         // ---- phase B: this wave's pair batches of this round
         const int seg = it * P.schur_segs + round * (kBlock / kWave) + wid;
-        const int bt1 = P.seg_batch[seg + 1];
-        for (int bt = P.seg_batch[seg]; bt < bt1; ++bt) {
-            const int e0 = P.batch_ptr[bt], nrow = (P.batch_ptr[bt + 1] - e0) / kWave;
-            // the pair entries of a group of kRunLen rows, then their weights, each
-            // issued together (one round trip each per group, not per pair)
-            int2 pe[kRunLen];
-            double pw[kRunLen];
+        const int bt0 = P.seg_batch[seg], bt1 = P.seg_batch[seg + 1];
+        // the pair entries of a group of kRunLen rows, then their weights, each
+        // issued together (one round trip each per group, not per pair); the
+        // next batch's first entries are loaded while a batch is formed
+        int2 pe[kRunLen];
+        if (bt0 < bt1) {
+            const int e0 = P.batch_ptr[bt0], nrow = (P.batch_ptr[bt0 + 1] - e0) / kWave;
 #pragma unroll
             for (int j = 0; j < kRunLen; ++j) pe[j] = j < nrow ? P.pairs[e0 + j * kWave + lane] : int2{-1, 0};
+        }
+        for (int bt = bt0; bt < bt1; ++bt) {
+            const int e0 = P.batch_ptr[bt], nrow = (P.batch_ptr[bt + 1] - e0) / kWave;
+            double pw[kRunLen];
+            int2 pn[kRunLen];
+            if (bt + 1 < bt1) {
+                const int f0 = P.batch_ptr[bt + 1], nr1 = (P.batch_ptr[bt + 2] - f0) / kWave;
+#pragma unroll
+                for (int j = 0; j < kRunLen; ++j) pn[j] = j < nr1 ? P.pairs[f0 + j * kWave + lane] : int2{-1, 0};
+            }
             const int2 p0 = pe[0];
             const bool has = p0.x >= 0;
             const int s = (p0.y >> 6) & 255, cb = has ? p0.y >> 14 : 0;
@@ -226,6 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                         else a = fma(M1[r], B1[c], fma(M0[r], B0[c], a));
                     }
                 }
+            }
+            if (bt + 1 < bt1) {
+#pragma unroll
+                for (int j = 0; j < kRunLen; ++j) pe[j] = pn[j];
             }
             if (has) {
 #pragma unroll
