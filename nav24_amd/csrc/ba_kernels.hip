@@ -170,10 +170,13 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
         uk[u] = P.c_uv[2 * (size_t)k];
         vk[u] = P.c_uv[2 * (size_t)k + 1];
     }
+    D4 xs[kU];  // the point gathers, all in flight before the first is used
+#pragma unroll
+    for (int u = 0; u < kU; ++u) xs[u] = X[pk[u]];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
         if (a0 + (int)threadIdx.x + kBlock * u >= a1) break;
-        const D4 xp = X[pk[u]];
+        const D4 xp = xs[u];
         EdgeLin<T> L;
         const double s = edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, uk[u], vk[u], L);
         const double wa = (double)huber_weight<T>(s, P);
@@ -623,12 +626,15 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restr
         uu[q] = P.o_uv[2 * (size_t)a];
         vv[q] = P.o_uv[2 * (size_t)a + 1];
     }
+    D4 xs[kU];  // the point gathers, all in flight before the first is used
+#pragma unroll
+    for (int q = 0; q < kU; ++q) xs[q] = X[pi[q]];
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < kU; ++q) {
         double R[9], tt[3];
         load_cam(cam, ci[q], R, tt);
-        const D4 xp = X[pi[q]];
+        const D4 xp = xs[q];
         double e0, e1, zc, w;
         const double s = edge_residual(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, uu[q], vv[q], e0, e1, zc);
         const double rho = robustify(s, P.robust, P.delta, w);
