@@ -492,6 +492,11 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
 // problem's one window) and the camera part of x'(lambda x + b).
 constexpr int kBackThreads = 512;
 
+// NP: 128-column passes of the widest update (64 (nt - 1) columns), so that
+// the L loads of a step fit in registers.
+constexpr int kBackPassMax = (64 * (kBackSingleMax - 1) + 127) / 128;
+
+template <int NP>
 __global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const double* __restrict__ Lm,
                                                             const double* __restrict__ Li, int lda, int nt,
                                                             const double* __restrict__ y, double* __restrict__ dp,
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const 
                                                             CamRT* __restrict__ cn, double* __restrict__ red_scale) {
     extern __shared__ double rs[];  // [64 nt]
     __shared__ double part[8][64];
-    __shared__ double red[4][128];
+    __shared__ double red[4][128 * NP];
     __shared__ double xk[64];
     __shared__ double shs[kBackThreads / kWave];
     const int tid = threadIdx.x, n = 64 * nt;
@@ -509,29 +514,45 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const 
     if (fail) {
         for (int i = tid; i < n; i += kBackThreads) dp[i] = 0.0;
     } else {
+    const int c = tid & 63, grp = tid >> 6;   // x_k: entry c, rows grp + 8u
+    const int pc = tid & 127, mq = tid >> 7;  // updates: column pc of each pass, rows 16 mq + u
+    // M_k of the next step is loaded during the current one
+    double mv[8];
+    {
+        const double* Mk = Li + (size_t)4096 * (nt - 1);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mv[u] = Mk[(grp + 8 * u) * 64 + c];
+    }
     for (int i = tid; i < n; i += kBackThreads) rs[i] = y[i];
     __syncthreads();
-    const int c = tid & 63, grp = tid >> 6;   // x_k: entry c, rows grp + 8u
-    const int pc = tid & 127, mq = tid >> 7;  // updates: column pc of a pass, rows 16 mq + u
     for (int k = nt - 1; k >= 0; --k) {
         const int ncols = 64 * k;
         const double* Lk = Lm + (size_t)(64 * k) * lda;
-        double lv[16];
-        if (pc < ncols) {
+        // every L_ki load of the step, all passes, before x_k is formed
+        double lv[NP][16];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) lv[u] = Lk[(size_t)(16 * mq + u) * lda + pc];
+        for (int q = 0; q < NP; ++q) {
+            const int col = 128 * q + pc;
+            if (col < ncols) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) lv[q][u] = Lk[(size_t)(16 * mq + u) * lda + col];
+            }
         }
         {
-            const double* Mk = Li + (size_t)4096 * k;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
             for (int u = 0; u < 8; u += 4) {
-                s0 += Mk[(grp + 8 * u) * 64 + c] * rs[64 * k + grp + 8 * u];
-                s1 += Mk[(grp + 8 * u + 8) * 64 + c] * rs[64 * k + grp + 8 * u + 8];
-                s2 += Mk[(grp + 8 * u + 16) * 64 + c] * rs[64 * k + grp + 8 * u + 16];
-                s3 += Mk[(grp + 8 * u + 24) * 64 + c] * rs[64 * k + grp + 8 * u + 24];
+                s0 += mv[u] * rs[64 * k + grp + 8 * u];
+                s1 += mv[u + 1] * rs[64 * k + grp + 8 * u + 8];
+                s2 += mv[u + 2] * rs[64 * k + grp + 8 * u + 16];
+                s3 += mv[u + 3] * rs[64 * k + grp + 8 * u + 24];
             }
             part[grp][c] = (s0 + s1) + (s2 + s3);
+        }
+        if (k > 0) {
+            const double* Mk = Li + (size_t)4096 * (k - 1);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) mv[u] = Mk[(grp + 8 * u) * 64 + c];
         }
         __syncthreads();
         if (tid < 64) {
@@ -542,29 +563,25 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const 
             dp[64 * k + tid] = x;
         }
         __syncthreads();
-        for (int c0 = 0; c0 < ncols; c0 += 128) {
-            const int col = c0 + pc;
-            double s = 0.0;
-            if (col < ncols) {
-                if (c0 > 0) {
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) lv[u] = Lk[(size_t)(16 * mq + u) * lda + col];
-                }
+        for (int q = 0; q < NP; ++q) {
+            const int col = 128 * q + pc;
+            if (col < ncols) {
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
                 for (int u = 0; u < 16; u += 4) {
-                    s0 += lv[u] * xk[16 * mq + u];
-                    s1 += lv[u + 1] * xk[16 * mq + u + 1];
-                    s2 += lv[u + 2] * xk[16 * mq + u + 2];
-                    s3 += lv[u + 3] * xk[16 * mq + u + 3];
+                    s0 += lv[q][u] * xk[16 * mq + u];
+                    s1 += lv[q][u + 1] * xk[16 * mq + u + 1];
+                    s2 += lv[q][u + 2] * xk[16 * mq + u + 2];
+                    s3 += lv[q][u + 3] * xk[16 * mq + u + 3];
                 }
-                s = (s0 + s1) + (s2 + s3);
+                red[mq][col] = (s0 + s1) + (s2 + s3);
             }
-            red[mq][pc] = s;
-            __syncthreads();
-            if (tid < 128 && col < ncols) rs[col] -= (red[0][pc] + red[1][pc]) + (red[2][pc] + red[3][pc]);
-            __syncthreads();
         }
+        __syncthreads();
+        for (int col = tid; col < ncols; col += kBackThreads)
+            rs[col] -= (red[0][col] + red[1][col]) + (red[2][col] + red[3][col]);
+        __syncthreads();
     }
     }
     const double lambda = lm->lambda;
@@ -1054,8 +1071,11 @@ int prepare_cholesky(const DevProblem& P) {
         // the tiled path's backward step holds the right-hand sides in LDS
         const size_t smem = sizeof(double) * (size_t)std::max(P.n_pad, 64);
         if (smem + 9 * 1024 > 160 * 1024) return -1;
-        return hipFuncSetAttribute((const void*)k_chol_back, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) ==
-                       hipSuccess ? 0 : -1;
+        const void* kerns[] = {(const void*)k_chol_back<1>, (const void*)k_chol_back<2>, (const void*)k_chol_back<3>,
+                               (const void*)k_chol_back<4>, (const void*)k_chol_back<kBackPassMax>};
+        for (const void* kf : kerns)
+            if (hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) != hipSuccess) return -1;
+        return 0;
     }
     const size_t smem = cholesky_blocks_smem(P.n_fc);
     if (smem > 160 * 1024) return -1;
@@ -1080,7 +1100,10 @@ void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevSt
                            (const double*)W.g, W.y, W.fpart, nt, W.lm);
     }
     if (nt <= kBackSingleMax) {
-        hipLaunchKernelGGL(k_chol_back, dim3(1), dim3(kBackThreads), sizeof(double) * (size_t)lda, st, P,
+        const int np = std::max(1, (64 * (nt - 1) + 127) / 128);
+        auto kern = np == 1 ? k_chol_back<1> : np == 2 ? k_chol_back<2> : np == 3 ? k_chol_back<3>
+                  : np == 4 ? k_chol_back<4> : k_chol_back<kBackPassMax>;
+        hipLaunchKernelGGL(kern, dim3(1), dim3(kBackThreads), sizeof(double) * (size_t)lda, st, P,
                            (const double*)W.Lm, (const double*)W.Li, lda, nt, (const double*)W.y, W.dp,
                            (const LMDev*)W.lm, (const PoseQT*)cur.pose, (const double*)W.Hc, cand.pose, cand.cam,
                            W.red_scale);
