@@ -176,25 +176,41 @@ __device__ __forceinline__ void load_cam(const CamRT* cam, int c, T R[9], T t[3]
     for (int k = 0; k < 3; ++k) t[k] = (T)p[9 + k];
 }
 
+// The inputs of camera c that do not depend on the step: its pose, free index
+// and b_p (zero for a fixed camera); loaded ahead of the step by some callers.
+struct CamIn {
+    PoseQT pose;
+    double bp[6];
+    int f;
+};
+
+__device__ __forceinline__ CamIn camera_inputs(const DevProblem& P, const PoseQT* __restrict__ pose,
+                                              const double* __restrict__ Hc, int c) {
+    CamIn ci;
+    ci.f = P.cam_free[c];
+    ci.pose = pose[c];
+    const double* bp = Hc + 28 * (size_t)(ci.f >= 0 ? ci.f : 0) + 21;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) ci.bp[i] = ci.f >= 0 ? bp[i] : 0.0;
+    return ci;
+}
+
 // Candidate pose of camera c, T <- exp(xi) T with xi = step(f, 0..5) for a
 // free camera f (a copy for a fixed camera or a failed factorisation), written
 // to pn / cn; returns the camera's term of x'(lambda x + b).
 template <class Step>
-__device__ __forceinline__ double camera_candidate(const DevProblem& P, const PoseQT* __restrict__ pose,
-                                                   const double* __restrict__ Hc, double lambda, int fail, int c,
-                                                   Step step, PoseQT* __restrict__ pn, CamRT* __restrict__ cn) {
-    const int f = P.cam_free[c];
-    const PoseQT in = pose[c];
-    PoseQT o = in;
+__device__ __forceinline__ double camera_candidate(const CamIn& ci, double lambda, int fail, int c, Step step,
+                                                   PoseQT* __restrict__ pn, CamRT* __restrict__ cn) {
+    const int f = ci.f;
+    PoseQT o = ci.pose;
     double sc = 0.0;
     if (f >= 0 && !fail) {
         double xi[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) xi[i] = step(f, i);
-        se3_oplus(xi, in, o);
-        const double* bp = Hc + 28 * (size_t)f + 21;
+        se3_oplus(xi, ci.pose, o);
 #pragma unroll
-        for (int i = 0; i < 6; ++i) sc += xi[i] * (lambda * xi[i] + bp[i]);
+        for (int i = 0; i < 6; ++i) sc += xi[i] * (lambda * xi[i] + ci.bp[i]);
     }
     pn[c] = o;
     CamRT r;

@@ -590,7 +590,8 @@ __global__ __launch_bounds__(256) void k_update_cam(DevProblem P, const PoseQT* 
     const int fail = lm[win].fail;
     double sc = 0.0;
     for (int c = P.win_cam0[win] + threadIdx.x; c < P.win_cam0[win + 1]; c += 256)
-        sc += camera_candidate(P, pose, Hc, lambda, fail, c, [&](int f, int i) { return dp[6 * (size_t)f + i]; }, pn, cn);
+        sc += camera_candidate(camera_inputs(P, pose, Hc, c), lambda, fail, c,
+                               [&](int f, int i) { return dp[6 * (size_t)f + i]; }, pn, cn);
     const double s = block_sum<double, 256>(sc, sh);
     if (threadIdx.x == 0) red_scale[win] = s;
 }

@@ -509,6 +509,9 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const 
     __shared__ double xk[64];
     __shared__ double shs[kBackThreads / kWave];
     const int tid = threadIdx.x, n = 64 * nt;
+    // the camera inputs (pose, b_p) of the first kBackThreads cameras load
+    // while the solve runs
+    const CamIn cin = camera_inputs(P, pose, Hc, max(0, min(tid, P.n_cam - 1)));
     if (lm->phase == kPhaseDone) return;
     const int fail = lm->fail;
     if (fail) {
@@ -587,7 +590,8 @@ __global__ __launch_bounds__(kBackThreads) void k_chol_back(DevProblem P, const 
     const double lambda = lm->lambda;
     double sc = 0.0;
     for (int c = tid; c < P.n_cam; c += kBackThreads)
-        sc += camera_candidate(P, pose, Hc, lambda, fail, c, [&](int f, int i) { return rs[6 * f + i]; }, pn, cn);
+        sc += camera_candidate(c == tid ? cin : camera_inputs(P, pose, Hc, c), lambda, fail, c,
+                               [&](int f, int i) { return rs[6 * f + i]; }, pn, cn);
     const double s = block_sum<double, kBackThreads>(sc, shs);
     if (tid == 0) red_scale[0] = s;
 }
