@@ -349,12 +349,14 @@ __global__ void k_fill_system(DevProblem P, double* __restrict__ S, double* __re
 // S_ff = Hpp + lambda I - sum, S_gf = -sum^T (lower triangle), and write
 // zeros to the row's blocks without co-visibility (the factorisation of the
 // previous trial left values there); part 0 also forms g = bp - coef.
-constexpr int kReduceParts = 4;
+constexpr int kReduceParts = 8;
+constexpr int kReduceStage = 2048;  // item offsets of one S row staged in LDS
 
 __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const double* __restrict__ spart,
                                                          const double* __restrict__ Hc, const LMDev* lm,
                                                          double* __restrict__ S, double* __restrict__ gv) {
     __shared__ double cpart[4][6];
+    __shared__ long long pofs_s[kReduceStage];
     const int f = blockIdx.x;
     const int win = P.cam_win[P.free_cam[f]];
     if (!running(lm, win)) return;
@@ -382,21 +384,37 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
     const int nent = ncols * 36;
     const int per = (nent + kReduceParts - 1) / kReduceParts;
     const int e0 = blockIdx.y * per, e1 = min(nent, e0 + per);
+    // the offsets of the row's item partials, staged in LDS once (the sums
+    // below then have no dependent global load per chunk)
+    const int nitems = P.row_sitem_ptr[f + 1] - base;
+    const bool staged = nitems <= kReduceStage;
+    if (staged)
+        for (int q = threadIdx.x; q < nitems; q += kBlock) pofs_s[q] = P.sitem_pofs[base + q];
+    __syncthreads();
     for (int idx = e0 + threadIdx.x; idx < e1; idx += kBlock) {
         const int s = idx / 36, e = idx % 36;
         const int gr = s / kMaxSlots, sl = s % kMaxSlots;
-        // chunks summed in a fixed order by four interleaved partial sums (loads in flight)
-        const int it0 = base + gr * nch;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        // chunks summed in a fixed order by eight interleaved partial sums (loads in flight)
+        const int q0 = gr * nch;
+        const double* sp = spart + 36 * sl + e;
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = 0.0;
         int ch = 0;
-        for (; ch + 4 <= nch; ch += 4) {
-            s0 += spart[P.sitem_pofs[it0 + ch] + 36 * sl + e];
-            s1 += spart[P.sitem_pofs[it0 + ch + 1] + 36 * sl + e];
-            s2 += spart[P.sitem_pofs[it0 + ch + 2] + 36 * sl + e];
-            s3 += spart[P.sitem_pofs[it0 + ch + 3] + 36 * sl + e];
+        if (staged) {
+            for (; ch + 8 <= nch; ch += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] += sp[pofs_s[q0 + ch + u]];
+            }
+            for (; ch < nch; ++ch) a[0] += sp[pofs_s[q0 + ch]];
+        } else {
+            for (; ch + 8 <= nch; ch += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] += sp[P.sitem_pofs[base + q0 + ch + u]];
+            }
+            for (; ch < nch; ++ch) a[0] += sp[P.sitem_pofs[base + q0 + ch]];
         }
-        for (; ch < nch; ++ch) s0 += spart[P.sitem_pofs[it0 + ch] + 36 * sl + e];
-        const double sum = (s0 + s1) + (s2 + s3);
+        const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
         const int i = e / 6, c = e % 6;
         if (s == 0) {
             Sw[(size_t)(6 * lf + i) * lda + 6 * lf + c] = Hc[28 * (size_t)f + sym6(i, c)] + (i == c ? lambda : 0.0) - sum;
