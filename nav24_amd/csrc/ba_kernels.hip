@@ -472,20 +472,12 @@ void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) 
 // dX = Dinv cl, X' = X + dX, and the point part of x'(lambda x + b), summed
 // per tile.  A point with more than kTile observations has a tile of its own
 // and is summed by a block reduction.
+// c_b of one observation with its camera, weight and camera step d (6) loaded.
 template <typename T>
-__device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __restrict__ cam, const D4& xp,
-                                            int ob, const T* __restrict__ w, const double* __restrict__ dp,
-                                            double c[3]) {
-    c[0] = c[1] = c[2] = 0.0;
-    const int cb = P.o_cam[ob];
-    const int fb = P.cam_free[cb];
-    if (fb < 0) return;
-    double R[9], tt[3];
-    load_cam(cam, cb, R, tt);
+__device__ __forceinline__ void backsub_obs_at(const DevProblem& P, const double R[9], const double tt[3], const D4& xp,
+                                               double wb, const double* __restrict__ d, double c[3]) {
     EdgeLin<T> L;
     edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, L);
-    const double wb = (double)w[ob];
-    const double* d = dp + 6 * (size_t)fb;
     double t0 = 0.0, t1 = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {  // J_xi is zero at (0, 4) and (1, 3)
@@ -498,18 +490,40 @@ __device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __
     c[2] = (double)L.JX[2] * t0 + (double)L.JX[5] * t1;
 }
 
-__device__ __forceinline__ double backsub_point(const DevProblem& P, int p, double lambda, const double* __restrict__ Hl,
-                                                const double* __restrict__ ptsys, const double s[3],
-                                                const D4& xp, D4* __restrict__ Xn) {
-    const double* h = Hl + 9 * (size_t)p;
-    const double bl0 = h[6], bl1 = h[7], bl2 = h[8];
+template <typename T>
+__device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __restrict__ cam, const D4& xp,
+                                            int ob, const T* __restrict__ w, const double* __restrict__ dp,
+                                            double c[3]) {
+    c[0] = c[1] = c[2] = 0.0;
+    const int cb = P.o_cam[ob];
+    const int fb = P.cam_free[cb];
+    if (fb < 0) return;
+    double R[9], tt[3];
+    load_cam(cam, cb, R, tt);
+    backsub_obs_at<T>(P, R, tt, xp, (double)w[ob], dp + 6 * (size_t)fb, c);
+}
+
+// Point p's update from bl (3) and Dinv (6, packed) already loaded.
+__device__ __forceinline__ double backsub_point_at(int p, double lambda, const double bl[3], const double D[6],
+                                                   const double s[3], const D4& xp, D4* __restrict__ Xn) {
+    const double bl0 = bl[0], bl1 = bl[1], bl2 = bl[2];
     const double c0 = bl0 - s[0], c1 = bl1 - s[1], c2 = bl2 - s[2];
-    const double* D = ptsys + 12 * (size_t)p;
     const double d0 = D[0] * c0 + D[1] * c1 + D[2] * c2;
     const double d1 = D[1] * c0 + D[3] * c1 + D[4] * c2;
     const double d2 = D[2] * c0 + D[4] * c1 + D[5] * c2;
     Xn[p] = D4{xp.x + d0, xp.y + d1, xp.z + d2, xp.w};
     return d0 * (lambda * d0 + bl0) + d1 * (lambda * d1 + bl1) + d2 * (lambda * d2 + bl2);
+}
+
+__device__ __forceinline__ double backsub_point(const DevProblem& P, int p, double lambda, const double* __restrict__ Hl,
+                                                const double* __restrict__ ptsys, const double s[3],
+                                                const D4& xp, D4* __restrict__ Xn) {
+    double bl[3], D[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bl[k] = Hl[9 * (size_t)p + 6 + k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) D[k] = ptsys[12 * (size_t)p + k];
+    return backsub_point_at(p, lambda, bl, D, s, xp, Xn);
 }
 
 template <typename T>
@@ -529,24 +543,64 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
     const double lambda = lm[win].lambda;
     double sc = 0.0;
     if (o1 - o0 <= kTile) {
+        // loads in three waves, each issued together: the observation's indices
+        // and weight with this thread's first point (X, pt_free, bl, Dinv); the
+        // gathers they index (point, camera, free index); the camera step
         const int a = o0 + (int)threadIdx.x;
-        double c[3] = {0.0, 0.0, 0.0};
-        if (a < o1 && !fail) {
-            const int p = P.o_pt[a];
-            if (P.pt_free[p]) backsub_obs<T>(P, cam, X[p], a, w, dp, c);
+        const bool va = a < o1;  // a tile of unobserved points has no observations
+        int p = 0, cb = 0;
+        double wb = 0.0;
+        if (va) {
+            p = P.o_pt[a];
+            cb = P.o_cam[a];
+            wb = (double)w[a];
         }
+        const int q = p0 + (int)threadIdx.x;
+        D4 xq = {0.0, 0.0, 0.0, 0.0};
+        bool fq = false;
+        double blq[3] = {0.0, 0.0, 0.0}, Dq[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (q < p1) {
+            xq = X[q];
+            fq = P.pt_free[q] != 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) blq[k] = Hl[9 * (size_t)q + 6 + k];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Dq[k] = ptsys[12 * (size_t)q + k];
+        }
+        bool fp = false;
+        D4 xp = {0.0, 0.0, 0.0, 0.0};
+        int fb = -1;
+        double R[9], tt[3];
+        if (va) {
+            fp = P.pt_free[p] != 0;
+            xp = X[p];
+            fb = P.cam_free[cb];
+            load_cam(cam, cb, R, tt);
+        }
+        double c[3] = {0.0, 0.0, 0.0};
+        if (va && !fail && fp && fb >= 0) backsub_obs_at<T>(P, R, tt, xp, wb, dp + 6 * (size_t)fb, c);
         sh[0][threadIdx.x] = c[0]; sh[1][threadIdx.x] = c[1]; sh[2][threadIdx.x] = c[2];
         __syncthreads();
-        for (int p = p0 + (int)threadIdx.x; p < p1; p += kBlock) {
-            const D4 xp = X[p];
-            if (P.pt_free[p] && !fail) {
+        for (int pp = q; pp < p1; pp += kBlock) {
+            const bool first = pp == q;
+            const D4 xpp = first ? xq : X[pp];
+            if ((first ? fq : P.pt_free[pp] != 0) && !fail) {
                 double s[3] = {0.0, 0.0, 0.0};
-                for (int b = P.pt_ptr[p]; b < P.pt_ptr[p + 1]; ++b) {
+                for (int b = P.pt_ptr[pp]; b < P.pt_ptr[pp + 1]; ++b) {
                     s[0] += sh[0][b - o0]; s[1] += sh[1][b - o0]; s[2] += sh[2][b - o0];
                 }
-                sc += backsub_point(P, p, lambda, Hl, ptsys, s, xp, Xn);
+                if (first) {
+                    sc += backsub_point_at(pp, lambda, blq, Dq, s, xpp, Xn);
+                } else {
+                    double bl[3], D[6];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) bl[k] = Hl[9 * (size_t)pp + 6 + k];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) D[k] = ptsys[12 * (size_t)pp + k];
+                    sc += backsub_point_at(pp, lambda, bl, D, s, xpp, Xn);
+                }
             } else {
-                Xn[p] = xp;
+                Xn[pp] = xpp;
             }
         }
     } else {
