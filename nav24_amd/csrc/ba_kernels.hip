@@ -725,10 +725,23 @@ void launch_cost(const DevProblem& P, const DevState& s, const DevWork& W, int t
 }
 
 // ---------------------------------------------------------------- LM decision
+// This thread's part of the sum of v[i0..i1) over 256 threads: four
+// interleaved partial sums (four loads in flight), in a fixed order.
+__device__ __forceinline__ double range_part(const double* v, int i0, int i1) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = i0 + threadIdx.x;
+    for (; i + 768 < i1; i += 1024) {
+        s0 += v[i];
+        s1 += v[i + 256];
+        s2 += v[i + 512];
+        s3 += v[i + 768];
+    }
+    for (; i < i1; i += 256) s0 += v[i];
+    return (s0 + s1) + (s2 + s3);
+}
+
 __device__ double sum_range(const double* v, int i0, int i1, double* sh) {
-    double s = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += 256) s += v[i];
-    return block_sum<double, 256>(s, sh);
+    return block_sum<double, 256>(range_part(v, i0, i1), sh);
 }
 
 // g2o OptimizationAlgorithmLevenberg trial decision, one workgroup per window:
@@ -747,10 +760,12 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
         if (threadIdx.x == 0) lm[win].accepted = 0;
         return;
     }
-    const double cost = sum_range(red_cost, P.win_cblk0[win], P.win_cblk0[win + 1], sh);
+    // both ranges' loads in flight together
+    const double pc = range_part(red_cost, P.win_cblk0[win], P.win_cblk0[win + 1]);
+    const double ps = range_part(red_scale, P.n_win + P.win_tile0[win], P.n_win + P.win_tile0[win + 1]);
+    const double cost = block_sum<double, 256>(pc, sh);
     __syncthreads();
-    const double scale = red_scale[win] +
-                         sum_range(red_scale, P.n_win + P.win_tile0[win], P.n_win + P.win_tile0[win + 1], sh);
+    const double scale = red_scale[win] + block_sum<double, 256>(ps, sh);
     if (threadIdx.x == 0) {
         LMDev L = lm[win];
         const double temp = L.fail ? DBL_MAX : cost;
