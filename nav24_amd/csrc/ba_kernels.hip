@@ -219,11 +219,21 @@ __global__ void k_cam_reduce(DevProblem P, const double* __restrict__ hpart, con
     const int f = blockIdx.x;
     if (!needs_lin(lm, P.cam_win[P.free_cam[f]])) return;
     if (threadIdx.x < 28) {
-        double s = 0.0;
-        if (threadIdx.x < 27)
-            for (int it = P.cam_hitem_ptr[f]; it < P.cam_hitem_ptr[f + 1]; ++it)
-                s += hpart[27 * (size_t)it + threadIdx.x];
-        Hc[28 * (size_t)f + threadIdx.x] = s;
+        // chunks in a fixed order, four interleaved partial sums (loads in flight)
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (threadIdx.x < 27) {
+            const int i1 = P.cam_hitem_ptr[f + 1];
+            int it = P.cam_hitem_ptr[f];
+            const double* hp = hpart + threadIdx.x;
+            for (; it + 4 <= i1; it += 4) {
+                s0 += hp[27 * (size_t)it];
+                s1 += hp[27 * (size_t)(it + 1)];
+                s2 += hp[27 * (size_t)(it + 2)];
+                s3 += hp[27 * (size_t)(it + 3)];
+            }
+            for (; it < i1; ++it) s0 += hp[27 * (size_t)it];
+        }
+        Hc[28 * (size_t)f + threadIdx.x] = (s0 + s1) + (s2 + s3);
     }
 }
 
