@@ -212,27 +212,36 @@ __device__ __forceinline__ double lane_value(double v, int l) {
 // After the last stage wave 3 forms block row 3.  A pivot that is not
 // positive and finite is replaced by 1 (the values stay finite); the return
 // value, the same in every thread, says whether one was.
-__device__ __forceinline__ bool tile_potrf_inv(TileLds& A, double* rpiv, double* scratch, TileLds& M) {
+__device__ __forceinline__ bool tile_potrf_inv(TileLds& A, double* rpiv, double* scratch, TileLds& M, int nst) {
     const int lane = threadIdx.x & 63, w = wave_id();
     const int r16 = lane & 15, kq = lane >> 4;
     const int c0 = 16 * w;
     double (*cb)[64] = reinterpret_cast<double (*)[64]>(scratch);
     bool bad = false;
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < nst; ++s) {
         if (s == 0 && w > 0) {
             for (int idx = (int)threadIdx.x - 64; idx < 4096; idx += 192) {
                 const int i = idx >> 6, j = idx & 63;
                 if ((j >> 3) > (i >> 3)) M[i][j] = 0.0;
             }
         }
+        if (s == 0 && w >= nst) {
+            // identity padding rows (the last tile of the system): their part of
+            // the factor is the identity already in A; M's rows are the identity
+            for (int idx = lane; idx < 16 * 64; idx += kWave) {
+                const int i = c0 + (idx >> 6), j = idx & 63;
+                if ((j >> 3) <= (i >> 3)) M[i][j] = (i == j) ? 1.0 : 0.0;
+            }
+            if (lane < 16) rpiv[c0 + lane] = 1.0;
+        }
         if (s > 0 && w == s - 1) {
             // block row s-1 of the inverse and its terms of the rows below
             wave_inv16(A, rpiv, M, 16 * w);
             for (int j = 0; j < w; ++j) blk_finish(M, w, j);
-            for (int m = w + 1; m < 4; ++m)
+            for (int m = w + 1; m < nst; ++m)
                 for (int j = 0; j <= w; ++j) blk_term(A, M, m, w, j, j == w);
         }
-        if (s > 0 && w >= s) {
+        if (s > 0 && w >= s && w < nst) {
             const int p = 16 * (s - 1);
             double bv[4], av[4][4];
 #pragma unroll
@@ -303,12 +312,12 @@ __device__ __forceinline__ bool tile_potrf_inv(TileLds& A, double* rpiv, double*
                 for (int j = 0; j < 16; ++j) rpiv[c0 + j] = rp[j];
             }
         }
-        if (s < 3) __syncthreads();
+        if (s < nst - 1) __syncthreads();
     }
     bad = __syncthreads_or(bad);
-    if (w == 3) {
-        wave_inv16(A, rpiv, M, 48);
-        for (int j = 0; j < 3; ++j) blk_finish(M, 3, j);
+    if (w == nst - 1) {
+        wave_inv16(A, rpiv, M, 16 * w);
+        for (int j = 0; j < w; ++j) blk_finish(M, w, j);
     }
     __syncthreads();
     return bad;
@@ -396,8 +405,12 @@ __device__ __forceinline__ void tile_forward(const TileLds& M, const double* __r
     __syncthreads();
 }
 
+// 16-column stages of diagonal tile i that hold rows of the n-row system (the
+// last tile's padding rows beyond are the identity and are not factored).
+__device__ __forceinline__ int tile_stages(int n, int i) { return min(4, (n - 64 * i + 15) >> 4); }
+
 __global__ __launch_bounds__(256) void k_chol_first(const double* __restrict__ S, double* __restrict__ Lm,
-                                                    double* __restrict__ Li, int lda, const double* __restrict__ g,
+                                                    double* __restrict__ Li, int lda, int n, const double* __restrict__ g,
                                                     double* __restrict__ y, LMDev* lm) {
     __shared__ TileLds A;
     __shared__ TileLds M;
@@ -407,13 +420,13 @@ __global__ __launch_bounds__(256) void k_chol_first(const double* __restrict__ S
     tile_load(A, S, lda);
     if (lm->phase == kPhaseDone) return;  // a step launched after the solve has finished
     __syncthreads();
-    if (tile_potrf_inv(A, rpiv, scratch, M) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
+    if (tile_potrf_inv(A, rpiv, scratch, M, tile_stages(n, 0)) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
     tile_store(M, Li, 64);
     tile_forward(M, nullptr, 0, g, y, rv);
 }
 
 __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, double* __restrict__ Lm,
-                                                   double* __restrict__ Li, int lda, int k,
+                                                   double* __restrict__ Li, int lda, int n, int k,
                                                    const double* __restrict__ g, double* __restrict__ y,
                                                    double* __restrict__ fpart, int nt, LMDev* lm) {
     __shared__ TileLds Mk;
@@ -461,7 +474,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double* __restrict__ S, doubl
 #pragma unroll
             for (int v = 0; v < 4; ++v) Mk[16 * w + 4 * v + kq][16 * b + r16] = cur[b][v] - acc[b][v];
         __syncthreads();
-        if (tile_potrf_inv(Mk, rpiv, &Bi[0][0], Bj) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
+        if (tile_potrf_inv(Mk, rpiv, &Bi[0][0], Bj, tile_stages(n, i)) && threadIdx.x == 0) atomicOr(&lm->fail, 1);
         tile_store(Bj, Li + (size_t)4096 * i, 64);
         __syncthreads();  // fpart[i][k] (written above by this workgroup) is read back by the forward step
         tile_forward(Bj, fpart + (size_t)i * nt * 64, k + 1, g + 64 * i, y + 64 * i, rv);
@@ -1096,11 +1109,11 @@ void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevSt
                            (const double*)W.S, lda, P.n_fc, (const double*)W.g, W.Lm, W.dp, W.lm);
         return;
     }
-    hipLaunchKernelGGL(k_chol_first, dim3(1), dim3(256), 0, st, (const double*)W.S, W.Lm, W.Li, lda,
+    hipLaunchKernelGGL(k_chol_first, dim3(1), dim3(256), 0, st, (const double*)W.S, W.Lm, W.Li, lda, P.n,
                        (const double*)W.g, W.y, W.lm);
     for (int k = 0; k + 1 < nt; ++k) {
         const int rem = nt - k - 1;
-        hipLaunchKernelGGL(k_chol_step, dim3(rem * (rem + 1) / 2), dim3(256), 0, st, W.S, W.Lm, W.Li, lda, k,
+        hipLaunchKernelGGL(k_chol_step, dim3(rem * (rem + 1) / 2), dim3(256), 0, st, W.S, W.Lm, W.Li, lda, P.n, k,
                            (const double*)W.g, W.y, W.fpart, nt, W.lm);
     }
     if (nt <= kBackSingleMax) {

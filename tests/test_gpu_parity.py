@@ -50,6 +50,25 @@ def test_fp64_both_factorisations_match_oracle(solver, oracle_mod, mode):
     check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
 
 
+@pytest.mark.parametrize("n_cam", [24, 19, 20])
+def test_fp64_tiled_partial_last_tile(solver, oracle_mod, n_cam):
+    """Tiled factorisation whose last 64-row tile holds 4, 38 and 44 rows of
+    the system (22, 17 and 18 free cameras): one, three and three of its four
+    16-column stages are factored, the rest is identity padding."""
+    _say(f"fp64 solve workload=1 n_cam={n_cam} n_pt=3000 tiled factorisation")
+    prob = problems.workload(1, seed=8, n_cam=n_cam, n_pt=3000)
+    opts = workload_options(1)
+    opts.precision = NAV_BA_FP64
+    opts.n_iter = 6
+    ref = oracle_mod.solve(prob, opts)
+    solver.set_solver(1)
+    try:
+        got = solver.solve(prob, opts)
+    finally:
+        solver.set_solver(0)
+    check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
+
+
 def test_loop_tiled_factorisation_fp64_and_mixed(solver, oracle_mod):
     """300-keyframe loop (298 free cameras, 1788 pose unknowns: the tiled
     multi-workgroup factorisation), fewer points so the oracle stays quick."""
