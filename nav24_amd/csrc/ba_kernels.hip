@@ -595,10 +595,16 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
             const bool first = pp == q;
             const D4 xpp = first ? xq : X[pp];
             if ((first ? fq : P.pt_free[pp] != 0) && !fail) {
-                double s[3] = {0.0, 0.0, 0.0};
-                for (int b = P.pt_ptr[pp]; b < P.pt_ptr[pp + 1]; ++b) {
-                    s[0] += sh[0][b - o0]; s[1] += sh[1][b - o0]; s[2] += sh[2][b - o0];
+                // the point's observations in a fixed order, two interleaved sums per component
+                const int b1 = P.pt_ptr[pp + 1] - o0;
+                int b = P.pt_ptr[pp] - o0;
+                double s[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+                for (; b + 2 <= b1; b += 2) {
+                    s[0] += sh[0][b]; s[1] += sh[1][b]; s[2] += sh[2][b];
+                    t[0] += sh[0][b + 1]; t[1] += sh[1][b + 1]; t[2] += sh[2][b + 1];
                 }
+                for (; b < b1; ++b) { s[0] += sh[0][b]; s[1] += sh[1][b]; s[2] += sh[2][b]; }
+                s[0] += t[0]; s[1] += t[1]; s[2] += t[2];
                 if (first) {
                     sc += backsub_point_at(pp, lambda, blq, Dq, s, xpp, Xn);
                 } else {
