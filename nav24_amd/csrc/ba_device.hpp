@@ -150,6 +150,14 @@ __device__ __forceinline__ void inv3_sym(const double a[6], double o[6]) {
     o[5] = (a00 * a11 - a01 * a01) * id;
 }
 
+// D = (Hll + lambda I)^-1 of one point (packed 00 01 02 11 12 22) from its
+// block h = (Hll packed (6), bl (3)): the damped point block the Schur
+// complement eliminates and the back-substitution applies.
+__device__ __forceinline__ void point_dinv(const double* __restrict__ h, double lambda, double D[6]) {
+    double a[6] = {h[0] + lambda, h[1], h[2], h[3] + lambda, h[4], h[5] + lambda};
+    inv3_sym(a, D);
+}
+
 // ---- quaternions (FP64) ----
 __device__ __forceinline__ void quat_to_R(const double q[4], double R[9]) {
     const double tx = 2 * q[0], ty = 2 * q[1], tz = 2 * q[2];

@@ -131,7 +131,6 @@ struct DevWork {
     double* Hl;         // [n_pt * 9]: Hll (6 packed) + bl (3)
     double* hpart;      // [n_hitems * 27]: Hpp (21 packed) + bp (6) per chunk
     double* Hc;         // [n_fc * 28]: Hpp packed (21) + bp (6)
-    double* ptsys;      // [n_pt * 12]: Dinv packed (6) + Dinv*bl (3)
     double* spart;      // Schur row partials (FP64 for both compute types)
     double* S;          // dense reduced systems
     double* g;          // reduced right-hand sides (free-camera order, padded for the tiled path)
@@ -227,7 +226,6 @@ void launch_lm_reset(const DevProblem& P, const DevWork& W, hipStream_t st);
 template <typename T> void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 template <typename T> void launch_lin_cam(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 template <typename T> void launch_lambda_init(const DevProblem& P, const DevWork& W, hipStream_t st);
-template <typename T> void launch_hinv(const DevProblem& P, const DevWork& W, hipStream_t st);
 template <typename T> void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 void launch_fill_system(const DevProblem& P, const DevWork& W, hipStream_t st);  // once per prepare
 template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st);

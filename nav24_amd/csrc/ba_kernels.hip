@@ -8,11 +8,11 @@
 //              (segment sums in LDS)                                     [HBM-bound]
 //   lin_cam    Hpp / bp per free camera from camera-major chunks          [HBM-bound]
 //   lambda     first iteration without a user lambda: 1e-5 max |diag H|
-//   hinv       (Hll + lambda I)^-1 and Hll^-1 bl per point, FP64
-//   schur_row  reduced camera system rows (schur_kernels.hip)
+//   schur_row  reduced camera system rows (schur_kernels.hip); each forms
+//              the damped point inverse (Hll + lambda I)^-1 itself, FP64
 //   schur_red  partials -> dense S (FP64) and g per window
 //   cholesky   tiled (one large window) or in-LDS per window (chol_kernels.hip)
-//   backsub    dX = Hll^-1 (bl - Hpl^T dxi), candidate points, x'(lambda x + b)
+//   backsub    dX = (Hll + lambda I)^-1 (bl - Hpl^T dxi), candidate points, x'(lambda x + b)
 //   update_cam T <- exp(dxi) T (FP64 SE3 exp-map), candidate cameras
 //   cost       robust chi2 at the candidate, window-aligned partial sums
 //   accept     rho, lambda / nu schedule, accept or reject, next phase
@@ -305,33 +305,6 @@ void launch_lambda_init(const DevProblem& P, const DevWork& W, hipStream_t st) {
     hipLaunchKernelGGL(k_lambda_scale, dim3((P.n_win + 255) / 256), dim3(256), 0, st, P.n_win, W.lm);
 }
 
-// ---------------------------------------------------------------- hinv
-// Per free point: Dinv = (Hll + lambda I)^-1 (FP64 adjugate) and Dinv * bl.
-__global__ void k_hinv(DevProblem P, const double* __restrict__ Hl, const LMDev* lm, double* __restrict__ ptsys) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.n_pt || !P.pt_free[p]) return;
-    const int win = P.pt_win[p];
-    if (!running(lm, win)) return;
-    const double lambda = lm[win].lambda;
-    const double* h = Hl + 9 * (size_t)p;
-    double a[6] = {h[0] + lambda, h[1], h[2], h[3] + lambda, h[4], h[5] + lambda};
-    double d[6];
-    inv3_sym(a, d);
-    const double b0 = h[6], b1 = h[7], b2 = h[8];
-    double* o = ptsys + 12 * (size_t)p;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) o[k] = d[k];
-    o[6] = d[0] * b0 + d[1] * b1 + d[2] * b2;
-    o[7] = d[1] * b0 + d[3] * b1 + d[4] * b2;
-    o[8] = d[2] * b0 + d[4] * b1 + d[5] * b2;
-}
-
-template <typename T>
-void launch_hinv(const DevProblem& P, const DevWork& W, hipStream_t st) {
-    hipLaunchKernelGGL(k_hinv, dim3((P.n_pt + 255) / 256), dim3(256), 0, st, P, (const double*)W.Hl,
-                       (const LMDev*)W.lm, W.ptsys);
-}
-
 // ---------------------------------------------------------------- schur reduce
 // Clear the dense systems (padding rows of the tiled path get a unit
 // diagonal) and their right-hand sides; run once per prepare.  The reduction
@@ -526,20 +499,17 @@ __device__ __forceinline__ double backsub_point_at(int p, double lambda, const d
 }
 
 __device__ __forceinline__ double backsub_point(const DevProblem& P, int p, double lambda, const double* __restrict__ Hl,
-                                                const double* __restrict__ ptsys, const double s[3],
-                                                const D4& xp, D4* __restrict__ Xn) {
-    double bl[3], D[6];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) bl[k] = Hl[9 * (size_t)p + 6 + k];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) D[k] = ptsys[12 * (size_t)p + k];
-    return backsub_point_at(p, lambda, bl, D, s, xp, Xn);
+                                                const double s[3], const D4& xp, D4* __restrict__ Xn) {
+    const double* h = Hl + 9 * (size_t)p;
+    double D[6];
+    point_dinv(h, lambda, D);
+    return backsub_point_at(p, lambda, h + 6, D, s, xp, Xn);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* __restrict__ cam,
                                                     const D4* __restrict__ X, const T* __restrict__ w,
-                                                    const double* __restrict__ Hl, const double* __restrict__ ptsys,
+                                                    const double* __restrict__ Hl,
                                                     const double* __restrict__ dp, const LMDev* lm,
                                                     D4* __restrict__ Xn, double* __restrict__ red_scale) {
     __shared__ double sh[3][kTile];
@@ -568,14 +538,12 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
         const int q = p0 + (int)threadIdx.x;
         D4 xq = {0.0, 0.0, 0.0, 0.0};
         bool fq = false;
-        double blq[3] = {0.0, 0.0, 0.0}, Dq[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double hq[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (q < p1) {
             xq = X[q];
             fq = P.pt_free[q] != 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) blq[k] = Hl[9 * (size_t)q + 6 + k];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) Dq[k] = ptsys[12 * (size_t)q + k];
+            for (int k = 0; k < 9; ++k) hq[k] = Hl[9 * (size_t)q + k];
         }
         bool fp = false;
         D4 xp = {0.0, 0.0, 0.0, 0.0};
@@ -606,14 +574,11 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
                 for (; b < b1; ++b) { s[0] += sh[0][b]; s[1] += sh[1][b]; s[2] += sh[2][b]; }
                 s[0] += t[0]; s[1] += t[1]; s[2] += t[2];
                 if (first) {
-                    sc += backsub_point_at(pp, lambda, blq, Dq, s, xpp, Xn);
+                    double D[6];
+                    point_dinv(hq, lambda, D);
+                    sc += backsub_point_at(pp, lambda, hq + 6, D, s, xpp, Xn);
                 } else {
-                    double bl[3], D[6];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) bl[k] = Hl[9 * (size_t)pp + 6 + k];
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) D[k] = ptsys[12 * (size_t)pp + k];
-                    sc += backsub_point_at(pp, lambda, bl, D, s, xpp, Xn);
+                    sc += backsub_point(P, pp, lambda, Hl, s, xpp, Xn);
                 }
             } else {
                 Xn[pp] = xpp;
@@ -645,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
                     s[k] = 0.0;
                     for (int i = 0; i < kBlock / kWave; ++i) s[k] += red[i][k];
                 }
-                sc = backsub_point(P, p0, lambda, Hl, ptsys, s, xp, Xn);
+                sc = backsub_point(P, p0, lambda, Hl, s, xp, Xn);
             } else {
                 Xn[p0] = xp;
             }
@@ -661,7 +626,7 @@ template <typename T>
 void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st) {
     if (P.n_tiles == 0) return;
     hipLaunchKernelGGL(k_backsub<T>, dim3(P.n_tiles), dim3(kBlock), 0, st, P, cur.cam, cur.X, (const T*)W.w,
-                       (const double*)W.Hl, W.ptsys, W.dp, (const LMDev*)W.lm, cand.X, W.red_scale);
+                       (const double*)W.Hl, W.dp, (const LMDev*)W.lm, cand.X, W.red_scale);
 }
 
 // ---------------------------------------------------------------- camera update
@@ -913,7 +878,6 @@ void launch_final(const DevProblem& P, const DevWork& W, hipStream_t st) {
     template void launch_lin_pt<T>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);         \
     template void launch_lin_cam<T>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);        \
     template void launch_lambda_init<T>(const DevProblem&, const DevWork&, hipStream_t);                     \
-    template void launch_hinv<T>(const DevProblem&, const DevWork&, hipStream_t);                            \
     template void launch_schur_reduce<T>(const DevProblem&, const DevWork&, hipStream_t);                    \
     template void launch_backsub<T>(const DevProblem&, const DevState&, const DevState&, const DevWork&,     \
                                     hipStream_t);                                                            \

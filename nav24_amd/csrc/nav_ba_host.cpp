@@ -238,7 +238,6 @@ void enqueue_lin(nav_ba_handle* h, bool first) {
 
 template <typename T>
 void enqueue_reduced_system(nav_ba_handle* h) {
-    stage(h, NAV_BA_ST_HINV, [&] { launch_hinv<T>(h->P, h->W, h->stream); });
     if (h->P.n_fc > 0) {
         stage(h, NAV_BA_ST_SCHUR_ROW, [&] { launch_schur_row<T>(h->P, h->cur, h->W, h->stream); });
         stage(h, NAV_BA_ST_SCHUR_REDUCE, [&] { launch_schur_reduce<T>(h->P, h->W, h->stream); });
@@ -514,8 +513,7 @@ int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems
     for (auto& l : L) { l = LMDev{}; l.lambda = 1e-4; l.nu = 2.0; l.phase = kPhaseLin; l.iter = 1; }
     HIPTRY(h, hipMemcpyAsync(h->W.lm, L.data(), sizeof(LMDev) * L.size(), hipMemcpyHostToDevice, s));
     launch_cam_rt(P, h->cur, s);
-    if (mixed(h)) { enqueue_lin<float>(h, false); launch_hinv<float>(P, h->W, s); }
-    else { enqueue_lin<double>(h, false); launch_hinv<double>(P, h->W, s); }
+    if (mixed(h)) enqueue_lin<float>(h, false); else enqueue_lin<double>(h, false);
     int st;
     float best_ms;
     if ((st = time_schur(h, &best_ms)) != NAV_BA_OK) return st;
@@ -865,7 +863,6 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.Hl, sizeof(double) * 9 * (size_t)np);
     AL(W.hpart, sizeof(double) * 27 * std::max<size_t>(hf.size(), 1));
     AL(W.Hc, sizeof(double) * 28 * std::max(nfc, 1));
-    AL(W.ptsys, sizeof(double) * 12 * (size_t)np);
     AL(W.S, sizeof(double) * (size_t)std::max<long long>(S_size, 1));
     AL(W.g, sizeof(double) * nvec);
     AL(W.y, sizeof(double) * nvec);

@@ -74,7 +74,7 @@ __device__ __forceinline__ void cam_get(const double* camt, const CamRT* cam, in
 template <typename T, bool kCamLds>
 __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT* __restrict__ cam,
                                                       const D4* __restrict__ X, const T* __restrict__ w,
-                                                      const double* __restrict__ ptsys, const LMDev* lm,
+                                                      const double* __restrict__ Hl, const LMDev* lm,
                                                       double* __restrict__ spart) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int it = P.sitem_order[blockIdx.x];
@@ -83,6 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     const int cf = P.free_cam[f];
     const int win = P.cam_win[cf];
     if (!running(lm, win)) return;
+    const double lambda = lm[win].lambda;
     // cameras are addressed relative to the window's first camera
     const int cam0 = P.win_cam0[win], ncw = P.win_cam0[win + 1] - cam0;
     const bool g0 = (g == 0);
@@ -133,8 +134,10 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                     for (int j = 0; j < 3; ++j)  // J_xi is zero at (0, 4) and (1, 3)
                         Ha[3 * i + j] = i == 4 ? A1 * JXa[3 + j] : i == 3 ? A0 * JXa[j] : A0 * JXa[j] + A1 * JXa[3 + j];
                 }
-                const double* ps = ptsys + 12 * (size_t)p;
-                const double d00 = ps[0], d01 = ps[1], d02 = ps[2], d11 = ps[3], d12 = ps[4], d22 = ps[5];
+                const double* hp = Hl + 9 * (size_t)p;
+                double Dp[6];
+                point_dinv(hp, lambda, Dp);
+                const double d00 = Dp[0], d01 = Dp[1], d02 = Dp[2], d11 = Dp[3], d12 = Dp[4], d22 = Dp[5];
                 double* wb = wbuf + lane * SchurLayout<T>::kWbuf;
                 double W[18];
 #pragma unroll
@@ -148,7 +151,11 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
                 for (int e = 0; e < 18; ++e) wb[e] = W[e];
                 wb[18] = xp.x; wb[19] = xp.y; wb[20] = xp.z;
                 if (g0) {
-                    const double e0 = ps[6], e1 = ps[7], e2 = ps[8];
+                    // Dinv bl of the point (the right-hand side's point term)
+                    const double b0 = hp[6], b1 = hp[7], b2 = hp[8];
+                    const double e0 = d00 * b0 + d01 * b1 + d02 * b2;
+                    const double e1 = d01 * b0 + d11 * b1 + d12 * b2;
+                    const double e2 = d02 * b0 + d12 * b1 + d22 * b2;
 #pragma unroll
                     for (int i = 0; i < 6; ++i)
                         coef[i] = fma(Ha[3 * i + 2], e2, fma(Ha[3 * i + 1], e1, fma(Ha[3 * i], e0, coef[i])));
@@ -299,10 +306,10 @@ void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, 
     const size_t smem = schur_row_smem<T>(P);
     if (P.max_win_cams <= kCamLdsMax)
         hipLaunchKernelGGL((k_schur_row<T, true>), dim3(P.n_sitems), dim3(kBlock), smem, st, P, s.cam, s.X,
-                           (const T*)W.w, W.ptsys, (const LMDev*)W.lm, W.spart);
+                           (const T*)W.w, (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
     else
         hipLaunchKernelGGL((k_schur_row<T, false>), dim3(P.n_sitems), dim3(kBlock), smem, st, P, s.cam, s.X,
-                           (const T*)W.w, W.ptsys, (const LMDev*)W.lm, W.spart);
+                           (const T*)W.w, (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
 }
 
 template void launch_schur_row<double>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);
