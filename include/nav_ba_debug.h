@@ -27,7 +27,7 @@ enum nav_ba_stage {
     NAV_BA_ST_LIN_PT = 0,      /* per-observation residual/Jacobian, Hll/bl per point */
     NAV_BA_ST_LIN_CAM,         /* Hpp/bp per camera (+ chunk reduction) */
     NAV_BA_ST_LAMBDA,          /* lambda initialisation */
-    NAV_BA_ST_HINV,            /* (Hll + lambda I)^-1 per point */
+    NAV_BA_ST_HINV,            /* unused: (Hll + lambda I)^-1 is formed inside the Schur rows and back-substitution */
     NAV_BA_ST_SCHUR_ROW,       /* reduced camera system rows */
     NAV_BA_ST_SCHUR_REDUCE,    /* partial sums -> dense S, g */
     NAV_BA_ST_CHOLESKY,        /* dense factorisation and substitutions */
