@@ -325,6 +325,48 @@ def diffcost_leg(n_problems, steps):
     return out
 
 
+def undistort_leg(n_points, steps, with_cpu):
+    """Key-point undistortion of the radial-tangential camera
+    (PinholeRadTan::UndistortKeyPoints): n_points key points resident in HBM,
+    one per observation of the 3 M-observation loop; K launches timed with HIP
+    events on torch's stream.  8 bytes read and 8 written per point.  With the
+    CPU baseline: the oracle on one host thread over the same points, and
+    whether the device result equals it."""
+    import torch
+    from nav24_amd.undistort import PinholeRadTanHip
+
+    prm = [450.0, 450.0, 320.0, 240.0, -0.28, 0.07, 5e-4, 3e-4]
+    r = np.random.default_rng(0)
+    uv = np.stack([r.uniform(0, 640, n_points), r.uniform(0, 480, n_points)], 1).astype(np.float32)
+    cam = PinholeRadTanHip(prm)
+    src = torch.from_numpy(uv).cuda()
+    dst = torch.empty_like(src)
+    cam.undistort_device(src, out=dst)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        cam.undistort_device(src, out=dst)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / steps
+    nbytes = 16 * n_points
+    out = {"workload": f"{n_points} key points, 640x480 pinhole, k1 k2 p1 p2 = {prm[4:]}", "n_points": n_points,
+           "us_per_launch": round(us, 2), "points_per_s": round(n_points / (us * 1e-6), 1),
+           "roofline": {"bound": "fp64 issue (5 iterations with a division each)", "algorithmic_bytes": nbytes,
+                        "achieved_GBs": round(nbytes / (us * 1e-6) / 1e9, 1),
+                        "frac_of_hbm_peak": round(nbytes / (us * 1e-6) / (HBM_PEAK_GBS * 1e9), 4)}}
+    if with_cpu:
+        from oracle import oracle
+        t0 = time.perf_counter()
+        ref = oracle.kp_undistort(uv, prm)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n_points / dt, 1), "unit": "points/s", "cores": 1, "kind": "port",
+                               "sample": f"the same {n_points} points, FP64 oracle, 1 thread, {dt:.3f} s"}
+        out["equal_to_oracle"] = bool(np.array_equal(dst.cpu().numpy(), ref))
+    return out
+
+
 def copy_calibration(mb, steps):
     """A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
     scales the same counters on the solver kernels."""
@@ -391,6 +433,8 @@ def main():
     ap.add_argument("--loop", type=int, default=1, help="loop leg (300 keyframes, 150 000 points): 0 = skip")
     ap.add_argument("--diffcost", type=int, default=64,
                     help="differentiable-cost leg: number of distorted-camera problems (0 = skip)")
+    ap.add_argument("--undistort", type=int, default=3_000_000,
+                    help="key-point undistortion leg: number of key points (0 = skip)")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling: run only the batched mixed solve (K timed runs) and print its timing")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -494,6 +538,8 @@ def main():
         line["loop"] = loop_leg(max(1, min(args.steps, 3)), 1)
     if args.diffcost > 0:
         line["diffcost"] = diffcost_leg(args.diffcost, max(args.steps, 5))
+    if args.undistort > 0:
+        line["undistort"] = undistort_leg(args.undistort, max(args.steps, 20), not args.no_cpu)
 
     if not args.no_cpu:
         cb, ref = cpu_baseline(prob, opts)

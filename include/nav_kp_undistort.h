@@ -1,0 +1,53 @@
+/*
+ * nav_kp_undistort.h — key-point undistortion of the radial-tangential
+ * pinhole camera on the device: the step that produces the BA measurements.
+ *
+ * Replaces PinholeRadTan::UndistortKeyPoints
+ * (core/sensor/camera/models/PinholeRadTan.cpp:11-24), reached per
+ * observation through Calibration::undistort (core/sensor/camera/Calibration.cpp:135-149,
+ * called at core/sensor/camera/Camera.cpp:175), whose result becomes the
+ * measurement the BA edge reads (KeyPoint2D::getPointUd).  The reference calls
+ * OpenCV's cv::undistortPoints(pts, pts, K, D, R, P) (OpenCV >= 4.2.0,
+ * core/CMakeLists.txt:10; absent from this image) with
+ *     K = [fx 0 cx; 0 fy cy; 0 0 1] and P = K, R = I   (GeometricCamera.h:62-66)
+ *     D = (k1, k2, p1, p2)                            (PinholeRadTan.hpp:13-18)
+ * all float, key points cv::Point2f.  The published algorithm of that call,
+ * in double precision:
+ *     x0 = x = (u - cx) / fx,  y0 = y = (v - cy) / fy   (as (u - cx) * (1 / fx))
+ *     5 times:  r2 = x^2 + y^2
+ *               icdist = 1 / (1 + (k2 r2 + k1) r2)      (icdist < 0: x, y = x0, y0; stop)
+ *               dx = 2 p1 x y + p2 (r2 + 2 x^2),  dy = p1 (r2 + 2 y^2) + 2 p2 x y
+ *               x = (x0 - dx) icdist,  y = (y0 - dy) icdist
+ *     (u', v') = (RR x) / (RR x)_z with RR = P R, rounded to float.
+ * The device evaluates exactly these double operations (no fused
+ * multiply-adds), so its output equals a plain double evaluation bit for bit.
+ *
+ * Stateless: no handle.  In place (uv_out == uv) is allowed, as the
+ * reference's call is.
+ */
+#ifndef NAV_KP_UNDISTORT_H
+#define NAV_KP_UNDISTORT_H
+
+#include <stdint.h>
+#include "nav_ba.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* params: fx fy cx cy k1 k2 p1 p2 (the reference's vParams order,
+ * PinholeRadTan(vParams), float).  uv, uv_out: n key points (u, v), float. */
+
+/* Device arrays, on `stream` (a hipStream_t, NULL = the null stream); returns
+ * after the launch is queued.  NAV_BA_BAD_ARG for n < 0 or a null pointer
+ * with n > 0, NAV_BA_HIP_ERROR if the launch fails. */
+int32_t nav_ba_kp_undistort(const float* uv, int32_t n, const float* params, float* uv_out, void* stream);
+
+/* Host arrays, through device `device` (copies in and out; blocks until done).
+ * NAV_BA_NO_DEVICE without a usable gfx950 device. */
+int32_t nav_ba_kp_undistort_host(const float* uv, int32_t n, const float* params, float* uv_out, int32_t device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NAV_KP_UNDISTORT_H */

@@ -1,6 +1,6 @@
 """Build the package's native libraries in-tree (nav24_amd/_lib/).
 
-  libnav24_ba.so   HIP kernels for gfx950 + host runtime + C ABI (include/nav_ba.h)
+  libnav24_ba.so   HIP kernels for gfx950 + host runtime + C ABI (include/*.h)
   libnav24_gen.so  seeded synthetic problem generator (host only)
 
 Run:  python -m nav24_amd.build      (or __graft_entry__.build())
@@ -25,7 +25,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 
-HIP_SOURCES = ["ba_kernels.hip", "schur_kernels.hip", "chol_kernels.hip", "nav_ba_host.cpp", "cost_kernels.hip"]
+HIP_SOURCES = ["ba_kernels.hip", "schur_kernels.hip", "chol_kernels.hip", "nav_ba_host.cpp", "cost_kernels.hip",
+               "undistort_kernels.hip"]
 
 
 def _run(cmd):

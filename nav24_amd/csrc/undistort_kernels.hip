@@ -1,0 +1,113 @@
+// undistort_kernels.hip — key-point undistortion of the radial-tangential
+// pinhole camera (include/nav_kp_undistort.h): PinholeRadTan::UndistortKeyPoints
+// (core/sensor/camera/models/PinholeRadTan.cpp:11-24), i.e. OpenCV's
+// cv::undistortPoints with K, D = (k1 k2 p1 p2), R = I, P = K and its default
+// termination (5 fixed-point iterations).
+//
+// One lane per key point, the double operations of that algorithm in its
+// order, without fused multiply-adds (contraction is off in this file), so
+// the result equals a plain double evaluation bit for bit for finite input.
+// 8 bytes read and 8 written per point; about 150 FP64 instructions per
+// point (5 iterations, each with one division), so the kernel is bound by
+// FP64 issue rather than HBM at 3 M points (~13 µs against ~6 µs of bytes).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/nav_kp_undistort.h"
+
+#pragma clang fp contract(off)
+
+namespace navba {
+
+struct KpParams {
+    double cx, cy, ifx, ify;  // ifx = 1 / fx in double, as the reference's call forms it
+    double k1, k2, p1, p2;
+    double RR[9];             // P R (= K here), row-major
+};
+
+constexpr int kKpIters = 5;  // cv::undistortPoints default: TermCriteria(COUNT, 5, 0.01)
+
+__global__ __launch_bounds__(256) void k_kp_undistort(const float2* __restrict__ uv, int n, KpParams q,
+                                                      float2* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float2 p = uv[i];
+    const double u = p.x, v = p.y;
+    double x = (u - q.cx) * q.ifx;
+    double y = (v - q.cy) * q.ify;
+    const double x0 = x, y0 = y;
+    for (int j = 0; j < kKpIters; ++j) {
+        const double r2 = x * x + y * y;
+        const double icdist = 1.0 / (1.0 + (q.k2 * r2 + q.k1) * r2);
+        if (icdist < 0.0) {  // no undistortion possible: the normalised input point
+            x = (u - q.cx) * q.ifx;
+            y = (v - q.cy) * q.ify;
+            break;
+        }
+        const double dx = 2.0 * q.p1 * x * y + q.p2 * (r2 + 2.0 * x * x);
+        const double dy = q.p1 * (r2 + 2.0 * y * y) + 2.0 * q.p2 * x * y;
+        x = (x0 - dx) * icdist;
+        y = (y0 - dy) * icdist;
+    }
+    const double xx = q.RR[0] * x + q.RR[1] * y + q.RR[2];
+    const double yy = q.RR[3] * x + q.RR[4] * y + q.RR[5];
+    const double ww = 1.0 / (q.RR[6] * x + q.RR[7] * y + q.RR[8]);
+    out[i] = make_float2((float)(xx * ww), (float)(yy * ww));
+}
+
+static KpParams kp_params(const float* prm) {
+    KpParams q;
+    const double fx = prm[0], fy = prm[1], cx = prm[2], cy = prm[3];
+    q.cx = cx;
+    q.cy = cy;
+    q.ifx = 1.0 / fx;
+    q.ify = 1.0 / fy;
+    q.k1 = prm[4];
+    q.k2 = prm[5];
+    q.p1 = prm[6];
+    q.p2 = prm[7];
+    // RR = P R with P = K, R = I (GeometricCamera.h:62-66): K itself
+    const double K[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};
+    std::memcpy(q.RR, K, sizeof(K));
+    return q;
+}
+
+static bool gfx950_device(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return false;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return false;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+}  // namespace navba
+
+using namespace navba;
+
+extern "C" int32_t nav_ba_kp_undistort(const float* uv, int32_t n, const float* params, float* uv_out, void* stream) {
+    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
+    if (n == 0) return NAV_BA_OK;
+    const KpParams q = kp_params(params);
+    hipLaunchKernelGGL(k_kp_undistort, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float2*>(uv), n, q, reinterpret_cast<float2*>(uv_out));
+    return hipGetLastError() == hipSuccess ? NAV_BA_OK : NAV_BA_HIP_ERROR;
+}
+
+extern "C" int32_t nav_ba_kp_undistort_host(const float* uv, int32_t n, const float* params, float* uv_out,
+                                            int32_t device) {
+    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
+    if (!gfx950_device(device)) return NAV_BA_NO_DEVICE;
+    if (n == 0) return NAV_BA_OK;
+    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
+    const size_t bytes = sizeof(float) * 2 * (size_t)n;
+    float* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return NAV_BA_HIP_ERROR;
+    int32_t st = NAV_BA_OK;
+    if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    if (st == NAV_BA_OK) st = nav_ba_kp_undistort(d, n, params, d, nullptr);
+    if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    (void)hipFree(d);
+    return st;
+}

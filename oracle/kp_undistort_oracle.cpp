@@ -1,0 +1,84 @@
+// kp_undistort_oracle.cpp — CPU restatement of the key-point undistortion of
+// the radial-tangential pinhole camera.
+//
+// TEST INFRASTRUCTURE ONLY: the checker of nav_ba_kp_undistort
+// (include/nav_kp_undistort.h); the product library never links or calls it.
+//
+// Restates PinholeRadTan::UndistortKeyPoints
+// (core/sensor/camera/models/PinholeRadTan.cpp:11-24), which calls
+// cv::undistortPoints(pts, pts, K, D, R, P) of OpenCV >= 4.2.0
+// (core/CMakeLists.txt:10; OpenCV is absent from this image) with K from
+// (fx, fy, cx, cy), D = (k1, k2, p1, p2), R = I and P = K
+// (GeometricCamera.h:62-66, PinholeRadTan.hpp:13-18).  The restated steps are
+// those of OpenCV's published undistortPoints: normalise by K (multiplying by
+// 1/fx, 1/fy formed in double), a fixed number of fixed-point iterations
+// x = (x0 - delta_tan(x)) / (1 + k1 r^2 + k2 r^4) (5 under the default
+// termination; the iteration stops with the normalised input point when the
+// radial factor's reciprocal is negative), then P R and the float result.
+// Double arithmetic in the order written, no fused multiply-adds (compiled
+// with -ffp-contract=off).
+//
+// Parity: unpinned against OpenCV itself (it cannot be run here); pinned by
+// what the mathematics fixes (tests/test_kp_undistort_oracle.py):
+//   - zero distortion returns the input point (u = fx ((u - cx) / fx) + cx
+//     to within double rounding, then rounded to float);
+//   - with p1 = p2 = 0 a point on the principal row stays on it (y = 0 is a
+//     fixed point), and the radial model is symmetric in (x, y) -> (-x, -y);
+//   - with many iterations the result inverts the forward Brown-Conrady
+//     model (oracle_kp_distort) to 1e-9 px for mild distortion;
+//   - the first iteration equals its closed form on hand-picked points;
+//   - a radial factor that turns negative returns the input point.
+#include <cstdint>
+
+extern "C" {
+
+// uv, out: n points (u, v); params: fx fy cx cy k1 k2 p1 p2; iters: 5 = the reference's call.
+void oracle_kp_undistort(const float* uv, int32_t n, const float* params, int32_t iters, float* out) {
+    const double fx = params[0], fy = params[1], cx = params[2], cy = params[3];
+    const double k1 = params[4], k2 = params[5], p1 = params[6], p2 = params[7];
+    const double ifx = 1.0 / fx, ify = 1.0 / fy;
+    const double RR[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};  // P R = K
+    for (int32_t i = 0; i < n; ++i) {
+        const double u = uv[2 * i], v = uv[2 * i + 1];
+        double x = (u - cx) * ifx;
+        double y = (v - cy) * ify;
+        const double x0 = x, y0 = y;
+        for (int32_t j = 0; j < iters; ++j) {
+            const double r2 = x * x + y * y;
+            const double icdist = 1.0 / (1.0 + (k2 * r2 + k1) * r2);
+            if (icdist < 0.0) {
+                x = (u - cx) * ifx;
+                y = (v - cy) * ify;
+                break;
+            }
+            const double dx = 2.0 * p1 * x * y + p2 * (r2 + 2.0 * x * x);
+            const double dy = p1 * (r2 + 2.0 * y * y) + 2.0 * p2 * x * y;
+            x = (x0 - dx) * icdist;
+            y = (y0 - dy) * icdist;
+        }
+        const double xx = RR[0] * x + RR[1] * y + RR[2];
+        const double yy = RR[3] * x + RR[4] * y + RR[5];
+        const double ww = 1.0 / (RR[6] * x + RR[7] * y + RR[8]);
+        out[2 * i] = (float)(xx * ww);
+        out[2 * i + 1] = (float)(yy * ww);
+    }
+}
+
+// Forward model (the one undistortion inverts), in double: normalised
+// (x, y) -> distorted pixel, x_d = x (1 + k1 r^2 + k2 r^4) + 2 p1 x y + p2 (r^2 + 2 x^2),
+// y_d = y (1 + k1 r^2 + k2 r^4) + p1 (r^2 + 2 y^2) + 2 p2 x y.
+void oracle_kp_distort(const double* xy, int32_t n, const float* params, double* uv) {
+    const double fx = params[0], fy = params[1], cx = params[2], cy = params[3];
+    const double k1 = params[4], k2 = params[5], p1 = params[6], p2 = params[7];
+    for (int32_t i = 0; i < n; ++i) {
+        const double x = xy[2 * i], y = xy[2 * i + 1];
+        const double r2 = x * x + y * y;
+        const double rad = 1.0 + k1 * r2 + k2 * r2 * r2;
+        const double xd = x * rad + 2.0 * p1 * x * y + p2 * (r2 + 2.0 * x * x);
+        const double yd = y * rad + p1 * (r2 + 2.0 * y * y) + 2.0 * p2 * x * y;
+        uv[2 * i] = fx * xd + cx;
+        uv[2 * i + 1] = fy * yd + cy;
+    }
+}
+
+}  // extern "C"

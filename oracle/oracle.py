@@ -54,6 +54,11 @@ def lib():
         L.oracle_inv3.argtypes = [_dp, _dp]
         L.oracle_edge.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                   _dp, _dp, _dp, _dp, _dp]
+        _fp = C.POINTER(C.c_float)
+        L.oracle_kp_undistort.argtypes = [_fp, C.c_int32, _fp, C.c_int32, _fp]
+        L.oracle_kp_undistort.restype = None
+        L.oracle_kp_distort.argtypes = [_dp, C.c_int32, _fp, _dp]
+        L.oracle_kp_distort.restype = None
         _lib = L
     return _lib
 
@@ -154,3 +159,24 @@ def edge(q, t, X, K, uv):
     e = np.zeros(2); JX = np.zeros(6); Jxi = np.zeros(12); Xc = np.zeros(3)
     lib().oracle_edge(_d(q), _d(t), _d(X), K[0], K[1], K[2], K[3], _d(uv), _d(e), _d(JX), _d(Jxi), _d(Xc))
     return e, JX.reshape(2, 3), Jxi.reshape(2, 6), Xc
+
+
+def kp_undistort(uv: np.ndarray, params, iters: int = 5) -> np.ndarray:
+    """Undistorted key points (n, 2) float32, as the reference's
+    PinholeRadTan::UndistortKeyPoints (cv::undistortPoints, 5 iterations)."""
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    prm = np.ascontiguousarray(params, np.float32)
+    out = np.empty_like(uv)
+    fp = C.POINTER(C.c_float)
+    lib().oracle_kp_undistort(uv.ctypes.data_as(fp), uv.shape[0], prm.ctypes.data_as(fp), iters,
+                              out.ctypes.data_as(fp))
+    return out
+
+
+def kp_distort(xy: np.ndarray, params) -> np.ndarray:
+    """Forward radial-tangential model: normalised (n, 2) -> distorted pixels (FP64)."""
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    prm = np.ascontiguousarray(params, np.float32)
+    out = np.empty_like(xy)
+    lib().oracle_kp_distort(_d(xy), xy.shape[0], prm.ctypes.data_as(C.POINTER(C.c_float)), _d(out))
+    return out

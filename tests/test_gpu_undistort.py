@@ -1,0 +1,62 @@
+"""GPU key-point undistortion (nav_ba_kp_undistort) against the oracle: equal
+float results (the device evaluates the same double operations without fused
+multiply-adds), through the host entry and on device tensors, in place."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from parity_util import say as _say
+
+pytestmark = pytest.mark.gpu
+
+CAM = [450.0, 450.0, 320.0, 240.0]
+PARAMS = {
+    "none": CAM + [0.0, 0.0, 0.0, 0.0],
+    "mild": CAM + [-0.1, 0.01, 1e-4, -2e-4],
+    "strong": CAM + [-0.28, 0.07, 5e-4, 3e-4],
+    "barrel_fails": CAM + [-5.0, 0.0, 0.0, 0.0],  # radial factor negative at the corners
+    "pincushion": [458.6, 457.3, 367.2, 248.4, 0.05, -0.01, -8e-4, 6e-4],
+}
+
+
+def _pts(n, seed):
+    r = np.random.default_rng(seed)
+    return np.stack([r.uniform(-80, 720, n), r.uniform(-80, 560, n)], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", sorted(PARAMS))
+def test_host_entry_equals_oracle(name):
+    from nav24_amd.undistort import PinholeRadTanHip
+    _say(f"kp undistort host entry, params {name}, 200000 points")
+    uv = _pts(200000, 7)
+    got = PinholeRadTanHip(PARAMS[name]).UndistortKeyPoints(uv)
+    ref = oracle.kp_undistort(uv, PARAMS[name])
+    assert np.array_equal(got, ref)
+
+
+def test_device_entry_in_place_and_sizes():
+    import torch
+    from nav24_amd.undistort import PinholeRadTanHip
+    _say("kp undistort device tensors: in place, n = 1, 255, 256, 257, 3 000 000")
+    cam = PinholeRadTanHip(PARAMS["strong"])
+    for n in (1, 255, 256, 257, 3_000_000):
+        uv = _pts(n, n)
+        t = torch.from_numpy(uv).cuda()
+        out = cam.undistort_device(t)
+        cam.undistort_device(t, out=t)  # in place
+        torch.cuda.synchronize()
+        ref = oracle.kp_undistort(uv, PARAMS["strong"])
+        assert np.array_equal(out.cpu().numpy(), ref), n
+        assert np.array_equal(t.cpu().numpy(), ref), n
+
+
+def test_empty_input_and_argument_errors():
+    import ctypes
+    from nav24_amd.abi import NAV_BA_BAD_ARG, NAV_BA_OK
+    from nav24_amd.undistort import PinholeRadTanHip, lib
+    _say("kp undistort: n = 0 and argument checks")
+    out = PinholeRadTanHip(PARAMS["mild"]).UndistortKeyPoints(np.zeros((0, 2), np.float32))
+    assert out.shape == (0, 2)
+    prm = (ctypes.c_float * 8)(*PARAMS["mild"])
+    assert lib().nav_ba_kp_undistort(None, 0, prm, None, None) == NAV_BA_OK
+    assert lib().nav_ba_kp_undistort(None, -3, prm, None, None) == NAV_BA_BAD_ARG

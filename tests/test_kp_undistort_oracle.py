@@ -1,0 +1,76 @@
+"""Pins of the key-point undistortion oracle (oracle/kp_undistort_oracle.cpp),
+which restates PinholeRadTan::UndistortKeyPoints (PinholeRadTan.cpp:11-24),
+i.e. OpenCV's cv::undistortPoints with R = I, P = K.  OpenCV cannot run here
+and the reference holds no test vectors for it, so each check is a property
+the mathematics fixes."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from nav24_amd.abi import NAV_BA_BAD_ARG, NAV_BA_NO_DEVICE
+from oracle import oracle
+
+CAM = [450.0, 450.0, 320.0, 240.0]
+
+
+def _pts(n, seed, lo=(-50, -50), hi=(690, 530)):
+    r = np.random.default_rng(seed)
+    return np.stack([r.uniform(lo[0], hi[0], n), r.uniform(lo[1], hi[1], n)], 1).astype(np.float32)
+
+
+def test_zero_distortion_returns_the_input():
+    uv = _pts(20000, 1)
+    out = oracle.kp_undistort(uv, CAM + [0, 0, 0, 0])
+    assert np.array_equal(out, uv)
+
+
+def test_radial_only_keeps_the_principal_row_and_is_odd():
+    prm = CAM + [-0.28, 0.07, 0.0, 0.0]
+    u = np.linspace(10, 630, 257, dtype=np.float32)
+    row = np.stack([u, np.full_like(u, 240.0)], 1)
+    out = oracle.kp_undistort(row, prm)
+    assert np.all(out[:, 1] == np.float32(240.0))  # y = 0 is a fixed point of every iteration
+    a = _pts(5000, 2, lo=(0, 0), hi=(300, 230))
+    plus = oracle.kp_undistort(np.float32([320, 240]) + a, prm)
+    minus = oracle.kp_undistort(np.float32([320, 240]) - a, prm)
+    np.testing.assert_allclose((plus[:, 0] - 320.0) + (minus[:, 0] - 320.0), 0.0, atol=2e-4)
+    np.testing.assert_allclose((plus[:, 1] - 240.0) + (minus[:, 1] - 240.0), 0.0, atol=2e-4)
+
+
+def test_one_iteration_closed_form_on_the_principal_row():
+    # k2 = p1 = p2 = 0, y = 0: one step gives x1 = x0 / (1 + k1 x0^2)
+    k1 = -0.2
+    u = np.float32([400.0, 100.0, 639.0])
+    out = oracle.kp_undistort(np.stack([u, np.full_like(u, 240.0)], 1), CAM + [k1, 0, 0, 0], iters=1)
+    x0 = (u.astype(np.float64) - 320.0) / 450.0
+    np.testing.assert_allclose(out[:, 0], 450.0 * x0 / (1 + k1 * x0 * x0) + 320.0, rtol=0, atol=1e-4)
+
+
+@pytest.mark.parametrize("prm", [[-0.1, 0.01, 1e-4, -2e-4], [-0.28, 0.07, 5e-4, 3e-4], [0.05, -0.01, -8e-4, 6e-4]])
+def test_many_iterations_invert_the_forward_model(prm):
+    r = np.random.default_rng(3)
+    xy = np.stack([r.uniform(-0.6, 0.6, 4000), r.uniform(-0.45, 0.45, 4000)], 1)
+    uv_d = oracle.kp_distort(xy, CAM + prm).astype(np.float32)
+    back = oracle.kp_undistort(uv_d, CAM + prm, iters=60)
+    ideal = np.stack([450.0 * xy[:, 0] + 320.0, 450.0 * xy[:, 1] + 240.0], 1)
+    # the float32 distorted input (~3e-5 px) bounds the agreement
+    np.testing.assert_allclose(back, ideal, rtol=0, atol=2e-3)
+
+
+def test_negative_radial_factor_returns_the_input():
+    # 1 + k1 r^2 < 0 at the corners: the iteration stops at the normalised input point
+    uv = np.float32([[639.0, 479.0], [0.0, 0.0]])
+    out = oracle.kp_undistort(uv, CAM + [-5.0, 0.0, 0.0, 0.0])
+    assert np.array_equal(out, uv)
+
+
+def test_library_entry_points_check_arguments_without_a_device():
+    from nav24_amd.undistort import lib
+    L = lib()
+    prm = (ctypes.c_float * 8)(*(CAM + [0, 0, 0, 0]))
+    buf = (ctypes.c_float * 4)()
+    assert L.nav_ba_kp_undistort(None, -1, prm, None, None) == NAV_BA_BAD_ARG
+    assert L.nav_ba_kp_undistort(None, 2, prm, None, None) == NAV_BA_BAD_ARG
+    st = L.nav_ba_kp_undistort_host(buf, 2, prm, buf, 0)
+    assert st in (NAV_BA_NO_DEVICE, 0)  # 0 only on a GPU box
