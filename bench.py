@@ -328,8 +328,8 @@ def diffcost_leg(n_problems, steps):
 def undistort_leg(n_points, steps, with_cpu):
     """Key-point undistortion of the radial-tangential camera
     (PinholeRadTan::UndistortKeyPoints): n_points key points resident in HBM,
-    one per observation of the 3 M-observation loop; K launches timed with HIP
-    events on torch's stream.  8 bytes read and 8 written per point.  With the
+    one per observation of the 3 M-observation loop; each of K launches timed
+    with HIP events on torch's stream.  8 bytes read and 8 written per point.  With the
     CPU baseline: the oracle on one host thread over the same points, and
     whether the device result equals it."""
     import torch
@@ -343,13 +343,15 @@ def undistort_leg(n_points, steps, with_cpu):
     dst = torch.empty_like(src)
     cam.undistort_device(src, out=dst)
     torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
+    # each launch between its own pair of events (back to back, the host's
+    # per-call overhead would be timed instead of the kernel)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record()
         cam.undistort_device(src, out=dst)
-    e1.record()
+        b.record()
     torch.cuda.synchronize()
-    us = e0.elapsed_time(e1) * 1e3 / steps
+    us = sum(a.elapsed_time(b) for a, b in ev) * 1e3 / steps
     nbytes = 16 * n_points
     out = {"workload": f"{n_points} key points, 640x480 pinhole, k1 k2 p1 p2 = {prm[4:]}", "n_points": n_points,
            "us_per_launch": round(us, 2), "points_per_s": round(n_points / (us * 1e-6), 1),
