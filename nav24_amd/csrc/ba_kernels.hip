@@ -62,8 +62,10 @@ void launch_lm_reset(const DevProblem& P, const DevWork& W, hipStream_t st) {
 // Jacobian in T, Huber weight (stored as T), and the point's Hll / bl terms
 // formed and summed in FP64 (fixed order from LDS).  A point with more than
 // kTile observations has a tile of its own and is summed by a block reduction.
+// Registers capped at 72 (7 waves per SIMD instead of 6): the kernel waits on
+// its gathers, and the extra wave gave 6 % at 3 M observations.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_lin_pt(DevProblem P, const CamRT* __restrict__ cam,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_lin_pt(DevProblem P, const CamRT* __restrict__ cam,
                                                    const D4* __restrict__ X, const LMDev* lm,
                                                    T* __restrict__ w_out, double* __restrict__ Hl) {
     __shared__ double sh[9][kTile];
