@@ -366,6 +366,28 @@ def undistort_leg(n_points, steps, with_cpu):
         out["cpu_baseline"] = {"value": round(n_points / dt, 1), "unit": "points/s", "cores": 1, "kind": "port",
                                "sample": f"the same {n_points} points, FP64 oracle, 1 thread, {dt:.3f} s"}
         out["equal_to_oracle"] = bool(np.array_equal(dst.cpu().numpy(), ref))
+    # the Kannala-Brandt fisheye camera (KannalaBrandt8::UndistortKeyPoints) on the same points
+    from nav24_amd.undistort import KannalaBrandt8Hip
+    kb = [380.0, 380.0, 320.0, 240.0, 0.05, -0.01, 0.002, -0.0003]
+    fish = KannalaBrandt8Hip(kb)
+    fish.undistort_device(src, out=dst)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record()
+        fish.undistort_device(src, out=dst)
+        b.record()
+    torch.cuda.synchronize()
+    us_f = sum(a.elapsed_time(b) for a, b in ev) * 1e3 / steps
+    out["fisheye"] = {"params": kb, "us_per_launch": round(us_f, 2), "points_per_s": round(n_points / (us_f * 1e-6), 1),
+                      "achieved_GBs": round(nbytes / (us_f * 1e-6) / 1e9, 1)}
+    if with_cpu:
+        t0 = time.perf_counter()
+        ref = oracle.kp_undistort_fisheye(uv, kb)
+        dt = time.perf_counter() - t0
+        got = dst.cpu().numpy()
+        out["fisheye"]["cpu_baseline"] = {"value": round(n_points / dt, 1), "unit": "points/s", "cores": 1,
+                                          "kind": "port", "sample": f"the same points, 1 thread, {dt:.3f} s"}
+        out["fisheye"]["values_differing_from_oracle"] = int((got != ref).sum())
     return out
 
 

@@ -23,7 +23,8 @@
  * multiply-adds), so its output equals a plain double evaluation bit for bit.
  *
  * Stateless: no handle.  In place (uv_out == uv) is allowed, as the
- * reference's call is.
+ * reference's call is.  The same entry points exist for the Kannala-Brandt
+ * fisheye camera below.
  */
 #ifndef NAV_KP_UNDISTORT_H
 #define NAV_KP_UNDISTORT_H
@@ -46,6 +47,23 @@ int32_t nav_ba_kp_undistort(const float* uv, int32_t n, const float* params, flo
 /* Host arrays, through device `device` (copies in and out; blocks until done).
  * NAV_BA_NO_DEVICE without a usable gfx950 device. */
 int32_t nav_ba_kp_undistort_host(const float* uv, int32_t n, const float* params, float* uv_out, int32_t device);
+
+/* The Kannala-Brandt fisheye camera: KannalaBrandt8::UndistortKeyPoints
+ * (core/sensor/camera/models/KannalaBrandt8.cpp:231-242), which calls
+ * cv::fisheye::undistortPoints(pts, pts, K, D, R, P) with D = (k1, k2, k3, k4)
+ * (KannalaBrandt8.cpp:27-29), R = I and P = K.  The published algorithm of that
+ * call in OpenCV 4.2 (the reference's minimum version), in double precision:
+ *     pw = ((u - cx) / fx, (v - cy) / fy),  theta_d = min(|pw|, pi / 2)
+ *     theta_d > 1e-8:  theta = theta_d; up to 10 Newton steps on
+ *         theta (1 + k1 theta^2 + k2 theta^4 + k3 theta^6 + k4 theta^8) = theta_d,
+ *         stopping once a step is below 1e-8;  scale = tan(theta) / theta_d
+ *     otherwise scale = 1;  (u', v') = (RR pw scale) / (RR pw scale)_z, RR = P R.
+ * tan() comes from the device math library, so a result can differ from a
+ * host evaluation by a float ulp (not observed in the tests).  params: fx fy
+ * cx cy k1 k2 k3 k4 (the reference's vParameters order). */
+int32_t nav_ba_kp_undistort_fisheye(const float* uv, int32_t n, const float* params, float* uv_out, void* stream);
+int32_t nav_ba_kp_undistort_fisheye_host(const float* uv, int32_t n, const float* params, float* uv_out,
+                                         int32_t device);
 
 #ifdef __cplusplus
 }

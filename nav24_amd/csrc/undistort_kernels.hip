@@ -1,8 +1,12 @@
-// undistort_kernels.hip — key-point undistortion of the radial-tangential
-// pinhole camera (include/nav_kp_undistort.h): PinholeRadTan::UndistortKeyPoints
-// (core/sensor/camera/models/PinholeRadTan.cpp:11-24), i.e. OpenCV's
-// cv::undistortPoints with K, D = (k1 k2 p1 p2), R = I, P = K and its default
-// termination (5 fixed-point iterations).
+// undistort_kernels.hip — key-point undistortion on the device
+// (include/nav_kp_undistort.h):
+//   radial-tangential pinhole: PinholeRadTan::UndistortKeyPoints
+//     (core/sensor/camera/models/PinholeRadTan.cpp:11-24), i.e. OpenCV's
+//     cv::undistortPoints with K, D = (k1 k2 p1 p2), R = I, P = K and its
+//     default termination (5 fixed-point iterations);
+//   Kannala-Brandt fisheye: KannalaBrandt8::UndistortKeyPoints
+//     (KannalaBrandt8.cpp:231-242), i.e. cv::fisheye::undistortPoints with
+//     D = (k1 k2 k3 k4), R = I, P = K (up to 10 Newton steps on theta).
 //
 // One lane per key point, the double operations of that algorithm in its
 // order, without fused multiply-adds (contraction is off in this file), so
@@ -57,6 +61,60 @@ __global__ __launch_bounds__(256) void k_kp_undistort(const float2* __restrict__
     out[i] = make_float2((float)(xx * ww), (float)(yy * ww));
 }
 
+// Fisheye (Kannala-Brandt): OpenCV 4.2's fisheye::undistortPoints steps.
+struct KbParams {
+    double fx, fy, cx, cy;
+    double k[4];
+    double RR[9];  // P R (= K here), row-major
+};
+
+constexpr int kKbIters = 10;         // fisheye::undistortPoints: at most 10 Newton steps
+constexpr double kKbEps = 1e-8;      // ... stopping once a step is below this
+constexpr double kHalfPi = 1.5707963267948966;  // CV_PI / 2
+
+__global__ __launch_bounds__(256) void k_kp_undistort_fisheye(const float2* __restrict__ uv, int n, KbParams q,
+                                                              float2* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float2 p = uv[i];
+    const double pw0 = ((double)p.x - q.cx) / q.fx, pw1 = ((double)p.y - q.cy) / q.fy;
+    double scale = 1.0;
+    double theta_d = sqrt(pw0 * pw0 + pw1 * pw1);
+    theta_d = fmin(fmax(-kHalfPi, theta_d), kHalfPi);
+    if (theta_d > kKbEps) {
+        double theta = theta_d;
+        for (int j = 0; j < kKbIters; ++j) {
+            const double theta2 = theta * theta, theta4 = theta2 * theta2, theta6 = theta4 * theta2,
+                         theta8 = theta6 * theta2;
+            const double k0t2 = q.k[0] * theta2, k1t4 = q.k[1] * theta4, k2t6 = q.k[2] * theta6,
+                         k3t8 = q.k[3] * theta8;
+            const double fix = (theta * (1 + k0t2 + k1t4 + k2t6 + k3t8) - theta_d) /
+                               (1 + 3 * k0t2 + 5 * k1t4 + 7 * k2t6 + 9 * k3t8);
+            theta = theta - fix;
+            if (fabs(fix) < kKbEps) break;
+        }
+        scale = tan(theta) / theta_d;
+    }
+    const double pu0 = pw0 * scale, pu1 = pw1 * scale;
+    // RR (pu, 1) as a 3x3 times 3-vector product: ((0 + a0 b0) + a1 b1) + a2 b2
+    const double pr0 = ((0.0 + q.RR[0] * pu0) + q.RR[1] * pu1) + q.RR[2] * 1.0;
+    const double pr1 = ((0.0 + q.RR[3] * pu0) + q.RR[4] * pu1) + q.RR[5] * 1.0;
+    const double pr2 = ((0.0 + q.RR[6] * pu0) + q.RR[7] * pu1) + q.RR[8] * 1.0;
+    out[i] = make_float2((float)(pr0 / pr2), (float)(pr1 / pr2));
+}
+
+static KbParams kb_params(const float* prm) {
+    KbParams q;
+    q.fx = prm[0];
+    q.fy = prm[1];
+    q.cx = prm[2];
+    q.cy = prm[3];
+    for (int c = 0; c < 4; ++c) q.k[c] = prm[4 + c];
+    const double K[9] = {q.fx, 0.0, q.cx, 0.0, q.fy, q.cy, 0.0, 0.0, 1.0};
+    std::memcpy(q.RR, K, sizeof(K));
+    return q;
+}
+
 static KpParams kp_params(const float* prm) {
     KpParams q;
     const double fx = prm[0], fy = prm[1], cx = prm[2], cy = prm[3];
@@ -107,6 +165,33 @@ extern "C" int32_t nav_ba_kp_undistort_host(const float* uv, int32_t n, const fl
     int32_t st = NAV_BA_OK;
     if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
     if (st == NAV_BA_OK) st = nav_ba_kp_undistort(d, n, params, d, nullptr);
+    if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    (void)hipFree(d);
+    return st;
+}
+
+extern "C" int32_t nav_ba_kp_undistort_fisheye(const float* uv, int32_t n, const float* params, float* uv_out,
+                                               void* stream) {
+    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
+    if (n == 0) return NAV_BA_OK;
+    const KbParams q = kb_params(params);
+    hipLaunchKernelGGL(k_kp_undistort_fisheye, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float2*>(uv), n, q, reinterpret_cast<float2*>(uv_out));
+    return hipGetLastError() == hipSuccess ? NAV_BA_OK : NAV_BA_HIP_ERROR;
+}
+
+extern "C" int32_t nav_ba_kp_undistort_fisheye_host(const float* uv, int32_t n, const float* params, float* uv_out,
+                                                    int32_t device) {
+    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
+    if (!gfx950_device(device)) return NAV_BA_NO_DEVICE;
+    if (n == 0) return NAV_BA_OK;
+    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
+    const size_t bytes = sizeof(float) * 2 * (size_t)n;
+    float* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return NAV_BA_HIP_ERROR;
+    int32_t st = NAV_BA_OK;
+    if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    if (st == NAV_BA_OK) st = nav_ba_kp_undistort_fisheye(d, n, params, d, nullptr);
     if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
     (void)hipFree(d);
     return st;

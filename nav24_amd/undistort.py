@@ -1,4 +1,5 @@
-"""Key-point undistortion of the radial-tangential pinhole camera on the GPU.
+"""Key-point undistortion of the radial-tangential pinhole camera and of the
+Kannala-Brandt fisheye camera on the GPU.
 
 Host-side mirror of PinholeRadTan (core/sensor/camera/models/PinholeRadTan.hpp:11-21,
 PinholeRadTan.cpp:11-24): built from the same parameter vector (fx, fy, cx,
@@ -11,6 +12,9 @@ the calls raise.
     cam = PinholeRadTanHip([fx, fy, cx, cy, k1, k2, p1, p2])
     uv_ud = cam.UndistortKeyPoints(uv)            # numpy (n, 2) float32, host
     cam.undistort_device(uv_t, out=uv_t)          # torch (n, 2) float32 on cuda, in place
+
+KannalaBrandt8Hip (KannalaBrandt8.cpp:231-242, cv::fisheye::undistortPoints)
+has the same calls, with params fx fy cx cy k1 k2 k3 k4.
 """
 from __future__ import annotations
 
@@ -34,6 +38,10 @@ def lib() -> C.CDLL:
         L.nav_ba_kp_undistort.restype = C.c_int32
         L.nav_ba_kp_undistort_host.argtypes = [fp, C.c_int32, fp, fp, C.c_int32]
         L.nav_ba_kp_undistort_host.restype = C.c_int32
+        L.nav_ba_kp_undistort_fisheye.argtypes = [C.c_void_p, C.c_int32, fp, C.c_void_p, C.c_void_p]
+        L.nav_ba_kp_undistort_fisheye.restype = C.c_int32
+        L.nav_ba_kp_undistort_fisheye_host.argtypes = [fp, C.c_int32, fp, fp, C.c_int32]
+        L.nav_ba_kp_undistort_fisheye_host.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -46,6 +54,9 @@ def _check(st: int, what: str) -> None:
 class PinholeRadTanHip:
     """Radial-tangential pinhole camera (the reference's PinholeRadTan) whose
     key-point undistortion runs on device `device`."""
+
+    _host_entry = "nav_ba_kp_undistort_host"
+    _device_entry = "nav_ba_kp_undistort"
 
     def __init__(self, params: Sequence[float], device: int = 0):
         p = np.asarray(params, np.float32).reshape(-1)
@@ -61,9 +72,9 @@ class PinholeRadTanHip:
         a = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
         out = np.empty_like(a)
         fp = C.POINTER(C.c_float)
-        st = lib().nav_ba_kp_undistort_host(a.ctypes.data_as(fp), a.shape[0], self.params.ctypes.data_as(fp),
-                                            out.ctypes.data_as(fp), self.device)
-        _check(st, "nav_ba_kp_undistort_host")
+        st = getattr(lib(), self._host_entry)(a.ctypes.data_as(fp), a.shape[0], self.params.ctypes.data_as(fp),
+                                              out.ctypes.data_as(fp), self.device)
+        _check(st, self._host_entry)
         return out
 
     def undistort_device(self, uv, out: Optional[object] = None):
@@ -78,8 +89,17 @@ class PinholeRadTanHip:
         elif out.shape != uv.shape or out.dtype != torch.float32 or not out.is_contiguous() or not out.is_cuda:
             raise ValueError("out: contiguous (n, 2) float32 cuda tensor")
         stream = torch.cuda.current_stream(uv.device).cuda_stream
-        st = lib().nav_ba_kp_undistort(C.c_void_p(uv.data_ptr()), uv.shape[0],
-                                       self.params.ctypes.data_as(C.POINTER(C.c_float)),
-                                       C.c_void_p(out.data_ptr()), C.c_void_p(stream))
-        _check(st, "nav_ba_kp_undistort")
+        st = getattr(lib(), self._device_entry)(C.c_void_p(uv.data_ptr()), uv.shape[0],
+                                                self.params.ctypes.data_as(C.POINTER(C.c_float)),
+                                                C.c_void_p(out.data_ptr()), C.c_void_p(stream))
+        _check(st, self._device_entry)
         return out
+
+
+class KannalaBrandt8Hip(PinholeRadTanHip):
+    """Kannala-Brandt fisheye camera (the reference's KannalaBrandt8,
+    core/sensor/camera/models/KannalaBrandt8.cpp:231-242): params fx fy cx cy
+    k1 k2 k3 k4; the same calls."""
+
+    _host_entry = "nav_ba_kp_undistort_fisheye_host"
+    _device_entry = "nav_ba_kp_undistort_fisheye"

@@ -1,5 +1,5 @@
 // kp_undistort_oracle.cpp — CPU restatement of the key-point undistortion of
-// the radial-tangential pinhole camera.
+// the radial-tangential pinhole camera and of the Kannala-Brandt fisheye camera.
 //
 // TEST INFRASTRUCTURE ONLY: the checker of nav_ba_kp_undistort
 // (include/nav_kp_undistort.h); the product library never links or calls it.
@@ -28,6 +28,8 @@
 //     model (oracle_kp_distort) to 1e-9 px for mild distortion;
 //   - the first iteration equals its closed form on hand-picked points;
 //   - a radial factor that turns negative returns the input point.
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 
 extern "C" {
@@ -78,6 +80,69 @@ void oracle_kp_distort(const double* xy, int32_t n, const float* params, double*
         const double yd = y * rad + p1 * (r2 + 2.0 * y * y) + 2.0 * p2 * x * y;
         uv[2 * i] = fx * xd + cx;
         uv[2 * i + 1] = fy * yd + cy;
+    }
+}
+
+// Kannala-Brandt fisheye: KannalaBrandt8::UndistortKeyPoints
+// (core/sensor/camera/models/KannalaBrandt8.cpp:231-242) calls
+// cv::fisheye::undistortPoints(pts, pts, K, D = (k1..k4), R = I, P = K).
+// Restated from OpenCV 4.2's published fisheye::undistortPoints (the
+// reference's minimum version; later versions add a termination argument
+// and mark non-converged points): pw = (p - c) / f; theta_d = |pw| clipped
+// to [-pi/2, pi/2]; above 1e-8, up to 10 Newton steps on
+// theta (1 + k1 t^2 + k2 t^4 + k3 t^6 + k4 t^8) = theta_d, stopping once a
+// step is below 1e-8, and scale = tan(theta) / theta_d; then P R (pw scale, 1)
+// (a 3x3 product summed from 0) and the float result.  Parity: unpinned
+// against OpenCV (absent); pinned by tests/test_kp_undistort_oracle.py
+// (equidistant closed form at k = 0, collinearity with the principal point,
+// the principal point fixed, and round trips through oracle_kp_distort_fisheye).
+void oracle_kp_undistort_fisheye(const float* uv, int32_t n, const float* params, float* out) {
+    const double fx = params[0], fy = params[1], cx = params[2], cy = params[3];
+    const double k[4] = {params[4], params[5], params[6], params[7]};
+    const double RR[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};
+    const double half_pi = 1.5707963267948966;
+    for (int32_t i = 0; i < n; ++i) {
+        const double pw0 = ((double)uv[2 * i] - cx) / fx, pw1 = ((double)uv[2 * i + 1] - cy) / fy;
+        double scale = 1.0;
+        double theta_d = std::sqrt(pw0 * pw0 + pw1 * pw1);
+        theta_d = std::min(std::max(-half_pi, theta_d), half_pi);
+        if (theta_d > 1e-8) {
+            double theta = theta_d;
+            for (int j = 0; j < 10; ++j) {
+                const double theta2 = theta * theta, theta4 = theta2 * theta2, theta6 = theta4 * theta2,
+                             theta8 = theta6 * theta2;
+                const double k0t2 = k[0] * theta2, k1t4 = k[1] * theta4, k2t6 = k[2] * theta6, k3t8 = k[3] * theta8;
+                const double fix = (theta * (1 + k0t2 + k1t4 + k2t6 + k3t8) - theta_d) /
+                                   (1 + 3 * k0t2 + 5 * k1t4 + 7 * k2t6 + 9 * k3t8);
+                theta = theta - fix;
+                if (std::fabs(fix) < 1e-8) break;
+            }
+            scale = std::tan(theta) / theta_d;
+        }
+        const double pu0 = pw0 * scale, pu1 = pw1 * scale;
+        const double pr0 = ((0.0 + RR[0] * pu0) + RR[1] * pu1) + RR[2] * 1.0;
+        const double pr1 = ((0.0 + RR[3] * pu0) + RR[4] * pu1) + RR[5] * 1.0;
+        const double pr2 = ((0.0 + RR[6] * pu0) + RR[7] * pu1) + RR[8] * 1.0;
+        out[2 * i] = (float)(pr0 / pr2);
+        out[2 * i + 1] = (float)(pr1 / pr2);
+    }
+}
+
+// Forward Kannala-Brandt model (the one fisheye undistortion inverts), in
+// double: normalised pinhole (a, b) = (X / Z, Y / Z) -> distorted pixel,
+// theta = atan(|(a, b)|), theta_d = theta (1 + k1 theta^2 + ... + k4 theta^8),
+// u = fx theta_d / |(a, b)| a + cx (likewise v).
+void oracle_kp_distort_fisheye(const double* ab, int32_t n, const float* params, double* uv) {
+    const double fx = params[0], fy = params[1], cx = params[2], cy = params[3];
+    const double k1 = params[4], k2 = params[5], k3 = params[6], k4 = params[7];
+    for (int32_t i = 0; i < n; ++i) {
+        const double a = ab[2 * i], b = ab[2 * i + 1];
+        const double r = std::sqrt(a * a + b * b);
+        const double th = std::atan(r), t2 = th * th;
+        const double thd = th * (1 + t2 * (k1 + t2 * (k2 + t2 * (k3 + t2 * k4))));
+        const double s = r > 0 ? thd / r : 1.0;
+        uv[2 * i] = fx * s * a + cx;
+        uv[2 * i + 1] = fy * s * b + cy;
     }
 }
 

@@ -59,6 +59,10 @@ def lib():
         L.oracle_kp_undistort.restype = None
         L.oracle_kp_distort.argtypes = [_dp, C.c_int32, _fp, _dp]
         L.oracle_kp_distort.restype = None
+        L.oracle_kp_undistort_fisheye.argtypes = [_fp, C.c_int32, _fp, _fp]
+        L.oracle_kp_undistort_fisheye.restype = None
+        L.oracle_kp_distort_fisheye.argtypes = [_dp, C.c_int32, _fp, _dp]
+        L.oracle_kp_distort_fisheye.restype = None
         _lib = L
     return _lib
 
@@ -179,4 +183,25 @@ def kp_distort(xy: np.ndarray, params) -> np.ndarray:
     prm = np.ascontiguousarray(params, np.float32)
     out = np.empty_like(xy)
     lib().oracle_kp_distort(_d(xy), xy.shape[0], prm.ctypes.data_as(C.POINTER(C.c_float)), _d(out))
+    return out
+
+
+def kp_undistort_fisheye(uv: np.ndarray, params) -> np.ndarray:
+    """Undistorted key points (n, 2) float32 of the Kannala-Brandt camera, as
+    KannalaBrandt8::UndistortKeyPoints (cv::fisheye::undistortPoints)."""
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    prm = np.ascontiguousarray(params, np.float32)
+    out = np.empty_like(uv)
+    fp = C.POINTER(C.c_float)
+    lib().oracle_kp_undistort_fisheye(uv.ctypes.data_as(fp), uv.shape[0], prm.ctypes.data_as(fp),
+                                      out.ctypes.data_as(fp))
+    return out
+
+
+def kp_distort_fisheye(ab: np.ndarray, params) -> np.ndarray:
+    """Forward Kannala-Brandt model: normalised pinhole (n, 2) -> distorted pixels (FP64)."""
+    ab = np.ascontiguousarray(ab, np.float64).reshape(-1, 2)
+    prm = np.ascontiguousarray(params, np.float32)
+    out = np.empty_like(ab)
+    lib().oracle_kp_distort_fisheye(_d(ab), ab.shape[0], prm.ctypes.data_as(C.POINTER(C.c_float)), _d(out))
     return out

@@ -60,3 +60,40 @@ def test_empty_input_and_argument_errors():
     prm = (ctypes.c_float * 8)(*PARAMS["mild"])
     assert lib().nav_ba_kp_undistort(None, 0, prm, None, None) == NAV_BA_OK
     assert lib().nav_ba_kp_undistort(None, -3, prm, None, None) == NAV_BA_BAD_ARG
+
+
+KB_PARAMS = {
+    "zero": [380.0, 380.0, 320.0, 240.0, 0.0, 0.0, 0.0, 0.0],
+    "typical": [380.0, 380.0, 320.0, 240.0, 0.05, -0.01, 0.002, -0.0003],
+    "strong": [286.2, 286.4, 321.4, 244.9, -0.0124, 0.0574, -0.0573, 0.0120],
+}
+
+
+def _ulps(a, b):
+    return np.abs(a.astype(np.float64) - b.astype(np.float64)) / np.spacing(np.abs(b).astype(np.float32)).astype(np.float64)
+
+
+@pytest.mark.parametrize("name", sorted(KB_PARAMS))
+def test_fisheye_host_entry_matches_oracle(name):
+    from nav24_amd.undistort import KannalaBrandt8Hip
+    uv = _pts(200000, 11)
+    got = KannalaBrandt8Hip(KB_PARAMS[name]).UndistortKeyPoints(uv)
+    ref = oracle.kp_undistort_fisheye(uv, KB_PARAMS[name])
+    neq = int((got != ref).sum())
+    _say(f"kp undistort fisheye host entry, params {name}, 200000 points: {neq} values differ from the oracle")
+    assert np.all(_ulps(got, ref) <= 1.0)  # tan() of the device and host libraries may differ in the last bit
+
+
+def test_fisheye_device_entry_in_place():
+    import torch
+    from nav24_amd.undistort import KannalaBrandt8Hip
+    cam = KannalaBrandt8Hip(KB_PARAMS["typical"])
+    for n in (1, 257, 1_000_000):
+        uv = _pts(n, 100 + n)
+        t = torch.from_numpy(uv).cuda()
+        cam.undistort_device(t, out=t)
+        torch.cuda.synchronize()
+        ref = oracle.kp_undistort_fisheye(uv, KB_PARAMS["typical"])
+        got = t.cpu().numpy()
+        _say(f"kp undistort fisheye device in place n={n}: {int((got != ref).sum())} values differ")
+        assert np.all(_ulps(got, ref) <= 1.0), n

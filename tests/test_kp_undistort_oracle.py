@@ -74,3 +74,42 @@ def test_library_entry_points_check_arguments_without_a_device():
     assert L.nav_ba_kp_undistort(None, 2, prm, None, None) == NAV_BA_BAD_ARG
     st = L.nav_ba_kp_undistort_host(buf, 2, prm, buf, 0)
     assert st in (NAV_BA_NO_DEVICE, 0)  # 0 only on a GPU box
+
+
+# ---- Kannala-Brandt fisheye (KannalaBrandt8::UndistortKeyPoints, cv::fisheye::undistortPoints)
+
+KB = [380.0, 380.0, 320.0, 240.0]
+
+
+def test_fisheye_zero_coefficients_is_the_equidistant_inverse():
+    # k = 0: the model is r_d = theta, so the undistorted radius is tan(|pw|) (|pw| < pi / 2)
+    uv = _pts(20000, 4, lo=(0, 0), hi=(640, 480))
+    out = oracle.kp_undistort_fisheye(uv, KB + [0, 0, 0, 0]).astype(np.float64)
+    pw = (uv.astype(np.float64) - [320.0, 240.0]) / 380.0
+    td = np.hypot(pw[:, 0], pw[:, 1])
+    s = np.where(td > 1e-8, np.tan(np.minimum(td, np.pi / 2)) / np.where(td > 0, td, 1), 1.0)
+    ideal = pw * s[:, None] * 380.0 + [320.0, 240.0]
+    np.testing.assert_allclose(out, ideal, rtol=2e-6, atol=1e-4)
+
+
+def test_fisheye_fixes_the_principal_point_and_keeps_the_ray():
+    prm = KB + [0.05, -0.01, 0.002, -0.0003]
+    assert np.array_equal(oracle.kp_undistort_fisheye(np.float32([[320, 240]]), prm), np.float32([[320, 240]]))
+    uv = _pts(5000, 5, lo=(0, 0), hi=(640, 480))
+    out = oracle.kp_undistort_fisheye(uv, prm).astype(np.float64)
+    a, b = uv.astype(np.float64) - [320, 240], out - [320, 240]
+    cross = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
+    assert np.all(np.abs(cross) <= 1e-4 * np.hypot(*a.T) * np.hypot(*b.T) + 1e-3)  # collinear with the centre
+    assert np.all((a * b).sum(1) >= 0)  # same side of it
+
+
+@pytest.mark.parametrize("prm", [[0.05, -0.01, 0.002, -0.0003], [-0.02, 0.004, 0.0, 0.0], [0.1, 0.0, -0.01, 0.001]])
+def test_fisheye_inverts_the_forward_model(prm):
+    r = np.random.default_rng(6)
+    ang, rad = r.uniform(0, 2 * np.pi, 4000), r.uniform(0, np.tan(1.0), 4000)  # theta up to 1 rad
+    ab = np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)
+    uv = oracle.kp_distort_fisheye(ab, KB + prm).astype(np.float32)
+    back = oracle.kp_undistort_fisheye(uv, KB + prm).astype(np.float64)
+    ideal = ab * 380.0 + [320.0, 240.0]
+    # the float32 distorted input and the 1e-8 Newton tolerance bound the agreement
+    np.testing.assert_allclose(back, ideal, rtol=1e-5, atol=2e-3)
