@@ -391,6 +391,58 @@ def undistort_leg(n_points, steps, with_cpu):
     return out
 
 
+def twoview_leg(n_matches, n_hyp, steps, with_cpu):
+    """Two-view RANSAC scoring (TwoViewReconstruction::FindHomography /
+    FindFundamental scoring loops): n_matches matches of a synthetic planar
+    scene and n_hyp homography and n_hyp fundamental hypotheses resident in
+    HBM; each scoring call (all hypotheses, the kept one, its inlier flags)
+    timed between its own HIP events on torch's stream.  With the CPU
+    baseline: the oracle on one host thread over the same input, and whether
+    scores, kept index and flags equal it."""
+    import torch
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    from nav24_amd.twoview_synth import ransac_hypotheses, synthetic_two_view
+
+    sc = synthetic_two_view(n_matches, seed=1, planar=True)
+    H21, H12, F21 = ransac_hypotheses(sc, n_hyp, seed=1)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    k1, k2, mt, h21, h12, f21 = d(sc.kp1), d(sc.kp2), d(sc.matches), d(H21), d(H12), d(F21)
+    tv = TwoViewReconstructionHip()
+    outH = tv.score_device("H", k1, k2, mt, h21, h12)
+    outF = tv.score_device("F", k1, k2, mt, f21)
+    torch.cuda.synchronize()
+    res = {"workload": f"{n_matches} matches (planar scene, 20 % outliers), {n_hyp} H and {n_hyp} F hypotheses",
+           "n_matches": n_matches, "n_hyp": n_hyp}
+    for kind, out, call in (("homography", outH, lambda: tv.score_device("H", k1, k2, mt, h21, h12, out=outH)),
+                            ("fundamental", outF, lambda: tv.score_device("F", k1, k2, mt, f21, out=outF))):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record()
+            call()
+            b.record()
+        torch.cuda.synchronize()
+        us = sum(a.elapsed_time(b) for a, b in ev) * 1e3 / steps
+        res[kind] = {"us_per_call": round(us, 2),
+                     "match_hypotheses_per_s": round(n_matches * n_hyp / (us * 1e-6), 1),
+                     "kept": int(out[1].item()), "inliers": int(out[3].sum().item())}
+    if with_cpu:
+        from oracle import oracle
+        reps = 20
+        for kind, args in (("homography", (H21, H12)), ("fundamental", (F21,))):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, *args)
+            dt = (time.perf_counter() - t0) / reps
+            out = outH if kind == "homography" else outF
+            res[kind]["cpu_baseline"] = {"value": round(n_matches * n_hyp / dt, 1), "unit": "match-hypotheses/s",
+                                         "us_per_call": round(dt * 1e6, 1), "cores": 1, "kind": "port",
+                                         "sample": f"the same input, oracle, 1 thread, {reps} calls"}
+            res[kind]["equal_to_oracle"] = bool(
+                np.array_equal(out[0].cpu().numpy().view(np.uint32), ref[0].view(np.uint32))
+                and out[1].item() == ref[1] and np.array_equal(out[3].cpu().numpy(), ref[3]))
+    return res
+
+
 def copy_calibration(mb, steps):
     """A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
     scales the same counters on the solver kernels."""
@@ -459,6 +511,8 @@ def main():
                     help="differentiable-cost leg: number of distorted-camera problems (0 = skip)")
     ap.add_argument("--undistort", type=int, default=3_000_000,
                     help="key-point undistortion leg: number of key points (0 = skip)")
+    ap.add_argument("--twoview", type=int, default=1000,
+                    help="two-view RANSAC scoring leg: number of matches, 200 H and 200 F hypotheses (0 = skip)")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling: run only the batched mixed solve (K timed runs) and print its timing")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -564,6 +618,8 @@ def main():
         line["diffcost"] = diffcost_leg(args.diffcost, max(args.steps, 5))
     if args.undistort > 0:
         line["undistort"] = undistort_leg(args.undistort, max(args.steps, 20), not args.no_cpu)
+    if args.twoview > 0:
+        line["twoview"] = twoview_leg(args.twoview, 200, max(args.steps, 20), not args.no_cpu)
 
     if not args.no_cpu:
         cb, ref = cpu_baseline(prob, opts)

@@ -1,0 +1,84 @@
+/*
+ * nav_twoview.h — scoring of two-view RANSAC hypotheses on the device: the
+ * per-match half of map initialisation, which produces BA's first state.
+ *
+ * Replaces the scoring loops of TwoViewReconstruction::FindHomography and
+ * ::FindFundamental (core/operators/mapInit/OP_2ViewReconstruction.cpp:266-315
+ * and :317-366), i.e. CheckHomography (:447-528) and CheckFundamental
+ * (:530-611) called once per RANSAC iteration, followed by the keep-the-best
+ * rule `if (currentScore > score) { best = it; score = currentScore; }`
+ * starting from score = 0.  The hypotheses themselves (8-point sets, the SVD
+ * in ComputeH21 / ComputeF21, the normalisation T2inv Hn T1, H21.inv()) stay
+ * with the caller; every hypothesis is scored in one launch and the best is
+ * chosen on the device.
+ *
+ * Per match i (u1, v1) = kp1[matches[i].first], (u2, v2) = kp2[matches[i].second],
+ * all float as in the reference, evaluated left to right without fused
+ * multiply-adds:
+ *   homography:  w = 1 / (h31' u2 + h32' v2 + h33'),  x2in1 = ((h11' u2 + h12' v2 + h13') w, ...)
+ *                chi1 = |x1 - x2in1|^2 * (1 / sigma^2), and chi2 likewise with H21 and x1;
+ *                each chi <= th_score adds (th_score - chi) to the score, and
+ *                a match is an inlier when neither chi exceeds th_score.
+ *   fundamental: (a2, b2, c2) = F21 x1, chi1 = (a2 u2 + b2 v2 + c2)^2 / (a2^2 + b2^2) * (1 / sigma^2),
+ *                (a1, b1, c1) = F21^T x2, chi2 likewise; each chi <= th_f adds
+ *                (th_score - chi) to the score; inlier when neither exceeds th_f.
+ * The score of one hypothesis is summed in float in the reference's order
+ * (match 0 term 1, match 0 term 2, match 1 term 1, ...), so it equals the
+ * reference's float sum bit for bit.  A NaN chi counts as the reference's
+ * does (not above the threshold, so the NaN enters the score, and such a score
+ * never wins).  The reference's defaults: sigma 1, th_score 5.991, th_f 3.841
+ * (OP_2ViewReconstruction.hpp:32-44).
+ */
+#ifndef NAV_TWOVIEW_H
+#define NAV_TWOVIEW_H
+
+#include <stdint.h>
+#include "nav_ba.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Inputs (device arrays on `stream`, a hipStream_t, NULL = the null stream):
+ *   kp1, kp2       key points (u, v) float of image 1 and image 2 (cv::KeyPoint::pt)
+ *   matches        n_matches pairs (index into kp1, index into kp2) int32, the
+ *                  reference's mvMatches12; indices are not range-checked
+ *   H21, H12       n_hyp row-major 3x3 float matrices each (H12 = H21^-1 as the caller formed it)
+ *   F21            n_hyp row-major 3x3 float matrices
+ * Outputs (device):
+ *   scores         n_hyp floats, the score of every hypothesis
+ *   best           one int32: the index the reference keeps (the first one
+ *                  whose score is above every earlier score and above 0), -1 if none
+ *   best_score     one float: that score, 0 if none
+ *   inliers        n_matches bytes, 1 where the best hypothesis holds the
+ *                  match as an inlier, all 0 if none (vbMatchesInliers)
+ * n_matches, n_hyp >= 0; n_matches <= NAV_TV_MAX_MATCHES.  Returns after the
+ * launches are queued; NAV_BA_BAD_ARG for sizes out of range, a null pointer
+ * or sigma <= 0, NAV_BA_HIP_ERROR if a launch fails. */
+#define NAV_TV_MAX_MATCHES (1 << 24)
+
+int32_t nav_tv_score_homography(const float* kp1, const float* kp2, const int32_t* matches, int32_t n_matches,
+                                const float* H21, const float* H12, int32_t n_hyp, float sigma, float th_score,
+                                float* scores, int32_t* best, float* best_score, uint8_t* inliers, void* stream);
+
+int32_t nav_tv_score_fundamental(const float* kp1, const float* kp2, const int32_t* matches, int32_t n_matches,
+                                 const float* F21, int32_t n_hyp, float sigma, float th_f, float th_score,
+                                 float* scores, int32_t* best, float* best_score, uint8_t* inliers, void* stream);
+
+/* Host arrays, through device `device` (copies in and out; blocks until done);
+ * n1, n2 are the key-point counts of the two images, and matches are checked
+ * against them.  NAV_BA_NO_DEVICE without a usable gfx950 device. */
+int32_t nav_tv_score_homography_host(const float* kp1, int32_t n1, const float* kp2, int32_t n2,
+                                     const int32_t* matches, int32_t n_matches, const float* H21, const float* H12,
+                                     int32_t n_hyp, float sigma, float th_score, float* scores, int32_t* best,
+                                     float* best_score, uint8_t* inliers, int32_t device);
+
+int32_t nav_tv_score_fundamental_host(const float* kp1, int32_t n1, const float* kp2, int32_t n2,
+                                      const int32_t* matches, int32_t n_matches, const float* F21, int32_t n_hyp,
+                                      float sigma, float th_f, float th_score, float* scores, int32_t* best,
+                                      float* best_score, uint8_t* inliers, int32_t device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NAV_TWOVIEW_H */

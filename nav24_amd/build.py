@@ -26,7 +26,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["ba_kernels.hip", "schur_kernels.hip", "chol_kernels.hip", "nav_ba_host.cpp", "cost_kernels.hip",
-               "undistort_kernels.hip"]
+               "undistort_kernels.hip", "twoview_kernels.hip"]
 
 
 def _run(cmd):

@@ -1,0 +1,181 @@
+"""Scoring of two-view RANSAC hypotheses on the GPU.
+
+Host-side mirror of the scoring half of TwoViewReconstruction
+(core/operators/mapInit/OP_2ViewReconstruction.hpp:94, .cpp:266-366,
+:447-611): built from the same parameters (sigma, the chi-square thresholds
+mThChiSqScore and mThChiSqF of Params2VR) and exposing FindHomography /
+FindFundamental, which score all RANSAC hypotheses at once (CheckHomography /
+CheckFundamental of each) and keep the best by the reference's rule, with
+the reference's outputs (inlier flags, score, kept matrix); FindHomographyScores /
+FindFundamentalScores also return every score.  The
+hypotheses come from the caller (the 8-point solves stay on the host).  The
+arithmetic runs in the C ABI (include/nav_twoview.h); without the built library
+or a gfx950 device the calls raise.
+
+    tv = TwoViewReconstructionHip(sigma=1.0)
+    scores, best, best_score, inliers = tv.FindHomographyScores(kp1, kp2, matches, H21s, H12s)
+    scores, best, best_score, inliers = tv.FindFundamentalScores(kp1, kp2, matches, F21s)
+    tv.score_device("F", kp1_t, kp2_t, matches_t, F21s_t, out=(scores_t, best_t, best_score_t, inliers_t))
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+from ._loader import load_library
+from .abi import NAV_BA_OK, STATUS_NAMES
+
+# Params2VR defaults (OP_2ViewReconstruction.hpp:32-44)
+DEF_TH_CHISQ_SCORE = 5.991
+DEF_TH_CHISQ_F = 3.841
+DEF_2VR_SIGMA = 1.0
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        L = load_library("libnav24_ba.so")
+        fp, ip, u8 = C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)
+        v, i32, f32 = C.c_void_p, C.c_int32, C.c_float
+        L.nav_tv_score_homography.argtypes = [v, v, v, i32, v, v, i32, f32, f32, v, v, v, v, v]
+        L.nav_tv_score_homography.restype = i32
+        L.nav_tv_score_fundamental.argtypes = [v, v, v, i32, v, i32, f32, f32, f32, v, v, v, v, v]
+        L.nav_tv_score_fundamental.restype = i32
+        L.nav_tv_score_homography_host.argtypes = [fp, i32, fp, i32, ip, i32, fp, fp, i32, f32, f32, fp, ip, fp,
+                                                   u8, i32]
+        L.nav_tv_score_homography_host.restype = i32
+        L.nav_tv_score_fundamental_host.argtypes = [fp, i32, fp, i32, ip, i32, fp, i32, f32, f32, f32, fp, ip, fp,
+                                                    u8, i32]
+        L.nav_tv_score_fundamental_host.restype = i32
+        _lib = L
+    return _lib
+
+
+def _check(st: int, what: str) -> None:
+    if st != NAV_BA_OK:
+        raise RuntimeError(f"{what}: {STATUS_NAMES.get(st, st)}")
+
+
+def _pts(a) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float32).reshape(-1, 2)
+
+
+def _mats(a) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float32).reshape(-1, 9)
+
+
+class TwoViewReconstructionHip:
+    """Hypothesis scoring of the reference's TwoViewReconstruction on device
+    `device`.  Key points are (n, 2) float (cv::KeyPoint::pt), matches (m, 2)
+    int pairs (index in image 1, index in image 2), the reference's
+    mvMatches12; matrices 3x3 row-major float."""
+
+    def __init__(self, sigma: float = DEF_2VR_SIGMA, th_chisq_score: float = DEF_TH_CHISQ_SCORE,
+                 th_chisq_f: float = DEF_TH_CHISQ_F, device: int = 0):
+        if not sigma > 0:
+            raise ValueError("sigma must be > 0")
+        self.sigma = float(sigma)
+        self.th_score = float(th_chisq_score)
+        self.th_f = float(th_chisq_f)
+        self.device = device
+
+    # ---- all hypotheses of one RANSAC run ---------------------------------
+    def FindHomographyScores(self, kp1, kp2, matches, H21s, H12s) -> Tuple[np.ndarray, int, np.float32, np.ndarray]:
+        """Scores of every (H21, H12) pair, the index FindHomography keeps (-1
+        if none scores above 0), its score and its inlier flags (uint8)."""
+        return self._find(True, kp1, kp2, matches, H21s, H12s)
+
+    def FindFundamentalScores(self, kp1, kp2, matches, F21s) -> Tuple[np.ndarray, int, np.float32, np.ndarray]:
+        """As FindHomographyScores for fundamental-matrix hypotheses."""
+        return self._find(False, kp1, kp2, matches, F21s, None)
+
+    # ---- the reference's outputs: FindHomography / FindFundamental ----------
+    def FindHomography(self, kp1, kp2, matches, H21s, H12s):
+        """(vbMatchesInliers bool[m], score, H21) as FindHomography leaves them:
+        H21 None and every flag false when no hypothesis scores above 0."""
+        _, best, score, inl = self._find(True, kp1, kp2, matches, H21s, H12s)
+        return inl.astype(bool), float(score), (None if best < 0 else _mats(H21s)[best].reshape(3, 3).copy())
+
+    def FindFundamental(self, kp1, kp2, matches, F21s):
+        """(vbMatchesInliers bool[m], score, F21) as FindFundamental leaves them."""
+        _, best, score, inl = self._find(False, kp1, kp2, matches, F21s, None)
+        return inl.astype(bool), float(score), (None if best < 0 else _mats(F21s)[best].reshape(3, 3).copy())
+
+    def _find(self, hom: bool, kp1, kp2, matches, M0, M1):
+        k1, k2 = _pts(kp1), _pts(kp2)
+        m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+        A = _mats(M0)
+        B = _mats(M1) if hom else None
+        if hom and B.shape != A.shape:
+            raise ValueError("H21s and H12s differ in count")
+        n_hyp = A.shape[0]
+        scores = np.zeros(n_hyp, np.float32)
+        best = np.zeros(1, np.int32)
+        bs = np.zeros(1, np.float32)
+        inl = np.zeros(m.shape[0], np.uint8)
+        fp, ip, u8 = C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)
+        common = (k1.ctypes.data_as(fp), k1.shape[0], k2.ctypes.data_as(fp), k2.shape[0], m.ctypes.data_as(ip),
+                  m.shape[0])
+        outs = (scores.ctypes.data_as(fp), best.ctypes.data_as(ip), bs.ctypes.data_as(fp), inl.ctypes.data_as(u8),
+                self.device)
+        if hom:
+            st = lib().nav_tv_score_homography_host(*common, A.ctypes.data_as(fp), B.ctypes.data_as(fp), n_hyp,
+                                                    self.sigma, self.th_score, *outs)
+            _check(st, "nav_tv_score_homography_host")
+        else:
+            st = lib().nav_tv_score_fundamental_host(*common, A.ctypes.data_as(fp), n_hyp, self.sigma, self.th_f,
+                                                     self.th_score, *outs)
+            _check(st, "nav_tv_score_fundamental_host")
+        return scores, int(best[0]), bs[0], inl
+
+    # ---- device tensors -----------------------------------------------------
+    def score_device(self, kind: str, kp1, kp2, matches, M0, M1=None, out: Optional[tuple] = None):
+        """kind "H" (M0 = H21s, M1 = H12s) or "F" (M0 = F21s): torch tensors on
+        cuda — key points (n, 2) float32, matches (m, 2) int32, matrices (k, 3, 3)
+        or (k, 9) float32, all contiguous.  Returns (scores (k,) float32, best (1,)
+        int32, best_score (1,) float32, inliers (m,) uint8) on torch's current
+        stream; `out` may supply those four tensors."""
+        import torch
+
+        hom = kind == "H"
+        if kind not in ("H", "F"):
+            raise ValueError('kind: "H" or "F"')
+        dev = kp1.device
+
+        def need(t, dtype, name):
+            if t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{name}: contiguous {dtype} cuda tensor")
+
+        need(kp1, torch.float32, "kp1")
+        need(kp2, torch.float32, "kp2")
+        need(matches, torch.int32, "matches")
+        need(M0, torch.float32, "M0")
+        if hom:
+            if M1 is None:
+                raise ValueError("H12s needed for kind H")
+            need(M1, torch.float32, "M1")
+            if M1.numel() != M0.numel():
+                raise ValueError("H21s and H12s differ in count")
+        if M0.numel() % 9 or matches.numel() % 2 or kp1.numel() % 2 or kp2.numel() % 2:
+            raise ValueError("shapes: key points (n, 2), matches (m, 2), matrices (k, 9)")
+        n_hyp, n = M0.numel() // 9, matches.numel() // 2
+        if out is None:
+            out = (torch.empty(n_hyp, dtype=torch.float32, device=dev), torch.empty(1, dtype=torch.int32, device=dev),
+                   torch.empty(1, dtype=torch.float32, device=dev), torch.empty(n, dtype=torch.uint8, device=dev))
+        scores, best, bs, inl = out
+        if scores.numel() < n_hyp or inl.numel() < n:
+            raise ValueError("out tensors too small")
+        stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        if hom:
+            st = lib().nav_tv_score_homography(p(kp1), p(kp2), p(matches), n, p(M0), p(M1), n_hyp, self.sigma,
+                                               self.th_score, p(scores), p(best), p(bs), p(inl), stream)
+        else:
+            st = lib().nav_tv_score_fundamental(p(kp1), p(kp2), p(matches), n, p(M0), n_hyp, self.sigma, self.th_f,
+                                                self.th_score, p(scores), p(best), p(bs), p(inl), stream)
+        _check(st, "nav_tv_score_" + ("homography" if hom else "fundamental"))
+        return scores, best, bs, inl

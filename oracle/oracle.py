@@ -63,6 +63,15 @@ def lib():
         L.oracle_kp_undistort_fisheye.restype = None
         L.oracle_kp_distort_fisheye.argtypes = [_dp, C.c_int32, _fp, _dp]
         L.oracle_kp_distort_fisheye.restype = None
+        fp = _fp
+        L.oracle_tv_check_homography.argtypes = [fp, fp, _i32p, C.c_int32, fp, fp, C.c_float, C.c_float, _u8p]
+        L.oracle_tv_check_homography.restype = C.c_float
+        L.oracle_tv_check_fundamental.argtypes = [fp, fp, _i32p, C.c_int32, fp, C.c_float, C.c_float, C.c_float,
+                                                  _u8p]
+        L.oracle_tv_check_fundamental.restype = C.c_float
+        L.oracle_tv_find.argtypes = [C.c_int32, fp, fp, _i32p, C.c_int32, fp, fp, C.c_int32, C.c_float, C.c_float,
+                                     C.c_float, fp, _i32p, fp, _u8p, _u8p]
+        L.oracle_tv_find.restype = None
         _lib = L
     return _lib
 
@@ -205,3 +214,63 @@ def kp_distort_fisheye(ab: np.ndarray, params) -> np.ndarray:
     out = np.empty_like(ab)
     lib().oracle_kp_distort_fisheye(_d(ab), ab.shape[0], prm.ctypes.data_as(C.POINTER(C.c_float)), _d(out))
     return out
+
+
+# Defaults of the reference's two-view reconstruction (OP_2ViewReconstruction.hpp:32-44).
+TV_SIGMA = 1.0
+TV_TH_SCORE = 5.991
+TV_TH_F = 3.841
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def tv_check_homography(kp1, kp2, matches, H21, H12, sigma=TV_SIGMA, th=TV_TH_SCORE):
+    """CheckHomography: (score float32, inlier flags uint8[n])."""
+    fp = C.POINTER(C.c_float)
+    kp1 = _f32(kp1, (-1, 2)); kp2 = _f32(kp2, (-1, 2))
+    m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+    H21 = _f32(H21, 9); H12 = _f32(H12, 9)
+    inl = np.zeros(m.shape[0], np.uint8)
+    s = lib().oracle_tv_check_homography(kp1.ctypes.data_as(fp), kp2.ctypes.data_as(fp), m.ctypes.data_as(_i32p),
+                                         m.shape[0], H21.ctypes.data_as(fp), H12.ctypes.data_as(fp), sigma, th,
+                                         inl.ctypes.data_as(_u8p))
+    return np.float32(s), inl
+
+
+def tv_check_fundamental(kp1, kp2, matches, F21, sigma=TV_SIGMA, th=TV_TH_F, th_score=TV_TH_SCORE):
+    """CheckFundamental: (score float32, inlier flags uint8[n])."""
+    fp = C.POINTER(C.c_float)
+    kp1 = _f32(kp1, (-1, 2)); kp2 = _f32(kp2, (-1, 2))
+    m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+    F21 = _f32(F21, 9)
+    inl = np.zeros(m.shape[0], np.uint8)
+    s = lib().oracle_tv_check_fundamental(kp1.ctypes.data_as(fp), kp2.ctypes.data_as(fp), m.ctypes.data_as(_i32p),
+                                          m.shape[0], F21.ctypes.data_as(fp), sigma, th, th_score,
+                                          inl.ctypes.data_as(_u8p))
+    return np.float32(s), inl
+
+
+def tv_find(kp1, kp2, matches, M0, M1=None, sigma=TV_SIGMA, th=None, th_score=TV_TH_SCORE):
+    """Every hypothesis scored in turn and the kept one: M1 given -> homographies
+    (M0 = H21s, M1 = H12s), else fundamentals (M0 = F21s).  Returns
+    (scores float32[k], best int, best_score float32, inlier flags uint8[n])."""
+    fp = C.POINTER(C.c_float)
+    kp1 = _f32(kp1, (-1, 2)); kp2 = _f32(kp2, (-1, 2))
+    m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+    M0 = _f32(M0, (-1, 9))
+    hom = M1 is not None
+    M1a = _f32(M1, (-1, 9)) if hom else np.zeros((0, 9), np.float32)
+    if th is None:
+        th = th_score if hom else TV_TH_F
+    k = M0.shape[0]
+    scores = np.zeros(k, np.float32)
+    best = C.c_int32(0); bs = C.c_float(0)
+    inl = np.zeros(m.shape[0], np.uint8); work = np.zeros(m.shape[0], np.uint8)
+    lib().oracle_tv_find(int(hom), kp1.ctypes.data_as(fp), kp2.ctypes.data_as(fp), m.ctypes.data_as(_i32p),
+                         m.shape[0], M0.ctypes.data_as(fp), M1a.ctypes.data_as(fp), k, sigma, th, th_score,
+                         scores.ctypes.data_as(fp), C.byref(best), C.byref(bs), inl.ctypes.data_as(_u8p),
+                         work.ctypes.data_as(_u8p))
+    return scores, best.value, np.float32(bs.value), inl

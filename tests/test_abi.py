@@ -20,7 +20,7 @@ def declared(headers=None):
         headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
     for h in headers:
         src = open(os.path.join(ROOT, "include", h)).read()
-        names += re.findall(r"^\s*(?:const char\*|int32_t|void)\s+(nav_ba_\w+)\s*\(", src, re.M)
+        names += re.findall(r"^\s*(?:const char\*|int32_t|void)\s+(nav_\w+)\s*\(", src, re.M)
     return names
 
 
@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(lib_path("libnav24_ba.so"))
     names = declared()
     assert sorted(declared(["nav_ba.h", "nav_ba_debug.h"])) == sorted(ba.EXPORTED)
-    assert "nav_ba_cost_eval" in names
+    assert "nav_ba_cost_eval" in names and "nav_tv_score_fundamental_host" in names
     for n in names:
         assert hasattr(lib, n), n
 

@@ -1,0 +1,146 @@
+"""GPU scoring of two-view RANSAC hypotheses (nav_tv_score_homography /
+nav_tv_score_fundamental) against the oracle: equal float scores bit for bit
+(NaN where the oracle has NaN), the same kept hypothesis and score, the same
+inlier flags; through the host entry and on device tensors."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from parity_util import say as _say
+from nav24_amd.twoview_synth import ransac_hypotheses, synthetic_two_view
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_scores(a, b):
+    """Equal bit patterns, NaN where the other has NaN (a NaN's sign and
+    payload differ between the x86 default NaN and the GPU's)."""
+    a, b = np.atleast_1d(np.asarray(a, np.float32)), np.atleast_1d(np.asarray(b, np.float32))
+    if a.shape != b.shape or not np.array_equal(np.isnan(a), np.isnan(b)):
+        return False
+    k = ~np.isnan(a)
+    return np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32))
+
+
+def _scene(n, seed, planar, n_hyp):
+    sc = synthetic_two_view(max(n, 8), seed=seed, planar=planar)
+    H21, H12, F21 = ransac_hypotheses(sc, n_hyp, seed=seed)  # 8 matches per hypothesis
+    sc.matches = sc.matches[:n]
+    return sc, H21, H12, F21
+
+
+@pytest.mark.parametrize("n,planar", [(1000, True), (1000, False), (5000, True), (2048, False), (1, True)])
+def test_host_entry_equals_oracle(n, planar):
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say(f"two-view scoring host entry: {n} matches, planar {planar}, 200 hypotheses each H and F")
+    sc, H21, H12, F21 = _scene(n, 100 + n, planar, 200)
+    tv = TwoViewReconstructionHip()
+    got = tv.FindHomographyScores(sc.kp1, sc.kp2, sc.matches, H21, H12)
+    ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, H21, H12)
+    assert _same_scores(got[0], ref[0])
+    assert got[1] == ref[1] and _same_scores(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+    got = tv.FindFundamentalScores(sc.kp1, sc.kp2, sc.matches, F21)
+    ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, F21)
+    assert _same_scores(got[0], ref[0])
+    assert got[1] == ref[1] and _same_scores(got[2], ref[2]) and np.array_equal(got[3], ref[3])
+    if n >= 1000:
+        assert ref[1] >= 0 and ref[3].sum() > n // 4
+
+
+def test_reference_outputs_and_sigma():
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say("two-view scoring: FindHomography / FindFundamental outputs, sigma 1.5, other thresholds")
+    sc, H21, H12, F21 = _scene(1500, 9, True, 64)
+    tv = TwoViewReconstructionHip(sigma=1.5, th_chisq_score=7.0, th_chisq_f=4.5)
+    inl, score, H = tv.FindHomography(sc.kp1, sc.kp2, sc.matches, H21, H12)
+    _, best, bs, rinl = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, H21, H12, sigma=1.5, th_score=7.0)
+    assert np.array_equal(inl, rinl.astype(bool)) and np.float32(score) == bs
+    assert np.array_equal(H, H21[best])
+    inl, score, F = tv.FindFundamental(sc.kp1, sc.kp2, sc.matches, F21)
+    _, best, bs, rinl = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, F21, sigma=1.5, th=4.5, th_score=7.0)
+    assert np.array_equal(inl, rinl.astype(bool)) and np.float32(score) == bs
+    assert np.array_equal(F, F21[best])
+
+
+def test_keep_the_best_rule_nan_and_ties():
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say("two-view scoring: ties keep the first, NaN and zero scores never kept, no winner")
+    rng = np.random.default_rng(2)
+    n = 300
+    kp1 = rng.integers(0, 600, (n, 2)).astype(np.float32)
+    kp2 = kp1 + np.float32([20.0, 4.0]) + rng.integers(-2, 3, (n, 2)).astype(np.float32)
+    m = np.stack([np.arange(n), np.arange(n)], 1).astype(np.int32)
+
+    def H(tx, ty):
+        return (np.array([[1, 0, tx], [0, 1, ty], [0, 0, 1]], np.float32),
+                np.array([[1, 0, -tx], [0, 1, -ty], [0, 0, 1]], np.float32))
+    nan = (np.eye(3, dtype=np.float32), np.zeros((3, 3), np.float32))
+    hyps = [H(500, 500), H(21, 4), nan, H(20, 4), H(20, 4), nan] * 50  # 300 hypotheses: more than one block
+    A, B = np.stack([h[0] for h in hyps]), np.stack([h[1] for h in hyps])
+    tv = TwoViewReconstructionHip()
+    got = tv.FindHomographyScores(kp1, kp2, m, A, B)
+    ref = oracle.tv_find(kp1, kp2, m, A, B)
+    assert _same_scores(got[0], ref[0]) and np.isnan(got[0][2])
+    assert got[1] == ref[1] == 3 and got[2] == ref[2] and np.array_equal(got[3], ref[3])
+    A, B = np.stack([H(500, 500)[0], nan[0]]), np.stack([H(500, 500)[1], nan[1]])
+    got = tv.FindHomographyScores(kp1, kp2, m, A, B)
+    assert got[1] == -1 and got[2] == 0 and not got[3].any()
+    got = tv.FindHomographyScores(kp1, kp2, m, np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32))
+    assert got[0].size == 0 and got[1] == -1 and got[2] == 0 and not got[3].any()
+    F0 = np.zeros((1, 9), np.float32)  # every line is 0: 0/0 = NaN chi, a NaN score
+    got = tv.FindFundamentalScores(kp1, kp2, m, F0)
+    ref = oracle.tv_find(kp1, kp2, m, F0)
+    assert _same_scores(got[0], ref[0]) and got[1] == ref[1] == -1
+
+
+def test_device_tensors_and_reuse():
+    import torch
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say("two-view scoring on device tensors: H then F into the same outputs, then a smaller scene")
+    tv = TwoViewReconstructionHip()
+    sc, H21, H12, F21 = _scene(3000, 4, False, 200)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    k1, k2, mt = d(sc.kp1), d(sc.kp2), d(sc.matches)
+    out = tv.score_device("H", k1, k2, mt, d(H21), d(H12))
+    torch.cuda.synchronize()
+    ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, H21, H12)
+    assert _same_scores(out[0].cpu().numpy(), ref[0]) and out[1].item() == ref[1]
+    assert np.array_equal(out[3].cpu().numpy(), ref[3])
+    tv.score_device("F", k1, k2, mt, d(F21), out=out)
+    torch.cuda.synchronize()
+    ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, F21)
+    assert _same_scores(out[0].cpu().numpy(), ref[0]) and out[1].item() == ref[1]
+    assert _same_scores(out[2].cpu().numpy()[0], ref[2]) and np.array_equal(out[3].cpu().numpy(), ref[3])
+    sc, H21, H12, F21 = _scene(700, 5, True, 50)
+    k1, k2, mt = d(sc.kp1), d(sc.kp2), d(sc.matches)
+    tv.score_device("F", k1, k2, mt, d(F21), out=out)
+    torch.cuda.synchronize()
+    ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, F21)
+    assert _same_scores(out[0].cpu().numpy()[:50], ref[0]) and out[1].item() == ref[1]
+    assert np.array_equal(out[3].cpu().numpy()[:700], ref[3])
+
+
+def test_argument_errors():
+    from nav24_amd.abi import NAV_BA_BAD_ARG
+    from nav24_amd.twoview import TwoViewReconstructionHip, lib
+    _say("two-view scoring: argument checks")
+    L = lib()
+    f = ctypes.c_float
+    kp = (f * 4)(0, 0, 1, 1)
+    mt = (ctypes.c_int32 * 2)(0, 5)  # index 5 beyond 2 key points
+    H = (f * 9)(1, 0, 0, 0, 1, 0, 0, 0, 1)
+    sc, bs = (f * 1)(), (f * 1)()
+    best = (ctypes.c_int32 * 1)()
+    inl = (ctypes.c_uint8 * 1)()
+    args = (kp, 2, kp, 2, mt, 1, H, H, 1)
+    assert L.nav_tv_score_homography_host(*args, 1.0, 5.991, sc, best, bs, inl, 0) == NAV_BA_BAD_ARG
+    mt[1] = 1
+    assert L.nav_tv_score_homography_host(*args, 0.0, 5.991, sc, best, bs, inl, 0) == NAV_BA_BAD_ARG
+    assert L.nav_tv_score_homography_host(kp, 2, kp, 2, mt, -1, H, H, 1, 1.0, 5.991, sc, best, bs, inl,
+                                          0) == NAV_BA_BAD_ARG
+    assert L.nav_tv_score_fundamental(None, None, None, 1, H, 1, 1.0, 3.841, 5.991, sc, best, bs, inl,
+                                      None) == NAV_BA_BAD_ARG
+    with pytest.raises(ValueError):
+        TwoViewReconstructionHip(sigma=0.0)
