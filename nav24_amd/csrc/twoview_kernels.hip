@@ -4,13 +4,15 @@
 // every hypothesis of FindHomography / FindFundamental (:266-315, :317-366) at
 // once, and the keep-the-best rule of those loops.
 //
-// k_tv_score: one workgroup per hypothesis.  Its 256 lanes evaluate the two
-// chi-square terms of 2048 matches at a time (float, no fused multiply-adds:
-// contraction is off in this file) into LDS, then one lane adds them to the
-// running score in match order, so the score is the reference's sequential
-// float sum bit for bit.  That serial chain (2 adds per match) bounds the
-// kernel: ~2 N dependent adds of a few cycles each, against 16 B of key points
-// and 8 B of match per match, which come from L2 after the first hypothesis.
+// k_tv_score: one workgroup of 1024 lanes per hypothesis.  The lanes evaluate
+// the two chi-square terms of up to 2048 matches at a time (float, no fused
+// multiply-adds: contraction is off in this file), drop the terms that are 0
+// (matches above the threshold) and list the rest in LDS in match order; one
+// lane then adds the list to the running score in that order, so the score is
+// the reference's sequential float sum bit for bit.  That serial chain (one
+// dependent add per kept term, up to 2 per match) bounds the kernel; the 24 B
+// per match of key points and match indices come from L2 after the first
+// hypothesis, so HBM is not the limit.
 // k_tv_select: every workgroup picks the kept hypothesis from the scores
 // (first strict maximum above 0: the reference's `currentScore > score` scan
 // from score = 0) and writes the inlier flags of its share of the matches.
@@ -26,8 +28,9 @@
 
 namespace navba {
 
-constexpr int kTvThreads = 256;
-constexpr int kTvChunk = 2048;  // matches per LDS pass: 2 terms each, 16 KB
+constexpr int kTvThreads = 256;        // k_tv_select
+constexpr int kTvScoreThreads = 1024;  // k_tv_score: up to one match per lane per pass
+constexpr int kTvChunk = 2048;         // matches per LDS pass: 2 terms each, 16 KB (a multiple of 32)
 
 struct TvArgs {
     float inv_s2;    // 1 / sigma^2, formed in float as the reference does
@@ -92,13 +95,46 @@ __device__ __forceinline__ void tv_match(const float2* __restrict__ kp1, const f
         tv_terms_f(m0, x1.x, x1.y, x2.x, x2.y, a, t1, t2, in);
 }
 
+// Lane 0's in-order sum of terms[0, p): p a multiple of 64, read as float4
+// in two alternating batches of 8 so that the next batch's LDS reads are in
+// flight while the current one is added.
+__device__ __forceinline__ float tv_serial_sum(const float* terms, int p, float score) {
+    const float4* T = reinterpret_cast<const float4*>(terms);
+    const int nb2 = p >> 6;  // pairs of 8-float4 batches
+    float4 a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = T[k];
+    for (int i = 0; i < nb2; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) b[k] = T[16 * i + 8 + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            score += a[k].x;
+            score += a[k].y;
+            score += a[k].z;
+            score += a[k].w;
+        }
+        const int nx = 16 * min(i + 1, nb2 - 1);  // the last pass rereads a batch in range
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = T[nx + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            score += b[k].x;
+            score += b[k].y;
+            score += b[k].z;
+            score += b[k].w;
+        }
+    }
+    return score;
+}
+
 template <bool kHom>
-__global__ __launch_bounds__(kTvThreads) void k_tv_score(const float2* __restrict__ kp1,
-                                                         const float2* __restrict__ kp2,
-                                                         const int2* __restrict__ mt, int n,
-                                                         const float* __restrict__ M0,
-                                                         const float* __restrict__ M1, TvArgs a,
-                                                         float* __restrict__ scores) {
+__global__ __launch_bounds__(kTvScoreThreads) void k_tv_score(const float2* __restrict__ kp1,
+                                                              const float2* __restrict__ kp2,
+                                                              const int2* __restrict__ mt, int n,
+                                                              const float* __restrict__ M0,
+                                                              const float* __restrict__ M1, TvArgs a,
+                                                              float* __restrict__ scores) {
     __shared__ __align__(16) float terms[2 * kTvChunk];
     const int h = blockIdx.x;
     float m0[9], m1[9];
@@ -107,31 +143,45 @@ __global__ __launch_bounds__(kTvThreads) void k_tv_score(const float2* __restric
         m0[c] = M0[9 * h + c];
         m1[c] = kHom ? M1[9 * h + c] : 0.0f;
     }
+    __shared__ int wtot[kTvScoreThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     float score = 0.0f;  // lane 0 only
     for (int c0 = 0; c0 < n; c0 += kTvChunk) {
         const int cn = min(kTvChunk, n - c0);
-        for (int i = threadIdx.x; i < cn; i += kTvThreads) {
-            float t1, t2;
-            bool in;
-            tv_match<kHom>(kp1, kp2, mt, c0 + i, m0, m1, a, t1, t2, in);
-            *reinterpret_cast<float2*>(&terms[2 * i]) = make_float2(t1, t2);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            // the reference's order: term 1 then term 2 of each match in turn;
-            // adding 0 for a term above the threshold leaves the sum as skipping it does
-            const int m = 2 * cn;
-            int j = 0;
-#pragma unroll 4
-            for (; j + 4 <= m; j += 4) {
-                const float4 q = *reinterpret_cast<const float4*>(&terms[j]);
-                score += q.x;
-                score += q.y;
-                score += q.z;
-                score += q.w;
+        // The reference adds term 1 then term 2 of each match in turn.  A term
+        // above the threshold is skipped there; here it is 0 and is left out
+        // of the list, and so is any other +0 term: adding +0 to a sum that is
+        // never -0 leaves it unchanged.  The kept terms go to terms[] in match
+        // order (ballot counts within a wave, wave totals across the group).
+        int fill = 0;
+        for (int i0 = 0; i0 < cn; i0 += kTvScoreThreads) {
+            const int i = i0 + threadIdx.x;
+            float t1 = 0.0f, t2 = 0.0f;
+            if (i < cn) {
+                bool in;
+                tv_match<kHom>(kp1, kp2, mt, c0 + i, m0, m1, a, t1, t2, in);
             }
-            for (; j < m; ++j) score += terms[j];
+            const bool k1 = t1 != 0.0f, k2 = t2 != 0.0f;  // NaN is kept
+            const uint64_t b1 = __ballot(k1), b2 = __ballot(k2);
+            if (lane == 0) wtot[w] = __popcll(b1) + __popcll(b2);
+            __syncthreads();
+            int pos = fill + __popcll(b1 & below) + __popcll(b2 & below), tot = 0;
+#pragma unroll
+            for (int v = 0; v < kTvScoreThreads / 64; ++v) {
+                const int x = wtot[v];
+                pos += v < w ? x : 0;
+                tot += x;
+            }
+            if (k1) terms[pos++] = t1;
+            if (k2) terms[pos] = t2;
+            fill += tot;
+            __syncthreads();  // wtot is rewritten by the next pass
         }
+        const int p = (fill + 63) & ~63;  // zero padding to whole read batches
+        for (int i = fill + threadIdx.x; i < p; i += kTvScoreThreads) terms[i] = 0.0f;
+        __syncthreads();
+        if (threadIdx.x == 0 && p > 0) score = tv_serial_sum(terms, p, score);
         __syncthreads();
     }
     if (threadIdx.x == 0) scores[h] = score;
@@ -159,19 +209,30 @@ __global__ __launch_bounds__(kTvThreads) void k_tv_select(const float2* __restri
                                                           const float* __restrict__ scores, int n_hyp,
                                                           int32_t* __restrict__ best, float* __restrict__ best_score,
                                                           uint8_t* __restrict__ inliers) {
-    __shared__ TvBest red[kTvThreads];
+    __shared__ TvBest red[kTvThreads / 64];
+    // this lane's first match, loaded while the kept hypothesis is chosen
+    const int i0 = blockIdx.x * kTvThreads + threadIdx.x, stride = gridDim.x * kTvThreads;
+    float2 x1 = make_float2(0.0f, 0.0f), x2 = x1;
+    if (i0 < n) {
+        const int2 p = mt[i0];
+        x1 = kp1[p.x];
+        x2 = kp2[p.y];
+    }
     TvBest b{0.0f, -1};
     for (int k = threadIdx.x; k < n_hyp; k += kTvThreads) {
         const float s = scores[k];
         if (s > 0.0f) b = tv_better(b, TvBest{s, k});
     }
-    red[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = kTvThreads / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = tv_better(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const TvBest o{__shfl_xor(b.s, off), __shfl_xor(b.i, off)};
+        b = tv_better(b, o);
     }
-    const TvBest r = red[0];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    TvBest r = red[0];
+#pragma unroll
+    for (int w = 1; w < kTvThreads / 64; ++w) r = tv_better(r, red[w]);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *best = r.i;
         *best_score = r.i < 0 ? 0.0f : r.s;
@@ -183,11 +244,19 @@ __global__ __launch_bounds__(kTvThreads) void k_tv_select(const float2* __restri
         m0[c] = r.i < 0 ? 0.0f : M0[9 * h + c];
         m1[c] = (kHom && r.i >= 0) ? M1[9 * h + c] : 0.0f;
     }
-    for (int i = blockIdx.x * kTvThreads + threadIdx.x; i < n; i += gridDim.x * kTvThreads) {
+    for (int i = i0; i < n; i += stride) {
+        if (i != i0) {
+            const int2 p = mt[i];
+            x1 = kp1[p.x];
+            x2 = kp2[p.y];
+        }
         bool in = false;
         if (r.i >= 0) {
             float t1, t2;
-            tv_match<kHom>(kp1, kp2, mt, i, m0, m1, a, t1, t2, in);
+            if (kHom)
+                tv_terms_h(m0, m1, x1.x, x1.y, x2.x, x2.y, a, t1, t2, in);
+            else
+                tv_terms_f(m0, x1.x, x1.y, x2.x, x2.y, a, t1, t2, in);
         }
         inliers[i] = in ? 1 : 0;
     }
@@ -209,7 +278,7 @@ static int32_t tv_launch(const float* kp1, const float* kp2, const int32_t* matc
     const float2* k2 = reinterpret_cast<const float2*>(kp2);
     const int2* mt = reinterpret_cast<const int2*>(matches);
     if (n_hyp > 0)
-        hipLaunchKernelGGL(k_tv_score<kHom>, dim3(n_hyp), dim3(kTvThreads), 0, st, k1, k2, mt, n, M0, M1, a,
+        hipLaunchKernelGGL(k_tv_score<kHom>, dim3(n_hyp), dim3(kTvScoreThreads), 0, st, k1, k2, mt, n, M0, M1, a,
                            scores);
     const int blocks = std::max(1, std::min(1024, (n + kTvThreads - 1) / kTvThreads));
     hipLaunchKernelGGL(k_tv_select<kHom>, dim3(blocks), dim3(kTvThreads), 0, st, k1, k2, mt, n, M0, M1, a,
