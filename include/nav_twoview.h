@@ -1,6 +1,7 @@
 /*
- * nav_twoview.h — scoring of two-view RANSAC hypotheses on the device: the
- * per-match half of map initialisation, which produces BA's first state.
+ * nav_twoview.h — the per-match work of two-view map initialisation on the
+ * device, which produces BA's first state: scoring of the RANSAC hypotheses,
+ * and triangulation with the cheirality test of the motion candidates.
  *
  * Replaces the scoring loops of TwoViewReconstruction::FindHomography and
  * ::FindFundamental (core/operators/mapInit/OP_2ViewReconstruction.cpp:266-315
@@ -77,6 +78,49 @@ int32_t nav_tv_score_fundamental_host(const float* kp1, int32_t n1, const float*
                                       const int32_t* matches, int32_t n_matches, const float* F21, int32_t n_hyp,
                                       float sigma, float th_f, float th_score, float* scores, int32_t* best,
                                       float* best_score, uint8_t* inliers, int32_t device);
+
+/* Motion candidates: TwoViewReconstruction::CheckRT (OP_2ViewReconstruction.cpp:1244-1352)
+ * with Triangulate (:1179-1194) for n_cand candidates (R, t) at once, as
+ * ReconstructF (4 candidates, :612-700) and ReconstructH (8) call it.  Per
+ * inlier match: the 4x4 system of Triangulate, its null vector, then the
+ * reference's float tests in order (finite point; parallax cosine between the
+ * rays from both camera centres; depth > 0 in both cameras unless
+ * cos >= min_parallax_cos; squared reprojection error <= th2 in both images).
+ * The reference takes the null vector from OpenCV's float SVD; here it comes
+ * from a one-sided Jacobi SVD in double, rounded to float, and matrix
+ * products, norms and dot products sum float products in double, rounded to
+ * float.  So results equal the reference's up to OpenCV's own rounding, and
+ * a match close to a threshold can fall on the other side.
+ * Inputs:  kp1 (n_kp1 points), kp2, matches (n_matches pairs), inliers
+ *          (n_matches bytes, vbMatchesInliers), K (3x3 row-major float,
+ *          host), R (n_cand row-major 3x3, host), t (n_cand x 3, host),
+ *          th2 (the reference passes 4 sigma^2), min_parallax_cos
+ *          (mMinParallaxCRT, 0.99998), min_triangulated (mMinTriangulated, 50).
+ * Outputs per candidate c (device for nav_tv_check_rt, host for _host):
+ *   n_good[c]         CheckRT's return value
+ *   parallax[c]       its parallax (degrees): acos of cos_sel[c], 0 if n_good is 0
+ *   cos_sel[c]        the cosine at index min(min_triangulated, n_good - 1) of the
+ *                     ascending passing cosines (NaN if none)
+ *   p3d[c][n_kp1][3]  vP3D: the point at the image-1 key point of each passing
+ *                     match, 0 elsewhere
+ *   good[c][n_kp1]    vbGood: 1 where a passing match has cos < min_parallax_cos
+ *   cos_parallax[c][n_matches], passed[c][n_matches] (device entry only):
+ *                     each match's cosine and pass flag (also the selection's input)
+ * 1 <= n_cand <= NAV_TV_MAX_CANDIDATES.  The device entry does not range-check
+ * the match indices; the _host entry does. */
+#define NAV_TV_MAX_CANDIDATES 8
+
+int32_t nav_tv_check_rt(const float* kp1, int32_t n_kp1, const float* kp2, const int32_t* matches,
+                        const uint8_t* inliers, int32_t n_matches, const float* K, const float* R, const float* t,
+                        int32_t n_cand, float th2, float min_parallax_cos, int32_t min_triangulated,
+                        int32_t* n_good, float* parallax, float* cos_sel, float* p3d, uint8_t* good,
+                        float* cos_parallax, uint8_t* passed, void* stream);
+
+int32_t nav_tv_check_rt_host(const float* kp1, int32_t n_kp1, const float* kp2, int32_t n_kp2,
+                             const int32_t* matches, const uint8_t* inliers, int32_t n_matches, const float* K,
+                             const float* R, const float* t, int32_t n_cand, float th2, float min_parallax_cos,
+                             int32_t min_triangulated, int32_t* n_good, float* parallax, float* cos_sel, float* p3d,
+                             uint8_t* good, int32_t device);
 
 #ifdef __cplusplus
 }

@@ -262,6 +262,242 @@ __global__ __launch_bounds__(kTvThreads) void k_tv_select(const float2* __restri
     }
 }
 
+// ---- CheckRT (OP_2ViewReconstruction.cpp:1244-1352) with Triangulate (:1179-1194)
+//
+// k_tv_check_rt: one lane per (match, motion candidate).  Each inlier match is
+// triangulated from the 4x4 system of Triangulate, its null vector taken by a
+// one-sided Jacobi SVD in double and rounded to float; then CheckRT's float
+// tests in its order: finite point, parallax cosine, depth in front of both
+// cameras unless the parallax is tiny, reprojection error in both images.
+// Matrix products, norms and dot products accumulate float products in
+// double and round to float, as OpenCV's 32F gemm / norm / dot do.  A passing
+// match writes its point and good flag at its image-1 key point, and its
+// cosine and pass flag at its own index.
+// k_tv_rt_select: one workgroup per candidate counts the passing matches and
+// selects the cosine at index min(min_tri, nGood - 1) of their ascending
+// order (the reference sorts and indexes) by a 4-pass radix select over
+// order-preserving keys of the float bits; parallax = acos of it in degrees.
+
+constexpr int kTvMaxCand = NAV_TV_MAX_CANDIDATES;
+
+struct RtCand {
+    float P2[12];  // K [R|t], row-major 3x4
+    float O2[3];   // -R^T t
+    float R[9];
+    float t[3];
+};
+
+struct RtArgs {
+    float fx, fy, cx, cy;
+    float P1[12];  // K [I|0]
+    float th2, min_cos;
+    RtCand c[kTvMaxCand];
+};
+
+// Right singular vector of the smallest singular value of the 4x4 row-major a
+// (overwritten): Jacobi rotations on column pairs (0,1) (0,2) (0,3) (1,2)
+// (1,3) (2,3), up to 30 sweeps, a pair skipped once orthogonal to 1e-15 of
+// its norms; the column of V with the smallest rotated norm (the first of equal).
+__device__ void tv_null_vector4(double* a, double* x) {
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        bool rotated = false;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 4; ++q) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    al = al + a[4 * r + p] * a[4 * r + p];
+                    be = be + a[4 * r + q] * a[4 * r + q];
+                    ga = ga + a[4 * r + p] * a[4 * r + q];
+                }
+                if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double ap = a[4 * r + p], aq = a[4 * r + q];
+                    a[4 * r + p] = c * ap - s * aq;
+                    a[4 * r + q] = s * ap + c * aq;
+                    const double vp = v[4 * r + p], vq = v[4 * r + q];
+                    v[4 * r + p] = c * vp - s * vq;
+                    v[4 * r + q] = s * vp + c * vq;
+                }
+            }
+        if (!rotated) break;
+    }
+    int k = 0;
+    double best = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double nrm = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) nrm = nrm + a[4 * r + c] * a[4 * r + c];
+        if (c == 0 || nrm < best) {
+            best = nrm;
+            k = c;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[r] = k == 0 ? v[4 * r] : k == 1 ? v[4 * r + 1] : k == 2 ? v[4 * r + 2] : v[4 * r + 3];
+}
+
+// a . b over 3 floats (b with stride sb), products and sum in double, rounded to float
+__device__ __forceinline__ float tv_dot3(const float* a, const float* b, int sb) {
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r = r + (double)a[k] * (double)b[sb * k];
+    return (float)r;
+}
+
+__device__ __forceinline__ float tv_norm3(const float* x) {
+    return (float)sqrt(((0.0 + (double)x[0] * x[0]) + (double)x[1] * x[1]) + (double)x[2] * x[2]);
+}
+
+__global__ __launch_bounds__(256) void k_tv_rt_clear(float* __restrict__ p3d, uint8_t* __restrict__ good, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    p3d[3 * i] = 0.0f;
+    p3d[3 * i + 1] = 0.0f;
+    p3d[3 * i + 2] = 0.0f;
+    good[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_tv_check_rt(const float2* __restrict__ kp1, const float2* __restrict__ kp2,
+                                                     const int2* __restrict__ mt, const uint8_t* __restrict__ inl,
+                                                     int n, int n_kp1, RtArgs g, float* __restrict__ p3d,
+                                                     uint8_t* __restrict__ good, float* __restrict__ cosp,
+                                                     uint8_t* __restrict__ passed) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int c = blockIdx.y;
+    const RtCand& C = g.c[c];
+    float cp = 0.0f;
+    uint8_t ok = 0;
+    if (inl[i]) {
+        const int2 m = mt[i];
+        const float2 x1 = kp1[m.x], x2 = kp2[m.y];
+        double A[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            A[q] = (double)(x1.x * g.P1[8 + q] - g.P1[q]);
+            A[4 + q] = (double)(x1.y * g.P1[8 + q] - g.P1[4 + q]);
+            A[8 + q] = (double)(x2.x * C.P2[8 + q] - C.P2[q]);
+            A[12 + q] = (double)(x2.y * C.P2[8 + q] - C.P2[4 + q]);
+        }
+        double xh[4];
+        tv_null_vector4(A, xh);
+        const float w = (float)xh[3];
+        const float X[3] = {(float)xh[0] / w, (float)xh[1] / w, (float)xh[2] / w};
+        do {
+            if (!isfinite(X[0]) || !isfinite(X[1]) || !isfinite(X[2])) break;
+            const float n2[3] = {X[0] - C.O2[0], X[1] - C.O2[1], X[2] - C.O2[2]};
+            const float d1 = tv_norm3(X), d2 = tv_norm3(n2);
+            const double dt = ((0.0 + (double)X[0] * n2[0]) + (double)X[1] * n2[1]) + (double)X[2] * n2[2];
+            const float cosP = (float)(dt / (double)(d1 * d2));
+            if (X[2] <= 0 && cosP < g.min_cos) break;
+            float X2[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) X2[r] = tv_dot3(C.R + 3 * r, X, 1) + C.t[r];
+            if (X2[2] <= 0 && cosP < g.min_cos) break;
+            const float iz1 = 1.0f / X[2];
+            const float im1x = g.fx * X[0] * iz1 + g.cx, im1y = g.fy * X[1] * iz1 + g.cy;
+            const float e1 = (im1x - x1.x) * (im1x - x1.x) + (im1y - x1.y) * (im1y - x1.y);
+            if (e1 > g.th2) break;
+            const float iz2 = 1.0f / X2[2];
+            const float im2x = g.fx * X2[0] * iz2 + g.cx, im2y = g.fy * X2[1] * iz2 + g.cy;
+            const float e2 = (im2x - x2.x) * (im2x - x2.x) + (im2y - x2.y) * (im2y - x2.y);
+            if (e2 > g.th2) break;
+            ok = 1;
+            cp = cosP;
+            float* P = p3d + 3 * ((size_t)n_kp1 * c + m.x);
+            P[0] = X[0];
+            P[1] = X[1];
+            P[2] = X[2];
+            good[(size_t)n_kp1 * c + m.x] = cosP < g.min_cos ? 1 : 0;
+        } while (false);
+    }
+    cosp[(size_t)n * c + i] = cp;
+    passed[(size_t)n * c + i] = ok;
+}
+
+// order-preserving unsigned key of a float (NaN above +inf)
+__device__ __forceinline__ uint32_t tv_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float tv_unkey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+constexpr int kTvSelThreads = 1024;
+
+__global__ __launch_bounds__(kTvSelThreads) void k_tv_rt_select(const float* __restrict__ cosp,
+                                                                const uint8_t* __restrict__ passed, int n,
+                                                                int min_tri, int32_t* __restrict__ n_good,
+                                                                float* __restrict__ parallax,
+                                                                float* __restrict__ cos_sel) {
+    __shared__ int hist[256];
+    __shared__ int wsum[kTvSelThreads / 64];
+    __shared__ int sel[2];  // digit, rank left
+    const int c = blockIdx.x;
+    const float* cs = cosp + (size_t)n * c;
+    const uint8_t* ps = passed + (size_t)n * c;
+    int cnt = 0;
+    for (int i = threadIdx.x; i < n; i += kTvSelThreads) cnt += ps[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    int ng = 0;
+#pragma unroll
+    for (int w = 0; w < kTvSelThreads / 64; ++w) ng += wsum[w];
+    if (ng == 0) {
+        if (threadIdx.x == 0) {
+            n_good[c] = 0;
+            parallax[c] = 0.0f;
+            cos_sel[c] = __uint_as_float(0x7fc00000u);
+        }
+        return;
+    }
+    int rank = min(min_tri, ng - 1);
+    uint32_t prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256; b += kTvSelThreads) hist[b] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kTvSelThreads) {
+            if (!ps[i]) continue;
+            const uint32_t k = tv_key(cs[i]);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int r = rank, b = 0;
+            for (; b < 255 && r >= hist[b]; ++b) r -= hist[b];
+            sel[0] = b;
+            sel[1] = r;
+        }
+        __syncthreads();
+        prefix |= (uint32_t)sel[0] << shift;
+        mask |= 255u << shift;
+        rank = sel[1];
+        __syncthreads();  // hist and sel are rewritten by the next pass
+    }
+    if (threadIdx.x == 0) {
+        const float cv = tv_unkey(prefix);
+        n_good[c] = ng;
+        cos_sel[c] = cv;
+        parallax[c] = static_cast<float>(acosf(cv) * 180 / 3.14159265358979323846);
+    }
+}
+
 static TvArgs tv_args(float sigma, float th, float th_score) {
     TvArgs a;
     a.inv_s2 = 1.0f / (sigma * sigma);
@@ -412,4 +648,140 @@ extern "C" int32_t nav_tv_score_fundamental_host(const float* kp1, int32_t n1, c
     if (!tv_matches_ok(matches, n_matches, n1, n2)) return NAV_BA_BAD_ARG;
     return tv_host<false>(kp1, n1, kp2, n2, matches, n_matches, F21, nullptr, n_hyp, tv_args(sigma, th_f, th_score),
                           scores, best, best_score, inliers, device);
+}
+
+// ---- CheckRT entry points
+
+static bool rt_args(const float* K, const float* R, const float* t, int32_t n_cand, float th2, float min_cos,
+                    RtArgs& g) {
+    if (!K || !R || !t || n_cand < 1 || n_cand > kTvMaxCand) return false;
+    g = RtArgs{};
+    g.fx = K[0];
+    g.fy = K[4];
+    g.cx = K[2];
+    g.cy = K[5];
+    for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 4; ++q) g.P1[4 * r + q] = q < 3 ? K[3 * r + q] : 0.0f;
+    g.th2 = th2;
+    g.min_cos = min_cos;
+    for (int c = 0; c < n_cand; ++c) {
+        RtCand& C = g.c[c];
+        const float* Rc = R + 9 * c;
+        const float* tc = t + 3 * c;
+        std::memcpy(C.R, Rc, sizeof(C.R));
+        std::memcpy(C.t, tc, sizeof(C.t));
+        float Rt[12];
+        for (int r = 0; r < 3; ++r) {
+            for (int q = 0; q < 3; ++q) Rt[4 * r + q] = Rc[3 * r + q];
+            Rt[4 * r + 3] = tc[r];
+        }
+        // P2 = K [R|t] and O2 = -R^T t, float products summed in double and rounded to float
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 4; ++q) {
+                double acc = 0.0;
+                for (int k = 0; k < 3; ++k) acc = acc + (double)K[3 * r + k] * (double)Rt[4 * k + q];
+                C.P2[4 * r + q] = (float)acc;
+            }
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) acc = acc + (double)Rc[3 * k + r] * (double)tc[k];
+            C.O2[r] = -(float)acc;
+        }
+    }
+    return true;
+}
+
+extern "C" int32_t nav_tv_check_rt(const float* kp1, int32_t n_kp1, const float* kp2, const int32_t* matches,
+                                   const uint8_t* inliers, int32_t n_matches, const float* K, const float* R,
+                                   const float* t, int32_t n_cand, float th2, float min_parallax_cos,
+                                   int32_t min_triangulated, int32_t* n_good, float* parallax, float* cos_sel,
+                                   float* p3d, uint8_t* good, float* cos_parallax, uint8_t* passed, void* stream) {
+    RtArgs g;
+    if (n_matches < 0 || n_matches > NAV_TV_MAX_MATCHES || n_kp1 < 0 || min_triangulated < 0) return NAV_BA_BAD_ARG;
+    if (!rt_args(K, R, t, n_cand, th2, min_parallax_cos, g)) return NAV_BA_BAD_ARG;
+    if (!n_good || !parallax || !cos_sel) return NAV_BA_BAD_ARG;
+    if (n_kp1 > 0 && (!p3d || !good)) return NAV_BA_BAD_ARG;
+    if (n_matches > 0 && (!kp1 || !kp2 || !matches || !inliers || !cos_parallax || !passed)) return NAV_BA_BAD_ARG;
+    if ((size_t)n_kp1 * n_cand > (size_t)INT32_MAX) return NAV_BA_BAD_ARG;
+    const hipStream_t st = (hipStream_t)stream;
+    const int nk = n_kp1 * n_cand;
+    if (nk > 0) hipLaunchKernelGGL(k_tv_rt_clear, dim3((nk + 255) / 256), dim3(256), 0, st, p3d, good, nk);
+    if (n_matches > 0)
+        hipLaunchKernelGGL(k_tv_check_rt, dim3((n_matches + 255) / 256, n_cand), dim3(256), 0, st,
+                           reinterpret_cast<const float2*>(kp1), reinterpret_cast<const float2*>(kp2),
+                           reinterpret_cast<const int2*>(matches), inliers, n_matches, n_kp1, g, p3d, good,
+                           cos_parallax, passed);
+    hipLaunchKernelGGL(k_tv_rt_select, dim3(n_cand), dim3(kTvSelThreads), 0, st, (const float*)cos_parallax,
+                       (const uint8_t*)passed, n_matches, min_triangulated, n_good, parallax, cos_sel);
+    return hipGetLastError() == hipSuccess ? NAV_BA_OK : NAV_BA_HIP_ERROR;
+}
+
+extern "C" int32_t nav_tv_check_rt_host(const float* kp1, int32_t n_kp1, const float* kp2, int32_t n_kp2,
+                                        const int32_t* matches, const uint8_t* inliers, int32_t n_matches,
+                                        const float* K, const float* R, const float* t, int32_t n_cand, float th2,
+                                        float min_parallax_cos, int32_t min_triangulated, int32_t* n_good,
+                                        float* parallax, float* cos_sel, float* p3d, uint8_t* good,
+                                        int32_t device) {
+    RtArgs g;
+    if (n_matches < 0 || n_matches > NAV_TV_MAX_MATCHES || n_kp1 < 0 || n_kp2 < 0 || min_triangulated < 0)
+        return NAV_BA_BAD_ARG;
+    if (!rt_args(K, R, t, n_cand, th2, min_parallax_cos, g)) return NAV_BA_BAD_ARG;
+    if (!n_good || !parallax || !cos_sel) return NAV_BA_BAD_ARG;
+    if (n_kp1 > 0 && (!p3d || !good)) return NAV_BA_BAD_ARG;
+    if (n_matches > 0 && (!kp1 || !kp2 || !matches || !inliers)) return NAV_BA_BAD_ARG;
+    if (!tv_matches_ok(matches, n_matches, n_kp1, n_kp2)) return NAV_BA_BAD_ARG;
+    if ((size_t)n_kp1 * n_cand > (size_t)INT32_MAX) return NAV_BA_BAD_ARG;
+    if (!tv_gfx950(device)) return NAV_BA_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t nm = (size_t)n_matches, nk = (size_t)n_kp1, nc = (size_t)n_cand;
+    const size_t b_k1 = al(8 * nk), b_k2 = al(8 * (size_t)n_kp2), b_mt = al(8 * nm), b_in = al(nm),
+                 b_p3 = al(12 * nk * nc), b_gd = al(nk * nc), b_cp = al(4 * nm * nc), b_ps = al(nm * nc),
+                 b_o = al(12 * nc);
+    char* d = nullptr;
+    if (hipMalloc(&d, b_k1 + b_k2 + b_mt + b_in + b_p3 + b_gd + b_cp + b_ps + b_o) != hipSuccess)
+        return NAV_BA_HIP_ERROR;
+    char* p = d;
+    float* d_k1 = (float*)p;
+    p += b_k1;
+    float* d_k2 = (float*)p;
+    p += b_k2;
+    int32_t* d_mt = (int32_t*)p;
+    p += b_mt;
+    uint8_t* d_in = (uint8_t*)p;
+    p += b_in;
+    float* d_p3 = (float*)p;
+    p += b_p3;
+    uint8_t* d_gd = (uint8_t*)p;
+    p += b_gd;
+    float* d_cp = (float*)p;
+    p += b_cp;
+    uint8_t* d_ps = (uint8_t*)p;
+    p += b_ps;
+    int32_t* d_ng = (int32_t*)p;
+    float* d_par = (float*)(p + 4 * nc);
+    float* d_cs = (float*)(p + 8 * nc);
+    int32_t st = NAV_BA_OK;
+    auto up = [&](void* dst, const void* src, size_t b) {
+        if (st == NAV_BA_OK && b > 0 && hipMemcpy(dst, src, b, hipMemcpyHostToDevice) != hipSuccess)
+            st = NAV_BA_HIP_ERROR;
+    };
+    auto down = [&](void* dst, const void* src, size_t b) {
+        if (st == NAV_BA_OK && b > 0 && hipMemcpy(dst, src, b, hipMemcpyDeviceToHost) != hipSuccess)
+            st = NAV_BA_HIP_ERROR;
+    };
+    up(d_k1, kp1, 8 * nk);
+    up(d_k2, kp2, 8 * (size_t)n_kp2);
+    up(d_mt, matches, 8 * nm);
+    up(d_in, inliers, nm);
+    if (st == NAV_BA_OK)
+        st = nav_tv_check_rt(d_k1, n_kp1, d_k2, d_mt, d_in, n_matches, K, R, t, n_cand, th2, min_parallax_cos,
+                             min_triangulated, d_ng, d_par, d_cs, d_p3, d_gd, d_cp, d_ps, nullptr);
+    down(n_good, d_ng, 4 * nc);
+    down(parallax, d_par, 4 * nc);
+    down(cos_sel, d_cs, 4 * nc);
+    down(p3d, d_p3, 12 * nk * nc);
+    down(good, d_gd, nk * nc);
+    (void)hipFree(d);
+    return st;
 }

@@ -7,7 +7,8 @@ mThChiSqScore and mThChiSqF of Params2VR) and exposing FindHomography /
 FindFundamental, which score all RANSAC hypotheses at once (CheckHomography /
 CheckFundamental of each) and keep the best by the reference's rule, with
 the reference's outputs (inlier flags, score, kept matrix); FindHomographyScores /
-FindFundamentalScores also return every score.  The
+FindFundamentalScores also return every score.  CheckRT (:1244-1352) runs
+for all motion candidates of ReconstructF / ReconstructH at once.  The
 hypotheses come from the caller (the 8-point solves stay on the host).  The
 arithmetic runs in the C ABI (include/nav_twoview.h); without the built library
 or a gfx950 device the calls raise.
@@ -15,6 +16,7 @@ or a gfx950 device the calls raise.
     tv = TwoViewReconstructionHip(sigma=1.0)
     scores, best, best_score, inliers = tv.FindHomographyScores(kp1, kp2, matches, H21s, H12s)
     scores, best, best_score, inliers = tv.FindFundamentalScores(kp1, kp2, matches, F21s)
+    n_good, p3d, good, parallax = tv.CheckRT(kp1, kp2, matches, inliers, K, Rs, ts)   # 4 or 8 candidates
     tv.score_device("F", kp1_t, kp2_t, matches_t, F21s_t, out=(scores_t, best_t, best_score_t, inliers_t))
 """
 from __future__ import annotations
@@ -31,6 +33,9 @@ from .abi import NAV_BA_OK, STATUS_NAMES
 DEF_TH_CHISQ_SCORE = 5.991
 DEF_TH_CHISQ_F = 3.841
 DEF_2VR_SIGMA = 1.0
+DEF_MIN_PARALLAX_CRT = 0.99998
+DEF_MIN_TRIANGULATED = 50
+MAX_CANDIDATES = 8  # NAV_TV_MAX_CANDIDATES
 
 _lib = None
 
@@ -51,6 +56,11 @@ def lib() -> C.CDLL:
         L.nav_tv_score_fundamental_host.argtypes = [fp, i32, fp, i32, ip, i32, fp, i32, f32, f32, f32, fp, ip, fp,
                                                     u8, i32]
         L.nav_tv_score_fundamental_host.restype = i32
+        L.nav_tv_check_rt.argtypes = [v, i32, v, v, v, i32, fp, fp, fp, i32, f32, f32, i32] + [v] * 8
+        L.nav_tv_check_rt.restype = i32
+        L.nav_tv_check_rt_host.argtypes = [fp, i32, fp, i32, ip, u8, i32, fp, fp, fp, i32, f32, f32, i32, ip, fp, fp,
+                                           fp, u8, i32]
+        L.nav_tv_check_rt_host.restype = i32
         _lib = L
     return _lib
 
@@ -75,13 +85,94 @@ class TwoViewReconstructionHip:
     mvMatches12; matrices 3x3 row-major float."""
 
     def __init__(self, sigma: float = DEF_2VR_SIGMA, th_chisq_score: float = DEF_TH_CHISQ_SCORE,
-                 th_chisq_f: float = DEF_TH_CHISQ_F, device: int = 0):
+                 th_chisq_f: float = DEF_TH_CHISQ_F, min_parallax_crt: float = DEF_MIN_PARALLAX_CRT,
+                 min_triangulated: int = DEF_MIN_TRIANGULATED, device: int = 0):
         if not sigma > 0:
             raise ValueError("sigma must be > 0")
         self.sigma = float(sigma)
         self.th_score = float(th_chisq_score)
         self.th_f = float(th_chisq_f)
+        self.min_parallax_crt = float(min_parallax_crt)
+        self.min_triangulated = int(min_triangulated)
         self.device = device
+
+    @property
+    def th2(self) -> float:
+        """CheckRT's reprojection threshold as ReconstructF / ReconstructH pass it: 4 sigma^2 (float)."""
+        s = np.float32(self.sigma)
+        return float(np.float32(4.0) * (s * s))
+
+    # ---- CheckRT for a list of motion candidates ---------------------------
+    def CheckRT(self, kp1, kp2, matches, inliers, K, Rs, ts):
+        """CheckRT (OP_2ViewReconstruction.cpp:1244-1352) of every candidate
+        (Rs (k, 3, 3), ts (k, 3), k <= 8) with th2 = 4 sigma^2.  Returns
+        (nGood int32[k], vP3D float32[k, n_kp1, 3], vbGood bool[k, n_kp1],
+        parallax float32[k])."""
+        k1, k2 = _pts(kp1), _pts(kp2)
+        m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+        inl = np.ascontiguousarray(inliers, np.uint8).reshape(-1)
+        if inl.size != m.shape[0]:
+            raise ValueError("one inlier flag per match")
+        Kf = np.ascontiguousarray(K, np.float32).reshape(9)
+        R = _mats(Rs)
+        t = np.ascontiguousarray(ts, np.float32).reshape(-1, 3)
+        nc, nk = R.shape[0], k1.shape[0]
+        if t.shape[0] != nc or not 1 <= nc <= MAX_CANDIDATES:
+            raise ValueError(f"1 to {MAX_CANDIDATES} candidates, as many R as t")
+        ng = np.zeros(nc, np.int32)
+        par = np.zeros(nc, np.float32)
+        cs = np.zeros(nc, np.float32)
+        p3d = np.zeros((nc, nk, 3), np.float32)
+        good = np.zeros((nc, nk), np.uint8)
+        fp, ip, u8 = C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)
+        st = lib().nav_tv_check_rt_host(k1.ctypes.data_as(fp), nk, k2.ctypes.data_as(fp), k2.shape[0],
+                                        m.ctypes.data_as(ip), inl.ctypes.data_as(u8), m.shape[0],
+                                        Kf.ctypes.data_as(fp), R.ctypes.data_as(fp), t.ctypes.data_as(fp), nc,
+                                        self.th2, self.min_parallax_crt, self.min_triangulated,
+                                        ng.ctypes.data_as(ip), par.ctypes.data_as(fp), cs.ctypes.data_as(fp),
+                                        p3d.ctypes.data_as(fp), good.ctypes.data_as(u8), self.device)
+        _check(st, "nav_tv_check_rt_host")
+        self.last_cos_sel = cs
+        return ng, p3d, good.astype(bool), par
+
+    def check_rt_device(self, kp1, kp2, matches, inliers, K, Rs, ts, out: Optional[dict] = None):
+        """CheckRT on device tensors (key points float32 (n, 2), matches int32
+        (m, 2), inliers uint8 (m,), cuda, contiguous); K, Rs, ts on the host.
+        Returns a dict of tensors n_good, parallax, cos_sel, p3d, good,
+        cos_parallax, passed on torch's current stream."""
+        import torch
+
+        for t_, dt, nm in ((kp1, torch.float32, "kp1"), (kp2, torch.float32, "kp2"), (matches, torch.int32, "matches"),
+                           (inliers, torch.uint8, "inliers")):
+            if t_.dtype != dt or not t_.is_cuda or not t_.is_contiguous():
+                raise ValueError(f"{nm}: contiguous {dt} cuda tensor")
+        n, nk = matches.numel() // 2, kp1.numel() // 2
+        if inliers.numel() != n:
+            raise ValueError("one inlier flag per match")
+        Kf = np.ascontiguousarray(K, np.float32).reshape(9)
+        R = _mats(Rs)
+        t = np.ascontiguousarray(ts, np.float32).reshape(-1, 3)
+        nc = R.shape[0]
+        if t.shape[0] != nc or not 1 <= nc <= MAX_CANDIDATES:
+            raise ValueError(f"1 to {MAX_CANDIDATES} candidates, as many R as t")
+        dev = kp1.device
+        if out is None:
+            out = dict(n_good=torch.empty(nc, dtype=torch.int32, device=dev),
+                       parallax=torch.empty(nc, dtype=torch.float32, device=dev),
+                       cos_sel=torch.empty(nc, dtype=torch.float32, device=dev),
+                       p3d=torch.empty((nc, nk, 3), dtype=torch.float32, device=dev),
+                       good=torch.empty((nc, nk), dtype=torch.uint8, device=dev),
+                       cos_parallax=torch.empty((nc, n), dtype=torch.float32, device=dev),
+                       passed=torch.empty((nc, n), dtype=torch.uint8, device=dev))
+        fp = C.POINTER(C.c_float)
+        p = lambda x: C.c_void_p(x.data_ptr())  # noqa: E731
+        st = lib().nav_tv_check_rt(p(kp1), nk, p(kp2), p(matches), p(inliers), n, Kf.ctypes.data_as(fp),
+                                   R.ctypes.data_as(fp), t.ctypes.data_as(fp), nc, self.th2, self.min_parallax_crt,
+                                   self.min_triangulated, p(out["n_good"]), p(out["parallax"]), p(out["cos_sel"]),
+                                   p(out["p3d"]), p(out["good"]), p(out["cos_parallax"]), p(out["passed"]),
+                                   C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+        _check(st, "nav_tv_check_rt")
+        return out
 
     # ---- all hypotheses of one RANSAC run ---------------------------------
     def FindHomographyScores(self, kp1, kp2, matches, H21s, H12s) -> Tuple[np.ndarray, int, np.float32, np.ndarray]:

@@ -72,6 +72,13 @@ def lib():
         L.oracle_tv_find.argtypes = [C.c_int32, fp, fp, _i32p, C.c_int32, fp, fp, C.c_int32, C.c_float, C.c_float,
                                      C.c_float, fp, _i32p, fp, _u8p, _u8p]
         L.oracle_tv_find.restype = None
+        L.oracle_tv_null_vector4.argtypes = [_dp, _dp]
+        L.oracle_tv_null_vector4.restype = None
+        L.oracle_tv_triangulate.argtypes = [C.c_float] * 4 + [fp, fp, fp]
+        L.oracle_tv_triangulate.restype = None
+        L.oracle_tv_check_rt.argtypes = [fp, fp, _i32p, C.c_int32, _u8p, fp, fp, fp, C.c_int32, C.c_float, C.c_float,
+                                         C.c_int32, C.c_int32, _i32p, fp, fp, fp, _u8p, fp, _u8p]
+        L.oracle_tv_check_rt.restype = None
         _lib = L
     return _lib
 
@@ -274,3 +281,49 @@ def tv_find(kp1, kp2, matches, M0, M1=None, sigma=TV_SIGMA, th=None, th_score=TV
                          scores.ctypes.data_as(fp), C.byref(best), C.byref(bs), inl.ctypes.data_as(_u8p),
                          work.ctypes.data_as(_u8p))
     return scores, best.value, np.float32(bs.value), inl
+
+
+# Params2VR defaults used by CheckRT (OP_2ViewReconstruction.hpp:35-36)
+TV_MIN_PARALLAX_CRT = 0.99998
+TV_MIN_TRIANGULATED = 50
+
+
+def tv_null_vector4(A):
+    """Right singular vector of the smallest singular value of a 4x4 (Jacobi)."""
+    A = np.ascontiguousarray(A, np.float64).reshape(16)
+    x = np.zeros(4)
+    lib().oracle_tv_null_vector4(_d(A), _d(x))
+    return x
+
+
+def tv_triangulate(u1, v1, u2, v2, P1, P2):
+    """Triangulate: the 3D point (float32) of one correspondence, P1 and P2 3x4."""
+    fp = C.POINTER(C.c_float)
+    P1 = _f32(P1, 12); P2 = _f32(P2, 12)
+    X = np.zeros(3, np.float32)
+    lib().oracle_tv_triangulate(u1, v1, u2, v2, P1.ctypes.data_as(fp), P2.ctypes.data_as(fp), X.ctypes.data_as(fp))
+    return X
+
+
+def tv_check_rt(kp1, kp2, matches, inliers, K, R, t, th2=4.0, min_cos=TV_MIN_PARALLAX_CRT,
+                min_tri=TV_MIN_TRIANGULATED):
+    """CheckRT over candidates R (k, 3, 3), t (k, 3).  Returns a dict of
+    n_good (k,), parallax (k,), cos_sel (k,), p3d (k, n_kp1, 3), good (k, n_kp1),
+    cosp (k, m), passed (k, m)."""
+    fp = C.POINTER(C.c_float)
+    kp1 = _f32(kp1, (-1, 2)); kp2 = _f32(kp2, (-1, 2))
+    m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+    inl = np.ascontiguousarray(inliers, np.uint8).reshape(-1)
+    K = _f32(K, 9); R = _f32(R, (-1, 9)); t = _f32(t, (-1, 3))
+    k, n, nk = R.shape[0], m.shape[0], kp1.shape[0]
+    out = dict(n_good=np.zeros(k, np.int32), parallax=np.zeros(k, np.float32), cos_sel=np.zeros(k, np.float32),
+               p3d=np.zeros((k, nk, 3), np.float32), good=np.zeros((k, nk), np.uint8),
+               cosp=np.zeros((k, n), np.float32), passed=np.zeros((k, n), np.uint8))
+    lib().oracle_tv_check_rt(kp1.ctypes.data_as(fp), kp2.ctypes.data_as(fp), m.ctypes.data_as(_i32p), n,
+                             inl.ctypes.data_as(_u8p), K.ctypes.data_as(fp), R.ctypes.data_as(fp),
+                             t.ctypes.data_as(fp), k, th2, min_cos, min_tri, nk,
+                             out["n_good"].ctypes.data_as(_i32p), out["parallax"].ctypes.data_as(fp),
+                             out["cos_sel"].ctypes.data_as(fp), out["p3d"].ctypes.data_as(fp),
+                             out["good"].ctypes.data_as(_u8p), out["cosp"].ctypes.data_as(fp),
+                             out["passed"].ctypes.data_as(_u8p))
+    return out

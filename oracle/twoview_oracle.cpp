@@ -27,8 +27,27 @@
 //     distances |l . x| / |l_(0,1)| with l = F x1 and l = F^T x2;
 //   - the keep-the-best rule: equal maxima keep the first, NaN scores and
 //     scores of 0 never win, no winner leaves every flag false.
+//
+// Also restates TwoViewReconstruction::CheckRT (:1244-1352) with Triangulate
+// (:1179-1194) for a list of motion candidates (R, t).  The reference takes
+// the 3D point from OpenCV's float SVD of the 4x4 system (cv::SVD::compute)
+// and forms P2 = K [R|t], O2 = -R^T t and R X + t with cv::gemm, and the norms
+// and dot products with cv::norm / Mat::dot.  OpenCV is absent here, so the
+// restatement fixes its own reading of those calls: matrix products and the
+// norm and dot sums accumulate float products in double and round to float
+// (the published behaviour of OpenCV's 32F gemm / norm / dot kernels), and the
+// null vector comes from a one-sided (Hestenes) Jacobi SVD in double, rounded
+// to float.  Parity against OpenCV's bits is unpinned; pinned instead
+// (tests/test_twoview_oracle.py) by: the null vector against LAPACK's SVD
+// (numpy) on random and rank-3 systems; exact correspondences triangulating
+// to the generating points; the cheirality, reprojection and parallax
+// decisions against a float64 evaluation of the same geometry away from the
+// thresholds; the parallax against the float64 angle at the reference's index
+// into the sorted cosines.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 extern "C" {
 
@@ -134,6 +153,162 @@ void oracle_tv_find(int32_t homography, const float* kp1, const float* kp2, cons
         }
     }
     *best_score = score;
+}
+
+// Right singular vector of the smallest singular value of the 4x4 row-major A:
+// one-sided Jacobi rotations on A's columns (pairs (0,1) (0,2) (0,3) (1,2)
+// (1,3) (2,3) per sweep, up to 30 sweeps, a pair skipped once its columns are
+// orthogonal to 1e-15 of their norms), V accumulating the rotations; the
+// column of V whose rotated A column has the smallest norm (the first of
+// equal ones).  Double, left to right, no fused multiply-adds.
+void oracle_tv_null_vector4(const double* A, double* x) {
+    double a[16], v[16];
+    for (int i = 0; i < 16; ++i) {
+        a[i] = A[i];
+        v[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        bool rotated = false;
+        for (int p = 0; p < 3; ++p)
+            for (int q = p + 1; q < 4; ++q) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int r = 0; r < 4; ++r) {
+                    al = al + a[4 * r + p] * a[4 * r + p];
+                    be = be + a[4 * r + q] * a[4 * r + q];
+                    ga = ga + a[4 * r + p] * a[4 * r + q];
+                }
+                if (ga == 0.0 || std::fabs(ga) <= 1e-15 * std::sqrt(al * be)) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+                for (int r = 0; r < 4; ++r) {
+                    const double ap = a[4 * r + p], aq = a[4 * r + q];
+                    a[4 * r + p] = c * ap - s * aq;
+                    a[4 * r + q] = s * ap + c * aq;
+                    const double vp = v[4 * r + p], vq = v[4 * r + q];
+                    v[4 * r + p] = c * vp - s * vq;
+                    v[4 * r + q] = s * vp + c * vq;
+                }
+            }
+        if (!rotated) break;
+    }
+    int k = 0;
+    double best = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        double nrm = 0.0;
+        for (int r = 0; r < 4; ++r) nrm = nrm + a[4 * r + c] * a[4 * r + c];
+        if (c == 0 || nrm < best) {
+            best = nrm;
+            k = c;
+        }
+    }
+    for (int r = 0; r < 4; ++r) x[r] = v[4 * r + k];
+}
+
+// 3x3 (or 3x4) float product with double accumulation, rounded to float.
+static float dot3d(const float* a, const float* b, int sb) {
+    double r = 0.0;
+    for (int k = 0; k < 3; ++k) r = r + (double)a[k] * (double)b[sb * k];
+    return (float)r;
+}
+
+// Triangulate (:1179-1194): A rows u1 P1_2 - P1_0, v1 P1_2 - P1_1, u2 P2_2 -
+// P2_0, v2 P2_2 - P2_1 in float; x = null vector (float); X = x_0..2 / x_3.
+void oracle_tv_triangulate(float u1, float v1, float u2, float v2, const float* P1, const float* P2, float* X) {
+    double A[16];
+    for (int c = 0; c < 4; ++c) {
+        A[c] = (double)(u1 * P1[8 + c] - P1[c]);
+        A[4 + c] = (double)(v1 * P1[8 + c] - P1[4 + c]);
+        A[8 + c] = (double)(u2 * P2[8 + c] - P2[c]);
+        A[12 + c] = (double)(v2 * P2[8 + c] - P2[4 + c]);
+    }
+    double x[4];
+    oracle_tv_null_vector4(A, x);
+    const float w = (float)x[3];
+    for (int r = 0; r < 3; ++r) X[r] = (float)x[r] / w;
+}
+
+// CheckRT for n_cand candidates (R: 9 floats, t: 3 floats each, row-major).
+// K: 3x3 row-major float.  inliers: per match.  Outputs per candidate c:
+// n_good[c], parallax[c] (degrees; 0 if none), cos_sel[c] (the cosine at the
+// reference's index, NaN if none), p3d[c][n_kp1][3] (0 where not set),
+// good[c][n_kp1] (vbGood), and per match cosp[c][n] / passed[c][n].
+void oracle_tv_check_rt(const float* kp1, const float* kp2, const int32_t* matches, int32_t n,
+                        const uint8_t* inliers, const float* K, const float* R, const float* t, int32_t n_cand,
+                        float th2, float min_cos, int32_t min_tri, int32_t n_kp1, int32_t* n_good, float* parallax,
+                        float* cos_sel, float* p3d, uint8_t* good, float* cosp, uint8_t* passed) {
+    const float fx = K[0], fy = K[4], cx = K[2], cy = K[5];
+    for (int32_t c = 0; c < n_cand; ++c) {
+        const float* Rc = R + 9 * c;
+        const float* tc = t + 3 * c;
+        float P1[12] = {0}, P2[12], Rt[12];
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) P1[4 * r + q] = K[3 * r + q];
+        for (int r = 0; r < 3; ++r) {
+            for (int q = 0; q < 3; ++q) Rt[4 * r + q] = Rc[3 * r + q];
+            Rt[4 * r + 3] = tc[r];
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 4; ++q) P2[4 * r + q] = dot3d(K + 3 * r, Rt + q, 4);  // P2 = K [R|t]
+        float O2[3];  // -(R^T t): row r of R^T is column r of R
+        for (int r = 0; r < 3; ++r) {
+            const float col[3] = {Rc[r], Rc[3 + r], Rc[6 + r]};
+            O2[r] = -dot3d(col, tc, 1);
+        }
+        float* P3 = p3d + (size_t)3 * n_kp1 * c;
+        uint8_t* G = good + (size_t)n_kp1 * c;
+        for (int32_t i = 0; i < 3 * n_kp1; ++i) P3[i] = 0.0f;
+        for (int32_t i = 0; i < n_kp1; ++i) G[i] = 0;
+        std::vector<float> vcos;
+        int32_t ng = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            cosp[(size_t)n * c + i] = 0.0f;
+            passed[(size_t)n * c + i] = 0;
+            if (!inliers[i]) continue;
+            const int32_t i1 = matches[2 * i], i2 = matches[2 * i + 1];
+            const float u1 = kp1[2 * i1], v1 = kp1[2 * i1 + 1], u2 = kp2[2 * i2], v2 = kp2[2 * i2 + 1];
+            float X[3];
+            oracle_tv_triangulate(u1, v1, u2, v2, P1, P2, X);
+            if (!std::isfinite(X[0]) || !std::isfinite(X[1]) || !std::isfinite(X[2])) continue;
+            const float n2[3] = {X[0] - O2[0], X[1] - O2[1], X[2] - O2[2]};
+            const float d1 = (float)std::sqrt(((0.0 + (double)X[0] * X[0]) + (double)X[1] * X[1]) + (double)X[2] * X[2]);
+            const float d2 =
+                (float)std::sqrt(((0.0 + (double)n2[0] * n2[0]) + (double)n2[1] * n2[1]) + (double)n2[2] * n2[2]);
+            const double dt = ((0.0 + (double)X[0] * n2[0]) + (double)X[1] * n2[1]) + (double)X[2] * n2[2];
+            const float cosP = (float)(dt / (double)(d1 * d2));
+            if (X[2] <= 0 && cosP < min_cos) continue;
+            float X2[3];
+            for (int r = 0; r < 3; ++r) X2[r] = dot3d(Rc + 3 * r, X, 1) + tc[r];
+            if (X2[2] <= 0 && cosP < min_cos) continue;
+            const float iz1 = 1.0f / X[2];
+            const float im1x = fx * X[0] * iz1 + cx, im1y = fy * X[1] * iz1 + cy;
+            const float e1 = (im1x - u1) * (im1x - u1) + (im1y - v1) * (im1y - v1);
+            if (e1 > th2) continue;
+            const float iz2 = 1.0f / X2[2];
+            const float im2x = fx * X2[0] * iz2 + cx, im2y = fy * X2[1] * iz2 + cy;
+            const float e2 = (im2x - u2) * (im2x - u2) + (im2y - v2) * (im2y - v2);
+            if (e2 > th2) continue;
+            vcos.push_back(cosP);
+            P3[3 * i1] = X[0];
+            P3[3 * i1 + 1] = X[1];
+            P3[3 * i1 + 2] = X[2];
+            ++ng;
+            if (cosP < min_cos) G[i1] = 1;
+            cosp[(size_t)n * c + i] = cosP;
+            passed[(size_t)n * c + i] = 1;
+        }
+        n_good[c] = ng;
+        if (ng > 0) {
+            std::sort(vcos.begin(), vcos.end());
+            const size_t idx = (size_t)std::min(min_tri, (int32_t)vcos.size() - 1);
+            cos_sel[c] = vcos[idx];
+            parallax[c] = static_cast<float>(std::acos(vcos[idx]) * 180 / 3.14159265358979323846);
+        } else {
+            cos_sel[c] = NAN;
+            parallax[c] = 0.0f;
+        }
+    }
 }
 
 }  // extern "C"

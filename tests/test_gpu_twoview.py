@@ -144,3 +144,99 @@ def test_argument_errors():
                                       None) == NAV_BA_BAD_ARG
     with pytest.raises(ValueError):
         TwoViewReconstructionHip(sigma=0.0)
+
+
+# ---- CheckRT for motion candidates
+
+def _candidates(sc, extra=0, seed=0):
+    """The four motions of DecomposeE (R and its twist about t, with +t and -t),
+    plus `extra` perturbed ones (ReconstructH passes 8)."""
+    th = sc.t / np.linalg.norm(sc.t)
+    Rtw = (2 * np.outer(th, th) - np.eye(3)) @ sc.R
+    R = [sc.R, Rtw, sc.R, Rtw]
+    t = [sc.t, sc.t, -sc.t, -sc.t]
+    rng = np.random.default_rng(seed)
+    for _ in range(extra):
+        w = rng.normal(0, 0.02, 3)
+        a = np.linalg.norm(w)
+        Wx = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+        R.append((np.eye(3) + np.sin(a) / a * Wx + (1 - np.cos(a)) / a ** 2 * Wx @ Wx) @ sc.R)
+        t.append(sc.t + rng.normal(0, 0.02, 3))
+    return np.stack(R).astype(np.float32), np.stack(t).astype(np.float32)
+
+
+def _ulps(a, b):
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+@pytest.mark.parametrize("n,planar,extra", [(1000, False, 0), (1000, True, 4), (3000, False, 4), (60, False, 0)])
+def test_check_rt_host_entry_equals_oracle(n, planar, extra):
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say(f"CheckRT host entry: {n} matches, planar {planar}, {4 + extra} candidates")
+    sc = synthetic_two_view(n, seed=200 + n, planar=planar, outlier_frac=0.2, noise_px=1.0)
+    R, t = _candidates(sc, extra, seed=n)
+    inl = (np.random.default_rng(n).random(n) < 0.85).astype(np.uint8)
+    tv = TwoViewReconstructionHip()
+    ng, p3d, good, par = tv.CheckRT(sc.kp1, sc.kp2, sc.matches, inl, sc.K, R, t)
+    ref = oracle.tv_check_rt(sc.kp1, sc.kp2, sc.matches, inl, sc.K, R, t, th2=4.0)
+    assert np.array_equal(ng, ref["n_good"])
+    assert np.array_equal(p3d.view(np.uint32), ref["p3d"].view(np.uint32))
+    assert np.array_equal(good, ref["good"].astype(bool))
+    assert _same_scores(tv.last_cos_sel, ref["cos_sel"])
+    assert _ulps(par, ref["parallax"]).max() <= 2  # acosf of the device library against the host's
+    assert ng[0] > 0.5 * inl.sum() and ng[0] > 2 * ng[1:4].max()
+
+
+def test_check_rt_device_outputs_and_edges():
+    import torch
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say("CheckRT on device tensors: per-match cosines and flags, no inliers, index beyond nGood")
+    sc = synthetic_two_view(1500, seed=31, planar=False)
+    R, t = _candidates(sc, 4, seed=31)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    inl = np.ones(1500, np.uint8)
+    k1, k2, mt, di = d(sc.kp1), d(sc.kp2), d(sc.matches), d(inl)
+    tv = TwoViewReconstructionHip()
+    out = tv.check_rt_device(k1, k2, mt, di, sc.K, R, t)
+    torch.cuda.synchronize()
+    ref = oracle.tv_check_rt(sc.kp1, sc.kp2, sc.matches, inl, sc.K, R, t, th2=4.0)
+    assert np.array_equal(out["passed"].cpu().numpy(), ref["passed"])
+    got_c = out["cos_parallax"].cpu().numpy()
+    assert np.array_equal(got_c.view(np.uint32), ref["cosp"].view(np.uint32))
+    assert np.array_equal(out["n_good"].cpu().numpy(), ref["n_good"])
+    assert _same_scores(out["cos_sel"].cpu().numpy(), ref["cos_sel"])
+    # no inliers: nothing passes, parallax 0, cosine NaN; the outputs are rewritten in place
+    tv.check_rt_device(k1, k2, mt, d(np.zeros(1500, np.uint8)), sc.K, R, t, out=out)
+    torch.cuda.synchronize()
+    assert not out["n_good"].any().item() and not out["parallax"].any().item()
+    assert torch.isnan(out["cos_sel"]).all().item() and not out["good"].any().item()
+    assert not out["p3d"].any().item()
+    # an index beyond nGood takes the largest cosine
+    tv2 = TwoViewReconstructionHip(min_triangulated=100000)
+    ng, _, _, par = tv2.CheckRT(sc.kp1, sc.kp2, sc.matches, inl, sc.K, R[:1], t[:1])
+    ref = oracle.tv_check_rt(sc.kp1, sc.kp2, sc.matches, inl, sc.K, R[:1], t[:1], th2=4.0, min_tri=100000)
+    assert ng[0] == ref["n_good"][0] and tv2.last_cos_sel[0] == ref["cosp"][0][ref["passed"][0] == 1].max()
+
+
+def test_check_rt_argument_errors():
+    from nav24_amd.abi import NAV_BA_BAD_ARG
+    from nav24_amd.twoview import TwoViewReconstructionHip, lib
+    _say("CheckRT: argument checks")
+    sc = synthetic_two_view(20, seed=3)
+    R, t = _candidates(sc)
+    tv = TwoViewReconstructionHip()
+    with pytest.raises(ValueError):
+        tv.CheckRT(sc.kp1, sc.kp2, sc.matches, np.ones(20), sc.K, np.concatenate([R, R, R]), np.concatenate([t, t, t]))
+    with pytest.raises(ValueError):
+        tv.CheckRT(sc.kp1, sc.kp2, sc.matches, np.ones(19), sc.K, R, t)
+    bad = sc.matches.copy()
+    bad[3, 1] = len(sc.kp2)
+    with pytest.raises(RuntimeError, match="BAD_ARG"):
+        tv.CheckRT(sc.kp1, sc.kp2, bad, np.ones(20), sc.K, R, t)
+    f = ctypes.c_float
+    K = (f * 9)(*sc.K.astype(np.float32).reshape(-1))
+    z = (f * 9)()
+    o = (ctypes.c_int32 * 1)()
+    assert lib().nav_tv_check_rt(None, 0, None, None, None, 0, K, z, z, 0, 4.0, 0.99998, 50, o, o, o, None, None,
+                                 None, None, None) == NAV_BA_BAD_ARG

@@ -181,3 +181,113 @@ def test_find_equals_checks_in_turn():
     scores, best, bs, inl = o.tv_find(sc.kp1, sc.kp2, sc.matches, F21)
     assert best == int(np.argmax(scores)) and bs == scores.max()
     assert (inl == o.tv_check_fundamental(sc.kp1, sc.kp2, sc.matches, F21[best])[1]).all()
+
+
+# ---- CheckRT (OP_2ViewReconstruction.cpp:1244-1352) and Triangulate (:1179-1194)
+
+def test_null_vector_matches_lapack():
+    rng = np.random.default_rng(3)
+    for k in range(40):
+        A = rng.normal(size=(4, 4)) * 10.0 ** rng.uniform(-2, 3)
+        if k % 2:  # rank 3: an exact null vector
+            A[3] = A[:3].T @ rng.normal(size=3)
+        x = o.tv_null_vector4(A)
+        U, S, Vt = np.linalg.svd(A)
+        assert abs(np.linalg.norm(x) - 1) < 1e-12
+        assert abs(abs(x @ Vt[-1]) - 1) < 1e-9
+        assert np.linalg.norm(A @ x) <= S[-1] * (1 + 1e-9) + 1e-12 * S[0]
+
+
+def _candidates(sc):
+    """The four motions DecomposeE yields: R and the 180-degree twist about t, with +t and -t."""
+    th = sc.t / np.linalg.norm(sc.t)
+    Rtw = (2 * np.outer(th, th) - np.eye(3)) @ sc.R
+    R = np.stack([sc.R, Rtw, sc.R, Rtw]).astype(np.float32)
+    t = np.stack([sc.t, sc.t, -sc.t, -sc.t]).astype(np.float32)
+    return R, t
+
+
+def _f64_check(sc, R, t, th2, min_cos):
+    """The checks of CheckRT in float64 with LAPACK's SVD: (passed, cos, margin_ok)."""
+    K = sc.K.astype(np.float32).astype(np.float64)
+    P1 = np.c_[K, np.zeros(3)]
+    P2 = K @ np.c_[R.astype(np.float64), t.astype(np.float64)]
+    O2 = -R.astype(np.float64).T @ t.astype(np.float64)
+    passed, cosv, clear = [], [], []
+    for i1, i2 in sc.matches:
+        u1, v1 = sc.kp1[i1].astype(np.float64)
+        u2, v2 = sc.kp2[i2].astype(np.float64)
+        A = np.stack([u1 * P1[2] - P1[0], v1 * P1[2] - P1[1], u2 * P2[2] - P2[0], v2 * P2[2] - P2[1]])
+        x = np.linalg.svd(A)[2][-1]
+        X = x[:3] / x[3]
+        n2 = X - O2
+        c = X @ n2 / (np.linalg.norm(X) * np.linalg.norm(n2))
+        X2 = R.astype(np.float64) @ X + t.astype(np.float64)
+        e1 = ((K[0, 0] * X[0] / X[2] + K[0, 2] - u1) ** 2 + (K[1, 1] * X[1] / X[2] + K[1, 2] - v1) ** 2)
+        e2 = ((K[0, 0] * X2[0] / X2[2] + K[0, 2] - u2) ** 2 + (K[1, 1] * X2[1] / X2[2] + K[1, 2] - v2) ** 2)
+        ok = not ((X[2] <= 0 and c < min_cos) or (X2[2] <= 0 and c < min_cos)) and e1 <= th2 and e2 <= th2
+        margin = (abs(e1 - th2) > 1e-3 * th2 and abs(e2 - th2) > 1e-3 * th2 and abs(c - min_cos) > 1e-6
+                  and abs(X[2]) > 1e-3 and abs(X2[2]) > 1e-3)
+        passed.append(ok)
+        cosv.append(c)
+        clear.append(margin)
+    return np.array(passed), np.array(cosv), np.array(clear)
+
+
+def test_exact_correspondences_triangulate_to_the_scene():
+    sc = synthetic_two_view(300, seed=12, planar=False, outlier_frac=0.0, noise_px=0.0)
+    R, t = _candidates(sc)
+    r = o.tv_check_rt(sc.kp1, sc.kp2, sc.matches, np.ones(300, np.uint8), sc.K, R, t)
+    assert r["n_good"][0] == 300 and r["n_good"][1:].max() < 30
+    # rebuild the generating points: depth along the first camera's ray
+    p1 = sc.kp1[sc.matches[:, 0]].astype(np.float64)
+    X = r["p3d"][0][sc.matches[:, 0]].astype(np.float64)
+    proj = (sc.K @ X.T).T
+    assert np.abs(proj[:, :2] / proj[:, 2:] - p1).max() < 2e-2
+    X2 = (sc.R @ X.T).T + sc.t
+    proj2 = (sc.K @ X2.T).T
+    assert np.abs(proj2[:, :2] / proj2[:, 2:] - sc.kp2[sc.matches[:, 1]]).max() < 2e-2
+
+
+@pytest.mark.parametrize("seed,planar", [(13, False), (14, True)])
+def test_check_rt_decisions_match_float64_geometry(seed, planar):
+    sc = synthetic_two_view(400, seed=seed, planar=planar, outlier_frac=0.25, noise_px=1.0)
+    R, t = _candidates(sc)
+    inl = (np.random.default_rng(seed).random(400) < 0.9).astype(np.uint8)
+    r = o.tv_check_rt(sc.kp1, sc.kp2, sc.matches, inl, sc.K, R, t, th2=4.0)
+    for c in range(4):
+        passed, cosv, clear = _f64_check(sc, R[c], t[c], 4.0, np.float32(0.99998))
+        passed &= inl.astype(bool)
+        got = r["passed"][c].astype(bool)
+        assert clear.mean() > 0.95
+        assert (got[clear] == passed[clear]).all(), c
+        assert r["n_good"][c] == got.sum()
+        # p3d and vbGood land on the image-1 key points of the passing matches only
+        i1 = sc.matches[:, 0]
+        assert (np.abs(r["p3d"][c][i1[got]]).sum(1) > 0).all()
+        mask = np.ones(len(sc.kp1), bool); mask[i1[got]] = False
+        assert not r["p3d"][c][mask].any() and not r["good"][c][mask].any()
+        assert (r["good"][c][i1[got]] == (r["cosp"][c][got] < np.float32(0.99998))).all()
+        if got.any():
+            assert np.abs(r["cosp"][c][got] - cosv[got]).max() < 1e-5
+            cs = np.sort(r["cosp"][c][got])
+            idx = min(50, got.sum() - 1)
+            assert r["cos_sel"][c] == cs[idx]
+            assert abs(r["parallax"][c] - np.degrees(np.arccos(np.sort(cosv[got])[idx]))) < 2e-2
+    assert r["n_good"][0] > 2 * r["n_good"][1:].max()
+
+
+def test_check_rt_edge_cases():
+    sc = synthetic_two_view(80, seed=15, planar=False, outlier_frac=0.0, noise_px=0.3)
+    R, t = _candidates(sc)
+    r = o.tv_check_rt(sc.kp1, sc.kp2, sc.matches, np.zeros(80, np.uint8), sc.K, R, t)
+    assert not r["n_good"].any() and not r["parallax"].any() and np.isnan(r["cos_sel"]).all()
+    # fewer passing points than the index: the largest cosine (smallest parallax) is taken
+    r = o.tv_check_rt(sc.kp1, sc.kp2, sc.matches, np.ones(80, np.uint8), sc.K, R[:1], t[:1], min_tri=500)
+    got = r["passed"][0].astype(bool)
+    assert r["cos_sel"][0] == r["cosp"][0][got].max()
+    r0 = o.tv_check_rt(sc.kp1, sc.kp2, sc.matches, np.ones(80, np.uint8), sc.K, R[:1], t[:1], min_tri=0)
+    assert r0["cos_sel"][0] == r0["cosp"][0][got].min()
+    # a parallax threshold above every cosine makes every passing point good
+    r1 = o.tv_check_rt(sc.kp1, sc.kp2, sc.matches, np.ones(80, np.uint8), sc.K, R[:1], t[:1], min_cos=1.5)
+    assert r1["good"][0].sum() == r1["n_good"][0]
