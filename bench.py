@@ -440,6 +440,40 @@ def twoview_leg(n_matches, n_hyp, steps, with_cpu):
             res[kind]["equal_to_oracle"] = bool(
                 np.array_equal(out[0].cpu().numpy().view(np.uint32), ref[0].view(np.uint32))
                 and out[1].item() == ref[1] and np.array_equal(out[3].cpu().numpy(), ref[3]))
+    # CheckRT for the motion candidates (4 as ReconstructF, 8 as ReconstructH)
+    # on the fundamental matrix's inliers, each call between its own events
+    from nav24_amd.twoview_synth import motion_candidates
+    inl = outF[3]
+    for n_c in (4, 8):
+        R, t = motion_candidates(sc, n_c - 4, seed=1)
+        rt = tv.check_rt_device(k1, k2, mt, inl, sc.K, R, t)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record()
+            tv.check_rt_device(k1, k2, mt, inl, sc.K, R, t, out=rt)
+            b.record()
+        torch.cuda.synchronize()
+        us = sum(a.elapsed_time(b) for a, b in ev) * 1e3 / steps
+        n_inl = int(inl.sum().item())
+        entry = {"us_per_call": round(us, 2), "inlier_matches": n_inl,
+                 "triangulations_per_s": round(n_inl * n_c / (us * 1e-6), 1),
+                 "n_good": rt["n_good"].cpu().tolist()}
+        if with_cpu:
+            from oracle import oracle
+            inl_h = inl.cpu().numpy()
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ref = oracle.tv_check_rt(sc.kp1, sc.kp2, sc.matches, inl_h, sc.K, R, t, th2=tv.th2)
+            dt = (time.perf_counter() - t0) / reps
+            entry["cpu_baseline"] = {"value": round(n_inl * n_c / dt, 1), "unit": "triangulations/s",
+                                     "us_per_call": round(dt * 1e6, 1), "cores": 1, "kind": "port",
+                                     "sample": f"the same input, oracle, 1 thread, {reps} calls"}
+            entry["equal_to_oracle"] = bool(
+                np.array_equal(rt["n_good"].cpu().numpy(), ref["n_good"])
+                and np.array_equal(rt["p3d"].cpu().numpy().view(np.uint32), ref["p3d"].view(np.uint32))
+                and np.array_equal(rt["good"].cpu().numpy(), ref["good"]))
+        res[f"check_rt_{n_c}_candidates"] = entry
     return res
 
 

@@ -478,11 +478,29 @@ __global__ __launch_bounds__(kTvSelThreads) void k_tv_rt_select(const float* __r
             if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int r = rank, b = 0;
-            for (; b < 255 && r >= hist[b]; ++b) r -= hist[b];
-            sel[0] = b;
-            sel[1] = r;
+        // the bucket holding the rank: inclusive prefix of the 256 counts over
+        // the first 4 waves (shuffles within a wave, wave totals through LDS);
+        // exactly one bucket has excl <= rank < incl
+        int h = 0, incl = 0;
+        if (threadIdx.x < 256) {
+            h = hist[threadIdx.x];
+            incl = h;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(incl, off);
+                if ((threadIdx.x & 63) >= off) incl += y;
+            }
+            if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+        }
+        __syncthreads();
+        if (threadIdx.x < 256) {
+#pragma unroll
+            for (int w = 0; w < 4; ++w) incl += w < (threadIdx.x >> 6) ? wsum[w] : 0;
+            const int excl = incl - h;
+            if (excl <= rank && rank < incl) {
+                sel[0] = threadIdx.x;
+                sel[1] = rank - excl;
+            }
         }
         __syncthreads();
         prefix |= (uint32_t)sel[0] << shift;

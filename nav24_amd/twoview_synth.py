@@ -113,3 +113,22 @@ def ransac_hypotheses(scene: TwoViewScene, n_hyp: int, seed: int = 0):
         Fn = _f_8pt(n1[idx], n2[idx])
         F21s.append((T2.T @ Fn @ T1).astype(np.float32))
     return np.stack(H21s), np.stack(H12s), np.stack(F21s)
+
+
+def motion_candidates(scene: TwoViewScene, n_extra: int = 0, seed: int = 0):
+    """(R (k, 3, 3), t (k, 3)) float32: the four motions DecomposeE yields for
+    the scene's essential matrix (R and its 180-degree twist about t, with +t
+    and -t; OP_2ViewReconstruction.cpp:1356-1376), then n_extra perturbed
+    motions (ReconstructH checks 8 candidates)."""
+    th = scene.t / np.linalg.norm(scene.t)
+    Rtw = (2 * np.outer(th, th) - np.eye(3)) @ scene.R
+    R = [scene.R, Rtw, scene.R, Rtw]
+    t = [scene.t, scene.t, -scene.t, -scene.t]
+    rng = np.random.default_rng(seed)
+    for _ in range(n_extra):
+        w = rng.normal(0, 0.02, 3)
+        a = np.linalg.norm(w)
+        Wx = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+        R.append((np.eye(3) + np.sin(a) / a * Wx + (1 - np.cos(a)) / a ** 2 * Wx @ Wx) @ scene.R)
+        t.append(scene.t + rng.normal(0, 0.02, 3))
+    return np.stack(R).astype(np.float32), np.stack(t).astype(np.float32)

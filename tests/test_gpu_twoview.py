@@ -9,7 +9,7 @@ import pytest
 
 from oracle import oracle
 from parity_util import say as _say
-from nav24_amd.twoview_synth import ransac_hypotheses, synthetic_two_view
+from nav24_amd.twoview_synth import motion_candidates, ransac_hypotheses, synthetic_two_view
 
 pytestmark = pytest.mark.gpu
 
@@ -149,20 +149,7 @@ def test_argument_errors():
 # ---- CheckRT for motion candidates
 
 def _candidates(sc, extra=0, seed=0):
-    """The four motions of DecomposeE (R and its twist about t, with +t and -t),
-    plus `extra` perturbed ones (ReconstructH passes 8)."""
-    th = sc.t / np.linalg.norm(sc.t)
-    Rtw = (2 * np.outer(th, th) - np.eye(3)) @ sc.R
-    R = [sc.R, Rtw, sc.R, Rtw]
-    t = [sc.t, sc.t, -sc.t, -sc.t]
-    rng = np.random.default_rng(seed)
-    for _ in range(extra):
-        w = rng.normal(0, 0.02, 3)
-        a = np.linalg.norm(w)
-        Wx = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
-        R.append((np.eye(3) + np.sin(a) / a * Wx + (1 - np.cos(a)) / a ** 2 * Wx @ Wx) @ sc.R)
-        t.append(sc.t + rng.normal(0, 0.02, 3))
-    return np.stack(R).astype(np.float32), np.stack(t).astype(np.float32)
+    return motion_candidates(sc, extra, seed)
 
 
 def _ulps(a, b):

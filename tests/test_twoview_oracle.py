@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as o
-from nav24_amd.twoview_synth import synthetic_two_view, ransac_hypotheses
+from nav24_amd.twoview_synth import motion_candidates, ransac_hypotheses, synthetic_two_view
 
 TH = np.float32(5.991)
 THF = np.float32(3.841)
@@ -200,11 +200,7 @@ def test_null_vector_matches_lapack():
 
 def _candidates(sc):
     """The four motions DecomposeE yields: R and the 180-degree twist about t, with +t and -t."""
-    th = sc.t / np.linalg.norm(sc.t)
-    Rtw = (2 * np.outer(th, th) - np.eye(3)) @ sc.R
-    R = np.stack([sc.R, Rtw, sc.R, Rtw]).astype(np.float32)
-    t = np.stack([sc.t, sc.t, -sc.t, -sc.t]).astype(np.float32)
-    return R, t
+    return motion_candidates(sc)
 
 
 def _f64_check(sc, R, t, th2, min_cos):
