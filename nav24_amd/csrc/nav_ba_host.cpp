@@ -122,6 +122,10 @@ struct nav_ba_handle {
     int schur_run = kRunLen;  // Schur pairs per register run
     StageTimer timer;
     int* hdone = nullptr;      // pinned: windows finished
+    char* stage = nullptr;     // pinned staging of a prepare's uploads (grown on demand)
+    size_t stage_cap = 0;
+    char* stage_s = nullptr;   // pinned staging of the Schur work items (the tuner re-uploads them)
+    size_t stage_s_cap = 0;
     // results of the last run
     bool ran = false;
     std::vector<LMDev> lm_out;
@@ -165,15 +169,47 @@ int alloc(nav_ba_handle* h, void** out, size_t bytes) {
     return NAV_BA_OK;
 }
 
-template <class T>
-int upload(nav_ba_handle* h, const T** out, const std::vector<T>& v) {
-    void* p = nullptr;
-    int st = alloc(h, &p, sizeof(T) * v.size());
-    if (st) return st;
-    if (!v.empty()) HIPTRY(h, hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, h->stream));
-    *out = (const T*)p;
-    return NAV_BA_OK;
-}
+// Host arrays packed into a pinned staging buffer of the handle and sent with
+// one copy into one device slot (each array 256-byte aligned, at least 256
+// bytes, so an empty one still gets a distinct pointer): one pinned copy in
+// place of one pageable copy per array.  A staging buffer is reused only after
+// the stream has been synchronised since its last copy.
+struct Packer {
+    char** buf;
+    size_t* cap;
+    Packer(char** b, size_t* c) : buf(b), cap(c) {}
+    struct Item {
+        const void* src;
+        size_t bytes, off;
+        const void** dst;
+    };
+    std::vector<Item> items;
+    size_t total = 0;
+    template <class T>
+    void add(const T** out, const std::vector<T>& v) {
+        const size_t b = sizeof(T) * v.size();
+        items.push_back({v.data(), b, total, (const void**)out});
+        total += (std::max<size_t>(b, 1) + 255) & ~size_t(255);
+    }
+    int flush(nav_ba_handle* h) {
+        void* d = nullptr;
+        int st = alloc(h, &d, total);
+        if (st) return st;
+        if (*cap < total) {
+            if (*buf) (void)hipHostFree(*buf);
+            *buf = nullptr;
+            *cap = 0;
+            HIPTRY(h, hipHostMalloc((void**)buf, total, hipHostMallocDefault));
+            *cap = total;
+        }
+        for (const Item& it : items) {
+            if (it.bytes) std::memcpy(*buf + it.off, it.src, it.bytes);
+            *it.dst = (const char*)d + it.off;
+        }
+        HIPTRY(h, hipMemcpyAsync(d, *buf, total, hipMemcpyHostToDevice, h->stream));
+        return NAV_BA_OK;
+    }
+};
 
 // Shoemake rotation -> quaternion, then g2o's w >= 0 unit normalisation
 // (SE3Quat(R, t) constructor used at BE_GraphOptim.cpp:92-93).
@@ -441,12 +477,15 @@ bool build_schur_items(const SchurGraph& G, int chunk, int run, SchurItems& S) {
 int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
     DevProblem& P = h->P;
     int st;
-#define UP(dst, vec) if ((st = upload(h, &(dst), (vec))) != NAV_BA_OK) return st
+    // staging: prepare synchronises the stream before the tuner, and the tuner after every timing
+    Packer pk(&h->stage_s, &h->stage_s_cap);
+#define UP(dst, vec) pk.add(&(dst), (vec))
     UP(P.sitem_f, S.sf); UP(P.sitem_a0, S.sa0); UP(P.sitem_a1, S.sa1); UP(P.sitem_g, S.sg); UP(P.sitem_nslots, S.sns);
     UP(P.sitem_pofs, S.spofs); UP(P.row_sitem_ptr, S.row_sitem_ptr); UP(P.row_nchunks, S.row_nch);
     UP(P.sitem_order, S.sorder);
     UP(P.seg_batch, S.seg_batch); UP(P.batch_ptr, S.batch_ptr); UP(P.pairs, S.pairs);
 #undef UP
+    if ((st = pk.flush(h)) != NAV_BA_OK) return st;
     if ((st = alloc(h, (void**)&h->W.spart, sizeof(double) * std::max<long long>(S.pofs, 1))) != NAV_BA_OK) return st;
     P.schur_segs = S.segs;
     P.n_sitems = (int)S.sf.size();
@@ -824,7 +863,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     P.auto_lambda = !(o->lambda_init > 0);
     P.lambda0 = o->lambda_init;
     int st;
-#define UP(dst, vec) if ((st = upload(h, &(dst), (vec))) != NAV_BA_OK) return st
+    Packer pk(&h->stage, &h->stage_cap);  // free again: every prepare ends with a stream synchronisation
+#define UP(dst, vec) pk.add(&(dst), (vec))
     UP(P.o_cam, o_cam); UP(P.o_pt, o_pt); UP(P.o_uv, o_uv); UP(P.o_orig, o_orig);
     UP(P.pt_ptr, pt_ptr); UP(P.pt_free, pt_free); UP(P.cam_free, cam_free); UP(P.free_cam, free_cam);
     UP(P.cam_ptr, cam_ptr); UP(P.cam_obs, cam_obs); UP(P.cam_obs_b0, cam_obs_b0);
@@ -837,6 +877,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.cblk_o0, cblk_o0); UP(P.cblk_win, cblk_win); UP(P.win_cblk0, win_cblk0);
     UP(P.win_tile0, win_tile0);
 #undef UP
+    if ((st = pk.flush(h)) != NAV_BA_OK) return st;
     P.n_tiles = (int)tile_win.size();
     P.n_hitems = (int)hf.size();
     P.n_win = n_win;
@@ -1016,6 +1057,8 @@ void nav_ba_destroy(nav_ba_handle* h) {
     for (auto& b : h->bufs)
         if (b.p) (void)hipFree(b.p);
     if (h->hdone) (void)hipHostFree(h->hdone);
+    if (h->stage) (void)hipHostFree(h->stage);
+    if (h->stage_s) (void)hipHostFree(h->stage_s);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
