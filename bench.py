@@ -391,6 +391,60 @@ def undistort_leg(n_points, steps, with_cpu):
     return out
 
 
+def window_leg(n_windows, reps, with_cpu):
+    """One local window per call, as the reference's front end solves them
+    (FE_SlamMonoV.cpp:154-165): nav_ba_solve end to end (upload and ordering,
+    the LM solve, results back to the host) for windows 0..n_windows-1 of the
+    uneven-window generator, each solved `reps` times after one warm-up call;
+    the prepare / run / fetch split of the same calls; the FP64 oracle on
+    window 0 as the CPU baseline."""
+    import torch
+    from nav24_amd import problems
+    from nav24_amd.abi import workload_options
+    from nav24_amd.ba import GraphOptimHip
+
+    s = GraphOptimHip(0)
+    o = workload_options(3)
+    per = []
+    for wi in range(n_windows):
+        w = problems.workload(3, seed=wi)
+        s.solve(w, o)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = s.solve(w, o)
+        t_solve = (time.perf_counter() - t0) / reps
+        tp = tr = tf = 0.0
+        for _ in range(reps):
+            a = time.perf_counter()
+            s.prepare(w, o)
+            b = time.perf_counter()
+            s.run()
+            torch.cuda.synchronize()
+            c = time.perf_counter()
+            s.fetch()
+            d = time.perf_counter()
+            tp, tr, tf = tp + b - a, tr + c - b, tf + d - c
+        per.append({"n_cam": w.n_cam, "n_obs": w.n_obs, "lm_iterations": r.iterations,
+                    "ms_per_solve": round(t_solve * 1e3, 3), "prepare_ms": round(tp / reps * 1e3, 3),
+                    "run_ms": round(tr / reps * 1e3, 3), "fetch_ms": round(tf / reps * 1e3, 3)})
+    s.close()
+    ms = np.array([x["ms_per_solve"] for x in per])
+    out = {"workload": f"windows 0..{n_windows - 1} of the uneven-window generator, one solve call each",
+           "precision": "mixed (the window options)", "median_ms_per_solve": round(float(np.median(ms)), 3),
+           "max_ms_per_solve": round(float(ms.max()), 3), "windows": per}
+    if with_cpu:
+        from oracle import oracle
+        w = problems.workload(3, seed=0)
+        o64 = workload_options(3)
+        t0 = time.perf_counter()
+        ref = oracle.solve(w, o64)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(dt * 1e3, 3), "unit": "ms per solve of window 0", "cores": 1,
+                               "kind": "port", "sample": f"window 0 ({w.n_obs} observations, {ref.iterations} "
+                                                         f"LM iterations), FP64 oracle, 1 thread"}
+    return out
+
+
 def twoview_leg(n_matches, n_hyp, steps, with_cpu):
     """Two-view RANSAC scoring (TwoViewReconstruction::FindHomography /
     FindFundamental scoring loops): n_matches matches of a synthetic planar
@@ -545,6 +599,8 @@ def main():
                     help="differentiable-cost leg: number of distorted-camera problems (0 = skip)")
     ap.add_argument("--undistort", type=int, default=3_000_000,
                     help="key-point undistortion leg: number of key points (0 = skip)")
+    ap.add_argument("--window-solves", type=int, default=8,
+                    help="per-call window leg: number of windows solved one call at a time (0 = skip)")
     ap.add_argument("--twoview", type=int, default=1000,
                     help="two-view RANSAC scoring leg: number of matches, 200 H and 200 F hypotheses (0 = skip)")
     ap.add_argument("--batched-only", action="store_true",
@@ -652,6 +708,8 @@ def main():
         line["diffcost"] = diffcost_leg(args.diffcost, max(args.steps, 5))
     if args.undistort > 0:
         line["undistort"] = undistort_leg(args.undistort, max(args.steps, 20), not args.no_cpu)
+    if args.window_solves > 0:
+        line["window_solve"] = window_leg(args.window_solves, 10, not args.no_cpu)
     if args.twoview > 0:
         line["twoview"] = twoview_leg(args.twoview, 200, max(args.steps, 20), not args.no_cpu)
 
