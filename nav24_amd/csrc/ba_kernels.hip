@@ -68,7 +68,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_lin_pt(DevProblem P, const CamRT* __restrict__ cam,
                                                    const D4* __restrict__ X, const LMDev* lm,
                                                    T* __restrict__ w_out, double* __restrict__ Hl) {
-    __shared__ double sh[9][kTile];
+    // rows padded by one double: the 9 sums of one point read one column, which
+    // an unpadded 2 KB row stride would put in a single LDS bank
+    __shared__ double sh[9][kTile + 1];
     __shared__ double red[kBlock / kWave][9];
     const int t = blockIdx.x;
     if (!needs_lin(lm, P.tile_win[t])) return;
@@ -113,9 +115,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) 
         const int n = (p1 - p0) * 9;
         for (int idx = threadIdx.x; idx < n; idx += kBlock) {
             const int p = p0 + idx / 9, k = idx % 9;
-            double s = 0.0;
-            for (int a = P.pt_ptr[p]; a < P.pt_ptr[p + 1]; ++a) s += sh[k][a - o0];
-            Hl[9 * (size_t)p + k] = s;
+            const int b0 = P.pt_ptr[p] - o0, b1 = P.pt_ptr[p + 1] - o0;
+            // four interleaved partial sums: four LDS reads in flight per step
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int a = b0;
+            for (; a + 4 <= b1; a += 4) {
+                s0 += sh[k][a];
+                s1 += sh[k][a + 1];
+                s2 += sh[k][a + 2];
+                s3 += sh[k][a + 3];
+            }
+            if (a < b1) s0 += sh[k][a];
+            if (a + 1 < b1) s1 += sh[k][a + 1];
+            if (a + 2 < b1) s2 += sh[k][a + 2];
+            Hl[9 * (size_t)p + k] = (s0 + s1) + (s2 + s3);
         }
     } else {
         const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
