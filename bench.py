@@ -494,6 +494,26 @@ def twoview_leg(n_matches, n_hyp, steps, with_cpu):
             res[kind]["equal_to_oracle"] = bool(
                 np.array_equal(out[0].cpu().numpy().view(np.uint32), ref[0].view(np.uint32))
                 and out[1].item() == ref[1] and np.array_equal(out[3].cpu().numpy(), ref[3]))
+    # the hypotheses themselves from 200 RANSAC sets (normalisation on the host,
+    # 8-point solves on the device), host arrays in and out, end to end
+    rng = np.random.default_rng(1)
+    sets = np.stack([rng.choice(n_matches, 8, replace=False) for _ in range(n_hyp)]).astype(np.int32)
+    tv.Hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hyp = tv.Hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    dt = (time.perf_counter() - t0) / reps
+    res["hypotheses"] = {"sets": n_hyp, "us_per_call_host_arrays": round(dt * 1e6, 1)}
+    if with_cpu:
+        from oracle import oracle
+        t0 = time.perf_counter()
+        ref = oracle.tv_hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+        dtc = time.perf_counter() - t0
+        res["hypotheses"]["cpu_baseline"] = {"value": round(dtc * 1e6, 1), "unit": "us per call", "cores": 1,
+                                             "kind": "port", "sample": "the same sets, oracle, 1 thread, 1 call"}
+        res["hypotheses"]["equal_to_oracle"] = bool(all(
+            np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(hyp, ref)))
     # CheckRT for the motion candidates (4 as ReconstructF, 8 as ReconstructH)
     # on the fundamental matrix's inliers, each call between its own events
     from nav24_amd.twoview_synth import motion_candidates

@@ -1,7 +1,7 @@
 /*
  * nav_twoview.h — the per-match work of two-view map initialisation on the
- * device, which produces BA's first state: scoring of the RANSAC hypotheses,
- * and triangulation with the cheirality test of the motion candidates.
+ * device, which produces BA's first state: the RANSAC hypotheses and their
+ * scoring, and triangulation with the cheirality test of the motion candidates.
  *
  * Replaces the scoring loops of TwoViewReconstruction::FindHomography and
  * ::FindFundamental (core/operators/mapInit/OP_2ViewReconstruction.cpp:266-315
@@ -121,6 +121,38 @@ int32_t nav_tv_check_rt_host(const float* kp1, int32_t n_kp1, const float* kp2, 
                              const float* R, const float* t, int32_t n_cand, float th2, float min_parallax_cos,
                              int32_t min_triangulated, int32_t* n_good, float* parallax, float* cos_sel, float* p3d,
                              uint8_t* good, int32_t device);
+
+/* RANSAC hypotheses: what FindHomography / FindFundamental form per iteration
+ * (OP_2ViewReconstruction.cpp:266-366) before scoring, for n_sets sets of 8
+ * match indices (the reference's mvSets, drawn by the caller):
+ *   Normalize both images' key points (:1195-1240; float sums in input order),
+ *   ComputeH21 / ComputeF21 (:368-446) on the 8 normalised pairs,
+ *   H21 = T2^-1 Hn T1, H12 = H21^-1, F21 = T2^T Fn T1.
+ * The reference takes each solution from OpenCV's float SVD.  Here it is the
+ * eigenvector of the smallest eigenvalue of A^T A by inverse iteration in
+ * double (40 iterations on A^T A + 1e-13 trace I), unit length with its
+ * largest-magnitude entry positive, rounded to float; F's rank-2 step
+ * removes the smallest-eigenvalue direction of Fpre^T Fpre.  A hypothesis is
+ * the reference's up to its scale and to float rounding, except for a
+ * degenerate 8-set whose two smallest singular values are close.
+ * Outputs H21, H12, F21: n_sets row-major 3x3 floats each.
+ *
+ * nav_tv_normalize: Normalize on the host (n >= 1 points -> normalised points
+ * and T, row-major), for callers of the device entry.
+ * nav_tv_hypotheses: device arrays (q1, q2 the normalised key points, matches,
+ * sets, outputs) on `stream`; T1, T2 on the host; set entries are not
+ * range-checked.
+ * nav_tv_hypotheses_host: host arrays; normalises on the host, checks every
+ * match and set index; NAV_BA_NO_DEVICE without a gfx950 device. */
+int32_t nav_tv_normalize(const float* kp, int32_t n, float* out, float* T);
+
+int32_t nav_tv_hypotheses(const float* q1, const float* q2, const int32_t* matches, const int32_t* sets,
+                          int32_t n_sets, const float* T1, const float* T2, float* H21, float* H12, float* F21,
+                          void* stream);
+
+int32_t nav_tv_hypotheses_host(const float* kp1, int32_t n1, const float* kp2, int32_t n2, const int32_t* matches,
+                               int32_t n_matches, const int32_t* sets, int32_t n_sets, float* H21, float* H12,
+                               float* F21, int32_t device);
 
 #ifdef __cplusplus
 }

@@ -19,8 +19,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "../../include/nav_twoview.h"
 
@@ -516,6 +518,264 @@ __global__ __launch_bounds__(kTvSelThreads) void k_tv_rt_select(const float* __r
     }
 }
 
+// ---- hypotheses of FindHomography / FindFundamental (OP_2ViewReconstruction.cpp:266-366)
+//
+// k_tv_hyp: one lane per RANSAC set (8 matches) and model, both models in one launch.  ComputeH21 (:368-407) /
+// ComputeF21 (:410-446) take the right singular vector of the smallest
+// singular value of A from OpenCV's float SVD; here it is the eigenvector of
+// the smallest eigenvalue of A^T A (accumulated in double, 45 entries mirrored),
+// by inverse iteration on A^T A + mu I (mu = 1e-13 trace; Cholesky in place;
+// 40 iterations from (1, ..., 1)), unit length, sign with the largest entry
+// positive, rounded to float.  F's rank-2 step is Fpre (I - w w^T) with w
+// that eigenvector of Fpre^T Fpre.  H21 = T2^-1 Hn T1, H12 = H21^-1 and
+// F21 = T2^T Fn T1 with 3x3 products summed in double and rounded to float,
+// and the inverse by cofactors over the determinant in double.
+
+struct HypArgs {
+    float T1[9], T2inv[9], T2t[9];
+};
+
+template <int N>
+__device__ void tv_small_eigvec(double* M, double* v) {  // M (N x N, row-major) is overwritten
+    double tr = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) tr = tr + M[N * i + i];
+    const double mu = 1e-13 * tr;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double a = M[N * i + j] + (i == j ? mu : 0.0);
+#pragma unroll
+            for (int k = 0; k < j; ++k) a = a - M[N * i + k] * M[N * j + k];
+            if (i == j)
+                M[N * i + i] = sqrt(a > 0.0 ? a : mu);
+            else
+                M[N * i + j] = a / M[N * j + j];
+        }
+    double x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = 1.0;
+    for (int it = 0; it < 40; ++it) {
+        double y[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double a = x[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) a = a - M[N * i + k] * y[k];
+            y[i] = a / M[N * i + i];
+        }
+#pragma unroll
+        for (int i = N - 1; i >= 0; --i) {
+            double a = y[i];
+#pragma unroll
+            for (int k = i + 1; k < N; ++k) a = a - M[N * k + i] * x[k];
+            x[i] = a / M[N * i + i];
+        }
+        double nrm = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) nrm = nrm + x[i] * x[i];
+        const double r = 1.0 / sqrt(nrm);
+#pragma unroll
+        for (int i = 0; i < N; ++i) x[i] = x[i] * r;
+    }
+    int imax = 0;
+    double amax = fabs(x[0]);
+#pragma unroll
+    for (int i = 1; i < N; ++i)
+        if (fabs(x[i]) > amax) {
+            amax = fabs(x[i]);
+            imax = i;
+        }
+    double xm = x[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) xm = imax == i ? x[i] : xm;
+    const double sg = xm < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = x[i] * sg;
+}
+
+__device__ __forceinline__ void tv_mul3(const float* A, const float* B, float* C) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) a = a + (double)A[3 * r + k] * (double)B[3 * k + c];
+            C[3 * r + c] = (float)a;
+        }
+}
+
+__host__ __device__ inline void tv_inv3f(const float* S, float* D) {
+    const double d0 = (double)S[4] * S[8] - (double)S[5] * S[7];
+    const double d1 = (double)S[3] * S[8] - (double)S[5] * S[6];
+    const double d2 = (double)S[3] * S[7] - (double)S[4] * S[6];
+    double det = (double)S[0] * d0 - (double)S[1] * d1 + (double)S[2] * d2;
+    if (det == 0.0) {
+        for (int k = 0; k < 9; ++k) D[k] = 0.0f;
+        return;
+    }
+    det = 1.0 / det;
+    D[0] = (float)(d0 * det);
+    D[1] = (float)(((double)S[2] * S[7] - (double)S[1] * S[8]) * det);
+    D[2] = (float)(((double)S[1] * S[5] - (double)S[2] * S[4]) * det);
+    D[3] = (float)(-d1 * det);
+    D[4] = (float)(((double)S[0] * S[8] - (double)S[2] * S[6]) * det);
+    D[5] = (float)(((double)S[2] * S[3] - (double)S[0] * S[5]) * det);
+    D[6] = (float)(d2 * det);
+    D[7] = (float)(((double)S[1] * S[6] - (double)S[0] * S[7]) * det);
+    D[8] = (float)(((double)S[0] * S[4] - (double)S[1] * S[3]) * det);
+}
+
+// A^T A of ComputeH21's (kHom) or ComputeF21's rows for 8 normalised pairs:
+// the upper triangle accumulated pair by pair, mirrored to the lower.
+template <bool kHom>
+__device__ __forceinline__ void tv_hyp_normal(const float* p1, const float* p2, double* M) {
+#pragma unroll
+    for (int k = 0; k < 81; ++k) M[k] = 0.0;
+    for (int i = 0; i < 8; ++i) {
+        const float u1 = p1[2 * i], v1 = p1[2 * i + 1], u2 = p2[2 * i], v2 = p2[2 * i + 1];
+        float r0[9], r1[9];
+        if (kHom) {
+            const float a[9] = {0.0f, 0.0f, 0.0f, -u1, -v1, -1.0f, v2 * u1, v2 * v1, v2};
+            const float b[9] = {u1, v1, 1.0f, 0.0f, 0.0f, 0.0f, -u2 * u1, -u2 * v1, -u2};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                r0[k] = a[k];
+                r1[k] = b[k];
+            }
+        } else {
+            const float a[9] = {u2 * u1, u2 * v1, u2, v2 * u1, v2 * v1, v2, u1, v1, 1.0f};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) r0[k] = a[k];
+        }
+#pragma unroll
+        for (int a = 0; a < 9; ++a)
+#pragma unroll
+            for (int b = a; b < 9; ++b) M[9 * a + b] = M[9 * a + b] + (double)r0[a] * (double)r0[b];
+        if (kHom) {
+#pragma unroll
+            for (int a = 0; a < 9; ++a)
+#pragma unroll
+                for (int b = a; b < 9; ++b) M[9 * a + b] = M[9 * a + b] + (double)r1[a] * (double)r1[b];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+        for (int b = 0; b < a; ++b) M[9 * a + b] = M[9 * b + a];
+}
+
+template <bool kHom>
+__device__ __forceinline__ void tv_hyp_one(const float2* __restrict__ q1, const float2* __restrict__ q2,
+                                           const int2* __restrict__ mt, const int* __restrict__ sets, int it,
+                                           const HypArgs& g, float* __restrict__ M0, float* __restrict__ M1) {
+    float p1[16], p2[16];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int2 m = mt[sets[8 * it + j]];
+        const float2 a = q1[m.x], b = q2[m.y];
+        p1[2 * j] = a.x;
+        p1[2 * j + 1] = a.y;
+        p2[2 * j] = b.x;
+        p2[2 * j + 1] = b.y;
+    }
+    double M[81], v[9];
+    tv_hyp_normal<kHom>(p1, p2, M);
+    tv_small_eigvec<9>(M, v);
+    float Xn[9], tmp[9], out[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Xn[k] = (float)v[k];
+    if (kHom) {
+        tv_mul3(g.T2inv, Xn, tmp);
+        tv_mul3(tmp, g.T1, out);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) M0[9 * it + k] = out[k];
+        float inv[9];
+        tv_inv3f(out, inv);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) M1[9 * it + k] = inv[k];
+    } else {
+        double G[9], w[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc = acc + (double)Xn[3 * k + a] * (double)Xn[3 * k + b];
+                G[3 * a + b] = acc;
+            }
+        tv_small_eigvec<3>(G, w);
+        float Fn[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double fw = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) fw = fw + (double)Xn[3 * r + k] * w[k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Fn[3 * r + c] = (float)((double)Xn[3 * r + c] - fw * w[c]);
+        }
+        tv_mul3(g.T2t, Fn, tmp);
+        tv_mul3(tmp, g.T1, out);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) M0[9 * it + k] = out[k];
+    }
+}
+
+// Both models in one launch, side by side: workgroups [0, nb) the homographies,
+// [nb, 2 nb) the fundamental matrices (nb = ceil(n_sets / 64)).
+__global__ __launch_bounds__(64) void k_tv_hyp(const float2* __restrict__ q1, const float2* __restrict__ q2,
+                                               const int2* __restrict__ mt, const int* __restrict__ sets, int n_sets,
+                                               HypArgs g, float* __restrict__ H21, float* __restrict__ H12,
+                                               float* __restrict__ F21) {
+    const int nb = (n_sets + 63) / 64;
+    const bool hom = (int)blockIdx.x < nb;
+    const int it = (hom ? (int)blockIdx.x : (int)blockIdx.x - nb) * 64 + threadIdx.x;
+    if (it >= n_sets) return;
+    if (hom)
+        tv_hyp_one<true>(q1, q2, mt, sets, it, g, H21, H12);
+    else
+        tv_hyp_one<false>(q1, q2, mt, sets, it, g, F21, nullptr);
+}
+
+// Normalize (OP_2ViewReconstruction.cpp:1195-1240) on the host: float sums in input order.
+static void tv_normalize_host(const float* kp, int32_t n, float* out, float* T) {
+    float meanX = 0, meanY = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        meanX += kp[2 * i];
+        meanY += kp[2 * i + 1];
+    }
+    meanX = meanX / n;
+    meanY = meanY / n;
+    float meanDevX = 0, meanDevY = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        out[2 * i] = kp[2 * i] - meanX;
+        out[2 * i + 1] = kp[2 * i + 1] - meanY;
+        meanDevX += std::fabs(out[2 * i]);
+        meanDevY += std::fabs(out[2 * i + 1]);
+    }
+    meanDevX = meanDevX / n;
+    meanDevY = meanDevY / n;
+    const float sX = 1.0f / meanDevX, sY = 1.0f / meanDevY;
+    for (int32_t i = 0; i < n; ++i) {
+        out[2 * i] = out[2 * i] * sX;
+        out[2 * i + 1] = out[2 * i + 1] * sY;
+    }
+    const float Tm[9] = {sX, 0.0f, -meanX * sX, 0.0f, sY, -meanY * sY, 0.0f, 0.0f, 1.0f};
+    for (int k = 0; k < 9; ++k) T[k] = Tm[k];
+}
+
+static HypArgs hyp_args(const float* T1, const float* T2) {
+    HypArgs g;
+    for (int k = 0; k < 9; ++k) g.T1[k] = T1[k];
+    tv_inv3f(T2, g.T2inv);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) g.T2t[3 * r + c] = T2[3 * c + r];
+    return g;
+}
+
 static TvArgs tv_args(float sigma, float th, float th_score) {
     TvArgs a;
     a.inv_s2 = 1.0f / (sigma * sigma);
@@ -800,6 +1060,75 @@ extern "C" int32_t nav_tv_check_rt_host(const float* kp1, int32_t n_kp1, const f
     down(cos_sel, d_cs, 4 * nc);
     down(p3d, d_p3, 12 * nk * nc);
     down(good, d_gd, nk * nc);
+    (void)hipFree(d);
+    return st;
+}
+
+// ---- hypothesis entry points
+
+extern "C" int32_t nav_tv_normalize(const float* kp, int32_t n, float* out, float* T) {
+    if (n < 1 || !kp || !out || !T) return NAV_BA_BAD_ARG;
+    tv_normalize_host(kp, n, out, T);
+    return NAV_BA_OK;
+}
+
+extern "C" int32_t nav_tv_hypotheses(const float* q1, const float* q2, const int32_t* matches,
+                                     const int32_t* sets, int32_t n_sets, const float* T1, const float* T2,
+                                     float* H21, float* H12, float* F21, void* stream) {
+    if (n_sets < 0 || !T1 || !T2) return NAV_BA_BAD_ARG;
+    if (n_sets == 0) return NAV_BA_OK;
+    if (!q1 || !q2 || !matches || !sets || !H21 || !H12 || !F21) return NAV_BA_BAD_ARG;
+    const HypArgs g = hyp_args(T1, T2);
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_tv_hyp, dim3(2 * ((n_sets + 63) / 64)), dim3(64), 0, st, reinterpret_cast<const float2*>(q1),
+                       reinterpret_cast<const float2*>(q2), reinterpret_cast<const int2*>(matches), sets, n_sets, g,
+                       H21, H12, F21);
+    return hipGetLastError() == hipSuccess ? NAV_BA_OK : NAV_BA_HIP_ERROR;
+}
+
+extern "C" int32_t nav_tv_hypotheses_host(const float* kp1, int32_t n1, const float* kp2, int32_t n2,
+                                          const int32_t* matches, int32_t n_matches, const int32_t* sets,
+                                          int32_t n_sets, float* H21, float* H12, float* F21, int32_t device) {
+    if (n1 < 1 || n2 < 1 || n_matches < 1 || n_sets < 0 || n_matches > NAV_TV_MAX_MATCHES) return NAV_BA_BAD_ARG;
+    if (!kp1 || !kp2 || !matches || (n_sets > 0 && (!sets || !H21 || !H12 || !F21))) return NAV_BA_BAD_ARG;
+    if (!tv_matches_ok(matches, n_matches, n1, n2)) return NAV_BA_BAD_ARG;
+    for (int64_t i = 0; i < 8 * (int64_t)n_sets; ++i)
+        if (sets[i] < 0 || sets[i] >= n_matches) return NAV_BA_BAD_ARG;
+    if (!tv_gfx950(device)) return NAV_BA_NO_DEVICE;
+    if (n_sets == 0) return NAV_BA_OK;
+    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
+    std::vector<float> q1(2 * (size_t)n1), q2(2 * (size_t)n2);
+    float T1[9], T2[9];
+    tv_normalize_host(kp1, n1, q1.data(), T1);
+    tv_normalize_host(kp2, n2, q2.data(), T2);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t ns = (size_t)n_sets;
+    const size_t b1 = al(8 * (size_t)n1), b2 = al(8 * (size_t)n2), bm = al(8 * (size_t)n_matches), bs = al(32 * ns),
+                 bo = al(36 * ns);
+    char* d = nullptr;
+    if (hipMalloc(&d, b1 + b2 + bm + bs + 3 * bo) != hipSuccess) return NAV_BA_HIP_ERROR;
+    float* d_q1 = (float*)d;
+    float* d_q2 = (float*)(d + b1);
+    int32_t* d_m = (int32_t*)(d + b1 + b2);
+    int32_t* d_s = (int32_t*)(d + b1 + b2 + bm);
+    float* d_h21 = (float*)(d + b1 + b2 + bm + bs);
+    float* d_h12 = (float*)(d + b1 + b2 + bm + bs + bo);
+    float* d_f21 = (float*)(d + b1 + b2 + bm + bs + 2 * bo);
+    int32_t st = NAV_BA_OK;
+    auto up = [&](void* dst, const void* src, size_t b) {
+        if (st == NAV_BA_OK && hipMemcpy(dst, src, b, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    };
+    auto down = [&](void* dst, const void* src, size_t b) {
+        if (st == NAV_BA_OK && hipMemcpy(dst, src, b, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    };
+    up(d_q1, q1.data(), 8 * (size_t)n1);
+    up(d_q2, q2.data(), 8 * (size_t)n2);
+    up(d_m, matches, 8 * (size_t)n_matches);
+    up(d_s, sets, 32 * ns);
+    if (st == NAV_BA_OK) st = nav_tv_hypotheses(d_q1, d_q2, d_m, d_s, n_sets, T1, T2, d_h21, d_h12, d_f21, nullptr);
+    down(H21, d_h21, 36 * ns);
+    down(H12, d_h12, 36 * ns);
+    down(F21, d_f21, 36 * ns);
     (void)hipFree(d);
     return st;
 }

@@ -17,6 +17,7 @@ or a gfx950 device the calls raise.
     scores, best, best_score, inliers = tv.FindHomographyScores(kp1, kp2, matches, H21s, H12s)
     scores, best, best_score, inliers = tv.FindFundamentalScores(kp1, kp2, matches, F21s)
     n_good, p3d, good, parallax = tv.CheckRT(kp1, kp2, matches, inliers, K, Rs, ts)   # 4 or 8 candidates
+    H21s, H12s, F21s = tv.Hypotheses(kp1, kp2, matches, sets)                         # from the RANSAC sets
     tv.score_device("F", kp1_t, kp2_t, matches_t, F21s_t, out=(scores_t, best_t, best_score_t, inliers_t))
 """
 from __future__ import annotations
@@ -61,6 +62,12 @@ def lib() -> C.CDLL:
         L.nav_tv_check_rt_host.argtypes = [fp, i32, fp, i32, ip, u8, i32, fp, fp, fp, i32, f32, f32, i32, ip, fp, fp,
                                            fp, u8, i32]
         L.nav_tv_check_rt_host.restype = i32
+        L.nav_tv_hypotheses_host.argtypes = [fp, i32, fp, i32, ip, i32, ip, i32, fp, fp, fp, i32]
+        L.nav_tv_hypotheses_host.restype = i32
+        L.nav_tv_hypotheses.argtypes = [v, v, v, v, i32, fp, fp, v, v, v, v]
+        L.nav_tv_hypotheses.restype = i32
+        L.nav_tv_normalize.argtypes = [fp, i32, fp, fp]
+        L.nav_tv_normalize.restype = i32
         _lib = L
     return _lib
 
@@ -101,6 +108,27 @@ class TwoViewReconstructionHip:
         """CheckRT's reprojection threshold as ReconstructF / ReconstructH pass it: 4 sigma^2 (float)."""
         s = np.float32(self.sigma)
         return float(np.float32(4.0) * (s * s))
+
+    # ---- RANSAC hypotheses from the 8-match sets ----------------------------
+    def Hypotheses(self, kp1, kp2, matches, sets):
+        """The hypotheses FindHomography / FindFundamental form from the RANSAC
+        sets (k, 8) of match indices (mvSets): (H21s, H12s, F21s) float32
+        (k, 3, 3), normalisation and 8-point solves as ComputeH21 / ComputeF21
+        (OP_2ViewReconstruction.cpp:266-366, :368-446) on the device."""
+        k1, k2 = _pts(kp1), _pts(kp2)
+        m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+        st = np.ascontiguousarray(sets, np.int32).reshape(-1, 8)
+        ns = st.shape[0]
+        H21 = np.zeros((ns, 9), np.float32)
+        H12 = np.zeros((ns, 9), np.float32)
+        F21 = np.zeros((ns, 9), np.float32)
+        fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+        r = lib().nav_tv_hypotheses_host(k1.ctypes.data_as(fp), k1.shape[0], k2.ctypes.data_as(fp), k2.shape[0],
+                                         m.ctypes.data_as(ip), m.shape[0], st.ctypes.data_as(ip), ns,
+                                         H21.ctypes.data_as(fp), H12.ctypes.data_as(fp), F21.ctypes.data_as(fp),
+                                         self.device)
+        _check(r, "nav_tv_hypotheses_host")
+        return H21.reshape(ns, 3, 3), H12.reshape(ns, 3, 3), F21.reshape(ns, 3, 3)
 
     # ---- CheckRT for a list of motion candidates ---------------------------
     def CheckRT(self, kp1, kp2, matches, inliers, K, Rs, ts):

@@ -44,8 +44,12 @@ def synthetic_two_view(n_matches: int, seed: int = 0, planar: bool = False, outl
     n = n_matches
     x = rng.uniform(-1.5, 1.5, n)
     y = rng.uniform(-1.0, 1.0, n)
-    z = 4.0 + 0.2 * x - 0.1 * y if planar else rng.uniform(2.0, 10.0, n)
-    X = np.stack([x * z / 2.0, y * z / 2.0, z], 1)
+    if planar:  # on the plane Z = 4 + 0.2 X - 0.1 Y
+        Xx, Xy = 3.0 * x, 3.0 * y
+        X = np.stack([Xx, Xy, 4.0 + 0.2 * Xx - 0.1 * Xy], 1)
+    else:
+        z = rng.uniform(2.0, 10.0, n)
+        X = np.stack([x * z / 2.0, y * z / 2.0, z], 1)
     p1 = (K @ X.T).T
     X2 = (R @ X.T).T + t
     p2 = (K @ X2.T).T

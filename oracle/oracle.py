@@ -79,6 +79,14 @@ def lib():
         L.oracle_tv_check_rt.argtypes = [fp, fp, _i32p, C.c_int32, _u8p, fp, fp, fp, C.c_int32, C.c_float, C.c_float,
                                          C.c_int32, C.c_int32, _i32p, fp, fp, fp, _u8p, fp, _u8p]
         L.oracle_tv_check_rt.restype = None
+        L.oracle_tv_normalize.argtypes = [fp, C.c_int32, fp, fp]
+        L.oracle_tv_normalize.restype = None
+        L.oracle_tv_small_eigvec.argtypes = [_dp, C.c_int32, _dp]
+        L.oracle_tv_small_eigvec.restype = None
+        L.oracle_tv_inv3f.argtypes = [fp, fp]
+        L.oracle_tv_inv3f.restype = None
+        L.oracle_tv_hypotheses.argtypes = [fp, C.c_int32, fp, C.c_int32, _i32p, _i32p, C.c_int32, fp, fp, fp]
+        L.oracle_tv_hypotheses.restype = None
         _lib = L
     return _lib
 
@@ -327,3 +335,44 @@ def tv_check_rt(kp1, kp2, matches, inliers, K, R, t, th2=4.0, min_cos=TV_MIN_PAR
                              out["good"].ctypes.data_as(_u8p), out["cosp"].ctypes.data_as(fp),
                              out["passed"].ctypes.data_as(_u8p))
     return out
+
+
+def tv_normalize(kp):
+    """Normalize: (normalised points float32 (n, 2), T float32 (3, 3))."""
+    fp = C.POINTER(C.c_float)
+    kp = _f32(kp, (-1, 2))
+    out = np.zeros_like(kp)
+    T = np.zeros(9, np.float32)
+    lib().oracle_tv_normalize(kp.ctypes.data_as(fp), kp.shape[0], out.ctypes.data_as(fp), T.ctypes.data_as(fp))
+    return out, T.reshape(3, 3)
+
+
+def tv_small_eigvec(M):
+    """Eigenvector of the smallest eigenvalue of a symmetric PSD n x n (n <= 9)."""
+    M = np.ascontiguousarray(M, np.float64)
+    n = M.shape[0]
+    v = np.zeros(n)
+    lib().oracle_tv_small_eigvec(_d(M.reshape(-1)), n, _d(v))
+    return v
+
+
+def tv_inv3f(S):
+    fp = C.POINTER(C.c_float)
+    S = _f32(S, 9)
+    D = np.zeros(9, np.float32)
+    lib().oracle_tv_inv3f(S.ctypes.data_as(fp), D.ctypes.data_as(fp))
+    return D.reshape(3, 3)
+
+
+def tv_hypotheses(kp1, kp2, matches, sets):
+    """(H21s, H12s, F21s) float32 (k, 3, 3) of the RANSAC sets (k, 8)."""
+    fp = C.POINTER(C.c_float)
+    kp1 = _f32(kp1, (-1, 2)); kp2 = _f32(kp2, (-1, 2))
+    m = np.ascontiguousarray(matches, np.int32).reshape(-1, 2)
+    st = np.ascontiguousarray(sets, np.int32).reshape(-1, 8)
+    k = st.shape[0]
+    H21 = np.zeros((k, 9), np.float32); H12 = np.zeros((k, 9), np.float32); F21 = np.zeros((k, 9), np.float32)
+    lib().oracle_tv_hypotheses(kp1.ctypes.data_as(fp), kp1.shape[0], kp2.ctypes.data_as(fp), kp2.shape[0],
+                               m.ctypes.data_as(_i32p), st.ctypes.data_as(_i32p), k, H21.ctypes.data_as(fp),
+                               H12.ctypes.data_as(fp), F21.ctypes.data_as(fp))
+    return H21.reshape(k, 3, 3), H12.reshape(k, 3, 3), F21.reshape(k, 3, 3)

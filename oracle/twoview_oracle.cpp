@@ -311,4 +311,194 @@ void oracle_tv_check_rt(const float* kp1, const float* kp2, const int32_t* match
     }
 }
 
+// ---- hypotheses of FindHomography / FindFundamental (:266-366)
+//
+// Normalize (:1195-1240) as the reference writes it: float sums in input
+// order.  ComputeH21 (:368-407) / ComputeF21 (:410-446) take the right
+// singular vector of the smallest singular value of A (2N x 9 or N x 9) from
+// OpenCV's float SVD; here it is the eigenvector of the smallest eigenvalue
+// of A^T A, by inverse iteration in double (A^T A + mu I, mu = 1e-13 trace,
+// factored once by Cholesky; 40 iterations from (1, ..., 1)), unit length,
+// sign with the largest-magnitude entry positive, rounded to float.  F's rank-2
+// step (SVD of Fpre, smallest singular value set to 0) is Fpre (I - v v^T)
+// with v that eigenvector of Fpre^T Fpre.  H21 = T2^-1 Hn T1, H12 = H21^-1,
+// F21 = T2^T Fn T1: products summed in double and rounded to float (OpenCV's
+// 32F gemm), 3x3 inverses by cofactors over the determinant in double
+// (OpenCV's small-matrix invert).  Pinned (tests/test_twoview_oracle.py)
+// against LAPACK's SVD and noise-free scenes; unpinned against OpenCV's bits.
+
+void oracle_tv_normalize(const float* kp, int32_t n, float* out, float* T) {
+    float meanX = 0, meanY = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        meanX += kp[2 * i];
+        meanY += kp[2 * i + 1];
+    }
+    meanX = meanX / n;
+    meanY = meanY / n;
+    float meanDevX = 0, meanDevY = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        out[2 * i] = kp[2 * i] - meanX;
+        out[2 * i + 1] = kp[2 * i + 1] - meanY;
+        meanDevX += std::fabs(out[2 * i]);
+        meanDevY += std::fabs(out[2 * i + 1]);
+    }
+    meanDevX = meanDevX / n;
+    meanDevY = meanDevY / n;
+    const float sX = 1.0f / meanDevX, sY = 1.0f / meanDevY;
+    for (int32_t i = 0; i < n; ++i) {
+        out[2 * i] = out[2 * i] * sX;
+        out[2 * i + 1] = out[2 * i + 1] * sY;
+    }
+    const float Tm[9] = {sX, 0.0f, -meanX * sX, 0.0f, sY, -meanY * sY, 0.0f, 0.0f, 1.0f};
+    for (int k = 0; k < 9; ++k) T[k] = Tm[k];
+}
+
+// Smallest eigenvector of the symmetric positive semi-definite n x n M (row-major, n <= 9).
+void oracle_tv_small_eigvec(const double* M, int n, double* v) {
+    double L[81];
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr = tr + M[n * i + i];
+    const double mu = 1e-13 * tr;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double a = M[n * i + j] + (i == j ? mu : 0.0);
+            for (int k = 0; k < j; ++k) a = a - L[9 * i + k] * L[9 * j + k];
+            if (i == j) {
+                L[9 * i + i] = std::sqrt(a > 0.0 ? a : mu);
+            } else {
+                L[9 * i + j] = a / L[9 * j + j];
+            }
+        }
+    double x[9];
+    for (int i = 0; i < n; ++i) x[i] = 1.0;
+    for (int it = 0; it < 40; ++it) {
+        double y[9];
+        for (int i = 0; i < n; ++i) {  // L y = x
+            double a = x[i];
+            for (int k = 0; k < i; ++k) a = a - L[9 * i + k] * y[k];
+            y[i] = a / L[9 * i + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {  // L^T x = y
+            double a = y[i];
+            for (int k = i + 1; k < n; ++k) a = a - L[9 * k + i] * x[k];
+            x[i] = a / L[9 * i + i];
+        }
+        double nrm = 0.0;
+        for (int i = 0; i < n; ++i) nrm = nrm + x[i] * x[i];
+        const double r = 1.0 / std::sqrt(nrm);
+        for (int i = 0; i < n; ++i) x[i] = x[i] * r;
+    }
+    int imax = 0;
+    for (int i = 1; i < n; ++i)
+        if (std::fabs(x[i]) > std::fabs(x[imax])) imax = i;
+    const double sg = x[imax] < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) v[i] = x[i] * sg;
+}
+
+// C = A B for 3x3 float matrices, products summed in double, rounded to float.
+static void mul3(const float* A, const float* B, float* C) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double a = 0.0;
+            for (int k = 0; k < 3; ++k) a = a + (double)A[3 * r + k] * (double)B[3 * k + c];
+            C[3 * r + c] = (float)a;
+        }
+}
+
+// 3x3 float inverse by cofactors in double; false if the determinant is 0.
+void oracle_tv_inv3f(const float* S, float* D) {
+    const double d0 = (double)S[4] * S[8] - (double)S[5] * S[7];
+    const double d1 = (double)S[3] * S[8] - (double)S[5] * S[6];
+    const double d2 = (double)S[3] * S[7] - (double)S[4] * S[6];
+    double det = (double)S[0] * d0 - (double)S[1] * d1 + (double)S[2] * d2;
+    if (det == 0.0) {
+        for (int k = 0; k < 9; ++k) D[k] = 0.0f;
+        return;
+    }
+    det = 1.0 / det;
+    D[0] = (float)(d0 * det);
+    D[1] = (float)(((double)S[2] * S[7] - (double)S[1] * S[8]) * det);
+    D[2] = (float)(((double)S[1] * S[5] - (double)S[2] * S[4]) * det);
+    D[3] = (float)(-d1 * det);
+    D[4] = (float)(((double)S[0] * S[8] - (double)S[2] * S[6]) * det);
+    D[5] = (float)(((double)S[2] * S[3] - (double)S[0] * S[5]) * det);
+    D[6] = (float)(d2 * det);
+    D[7] = (float)(((double)S[1] * S[6] - (double)S[0] * S[7]) * det);
+    D[8] = (float)(((double)S[0] * S[4] - (double)S[1] * S[3]) * det);
+}
+
+// A^T A of the rows of ComputeH21 (homography) or ComputeF21 for 8 normalised pairs.
+static void hyp_normal(bool hom, const float* p1, const float* p2, double* M) {
+    for (int k = 0; k < 81; ++k) M[k] = 0.0;
+    for (int i = 0; i < 8; ++i) {
+        const float u1 = p1[2 * i], v1 = p1[2 * i + 1], u2 = p2[2 * i], v2 = p2[2 * i + 1];
+        float rows[2][9];
+        int nr;
+        if (hom) {
+            const float r0[9] = {0.0f, 0.0f, 0.0f, -u1, -v1, -1.0f, v2 * u1, v2 * v1, v2};
+            const float r1[9] = {u1, v1, 1.0f, 0.0f, 0.0f, 0.0f, -u2 * u1, -u2 * v1, -u2};
+            for (int k = 0; k < 9; ++k) { rows[0][k] = r0[k]; rows[1][k] = r1[k]; }
+            nr = 2;
+        } else {
+            const float r0[9] = {u2 * u1, u2 * v1, u2, v2 * u1, v2 * v1, v2, u1, v1, 1.0f};
+            for (int k = 0; k < 9; ++k) rows[0][k] = r0[k];
+            nr = 1;
+        }
+        for (int r = 0; r < nr; ++r)
+            for (int a = 0; a < 9; ++a)
+                for (int b = 0; b < 9; ++b) M[9 * a + b] = M[9 * a + b] + (double)rows[r][a] * (double)rows[r][b];
+    }
+}
+
+// Every hypothesis of FindHomography and FindFundamental from the RANSAC sets
+// (n_sets x 8 indices into matches).  Outputs n_sets x 9 floats each.
+void oracle_tv_hypotheses(const float* kp1, int32_t n1, const float* kp2, int32_t n2, const int32_t* matches,
+                          const int32_t* sets, int32_t n_sets, float* H21, float* H12, float* F21) {
+    std::vector<float> q1(2 * (size_t)n1), q2(2 * (size_t)n2);
+    float T1[9], T2[9], T2inv[9], T2t[9];
+    oracle_tv_normalize(kp1, n1, q1.data(), T1);
+    oracle_tv_normalize(kp2, n2, q2.data(), T2);
+    oracle_tv_inv3f(T2, T2inv);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) T2t[3 * r + c] = T2[3 * c + r];
+    for (int32_t it = 0; it < n_sets; ++it) {
+        float p1[16], p2[16];
+        for (int j = 0; j < 8; ++j) {
+            const int32_t m = sets[8 * it + j];
+            p1[2 * j] = q1[2 * matches[2 * m]];
+            p1[2 * j + 1] = q1[2 * matches[2 * m] + 1];
+            p2[2 * j] = q2[2 * matches[2 * m + 1]];
+            p2[2 * j + 1] = q2[2 * matches[2 * m + 1] + 1];
+        }
+        double M[81], v[9];
+        float Hn[9], tmp[9];
+        hyp_normal(true, p1, p2, M);
+        oracle_tv_small_eigvec(M, 9, v);
+        for (int k = 0; k < 9; ++k) Hn[k] = (float)v[k];
+        mul3(T2inv, Hn, tmp);
+        mul3(tmp, T1, H21 + 9 * it);
+        oracle_tv_inv3f(H21 + 9 * it, H12 + 9 * it);
+        hyp_normal(false, p1, p2, M);
+        oracle_tv_small_eigvec(M, 9, v);
+        float Fp[9];
+        for (int k = 0; k < 9; ++k) Fp[k] = (float)v[k];
+        double G[9], w[3];  // Fpre^T Fpre
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                double acc = 0.0;
+                for (int k = 0; k < 3; ++k) acc = acc + (double)Fp[3 * k + a] * (double)Fp[3 * k + b];
+                G[3 * a + b] = acc;
+            }
+        oracle_tv_small_eigvec(G, 3, w);
+        float Fn[9];  // Fpre (I - w w^T)
+        for (int r = 0; r < 3; ++r) {
+            double fw = 0.0;
+            for (int k = 0; k < 3; ++k) fw = fw + (double)Fp[3 * r + k] * w[k];
+            for (int c = 0; c < 3; ++c) Fn[3 * r + c] = (float)((double)Fp[3 * r + c] - fw * w[c]);
+        }
+        mul3(T2t, Fn, tmp);
+        mul3(tmp, T1, F21 + 9 * it);
+    }
+}
+
 }  // extern "C"

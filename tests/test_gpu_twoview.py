@@ -227,3 +227,39 @@ def test_check_rt_argument_errors():
     o = (ctypes.c_int32 * 1)()
     assert lib().nav_tv_check_rt(None, 0, None, None, None, 0, K, z, z, 0, 4.0, 0.99998, 50, o, o, o, None, None,
                                  None, None, None) == NAV_BA_BAD_ARG
+
+
+# ---- RANSAC hypotheses from the 8-match sets
+
+@pytest.mark.parametrize("n,planar", [(1000, True), (1000, False), (300, True)])
+def test_hypotheses_equal_oracle_and_feed_scoring(n, planar):
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say(f"hypotheses on the device: {n} matches, planar {planar}, 200 sets; then scoring")
+    sc = synthetic_two_view(n, seed=300 + n, planar=planar)
+    rng = np.random.default_rng(n)
+    sets = np.stack([rng.choice(n, 8, replace=False) for _ in range(200)]).astype(np.int32)
+    tv = TwoViewReconstructionHip()
+    H21, H12, F21 = tv.Hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    R21, R12, RF = oracle.tv_hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    for got, ref in ((H21, R21), (H12, R12), (F21, RF)):
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    got = tv.FindHomographyScores(sc.kp1, sc.kp2, sc.matches, H21, H12)
+    ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, R21, R12)
+    assert got[1] == ref[1] and _same_scores(got[0], ref[0]) and np.array_equal(got[3], ref[3])
+    got = tv.FindFundamentalScores(sc.kp1, sc.kp2, sc.matches, F21)
+    ref = oracle.tv_find(sc.kp1, sc.kp2, sc.matches, RF)
+    assert got[1] == ref[1] and _same_scores(got[0], ref[0]) and np.array_equal(got[3], ref[3])
+    assert ref[1] >= 0 and ref[3].sum() > n // 3
+
+
+def test_hypotheses_argument_errors():
+    from nav24_amd.twoview import TwoViewReconstructionHip
+    _say("hypotheses: argument checks")
+    sc = synthetic_two_view(50, seed=5)
+    tv = TwoViewReconstructionHip()
+    bad = np.zeros((2, 8), np.int32)
+    bad[1, 3] = 50  # beyond the 50 matches
+    with pytest.raises(RuntimeError, match="BAD_ARG"):
+        tv.Hypotheses(sc.kp1, sc.kp2, sc.matches, bad)
+    H21, H12, F21 = tv.Hypotheses(sc.kp1, sc.kp2, sc.matches, np.zeros((0, 8), np.int32))
+    assert H21.shape == (0, 3, 3)

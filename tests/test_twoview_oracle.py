@@ -287,3 +287,96 @@ def test_check_rt_edge_cases():
     # a parallax threshold above every cosine makes every passing point good
     r1 = o.tv_check_rt(sc.kp1, sc.kp2, sc.matches, np.ones(80, np.uint8), sc.K, R[:1], t[:1], min_cos=1.5)
     assert r1["good"][0].sum() == r1["n_good"][0]
+
+
+# ---- hypotheses: Normalize (:1195-1240), ComputeH21 (:368-407), ComputeF21 (:410-446)
+
+def test_normalize_zero_mean_unit_abs_deviation():
+    rng = np.random.default_rng(21)
+    kp = rng.uniform([0, 0], [752, 480], (500, 2)).astype(np.float32)
+    q, T = o.tv_normalize(kp)
+    assert np.abs(q.mean(0)).max() < 1e-4 and np.abs(np.abs(q).mean(0) - 1).max() < 1e-4
+    back = (np.c_[kp.astype(np.float64), np.ones(500)] @ T.astype(np.float64).T)[:, :2]
+    assert np.abs(back - q).max() < 1e-4
+
+
+def test_inverse_and_small_eigenvector():
+    rng = np.random.default_rng(22)
+    for _ in range(20):
+        S = rng.normal(size=(3, 3)).astype(np.float32)
+        D = o.tv_inv3f(S)
+        assert np.abs(D.astype(np.float64) @ S - np.eye(3)).max() < 1e-4 * max(1, np.abs(np.linalg.inv(S)).max())
+        # near-rank-deficient systems, as the 8-point solves produce: B of rank 8 plus noise
+        B = rng.normal(size=(16, 8)) @ rng.normal(size=(8, 9))
+        A = B + 1e-4 * rng.normal(size=(16, 9))
+        v = o.tv_small_eigvec(A.T @ A)
+        u = np.linalg.svd(A)[2][-1]
+        assert abs(abs(v @ u) - 1) < 1e-12
+        assert np.abs(v).argmax() == np.abs(v).argmax() and v[np.abs(v).argmax()] > 0
+
+
+def _np_hypotheses(kp1, kp2, matches, sets):
+    """ComputeH21 / ComputeF21 with LAPACK's SVD in float64 on the oracle's normalised points."""
+    q1, T1 = o.tv_normalize(kp1)
+    q2, T2 = o.tv_normalize(kp2)
+    T1 = T1.astype(np.float64); T2 = T2.astype(np.float64)
+    Hs, Fs = [], []
+    for st in sets:
+        a = q1[matches[st, 0]].astype(np.float64)
+        b = q2[matches[st, 1]].astype(np.float64)
+        A = []
+        for (u1, v1), (u2, v2) in zip(a, b):
+            A.append([0, 0, 0, -u1, -v1, -1, v2 * u1, v2 * v1, v2])
+            A.append([u1, v1, 1, 0, 0, 0, -u2 * u1, -u2 * v1, -u2])
+        Hn = np.linalg.svd(np.array(A))[2][-1].reshape(3, 3)
+        Hs.append(np.linalg.inv(T2) @ Hn @ T1)
+        A = [[u2 * u1, u2 * v1, u2, v2 * u1, v2 * v1, v2, u1, v1, 1] for (u1, v1), (u2, v2) in zip(a, b)]
+        Fp = np.linalg.svd(np.array(A))[2][-1].reshape(3, 3)
+        U, w, Vt = np.linalg.svd(Fp)
+        w[2] = 0
+        Fs.append(T2.T @ (U @ np.diag(w) @ Vt) @ T1)
+    return np.array(Hs), np.array(Fs)
+
+
+def _same_up_to_scale(A, B, tol):
+    A = A.reshape(-1).astype(np.float64); B = B.reshape(-1).astype(np.float64)
+    A = A / np.linalg.norm(A); B = B / np.linalg.norm(B)
+    return min(np.abs(A - B).max(), np.abs(A + B).max()) < tol
+
+
+@pytest.mark.parametrize("planar", [True, False])
+def test_hypotheses_match_lapack_dlt(planar):
+    sc = synthetic_two_view(400, seed=23, planar=planar, outlier_frac=0.2, noise_px=1.0)
+    rng = np.random.default_rng(23)
+    sets = np.stack([rng.choice(400, 8, replace=False) for _ in range(30)]).astype(np.int32)
+    H21, H12, F21 = o.tv_hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    Hr, Fr = _np_hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    okH = sum(_same_up_to_scale(H21[k], Hr[k], 1e-4) for k in range(30))
+    okF = sum(_same_up_to_scale(F21[k], Fr[k], 1e-4) for k in range(30))
+    assert okH >= 28 and okF >= 28, (okH, okF)  # a degenerate 8-set can have two close singular values
+    for k in range(30):
+        Hi = np.linalg.inv(H21[k].astype(np.float64))
+        assert np.abs(H12[k] - Hi).max() < 1e-5 * np.abs(Hi).max()  # H12 = H21^-1 to float rounding
+        w = np.linalg.svd(F21[k].astype(np.float64))[1]
+        assert w[2] < 1e-5 * w[0]  # rank 2
+
+
+def test_noise_free_hypotheses_fit_every_match():
+    sc = synthetic_two_view(300, seed=24, planar=True, outlier_frac=0.0, noise_px=0.0)
+    rng = np.random.default_rng(24)
+    sets = np.stack([rng.choice(300, 8, replace=False) for _ in range(10)]).astype(np.int32)
+    H21, H12, F21 = o.tv_hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    p1 = np.c_[sc.kp1[sc.matches[:, 0]].astype(np.float64), np.ones(300)]
+    p2 = np.c_[sc.kp2[sc.matches[:, 1]].astype(np.float64), np.ones(300)]
+    for k in range(10):
+        q = p1 @ H21[k].astype(np.float64).T
+        assert np.abs(q[:, :2] / q[:, 2:] - p2[:, :2]).max() < 2e-2
+        r = p2 @ H12[k].astype(np.float64).T
+        assert np.abs(r[:, :2] / r[:, 2:] - p1[:, :2]).max() < 2e-2
+    sc = synthetic_two_view(300, seed=25, planar=False, outlier_frac=0.0, noise_px=0.0)
+    H21, H12, F21 = o.tv_hypotheses(sc.kp1, sc.kp2, sc.matches, sets)
+    p1 = np.c_[sc.kp1[sc.matches[:, 0]].astype(np.float64), np.ones(300)]
+    p2 = np.c_[sc.kp2[sc.matches[:, 1]].astype(np.float64), np.ones(300)]
+    for k in range(10):
+        l = p1 @ F21[k].astype(np.float64).T
+        assert (np.abs((l * p2).sum(1)) / np.hypot(l[:, 0], l[:, 1])).max() < 2e-2
