@@ -520,16 +520,17 @@ __global__ __launch_bounds__(kTvSelThreads) void k_tv_rt_select(const float* __r
 
 // ---- hypotheses of FindHomography / FindFundamental (OP_2ViewReconstruction.cpp:266-366)
 //
-// k_tv_hyp: one lane per RANSAC set (8 matches) and model, both models in one launch.  ComputeH21 (:368-407) /
-// ComputeF21 (:410-446) take the right singular vector of the smallest
-// singular value of A from OpenCV's float SVD; here it is the eigenvector of
-// the smallest eigenvalue of A^T A (accumulated in double, 45 entries mirrored),
-// by inverse iteration on A^T A + mu I (mu = 1e-13 trace; Cholesky in place;
-// 40 iterations from (1, ..., 1)), unit length, sign with the largest entry
-// positive, rounded to float.  F's rank-2 step is Fpre (I - w w^T) with w
-// that eigenvector of Fpre^T Fpre.  H21 = T2^-1 Hn T1, H12 = H21^-1 and
-// F21 = T2^T Fn T1 with 3x3 products summed in double and rounded to float,
-// and the inverse by cofactors over the determinant in double.
+// k_tv_hyp: one lane per RANSAC set (8 matches) and model, both models in one
+// launch.  ComputeH21 (:368-407) / ComputeF21 (:410-446) take the right
+// singular vector of the smallest singular value of A from OpenCV's float
+// SVD; here it is the eigenvector of the smallest eigenvalue of A^T A
+// (accumulated in double, 45 entries mirrored), by inverse iteration on
+// A^T A + mu I (mu = 1e-13 trace; Cholesky in place, the diagonal's
+// reciprocals formed once; 40 iterations from (1, ..., 1)), unit length, sign
+// with the largest entry positive, rounded to float.  F's rank-2 step is
+// Fpre (I - w w^T) with w that eigenvector of Fpre^T Fpre.  H21 = T2^-1 Hn T1,
+// H12 = H21^-1 and F21 = T2^T Fn T1 with 3x3 products summed in double and
+// rounded to float, and the inverse by cofactors over the determinant in double.
 
 struct HypArgs {
     float T1[9], T2inv[9], T2t[9];
@@ -553,9 +554,12 @@ __device__ void tv_small_eigvec(double* M, double* v) {  // M (N x N, row-major)
             else
                 M[N * i + j] = a / M[N * j + j];
         }
-    double x[N];
+    double x[N], rd[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = 1.0;
+    for (int i = 0; i < N; ++i) {
+        x[i] = 1.0;
+        rd[i] = 1.0 / M[N * i + i];  // the diagonal's reciprocals, formed once
+    }
     for (int it = 0; it < 40; ++it) {
         double y[N];
 #pragma unroll
@@ -563,14 +567,14 @@ __device__ void tv_small_eigvec(double* M, double* v) {  // M (N x N, row-major)
             double a = x[i];
 #pragma unroll
             for (int k = 0; k < i; ++k) a = a - M[N * i + k] * y[k];
-            y[i] = a / M[N * i + i];
+            y[i] = a * rd[i];
         }
 #pragma unroll
         for (int i = N - 1; i >= 0; --i) {
             double a = y[i];
 #pragma unroll
             for (int k = i + 1; k < N; ++k) a = a - M[N * k + i] * x[k];
-            x[i] = a / M[N * i + i];
+            x[i] = a * rd[i];
         }
         double nrm = 0.0;
 #pragma unroll

@@ -318,7 +318,7 @@ void oracle_tv_check_rt(const float* kp1, const float* kp2, const int32_t* match
 // singular vector of the smallest singular value of A (2N x 9 or N x 9) from
 // OpenCV's float SVD; here it is the eigenvector of the smallest eigenvalue
 // of A^T A, by inverse iteration in double (A^T A + mu I, mu = 1e-13 trace,
-// factored once by Cholesky; 40 iterations from (1, ..., 1)), unit length,
+// factored once by Cholesky, the diagonal's reciprocals formed once; 40 iterations from (1, ..., 1)), unit length,
 // sign with the largest-magnitude entry positive, rounded to float.  F's rank-2
 // step (SVD of Fpre, smallest singular value set to 0) is Fpre (I - v v^T)
 // with v that eigenvector of Fpre^T Fpre.  H21 = T2^-1 Hn T1, H12 = H21^-1,
@@ -369,19 +369,22 @@ void oracle_tv_small_eigvec(const double* M, int n, double* v) {
                 L[9 * i + j] = a / L[9 * j + j];
             }
         }
-    double x[9];
-    for (int i = 0; i < n; ++i) x[i] = 1.0;
+    double x[9], rd[9];
+    for (int i = 0; i < n; ++i) {
+        x[i] = 1.0;
+        rd[i] = 1.0 / L[9 * i + i];  // the diagonal's reciprocals, formed once
+    }
     for (int it = 0; it < 40; ++it) {
         double y[9];
         for (int i = 0; i < n; ++i) {  // L y = x
             double a = x[i];
             for (int k = 0; k < i; ++k) a = a - L[9 * i + k] * y[k];
-            y[i] = a / L[9 * i + i];
+            y[i] = a * rd[i];
         }
         for (int i = n - 1; i >= 0; --i) {  // L^T x = y
             double a = y[i];
             for (int k = i + 1; k < n; ++k) a = a - L[9 * k + i] * x[k];
-            x[i] = a / L[9 * i + i];
+            x[i] = a * rd[i];
         }
         double nrm = 0.0;
         for (int i = 0; i < n; ++i) nrm = nrm + x[i] * x[i];
