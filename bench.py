@@ -630,7 +630,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not run.", file=sys.stderr)
+              "sharding across GPUs is designed but not run.", file=sys.stderr)
         sys.exit(2)
 
     import torch  # device synchronisation around the timed region

@@ -70,7 +70,7 @@ typedef struct nav_ba_problem {
     double fx, fy, cx, cy;    /* pinhole intrinsics (Pinhole.cpp:30-36) */
 } nav_ba_problem;
 
-/* Solver settings.  The reference hard-codes these (SURVEY of defaults:
+/* Solver settings.  The reference hard-codes these (its defaults:
  * nIterations Problem.hpp:84, mbRobust Problem.hpp:112, Huber delta
  * BE_GraphOptim.cpp:125); here they are explicit. */
 typedef struct nav_ba_options {
