@@ -39,7 +39,7 @@ def test_cost_handle_reports_no_device_here():
     st = diffcost.lib().nav_ba_cost_create(ctypes.byref(h), 0)
     if st == NAV_BA_NO_DEVICE:
         assert not h.value
-    else:  # on a GPU box the handle is real
+    else:  # with a gfx950 device the handle is real
         diffcost.lib().nav_ba_cost_destroy(h)
 
 

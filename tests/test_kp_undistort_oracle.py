@@ -73,7 +73,7 @@ def test_library_entry_points_check_arguments_without_a_device():
     assert L.nav_ba_kp_undistort(None, -1, prm, None, None) == NAV_BA_BAD_ARG
     assert L.nav_ba_kp_undistort(None, 2, prm, None, None) == NAV_BA_BAD_ARG
     st = L.nav_ba_kp_undistort_host(buf, 2, prm, buf, 0)
-    assert st in (NAV_BA_NO_DEVICE, 0)  # 0 only on a GPU box
+    assert st in (NAV_BA_NO_DEVICE, 0)  # 0 only with a gfx950 device
 
 
 # ---- Kannala-Brandt fisheye (KannalaBrandt8::UndistortKeyPoints, cv::fisheye::undistortPoints)
