@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 
+#include "host_scratch.h"
 #include "../../include/nav_twoview.h"
 
 #pragma clang fp contract(off)
@@ -826,8 +827,10 @@ static int32_t tv_host(const float* kp1, int32_t n1, const float* kp2, int32_t n
     const size_t b_k1 = al(8 * (size_t)n1), b_k2 = al(8 * (size_t)n2), b_mt = al(8 * (size_t)n),
                  b_m = al(36 * (size_t)n_hyp), b_sc = al(4 * (size_t)n_hyp), b_out = al(8), b_in = al((size_t)n);
     const size_t total = b_k1 + b_k2 + b_mt + nm * b_m + b_sc + b_out + b_in;
-    char* d = nullptr;
-    if (hipMalloc(&d, total) != hipSuccess) return NAV_BA_HIP_ERROR;
+    HostScratch& hs = host_scratch(device);
+    std::lock_guard<std::mutex> lock(hs.mu);
+    char* d = (char*)host_scratch_get(hs, total);
+    if (!d) return NAV_BA_HIP_ERROR;
     char* p = d;
     float* d_k1 = (float*)p;
     p += b_k1;
@@ -865,7 +868,6 @@ static int32_t tv_host(const float* kp1, int32_t n1, const float* kp2, int32_t n
     down(best, d_best, 4);
     down(best_score, d_bs, 4);
     down(inliers, d_in, (size_t)n);
-    (void)hipFree(d);
     return st;
 }
 
@@ -1020,9 +1022,10 @@ extern "C" int32_t nav_tv_check_rt_host(const float* kp1, int32_t n_kp1, const f
     const size_t b_k1 = al(8 * nk), b_k2 = al(8 * (size_t)n_kp2), b_mt = al(8 * nm), b_in = al(nm),
                  b_p3 = al(12 * nk * nc), b_gd = al(nk * nc), b_cp = al(4 * nm * nc), b_ps = al(nm * nc),
                  b_o = al(12 * nc);
-    char* d = nullptr;
-    if (hipMalloc(&d, b_k1 + b_k2 + b_mt + b_in + b_p3 + b_gd + b_cp + b_ps + b_o) != hipSuccess)
-        return NAV_BA_HIP_ERROR;
+    HostScratch& hs = host_scratch(device);
+    std::lock_guard<std::mutex> lock(hs.mu);
+    char* d = (char*)host_scratch_get(hs, b_k1 + b_k2 + b_mt + b_in + b_p3 + b_gd + b_cp + b_ps + b_o);
+    if (!d) return NAV_BA_HIP_ERROR;
     char* p = d;
     float* d_k1 = (float*)p;
     p += b_k1;
@@ -1064,7 +1067,6 @@ extern "C" int32_t nav_tv_check_rt_host(const float* kp1, int32_t n_kp1, const f
     down(cos_sel, d_cs, 4 * nc);
     down(p3d, d_p3, 12 * nk * nc);
     down(good, d_gd, nk * nc);
-    (void)hipFree(d);
     return st;
 }
 
@@ -1109,8 +1111,10 @@ extern "C" int32_t nav_tv_hypotheses_host(const float* kp1, int32_t n1, const fl
     const size_t ns = (size_t)n_sets;
     const size_t b1 = al(8 * (size_t)n1), b2 = al(8 * (size_t)n2), bm = al(8 * (size_t)n_matches), bs = al(32 * ns),
                  bo = al(36 * ns);
-    char* d = nullptr;
-    if (hipMalloc(&d, b1 + b2 + bm + bs + 3 * bo) != hipSuccess) return NAV_BA_HIP_ERROR;
+    HostScratch& hs = host_scratch(device);
+    std::lock_guard<std::mutex> lock(hs.mu);
+    char* d = (char*)host_scratch_get(hs, b1 + b2 + bm + bs + 3 * bo);
+    if (!d) return NAV_BA_HIP_ERROR;
     float* d_q1 = (float*)d;
     float* d_q2 = (float*)(d + b1);
     int32_t* d_m = (int32_t*)(d + b1 + b2);
@@ -1133,6 +1137,5 @@ extern "C" int32_t nav_tv_hypotheses_host(const float* kp1, int32_t n1, const fl
     down(H21, d_h21, 36 * ns);
     down(H12, d_h12, 36 * ns);
     down(F21, d_f21, 36 * ns);
-    (void)hipFree(d);
     return st;
 }

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstring>
 
+#include "host_scratch.h"
 #include "../../include/nav_kp_undistort.h"
 
 #pragma clang fp contract(off)
@@ -160,13 +161,14 @@ extern "C" int32_t nav_ba_kp_undistort_host(const float* uv, int32_t n, const fl
     if (n == 0) return NAV_BA_OK;
     if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
     const size_t bytes = sizeof(float) * 2 * (size_t)n;
-    float* d = nullptr;
-    if (hipMalloc(&d, bytes) != hipSuccess) return NAV_BA_HIP_ERROR;
+    HostScratch& hs = host_scratch(device);
+    std::lock_guard<std::mutex> lock(hs.mu);
+    float* d = (float*)host_scratch_get(hs, bytes);
+    if (!d) return NAV_BA_HIP_ERROR;
     int32_t st = NAV_BA_OK;
     if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
     if (st == NAV_BA_OK) st = nav_ba_kp_undistort(d, n, params, d, nullptr);
     if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
-    (void)hipFree(d);
     return st;
 }
 
@@ -187,12 +189,13 @@ extern "C" int32_t nav_ba_kp_undistort_fisheye_host(const float* uv, int32_t n, 
     if (n == 0) return NAV_BA_OK;
     if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
     const size_t bytes = sizeof(float) * 2 * (size_t)n;
-    float* d = nullptr;
-    if (hipMalloc(&d, bytes) != hipSuccess) return NAV_BA_HIP_ERROR;
+    HostScratch& hs = host_scratch(device);
+    std::lock_guard<std::mutex> lock(hs.mu);
+    float* d = (float*)host_scratch_get(hs, bytes);
+    if (!d) return NAV_BA_HIP_ERROR;
     int32_t st = NAV_BA_OK;
     if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
     if (st == NAV_BA_OK) st = nav_ba_kp_undistort_fisheye(d, n, params, d, nullptr);
     if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
-    (void)hipFree(d);
     return st;
 }
