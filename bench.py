@@ -568,14 +568,14 @@ def copy_calibration(mb, steps):
                       "ms": round(dt * 1e3, 4), "GBps_read_plus_write": round(2 * n * 4 / dt / 1e9, 1)}), flush=True)
 
 
-def batched_profile(n_windows, steps, warmup):
+def batched_profile(n_windows, steps, warmup, precision="mixed"):
     import torch
     from nav24_amd import problems
-    from nav24_amd.abi import NAV_BA_MIXED, workload_options
+    from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
     from nav24_amd.ba import GraphOptimHip
     wins = [problems.workload(3, seed=w) for w in range(n_windows)]
     o = workload_options(3)
-    o.precision = NAV_BA_MIXED
+    o.precision = NAV_BA_MIXED if precision == "mixed" else NAV_BA_FP64
     s = GraphOptimHip(0)
     s.prepare_batched(wins, o)
     for _ in range(warmup):
@@ -625,6 +625,8 @@ def main():
                     help="two-view RANSAC scoring leg: number of matches, 200 H and 200 F hypotheses (0 = skip)")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling: run only the batched mixed solve (K timed runs) and print its timing")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="profiling (--batched-only): precision of the batched solve")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="profiling: K device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -638,7 +640,7 @@ def main():
         copy_calibration(args.copy_mb, args.steps)
         return
     if args.batched_only:
-        batched_profile(args.windows, args.steps, args.warmup)
+        batched_profile(args.windows, args.steps, args.warmup, args.precision)
         return
     from nav24_amd import problems
     from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options

@@ -71,7 +71,7 @@ typedef struct nav_ba_problem {
 } nav_ba_problem;
 
 /* Solver settings.  The reference hard-codes these (its defaults:
- * nIterations Problem.hpp:84, mbRobust Problem.hpp:112, Huber delta
+ * nIterations Problem.hpp:18,36, mbRobust Problem.hpp:46, Huber delta
  * BE_GraphOptim.cpp:125); here they are explicit. */
 typedef struct nav_ba_options {
     int32_t n_iter;         /* outer LM iterations, SparseOptimizer::optimize(n) (BE_GraphOptim.cpp:68) */
@@ -139,8 +139,8 @@ int32_t nav_ba_fetch(nav_ba_handle* h, nav_ba_result* r);
  * (FE_SlamMonoV.cpp:161-165, BE_GraphOptim.cpp:31-77 once per window).  Every
  * window gets its own LM schedule (lambda, nu, trials, termination) exactly as
  * if it were solved alone; all windows share the options and the intrinsics.
- * A window may hold at most 30 free cameras (NAV_BA_TOO_LARGE otherwise) and
- * at most 8192 cameras.  `results[w]` receives window w's results, with the
+ * A window may hold at most 30 free cameras (NAV_BA_TOO_LARGE otherwise).
+ * `results[w]` receives window w's results, with the
  * same meaning as for nav_ba_fetch; solve_ms is the time of the whole batch.
  * nav_ba_run runs a prepared batch; an empty window reports NAV_BA_STOP_EMPTY
  * and fetch returns NAV_BA_EMPTY if any window was empty (the others are

@@ -70,10 +70,16 @@ int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk);
  * the problem had nothing to optimise). */
 int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk);
 
-/* Schur pairs of one column slot that a lane sums in registers before one
- * LDS add, for the next nav_ba_prepare: 0 = default (4), or 1..64.  Any value
- * gives the same reduced system up to the order of FP64 additions. */
-int32_t nav_ba_debug_set_schur_run(nav_ba_handle* h, int32_t run);
+/* Host-only check of the structure a prepare builds (no device needed): the
+ * flat graph arrays and the Schur work items at `chunk` observations per item
+ * (0 = the default), every index against the buffer it addresses and every
+ * Schur round's pairs against a direct enumeration of the row's pairs.
+ * Returns NAV_BA_OK when consistent (pair entries and rounds in the outputs,
+ * either may be NULL), -1 with the first violation in msg, or a status for
+ * invalid input. */
+int32_t nav_ba_debug_check_layout(const nav_ba_problem* problems, int32_t n_windows, const nav_ba_options* o,
+                                  int32_t batched, int32_t chunk, int64_t* n_pairs, int32_t* n_rounds,
+                                  char* msg, int32_t msg_len);
 
 #ifdef __cplusplus
 }

@@ -131,7 +131,8 @@ class Problem:
 @dataclass
 class Options:
     """Solver settings (nav_ba_options).  Defaults are the reference's
-    (core/backEnd/Problem.hpp:84,112; BE_GraphOptim.cpp:125)."""
+    (nIterations core/backEnd/Problem.hpp:18,36; mbRobust Problem.hpp:46;
+    Huber delta BE_GraphOptim.cpp:125)."""
     n_iter: int = 5
     robust: bool = False
     huber_delta: float = float(np.float32(np.sqrt(5.99)))

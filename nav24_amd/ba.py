@@ -27,7 +27,7 @@ EXPORTED = [
     "nav_ba_default_options", "nav_ba_solve", "nav_ba_prepare", "nav_ba_run", "nav_ba_fetch",
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
-    "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_set_schur_run",
+    "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_check_layout",
 ]
 
 STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
@@ -82,8 +82,10 @@ def lib() -> C.CDLL:
         L.nav_ba_debug_set_schur_chunk.restype = C.c_int32
         L.nav_ba_debug_get_schur_chunk.argtypes = [H, C.POINTER(C.c_int32)]
         L.nav_ba_debug_get_schur_chunk.restype = C.c_int32
-        L.nav_ba_debug_set_schur_run.argtypes = [H, C.c_int32]
-        L.nav_ba_debug_set_schur_run.restype = C.c_int32
+        L.nav_ba_debug_check_layout.argtypes = [C.POINTER(NavBaProblem), C.c_int32, C.POINTER(NavBaOptions),
+                                                C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                                C.c_char_p, C.c_int32]
+        L.nav_ba_debug_check_layout.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -209,10 +211,6 @@ class GraphOptimHip:
         (tuned at prepare on large problems), or a multiple of 64 up to 4096."""
         self._check(lib().nav_ba_debug_set_schur_chunk(self._h, int(chunk)))
 
-    def set_schur_run(self, run: int) -> None:
-        """Schur pairs per register run for the next prepare: 0 default (4), or 1..64."""
-        self._check(lib().nav_ba_debug_set_schur_run(self._h, int(run)))
-
     @property
     def schur_chunk(self) -> int:
         """Observations per Schur work item chosen by the last prepare."""
@@ -228,6 +226,21 @@ class GraphOptimHip:
         self._check(lib().nav_ba_debug_reduced_system(self._h, lam, S.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
                                                       n.value, C.byref(n)), allow_empty=True)
         return S[: n.value, : n.value], g[: n.value]
+
+
+def check_layout(windows: Sequence[Problem], opts: Options, batched: bool = False, chunk: int = 0) -> dict:
+    """Host-only check of the structure prepare builds for these windows (no
+    device needed): raises BAError with the first violation, else returns the
+    Schur pair-entry and round counts."""
+    arr = (NavBaProblem * len(windows))(*[w.c_struct() for w in windows])
+    os_ = opts.c_struct()
+    n_pairs, n_rounds = C.c_int64(0), C.c_int32(0)
+    msg = C.create_string_buffer(256)
+    st = lib().nav_ba_debug_check_layout(arr, len(windows), C.byref(os_), int(bool(batched)), int(chunk),
+                                         C.byref(n_pairs), C.byref(n_rounds), msg, 256)
+    if st != NAV_BA_OK:
+        raise BAError(st, msg.value.decode())
+    return {"pairs": n_pairs.value, "rounds": n_rounds.value}
 
 
 def version() -> str:

@@ -19,11 +19,8 @@ constexpr int kBlock = 256;        // threads per workgroup for the per-observat
 constexpr int kTile = 256;         // observations per point-aligned linearisation tile
 constexpr int kChunk = 1024;       // camera-major observations per Hpp work item
 constexpr int kSchurChunk = 256;   // camera-major observations per Schur work item (default; tuned per problem)
-constexpr int kMaxSlots = 160;     // S-row column blocks per work item (LDS budget)
-constexpr int kRunLen = 4;         // Schur pairs of one slot a lane sums in registers before one LDS add
+constexpr int kMaxSlots = 112;     // S-row column blocks per work item: 78 KiB of LDS, two workgroups per CU
 constexpr int kNB = 64;            // tiled dense Cholesky tile
-constexpr int kMaxWinCams = 8192;  // cameras per window: Schur slot map in LDS is int16
-constexpr int kCamLdsMax = 512;    // Schur rows keep the window's camera table in LDS up to this size
 constexpr int kSmallMaxN = 180;    // per-window in-LDS Cholesky: at most 30 free cameras
 constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Cholesky is possible up to this size
 constexpr int kBlocksAutoFc = 16;  // ... and chosen automatically up to this size (one CU's latency chain
@@ -76,20 +73,23 @@ struct DevProblem {
     const int* hitem_a1;
     const int* cam_hitem_ptr;  // [n_fc + 1]
     int n_hitems;
-    // Schur work items (free camera, column group, chunk), group-major per camera
+    // Schur work items (free camera, column group, chunk), group-major per camera;
+    // each item is taken in rounds of kBlock observations
     const int* sitem_f;
     const int* sitem_a0;
     const int* sitem_a1;
     const int* sitem_g;
     const int* sitem_nslots;
     const long long* sitem_pofs;
+    const int* sitem_rnd0;     // [n_sitems] first round of the item
     const int* row_sitem_ptr;  // [n_fc + 1]
     const int* row_nchunks;    // [n_fc]
     const int* sitem_order;    // [n_sitems] dispatch order: estimated work, heaviest first
-    const int* seg_batch;      // [n_sitems * schur_segs + 1] batch range of each (item, round, wave)
-    const int* batch_ptr;      // [batches + 1] pair range of each batch (64 runs, R rows: 64 R entries)
-    const int2* pairs;         // {observation b, owner lane | slot << 6 | local camera << 14}; {-1, 0} padding
-    int schur_segs;
+    const uint8_t* rnd_lane_slot;  // [rounds][kBlock] slot of each lane (slots' lanes contiguous), 255 idle
+    const int* rnd_pair0;      // [rounds] first pair entry of the round
+    const int4* rnd_rows;      // [rounds] pair rows of each wave (64 entries a row)
+    const uint8_t* rnd_group;  // [rounds] lanes per group (2 or 4): a slot's lanes are whole aligned groups
+    const int2* pairs;         // {observation b, row of observation a in the round}; {-1, 0} pads a lane's list
     int n_sitems;
     int max_ns;                // largest slot count of any Schur work item
     // column blocks of each S row: free indices >= row (same window), first is the row itself
@@ -99,7 +99,6 @@ struct DevProblem {
     const int* zcolcam;        //   (written as zeros by the reduction every trial)
     // windows
     int n_win;
-    int max_win_cams;
     int max_win_fc;            // free cameras of the largest window
     int batched;               // 1: per-window dense systems factored in LDS
     const int* cam_win;        // [n_cam]

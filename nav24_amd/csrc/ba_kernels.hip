@@ -17,8 +17,7 @@
 //   cost       robust chi2 at the candidate, window-aligned partial sums
 //   accept     rho, lambda / nu schedule, accept or reject, next phase
 //   commit     candidate -> current for windows whose trial was accepted
-// Reductions use fixed trees (deterministic), except the off-diagonal S-row
-// blocks which use LDS atomics.
+// Every reduction has a fixed order: results are deterministic.
 #include <float.h>
 
 #include "ba_kernels.h"

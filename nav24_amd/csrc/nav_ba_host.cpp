@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -95,6 +96,57 @@ struct WinHost {
     bool empty = false;
 };
 
+// ---- Schur work items -------------------------------------------------------------------
+// Host structure the items are built from (camera-major lists of free cameras,
+// per-row co-visible column blocks, point-major observations).
+struct SchurGraph {
+    int nfc, nc;
+    const int* colptr;
+    const int* colcam;
+    const int* cam_ptr;
+    const int* cam_obs;
+    const int* cam_obs_b0;
+    const int* o_pt;
+    const int* o_cam;
+    const int* pt_ptr;
+    const uint8_t* pt_free;
+    const int* free_cam;
+    const int* cam_win;
+    const int* cam_off;
+};
+
+// Host-side structure of n_win windows laid end to end (see ba_kernels.h):
+// everything prepare uploads, built without touching the device.
+struct HostLayout {
+    int n_win = 0, nc = 0, np = 0, no = 0, nfc = 0;
+    bool batched = false, all_empty = true;
+    std::vector<WinHost> win;
+    std::vector<int> cam_off, obs_off, pt_first, cam_win, pt_win;
+    std::vector<int> o_cam, o_pt, o_orig, pt_ptr;
+    std::vector<double> o_uv;
+    std::vector<uint8_t> pt_free;
+    std::vector<int> cam_free, free_cam, win_fc0, win_cam0;
+    std::vector<int> cam_ptr, cam_obs, cam_obs_b0, c_pt;
+    std::vector<double> c_uv;
+    std::vector<int> tile_obs, tile_pt, tile_win, win_tile0;
+    std::vector<int> cblk_o0, cblk_win, win_cblk0, win_pt0;
+    std::vector<int> hf, ha0, ha1, cam_hitem_ptr;
+    std::vector<int> colptr, colcam, zcolptr, zcolcam;
+    std::vector<long long> win_S;
+    std::vector<int> win_lda;
+    long long S_size = 1;
+    int n = 0, n_pad = 0;
+    std::vector<PoseQT> pose0;
+    std::vector<D4> X0;
+    double fx = 0, fy = 0, cx = 0, cy = 0;
+
+    SchurGraph graph() const {
+        return SchurGraph{nfc, nc, colptr.data(), colcam.data(), cam_ptr.data(), cam_obs.data(), cam_obs_b0.data(),
+                          o_pt.data(), o_cam.data(), pt_ptr.data(), pt_free.data(), free_cam.data(), cam_win.data(),
+                          cam_off.data()};
+    }
+};
+
 }  // namespace
 
 struct nav_ba_handle {
@@ -105,7 +157,7 @@ struct nav_ba_handle {
     bool prepared = false;
     bool batched = false;
     nav_ba_options opt{};
-    std::vector<WinHost> win;
+    HostLayout layout;  // the last prepare's host structure (fetch reads its windows)
     bool all_empty = false;
     DevProblem P{};
     DevState cur{}, cand{}, init{};
@@ -119,7 +171,6 @@ struct nav_ba_handle {
     int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks
     int schur_chunk = 0;  // observations per Schur work item: 0 = default or tuned
     int schur_chunk_used = 0;
-    int schur_run = kRunLen;  // Schur pairs per register run
     StageTimer timer;
     int* hdone = nullptr;      // pinned: windows finished
     char* stage = nullptr;     // pinned staging of a prepare's uploads (grown on demand)
@@ -336,58 +387,70 @@ int validate(const nav_ba_problem* p) {
     return NAV_BA_OK;
 }
 
-// ---- Schur work items -------------------------------------------------------------------
-// Host structure the items are built from (camera-major lists of free cameras,
-// per-row co-visible column blocks, point-major observations).
-struct SchurGraph {
-    int nfc, nc;
-    const int* colptr;
-    const int* colcam;
-    const int* cam_ptr;
-    const int* cam_obs;
-    const int* cam_obs_b0;
-    const int* o_pt;
-    const int* o_cam;
-    const int* pt_ptr;
-    const uint8_t* pt_free;
-    const int* free_cam;
-    const int* cam_win;
-    const int* cam_off;
-};
-
 // Work items of one chunk size (camera-major observations per item) and their pair lists.
 struct SchurItems {
-    int chunk = 0, segs = 0, max_ns = 1;
+    int chunk = 0, max_ns = 1;
     long long pofs = 0;
-    std::vector<int> sf, sa0, sa1, sg, sns, row_sitem_ptr, row_nch, seg_batch, batch_ptr, sorder;
+    std::vector<int> sf, sa0, sa1, sg, sns, srnd0, row_sitem_ptr, row_nch, sorder, rnd_pair0;
     std::vector<long long> spofs;
+    std::vector<uint8_t> lane_slot;  // [rounds][kBlock]
+    std::vector<uint8_t> rnd_group;  // [rounds] lanes per group
+    std::vector<int4> rnd_rows;      // [rounds]
     std::vector<int2> pairs;
 };
 
-// Items: per S row, column groups of kMaxSlots, chunks of `chunk` observations,
-// dispatched heaviest first (pair iterations + observations).  Pairs: per
-// item, per (round, wave) segment, the pairs (a, b) of each lane's
-// observation a with the later observations b of its point on free cameras of
-// the item's column group (a itself excluded: the diagonal pair is summed in
-// registers).  Entry: {b, lane | slot << 6 | local camera << 14}; {-1, 0} is
-// padding.  A segment's pairs are grouped by slot into runs of at most
-// `run` pairs (a slot's pairs split evenly), the runs sorted longest first
-// (the runs of one slot spread apart) and packed 64 to a batch: pair j of the batch's run l at batch offset
-// 64 j + l, R = the batch's longest run rows.  A lane then works through one
-// run (one camera, one slot) and adds its 6x6 block to LDS once per batch.
-// False when the pair count exceeds int32.
-bool build_schur_items(const SchurGraph& G, int chunk, int run, SchurItems& S) {
+// Lanes of one round over its slots, in aligned groups of grp: one group per
+// slot with pairs, then one group at a time to the slot whose lanes carry the
+// most pairs each.  Returns the largest share (pairs of one lane).
+int assign_lanes(const std::vector<std::vector<int2>>& bucket, int ns, int lanes, int grp, std::vector<int>& nlane) {
+    nlane.assign(ns, 0);
+    int used = 0, worst = 0;
+    std::priority_queue<std::pair<int, int>> pq;
+    for (int sl = 0; sl < ns; ++sl)
+        if (!bucket[sl].empty()) {
+            nlane[sl] = grp;
+            used += grp;
+            const int c = (int)bucket[sl].size();
+            pq.push({(c + grp - 1) / grp, -sl});
+        }
+    if (used > lanes) return 1 << 30;
+    while (used + grp <= lanes && !pq.empty() && pq.top().first > 1) {
+        const int sl = -pq.top().second;
+        pq.pop();
+        nlane[sl] += grp;
+        used += grp;
+        const int c = (int)bucket[sl].size();
+        pq.push({(c + nlane[sl] - 1) / nlane[sl], -sl});
+    }
+    if (!pq.empty()) worst = pq.top().first;
+    return worst;
+}
+
+// Items: per S row, column groups of kMaxSlots slots, chunks of `chunk`
+// observations taken in rounds of kBlock, dispatched heaviest first.
+// Pairs of a round: the pairs (a, b) of each of its observations a with the
+// later observations b of a's point on free cameras of the item's column
+// group (a itself excluded: the diagonal pair is summed in phase A), grouped
+// by the slot of b's camera, in order of a then b.  The slots with pairs share
+// the round's lanes (assign_lanes; groups of 4 unless groups of 2 give a
+// smaller largest share), and each slot's pairs are split into contiguous,
+// nearly equal shares, one per lane; a slot's lanes are contiguous.  Entry:
+// {b, a's row in the round}.  Per wave, row j holds every lane's pair j
+// ({-1, 0} past a lane's share).  False when the pair entries exceed int32.
+bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
     S = SchurItems{};
     S.chunk = chunk;
+    const int lanes = kBlock;
+    const int sg_n = kMaxSlots;
     S.row_sitem_ptr.push_back(0);
     for (int f = 0; f < G.nfc; ++f) {
         const int ncols = G.colptr[f + 1] - G.colptr[f];
-        const int ngroups = (ncols + kMaxSlots - 1) / kMaxSlots;
+        const int ngroups = (ncols + sg_n - 1) / sg_n;
         const int nobs = G.cam_ptr[f + 1] - G.cam_ptr[f];
         const int nch = std::max(1, (nobs + chunk - 1) / chunk);
         S.row_nch.push_back(nch);
         for (int gi = 0; gi < ngroups; ++gi) {
-            const int ns = std::min(kMaxSlots, ncols - gi * kMaxSlots);
+            const int ns = std::min(sg_n, ncols - gi * sg_n);
             S.max_ns = std::max(S.max_ns, ns);
             for (int ch = 0; ch < nch; ++ch) {
                 S.sf.push_back(f);
@@ -401,70 +464,71 @@ bool build_schur_items(const SchurGraph& G, int chunk, int run, SchurItems& S) {
         }
         S.row_sitem_ptr.push_back((int)S.sf.size());
     }
-    const int rounds = (chunk + kBlock - 1) / kBlock;
-    S.segs = rounds * (kBlock / kWave);
-    S.seg_batch.assign(1, 0);
-    S.batch_ptr.assign(1, 0);
     std::vector<long long> scost(S.sf.size(), 0);
     std::vector<int> slotmap(G.nc, -1);
     std::vector<std::vector<int2>> bucket;
-    struct Run { int slot, start, len, u; };
-    std::vector<Run> runs;
+    std::vector<int> nlane, nlane2, lane_s, lane_beg, lane_len;
     for (size_t it = 0; it < S.sf.size(); ++it) {
         const int f = S.sf[it], gi = S.sg[it], ns = S.sns[it];
-        const int col0 = G.colptr[f] + gi * kMaxSlots;
-        const int w = G.cam_win[G.free_cam[f]];
+        const int col0 = G.colptr[f] + gi * sg_n;
         for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = sl;
         if (bucket.size() < (size_t)ns) bucket.resize(ns);
-        long long iters = 0;
-        for (int r = 0; r < rounds; ++r)
-            for (int wv = 0; wv < kBlock / kWave; ++wv) {
-                for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
-                const int kb = S.sa0[it] + r * kBlock + wv * kWave;
-                for (int k = kb; k < std::min(kb + kWave, S.sa1[it]); ++k) {
-                    const int a = G.cam_obs[k];
-                    const int q = G.o_pt[a];
-                    if (!G.pt_free[q]) continue;
-                    for (int b = G.cam_obs_b0[k]; b < G.pt_ptr[q + 1]; ++b) {
-                        if (b == a) continue;
-                        const int sl = slotmap[G.o_cam[b]];
-                        if (sl < 0) continue;
-                        bucket[sl].push_back(int2{b, (k - kb) | (sl << 6) | ((G.o_cam[b] - G.cam_off[w]) << 14)});
-                    }
+        S.srnd0.push_back((int)S.rnd_rows.size());
+        long long cost = 0;
+        for (int kb = S.sa0[it]; kb < S.sa1[it]; kb += lanes) {
+            for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
+            for (int k = kb; k < std::min(kb + lanes, S.sa1[it]); ++k) {
+                const int a = G.cam_obs[k];
+                const int q = G.o_pt[a];
+                if (!G.pt_free[q]) continue;
+                for (int b = G.cam_obs_b0[k]; b < G.pt_ptr[q + 1]; ++b) {
+                    if (b == a) continue;
+                    const int sl = slotmap[G.o_cam[b]];
+                    if (sl < 0) continue;
+                    bucket[sl].push_back(int2{b, k - kb});
                 }
-                runs.clear();
-                for (int sl = 0; sl < ns; ++sl) {
-                    const int c = (int)bucket[sl].size();
-                    if (c == 0) continue;
-                    const int nr = (c + run - 1) / run, base = c / nr, extra = c % nr;
-                    for (int u = 0, st = 0; u < nr; ++u) {
-                        const int len = base + (u < extra ? 1 : 0);
-                        runs.push_back(Run{sl, st, len, u});
-                        st += len;
-                    }
-                }
-                // longest first; among equal lengths the slots' first runs, then
-                // their second runs, ...: lanes next to each other hold
-                // different slots, so their LDS adds hit different addresses
-                std::stable_sort(runs.begin(), runs.end(), [](const Run& x, const Run& y) {
-                    return x.len != y.len ? x.len > y.len : x.u < y.u;
-                });
-                for (size_t r0 = 0; r0 < runs.size(); r0 += kWave) {
-                    const int R = runs[r0].len;
-                    const size_t base = S.pairs.size();
-                    if (base + (size_t)kWave * R > (size_t)0x7fffffff) return false;
-                    S.pairs.resize(base + (size_t)kWave * R, int2{-1, 0});
-                    for (size_t l = 0; l < (size_t)kWave && r0 + l < runs.size(); ++l) {
-                        const Run& rn = runs[r0 + l];
-                        for (int j = 0; j < rn.len; ++j) S.pairs[base + (size_t)kWave * j + l] = bucket[rn.slot][rn.start + j];
-                    }
-                    S.batch_ptr.push_back((int)S.pairs.size());
-                    iters += R;
-                }
-                S.seg_batch.push_back((int)S.batch_ptr.size() - 1);
             }
+            int grp = 4;
+            const int worst4 = assign_lanes(bucket, ns, lanes, 4, nlane);
+            if (assign_lanes(bucket, ns, lanes, 2, nlane2) < worst4) {
+                grp = 2;
+                nlane.swap(nlane2);
+            }
+            S.rnd_group.push_back((uint8_t)grp);
+            // lanes laid out slot by slot (each slot's lanes whole, aligned groups);
+            // a slot's pairs split into contiguous shares, one per lane
+            lane_s.assign(lanes, 255);
+            lane_beg.assign(lanes, 0);
+            lane_len.assign(lanes, 0);
+            for (int sl = 0, t = 0; sl < ns; ++sl) {
+                const int c = (int)bucket[sl].size(), m = nlane[sl];
+                for (int u = 0, st = 0; u < m; ++u, ++t) {
+                    const int len = c / m + (u < c % m ? 1 : 0);
+                    lane_s[t] = sl;
+                    lane_beg[t] = st;
+                    lane_len[t] = len;
+                    st += len;
+                }
+            }
+            int rows[4] = {0, 0, 0, 0};
+            for (int wv = 0; wv < lanes / kWave; ++wv)
+                for (int l = 0; l < kWave; ++l) rows[wv] = std::max(rows[wv], lane_len[wv * kWave + l]);
+            if (S.pairs.size() + (size_t)kWave * (rows[0] + rows[1] + rows[2] + rows[3]) > (size_t)0x7fffffff)
+                return false;
+            S.rnd_pair0.push_back((int)S.pairs.size());
+            S.rnd_rows.push_back(int4{rows[0], rows[1], rows[2], rows[3]});
+            for (int t = 0; t < lanes; ++t) S.lane_slot.push_back((uint8_t)lane_s[t]);
+            for (int wv = 0; wv < lanes / kWave; ++wv)
+                for (int j = 0; j < rows[wv]; ++j)
+                    for (int l = 0; l < kWave; ++l) {
+                        const int t = wv * kWave + l;
+                        S.pairs.push_back(j < lane_len[t] ? bucket[lane_s[t]][lane_beg[t] + j] : int2{-1, 0});
+                    }
+            const int rmax = std::max(std::max(rows[0], rows[1]), std::max(rows[2], rows[3]));
+            cost += (long long)kWave * rmax + 64;
+        }
         for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = -1;
-        scost[it] = (long long)kWave * iters + 4LL * (S.sa1[it] - S.sa0[it]);
+        scost[it] = cost;
     }
     S.sorder.resize(S.sf.size());
     for (size_t i = 0; i < S.sorder.size(); ++i) S.sorder[i] = (int)i;
@@ -482,12 +546,12 @@ int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
 #define UP(dst, vec) pk.add(&(dst), (vec))
     UP(P.sitem_f, S.sf); UP(P.sitem_a0, S.sa0); UP(P.sitem_a1, S.sa1); UP(P.sitem_g, S.sg); UP(P.sitem_nslots, S.sns);
     UP(P.sitem_pofs, S.spofs); UP(P.row_sitem_ptr, S.row_sitem_ptr); UP(P.row_nchunks, S.row_nch);
-    UP(P.sitem_order, S.sorder);
-    UP(P.seg_batch, S.seg_batch); UP(P.batch_ptr, S.batch_ptr); UP(P.pairs, S.pairs);
+    UP(P.sitem_order, S.sorder); UP(P.sitem_rnd0, S.srnd0);
+    UP(P.rnd_lane_slot, S.lane_slot); UP(P.rnd_pair0, S.rnd_pair0); UP(P.rnd_rows, S.rnd_rows);
+    UP(P.rnd_group, S.rnd_group); UP(P.pairs, S.pairs);
 #undef UP
     if ((st = pk.flush(h)) != NAV_BA_OK) return st;
     if ((st = alloc(h, (void**)&h->W.spart, sizeof(double) * std::max<long long>(S.pofs, 1))) != NAV_BA_OK) return st;
-    P.schur_segs = S.segs;
     P.n_sitems = (int)S.sf.size();
     P.max_ns = S.max_ns;
     h->schur_chunk_used = S.chunk;
@@ -496,10 +560,10 @@ int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
 
 // Process-wide memory of tuned chunk sizes, per device, precision and problem shape.
 struct TuneKey {
-    int device, mixed, run, n_win, nc, np, no, nfc;
+    int device, mixed, n_win, nc, np, no, nfc;
     bool operator<(const TuneKey& o) const {
-        return std::tie(device, mixed, run, n_win, nc, np, no, nfc) <
-               std::tie(o.device, o.mixed, o.run, o.n_win, o.nc, o.np, o.no, o.nfc);
+        return std::tie(device, mixed, n_win, nc, np, no, nfc) <
+               std::tie(o.device, o.mixed, o.n_win, o.nc, o.np, o.no, o.nfc);
     }
 };
 std::mutex g_tune_mu;
@@ -560,7 +624,7 @@ int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems
     SchurItems C;
     for (int chunk : kTuneChunks) {
         if (chunk == def) continue;
-        if (!build_schur_items(G, chunk, h->schur_run, C)) continue;
+        if (!build_schur_items(G, chunk, C)) continue;
         h->nbuf = mark;
         if ((st = upload_schur_items(h, C)) != NAV_BA_OK) return st;
         float ms;
@@ -573,73 +637,74 @@ int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems
     return NAV_BA_OK;
 }
 
-// Build the device structure of n_win windows laid end to end (see ba_kernels.h).
-int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, const nav_ba_options* o, bool batched) {
-    if (!h) return NAV_BA_BAD_ARG;
-    h->err.clear();
-    h->prepared = false;
-    h->ran = false;
-    h->schur_chunk_used = 0;
-    if (!o || !probs || n_win <= 0) return fail(h, NAV_BA_BAD_ARG, "null options or problems");
+// Check the arguments of a prepare; NAV_BA_OK or an error with its message.
+int check_args(const nav_ba_problem* probs, int n_win, const nav_ba_options* o, bool batched, std::string& err) {
+    if (!o || !probs || n_win <= 0) { err = "null options or problems"; return NAV_BA_BAD_ARG; }
+    if (!batched && n_win != 1) { err = "single-problem prepare takes one problem"; return NAV_BA_BAD_ARG; }
     for (int w = 0; w < n_win; ++w)
-        if (validate(&probs[w]) != NAV_BA_OK)
-            return fail(h, NAV_BA_BAD_ARG, "invalid problem " + std::to_string(w) + " (null array or index out of range)");
-    if (o->precision != NAV_BA_FP64 && o->precision != NAV_BA_MIXED) return fail(h, NAV_BA_BAD_ARG, "unknown precision");
-    if (o->n_iter < 0) return fail(h, NAV_BA_BAD_ARG, "negative iteration count");
+        if (validate(&probs[w]) != NAV_BA_OK) {
+            err = "invalid problem " + std::to_string(w) + " (null array or index out of range)";
+            return NAV_BA_BAD_ARG;
+        }
+    if (o->precision != NAV_BA_FP64 && o->precision != NAV_BA_MIXED) { err = "unknown precision"; return NAV_BA_BAD_ARG; }
+    if (o->n_iter < 0) { err = "negative iteration count"; return NAV_BA_BAD_ARG; }
     for (int w = 1; w < n_win; ++w)
-        if (probs[w].fx != probs[0].fx || probs[w].fy != probs[0].fy || probs[w].cx != probs[0].cx || probs[w].cy != probs[0].cy)
-            return fail(h, NAV_BA_BAD_ARG, "batched windows must share the camera intrinsics");
-    HIPTRY(h, hipSetDevice(h->device));
-    HIPTRY(h, hipStreamSynchronize(h->stream));
-    if (h->step) { (void)hipGraphExecDestroy(h->step); h->step = nullptr; }
-    if (h->step0) { (void)hipGraphExecDestroy(h->step0); h->step0 = nullptr; }
-    h->opt = *o;
-    if (h->opt.max_trials <= 0) h->opt.max_trials = 10;
-    h->batched = batched;
+        if (probs[w].fx != probs[0].fx || probs[w].fy != probs[0].fy || probs[w].cx != probs[0].cx || probs[w].cy != probs[0].cy) {
+            err = "batched windows must share the camera intrinsics";
+            return NAV_BA_BAD_ARG;
+        }
+    return NAV_BA_OK;
+}
+
+// The graph build of BE_GraphOptim.cpp:79-211 as flat arrays (arguments
+// already checked).  NAV_BA_OK or NAV_BA_TOO_LARGE with its message.
+int build_layout(const nav_ba_problem* probs, int n_win, const nav_ba_options* o, bool batched, HostLayout& L,
+                 std::string& err) {
     const bool compat = o->compat_fp32_io != 0;
+    L = HostLayout{};
+    L.n_win = n_win;
+    L.batched = batched;
 
     // ---- concatenate the windows; keep only points with observations
-    h->win.assign(n_win, WinHost());
-    std::vector<int> cam_off(n_win + 1, 0), obs_off(n_win + 1, 0);
+    L.win.assign(n_win, WinHost());
+    L.cam_off.assign(n_win + 1, 0);
+    L.obs_off.assign(n_win + 1, 0);
     for (int w = 0; w < n_win; ++w) {
-        cam_off[w + 1] = cam_off[w] + probs[w].n_cam;
-        obs_off[w + 1] = obs_off[w] + probs[w].n_obs;
+        L.cam_off[w + 1] = L.cam_off[w] + probs[w].n_cam;
+        L.obs_off[w + 1] = L.obs_off[w] + probs[w].n_obs;
     }
-    const int nc = cam_off[n_win];
-    const int no = obs_off[n_win];
-    std::vector<int> g_cam(no), g_pt(no), pt_first(n_win + 1, 0);
-    std::vector<int> pt_win_all;  // local point -> window
-    std::vector<int> cam_win(nc);
+    const int nc = L.nc = L.cam_off[n_win];
+    const int no = L.no = L.obs_off[n_win];
+    std::vector<int> g_cam(no), g_pt(no);
+    L.pt_first.assign(n_win + 1, 0);
+    L.cam_win.resize(nc);
     int np = 0;
     bool any = false;
     for (int w = 0; w < n_win; ++w) {
         const nav_ba_problem& p = probs[w];
-        WinHost& wh = h->win[w];
+        WinHost& wh = L.win[w];
         wh.n_cam = p.n_cam; wh.n_pt = p.n_pt; wh.n_obs = p.n_obs;
-        wh.cam0 = cam_off[w]; wh.obs0 = obs_off[w];
+        wh.cam0 = L.cam_off[w]; wh.obs0 = L.obs_off[w];
         wh.T_in.assign(p.T_cw, p.T_cw + 12 * (size_t)p.n_cam);
         wh.X_in.assign(p.X, p.X + 3 * (size_t)p.n_pt);
-        for (int c = 0; c < p.n_cam; ++c) cam_win[cam_off[w] + c] = w;
+        for (int c = 0; c < p.n_cam; ++c) L.cam_win[L.cam_off[w] + c] = w;
         std::vector<int> cnt(p.n_pt, 0), loc(p.n_pt, -1);
         for (int i = 0; i < p.n_obs; ++i) cnt[p.obs_pt[i]]++;
         wh.pt_local0 = np;
         for (int i = 0; i < p.n_pt; ++i)
-            if (cnt[i] > 0) { loc[i] = np++; wh.pt_orig.push_back(i); pt_win_all.push_back(w); }
+            if (cnt[i] > 0) { loc[i] = np++; wh.pt_orig.push_back(i); L.pt_win.push_back(w); }
         wh.pt_local1 = np;
         wh.empty = (p.n_cam == 0 || wh.pt_orig.empty());
         any = any || !wh.empty;
         for (int i = 0; i < p.n_obs; ++i) {
-            g_cam[obs_off[w] + i] = cam_off[w] + p.obs_cam[i];
-            g_pt[obs_off[w] + i] = loc[p.obs_pt[i]];
+            g_cam[L.obs_off[w] + i] = L.cam_off[w] + p.obs_cam[i];
+            g_pt[L.obs_off[w] + i] = loc[p.obs_pt[i]];
         }
-        pt_first[w + 1] = np;
+        L.pt_first[w + 1] = np;
     }
-    h->all_empty = !any;
-    if (!batched && n_win != 1) return fail(h, NAV_BA_BAD_ARG, "single-problem prepare takes one problem");
-    if (h->all_empty) {
-        h->prepared = true;
-        return NAV_BA_OK;
-    }
+    L.np = np;
+    L.all_empty = !any;
+    if (L.all_empty) return NAV_BA_OK;
 
     // ---- observations ordered by (point, camera, input order): two stable counting sorts
     std::vector<int> by_cam(no), ord(no);
@@ -656,128 +721,127 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
             ord[poff[g_pt[i]]++] = i;
         }
     }
-    std::vector<int> o_cam(no), o_pt(no), o_orig(no), pt_ptr(np + 1, 0);
-    std::vector<double> o_uv(2 * (size_t)no);
+    L.o_cam.resize(no); L.o_pt.resize(no); L.o_orig.resize(no); L.pt_ptr.assign(np + 1, 0);
+    L.o_uv.resize(2 * (size_t)no);
     for (int k = 0; k < no; ++k) {
         const int i = ord[k];
-        const int w = (int)(std::upper_bound(obs_off.begin(), obs_off.end(), i) - obs_off.begin()) - 1;
-        const double* uv = probs[w].obs_uv + 2 * (size_t)(i - obs_off[w]);
-        o_cam[k] = g_cam[i];
-        o_pt[k] = g_pt[i];
-        o_orig[k] = i;
-        o_uv[2 * (size_t)k] = compat ? rf32(uv[0]) : uv[0];
-        o_uv[2 * (size_t)k + 1] = compat ? rf32(uv[1]) : uv[1];
-        pt_ptr[o_pt[k] + 1]++;
+        const int w = (int)(std::upper_bound(L.obs_off.begin(), L.obs_off.end(), i) - L.obs_off.begin()) - 1;
+        const double* uv = probs[w].obs_uv + 2 * (size_t)(i - L.obs_off[w]);
+        L.o_cam[k] = g_cam[i];
+        L.o_pt[k] = g_pt[i];
+        L.o_orig[k] = i;
+        L.o_uv[2 * (size_t)k] = compat ? rf32(uv[0]) : uv[0];  // cv::Point2f measurement (Point2D.hpp:33)
+        L.o_uv[2 * (size_t)k + 1] = compat ? rf32(uv[1]) : uv[1];
+        L.pt_ptr[L.o_pt[k] + 1]++;
     }
-    for (int q = 0; q < np; ++q) pt_ptr[q + 1] += pt_ptr[q];
-    std::vector<uint8_t> pt_free(np);
+    for (int q = 0; q < np; ++q) L.pt_ptr[q + 1] += L.pt_ptr[q];
+    L.pt_free.resize(np);
     for (int q = 0; q < np; ++q) {
-        const int w = pt_win_all[q];
-        const int i = h->win[w].pt_orig[q - h->win[w].pt_local0];
-        pt_free[q] = (probs[w].pt_fixed && probs[w].pt_fixed[i]) ? 0 : 1;
+        const int w = L.pt_win[q];
+        const int i = L.win[w].pt_orig[q - L.win[w].pt_local0];
+        L.pt_free[q] = (probs[w].pt_fixed && probs[w].pt_fixed[i]) ? 0 : 1;
     }
 
     // ---- free cameras: not fixed and with at least one (therefore active) edge
     std::vector<int> cnt_cam(nc, 0);
     for (int i = 0; i < no; ++i) cnt_cam[g_cam[i]]++;
-    std::vector<int> cam_free(nc, -1), free_cam, win_fc0(n_win + 1, 0), win_cam0(n_win + 1, 0);
+    L.cam_free.assign(nc, -1);
+    L.win_fc0.assign(n_win + 1, 0);
+    L.win_cam0.assign(n_win + 1, 0);
     for (int w = 0; w < n_win; ++w) {
-        win_cam0[w] = cam_off[w];
-        win_fc0[w] = (int)free_cam.size();
-        for (int c = cam_off[w]; c < cam_off[w + 1]; ++c) {
-            const int lc = c - cam_off[w];
+        L.win_cam0[w] = L.cam_off[w];
+        L.win_fc0[w] = (int)L.free_cam.size();
+        for (int c = L.cam_off[w]; c < L.cam_off[w + 1]; ++c) {
+            const int lc = c - L.cam_off[w];
             if (!(probs[w].cam_fixed && probs[w].cam_fixed[lc]) && cnt_cam[c] > 0) {
-                cam_free[c] = (int)free_cam.size();
-                free_cam.push_back(c);
+                L.cam_free[c] = (int)L.free_cam.size();
+                L.free_cam.push_back(c);
             }
         }
     }
-    win_cam0[n_win] = nc;
-    win_fc0[n_win] = (int)free_cam.size();
-    const int nfc = (int)free_cam.size();
-    int max_win_cams = 1;
-    for (int w = 0; w < n_win; ++w) max_win_cams = std::max(max_win_cams, probs[w].n_cam);
-    if (max_win_cams > kMaxWinCams) return fail(h, NAV_BA_TOO_LARGE, "more than 8192 cameras in a window");
+    L.win_cam0[n_win] = nc;
+    L.win_fc0[n_win] = (int)L.free_cam.size();
+    const int nfc = L.nfc = (int)L.free_cam.size();
 
     // ---- camera-major observation lists of free cameras (ascending observation index)
-    std::vector<int> cam_ptr(nfc + 1, 0), cam_obs, cam_obs_b0;
-    for (int k = 0; k < no; ++k) if (cam_free[o_cam[k]] >= 0) cam_ptr[cam_free[o_cam[k]] + 1]++;
-    for (int f = 0; f < nfc; ++f) cam_ptr[f + 1] += cam_ptr[f];
-    cam_obs.resize(cam_ptr[nfc]);
+    L.cam_ptr.assign(nfc + 1, 0);
+    for (int k = 0; k < no; ++k) if (L.cam_free[L.o_cam[k]] >= 0) L.cam_ptr[L.cam_free[L.o_cam[k]] + 1]++;
+    for (int f = 0; f < nfc; ++f) L.cam_ptr[f + 1] += L.cam_ptr[f];
+    L.cam_obs.resize(L.cam_ptr[nfc]);
     {
-        std::vector<int> pos(cam_ptr.begin(), cam_ptr.end() - 1);
+        std::vector<int> pos(L.cam_ptr.begin(), L.cam_ptr.end() - 1);
         for (int k = 0; k < no; ++k) {
-            const int f = cam_free[o_cam[k]];
-            if (f >= 0) cam_obs[pos[f]++] = k;
+            const int f = L.cam_free[L.o_cam[k]];
+            if (f >= 0) L.cam_obs[pos[f]++] = k;
         }
     }
-    std::vector<int> c_pt(cam_obs.size());
-    std::vector<double> c_uv(2 * cam_obs.size());
-    for (size_t k = 0; k < cam_obs.size(); ++k) {
-        c_pt[k] = o_pt[cam_obs[k]];
-        c_uv[2 * k] = o_uv[2 * (size_t)cam_obs[k]];
-        c_uv[2 * k + 1] = o_uv[2 * (size_t)cam_obs[k] + 1];
+    L.c_pt.resize(L.cam_obs.size());
+    L.c_uv.resize(2 * L.cam_obs.size());
+    for (size_t k = 0; k < L.cam_obs.size(); ++k) {
+        L.c_pt[k] = L.o_pt[L.cam_obs[k]];
+        L.c_uv[2 * k] = L.o_uv[2 * (size_t)L.cam_obs[k]];
+        L.c_uv[2 * k + 1] = L.o_uv[2 * (size_t)L.cam_obs[k] + 1];
     }
     // first observation of the same point on the same camera (start of the
     // point's observations on cameras >= this one; duplicates are adjacent)
-    cam_obs_b0.resize(cam_obs.size());
-    for (size_t k = 0; k < cam_obs.size(); ++k) {
-        int b = cam_obs[k];
-        const int q = o_pt[b];
-        while (b > pt_ptr[q] && o_cam[b - 1] == o_cam[cam_obs[k]]) --b;
-        cam_obs_b0[k] = b;
+    L.cam_obs_b0.resize(L.cam_obs.size());
+    for (size_t k = 0; k < L.cam_obs.size(); ++k) {
+        int b = L.cam_obs[k];
+        const int q = L.o_pt[b];
+        while (b > L.pt_ptr[q] && L.o_cam[b - 1] == L.o_cam[L.cam_obs[k]]) --b;
+        L.cam_obs_b0[k] = b;
     }
 
     // ---- point-aligned linearisation tiles, never spanning two windows
-    std::vector<int> tile_obs{0}, tile_pt{0}, tile_win, win_tile0{0};
+    L.tile_obs = {0}; L.tile_pt = {0}; L.win_tile0 = {0};
     for (int w = 0; w < n_win; ++w) {
-        for (int q = pt_first[w]; q < pt_first[w + 1];) {
+        for (int q = L.pt_first[w]; q < L.pt_first[w + 1];) {
             int q1 = q;
-            while (q1 < pt_first[w + 1] && pt_ptr[q1 + 1] - pt_ptr[q] <= kTile) ++q1;
+            while (q1 < L.pt_first[w + 1] && L.pt_ptr[q1 + 1] - L.pt_ptr[q] <= kTile) ++q1;
             if (q1 == q) q1 = q + 1;  // one point with more than kTile observations
-            tile_obs.push_back(pt_ptr[q1]);
-            tile_pt.push_back(q1);
-            tile_win.push_back(w);
+            L.tile_obs.push_back(L.pt_ptr[q1]);
+            L.tile_pt.push_back(q1);
+            L.tile_win.push_back(w);
             q = q1;
         }
-        win_tile0.push_back((int)tile_win.size());
+        L.win_tile0.push_back((int)L.tile_win.size());
     }
     // ---- window-aligned cost blocks (observations) and point blocks; windows
     // are contiguous, so each block starts where the previous one ends
-    std::vector<int> cblk_o0{0}, cblk_win, win_cblk0{0};
-    std::vector<int> win_pt0(n_win + 1);
+    L.cblk_o0 = {0}; L.win_cblk0 = {0};
+    L.win_pt0.assign(n_win + 1, 0);
     for (int w = 0; w < n_win; ++w) {
-        const int o0 = pt_ptr[pt_first[w]], o1 = pt_ptr[pt_first[w + 1]];
-        for (int a = o0; a < o1; a += kCostObs) { cblk_o0.push_back(std::min(a + kCostObs, o1)); cblk_win.push_back(w); }
-        win_cblk0.push_back((int)cblk_win.size());
-        win_pt0[w] = pt_first[w];
+        const int o0 = L.pt_ptr[L.pt_first[w]], o1 = L.pt_ptr[L.pt_first[w + 1]];
+        for (int a = o0; a < o1; a += kCostObs) { L.cblk_o0.push_back(std::min(a + kCostObs, o1)); L.cblk_win.push_back(w); }
+        L.win_cblk0.push_back((int)L.cblk_win.size());
+        L.win_pt0[w] = L.pt_first[w];
     }
-    win_pt0[n_win] = np;
+    L.win_pt0[n_win] = np;
 
     // ---- Hpp work items: (free camera, chunk)
-    std::vector<int> hf, ha0, ha1, cam_hitem_ptr{0};
+    L.cam_hitem_ptr = {0};
     for (int f = 0; f < nfc; ++f) {
-        for (int a = cam_ptr[f]; a < cam_ptr[f + 1]; a += kChunk) {
-            hf.push_back(f); ha0.push_back(a); ha1.push_back(std::min(a + kChunk, cam_ptr[f + 1]));
+        for (int a = L.cam_ptr[f]; a < L.cam_ptr[f + 1]; a += kChunk) {
+            L.hf.push_back(f); L.ha0.push_back(a); L.ha1.push_back(std::min(a + kChunk, L.cam_ptr[f + 1]));
         }
-        cam_hitem_ptr.push_back((int)hf.size());
+        L.cam_hitem_ptr.push_back((int)L.hf.size());
     }
 
     // ---- co-visibility through free points, per window: column blocks g >= f of each S row
     // ... and the other blocks g > f of each row (zero in S, rewritten every trial
     // because the factorisation works in place)
-    std::vector<int> colptr{0}, colcam, zcolptr{0}, zcolcam;
+    L.colptr = {0}; L.zcolptr = {0};
     {
         std::vector<uint64_t> bits;
         std::vector<int> fl;
         for (int w = 0; w < n_win; ++w) {
-            const int f0 = win_fc0[w], nw = win_fc0[w + 1] - f0;
+            const int f0 = L.win_fc0[w], nw = L.win_fc0[w + 1] - f0;
             bits.assign(((size_t)nw * nw + 63) / 64, 0);
-            for (int q = pt_first[w]; q < pt_first[w + 1]; ++q) {
-                if (!pt_free[q]) continue;
+            for (int q = L.pt_first[w]; q < L.pt_first[w + 1]; ++q) {
+                if (!L.pt_free[q]) continue;
                 fl.clear();
-                for (int k = pt_ptr[q]; k < pt_ptr[q + 1]; ++k) {
-                    const int f = cam_free[o_cam[k]];
+                for (int k = L.pt_ptr[q]; k < L.pt_ptr[q + 1]; ++k) {
+                    const int f = L.cam_free[L.o_cam[k]];
                     if (f >= 0 && (fl.empty() || fl.back() != f - f0)) fl.push_back(f - f0);
                 }
                 for (size_t a = 0; a < fl.size(); ++a)
@@ -787,67 +851,179 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
                     }
             }
             for (int lf = 0; lf < nw; ++lf) {
-                colcam.push_back(f0 + lf);
+                L.colcam.push_back(f0 + lf);
                 for (int lg = lf + 1; lg < nw; ++lg) {
                     const size_t bit = (size_t)lf * nw + lg;
-                    if (bits[bit >> 6] >> (bit & 63) & 1ull) colcam.push_back(f0 + lg);
-                    else zcolcam.push_back(f0 + lg);
+                    if (bits[bit >> 6] >> (bit & 63) & 1ull) L.colcam.push_back(f0 + lg);
+                    else L.zcolcam.push_back(f0 + lg);
                 }
-                colptr.push_back((int)colcam.size());
-                zcolptr.push_back((int)zcolcam.size());
+                L.colptr.push_back((int)L.colcam.size());
+                L.zcolptr.push_back((int)L.zcolcam.size());
             }
         }
     }
 
-    // ---- Schur work items and pair lists at the default chunk size (tuned below)
-    const SchurGraph G{nfc, nc, colptr.data(), colcam.data(), cam_ptr.data(), cam_obs.data(), cam_obs_b0.data(),
-                       o_pt.data(), o_cam.data(), pt_ptr.data(), pt_free.data(), free_cam.data(), cam_win.data(),
-                       cam_off.data()};
-    const TuneKey key{h->device, mixed(h) ? 1 : 0, h->schur_run, n_win, nc, np, no, nfc};
-    int chunk0 = h->schur_chunk > 0 ? h->schur_chunk : kSchurChunk;
-    bool tune = h->schur_chunk == 0 && nfc > 0 && no >= kTuneMinObs;
-    if (tune && cached_chunk(key, &chunk0)) tune = false;  // this shape was tuned before on this device
-    SchurItems SI;
-    if (!build_schur_items(G, chunk0, h->schur_run, SI)) return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pairs");
-
     // ---- dense reduced systems: one padded tiled system, or one small system per window
-    std::vector<long long> win_S(n_win, 0);
-    std::vector<int> win_lda(n_win, 1);
-    long long S_size = 1;
-    const int n = 6 * nfc;
-    int n_pad = ((n + kNB - 1) / kNB) * kNB;
+    L.win_S.assign(n_win, 0);
+    L.win_lda.assign(n_win, 1);
+    L.n = 6 * nfc;
+    L.n_pad = ((L.n + kNB - 1) / kNB) * kNB;
     if (!batched) {
-        if (n > kMaxDenseN) return fail(h, NAV_BA_TOO_LARGE, "reduced camera system larger than 16384");
-        win_lda[0] = std::max(n_pad, 1);
-        S_size = std::max<long long>((long long)n_pad * n_pad, 1);
+        if (L.n > kMaxDenseN) { err = "reduced camera system larger than 16384"; return NAV_BA_TOO_LARGE; }
+        L.win_lda[0] = std::max(L.n_pad, 1);
+        L.S_size = std::max<long long>((long long)L.n_pad * L.n_pad, 1);
     } else {
-        S_size = 0;
+        L.S_size = 0;
         for (int w = 0; w < n_win; ++w) {
-            const int nw = 6 * (win_fc0[w + 1] - win_fc0[w]);
-            if (nw > kSmallMaxN) return fail(h, NAV_BA_TOO_LARGE, "a batched window has more than 30 free cameras");
-            win_S[w] = S_size;
-            win_lda[w] = std::max(nw, 1);
-            S_size += (long long)win_lda[w] * win_lda[w];
+            const int nw = 6 * (L.win_fc0[w + 1] - L.win_fc0[w]);
+            if (nw > kSmallMaxN) { err = "a batched window has more than 30 free cameras"; return NAV_BA_TOO_LARGE; }
+            L.win_S[w] = L.S_size;
+            L.win_lda[w] = std::max(nw, 1);
+            L.S_size += (long long)L.win_lda[w] * L.win_lda[w];
         }
-        n_pad = 0;
+        L.n_pad = 0;
     }
 
-    // ---- initial state
-    std::vector<PoseQT> pose0(nc);
+    // ---- initial state: SE3Quat(R, t) per frame (BE_GraphOptim.cpp:92-93), points
+    // rounded to float in compat mode (:147)
+    L.pose0.resize(nc);
     for (int w = 0; w < n_win; ++w)
         for (int c = 0; c < probs[w].n_cam; ++c) {
             const double* T = probs[w].T_cw + 12 * (size_t)c;
-            PoseQT& P0 = pose0[cam_off[w] + c];
+            PoseQT& P0 = L.pose0[L.cam_off[w] + c];
             host_quat_from_R(T, P0.q);
             P0.t[0] = T[9]; P0.t[1] = T[10]; P0.t[2] = T[11];
             P0.pad = 0;
         }
-    std::vector<D4> X0(np);
+    L.X0.resize(np);
     for (int q = 0; q < np; ++q) {
-        const int w = pt_win_all[q];
-        const double* x = probs[w].X + 3 * (size_t)h->win[w].pt_orig[q - h->win[w].pt_local0];
-        X0[q] = compat ? D4{rf32(x[0]), rf32(x[1]), rf32(x[2]), 0.0} : D4{x[0], x[1], x[2], 0.0};
+        const int w = L.pt_win[q];
+        const double* x = probs[w].X + 3 * (size_t)L.win[w].pt_orig[q - L.win[w].pt_local0];
+        L.X0[q] = compat ? D4{rf32(x[0]), rf32(x[1]), rf32(x[2]), 0.0} : D4{x[0], x[1], x[2], 0.0};
     }
+    // intrinsics are float in the reference's calibration (GeometricCamera.h:117)
+    L.fx = compat ? rf32(probs[0].fx) : probs[0].fx; L.fy = compat ? rf32(probs[0].fy) : probs[0].fy;
+    L.cx = compat ? rf32(probs[0].cx) : probs[0].cx; L.cy = compat ? rf32(probs[0].cy) : probs[0].cy;
+    return NAV_BA_OK;
+}
+
+// Every index of the layout and of the Schur items against the buffers they
+// address, and the items' pairs against a direct enumeration of each row's
+// pairs; "" if consistent, else the first violation.
+std::string check_layout(const HostLayout& L, const SchurItems& S) {
+    auto bad = [](const std::string& m) { return m; };
+    const int no = L.no, np = L.np, nc = L.nc, nfc = L.nfc;
+    for (int k = 0; k < no; ++k) {
+        if (L.o_cam[k] < 0 || L.o_cam[k] >= nc || L.o_pt[k] < 0 || L.o_pt[k] >= np) return bad("observation index");
+        if (k < L.pt_ptr[L.o_pt[k]] || k >= L.pt_ptr[L.o_pt[k] + 1]) return bad("pt_ptr range");
+        if (k > 0 && L.o_pt[k] == L.o_pt[k - 1] && L.o_cam[k] < L.o_cam[k - 1]) return bad("camera order in a point");
+    }
+    for (size_t k = 0; k < L.cam_obs.size(); ++k) {
+        const int a = L.cam_obs[k];
+        if (a < 0 || a >= no || L.c_pt[k] != L.o_pt[a]) return bad("camera-major list");
+        const int b0 = L.cam_obs_b0[k];
+        if (b0 > a || b0 < L.pt_ptr[L.o_pt[a]] || L.o_cam[b0] != L.o_cam[a]) return bad("cam_obs_b0");
+    }
+    const int lanes = kBlock;
+    if ((int)S.sf.size() != (int)S.srnd0.size()) return bad("item round table");
+    const size_t nrnd = S.rnd_rows.size();
+    if (S.lane_slot.size() != nrnd * lanes || S.rnd_group.size() != nrnd || S.rnd_pair0.size() != nrnd)
+        return bad("round tables");
+    std::vector<int> slotmap(nc, -1);
+    std::vector<long long> seen;
+    for (size_t it = 0; it < S.sf.size(); ++it) {
+        const int f = S.sf[it], ns = S.sns[it], a0 = S.sa0[it], a1 = S.sa1[it];
+        if (f < 0 || f >= nfc || ns < 1 || ns > kMaxSlots || a0 < L.cam_ptr[f] || a1 > L.cam_ptr[f + 1] || a0 > a1)
+            return bad("item range");
+        const int col0 = L.colptr[f] + S.sg[it] * kMaxSlots;
+        if (col0 + ns > L.colptr[f + 1]) return bad("item slots");
+        for (int sl = 0; sl < ns; ++sl) slotmap[L.free_cam[L.colcam[col0 + sl]]] = sl;
+        const int nr = (a1 - a0 + lanes - 1) / lanes;
+        if (S.srnd0[it] + nr > (int)nrnd) return bad("item rounds");
+        for (int r = 0; r < nr; ++r) {
+            const int rnd = S.srnd0[it] + r, kb = a0 + r * lanes;
+            const int grp = S.rnd_group[rnd];
+            if (grp != 2 && grp != 4) return bad("lane group");
+            // expected pairs of the round, as (b, row) keys
+            std::vector<long long> want;
+            for (int k = kb; k < std::min(kb + lanes, a1); ++k) {
+                const int a = L.cam_obs[k], q = L.o_pt[a];
+                if (!L.pt_free[q]) continue;
+                for (int b = L.cam_obs_b0[k]; b < L.pt_ptr[q + 1]; ++b)
+                    if (b != a && slotmap[L.o_cam[b]] >= 0) want.push_back((long long)b * 512 + (k - kb));
+            }
+            // the lanes' slots: contiguous per slot, aligned whole groups
+            const uint8_t* ls = &S.lane_slot[(size_t)rnd * lanes];
+            for (int t = 0; t < lanes; ++t) {
+                if (ls[t] != 255 && ls[t] >= ns) return bad("lane slot");
+                if (t > 0 && ls[t] != ls[t - 1]) {
+                    if (t % grp != 0) return bad("lane group alignment");
+                    if (ls[t - 1] != 255 && ls[t] != 255 && ls[t] < ls[t - 1]) return bad("lane slot order");
+                    for (int u = 0; u < t - 1; ++u)
+                        if (ls[u] == ls[t] && ls[t] != 255) return bad("slot lanes not contiguous");
+                }
+            }
+            const int4 rw = S.rnd_rows[rnd];
+            const int rowsv[4] = {rw.x, rw.y, rw.z, rw.w};
+            seen.clear();
+            size_t e = (size_t)S.rnd_pair0[rnd];
+            for (int wv = 0; wv < lanes / kWave; ++wv)
+                for (int j = 0; j < rowsv[wv]; ++j)
+                    for (int l = 0; l < kWave; ++l, ++e) {
+                        if (e >= S.pairs.size()) return bad("pair list end");
+                        const int2 pe = S.pairs[e];
+                        if (pe.x < 0) continue;
+                        const int t = wv * kWave + l;
+                        if (ls[t] == 255) return bad("pair on an idle lane");
+                        if (pe.x >= no || pe.y < 0 || pe.y >= lanes || kb + pe.y >= a1) return bad("pair index");
+                        if (slotmap[L.o_cam[pe.x]] != ls[t]) return bad("pair camera is not the lane's slot");
+                        if (L.o_pt[pe.x] != L.o_pt[L.cam_obs[kb + pe.y]]) return bad("pair point");
+                        seen.push_back((long long)pe.x * 512 + pe.y);
+                    }
+            std::sort(want.begin(), want.end());
+            std::sort(seen.begin(), seen.end());
+            if (want != seen) return bad("round pairs differ from the direct enumeration");
+        }
+        for (int sl = 0; sl < ns; ++sl) slotmap[L.free_cam[L.colcam[col0 + sl]]] = -1;
+    }
+    return "";
+}
+
+// Build the device structure of n_win windows laid end to end (see ba_kernels.h).
+int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, const nav_ba_options* o, bool batched) {
+    if (!h) return NAV_BA_BAD_ARG;
+    h->err.clear();
+    h->prepared = false;
+    h->ran = false;
+    h->schur_chunk_used = 0;
+    std::string msg;
+    int st = check_args(probs, n_win, o, batched, msg);
+    if (st != NAV_BA_OK) return fail(h, st, msg);
+    HIPTRY(h, hipSetDevice(h->device));
+    HIPTRY(h, hipStreamSynchronize(h->stream));
+    if (h->step) { (void)hipGraphExecDestroy(h->step); h->step = nullptr; }
+    if (h->step0) { (void)hipGraphExecDestroy(h->step0); h->step0 = nullptr; }
+    h->opt = *o;
+    if (h->opt.max_trials <= 0) h->opt.max_trials = 10;
+    h->batched = batched;
+    HostLayout& L = h->layout;
+    if ((st = build_layout(probs, n_win, o, batched, L, msg)) != NAV_BA_OK) return fail(h, st, msg);
+    h->all_empty = L.all_empty;
+    if (L.all_empty) {
+        h->prepared = true;
+        return NAV_BA_OK;
+    }
+    const int nc = L.nc, np = L.np, no = L.no, nfc = L.nfc;
+
+    // ---- Schur work items and pair lists at the default chunk size (tuned below)
+    const SchurGraph G = L.graph();
+    const TuneKey key{h->device, mixed(h) ? 1 : 0, n_win, nc, np, no, nfc};
+    int chunk0 = h->schur_chunk > 0 ? h->schur_chunk : kSchurChunk;
+    bool tune = h->schur_chunk == 0 && nfc > 0 && no >= kTuneMinObs;
+    if (tune && cached_chunk(key, &chunk0)) tune = false;  // this shape was tuned before on this device
+    SchurItems SI;
+    if (!build_schur_items(G, chunk0, SI))
+        return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pair entries");
 
     // ---- device upload
     h->nbuf = 0;
@@ -856,44 +1032,41 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     P.n_cam = nc; P.n_pt = np; P.n_obs = no; P.n_fc = nfc;
     P.robust = o->robust ? 1 : 0;
     P.delta = o->huber_delta;
-    P.fx = compat ? rf32(probs[0].fx) : probs[0].fx; P.fy = compat ? rf32(probs[0].fy) : probs[0].fy;
-    P.cx = compat ? rf32(probs[0].cx) : probs[0].cx; P.cy = compat ? rf32(probs[0].cy) : probs[0].cy;
+    P.fx = L.fx; P.fy = L.fy; P.cx = L.cx; P.cy = L.cy;
     P.n_iter = o->n_iter;
     P.max_trials = h->opt.max_trials;
     P.auto_lambda = !(o->lambda_init > 0);
     P.lambda0 = o->lambda_init;
-    int st;
     Packer pk(&h->stage, &h->stage_cap);  // free again: every prepare ends with a stream synchronisation
 #define UP(dst, vec) pk.add(&(dst), (vec))
-    UP(P.o_cam, o_cam); UP(P.o_pt, o_pt); UP(P.o_uv, o_uv); UP(P.o_orig, o_orig);
-    UP(P.pt_ptr, pt_ptr); UP(P.pt_free, pt_free); UP(P.cam_free, cam_free); UP(P.free_cam, free_cam);
-    UP(P.cam_ptr, cam_ptr); UP(P.cam_obs, cam_obs); UP(P.cam_obs_b0, cam_obs_b0);
-    UP(P.c_pt, c_pt); UP(P.c_uv, c_uv);
-    UP(P.tile_obs, tile_obs); UP(P.tile_pt, tile_pt); UP(P.tile_win, tile_win);
-    UP(P.hitem_f, hf); UP(P.hitem_a0, ha0); UP(P.hitem_a1, ha1); UP(P.cam_hitem_ptr, cam_hitem_ptr);
-    UP(P.colptr, colptr); UP(P.colcam, colcam); UP(P.zcolptr, zcolptr); UP(P.zcolcam, zcolcam);
-    UP(P.cam_win, cam_win); UP(P.pt_win, pt_win_all); UP(P.win_cam0, win_cam0); UP(P.win_fc0, win_fc0);
-    UP(P.win_pt0, win_pt0); UP(P.win_S, win_S); UP(P.win_lda, win_lda);
-    UP(P.cblk_o0, cblk_o0); UP(P.cblk_win, cblk_win); UP(P.win_cblk0, win_cblk0);
-    UP(P.win_tile0, win_tile0);
+    UP(P.o_cam, L.o_cam); UP(P.o_pt, L.o_pt); UP(P.o_uv, L.o_uv); UP(P.o_orig, L.o_orig);
+    UP(P.pt_ptr, L.pt_ptr); UP(P.pt_free, L.pt_free); UP(P.cam_free, L.cam_free); UP(P.free_cam, L.free_cam);
+    UP(P.cam_ptr, L.cam_ptr); UP(P.cam_obs, L.cam_obs); UP(P.cam_obs_b0, L.cam_obs_b0);
+    UP(P.c_pt, L.c_pt); UP(P.c_uv, L.c_uv);
+    UP(P.tile_obs, L.tile_obs); UP(P.tile_pt, L.tile_pt); UP(P.tile_win, L.tile_win);
+    UP(P.hitem_f, L.hf); UP(P.hitem_a0, L.ha0); UP(P.hitem_a1, L.ha1); UP(P.cam_hitem_ptr, L.cam_hitem_ptr);
+    UP(P.colptr, L.colptr); UP(P.colcam, L.colcam); UP(P.zcolptr, L.zcolptr); UP(P.zcolcam, L.zcolcam);
+    UP(P.cam_win, L.cam_win); UP(P.pt_win, L.pt_win); UP(P.win_cam0, L.win_cam0); UP(P.win_fc0, L.win_fc0);
+    UP(P.win_pt0, L.win_pt0); UP(P.win_S, L.win_S); UP(P.win_lda, L.win_lda);
+    UP(P.cblk_o0, L.cblk_o0); UP(P.cblk_win, L.cblk_win); UP(P.win_cblk0, L.win_cblk0);
+    UP(P.win_tile0, L.win_tile0);
 #undef UP
     if ((st = pk.flush(h)) != NAV_BA_OK) return st;
-    P.n_tiles = (int)tile_win.size();
-    P.n_hitems = (int)hf.size();
+    P.n_tiles = (int)L.tile_win.size();
+    P.n_hitems = (int)L.hf.size();
     P.n_win = n_win;
-    P.max_win_cams = max_win_cams;
     P.max_win_fc = 0;
-    for (int w = 0; w < n_win; ++w) P.max_win_fc = std::max(P.max_win_fc, win_fc0[w + 1] - win_fc0[w]);
+    for (int w = 0; w < n_win; ++w) P.max_win_fc = std::max(P.max_win_fc, L.win_fc0[w + 1] - L.win_fc0[w]);
     P.batched = batched ? 1 : 0;
-    P.n_cblk = (int)cblk_win.size();
-    P.n = n;
-    P.n_pad = n_pad;
-    P.nt = n_pad / kNB;
+    P.n_cblk = (int)L.cblk_win.size();
+    P.n = L.n;
+    P.n_pad = L.n_pad;
+    P.nt = L.n_pad / kNB;
     P.use_blocks = (!batched && nfc > 0 && nfc <= kBlocksMaxFc &&
                     (h->solver_mode == 2 || (h->solver_mode == 0 && nfc <= kBlocksAutoFc))) ? 1 : 0;
 
     const size_t tsz = mixed(h) ? sizeof(float) : sizeof(double);
-    const size_t nvec = batched ? std::max<size_t>(6 * (size_t)nfc, 1) : std::max<size_t>(n_pad, 1);
+    const size_t nvec = batched ? std::max<size_t>(6 * (size_t)nfc, 1) : std::max<size_t>(L.n_pad, 1);
     auto A = [&](void** dst, size_t bytes) { return alloc(h, dst, bytes); };
 #define AL(dst, bytes) if ((st = A((void**)&(dst), (bytes))) != NAV_BA_OK) return st
     AL(h->init.pose, sizeof(PoseQT) * nc); AL(h->init.X, sizeof(D4) * np); h->init.cam = nullptr;
@@ -902,13 +1075,13 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     DevWork& W = h->W;
     AL(W.w, tsz * no);
     AL(W.Hl, sizeof(double) * 9 * (size_t)np);
-    AL(W.hpart, sizeof(double) * 27 * std::max<size_t>(hf.size(), 1));
+    AL(W.hpart, sizeof(double) * 27 * std::max<size_t>(L.hf.size(), 1));
     AL(W.Hc, sizeof(double) * 28 * std::max(nfc, 1));
-    AL(W.S, sizeof(double) * (size_t)std::max<long long>(S_size, 1));
+    AL(W.S, sizeof(double) * (size_t)std::max<long long>(L.S_size, 1));
     AL(W.g, sizeof(double) * nvec);
     AL(W.y, sizeof(double) * nvec);
     AL(W.dp, sizeof(double) * nvec);
-    AL(W.Lm, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(S_size, 1)));
+    AL(W.Lm, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(L.S_size, 1)));
     AL(W.Li, sizeof(double) * 4096 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1)));
     AL(W.fpart, sizeof(double) * 64 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1) * std::max(P.nt, 1)));
     AL(W.red_cost, sizeof(double) * std::max(P.n_cblk, 1));
@@ -922,8 +1095,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
 #undef AL
     const int schur_mark = h->nbuf;
     if ((st = upload_schur_items(h, SI)) != NAV_BA_OK) return st;
-    HIPTRY(h, hipMemcpyAsync(h->init.pose, pose0.data(), sizeof(PoseQT) * nc, hipMemcpyHostToDevice, h->stream));
-    HIPTRY(h, hipMemcpyAsync(h->init.X, X0.data(), sizeof(D4) * np, hipMemcpyHostToDevice, h->stream));
+    HIPTRY(h, hipMemcpyAsync(h->init.pose, L.pose0.data(), sizeof(PoseQT) * nc, hipMemcpyHostToDevice, h->stream));
+    HIPTRY(h, hipMemcpyAsync(h->init.X, L.X0.data(), sizeof(D4) * np, hipMemcpyHostToDevice, h->stream));
     HIPTRY(h, hipStreamSynchronize(h->stream));
     // Schur rows and the per-window factorisation may use more than 64 KiB of LDS
     if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
@@ -942,7 +1115,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
 
 int fetch_window(nav_ba_handle* h, int w, const std::vector<PoseQT>& pose, const std::vector<D4>& X,
                  const std::vector<uint8_t>& flags, nav_ba_result* r) {
-    const WinHost& wh = h->win[w];
+    const WinHost& wh = h->layout.win[w];
     r->solve_ms = h->solve_ms;
     if (wh.empty || h->all_empty) {
         r->iterations = 0; r->trials = 0; r->stop_reason = NAV_BA_STOP_EMPTY; r->n_outliers = 0;
@@ -987,7 +1160,7 @@ int fetch_window(nav_ba_handle* h, int w, const std::vector<PoseQT>& pose, const
 int fetch_all(nav_ba_handle* h, nav_ba_result* results, int n) {
     if (!h || !results) return NAV_BA_BAD_ARG;
     if (!h->prepared || !h->ran) return fail(h, NAV_BA_NOT_PREPARED, "fetch before nav_ba_run");
-    if (n != (int)h->win.size()) return fail(h, NAV_BA_BAD_ARG, "result count differs from the prepared window count");
+    if (n != (int)h->layout.win.size()) return fail(h, NAV_BA_BAD_ARG, "result count differs from the prepared window count");
     std::vector<PoseQT> pose;
     std::vector<D4> X;
     std::vector<uint8_t> flags;
@@ -1184,15 +1357,41 @@ int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk) {
     return NAV_BA_OK;
 }
 
-int32_t nav_ba_debug_set_schur_run(nav_ba_handle* h, int32_t run) {
-    if (!h || run < 0 || run > 64) return NAV_BA_BAD_ARG;
-    h->schur_run = run == 0 ? kRunLen : run;
-    return NAV_BA_OK;
-}
 
 int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk) {
     if (!h || !chunk) return NAV_BA_BAD_ARG;
     *chunk = h->prepared ? h->schur_chunk_used : 0;
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_check_layout(const nav_ba_problem* problems, int32_t n_windows, const nav_ba_options* o,
+                                  int32_t batched, int32_t chunk, int64_t* n_pairs, int32_t* n_rounds,
+                                  char* msg, int32_t msg_len) {
+    auto say = [&](const std::string& m) {
+        if (msg && msg_len > 0) {
+            std::strncpy(msg, m.c_str(), (size_t)msg_len - 1);
+            msg[msg_len - 1] = 0;
+        }
+    };
+    say("");
+    if (chunk < 0 || chunk > 4096 || chunk % kWave != 0) { say("chunk"); return NAV_BA_BAD_ARG; }
+    std::string err;
+    int st = check_args(problems, n_windows, o, batched != 0, err);
+    if (st != NAV_BA_OK) { say(err); return st; }
+    HostLayout L;
+    if ((st = build_layout(problems, n_windows, o, batched != 0, L, err)) != NAV_BA_OK) { say(err); return st; }
+    if (L.all_empty) { say("empty"); return NAV_BA_EMPTY; }
+    SchurItems S;
+    if (!build_schur_items(L.graph(), chunk > 0 ? chunk : kSchurChunk, S)) {
+        say("more than 2^31 Schur pair entries");
+        return NAV_BA_TOO_LARGE;
+    }
+    long long np = 0;
+    for (const int2& e : S.pairs) np += e.x >= 0 ? 1 : 0;
+    if (n_pairs) *n_pairs = np;
+    if (n_rounds) *n_rounds = (int32_t)S.rnd_rows.size();
+    const std::string bad = check_layout(L, S);
+    if (!bad.empty()) { say(bad); return -1; }
     return NAV_BA_OK;
 }
 

@@ -10,10 +10,12 @@ tensors (HIP device memory, torch's current stream).  `BACost` is the
 with xi [n_cam, 6] the pose tangents at T_cw (left perturbation exp(xi) T_cw,
 xi = (omega, upsilon), the update of the reference edge's pose vertex,
 OptimizableTypes.hpp:221-227), X [n_pt, 3] the points and intr [8] =
-(fx, fy, cx, cy, k1, k2, p1, p2).  The pose gradient is the left-tangent
-gradient at exp(xi) T_cw, which is d cost / d xi exactly at xi = 0 (the usual
-use: the gradient at the current estimate).  There is no CPU fallback: without
-the built library or a gfx950 device the calls raise.
+(fx, fy, cx, cy, k1, k2, p1, p2).  The device kernel returns the
+left-tangent gradient g at T = exp(xi) T_cw (the gradient for a perturbation
+exp(d) T); backward maps it to d cost / d xi = J_l(xi)^T g, J_l the SE3 left
+Jacobian, by the chain rule through `se3_left`, so the two agree only at
+xi = 0 and every xi gets its true gradient.  There is no CPU fallback:
+without the built library or a gfx950 device the calls raise.
 """
 from __future__ import annotations
 
@@ -124,15 +126,19 @@ class CostBatch:
 
 
 def se3_left(xi, T_cw):
-    """exp(xi) T_cw for every pose row [R (9) | t (3)], xi = (omega, upsilon), in torch."""
+    """exp(xi) T_cw for every pose row [R (9) | t (3)], xi = (omega, upsilon), in torch.
+    Below theta^2 = 1e-6 the Rodrigues and V coefficients are their series to
+    theta^2 (error below 1e-13), so the map and its autograd derivative are
+    finite at xi = 0."""
     import torch
     w, u = xi[:, :3], xi[:, 3:]
-    th = torch.linalg.norm(w, dim=1, keepdim=True)
-    small = th < 1e-8
-    ths = torch.where(small, torch.ones_like(th), th)
-    a = torch.where(small, torch.ones_like(th), torch.sin(ths) / ths)
-    b = torch.where(small, 0.5 * torch.ones_like(th), (1 - torch.cos(ths)) / ths ** 2)
-    c = torch.where(small, torch.ones_like(th) / 6.0, (ths - torch.sin(ths)) / ths ** 3)
+    th2 = (w * w).sum(1, keepdim=True)
+    small = th2 < 1e-6
+    th2s = torch.where(small, torch.ones_like(th2), th2)
+    ths = torch.sqrt(th2s)
+    a = torch.where(small, 1.0 - th2 / 6.0, torch.sin(ths) / ths)
+    b = torch.where(small, 0.5 - th2 / 24.0, (1 - torch.cos(ths)) / th2s)
+    c = torch.where(small, 1.0 / 6.0 - th2 / 120.0, (ths - torch.sin(ths)) / (th2s * ths))
     Z = torch.zeros_like(w[:, 0])
     K = torch.stack([torch.stack([Z, -w[:, 2], w[:, 1]], 1), torch.stack([w[:, 2], Z, -w[:, 0]], 1),
                      torch.stack([-w[:, 1], w[:, 0], Z], 1)], 1)
@@ -145,6 +151,24 @@ def se3_left(xi, T_cw):
     Rn = Rw @ R
     tn = (Rw @ t[..., None])[..., 0] + (V @ u[..., None])[..., 0]
     return torch.cat([Rn.reshape(-1, 9), tn], 1).contiguous()
+
+
+def tangent_to_entry_grad(g, T):
+    """A gradient with respect to the 12 entries [R | t] of each pose T that
+    acts on every tangent perturbation exactly as the left-tangent gradient g
+    = (g_omega, g_upsilon) does.  A left perturbation d moves R by [d_omega]x R
+    and t by [d_omega]x t + d_upsilon, so g . d = a . d_omega + g_upsilon . dt
+    with a = g_omega - t x g_upsilon, and a . d_omega = <[a]x R / 2, dR>
+    (Frobenius) because [d_omega]x = dR R^T."""
+    import torch
+    R = T[:, :9].reshape(-1, 3, 3)
+    t = T[:, 9:]
+    a = g[:, :3] - torch.linalg.cross(t, g[:, 3:], dim=1)
+    Z = torch.zeros_like(a[:, 0])
+    A = torch.stack([torch.stack([Z, -a[:, 2], a[:, 1]], 1), torch.stack([a[:, 2], Z, -a[:, 0]], 1),
+                     torch.stack([-a[:, 1], a[:, 0], Z], 1)], 1)
+    GR = 0.5 * (A @ R)
+    return torch.cat([GR.reshape(-1, 9), g[:, 3:]], 1)
 
 
 try:
@@ -161,13 +185,22 @@ try:
                 need = any(ctx.needs_input_grad[:3])
                 r = batch.eval(T, X.contiguous(), intr.contiguous(), grad=need)
             if need:
-                ctx.save_for_backward(r.g_pose, r.g_X, r.g_intr)
+                ctx.save_for_backward(r.g_pose, r.g_X, r.g_intr, xi.detach(), T_cw.detach(), T)
             ctx.need = need
             return r.cost.sum()
 
         @staticmethod
         def backward(ctx, g):
-            gp, gx, gi = ctx.saved_tensors
-            return g * gp, g * gx, g * gi, None, None
+            gp, gx, gi, xi, T_cw, T = ctx.saved_tensors
+            g_xi = None
+            if ctx.needs_input_grad[0]:
+                # d cost / d xi = (d T / d xi)^T G_T, G_T the entry gradient of the
+                # left-tangent gradient gp at T: J_l(xi)^T gp
+                with torch.enable_grad():
+                    x = xi.clone().requires_grad_(True)
+                    Tx = se3_left(x, T_cw)
+                    (g_xi,) = torch.autograd.grad(Tx, x, tangent_to_entry_grad(gp, T))
+                g_xi = g * g_xi
+            return g_xi, g * gx, g * gi, None, None
 except ImportError:  # torch is optional for the C ABI users
     BACost = None

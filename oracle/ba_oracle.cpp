@@ -702,8 +702,10 @@ int32_t oracle_dense_step(const nav_ba_problem* p, const nav_ba_options* o, doub
 // max_trials trials, rho = (chi0 - chi1) / (x'(lambda x + b) + 1e-3), good step
 // lambda *= max(1/3, min(2/3, 1 - (2 rho - 1)^3)), bad step lambda *= nu, nu *= 2,
 // Terminate when the trial limit is reached or rho == 0.
-int32_t oracle_ba_solve(const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r,
-                        double* lambda_history, int32_t* trials_history) {
+static int32_t solve_impl(const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r,
+                          double* lambda_history, int32_t* trials_history, double* trace, int32_t max_trace,
+                          int32_t* n_trace) {
+    if (n_trace) *n_trace = 0;
     if (!p || !o || !r) return NAV_BA_BAD_ARG;
     Graph g;
     const int st = build_graph(p, o, g);
@@ -753,6 +755,11 @@ int32_t oracle_ba_solve(const nav_ba_problem* p, const nav_ba_options* o, nav_ba
             for (int j = 0; j < 3 * g.n_fp; ++j) scale += x[npd + j] * (lambda * x[npd + j] + s.bl[j]);
             scale += 1e-3;
             rho = (currentChi - tempChi) / scale;
+            if (trace && n_trace && *n_trace < max_trace) {
+                double* tr = trace + 7 * (size_t)(*n_trace)++;
+                tr[0] = it; tr[1] = q; tr[2] = lambda; tr[3] = nu; tr[4] = rho; tr[5] = currentChi;
+                tr[6] = tempChi;
+            }
             if (rho > 0 && std::isfinite(tempChi)) {
                 double alpha = 1. - std::pow((2 * rho - 1), 3);
                 alpha = std::min(alpha, 2. / 3.);
@@ -792,6 +799,16 @@ int32_t oracle_ba_solve(const nav_ba_problem* p, const nav_ba_options* o, nav_ba
     }
     r->n_outliers = nout;
     return NAV_BA_OK;
+}
+
+int32_t oracle_ba_solve(const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r,
+                        double* lambda_history, int32_t* trials_history) {
+    return solve_impl(p, o, r, lambda_history, trials_history, nullptr, 0, nullptr);
+}
+
+int32_t oracle_ba_solve_trace(const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r,
+                              double* trace, int32_t max_trace, int32_t* n_trace) {
+    return solve_impl(p, o, r, nullptr, nullptr, trace, max_trace, n_trace);
 }
 
 }  // extern "C"

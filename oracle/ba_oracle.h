@@ -34,6 +34,12 @@ int32_t oracle_ba_solve(const nav_ba_problem* p, const nav_ba_options* o, nav_ba
                         double* lambda_history /* [n_iter] lambda at start of each iteration, or NULL */,
                         int32_t* trials_history /* [n_iter] or NULL */);
 
+/* The same solve, recording every LM trial: trace[k][7] = {iteration, trial
+ * within the iteration, lambda used, nu at the trial, rho, chi2 before the
+ * trial, chi2 of the candidate}, at most max_trace trials; *n_trace = count. */
+int32_t oracle_ba_solve_trace(const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r,
+                              double* trace, int32_t max_trace, int32_t* n_trace);
+
 /* Robust chi2 at the state (T_cw, X) given in caller layout; optional per-observation
  * chi2 (unweighted e'e) and outlier flags. */
 double oracle_cost(const nav_ba_problem* p, const nav_ba_options* o,

@@ -36,6 +36,8 @@ def lib():
         P, O = C.POINTER(NavBaProblem), C.POINTER(NavBaOptions)
         L.oracle_ba_solve.argtypes = [P, O, C.c_void_p, _dp, _i32p]
         L.oracle_ba_solve.restype = C.c_int32
+        L.oracle_ba_solve_trace.argtypes = [P, O, C.c_void_p, _dp, C.c_int32, _i32p]
+        L.oracle_ba_solve_trace.restype = C.c_int32
         L.oracle_cost.argtypes = [P, O, _dp, _dp, _dp, _u8p]
         L.oracle_cost.restype = C.c_double
         L.oracle_linear_blocks.argtypes = [P, O, _dp, _dp, _dp, _dp]
@@ -106,6 +108,26 @@ def solve(prob: Problem, opts: Options, with_history: bool = False):
     if with_history:
         return res, lam[: res.iterations], tri[: res.iterations]
     return res
+
+
+TRACE_FIELDS = ("iter", "trial", "lam", "nu", "rho", "chi_before", "chi_after")
+
+
+def solve_trace(prob: Problem, opts: Options, max_trace: int = 4096, library=None):
+    """Solve and return (result, trace): trace is an (n, 7) array of the LM
+    trials, columns TRACE_FIELDS.  `library` may be another build of the same
+    source (a CDLL), for the tests that plant a slip in the restatement."""
+    L = library if library is not None else lib()
+    if library is not None:
+        L.oracle_ba_solve_trace.argtypes = [C.POINTER(NavBaProblem), C.POINTER(NavBaOptions), C.c_void_p, _dp,
+                                            C.c_int32, _i32p]
+        L.oracle_ba_solve_trace.restype = C.c_int32
+    ps, os_ = prob.c_struct(), opts.c_struct()
+    rb = ResultBuffers(prob, opts.n_iter)
+    tr = np.zeros((max_trace, 7), np.float64)
+    n = C.c_int32(0)
+    st = L.oracle_ba_solve_trace(C.byref(ps), C.byref(os_), C.addressof(rb.c), _d(tr), max_trace, C.byref(n))
+    return rb.result(st), tr[: n.value].copy()
 
 
 def cost(prob: Problem, opts: Options, T_cw: np.ndarray, X: np.ndarray):

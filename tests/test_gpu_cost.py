@@ -104,14 +104,20 @@ def test_points_without_observations_get_zero_gradient(dev):
     b.close()
 
 
-def test_autograd_gradcheck_fp64(dev):
+@pytest.mark.parametrize("xi_scale", [0.0, 0.3])
+def test_autograd_gradcheck_fp64(dev, xi_scale):
+    """gradcheck at xi = 0 and at random xi of norm about 0.3-0.5 per pose,
+    where d cost / d xi is J_l(xi)^T times the kernel's left-tangent gradient."""
     from nav24_amd.diffcost import BACost, CostBatch
-    say("autograd gradcheck fp64 (eps 1e-6)")
+    say(f"autograd gradcheck fp64 (eps 1e-6), xi scale {xi_scale}")
     probs = batch_problems(n=2, n_pt=12, n_cam=3)
     T, X, intr = concat(probs)
     b = CostBatch(probs)
     Tt = torch.tensor(T, device=dev)
-    xi = torch.zeros((T.shape[0], 6), dtype=torch.float64, device=dev, requires_grad=True)
+    rng = np.random.default_rng(11)
+    xi0 = xi_scale * rng.standard_normal((T.shape[0], 6)) / np.sqrt(2.0)
+    xi0[:, 3:] *= 0.2  # translations of a few centimetres keep every point in front
+    xi = torch.tensor(xi0, dtype=torch.float64, device=dev, requires_grad=True)
     Xt = torch.tensor(X, device=dev, requires_grad=True)
     it = torch.tensor(intr, device=dev, requires_grad=True)
     f = lambda a, x, k: BACost.apply(a, x, k, Tt, b)

@@ -1,4 +1,4 @@
-"""Schur work items: forced sizes, run lengths and the prepare-time tuner against the oracle.
+"""Schur work items: forced sizes, column groups and the prepare-time tuner against the oracle.
 
 The chunk size only changes how observations of one camera row are split over
 workgroups (and so the order of FP64 additions), never what is summed: the
@@ -33,19 +33,31 @@ def test_reduced_system_any_chunk(solver, oracle_mod, chunk):
     assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
 
 
-@pytest.mark.parametrize("run", [1, 3, 16])
-def test_reduced_system_any_run_length(solver, oracle_mod, run):
-    """Pairs of one slot summed in registers in runs of 1, 3 or 16 before the
-    LDS add: the same reduced system up to the order of additions."""
-    _say(f"reduced system workload1 n_pt=3000 schur run length {run}")
-    prob = problems.workload(1, seed=6, n_pt=3000)
+def max_covisible(prob):
+    """Largest number of free cameras sharing a free point with one free camera."""
+    free = np.ones(prob.n_cam, bool) if prob.cam_fixed is None else prob.cam_fixed == 0
+    seen = np.zeros((prob.n_cam, prob.n_pt), bool)
+    seen[prob.obs_cam, prob.obs_pt] = True
+    seen = seen[free]
+    co = (seen.astype(np.int32) @ seen.T.astype(np.int32)) > 0
+    return int(co.sum(1).max())
+
+
+@pytest.mark.parametrize("chunk", [256, 1024])
+def test_reduced_system_column_groups(solver, oracle_mod, chunk):
+    """Rows with more co-visible cameras than one work item's 112 slots are
+    split into column groups (each item then recomputes its W_a)."""
+    prob = problems.workload(1, seed=6, n_cam=160, n_pt=6000)
+    mc = max_covisible(prob)
+    _say(f"reduced system workload1 160 cameras n_pt=6000 (up to {mc} co-visible) schur chunk {chunk}")
+    assert mc > 112
     opts = workload_options(1)
-    solver.set_schur_run(run)
+    solver.set_schur_chunk(chunk)
     try:
         solver.prepare(prob, opts)
         S, g = solver.debug_reduced_system(1e-4)
     finally:
-        solver.set_schur_run(0)
+        solver.set_schur_chunk(0)
     So, go = oracle_mod.reduced_system(prob, opts, 1e-4)
     assert np.abs(S - So).max() <= 1e-11 * np.abs(So).max()
     assert np.abs(g - go).max() <= 1e-11 * np.abs(go).max()
@@ -75,6 +87,3 @@ def test_chunk_argument_checked(solver):
     for bad in (-64, 100, 8192):
         with pytest.raises(BAError):
             solver.set_schur_chunk(bad)
-    for bad in (-1, 65):
-        with pytest.raises(BAError):
-            solver.set_schur_run(bad)

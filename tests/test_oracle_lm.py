@@ -135,3 +135,98 @@ def test_outlier_flags_definition():
     assert np.array_equal(flags, r.outlier) or np.sum(flags != r.outlier) <= 1
     assert np.array_equal(r.outlier, (chi2 > o.outlier_chi2).astype(np.uint8)) or np.sum(
         r.outlier != (chi2 > o.outlier_chi2)) <= 1
+
+
+# ---- the reject branch of the LM schedule, pinned on a recorded trace ----
+# g2o's OptimizationAlgorithmLevenberg::solve (the algorithm nav24 builds at
+# core/backEnd/BE_GraphOptim.cpp:49): a rejected trial multiplies lambda by nu
+# and doubles nu; an accepted one multiplies lambda by
+# max(1/3, min(2/3, 1 - (2 rho - 1)^3)) and resets nu to 2; a trial follows a
+# rejection within the same iteration, and the next iteration starts after an
+# acceptance (or the solve ends).
+
+def lm_trace_problem():
+    """12 points seen noise-free by 10 cameras, started far off with a tiny
+    lambda: iteration 0 fails five factorisations before it accepts, and
+    iteration 2 rejects nine trials (a failed factorisation, then eight cost
+    increases) before it accepts at the tenth trial and terminates."""
+    prob = problems.workload(0, seed=12, n_pt=12, rot_sigma=0.6, trans_sigma=1.0, point_sigma=0.3,
+                             noise_px=0.0, outlier_rate=0.0)
+    o = opts(lambda_init=1e-14, n_iter=40, robust=False)
+    return prob, o
+
+
+def check_lm_trace(tr, res, lam0, max_trials=10):
+    """Replay the g2o rules over the oracle's recorded trials; raises on the
+    first trial whose lambda, nu, trial index or iteration breaks them."""
+    lam, nu = lam0, 2.0
+    it, q = 0, 0
+    for k, (t_it, t_q, t_lam, t_nu, rho, c0, c1) in enumerate(tr):
+        assert (int(t_it), int(t_q)) == (it, q), f"trial {k}: iteration/trial {t_it},{t_q} expected {it},{q}"
+        assert t_nu == nu, f"trial {k}: nu {t_nu} expected {nu}"
+        assert abs(t_lam - lam) <= 1e-15 * abs(lam), f"trial {k}: lambda {t_lam!r} expected {lam!r}"
+        accepted = rho > 0 and np.isfinite(c1) and c1 < np.finfo(np.float64).max
+        if accepted:
+            assert c1 < c0
+            lam *= max(1.0 / 3.0, min(1.0 - (2.0 * rho - 1.0) ** 3, 2.0 / 3.0))
+            nu = 2.0
+            assert res.cost_history[it + 1] == c1
+            it, q = it + 1, 0
+        else:
+            lam *= nu
+            nu *= 2.0
+            q += 1
+            if q == max_trials:
+                it, q = it + 1, 0
+        if t_q + 1 == max_trials:  # Terminate after the trial limit, accepted or not
+            assert k == len(tr) - 1
+    return True
+
+
+def test_lm_reject_branch_trace():
+    prob, o = lm_trace_problem()
+    res, tr = oracle.solve_trace(prob, o)
+    assert check_lm_trace(tr, res, o.lambda_init)
+    rej = tr[:, 4] <= 0
+    # the features the pin needs are in the trace: three rejections in a row
+    # (nu = 2, 4, 8), a rejection by a failed factorisation and one by a cost
+    # increase, and a rejection that follows an acceptance (nu back to 2)
+    assert list(tr[:3, 3]) == [2.0, 4.0, 8.0] and rej[:3].all()
+    assert np.all(tr[1:4, 2] / tr[:3, 2] == [2.0, 4.0, 8.0])
+    assert (rej & (tr[:, 6] == np.finfo(np.float64).max)).any()
+    assert (rej & (tr[:, 6] < np.finfo(np.float64).max) & (tr[:, 6] > tr[:, 5])).any()
+    after_accept = [k for k in range(1, len(tr)) if rej[k] and not rej[k - 1]]
+    assert after_accept and all(tr[k, 3] == 2.0 and tr[k, 2] < tr[k - 1, 2] for k in after_accept)
+    # the accepted lambda factor took both the cubic and a crop value somewhere
+    acc = ~rej
+    f = np.array([max(1 / 3, min(1 - (2 * r - 1) ** 3, 2 / 3)) for r in tr[acc, 4]])
+    assert ((f > 1 / 3) & (f < 2 / 3)).any()
+    assert res.stop_reason == 1 and res.iterations == int(tr[-1, 0]) + 1
+
+
+SLIPS = [
+    ("nu grows by 3", "                nu *= 2;\n", "                nu *= 3;\n"),
+    ("nu not reset on accept", "                nu = 2;\n                currentChi = tempChi;\n",
+     "                currentChi = tempChi;\n"),
+    ("lambda doubled instead of times nu", "                lambda *= nu;\n", "                lambda *= 2;\n"),
+]
+
+
+@pytest.mark.parametrize("name,old,new", SLIPS, ids=[s[0] for s in SLIPS])
+def test_lm_trace_pin_catches_planted_slip(tmp_path, name, old, new):
+    """Each slip is planted in a copy of the oracle source, built on the CPU;
+    the replay of its trace must fail."""
+    import ctypes
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    src = open(os.path.join(here, "ba_oracle.cpp")).read()
+    assert src.count(old) == 1, f"slip site for '{name}' not found once"
+    (tmp_path / "ba_oracle.cpp").write_text(src.replace(old, new))
+    so = tmp_path / "libslip.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-I", here,
+                    str(tmp_path / "ba_oracle.cpp"), "-o", str(so)], check=True)
+    prob, o = lm_trace_problem()
+    res, tr = oracle.solve_trace(prob, o, library=ctypes.CDLL(str(so)))
+    with pytest.raises(AssertionError):
+        check_lm_trace(tr, res, o.lambda_init)
