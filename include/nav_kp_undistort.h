@@ -9,8 +9,10 @@
  * measurement the BA edge reads (KeyPoint2D::getPointUd).  The reference calls
  * OpenCV's cv::undistortPoints(pts, pts, K, D, R, P) (OpenCV >= 4.2.0,
  * core/CMakeLists.txt:10; absent from this image) with
- *     K = [fx 0 cx; 0 fy cy; 0 0 1] and P = K, R = I   (GeometricCamera.h:62-66)
- *     D = (k1, k2, p1, p2)                            (PinholeRadTan.hpp:13-18)
+ *     K = [fx 0 cx; 0 fy cy; 0 0 1], D = (k1, k2, p1, p2)  (PinholeRadTan.hpp:13-18)
+ *     R = the camera's rectification, I unless the calibration sets one, and
+ *     P = its projection, K unless set  (GeometricCamera.h:47-48,65-66,108-109,
+ *     set from the configuration at Calibration.cpp:63-93)
  * all float, key points cv::Point2f.  The published algorithm of that call,
  * in double precision:
  *     x0 = x = (u - cx) / fx,  y0 = y = (v - cy) / fy   (as (u - cx) * (1 / fx))
@@ -18,7 +20,8 @@
  *               icdist = 1 / (1 + (k2 r2 + k1) r2)      (icdist < 0: x, y = x0, y0; stop)
  *               dx = 2 p1 x y + p2 (r2 + 2 x^2),  dy = p1 (r2 + 2 y^2) + 2 p2 x y
  *               x = (x0 - dx) icdist,  y = (y0 - dy) icdist
- *     (u', v') = (RR x) / (RR x)_z with RR = P R, rounded to float.
+ *     (u', v') = (RR x) / (RR x)_z with RR = P(:, 0:3) R formed in double,
+ *     rounded to float.
  * The device evaluates exactly these double operations (no fused
  * multiply-adds), so its output equals a plain double evaluation bit for bit.
  *
@@ -37,7 +40,8 @@ extern "C" {
 #endif
 
 /* params: fx fy cx cy k1 k2 p1 p2 (the reference's vParams order,
- * PinholeRadTan(vParams), float).  uv, uv_out: n key points (u, v), float. */
+ * PinholeRadTan(vParams), float).  uv, uv_out: n key points (u, v), float.
+ * The entries without R and P are the camera's defaults, R = I and P = K. */
 
 /* Device arrays, on `stream` (a hipStream_t, NULL = the null stream); returns
  * after the launch is queued.  NAV_BA_BAD_ARG for n < 0 or a null pointer
@@ -64,6 +68,23 @@ int32_t nav_ba_kp_undistort_host(const float* uv, int32_t n, const float* params
 int32_t nav_ba_kp_undistort_fisheye(const float* uv, int32_t n, const float* params, float* uv_out, void* stream);
 int32_t nav_ba_kp_undistort_fisheye_host(const float* uv, int32_t n, const float* params, float* uv_out,
                                          int32_t device);
+
+/* Either camera model with the camera's rectification R (3x3 row-major
+ * double, NULL = identity) and projection P (p_cols = 3 or 4 columns,
+ * row-major double, its left 3x3 used, NULL = K): the matrices the reference
+ * passes to both OpenCV calls (mR, mP).  RR = P(:, 0:3) R is formed on the
+ * host in double, each entry a sum of three products in column order.
+ * model: NAV_KP_RADTAN (params fx fy cx cy k1 k2 p1 p2) or NAV_KP_KB8
+ * (fx fy cx cy k1 k2 k3 k4).  Device arrays on `stream`, or host arrays
+ * through `device` (_host).  NAV_BA_BAD_ARG for an unknown model, p_cols
+ * other than 3 or 4 with P given, n < 0 or a null array with n > 0. */
+enum nav_kp_model { NAV_KP_RADTAN = 0, NAV_KP_KB8 = 1 };
+
+int32_t nav_ba_kp_undistort_rect(int32_t model, const float* uv, int32_t n, const float* params, const double* R,
+                                 const double* P, int32_t p_cols, float* uv_out, void* stream);
+int32_t nav_ba_kp_undistort_rect_host(int32_t model, const float* uv, int32_t n, const float* params,
+                                      const double* R, const double* P, int32_t p_cols, float* uv_out,
+                                      int32_t device);
 
 #ifdef __cplusplus
 }

@@ -2,11 +2,15 @@
 // (include/nav_kp_undistort.h):
 //   radial-tangential pinhole: PinholeRadTan::UndistortKeyPoints
 //     (core/sensor/camera/models/PinholeRadTan.cpp:11-24), i.e. OpenCV's
-//     cv::undistortPoints with K, D = (k1 k2 p1 p2), R = I, P = K and its
-//     default termination (5 fixed-point iterations);
+//     cv::undistortPoints with K, D = (k1 k2 p1 p2), the camera's R and P
+//     and its default termination (5 fixed-point iterations);
 //   Kannala-Brandt fisheye: KannalaBrandt8::UndistortKeyPoints
 //     (KannalaBrandt8.cpp:231-242), i.e. cv::fisheye::undistortPoints with
-//     D = (k1 k2 k3 k4), R = I, P = K (up to 10 Newton steps on theta).
+//     D = (k1 k2 k3 k4), R and P (up to 10 Newton steps on theta).
+// R is the camera's rectification (identity unless the calibration sets one)
+// and P its projection (K unless set): GeometricCamera.h:47-48,65-66,108-109,
+// set from the configuration at Calibration.cpp:63-93.  Both OpenCV calls form
+// RR = P(:, 0:3) R in double; here on the host, once per call.
 //
 // One lane per key point, the double operations of that algorithm in its
 // order, without fused multiply-adds (contraction is off in this file), so
@@ -29,7 +33,7 @@ namespace navba {
 struct KpParams {
     double cx, cy, ifx, ify;  // ifx = 1 / fx in double, as the reference's call forms it
     double k1, k2, p1, p2;
-    double RR[9];             // P R (= K here), row-major
+    double RR[9];             // P R, row-major
 };
 
 constexpr int kKpIters = 5;  // cv::undistortPoints default: TermCriteria(COUNT, 5, 0.01)
@@ -66,7 +70,7 @@ __global__ __launch_bounds__(256) void k_kp_undistort(const float2* __restrict__
 struct KbParams {
     double fx, fy, cx, cy;
     double k[4];
-    double RR[9];  // P R (= K here), row-major
+    double RR[9];  // P R, row-major
 };
 
 constexpr int kKbIters = 10;         // fisheye::undistortPoints: at most 10 Newton steps
@@ -104,19 +108,34 @@ __global__ __launch_bounds__(256) void k_kp_undistort_fisheye(const float2* __re
     out[i] = make_float2((float)(pr0 / pr2), (float)(pr1 / pr2));
 }
 
-static KbParams kb_params(const float* prm) {
+// RR = P3 R in double, P3 the left 3x3 of P (3 or 4 columns, row-major), K
+// when P is null, R the identity when null; each entry a sum of three products
+// in column order, without contraction.
+static void rect_rr(double fx, double fy, double cx, double cy, const double* R, const double* P, int p_cols,
+                    double RR[9]) {
+    double P3[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};
+    if (P)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) P3[3 * i + j] = P[p_cols * i + j];
+    double R3[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
+    if (R) std::memcpy(R3, R, sizeof(R3));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            RR[3 * i + j] = (P3[3 * i] * R3[j] + P3[3 * i + 1] * R3[3 + j]) + P3[3 * i + 2] * R3[6 + j];
+}
+
+static KbParams kb_params(const float* prm, const double* R, const double* P, int p_cols) {
     KbParams q;
     q.fx = prm[0];
     q.fy = prm[1];
     q.cx = prm[2];
     q.cy = prm[3];
     for (int c = 0; c < 4; ++c) q.k[c] = prm[4 + c];
-    const double K[9] = {q.fx, 0.0, q.cx, 0.0, q.fy, q.cy, 0.0, 0.0, 1.0};
-    std::memcpy(q.RR, K, sizeof(K));
+    rect_rr(q.fx, q.fy, q.cx, q.cy, R, P, p_cols, q.RR);
     return q;
 }
 
-static KpParams kp_params(const float* prm) {
+static KpParams kp_params(const float* prm, const double* R, const double* P, int p_cols) {
     KpParams q;
     const double fx = prm[0], fy = prm[1], cx = prm[2], cy = prm[3];
     q.cx = cx;
@@ -127,9 +146,7 @@ static KpParams kp_params(const float* prm) {
     q.k2 = prm[5];
     q.p1 = prm[6];
     q.p2 = prm[7];
-    // RR = P R with P = K, R = I (GeometricCamera.h:62-66): K itself
-    const double K[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};
-    std::memcpy(q.RR, K, sizeof(K));
+    rect_rr(fx, fy, cx, cy, R, P, p_cols, q.RR);
     return q;
 }
 
@@ -141,61 +158,72 @@ static bool gfx950_device(int device) {
     return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
 }
 
+static bool rect_args(int32_t model, int32_t n, const float* uv, const float* params, const double* P,
+                      int32_t p_cols, float* uv_out) {
+    if (model != NAV_KP_RADTAN && model != NAV_KP_KB8) return false;
+    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return false;
+    return !P || p_cols == 3 || p_cols == 4;
+}
+
+static int32_t launch_undistort(int32_t model, const float* uv, int32_t n, const float* params, const double* R,
+                                const double* P, int32_t p_cols, float* uv_out, hipStream_t stream) {
+    if (model == NAV_KP_RADTAN)
+        hipLaunchKernelGGL(k_kp_undistort, dim3((n + 255) / 256), dim3(256), 0, stream,
+                           reinterpret_cast<const float2*>(uv), n, kp_params(params, R, P, p_cols),
+                           reinterpret_cast<float2*>(uv_out));
+    else
+        hipLaunchKernelGGL(k_kp_undistort_fisheye, dim3((n + 255) / 256), dim3(256), 0, stream,
+                           reinterpret_cast<const float2*>(uv), n, kb_params(params, R, P, p_cols),
+                           reinterpret_cast<float2*>(uv_out));
+    return hipGetLastError() == hipSuccess ? NAV_BA_OK : NAV_BA_HIP_ERROR;
+}
+
 }  // namespace navba
 
 using namespace navba;
 
-extern "C" int32_t nav_ba_kp_undistort(const float* uv, int32_t n, const float* params, float* uv_out, void* stream) {
-    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
+extern "C" int32_t nav_ba_kp_undistort_rect(int32_t model, const float* uv, int32_t n, const float* params,
+                                            const double* R, const double* P, int32_t p_cols, float* uv_out,
+                                            void* stream) {
+    if (!rect_args(model, n, uv, params, P, p_cols, uv_out)) return NAV_BA_BAD_ARG;
     if (n == 0) return NAV_BA_OK;
-    const KpParams q = kp_params(params);
-    hipLaunchKernelGGL(k_kp_undistort, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                       reinterpret_cast<const float2*>(uv), n, q, reinterpret_cast<float2*>(uv_out));
-    return hipGetLastError() == hipSuccess ? NAV_BA_OK : NAV_BA_HIP_ERROR;
+    return launch_undistort(model, uv, n, params, R, P, p_cols, uv_out, (hipStream_t)stream);
+}
+
+extern "C" int32_t nav_ba_kp_undistort_rect_host(int32_t model, const float* uv, int32_t n, const float* params,
+                                                 const double* R, const double* P, int32_t p_cols, float* uv_out,
+                                                 int32_t device) {
+    if (!rect_args(model, n, uv, params, P, p_cols, uv_out)) return NAV_BA_BAD_ARG;
+    if (!gfx950_device(device)) return NAV_BA_NO_DEVICE;
+    if (n == 0) return NAV_BA_OK;
+    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
+    const size_t bytes = sizeof(float) * 2 * (size_t)n;
+    HostScratch& hs = host_scratch(device);
+    std::lock_guard<std::mutex> lock(hs.mu);
+    float* d = (float*)host_scratch_get(hs, bytes);
+    if (!d) return NAV_BA_HIP_ERROR;
+    int32_t st = NAV_BA_OK;
+    if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    if (st == NAV_BA_OK) st = launch_undistort(model, d, n, params, R, P, p_cols, d, nullptr);
+    if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    return st;
+}
+
+extern "C" int32_t nav_ba_kp_undistort(const float* uv, int32_t n, const float* params, float* uv_out, void* stream) {
+    return nav_ba_kp_undistort_rect(NAV_KP_RADTAN, uv, n, params, nullptr, nullptr, 3, uv_out, stream);
 }
 
 extern "C" int32_t nav_ba_kp_undistort_host(const float* uv, int32_t n, const float* params, float* uv_out,
                                             int32_t device) {
-    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
-    if (!gfx950_device(device)) return NAV_BA_NO_DEVICE;
-    if (n == 0) return NAV_BA_OK;
-    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
-    const size_t bytes = sizeof(float) * 2 * (size_t)n;
-    HostScratch& hs = host_scratch(device);
-    std::lock_guard<std::mutex> lock(hs.mu);
-    float* d = (float*)host_scratch_get(hs, bytes);
-    if (!d) return NAV_BA_HIP_ERROR;
-    int32_t st = NAV_BA_OK;
-    if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
-    if (st == NAV_BA_OK) st = nav_ba_kp_undistort(d, n, params, d, nullptr);
-    if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
-    return st;
+    return nav_ba_kp_undistort_rect_host(NAV_KP_RADTAN, uv, n, params, nullptr, nullptr, 3, uv_out, device);
 }
 
 extern "C" int32_t nav_ba_kp_undistort_fisheye(const float* uv, int32_t n, const float* params, float* uv_out,
                                                void* stream) {
-    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
-    if (n == 0) return NAV_BA_OK;
-    const KbParams q = kb_params(params);
-    hipLaunchKernelGGL(k_kp_undistort_fisheye, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                       reinterpret_cast<const float2*>(uv), n, q, reinterpret_cast<float2*>(uv_out));
-    return hipGetLastError() == hipSuccess ? NAV_BA_OK : NAV_BA_HIP_ERROR;
+    return nav_ba_kp_undistort_rect(NAV_KP_KB8, uv, n, params, nullptr, nullptr, 3, uv_out, stream);
 }
 
 extern "C" int32_t nav_ba_kp_undistort_fisheye_host(const float* uv, int32_t n, const float* params, float* uv_out,
                                                     int32_t device) {
-    if (n < 0 || !params || (n > 0 && (!uv || !uv_out))) return NAV_BA_BAD_ARG;
-    if (!gfx950_device(device)) return NAV_BA_NO_DEVICE;
-    if (n == 0) return NAV_BA_OK;
-    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
-    const size_t bytes = sizeof(float) * 2 * (size_t)n;
-    HostScratch& hs = host_scratch(device);
-    std::lock_guard<std::mutex> lock(hs.mu);
-    float* d = (float*)host_scratch_get(hs, bytes);
-    if (!d) return NAV_BA_HIP_ERROR;
-    int32_t st = NAV_BA_OK;
-    if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
-    if (st == NAV_BA_OK) st = nav_ba_kp_undistort_fisheye(d, n, params, d, nullptr);
-    if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
-    return st;
+    return nav_ba_kp_undistort_rect_host(NAV_KP_KB8, uv, n, params, nullptr, nullptr, 3, uv_out, device);
 }

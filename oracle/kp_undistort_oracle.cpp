@@ -8,8 +8,9 @@
 // (core/sensor/camera/models/PinholeRadTan.cpp:11-24), which calls
 // cv::undistortPoints(pts, pts, K, D, R, P) of OpenCV >= 4.2.0
 // (core/CMakeLists.txt:10; OpenCV is absent from this image) with K from
-// (fx, fy, cx, cy), D = (k1, k2, p1, p2), R = I and P = K
-// (GeometricCamera.h:62-66, PinholeRadTan.hpp:13-18).  The restated steps are
+// (fx, fy, cx, cy), D = (k1, k2, p1, p2) and the camera's R and P, identity
+// and K unless the calibration sets them (GeometricCamera.h:47-48,65-66,
+// 108-109, Calibration.cpp:63-93, PinholeRadTan.hpp:13-18).  The restated steps are
 // those of OpenCV's published undistortPoints: normalise by K (multiplying by
 // 1/fx, 1/fy formed in double), a fixed number of fixed-point iterations
 // x = (x0 - delta_tan(x)) / (1 + k1 r^2 + k2 r^4) (5 under the default
@@ -27,19 +28,37 @@
 //   - with many iterations the result inverts the forward Brown-Conrady
 //     model (oracle_kp_distort) to 1e-9 px for mild distortion;
 //   - the first iteration equals its closed form on hand-picked points;
-//   - a radial factor that turns negative returns the input point.
+//   - a radial factor that turns negative returns the input point;
+//   - with R and P set, the result is the R = I, P = K result mapped by the
+//     homography P R K^-1 (within the float rounding of the latter), for
+//     both camera models; a 4th column of P is ignored.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 
-extern "C" {
+namespace {
 
-// uv, out: n points (u, v); params: fx fy cx cy k1 k2 p1 p2; iters: 5 = the reference's call.
-void oracle_kp_undistort(const float* uv, int32_t n, const float* params, int32_t iters, float* out) {
+// RR = P3 R in double: P3 the left 3x3 of P (p_cols columns) or K when P is
+// null, R or the identity; each entry ((p0 r0 + p1 r1) + p2 r2) in column order.
+void form_rr(double fx, double fy, double cx, double cy, const double* R, const double* P, int p_cols,
+             double RR[9]) {
+    double P3[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};
+    if (P)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) P3[3 * i + j] = P[p_cols * i + j];
+    double R3[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
+    if (R) std::memcpy(R3, R, sizeof(R3));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            RR[3 * i + j] = (P3[3 * i] * R3[j] + P3[3 * i + 1] * R3[3 + j]) + P3[3 * i + 2] * R3[6 + j];
+}
+
+void undistort_radtan(const float* uv, int32_t n, const float* params, int32_t iters, const double RR[9],
+                      float* out) {
     const double fx = params[0], fy = params[1], cx = params[2], cy = params[3];
     const double k1 = params[4], k2 = params[5], p1 = params[6], p2 = params[7];
     const double ifx = 1.0 / fx, ify = 1.0 / fy;
-    const double RR[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};  // P R = K
     for (int32_t i = 0; i < n; ++i) {
         const double u = uv[2 * i], v = uv[2 * i + 1];
         double x = (u - cx) * ifx;
@@ -64,6 +83,17 @@ void oracle_kp_undistort(const float* uv, int32_t n, const float* params, int32_
         out[2 * i] = (float)(xx * ww);
         out[2 * i + 1] = (float)(yy * ww);
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+// uv, out: n points (u, v); params: fx fy cx cy k1 k2 p1 p2; iters: 5 = the reference's call.
+void oracle_kp_undistort(const float* uv, int32_t n, const float* params, int32_t iters, float* out) {
+    double RR[9];
+    form_rr(params[0], params[1], params[2], params[3], nullptr, nullptr, 3, RR);  // P R = K
+    undistort_radtan(uv, n, params, iters, RR, out);
 }
 
 // Forward model (the one undistortion inverts), in double: normalised
@@ -96,10 +126,9 @@ void oracle_kp_distort(const double* xy, int32_t n, const float* params, double*
 // against OpenCV (absent); pinned by tests/test_kp_undistort_oracle.py
 // (equidistant closed form at k = 0, collinearity with the principal point,
 // the principal point fixed, and round trips through oracle_kp_distort_fisheye).
-void oracle_kp_undistort_fisheye(const float* uv, int32_t n, const float* params, float* out) {
+static void undistort_kb8(const float* uv, int32_t n, const float* params, const double RR[9], float* out) {
     const double fx = params[0], fy = params[1], cx = params[2], cy = params[3];
     const double k[4] = {params[4], params[5], params[6], params[7]};
-    const double RR[9] = {fx, 0.0, cx, 0.0, fy, cy, 0.0, 0.0, 1.0};
     const double half_pi = 1.5707963267948966;
     for (int32_t i = 0; i < n; ++i) {
         const double pw0 = ((double)uv[2 * i] - cx) / fx, pw1 = ((double)uv[2 * i + 1] - cy) / fy;
@@ -126,6 +155,23 @@ void oracle_kp_undistort_fisheye(const float* uv, int32_t n, const float* params
         out[2 * i] = (float)(pr0 / pr2);
         out[2 * i + 1] = (float)(pr1 / pr2);
     }
+}
+
+void oracle_kp_undistort_fisheye(const float* uv, int32_t n, const float* params, float* out) {
+    double RR[9];
+    form_rr(params[0], params[1], params[2], params[3], nullptr, nullptr, 3, RR);
+    undistort_kb8(uv, n, params, RR, out);
+}
+
+// Either model (0 radial-tangential, 1 Kannala-Brandt) with the camera's R
+// (3x3, null = identity) and P (p_cols = 3 or 4 columns, null = K), as the
+// reference passes mR and mP to OpenCV (PinholeRadTan.cpp:20, KannalaBrandt8.cpp:238).
+void oracle_kp_undistort_rect(int32_t model, const float* uv, int32_t n, const float* params, const double* R,
+                              const double* P, int32_t p_cols, float* out) {
+    double RR[9];
+    form_rr(params[0], params[1], params[2], params[3], R, P, p_cols, RR);
+    if (model == 0) undistort_radtan(uv, n, params, 5, RR, out);
+    else undistort_kb8(uv, n, params, RR, out);
 }
 
 // Forward Kannala-Brandt model (the one fisheye undistortion inverts), in

@@ -61,6 +61,8 @@ def lib():
         L.oracle_kp_undistort.restype = None
         L.oracle_kp_distort.argtypes = [_dp, C.c_int32, _fp, _dp]
         L.oracle_kp_distort.restype = None
+        L.oracle_kp_undistort_rect.argtypes = [C.c_int32, _fp, C.c_int32, _fp, _dp, _dp, C.c_int32, _fp]
+        L.oracle_kp_undistort_rect.restype = None
         L.oracle_kp_undistort_fisheye.argtypes = [_fp, C.c_int32, _fp, _fp]
         L.oracle_kp_undistort_fisheye.restype = None
         L.oracle_kp_distort_fisheye.argtypes = [_dp, C.c_int32, _fp, _dp]
@@ -220,6 +222,22 @@ def kp_undistort(uv: np.ndarray, params, iters: int = 5) -> np.ndarray:
     fp = C.POINTER(C.c_float)
     lib().oracle_kp_undistort(uv.ctypes.data_as(fp), uv.shape[0], prm.ctypes.data_as(fp), iters,
                               out.ctypes.data_as(fp))
+    return out
+
+
+def kp_undistort_rect(model: int, uv: np.ndarray, params, R=None, P=None) -> np.ndarray:
+    """Either camera model (0 radial-tangential, 1 Kannala-Brandt) with the
+    camera's rectification R (3x3) and projection P (3x3 or 3x4); None = I, K."""
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    prm = np.ascontiguousarray(params, np.float32)
+    out = np.empty_like(uv)
+    fp = C.POINTER(C.c_float)
+    Ra = None if R is None else np.ascontiguousarray(R, np.float64).reshape(3, 3)
+    Pa = None if P is None else np.ascontiguousarray(P, np.float64)
+    p_cols = 3 if Pa is None else Pa.shape[1]
+    nd = lambda a: None if a is None else a.ctypes.data_as(_dp)  # noqa: E731
+    lib().oracle_kp_undistort_rect(int(model), uv.ctypes.data_as(fp), uv.shape[0], prm.ctypes.data_as(fp), nd(Ra),
+                                   nd(Pa), p_cols, out.ctypes.data_as(fp))
     return out
 
 

@@ -97,3 +97,28 @@ def test_fisheye_device_entry_in_place():
         got = t.cpu().numpy()
         _say(f"kp undistort fisheye device in place n={n}: {int((got != ref).sum())} values differ")
         assert np.all(_ulps(got, ref) <= 1.0), n
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_rectified_camera_equals_oracle(model):
+    """A calibration with a rectification R and a 3x4 projection P
+    (Calibration.cpp:88-93): host entry and device tensors equal the oracle
+    bit for bit, and differ from the unrectified result."""
+    import torch
+    from nav24_amd.undistort import KannalaBrandt8Hip, PinholeRadTanHip
+    from test_kp_undistort_oracle import RECT_P, RECT_R
+    prm = PARAMS["strong"] if model == 0 else [380.0, 380.0, 320.0, 240.0, 0.05, -0.01, 0.002, -0.0003]
+    _say(f"kp undistort with R and P, model {model}, 200000 points")
+    cam = (PinholeRadTanHip if model == 0 else KannalaBrandt8Hip)(prm)
+    uv = _pts(200000, 11)
+    plain = cam.UndistortKeyPoints(uv)
+    cam.setRectificationMat(RECT_R)
+    cam.setProjectionMat(RECT_P)
+    got = cam.UndistortKeyPoints(uv)
+    ref = oracle.kp_undistort_rect(model, uv, prm, RECT_R, RECT_P)
+    assert np.array_equal(got, ref)
+    assert np.abs(got - plain).max() > 1.0
+    t = torch.from_numpy(uv).cuda()
+    cam.undistort_device(t, out=t)
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy(), ref)

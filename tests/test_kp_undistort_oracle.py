@@ -113,3 +113,49 @@ def test_fisheye_inverts_the_forward_model(prm):
     ideal = ab * 380.0 + [320.0, 240.0]
     # the float32 distorted input and the 1e-8 Newton tolerance bound the agreement
     np.testing.assert_allclose(back, ideal, rtol=1e-5, atol=2e-3)
+
+
+# ---- the camera's rectification R and projection P (mR, mP) ----
+
+def _rot(ax, ang):
+    ax = np.asarray(ax, np.float64) / np.linalg.norm(ax)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+RECT_R = _rot([0.3, -0.5, 0.8], 0.04)
+RECT_P = np.array([[430.0, 0.0, 310.0, -25.0], [0.0, 430.0, 236.0, 0.0], [0.0, 0.0, 1.0, 0.0]])
+
+
+@pytest.mark.parametrize("model,prm", [(0, [450.0, 450.0, 320.0, 240.0, -0.2, 0.05, 1e-3, -5e-4]),
+                                       (1, [380.0, 380.0, 320.0, 240.0, 0.05, -0.01, 0.002, -0.0003])])
+def test_rect_identity_is_the_plain_call(model, prm):
+    r = np.random.default_rng(3)
+    uv = np.stack([r.uniform(0, 640, 2000), r.uniform(0, 480, 2000)], 1).astype(np.float32)
+    plain = oracle.kp_undistort(uv, prm) if model == 0 else oracle.kp_undistort_fisheye(uv, prm)
+    K = np.array([[prm[0], 0, prm[2]], [0, prm[1], prm[3]], [0, 0, 1.0]])
+    for R, P in ((None, None), (np.eye(3), None), (None, K), (np.eye(3), np.hstack([K, np.ones((3, 1))]))):
+        assert np.array_equal(oracle.kp_undistort_rect(model, uv, prm, R, P), plain)
+
+
+@pytest.mark.parametrize("model,prm", [(0, [450.0, 450.0, 320.0, 240.0, -0.2, 0.05, 1e-3, -5e-4]),
+                                       (1, [380.0, 380.0, 320.0, 240.0, 0.05, -0.01, 0.002, -0.0003])])
+def test_rect_is_the_homography_of_the_plain_result(model, prm):
+    """undistortPoints maps the undistorted normalised point x by RR = P R:
+    the rectified result is the plain (R = I, P = K) result mapped by
+    P R K^-1.  A transposed R or P's 4th column used moves points by pixels."""
+    r = np.random.default_rng(4)
+    uv = np.stack([r.uniform(40, 600, 3000), r.uniform(40, 440, 3000)], 1).astype(np.float32)
+    plain = (oracle.kp_undistort(uv, prm) if model == 0 else oracle.kp_undistort_fisheye(uv, prm)).astype(np.float64)
+    K = np.array([[prm[0], 0, prm[2]], [0, prm[1], prm[3]], [0, 0, 1.0]])
+    Hm = RECT_P[:, :3] @ RECT_R @ np.linalg.inv(K)
+    q = np.c_[plain, np.ones(len(plain))] @ Hm.T
+    want = q[:, :2] / q[:, 2:]
+    got = oracle.kp_undistort_rect(model, uv, prm, RECT_R, RECT_P).astype(np.float64)
+    tol = 2e-3 * (1 + np.abs(want))  # float rounding of the plain result, carried through the homography
+    assert np.all(np.abs(got - want) <= tol)
+    fourth_used = RECT_P[:, :3].copy()
+    fourth_used[:, 2] += RECT_P[:, 3]  # as if P's 4th column entered the product
+    for bad_R, bad_P in ((RECT_R.T, RECT_P), (RECT_R, fourth_used)):
+        moved = oracle.kp_undistort_rect(model, uv, prm, bad_R, bad_P).astype(np.float64)
+        assert np.abs(moved - want).max() > 0.5
