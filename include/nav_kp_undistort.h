@@ -86,6 +86,21 @@ int32_t nav_ba_kp_undistort_rect_host(int32_t model, const float* uv, int32_t n,
                                       const double* R, const double* P, int32_t p_cols, float* uv_out,
                                       int32_t device);
 
+/* nav_ba_kp_undistort_rect_host without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * key points are staged in a fixed device array of the library, at most
+ * NAV_KP_NOALLOC_MAX_POINTS (NAV_BA_TOO_LARGE above), which every device the
+ * library runs on holds until the process ends (16 MiB each).  Calls on one
+ * device share that array and are serialised by a per-device lock; the array
+ * keeps the last call's points.  Same arguments, results and errors as
+ * nav_ba_kp_undistort_rect_host otherwise. */
+#define NAV_KP_NOALLOC_MAX_POINTS (1 << 21)
+
+int32_t nav_ba_kp_undistort_rect_host_noalloc(int32_t model, const float* uv, int32_t n, const float* params,
+                                              const double* R, const double* P, int32_t p_cols, float* uv_out,
+                                              int32_t device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 
 #include "host_scratch.h"
 #include "../../include/nav_kp_undistort.h"
@@ -158,6 +159,22 @@ static bool gfx950_device(int device) {
     return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
 }
 
+// Staging of the allocation-free host entry: a file-scope device array of at
+// most kNoAllocMaxPoints key points; every device that runs the library holds
+// its own copy (at its own address) until the process ends.  Host code never
+// passes the array's name to the device: each device's address is taken once,
+// with that device current, by hipGetSymbolAddress and kept in a fixed slot
+// table (no allocation on any call), guarded by the device's lock.
+constexpr int kNoAllocMaxPoints = NAV_KP_NOALLOC_MAX_POINTS;  // 16 MiB of float2
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) float2 scratch[kNoAllocMaxPoints];
+
+struct NoAllocSlot {
+    std::mutex mu;
+    float2* ptr = nullptr;  // this device's address of `scratch`
+};
+static NoAllocSlot g_noalloc[kNoAllocMaxDevices];
+
 static bool rect_args(int32_t model, int32_t n, const float* uv, const float* params, const double* P,
                       int32_t p_cols, float* uv_out) {
     if (model != NAV_KP_RADTAN && model != NAV_KP_KB8) return false;
@@ -226,4 +243,28 @@ extern "C" int32_t nav_ba_kp_undistort_fisheye(const float* uv, int32_t n, const
 extern "C" int32_t nav_ba_kp_undistort_fisheye_host(const float* uv, int32_t n, const float* params, float* uv_out,
                                                     int32_t device) {
     return nav_ba_kp_undistort_rect_host(NAV_KP_KB8, uv, n, params, nullptr, nullptr, 3, uv_out, device);
+}
+
+extern "C" int32_t nav_ba_kp_undistort_rect_host_noalloc(int32_t model, const float* uv, int32_t n,
+                                                         const float* params, const double* R, const double* P,
+                                                         int32_t p_cols, float* uv_out, int32_t device) {
+    if (!rect_args(model, n, uv, params, P, p_cols, uv_out)) return NAV_BA_BAD_ARG;
+    if (n > kNoAllocMaxPoints) return NAV_BA_TOO_LARGE;
+    if (device < 0 || device >= kNoAllocMaxDevices || !gfx950_device(device)) return NAV_BA_NO_DEVICE;
+    if (n == 0) return NAV_BA_OK;
+    if (hipSetDevice(device) != hipSuccess) return NAV_BA_HIP_ERROR;
+    NoAllocSlot& slot = g_noalloc[device];
+    std::lock_guard<std::mutex> lock(slot.mu);
+    if (!slot.ptr) {  // first call on this device: its address of the array
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return NAV_BA_HIP_ERROR;
+        slot.ptr = static_cast<float2*>(p);
+    }
+    float* d = reinterpret_cast<float*>(slot.ptr);
+    const size_t bytes = sizeof(float) * 2 * (size_t)n;
+    int32_t st = NAV_BA_OK;
+    if (hipMemcpy(d, uv, bytes, hipMemcpyHostToDevice) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    if (st == NAV_BA_OK) st = launch_undistort(model, d, n, params, R, P, p_cols, d, nullptr);
+    if (st == NAV_BA_OK && hipMemcpy(uv_out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = NAV_BA_HIP_ERROR;
+    return st;
 }
