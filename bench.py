@@ -1,15 +1,20 @@
 #!/usr/bin/env python3
 """bench.py — bundle-adjustment LM throughput on one MI355X.
 
-One step = one full LM solve (nav_ba_run) of the workload, with the problem
-already resident in HBM (nav_ba_prepare runs before the timed region).  The
-headline is the plain-FP64 GPU path on the random-walk workload (60 keyframes,
-20 000 points, ~565 k observations, 20 LM iterations); the JSON line also
-carries observations/s, the mixed-precision path on the same problem (final
-cost relative to FP64), the per-stage roofline of the dominant kernel, and
-the FP64 CPU oracle timed on the host as the baseline.
+Headline (the metric's own comparison, at equal final cost): 512 uneven local
+windows (5-30 keyframes, 300-5000 points, 2-8 views per point, 10 LM
+iterations each) solved together by the batched FP64 path, against the same
+windows solved one at a time by the plain FP64 path.  One step = one batched
+solve of all windows (nav_ba_run) with the batch resident in HBM (prepare is
+timed apart).  `value` is window LM iterations/s; beside it observations/s,
+the per-window final-cost agreement with the plain path, the mixed-precision
+batch, the roofline of the batch's dominant kernel and the FP64 oracle on the
+host's cores.  Further legs: the single random-walk problem (60 keyframes,
+~565 k observations), the 300-keyframe loop (~3 M observations) against the
+FP64 oracle's solve, the differentiable cost, key-point undistortion, one
+window per call, and two-view initialisation.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -31,6 +36,7 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 matrix
 
 WORKLOAD_NAMES = {
     0: "circle-10kf-1500pt", 1: "randomwalk-60kf-20000pt", 2: "loop-300kf-150000pt",
+    3: "batched-512-windows",
 }
 
 
@@ -60,11 +66,11 @@ def stage_work(st, fp64=True):
     schur_flops = 108 * st["sum_k"] + 36 * st["sum_k"] + 216 * st["sum_pairs"]
     return {
         "lin_pt": ("hbm", lin_pt), "lin_cam": ("hbm", lin_cam), "cost": ("hbm", cost),
-        "schur_row": ("mfma", schur_flops),
+        "schur_row": ("fp64-valu", schur_flops),
     }
 
 
-STAGE_KERNEL = {"schur_row": "k_schur_row<double, true>", "lin_pt": "k_lin_pt<double>",
+STAGE_KERNEL = {"schur_row": "k_schur_row<double>", "lin_pt": "k_lin_pt<double>",
                 "lin_cam": "k_lin_cam<double>", "cost": "k_cost<double>"}
 
 
@@ -90,9 +96,9 @@ def roofline_entry(timing, st, fp64):
     if bound == "hbm":
         achieved = amount / avg_s / 1e9
         peak, unit = HBM_PEAK_GBS, "GB/s"
-    else:
+    else:  # the Schur rows run on the FP64 vector units (FP64 for both compute types)
         achieved = amount / avg_s / 1e12
-        peak, unit = (FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS), "TFLOP/s"
+        peak, unit = FP64_PEAK_TFLOPS, "TFLOP/s"
     return {
         "kernel": cand, "dominant_stage": dominant, "bound": bound, "achieved": achieved, "peak": peak,
         "unit": unit, "frac": achieved / peak, "traffic": None, "avg_launch_us": avg_s * 1e6,
@@ -118,25 +124,41 @@ def hbm_stage_rates(timing, st, fp64=True):
     return out
 
 
-def cpu_baseline(prob, opts):
-    from oracle import oracle  # checker and CPU baseline only
-    t0 = time.perf_counter()
-    r = oracle.solve(prob, opts)
-    dt = time.perf_counter() - t0
-    return {
-        "value": r.iterations / dt, "unit": "LM iterations/s", "cores": 1, "kind": "port",
-        "sample": f"one full solve of the same workload ({r.iterations} LM iterations, "
-                  f"{prob.n_obs} observations), FP64 oracle, 1 thread, {dt:.2f} s",
-        "obs_per_s": prob.n_obs * r.iterations / dt, "final_cost": r.final_cost,
-        "cpu": _cpu_model(), "host_cpus": os.cpu_count(),
-    }, r
+def _median_time(fn, reps):
+    """Median wall time of `reps` calls of fn (and the times)."""
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
 
 
-def batched_leg(n_windows, steps, warmup):
-    """Uneven local windows (5-30 keyframes, 300-5000 points, 2-8 views per
-    point, seed = window index): all windows in one batched solve, mixed and
-    FP64, against the same windows solved one at a time by the plain FP64
-    path (each window prepared before its timed run)."""
+def _sum_stats(stats):
+    out = {k: 0.0 for k in ("n_obs", "n_pt", "n_cam", "n_fc", "sum_k", "sum_pairs")}
+    for st in stats:
+        for k in out:
+            out[k] += st[k]
+    return out
+
+
+def _oracle_window(args):
+    """Oracle solve of uneven window `seed` (for the all-cores CPU baseline)."""
+    seed, n_iter = args
+    sys.path.insert(0, ROOT)
+    from nav24_amd import problems
+    from nav24_amd.abi import NAV_BA_FP64, workload_options
+    from oracle import oracle
+    o = workload_options(3)
+    o.precision = NAV_BA_FP64
+    r = oracle.solve(problems.workload(3, seed=seed), o)
+    return r.iterations
+
+
+def headline_leg(n_windows, steps, warmup, with_cpu):
+    """The metric's comparison: n_windows uneven windows in one batched FP64
+    solve against the plain FP64 path one window at a time, at equal final
+    cost; the mixed batch beside it."""
     import torch
     from nav24_amd import problems
     from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
@@ -144,100 +166,253 @@ def batched_leg(n_windows, steps, warmup):
 
     wins = [problems.workload(3, seed=w) for w in range(n_windows)]
     n_obs = np.array([w.n_obs for w in wins], np.float64)
-    out = {"workload": f"{n_windows} uneven windows (5-30 kf, 300-5000 pts)", "n_windows": n_windows,
-           "n_obs_total": int(n_obs.sum()), "n_cam_total": int(sum(w.n_cam for w in wins))}
-    res = {}
-    for name, prec in (("mixed", NAV_BA_MIXED), ("fp64", NAV_BA_FP64)):
-        o = workload_options(3)
-        o.precision = prec
-        s = GraphOptimHip(0)
-        s.prepare_batched(wins, o)
-        for _ in range(warmup):
-            s.run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            s.run()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / steps
-        r = s.fetch_batched()
-        s.close()
-        it = np.array([x.iterations for x in r], np.float64)
-        res[name] = r
-        out[name] = {"window_lm_iterations_per_s": round(float(it.sum()) / dt, 1),
-                     "obs_per_s": round(float((it * n_obs).sum()) / dt, 1),
-                     "ms_per_batch": round(dt * 1e3, 3), "lm_iterations_per_window": int(it.max())}
-    rel = np.array([abs(a.final_cost - b.final_cost) / b.final_cost for a, b in zip(res["mixed"], res["fp64"])])
-    # each window's own FP64 sensitivity: the same FP64 solve with every input
-    # point coordinate moved by one ulp (windows far from convergence after
-    # 10 iterations follow a different LM path under such a change)
-    import copy
-    pert = []
-    for w in wins:
-        q = copy.copy(w)
-        q.X = np.nextafter(w.X, np.inf)
-        pert.append(q)
-    o = workload_options(3)
-    o.precision = NAV_BA_FP64
+    st = _sum_stats([problem_stats(w) for w in wins])
+    o64 = workload_options(3)
+    o64.precision = NAV_BA_FP64
     s = GraphOptimHip(0)
-    rp = s.solve_batched(pert, o)
+    t0 = time.perf_counter()
+    s.prepare_batched(wins, o64)
+    prep = time.perf_counter() - t0
+    for _ in range(warmup):
+        s.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    r64 = s.fetch_batched()
+    it64 = np.array([x.iterations for x in r64], np.float64)
+    value = float(it64.sum()) / dt
+    # roofline of the dominant kernel, the Schur rows, with every window running
+    # (its own HIP events on the handle's stream, 10 launches at the initial
+    # linearisation, lambda 1e-4)
+    schur_us = s.time_schur_rows(1e-4, 10)
+    flops = stage_work(st)["schur_row"][1]
+    ach = flops / (schur_us * 1e-6) / 1e12
+    roof = {"kernel": "k_schur_row<double>", "bound": "fp64-valu", "achieved": round(ach, 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 5),
+            "traffic": measured_traffic(WORKLOAD_NAMES[3], "schur_row"), "avg_launch_us": round(schur_us, 2),
+            "algorithmic_flops_per_launch": flops,
+            "note": "FP64 vector ALU (no MFMA): 108 k + 36 k + 216 k(k+1)/2 flops per point of k free views"}
+    s.set_timing(True)
+    s.run()
+    stages = {k: round(v[0], 4) for k, v in s.timing().items() if v[1] > 0}
+    s.set_timing(False)
     s.close()
-    sens = np.array([abs(a.final_cost - b.final_cost) / b.final_cost for a, b in zip(rp, res["fp64"])])
-    c_mx = sum(x.final_cost for x in res["mixed"])
-    c_64 = sum(x.final_cost for x in res["fp64"])
-    out["mixed"]["final_cost_rel_vs_fp64"] = {
-        "max": float(rel.max()), "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
-        "windows_within_1e-5": int((rel <= 1e-5).sum()),
-        "windows_within_max(1e-5, 4 x own fp64 one-ulp sensitivity)": int((rel <= np.maximum(1e-5, 4 * sens)).sum()),
-        "summed_cost_rel": float(abs(c_mx - c_64) / c_64)}
-    out["fp64"]["one_ulp_point_sensitivity"] = {"windows_above_1e-5": int((sens > 1e-5).sum()),
-                                                "median": float(np.median(sens)), "max": float(sens.max())}
-    # plain FP64 path, one window at a time
-    o = workload_options(3)
-    o.precision = NAV_BA_FP64
+    # plain FP64 path, one window at a time (each prepared before its timed run)
     s = GraphOptimHip(0)
     tot = 0.0
     iters = 0
+    plain = []
     for w in wins:
-        s.prepare(w, o)
+        s.prepare(w, o64)
         s.run()  # warm-up of this window's graph
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s.run()
         torch.cuda.synchronize()
         tot += time.perf_counter() - t0
-        iters += s.fetch().iterations
+        r = s.fetch()
+        iters += r.iterations
+        plain.append(r)
     s.close()
-    out["plain_fp64_one_window_at_a_time"] = {"window_lm_iterations_per_s": round(iters / tot, 1),
-                                              "ms_total": round(tot * 1e3, 3)}
-    out["mixed_batched_speedup_vs_plain_fp64"] = round(
-        out["mixed"]["window_lm_iterations_per_s"] / out["plain_fp64_one_window_at_a_time"]["window_lm_iterations_per_s"], 2)
+    plain_rate = iters / tot
+    rel_bp = np.array([abs(a.final_cost - b.final_cost) / b.final_cost for a, b in zip(r64, plain)])
+    # the mixed batch
+    om = workload_options(3)
+    om.precision = NAV_BA_MIXED
+    s = GraphOptimHip(0)
+    s.prepare_batched(wins, om)
+    for _ in range(warmup):
+        s.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.run()
+    torch.cuda.synchronize()
+    dtm = (time.perf_counter() - t0) / steps
+    rm = s.fetch_batched()
+    s.close()
+    itm = np.array([x.iterations for x in rm], np.float64)
+    rel_m = np.array([abs(a.final_cost - b.final_cost) / b.final_cost for a, b in zip(rm, plain)])
+    out = {
+        "value": round(value, 1), "unit": "LM iterations/s", "ms_per_step": round(dt * 1e3, 3),
+        "config": {"workload": f"{n_windows} uneven windows (5-30 keyframes, 300-5000 points, 2-8 views per point),"
+                               " 10 LM iterations each, one batched solve", "n_windows": n_windows,
+                   "n_obs": int(n_obs.sum()), "n_cam": int(st["n_cam"]), "precision": "fp64",
+                   "huber_delta": "sqrt(5.991)", "lambda_init": 1e-4, "fixed_cameras_per_window": 2},
+        "obs_per_s": round(float((it64 * n_obs).sum()) / dt, 1),
+        "prepare_ms": round(prep * 1e3, 1),
+        "speedup_vs_plain_fp64": round(value / plain_rate, 2),
+        "plain_fp64_one_window_at_a_time": {"value": round(plain_rate, 1), "unit": "LM iterations/s",
+                                            "ms_total": round(tot * 1e3, 2)},
+        "equal_final_cost": {"windows": n_windows, "within_1e-9": int((rel_bp <= 1e-9).sum()),
+                             "within_1e-5": int((rel_bp <= 1e-5).sum()), "max_rel": float(rel_bp.max()),
+                             "median_rel": float(np.median(rel_bp)),
+                             "note": "batched FP64 against the plain FP64 path per window; the two factor each "
+                                     "window's system in different kernels (another order of FP64 additions)"},
+        "mixed_batched": {"value": round(float(itm.sum()) / dtm, 1), "unit": "LM iterations/s",
+                          "ms_per_step": round(dtm * 1e3, 3), "speedup_vs_plain_fp64":
+                              round(float(itm.sum()) / dtm / plain_rate, 2),
+                          "final_cost_rel_vs_plain_fp64": {"max": float(rel_m.max()), "median": float(np.median(rel_m)),
+                                                           "p99": float(np.quantile(rel_m, 0.99)),
+                                                           "windows_within_1e-5": int((rel_m <= 1e-5).sum())}},
+        "roofline": roof, "stages_ms_per_step_stage_timed": stages,
+    }
+    if with_cpu:
+        from oracle import oracle
+        sample = wins[:16]
+        done = {}
+
+        def one_core():
+            done["iters"] = sum(oracle.solve(w, o64).iterations for w in sample)
+        med, ts = _median_time(one_core, 5)
+        it_s = done["iters"]
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+        ncpu = max(1, min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        with ProcessPoolExecutor(max_workers=ncpu, mp_context=mp.get_context("spawn")) as ex:
+            all_it = sum(ex.map(_oracle_window, [(w, 10) for w in range(n_windows)], chunksize=8))
+        t_all = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(it_s / med, 1), "unit": "LM iterations/s", "cores": 1, "kind": "port",
+            "sample": f"windows 0-15 of the batch ({int(n_obs[:16].sum())} observations, {it_s} window LM "
+                      f"iterations) solved one after another by the FP64 oracle, median of 5 runs "
+                      f"({', '.join(f'{t:.2f}' for t in ts)} s)",
+            "all_cores": {"value": round(all_it / t_all, 1), "unit": "LM iterations/s", "cores": ncpu,
+                          "sample": f"all {n_windows} windows, one FP64 oracle solve each, {ncpu} host processes "
+                                    f"(oracle x {ncpu} cores), {t_all:.2f} s including process start"},
+            "cpu": _cpu_model(), "host_cpus": os.cpu_count()}
     return out
 
 
-def _pose_point_rmse(a, b):
-    """RMS rotation angle (rad), translation (m) and point (m) differences."""
-    Ra = a.T_cw[:, :9].reshape(-1, 3, 3)
-    Rb = b.T_cw[:, :9].reshape(-1, 3, 3)
+def _pose_point_rmse(a_T, b_T, a_X=None, b_X=None):
+    """RMS rotation angle (rad) and translation (m) differences of two pose
+    sets, and the RMS point difference (m) when points are given."""
+    Ra = a_T[:, :9].reshape(-1, 3, 3)
+    Rb = b_T[:, :9].reshape(-1, 3, 3)
     tr = np.einsum("nij,nij->n", Ra, Rb)  # trace(Ra Rb^T)
     ang = np.arccos(np.clip((tr - 1.0) / 2.0, -1.0, 1.0))
-    dt = np.linalg.norm(a.T_cw[:, 9:] - b.T_cw[:, 9:], axis=1)
-    dx = np.linalg.norm(a.X - b.X, axis=1)
-    return float(np.sqrt(np.mean(ang ** 2))), float(np.sqrt(np.mean(dt ** 2))), float(np.sqrt(np.mean(dx ** 2)))
+    dt = np.linalg.norm(a_T[:, 9:] - b_T[:, 9:], axis=1)
+    out = {"rotation_rad": float(np.sqrt(np.mean(ang ** 2))), "translation_m": float(np.sqrt(np.mean(dt ** 2)))}
+    if a_X is not None:
+        out["point_m"] = float(np.sqrt(np.mean(np.linalg.norm(a_X - b_X, axis=1) ** 2)))
+    return out
 
 
-def loop_leg(steps, warmup):
-    """300 keyframes on a 30 m loop, 150 000 points, 30 LM iterations: mixed
-    against FP64 on the GPU (final cost, pose and point RMSE)."""
+def single_leg(workload, steps, warmup, with_cpu):
+    """One problem (the 60-keyframe random walk): FP64 and mixed LM solves
+    resident in HBM, prepare and end-to-end nav_ba_solve times, per-stage
+    roofline, the factorisation stage, and the FP64 oracle on one core."""
+    import torch
+    from nav24_amd import problems
+    from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
+    from nav24_amd.ba import GraphOptimHip
+
+    prob = problems.workload(workload, seed=0)
+    opts = workload_options(workload)
+    opts.precision = NAV_BA_FP64
+    st = problem_stats(prob)
+    solver = GraphOptimHip(0)
+    t0 = time.perf_counter()
+    solver.prepare(prob, opts)
+    prep = time.perf_counter() - t0
+    for _ in range(warmup):
+        solver.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = solver.fetch()
+    iters = res.iterations
+    value = iters * steps / dt
+    solver.set_timing(True)
+    for _ in range(steps):
+        solver.run()
+    timing = solver.timing()
+    solver.set_timing(False)
+    per_step = {k: (v[0] / steps, v[1] / steps) for k, v in timing.items()}
+    roof = roofline_entry(per_step, st, True)
+    roof["achieved"] = round(roof["achieved"], 3)
+    roof["frac"] = round(roof["frac"], 5)
+    roof["traffic"] = measured_traffic(WORKLOAD_NAMES.get(workload, ""), roof["kernel"])
+    roof["hbm_kernels"] = hbm_stage_rates(per_step, st, True)
+    n = 6 * st["n_fc"]
+    chol_ms = timing.get("cholesky", (0.0, 0))[0] / steps
+    if chol_ms > 0 and res.trials > 0:
+        flops = n ** 3 / 3.0 + 2.0 * n * n
+        ach = flops / (chol_ms / res.trials / 1e3) / 1e12
+        roof["cholesky_stage"] = {"n": n, "flops_per_trial": flops, "us_per_trial": round(chol_ms / res.trials * 1e3, 2),
+                                  "achieved": round(ach, 4), "unit": "TFLOP/s",
+                                  "frac": round(ach / FP64_PEAK_TFLOPS, 6)}
+    e2e, _ = _median_time(lambda: solver.solve(prob, opts), 5)
+    out = {"workload": WORKLOAD_NAMES.get(workload, str(workload)), "n_cam": prob.n_cam, "n_pt": prob.n_pt,
+           "n_obs": prob.n_obs, "lm_iterations": iters, "trials": res.trials,
+           "value": round(value, 2), "unit": "LM iterations/s", "ms_per_solve": round(dt / steps * 1e3, 3),
+           "obs_per_s": round(prob.n_obs * value, 1), "prepare_ms": round(prep * 1e3, 2),
+           "end_to_end_solve_ms": round(e2e * 1e3, 2), "final_cost": res.final_cost, "roofline": roof}
+    solver.close()
+    mopts = workload_options(workload)
+    mopts.precision = NAV_BA_MIXED
+    ms = GraphOptimHip(0)
+    ms.prepare(prob, mopts)
+    for _ in range(warmup):
+        ms.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ms.run()
+    torch.cuda.synchronize()
+    mdt = time.perf_counter() - t0
+    mres = ms.fetch()
+    ms.close()
+    out["mixed"] = {"value": round(mres.iterations * steps / mdt, 2), "unit": "LM iterations/s",
+                    "ms_per_solve": round(mdt / steps * 1e3, 3),
+                    "final_cost_rel_vs_fp64": abs(mres.final_cost - res.final_cost) / res.final_cost,
+                    "speedup_vs_fp64": round((mres.iterations / mdt) / (iters / dt), 3)}
+    if with_cpu:
+        from oracle import oracle  # checker and CPU baseline only
+        holder = {}
+
+        def solve_once():
+            holder["r"] = oracle.solve(prob, opts)
+        med, ts = _median_time(solve_once, 3)
+        ref = holder["r"]
+        out["cpu_baseline"] = {"value": round(ref.iterations / med, 3), "unit": "LM iterations/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"the full solve ({ref.iterations} LM iterations, {prob.n_obs} observations),"
+                                         f" FP64 oracle, 1 thread, median of 3 ({', '.join(f'{t:.2f}' for t in ts)} s)"}
+        out["final_cost_rel_vs_oracle"] = abs(res.final_cost - ref.final_cost) / ref.final_cost
+        out["outlier_flags_identical"] = bool(np.array_equal(res.outlier, ref.outlier))
+    return out
+
+
+def loop_leg(steps, warmup, with_cpu):
+    """300 keyframes on a 30 m loop, 150 000 points, 30 LM iterations: FP64
+    and mixed on the GPU, each against the FP64 oracle's solve of the same
+    problem (tests/golden/loop_oracle.json: final cost, poses, every 50th
+    point), with that solve's own one-ulp sensitivity."""
     import torch
     from nav24_amd import problems
     from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
     from nav24_amd.ba import GraphOptimHip
 
     prob = problems.workload(2, seed=0)
+    st = problem_stats(prob)
     out = {"workload": WORKLOAD_NAMES[2], "n_cam": prob.n_cam, "n_pt": prob.n_pt, "n_obs": prob.n_obs}
+    fx = None
+    fx_path = os.path.join(ROOT, "tests", "golden", "loop_oracle.json")
+    if os.path.exists(fx_path):
+        with open(fx_path) as f:
+            fx = json.load(f)
+        if fx.get("n_obs") != prob.n_obs:
+            fx = None
     res = {}
-    for name, prec in (("mixed", NAV_BA_MIXED), ("fp64", NAV_BA_FP64)):
+    for name, prec in (("fp64", NAV_BA_FP64), ("mixed", NAV_BA_MIXED)):
         o = workload_options(2)
         o.precision = prec
         s = GraphOptimHip(0)
@@ -253,40 +428,49 @@ def loop_leg(steps, warmup):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         r = s.fetch()
+        if name == "fp64":
+            s.set_timing(True)
+            s.run()
+            out["hbm_kernels"] = hbm_stage_rates(s.timing(), st, True)
+            s.set_timing(False)
         s.close()
         res[name] = r
-        out[name] = {"lm_iterations_per_s": round(r.iterations / dt, 2), "obs_per_s": round(prob.n_obs * r.iterations / dt, 1),
-                     "ms_per_solve": round(dt * 1e3, 2), "lm_iterations": r.iterations, "trials": r.trials,
-                     "final_cost": r.final_cost, "prepare_s": round(prep, 3)}
-    # the problem's own sensitivity: FP64 from points moved by one ulp
-    from nav24_amd.abi import Problem
-    q = Problem(**{k: getattr(prob, k) for k in ("T_cw", "X", "obs_cam", "obs_pt", "obs_uv", "fx", "fy", "cx", "cy",
-                                                 "cam_fixed", "pt_fixed")})
-    q.X = np.nextafter(prob.X, np.inf)
-    o = workload_options(2)
-    o.precision = NAV_BA_FP64
-    s = GraphOptimHip(0)
-    rq = s.solve(q, o)
-    s.close()
+        entry = {"lm_iterations_per_s": round(r.iterations / dt, 2), "obs_per_s": round(prob.n_obs * r.iterations / dt, 1),
+                 "ms_per_solve": round(dt * 1e3, 2), "lm_iterations": r.iterations, "trials": r.trials,
+                 "final_cost": r.final_cost, "prepare_ms": round(prep * 1e3, 1)}
+        if fx is not None:
+            Xs = np.asarray(fx["X_sample"])
+            entry["vs_fp64_oracle"] = {
+                "final_cost_rel": abs(r.final_cost - fx["final_cost"]) / fx["final_cost"],
+                "rmse": _pose_point_rmse(r.T_cw, np.asarray(fx["T_cw"]), r.X[::fx["point_stride"]], Xs),
+                "point_rmse_sample": f"every {fx['point_stride']}th point ({len(Xs)})"}
+        out[name] = entry
+    if fx is not None:
+        out["oracle"] = {"final_cost": fx["final_cost"], "trials": fx["trials"],
+                         "one_ulp_sensitivity_final": fx["one_ulp_sensitivity"]["final"]}
     m, f = res["mixed"], res["fp64"]
-    out["fp64"]["one_ulp_point_change_rel"] = abs(rq.final_cost - f.final_cost) / f.final_cost
-    # stage-timed FP64 solve: HBM rate of the residual / Jacobian / accumulation kernels at 3 M observations
-    o = workload_options(2)
-    o.precision = NAV_BA_FP64
-    s = GraphOptimHip(0)
-    s.prepare(prob, o)
-    s.set_timing(True)
-    s.run()
-    out["fp64"]["hbm_kernels"] = hbm_stage_rates(s.timing(), problem_stats(prob), True)
-    s.close()
-    rot, trans, pts = _pose_point_rmse(m, f)
-    out["mixed"]["final_cost_rel_vs_fp64"] = abs(m.final_cost - f.final_cost) / f.final_cost
-    out["mixed"]["rmse_vs_fp64"] = {"rotation_rad": rot, "translation_m": trans, "point_m": pts}
+    out["mixed"]["final_cost_rel_vs_fp64_gpu"] = abs(m.final_cost - f.final_cost) / f.final_cost
     out["mixed"]["speedup_vs_fp64"] = round(out["mixed"]["lm_iterations_per_s"] / out["fp64"]["lm_iterations_per_s"], 3)
+    if with_cpu:
+        from oracle import oracle
+        o = workload_options(2)
+        o.precision = NAV_BA_FP64
+        o.n_iter = 1
+        holder = {}
+
+        def one_iter():
+            holder["r"] = oracle.solve(prob, o)
+        med, ts = _median_time(one_iter, 3)
+        r1 = holder["r"]
+        out["cpu_baseline"] = {"value": round(r1.iterations / med, 4), "unit": "LM iterations/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"the first LM iteration of the same problem ({r1.trials} trial, "
+                                         f"{prob.n_obs} observations), FP64 oracle, 1 thread, median of 3 "
+                                         f"({', '.join(f'{t:.2f}' for t in ts)} s)"}
     return out
 
 
-def diffcost_leg(n_problems, steps):
+def diffcost_leg(n_problems, steps, with_cpu=False):
     """Robust reprojection cost with radial-tangential distortion and its
     hand-written adjoint (poses, points, intrinsics, distortion) over a batch of
     20-keyframe, 5 000-point problems resident in HBM: forward (cost only) and
@@ -316,6 +500,20 @@ def diffcost_leg(n_problems, steps):
             out.setdefault(name, {})[key] = {"ms": round(dt_s * 1e3, 4), "obs_per_s": round(b.n_obs / dt_s, 1)}
             if grad:
                 res[name] = r
+    if with_cpu:
+        from oracle import cost_oracle as co
+
+        def oracle_all():
+            c0 = p0 = 0
+            for p in probs:
+                co.cost_and_grad(T[c0:c0 + p.n_cam], X[p0:p0 + p.n_pt], p.obs_cam, p.obs_pt, p.obs_uv, intr, True)
+                c0 += p.n_cam
+                p0 += p.n_pt
+        med, ts = _median_time(oracle_all, 3)
+        out["cpu_baseline"] = {"value": round(b.n_obs / med, 1), "unit": "observations/s (forward + backward)",
+                               "cores": 1, "kind": "port",
+                               "sample": f"all {len(probs)} problems, FP64 numpy oracle, median of 3 "
+                                         f"({', '.join(f'{t:.2f}' for t in ts)} s)"}
     rel = lambda a, r: float((a.double() - r).abs().max() / r.abs().max())
     out["fp32"]["rel_err_vs_fp64"] = {"cost": rel(res["fp32"].cost, res["fp64"].cost),
                                       "pose": rel(res["fp32"].g_pose, res["fp64"].g_pose),
@@ -610,10 +808,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=1, help="single-problem leg: generator workload")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--windows", type=int, default=512, help="batched leg: number of windows (0 = skip)")
+    ap.add_argument("--windows", type=int, default=512, help="headline: number of uneven windows in the batch")
     ap.add_argument("--loop", type=int, default=1, help="loop leg (300 keyframes, 150 000 points): 0 = skip")
     ap.add_argument("--diffcost", type=int, default=64,
                     help="differentiable-cost leg: number of distorted-camera problems (0 = skip)")
@@ -624,7 +821,7 @@ def main():
     ap.add_argument("--twoview", type=int, default=1000,
                     help="two-view RANSAC scoring leg: number of matches, 200 H and 200 F hypotheses (0 = skip)")
     ap.add_argument("--batched-only", action="store_true",
-                    help="profiling: run only the batched mixed solve (K timed runs) and print its timing")
+                    help="profiling: run only the batched solve (K timed runs) and print its timing")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="profiling (--batched-only): precision of the batched solve")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -635,112 +832,38 @@ def main():
               "sharding across GPUs is designed but not run.", file=sys.stderr)
         sys.exit(2)
 
-    import torch  # device synchronisation around the timed region
+    import torch  # noqa: F401  (device synchronisation inside the legs)
     if args.copy_mb > 0:
         copy_calibration(args.copy_mb, args.steps)
         return
     if args.batched_only:
         batched_profile(args.windows, args.steps, args.warmup, args.precision)
         return
-    from nav24_amd import problems
-    from nav24_amd.abi import NAV_BA_FP64, NAV_BA_MIXED, workload_options
-    from nav24_amd.ba import GraphOptimHip
-
-    prob = problems.workload(args.workload, seed=0)
-    opts = workload_options(args.workload)
-    opts.precision = NAV_BA_FP64
-    st = problem_stats(prob)
-    solver = GraphOptimHip(0)
-    solver.prepare(prob, opts)
-    for _ in range(args.warmup):
-        solver.run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        solver.run()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    res = solver.fetch()
-    iters = res.iterations
-    value = iters * args.steps / dt
-
-    solver.set_timing(True)
-    for _ in range(args.steps):
-        solver.run()
-    timing = solver.timing()
-    solver.set_timing(False)
-    per_step = {k: (v[0] / args.steps, v[1] / args.steps) for k, v in timing.items()}
-    roof = roofline_entry(per_step, st, True)
-    roof["achieved"] = round(roof["achieved"], 3)
-    roof["frac"] = round(roof["frac"], 5)
-    roof["traffic"] = measured_traffic(WORKLOAD_NAMES.get(args.workload, ""), roof["kernel"])
-    roof["hbm_kernels"] = hbm_stage_rates(per_step, st, True)
-    # the factorisation stage (latency-bound chain of tile launches), priced on FP64 flops
-    n = 6 * st["n_fc"]
-    trials = res.trials
-    chol_ms = timing.get("cholesky", (0.0, 0))[0] / args.steps
-    if chol_ms > 0 and trials > 0:
-        flops = n ** 3 / 3.0 + 2.0 * n * n
-        ach = flops / (chol_ms / trials / 1e3) / 1e12
-        roof["cholesky_stage"] = {"n": n, "flops_per_trial": flops, "us_per_trial": round(chol_ms / trials * 1e3, 2),
-                                  "achieved": round(ach, 4), "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 6)}
-
+    with_cpu = not args.no_cpu
+    head = headline_leg(args.windows, args.steps, args.warmup, with_cpu)
     line = {
-        "metric": METRIC, "value": round(value, 3), "unit": "LM iterations/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": WORKLOAD_NAMES.get(args.workload, str(args.workload)),
-                   "n_cam": prob.n_cam, "n_pt": prob.n_pt, "n_obs": prob.n_obs,
-                   "lm_iterations": iters, "precision": "fp64", "huber_delta": "sqrt(5.991)",
-                   "lambda_init": 1e-4, "fixed_cameras": 2},
-        "obs_per_s": round(prob.n_obs * value, 1),
-        "final_cost": res.final_cost, "trials": res.trials,
-        "roofline": roof,
+        "metric": METRIC, "value": head.pop("value"), "unit": head.pop("unit"), "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head.pop("ms_per_step"),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": head.pop("config"),
     }
-
-    if not args.no_mixed:
-        mopts = workload_options(args.workload)
-        mopts.precision = NAV_BA_MIXED
-        ms = GraphOptimHip(0)
-        ms.prepare(prob, mopts)
-        for _ in range(args.warmup):
-            ms.run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ms.run()
-        torch.cuda.synchronize()
-        mdt = time.perf_counter() - t0
-        mres = ms.fetch()
-        line["mixed"] = {
-            "value": round(mres.iterations * args.steps / mdt, 3), "unit": "LM iterations/s",
-            "ms_per_step": round(mdt / args.steps * 1e3, 3), "lm_iterations": mres.iterations,
-            "final_cost": mres.final_cost,
-            "final_cost_rel_vs_fp64": abs(mres.final_cost - res.final_cost) / res.final_cost,
-            "speedup_vs_fp64": round((mres.iterations / mdt) / (iters / dt), 3),
-        }
-        ms.close()
-
-    if args.windows > 0:
-        line["batched"] = batched_leg(args.windows, args.steps, args.warmup)
+    roof = head.pop("roofline")
+    cpu = head.pop("cpu_baseline", None)
+    line.update(head)
+    line["roofline"] = roof
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    line["single_problem"] = single_leg(args.workload, args.steps, args.warmup, with_cpu)
     if args.loop:
-        line["loop"] = loop_leg(max(1, min(args.steps, 3)), 1)
+        line["loop"] = loop_leg(max(1, min(args.steps, 3)), 1, with_cpu)
     if args.diffcost > 0:
-        line["diffcost"] = diffcost_leg(args.diffcost, max(args.steps, 5))
+        line["diffcost"] = diffcost_leg(args.diffcost, max(args.steps, 5), with_cpu)
     if args.undistort > 0:
-        line["undistort"] = undistort_leg(args.undistort, max(args.steps, 20), not args.no_cpu)
+        line["undistort"] = undistort_leg(args.undistort, max(args.steps, 20), with_cpu)
     if args.window_solves > 0:
-        line["window_solve"] = window_leg(args.window_solves, 10, not args.no_cpu)
+        line["window_solve"] = window_leg(args.window_solves, 10, with_cpu)
     if args.twoview > 0:
-        line["twoview"] = twoview_leg(args.twoview, 200, max(args.steps, 20), not args.no_cpu)
-
-    if not args.no_cpu:
-        cb, ref = cpu_baseline(prob, opts)
-        line["cpu_baseline"] = cb
-        line["final_cost_rel_vs_oracle"] = abs(res.final_cost - ref.final_cost) / ref.final_cost
-        line["outlier_flags_identical"] = bool(np.array_equal(res.outlier, ref.outlier))
-    solver.close()
+        line["twoview"] = twoview_leg(args.twoview, 200, max(args.steps, 20), with_cpu)
     print(json.dumps(line), flush=True)
 
 

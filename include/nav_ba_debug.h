@@ -70,6 +70,13 @@ int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk);
  * the problem had nothing to optimise). */
 int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk);
 
+/* After a prepare (one problem or a batch): linearise every window at its
+ * initial state, damping lambda, and time the Schur-row kernel over all
+ * windows: one warm-up launch, then `reps` launches back to back between two
+ * HIP events on the handle's stream; *avg_us = their average.  For roofline
+ * figures of that kernel with every window running. */
+int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t reps, double* avg_us);
+
 /* Host-only check of the structure a prepare builds (no device needed): the
  * flat graph arrays and the Schur work items at `chunk` observations per item
  * (0 = the default), every index against the buffer it addresses and every

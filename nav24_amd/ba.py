@@ -28,6 +28,7 @@ EXPORTED = [
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
     "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_check_layout",
+    "nav_ba_debug_time_schur_rows",
 ]
 
 STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
@@ -86,6 +87,8 @@ def lib() -> C.CDLL:
                                                 C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                                 C.c_char_p, C.c_int32]
         L.nav_ba_debug_check_layout.restype = C.c_int32
+        L.nav_ba_debug_time_schur_rows.argtypes = [H, C.c_double, C.c_int32, C.POINTER(C.c_double)]
+        L.nav_ba_debug_time_schur_rows.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -210,6 +213,13 @@ class GraphOptimHip:
         """Observations per Schur work item for the next prepare: 0 automatic
         (tuned at prepare on large problems), or a multiple of 64 up to 4096."""
         self._check(lib().nav_ba_debug_set_schur_chunk(self._h, int(chunk)))
+
+    def time_schur_rows(self, lam: float = 1e-4, reps: int = 10) -> float:
+        """Average device time (us) of the Schur-row kernel over all windows of
+        the prepared problem at its initial linearisation (every window running)."""
+        v = C.c_double(0.0)
+        self._check(lib().nav_ba_debug_time_schur_rows(self._h, lam, int(reps), C.byref(v)), allow_empty=True)
+        return v.value
 
     @property
     def schur_chunk(self) -> int:

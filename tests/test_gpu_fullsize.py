@@ -1,0 +1,162 @@
+"""Full-size parity of the batch and differentiable-cost workloads:
+  the 512 uneven windows (5-30 keyframes, 300-5000 points, 10 LM iterations)
+  solved in one batch in FP64, every window against the oracle's own solve of
+  it (pool of host processes); and the 64 distorted-camera problems of 20
+  keyframes and 5 000 points, cost and gradients against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from nav24_amd import problems
+from nav24_amd.abi import NAV_BA_FP64, workload_options
+from oracle_pool import solve_windows
+from parity_util import rel, say
+
+pytestmark = pytest.mark.gpu
+
+N_WINDOWS = 512
+
+
+def _hist_rel(g, r):
+    r = np.asarray(r)
+    n = min(len(g), len(r))
+    return float(np.max(np.abs(np.asarray(g[:n]) - r[:n]) / np.abs(r[:n])))
+
+
+def test_batch_of_512_windows_fp64_matches_oracle(solver):
+    """Per window, the single-problem bar: the same iterations, trials and stop
+    reason, final cost and cost history within 1e-9, identical flags.  A
+    window that misses it is held to its own one-ulp sensitivity s, the
+    largest change of the oracle's solve over five twins of the window (every
+    point coordinate one ulp up, or down; every camera translation one ulp up,
+    or down; points up and translations down): final cost and history within
+    max(1e-9, 4 s).  Where s exceeds 1e-6 the window is chaotic: the oracle's
+    own twins take other LM paths (other trial counts), and so may the GPU;
+    there the bar is max(1e-5, 10 s) and the schedule may differ.  Every
+    such window's flags must equal the oracle's flags at the GPU's final state
+    (observations within 1e-6 of the threshold aside)."""
+    from oracle import oracle
+    wins = [problems.workload(3, seed=s) for s in range(N_WINDOWS)]
+    o = workload_options(3)
+    o.precision = NAV_BA_FP64
+    say(f"batched FP64: {N_WINDOWS} windows, {sum(w.n_obs for w in wins)} observations")
+    got = solver.solve_batched(wins, o)
+    ref = solve_windows(range(N_WINDOWS), o)
+    exact, widened = 0, []
+    for i, (g, r) in enumerate(zip(got, ref)):
+        ok = (rel(g.final_cost, r["final_cost"]) <= 1e-9 and _hist_rel(g.cost_history, r["cost_history"]) <= 1e-9
+              and (g.iterations, g.trials, g.stop_reason) == (r["iterations"], r["trials"], r["stop_reason"])
+              and g.outlier.tobytes() == r["outlier"])
+        if ok:
+            exact += 1
+        else:
+            widened.append(i)
+    say(f"  {exact} of {N_WINDOWS} windows meet the 1e-9 bar with identical schedules and flags; "
+        f"{len(widened)} are checked against their own sensitivity")
+    if not widened:
+        return
+    twins = [solve_windows(widened, o, perturb=k) for k in ("X+", "X-", "t+", "t-", "X+t-")]
+    n_chaotic = 0
+    for j, i in enumerate(widened):
+        g, r = got[i], ref[i]
+        s_fin = max(rel(t[j]["final_cost"], r["final_cost"]) for t in twins)
+        s_hist = max(_hist_rel(t[j]["cost_history"], r["cost_history"]) for t in twins)
+        fin, hist = rel(g.final_cost, r["final_cost"]), _hist_rel(g.cost_history, r["cost_history"])
+        same = (g.iterations, g.trials, g.stop_reason) == (r["iterations"], r["trials"], r["stop_reason"])
+        _, c2, fl = oracle.cost(wins[i], o, g.T_cw, g.X)
+        near = np.abs(c2 - o.outlier_chi2) <= 1e-6 * o.outlier_chi2
+        flags_ok = np.array_equal(g.outlier[~near], fl[~near])
+        say(f"  window {i}: final {fin:.1e} (one ulp {s_fin:.1e}), history {hist:.1e} (one ulp {s_hist:.1e}), "
+            f"trials {g.trials}/{r['trials']}, flags vs oracle {int((g.outlier != np.frombuffer(r['outlier'], np.uint8)).sum())}, "
+            f"at the GPU state {int((g.outlier != fl).sum())} ({int(near.sum())} near the threshold)")
+        chaotic = max(s_fin, s_hist) > 1e-6
+        k, floor = (10, 1e-5) if chaotic else (4, 1e-9)
+        assert fin <= max(floor, k * s_fin), f"window {i}: final cost"
+        assert hist <= max(floor, k * s_hist), f"window {i}: cost history"
+        assert same or chaotic, f"window {i}: LM schedule differs on a window that is not chaotic"
+        assert flags_ok, f"window {i}: flags"
+        n_chaotic += chaotic
+    say(f"  {n_chaotic} of the {len(widened)} are chaotic (one ulp moves their cost by more than 1e-6)")
+
+
+torch = pytest.importorskip("torch")
+
+
+def test_differentiable_cost_64_problems_full_size():
+    """64 problems of 20 keyframes and 5 000 points with radial-tangential
+    distortion (3.2 M observations): FP64 cost and gradients within 1e-11 of
+    the oracle, FP32 within 2e-3 of FP64."""
+    from nav24_amd import diffcost
+    from nav24_amd.abi import NAV_BA_NO_DEVICE
+    from oracle import cost_oracle as co
+    import ctypes
+    h = ctypes.c_void_p()
+    if diffcost.lib().nav_ba_cost_create(ctypes.byref(h), 0) == NAV_BA_NO_DEVICE:
+        pytest.skip("no gfx950 device")
+    diffcost.lib().nav_ba_cost_destroy(h)
+    probs = [problems.workload(4, seed=s) for s in range(64)]
+    T = np.concatenate([p.T_cw for p in probs])
+    X = np.concatenate([p.X for p in probs])
+    intr = np.array([probs[0].fx, probs[0].fy, probs[0].cx, probs[0].cy, *probs[0].dist])
+    say(f"differentiable cost: 64 problems, {sum(p.n_obs for p in probs)} observations")
+    b = diffcost.CostBatch(probs)
+    dev = torch.device("cuda:0")
+    r64 = b.eval(*(torch.tensor(a, device=dev) for a in (T, X, intr)))
+    r32 = b.eval(*(torch.tensor(a, device=dev, dtype=torch.float32) for a in (T, X, intr)))
+    costs, gps, gxs, gi = [], [], [], np.zeros(8)
+    c0 = p0 = 0
+    for p in probs:
+        c, gp, gx, g8 = co.cost_and_grad(T[c0:c0 + p.n_cam], X[p0:p0 + p.n_pt], p.obs_cam, p.obs_pt, p.obs_uv,
+                                         intr, True)
+        costs.append(c); gps.append(gp); gxs.append(gx); gi += g8
+        c0 += p.n_cam; p0 += p.n_pt
+    ref = (np.array(costs), np.concatenate(gps), np.concatenate(gxs), gi)
+    for name, g, g32, r in zip(("cost", "pose", "point", "intr"),
+                               (r64.cost, r64.g_pose, r64.g_X, r64.g_intr),
+                               (r32.cost, r32.g_pose, r32.g_X, r32.g_intr), ref):
+        e64 = np.abs(g.cpu().numpy() - r).max() / np.abs(r).max()
+        e32 = np.abs(g32.double().cpu().numpy() - r).max() / np.abs(r).max()
+        say(f"  {name}: fp64 {e64:.2e}, fp32 {e32:.2e}")
+        assert e64 <= 1e-11, name
+        assert e32 <= 2e-3, name
+    b.close()
+
+
+def test_loop_full_size_fp64_against_oracle(solver, oracle_mod):
+    """The 300-keyframe loop at full size (150 000 points, 2.96 M observations,
+    30 LM iterations) against the FP64 oracle's solve of it
+    (tests/golden/loop_oracle.json, two minutes of oracle time per solve).
+    The loop is chaotic: hundreds of observations lie behind their cameras
+    from the start, and a point crossing a camera plane during the large
+    steps of iterations 7-9 makes the path depend on the last bits (one ulp of
+    input moves the cost by at most 3e-9 up to iteration 7, by 1e-6 at
+    iteration 8, by up to 2e-4 at the end).  So each iteration's cost is held to
+    max(1e-9, 4 x that iteration's one-ulp sensitivity), the final cost
+    likewise; the flags must be the oracle's flags at the GPU's final state."""
+    import json
+    import os
+    from golden.make_loop_fixture import problem_digest
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "loop_oracle.json")
+    with open(path) as f:
+        fx = json.load(f)
+    prob = problems.workload(2, seed=0)
+    assert problem_digest(prob) == fx["problem_sha1"], "the generator no longer makes the fixture's problem"
+    o = workload_options(2)
+    o.precision = NAV_BA_FP64
+    say(f"loop at full size: {prob.n_obs} observations, FP64, 30 iterations")
+    got = solver.solve(prob, o)
+    ref_h = np.array(fx["cost_history"])
+    sens_h = np.array(fx["one_ulp_sensitivity"]["history"])
+    n = min(len(got.cost_history), len(ref_h))
+    r = np.abs(got.cost_history[:n] - ref_h[:n]) / ref_h[:n]
+    bar = np.maximum(1e-9, 4 * sens_h[:n])
+    say("  per iteration rel " + " ".join(f"{x:.0e}" for x in r))
+    say("  per iteration bar " + " ".join(f"{x:.0e}" for x in bar))
+    assert np.all(r <= bar)
+    fb = max(1e-9, 4 * fx["one_ulp_sensitivity"]["final"])
+    say(f"  final gpu {got.final_cost!r} oracle {fx['final_cost']!r} rel {rel(got.final_cost, fx['final_cost']):.2e} "
+        f"(bar {fb:.1e}); trials {got.trials}/{fx['trials']}")
+    assert rel(got.final_cost, fx["final_cost"]) <= fb
+    _, c2, fl = oracle_mod.cost(prob, o, got.T_cw, got.X)
+    near = np.abs(c2 - o.outlier_chi2) <= 1e-6 * o.outlier_chi2
+    say(f"  flags at the GPU state: {int((got.outlier != fl).sum())} differ, {int(near.sum())} near the threshold")
+    assert np.array_equal(got.outlier[~near], fl[~near])
