@@ -72,15 +72,17 @@ def stage_work(st, fp64=True):
 
 STAGE_KERNEL = {"schur_row": "k_schur_row<double>", "lin_pt": "k_lin_pt<double>",
                 "lin_cam": "k_lin_cam<double>", "cost": "k_cost<double>"}
+SCHUR_KERNEL = {1: "k_schur_row<double>", 2: "k_schur_row_pc<double>"}
 
 
 def measured_traffic(workload_name, stage):
-    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+    """HBM bytes per launch of a kernel (a stage name of STAGE_KERNEL or a
+    kernel name) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload (copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload_name}_fp64.json")
     try:
         with open(path) as f:
-            return json.load(f)["bytes_per_launch"].get(STAGE_KERNEL.get(stage, ""))
+            return json.load(f)["bytes_per_launch"].get(STAGE_KERNEL.get(stage, stage))
     except (OSError, ValueError, KeyError):
         return None
 
@@ -190,9 +192,10 @@ def headline_leg(n_windows, steps, warmup, with_cpu):
     schur_us = s.time_schur_rows(1e-4, 10)
     flops = stage_work(st)["schur_row"][1]
     ach = flops / (schur_us * 1e-6) / 1e12
-    roof = {"kernel": "k_schur_row<double>", "bound": "fp64-valu", "achieved": round(ach, 3),
+    kname = SCHUR_KERNEL.get(s.schur_kernel, "k_schur_row<double>")
+    roof = {"kernel": kname, "bound": "fp64-valu", "achieved": round(ach, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 5),
-            "traffic": measured_traffic(WORKLOAD_NAMES[3], "schur_row"), "avg_launch_us": round(schur_us, 2),
+            "traffic": measured_traffic(WORKLOAD_NAMES[3], kname), "avg_launch_us": round(schur_us, 2),
             "algorithmic_flops_per_launch": flops,
             "note": "FP64 vector ALU (no MFMA): 108 k + 36 k + 216 k(k+1)/2 flops per point of k free views"}
     s.set_timing(True)
@@ -337,6 +340,8 @@ def single_leg(workload, steps, warmup, with_cpu):
     solver.set_timing(False)
     per_step = {k: (v[0] / steps, v[1] / steps) for k, v in timing.items()}
     roof = roofline_entry(per_step, st, True)
+    if roof["kernel"] == "schur_row":
+        roof["kernel"] = SCHUR_KERNEL.get(solver.schur_kernel, STAGE_KERNEL["schur_row"])
     roof["achieved"] = round(roof["achieved"], 3)
     roof["frac"] = round(roof["frac"], 5)
     roof["traffic"] = measured_traffic(WORKLOAD_NAMES.get(workload, ""), roof["kernel"])

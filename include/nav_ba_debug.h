@@ -60,15 +60,29 @@ int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode);
 
 /* Observations per Schur work item for the next nav_ba_prepare on this
  * handle: 0 = automatic (problems of at least 131072 observations time the
- * candidates 128, 256, ..., 4096 on their initial linearisation at prepare
- * and keep the fastest, remembered per device, precision and problem shape for
- * the life of the process; smaller problems use 256), or a multiple of 64 up
- * to 4096 to force it.  Any size gives the same reduced system up to the
- * order of FP64 additions. */
+ * candidates 128, 256, ..., 4096 with each allowed Schur-row kernel on their
+ * initial linearisation at prepare and keep the fastest, remembered per
+ * device, precision, kernel setting and problem shape for the life of the
+ * process; smaller problems use 256), or a multiple of 64 up to 4096 to force
+ * it.  Any size gives the same reduced system up to the order of FP64
+ * additions. */
 int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk);
 /* The chunk size the last nav_ba_prepare used (0 before a prepare, or when
  * the problem had nothing to optimise). */
 int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk);
+
+/* Schur-row kernel for the next nav_ba_prepare on this handle: 0 = automatic
+ * (timed with the chunk sizes above 131072 observations; below, producer/
+ * consumer for batches and rounds for one problem), 1 = rounds of 256
+ * observations with the lanes bound to column blocks per round, 2 =
+ * producer/consumer (two waves linearise rounds of 128 observations while two
+ * waves, bound to column blocks for the whole work item, sum the previous
+ * round's pairs).  Both give the same reduced system up to the order of FP64
+ * additions; each run of either gives the same bits. */
+int32_t nav_ba_debug_set_schur_kernel(nav_ba_handle* h, int32_t kernel);
+/* The kernel the last nav_ba_prepare used: 1 or 2 (0 before a prepare, or
+ * when the problem had nothing to optimise). */
+int32_t nav_ba_debug_get_schur_kernel(const nav_ba_handle* h, int32_t* kernel);
 
 /* After a prepare (one problem or a batch): linearise every window at its
  * initial state, damping lambda, and time the Schur-row kernel over all
@@ -78,15 +92,16 @@ int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk);
 int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t reps, double* avg_us);
 
 /* Host-only check of the structure a prepare builds (no device needed): the
- * flat graph arrays and the Schur work items at `chunk` observations per item
+ * flat graph arrays and the Schur work items of `kernel` (1 or 2, as
+ * nav_ba_debug_set_schur_kernel) at `chunk` observations per item
  * (0 = the default), every index against the buffer it addresses and every
  * Schur round's pairs against a direct enumeration of the row's pairs.
  * Returns NAV_BA_OK when consistent (pair entries and rounds in the outputs,
  * either may be NULL), -1 with the first violation in msg, or a status for
  * invalid input. */
 int32_t nav_ba_debug_check_layout(const nav_ba_problem* problems, int32_t n_windows, const nav_ba_options* o,
-                                  int32_t batched, int32_t chunk, int64_t* n_pairs, int32_t* n_rounds,
-                                  char* msg, int32_t msg_len);
+                                  int32_t batched, int32_t chunk, int32_t kernel, int64_t* n_pairs,
+                                  int32_t* n_rounds, char* msg, int32_t msg_len);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ EXPORTED = [
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
     "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_check_layout",
-    "nav_ba_debug_time_schur_rows",
+    "nav_ba_debug_time_schur_rows", "nav_ba_debug_set_schur_kernel", "nav_ba_debug_get_schur_kernel",
 ]
 
 STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
@@ -83,9 +83,13 @@ def lib() -> C.CDLL:
         L.nav_ba_debug_set_schur_chunk.restype = C.c_int32
         L.nav_ba_debug_get_schur_chunk.argtypes = [H, C.POINTER(C.c_int32)]
         L.nav_ba_debug_get_schur_chunk.restype = C.c_int32
+        L.nav_ba_debug_set_schur_kernel.argtypes = [H, C.c_int32]
+        L.nav_ba_debug_set_schur_kernel.restype = C.c_int32
+        L.nav_ba_debug_get_schur_kernel.argtypes = [H, C.POINTER(C.c_int32)]
+        L.nav_ba_debug_get_schur_kernel.restype = C.c_int32
         L.nav_ba_debug_check_layout.argtypes = [C.POINTER(NavBaProblem), C.c_int32, C.POINTER(NavBaOptions),
-                                                C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
-                                                C.c_char_p, C.c_int32]
+                                                C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64),
+                                                C.POINTER(C.c_int32), C.c_char_p, C.c_int32]
         L.nav_ba_debug_check_layout.restype = C.c_int32
         L.nav_ba_debug_time_schur_rows.argtypes = [H, C.c_double, C.c_int32, C.POINTER(C.c_double)]
         L.nav_ba_debug_time_schur_rows.restype = C.c_int32
@@ -214,6 +218,19 @@ class GraphOptimHip:
         (tuned at prepare on large problems), or a multiple of 64 up to 4096."""
         self._check(lib().nav_ba_debug_set_schur_chunk(self._h, int(chunk)))
 
+    def set_schur_kernel(self, kernel: int) -> None:
+        """Schur-row kernel for the next prepare: 0 automatic, 1 rounds of 256
+        observations (lanes bound per round), 2 producer/consumer (lanes bound
+        per work item)."""
+        self._check(lib().nav_ba_debug_set_schur_kernel(self._h, int(kernel)))
+
+    @property
+    def schur_kernel(self) -> int:
+        """Schur-row kernel chosen by the last prepare (1 or 2; 0 if none)."""
+        v = C.c_int32(0)
+        self._check(lib().nav_ba_debug_get_schur_kernel(self._h, C.byref(v)))
+        return v.value
+
     def time_schur_rows(self, lam: float = 1e-4, reps: int = 10) -> float:
         """Average device time (us) of the Schur-row kernel over all windows of
         the prepared problem at its initial linearisation (every window running)."""
@@ -238,16 +255,17 @@ class GraphOptimHip:
         return S[: n.value, : n.value], g[: n.value]
 
 
-def check_layout(windows: Sequence[Problem], opts: Options, batched: bool = False, chunk: int = 0) -> dict:
+def check_layout(windows: Sequence[Problem], opts: Options, batched: bool = False, chunk: int = 0,
+                 kernel: int = 1) -> dict:
     """Host-only check of the structure prepare builds for these windows (no
-    device needed): raises BAError with the first violation, else returns the
-    Schur pair-entry and round counts."""
+    device needed) with Schur-row kernel `kernel` (1 or 2): raises BAError with
+    the first violation, else returns the Schur pair-entry and round counts."""
     arr = (NavBaProblem * len(windows))(*[w.c_struct() for w in windows])
     os_ = opts.c_struct()
     n_pairs, n_rounds = C.c_int64(0), C.c_int32(0)
     msg = C.create_string_buffer(256)
     st = lib().nav_ba_debug_check_layout(arr, len(windows), C.byref(os_), int(bool(batched)), int(chunk),
-                                         C.byref(n_pairs), C.byref(n_rounds), msg, 256)
+                                         int(kernel), C.byref(n_pairs), C.byref(n_rounds), msg, 256)
     if st != NAV_BA_OK:
         raise BAError(st, msg.value.decode())
     return {"pairs": n_pairs.value, "rounds": n_rounds.value}

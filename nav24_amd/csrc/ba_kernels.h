@@ -20,6 +20,8 @@ constexpr int kTile = 256;         // observations per point-aligned linearisati
 constexpr int kChunk = 1024;       // camera-major observations per Hpp work item
 constexpr int kSchurChunk = 256;   // camera-major observations per Schur work item (default; tuned per problem)
 constexpr int kMaxSlots = 112;     // S-row column blocks per work item: 78 KiB of LDS, two workgroups per CU
+constexpr int kPcRound = 128;      // producer/consumer Schur rows: observations per round (two producer waves)
+constexpr int kPcCons = 128;       // ... and consumer lanes (two waves), each bound to one slot per item
 constexpr int kNB = 64;            // tiled dense Cholesky tile
 constexpr int kSmallMaxN = 180;    // per-window in-LDS Cholesky: at most 30 free cameras
 constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Cholesky is possible up to this size
@@ -85,9 +87,12 @@ struct DevProblem {
     const int* row_sitem_ptr;  // [n_fc + 1]
     const int* row_nchunks;    // [n_fc]
     const int* sitem_order;    // [n_sitems] dispatch order: estimated work, heaviest first
-    const uint8_t* rnd_lane_slot;  // [rounds][kBlock] slot of each lane (slots' lanes contiguous), 255 idle
+    int schur_pc;              // 0: rounds of kBlock, lanes bound per round; 1: producer/consumer rounds
+    const uint8_t* rnd_lane_slot;  // slot of each lane (slots' lanes contiguous), 255 idle: [rounds][kBlock],
+                                   // or (producer/consumer) [items][kPcCons] for the consumer lanes
     const int* rnd_pair0;      // [rounds] first pair entry of the round
-    const int4* rnd_rows;      // [rounds] pair rows of each wave (64 entries a row)
+    const int4* rnd_rows;      // [rounds] pair rows of each wave (64 entries a row); producer/consumer: .x, .y
+                               // are the two consumer waves
     const uint8_t* rnd_group;  // [rounds] lanes per group (2 or 4): a slot's lanes are whole aligned groups
     const int2* pairs;         // {observation b, row of observation a in the round}; {-1, 0} pads a lane's list
     int n_sitems;

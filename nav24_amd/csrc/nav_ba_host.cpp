@@ -171,6 +171,8 @@ struct nav_ba_handle {
     int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks
     int schur_chunk = 0;  // observations per Schur work item: 0 = default or tuned
     int schur_chunk_used = 0;
+    int schur_kernel = 0;  // Schur row kernel: 0 = default or tuned, 1 = rounds of kBlock, 2 = producer/consumer
+    bool schur_pc_used = false;
     StageTimer timer;
     int* hdone = nullptr;      // pinned: windows finished
     char* stage = nullptr;     // pinned staging of a prepare's uploads (grown on demand)
@@ -390,11 +392,12 @@ int validate(const nav_ba_problem* p) {
 // Work items of one chunk size (camera-major observations per item) and their pair lists.
 struct SchurItems {
     int chunk = 0, max_ns = 1;
+    bool pc = false;                 // producer/consumer kernel (lanes bound per item)
     long long pofs = 0;
     std::vector<int> sf, sa0, sa1, sg, sns, srnd0, row_sitem_ptr, row_nch, sorder, rnd_pair0;
     std::vector<long long> spofs;
-    std::vector<uint8_t> lane_slot;  // [rounds][kBlock]
-    std::vector<uint8_t> rnd_group;  // [rounds] lanes per group
+    std::vector<uint8_t> lane_slot;  // [rounds][kBlock], or [items][kPcCons] for the producer/consumer kernel
+    std::vector<uint8_t> rnd_group;  // [rounds] lanes per group (rounds of kBlock only)
     std::vector<int4> rnd_rows;      // [rounds]
     std::vector<int2> pairs;
 };
@@ -437,9 +440,10 @@ int assign_lanes(const std::vector<std::vector<int2>>& bucket, int ns, int lanes
 // nearly equal shares, one per lane; a slot's lanes are contiguous.  Entry:
 // {b, a's row in the round}.  Per wave, row j holds every lane's pair j
 // ({-1, 0} past a lane's share).  False when the pair entries exceed int32.
-bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
+bool build_schur_items(const SchurGraph& G, int chunk, bool pc, SchurItems& S) {
     S = SchurItems{};
     S.chunk = chunk;
+    S.pc = pc;
     const int lanes = kBlock;
     const int sg_n = kMaxSlots;
     S.row_sitem_ptr.push_back(0);
@@ -468,6 +472,54 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
     std::vector<int> slotmap(G.nc, -1);
     std::vector<std::vector<int2>> bucket;
     std::vector<int> nlane, nlane2, lane_s, lane_beg, lane_len;
+    // the pairs of the round of camera-major observations [kb, ke), by slot
+    auto round_pairs = [&](int kb, int ke, int ns) {
+        for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
+        for (int k = kb; k < ke; ++k) {
+            const int a = G.cam_obs[k];
+            const int q = G.o_pt[a];
+            if (!G.pt_free[q]) continue;
+            for (int b = G.cam_obs_b0[k]; b < G.pt_ptr[q + 1]; ++b) {
+                if (b == a) continue;
+                const int sl = slotmap[G.o_cam[b]];
+                if (sl < 0) continue;
+                bucket[sl].push_back(int2{b, k - kb});
+            }
+        }
+    };
+    // pair rows of a round whose lanes (lane_s / lane_beg / lane_len) are laid
+    // out, for waves w0 .. w0 + nw - 1 of kWave lanes from lane l0
+    auto emit_rows = [&](int l0, int nw, int rows[4]) -> bool {
+        size_t tot = 0;
+        for (int wv = 0; wv < 4; ++wv) rows[wv] = 0;
+        for (int wv = 0; wv < nw; ++wv) {
+            for (int l = 0; l < kWave; ++l) rows[wv] = std::max(rows[wv], lane_len[l0 + wv * kWave + l]);
+            tot += (size_t)kWave * rows[wv];
+        }
+        if (S.pairs.size() + tot > (size_t)0x7fffffff) return false;
+        S.rnd_pair0.push_back((int)S.pairs.size());
+        S.rnd_rows.push_back(int4{rows[0], rows[1], rows[2], rows[3]});
+        for (int wv = 0; wv < nw; ++wv)
+            for (int j = 0; j < rows[wv]; ++j)
+                for (int l = 0; l < kWave; ++l) {
+                    const int t = l0 + wv * kWave + l;
+                    S.pairs.push_back(j < lane_len[t] ? bucket[lane_s[t]][lane_beg[t] + j] : int2{-1, 0});
+                }
+        return true;
+    };
+    // a slot's pairs of one round split into contiguous, nearly equal shares
+    // over its m lanes from lane t
+    auto split = [&](int sl, int m, int t) {
+        const int c = (int)bucket[sl].size();
+        for (int u = 0, st = 0; u < m; ++u, ++t) {
+            const int len = c / m + (u < c % m ? 1 : 0);
+            lane_s[t] = sl;
+            lane_beg[t] = st;
+            lane_len[t] = len;
+            st += len;
+        }
+    };
+    std::vector<std::vector<int>> rcount;  // producer/consumer: [round][slot] pair counts of the item
     for (size_t it = 0; it < S.sf.size(); ++it) {
         const int f = S.sf[it], gi = S.sg[it], ns = S.sns[it];
         const int col0 = G.colptr[f] + gi * sg_n;
@@ -475,19 +527,59 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
         if (bucket.size() < (size_t)ns) bucket.resize(ns);
         S.srnd0.push_back((int)S.rnd_rows.size());
         long long cost = 0;
-        for (int kb = S.sa0[it]; kb < S.sa1[it]; kb += lanes) {
-            for (int sl = 0; sl < ns; ++sl) bucket[sl].clear();
-            for (int k = kb; k < std::min(kb + lanes, S.sa1[it]); ++k) {
-                const int a = G.cam_obs[k];
-                const int q = G.o_pt[a];
-                if (!G.pt_free[q]) continue;
-                for (int b = G.cam_obs_b0[k]; b < G.pt_ptr[q + 1]; ++b) {
-                    if (b == a) continue;
-                    const int sl = slotmap[G.o_cam[b]];
-                    if (sl < 0) continue;
-                    bucket[sl].push_back(int2{b, k - kb});
-                }
+        if (pc) {
+            // consumer lanes bound per item: one lane per slot with pairs, then one
+            // at a time to the slot whose lanes carry the most pair rows summed over
+            // the item's rounds
+            const int nr = (S.sa1[it] - S.sa0[it] + kPcRound - 1) / kPcRound;
+            rcount.assign(nr, std::vector<int>(ns, 0));
+            std::vector<long long> tot(ns, 0);
+            for (int r = 0; r < nr; ++r) {
+                const int kb = S.sa0[it] + r * kPcRound;
+                round_pairs(kb, std::min(kb + kPcRound, S.sa1[it]), ns);
+                for (int sl = 0; sl < ns; ++sl) { rcount[r][sl] = (int)bucket[sl].size(); tot[sl] += rcount[r][sl]; }
             }
+            auto load = [&](int sl, int m) {
+                long long v = 0;
+                for (int r = 0; r < nr; ++r) v += (rcount[r][sl] + m - 1) / m;
+                return v;
+            };
+            nlane.assign(ns, 0);
+            int used = 0;
+            std::priority_queue<std::pair<long long, int>> pq;
+            for (int sl = 0; sl < ns; ++sl)
+                if (tot[sl] > 0) { nlane[sl] = 1; ++used; pq.push({load(sl, 1), -sl}); }
+            while (used < kPcCons && !pq.empty() && pq.top().first > nr) {
+                const int sl = -pq.top().second;
+                pq.pop();
+                ++nlane[sl];
+                ++used;
+                pq.push({load(sl, nlane[sl]), -sl});
+            }
+            lane_s.assign(kPcCons, 255);
+            for (int sl = 0, t = 0; sl < ns; ++sl)
+                for (int u = 0; u < nlane[sl]; ++u) lane_s[t++] = sl;
+            for (int t = 0; t < kPcCons; ++t) S.lane_slot.push_back((uint8_t)lane_s[t]);
+            for (int r = 0; r < nr; ++r) {
+                const int kb = S.sa0[it] + r * kPcRound;
+                round_pairs(kb, std::min(kb + kPcRound, S.sa1[it]), ns);
+                lane_beg.assign(kPcCons, 0);
+                lane_len.assign(kPcCons, 0);
+                for (int sl = 0, t = 0; sl < ns; ++sl) {
+                    if (nlane[sl] > 0) split(sl, nlane[sl], t);
+                    t += nlane[sl];
+                }
+                int rows[4];
+                if (!emit_rows(0, kPcCons / kWave, rows)) return false;
+                S.rnd_group.push_back(1);
+                cost += (long long)kWave * std::max(std::max(rows[0], rows[1]), 2) + 64;
+            }
+            for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = -1;
+            scost[it] = cost;
+            continue;
+        }
+        for (int kb = S.sa0[it]; kb < S.sa1[it]; kb += lanes) {
+            round_pairs(kb, std::min(kb + lanes, S.sa1[it]), ns);
             int grp = 4;
             const int worst4 = assign_lanes(bucket, ns, lanes, 4, nlane);
             if (assign_lanes(bucket, ns, lanes, 2, nlane2) < worst4) {
@@ -501,29 +593,12 @@ bool build_schur_items(const SchurGraph& G, int chunk, SchurItems& S) {
             lane_beg.assign(lanes, 0);
             lane_len.assign(lanes, 0);
             for (int sl = 0, t = 0; sl < ns; ++sl) {
-                const int c = (int)bucket[sl].size(), m = nlane[sl];
-                for (int u = 0, st = 0; u < m; ++u, ++t) {
-                    const int len = c / m + (u < c % m ? 1 : 0);
-                    lane_s[t] = sl;
-                    lane_beg[t] = st;
-                    lane_len[t] = len;
-                    st += len;
-                }
+                split(sl, nlane[sl], t);
+                t += nlane[sl];
             }
-            int rows[4] = {0, 0, 0, 0};
-            for (int wv = 0; wv < lanes / kWave; ++wv)
-                for (int l = 0; l < kWave; ++l) rows[wv] = std::max(rows[wv], lane_len[wv * kWave + l]);
-            if (S.pairs.size() + (size_t)kWave * (rows[0] + rows[1] + rows[2] + rows[3]) > (size_t)0x7fffffff)
-                return false;
-            S.rnd_pair0.push_back((int)S.pairs.size());
-            S.rnd_rows.push_back(int4{rows[0], rows[1], rows[2], rows[3]});
+            int rows[4];
+            if (!emit_rows(0, lanes / kWave, rows)) return false;
             for (int t = 0; t < lanes; ++t) S.lane_slot.push_back((uint8_t)lane_s[t]);
-            for (int wv = 0; wv < lanes / kWave; ++wv)
-                for (int j = 0; j < rows[wv]; ++j)
-                    for (int l = 0; l < kWave; ++l) {
-                        const int t = wv * kWave + l;
-                        S.pairs.push_back(j < lane_len[t] ? bucket[lane_s[t]][lane_beg[t] + j] : int2{-1, 0});
-                    }
             const int rmax = std::max(std::max(rows[0], rows[1]), std::max(rows[2], rows[3]));
             cost += (long long)kWave * rmax + 64;
         }
@@ -554,32 +629,38 @@ int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
     if ((st = alloc(h, (void**)&h->W.spart, sizeof(double) * std::max<long long>(S.pofs, 1))) != NAV_BA_OK) return st;
     P.n_sitems = (int)S.sf.size();
     P.max_ns = S.max_ns;
+    P.schur_pc = S.pc ? 1 : 0;
     h->schur_chunk_used = S.chunk;
+    h->schur_pc_used = S.pc;
     return NAV_BA_OK;
 }
 
 // Process-wide memory of tuned chunk sizes, per device, precision and problem shape.
 struct TuneKey {
-    int device, mixed, n_win, nc, np, no, nfc;
+    int device, mixed, n_win, nc, np, no, nfc, kernel;
     bool operator<(const TuneKey& o) const {
-        return std::tie(device, mixed, n_win, nc, np, no, nfc) <
-               std::tie(o.device, o.mixed, o.n_win, o.nc, o.np, o.no, o.nfc);
+        return std::tie(device, mixed, n_win, nc, np, no, nfc, kernel) <
+               std::tie(o.device, o.mixed, o.n_win, o.nc, o.np, o.no, o.nfc, o.kernel);
     }
 };
+struct TuneChoice {
+    int chunk;
+    bool pc;
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, int> g_tuned;
+std::map<TuneKey, TuneChoice> g_tuned;
 
-bool cached_chunk(const TuneKey& k, int* chunk) {
+bool cached_choice(const TuneKey& k, TuneChoice* c) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(k);
     if (it == g_tuned.end()) return false;
-    *chunk = it->second;
+    *c = it->second;
     return true;
 }
 
-void remember_chunk(const TuneKey& k, int chunk) {
+void remember_choice(const TuneKey& k, TuneChoice c) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[k] = chunk;
+    g_tuned[k] = c;
 }
 
 // Device time of the Schur stage (rows + reduction) of the uploaded items, best of kTuneReps.
@@ -603,11 +684,16 @@ int time_schur(nav_ba_handle* h, float* best_ms) {
     return NAV_BA_OK;
 }
 
-// Pick the chunk size of the Schur items by timing each candidate on the
-// problem itself, linearised at its initial state.  `S` holds the items of
-// the default size, uploaded from allocation slot `mark` on; it returns
-// holding (and with the device holding) the fastest candidate's items.
-int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems& S) {
+// Pick the kernel and chunk size of the Schur items by timing candidates on
+// the problem itself, linearised at its initial state: every chunk size with
+// the default kernel, then the other kernel at the best size.  `S` holds the
+// default items, uploaded from allocation slot `mark` on; it returns holding
+// (and with the device holding) the fastest candidate's items.  Chunk sizes
+// above the longest camera row give the same items as the longest row's size
+// and are skipped.  `kernel` (1 or 2, also the default kernel then) and
+// `chunk_only` (> 0) limit the candidates to one kernel or one size; 0
+// allows all.
+int tune_schur_items(nav_ba_handle* h, const SchurGraph& G, int mark, int kernel, int chunk_only, SchurItems& S) {
     hipStream_t s = h->stream;
     const DevProblem& P = h->P;
     HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
@@ -621,18 +707,36 @@ int tune_schur_chunk(nav_ba_handle* h, const SchurGraph& G, int mark, SchurItems
     float best_ms;
     if ((st = time_schur(h, &best_ms)) != NAV_BA_OK) return st;
     const int def = S.chunk;
+    const bool def_pc = S.pc;
+    int longest = 0;
+    for (int f = 0; f < G.nfc; ++f) longest = std::max(longest, G.cam_ptr[f + 1] - G.cam_ptr[f]);
     SchurItems C;
-    for (int chunk : kTuneChunks) {
-        if (chunk == def) continue;
-        if (!build_schur_items(G, chunk, C)) continue;
+    auto trial = [&](int chunk, bool pc) -> int {
+        if (!build_schur_items(G, chunk, pc, C)) return NAV_BA_OK;
         h->nbuf = mark;
-        if ((st = upload_schur_items(h, C)) != NAV_BA_OK) return st;
+        int st2;
+        if ((st2 = upload_schur_items(h, C)) != NAV_BA_OK) return st2;
+        if ((mixed(h) ? prepare_schur_row<float>(h->P) : prepare_schur_row<double>(h->P)) != 0) return NAV_BA_OK;
         float ms;
-        if ((st = time_schur(h, &ms)) != NAV_BA_OK) return st;
+        if ((st2 = time_schur(h, &ms)) != NAV_BA_OK) return st2;
         if (ms < kTuneMargin * best_ms) { best_ms = ms; std::swap(S, C); }
+        return NAV_BA_OK;
+    };
+    // the sizes with the default kernel, then the other kernel at the best size
+    if (kernel == 0 || (kernel == 2) == def_pc) {
+        int prev = 0;
+        for (int chunk : kTuneChunks) {
+            if (prev >= longest) break;  // the same items as the previous size
+            prev = chunk;
+            if (chunk == def || (chunk_only > 0 && chunk != chunk_only)) continue;
+            if ((st = trial(chunk, def_pc)) != NAV_BA_OK) return st;
+        }
     }
+    if (kernel == 0 && (st = trial(S.chunk, !def_pc)) != NAV_BA_OK) return st;
     h->nbuf = mark;
     if ((st = upload_schur_items(h, S)) != NAV_BA_OK) return st;
+    if ((mixed(h) ? prepare_schur_row<float>(h->P) : prepare_schur_row<double>(h->P)) != 0)
+        return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
     HIPTRY(h, hipStreamSynchronize(s));
     return NAV_BA_OK;
 }
@@ -924,10 +1028,14 @@ std::string check_layout(const HostLayout& L, const SchurItems& S) {
         const int b0 = L.cam_obs_b0[k];
         if (b0 > a || b0 < L.pt_ptr[L.o_pt[a]] || L.o_cam[b0] != L.o_cam[a]) return bad("cam_obs_b0");
     }
-    const int lanes = kBlock;
+    // rounds of kBlock observations over kBlock lanes bound per round, or of
+    // kPcRound observations with kPcCons consumer lanes bound per item
+    const int lanes = S.pc ? kPcCons : kBlock;
+    const int rlen = S.pc ? kPcRound : kBlock;
     if ((int)S.sf.size() != (int)S.srnd0.size()) return bad("item round table");
     const size_t nrnd = S.rnd_rows.size();
-    if (S.lane_slot.size() != nrnd * lanes || S.rnd_group.size() != nrnd || S.rnd_pair0.size() != nrnd)
+    if (S.lane_slot.size() != (S.pc ? S.sf.size() : nrnd) * lanes || S.rnd_group.size() != nrnd ||
+        S.rnd_pair0.size() != nrnd)
         return bad("round tables");
     std::vector<int> slotmap(nc, -1);
     std::vector<long long> seen;
@@ -938,22 +1046,23 @@ std::string check_layout(const HostLayout& L, const SchurItems& S) {
         const int col0 = L.colptr[f] + S.sg[it] * kMaxSlots;
         if (col0 + ns > L.colptr[f + 1]) return bad("item slots");
         for (int sl = 0; sl < ns; ++sl) slotmap[L.free_cam[L.colcam[col0 + sl]]] = sl;
-        const int nr = (a1 - a0 + lanes - 1) / lanes;
+        const int nr = (a1 - a0 + rlen - 1) / rlen;
         if (S.srnd0[it] + nr > (int)nrnd) return bad("item rounds");
+        if (it + 1 < S.sf.size() && S.srnd0[it + 1] != S.srnd0[it] + nr) return bad("item rounds");
         for (int r = 0; r < nr; ++r) {
-            const int rnd = S.srnd0[it] + r, kb = a0 + r * lanes;
-            const int grp = S.rnd_group[rnd];
-            if (grp != 2 && grp != 4) return bad("lane group");
+            const int rnd = S.srnd0[it] + r, kb = a0 + r * rlen;
+            const int grp = S.pc ? 1 : S.rnd_group[rnd];
+            if (!S.pc && grp != 2 && grp != 4) return bad("lane group");
             // expected pairs of the round, as (b, row) keys
             std::vector<long long> want;
-            for (int k = kb; k < std::min(kb + lanes, a1); ++k) {
+            for (int k = kb; k < std::min(kb + rlen, a1); ++k) {
                 const int a = L.cam_obs[k], q = L.o_pt[a];
                 if (!L.pt_free[q]) continue;
                 for (int b = L.cam_obs_b0[k]; b < L.pt_ptr[q + 1]; ++b)
                     if (b != a && slotmap[L.o_cam[b]] >= 0) want.push_back((long long)b * 512 + (k - kb));
             }
             // the lanes' slots: contiguous per slot, aligned whole groups
-            const uint8_t* ls = &S.lane_slot[(size_t)rnd * lanes];
+            const uint8_t* ls = &S.lane_slot[(S.pc ? it : (size_t)rnd) * lanes];
             for (int t = 0; t < lanes; ++t) {
                 if (ls[t] != 255 && ls[t] >= ns) return bad("lane slot");
                 if (t > 0 && ls[t] != ls[t - 1]) {
@@ -965,6 +1074,8 @@ std::string check_layout(const HostLayout& L, const SchurItems& S) {
             }
             const int4 rw = S.rnd_rows[rnd];
             const int rowsv[4] = {rw.x, rw.y, rw.z, rw.w};
+            if (rw.x < 0 || rw.y < 0 || rw.z < 0 || rw.w < 0 || (S.pc && (rw.z != 0 || rw.w != 0)))
+                return bad("round rows");
             seen.clear();
             size_t e = (size_t)S.rnd_pair0[rnd];
             for (int wv = 0; wv < lanes / kWave; ++wv)
@@ -975,7 +1086,7 @@ std::string check_layout(const HostLayout& L, const SchurItems& S) {
                         if (pe.x < 0) continue;
                         const int t = wv * kWave + l;
                         if (ls[t] == 255) return bad("pair on an idle lane");
-                        if (pe.x >= no || pe.y < 0 || pe.y >= lanes || kb + pe.y >= a1) return bad("pair index");
+                        if (pe.x >= no || pe.y < 0 || pe.y >= rlen || kb + pe.y >= a1) return bad("pair index");
                         if (slotmap[L.o_cam[pe.x]] != ls[t]) return bad("pair camera is not the lane's slot");
                         if (L.o_pt[pe.x] != L.o_pt[L.cam_obs[kb + pe.y]]) return bad("pair point");
                         seen.push_back((long long)pe.x * 512 + pe.y);
@@ -1017,12 +1128,13 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
 
     // ---- Schur work items and pair lists at the default chunk size (tuned below)
     const SchurGraph G = L.graph();
-    const TuneKey key{h->device, mixed(h) ? 1 : 0, n_win, nc, np, no, nfc};
-    int chunk0 = h->schur_chunk > 0 ? h->schur_chunk : kSchurChunk;
-    bool tune = h->schur_chunk == 0 && nfc > 0 && no >= kTuneMinObs;
-    if (tune && cached_chunk(key, &chunk0)) tune = false;  // this shape was tuned before on this device
+    const TuneKey key{h->device, mixed(h) ? 1 : 0, n_win, nc, np, no, nfc, h->schur_kernel};
+    TuneChoice choice{h->schur_chunk > 0 ? h->schur_chunk : kSchurChunk,
+                      h->schur_kernel == 2 || (h->schur_kernel == 0 && batched)};
+    bool tune = (h->schur_chunk == 0 || h->schur_kernel == 0) && nfc > 0 && no >= kTuneMinObs;
+    if (tune && cached_choice(key, &choice)) tune = false;  // this shape was tuned before on this device
     SchurItems SI;
-    if (!build_schur_items(G, chunk0, SI))
+    if (!build_schur_items(G, choice.chunk, choice.pc, SI))
         return fail(h, NAV_BA_TOO_LARGE, "more than 2^31 Schur pair entries");
 
     // ---- device upload
@@ -1105,8 +1217,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     if (batched && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
     launch_fill_system(P, h->W, h->stream);
     if (tune) {
-        if ((st = tune_schur_chunk(h, G, schur_mark, SI)) != NAV_BA_OK) return st;
-        remember_chunk(key, SI.chunk);
+        if ((st = tune_schur_items(h, G, schur_mark, h->schur_kernel, h->schur_chunk, SI)) != NAV_BA_OK) return st;
+        remember_choice(key, TuneChoice{SI.chunk, SI.pc});
     }
     if ((st = capture_step(h)) != NAV_BA_OK) return st;
     h->prepared = true;
@@ -1364,6 +1476,18 @@ int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk) {
     return NAV_BA_OK;
 }
 
+int32_t nav_ba_debug_set_schur_kernel(nav_ba_handle* h, int32_t kernel) {
+    if (!h || kernel < 0 || kernel > 2) return NAV_BA_BAD_ARG;
+    h->schur_kernel = kernel;
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_get_schur_kernel(const nav_ba_handle* h, int32_t* kernel) {
+    if (!h || !kernel) return NAV_BA_BAD_ARG;
+    *kernel = h->prepared && !h->all_empty ? (h->schur_pc_used ? 2 : 1) : 0;
+    return NAV_BA_OK;
+}
+
 int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t reps, double* avg_us) {
     if (!h || !avg_us || reps <= 0) return NAV_BA_BAD_ARG;
     if (!h->prepared) return fail(h, NAV_BA_NOT_PREPARED, "debug call before nav_ba_prepare");
@@ -1395,8 +1519,8 @@ int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t re
 }
 
 int32_t nav_ba_debug_check_layout(const nav_ba_problem* problems, int32_t n_windows, const nav_ba_options* o,
-                                  int32_t batched, int32_t chunk, int64_t* n_pairs, int32_t* n_rounds,
-                                  char* msg, int32_t msg_len) {
+                                  int32_t batched, int32_t chunk, int32_t kernel, int64_t* n_pairs,
+                                  int32_t* n_rounds, char* msg, int32_t msg_len) {
     auto say = [&](const std::string& m) {
         if (msg && msg_len > 0) {
             std::strncpy(msg, m.c_str(), (size_t)msg_len - 1);
@@ -1405,6 +1529,7 @@ int32_t nav_ba_debug_check_layout(const nav_ba_problem* problems, int32_t n_wind
     };
     say("");
     if (chunk < 0 || chunk > 4096 || chunk % kWave != 0) { say("chunk"); return NAV_BA_BAD_ARG; }
+    if (kernel < 1 || kernel > 2) { say("kernel"); return NAV_BA_BAD_ARG; }
     std::string err;
     int st = check_args(problems, n_windows, o, batched != 0, err);
     if (st != NAV_BA_OK) { say(err); return st; }
@@ -1412,7 +1537,7 @@ int32_t nav_ba_debug_check_layout(const nav_ba_problem* problems, int32_t n_wind
     if ((st = build_layout(problems, n_windows, o, batched != 0, L, err)) != NAV_BA_OK) { say(err); return st; }
     if (L.all_empty) { say("empty"); return NAV_BA_EMPTY; }
     SchurItems S;
-    if (!build_schur_items(L.graph(), chunk > 0 ? chunk : kSchurChunk, S)) {
+    if (!build_schur_items(L.graph(), chunk > 0 ? chunk : kSchurChunk, kernel == 2, S)) {
         say("more than 2^31 Schur pair entries");
         return NAV_BA_TOO_LARGE;
     }

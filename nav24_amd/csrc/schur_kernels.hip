@@ -59,21 +59,53 @@ struct SchurLayout {
 };
 static_assert(kStageStride * (kBlock / 2) <= kWbuf * kBlock, "flush staging fits in the observation buffer");
 
-// Phase A for observation k of camera cf (camera-major index): W and the
-// point into row wb, diagonal pair and coefficient into the registers.
-template <typename T>
-__device__ __forceinline__ void schur_phase_a(const DevProblem& P, const CamRT* __restrict__ cam,
-                                              const D4* __restrict__ X, const double* __restrict__ Hl, int cf,
-                                              int k, double lambda, bool g0, double* wb, double coef[6],
-                                              double dsum[21]) {
-    const int p = P.c_pt[k];
-    if (!P.pt_free[p]) return;
-    double Rf[9], tf[3];
-    load_cam(cam, cf, Rf, tf);
+// Inputs of phase A for one observation: its point's state and Hll / bl and
+// its measurement, loaded ahead of use (every load needs only the point index).
+struct PaIn {
+    double x, y, z, u, v, h[9];
+    int live;  // observation in range and on a free point
+    __device__ __forceinline__ const double* hl() const { return h; }
+};
+// The same with Hll / bl read where they are used (fewer live registers).
+struct PaLazy {
+    double x, y, z, u, v;
+    const double* h;
+    int live;
+    __device__ __forceinline__ const double* hl() const { return h; }
+};
+
+__device__ __forceinline__ void pa_load_lazy(const DevProblem& P, const D4* __restrict__ X,
+                                             const double* __restrict__ Hl, int k, int p, PaLazy& in) {
+    in.live = P.pt_free[p];
     const D4 xp = X[p];
+    in.x = xp.x; in.y = xp.y; in.z = xp.z;
+    in.u = P.c_uv[2 * (size_t)k];
+    in.v = P.c_uv[2 * (size_t)k + 1];
+    in.h = Hl + 9 * (size_t)p;
+}
+
+__device__ __forceinline__ void pa_load(const DevProblem& P, const D4* __restrict__ X,
+                                        const double* __restrict__ Hl, int k, int p, PaIn& in) {
+    in.live = 0;
+    if (p < 0) return;
+    in.live = P.pt_free[p];
+    const D4 xp = X[p];
+    in.x = xp.x; in.y = xp.y; in.z = xp.z;
+    in.u = P.c_uv[2 * (size_t)k];
+    in.v = P.c_uv[2 * (size_t)k + 1];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) in.h[i] = Hl[9 * (size_t)p + i];
+}
+
+// Phase A for one observation of camera (Rf, tf): W and the point into row
+// wb, diagonal pair and coefficient into the registers.
+template <typename T, typename In>
+__device__ __forceinline__ void schur_phase_a(const DevProblem& P, const double Rf[9], const double tf[3],
+                                              const In& in, double lambda, bool g0, double* wb, double coef[6],
+                                              double dsum[21]) {
+    if (!in.live) return;
     EdgeLin<T> La;
-    const double sa = edge_linearize(Rf, tf, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, P.c_uv[2 * (size_t)k],
-                                     P.c_uv[2 * (size_t)k + 1], La);
+    const double sa = edge_linearize(Rf, tf, in.x, in.y, in.z, P.fx, P.fy, P.cx, P.cy, in.u, in.v, La);
     const double wa = (double)huber_weight<T>(sa, P);  // as k_lin_pt, without a gather of w
     double JXa[6];
 #pragma unroll
@@ -86,8 +118,8 @@ __device__ __forceinline__ void schur_phase_a(const DevProblem& P, const CamRT* 
         for (int j = 0; j < 3; ++j)  // J_xi is zero at (0, 4) and (1, 3)
             Ha[3 * i + j] = i == 4 ? A1 * JXa[3 + j] : i == 3 ? A0 * JXa[j] : A0 * JXa[j] + A1 * JXa[3 + j];
     }
-    const double* hp = Hl + 9 * (size_t)p;
     double Dp[6];
+    const double* hp = in.hl();
     point_dinv(hp, lambda, Dp);
     const double d00 = Dp[0], d01 = Dp[1], d02 = Dp[2], d11 = Dp[3], d12 = Dp[4], d22 = Dp[5];
     double W[18];
@@ -100,7 +132,7 @@ __device__ __forceinline__ void schur_phase_a(const DevProblem& P, const CamRT* 
     }
 #pragma unroll
     for (int e = 0; e < 18; ++e) wb[e] = W[e];
-    wb[18] = xp.x; wb[19] = xp.y; wb[20] = xp.z;
+    wb[18] = in.x; wb[19] = in.y; wb[20] = in.z;
     if (g0) {
         // Dinv bl of the point (the right-hand side's point term)
         const double b0 = hp[6], b1 = hp[7], b2 = hp[8];
@@ -120,18 +152,16 @@ __device__ __forceinline__ void schur_phase_a(const DevProblem& P, const CamRT* 
 }
 
 // Phase B: the lane's pairs (entry rows of stride kWave from pl, nrow rows; e0,
-// e1 and their weights already loaded) into sacc, with the camera (R, t).
+// e1 and the weight w0 of e0 already loaded) into sacc, with the camera (R, t).
+// Each load has a pair's work to land: at pair j the weight of entry j + 1
+// and the entry j + 2 are issued, and used one pair later.
 template <typename T>
 __device__ __forceinline__ void schur_pairs(const DevProblem& P, const T* __restrict__ w, const double* wbuf,
-                                            const int2* pl, int nrow, int2 e0, int2 e1, double w0, double w1,
+                                            const int2* pl, int nrow, int2 e0, int2 e1, double w0,
                                             const double Rb[9], const double tb[3], double sacc[36]) {
     for (int j = 0; j < nrow && e0.x >= 0; ++j) {
-        int2 e2 = int2{-1, 0};
-        double w2 = 1.0;
-        if (j + 2 < nrow) {
-            e2 = pl[(size_t)(j + 2) * kWave];
-            if (P.robust && e2.x >= 0) w2 = (double)w[e2.x];
-        }
+        const double w1 = P.robust ? (double)w[e1.x >= 0 ? e1.x : 0] : 1.0;
+        const int2 e2 = j + 2 < nrow ? pl[(size_t)(j + 2) * kWave] : int2{-1, 0};
         const double* wb = wbuf + e0.y * kWbuf;
         EdgeLin<T> Lb;
         edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
@@ -161,17 +191,18 @@ __device__ __forceinline__ void schur_pairs(const DevProblem& P, const T* __rest
                 else a = fma(M1[rr], B1[c], fma(M0[rr], B0[c], a));
             }
         e0 = e1; w0 = w1;
-        e1 = e2; w1 = w2;
+        e1 = e2;
     }
 }
 
-// The sum of an aligned group of G lanes (2 or 4), in every lane of the group.
+// The sum of an aligned group of G lanes (2 or 4, the same in the whole
+// workgroup), in every lane of the group.
 __device__ __forceinline__ void group_sum(double sacc[36], int G) {
 #pragma unroll
-    for (int e = 0; e < 36; ++e) {
-        double v = sacc[e] + dpp_move<0xB1>(sacc[e]);  // quad_perm [1,0,3,2]: lane pairs
-        if (G == 4) v += dpp_move<0x4E>(v);            // quad_perm [2,3,0,1]: pairs of pairs
-        sacc[e] = v;
+    for (int e = 0; e < 36; ++e) sacc[e] += dpp_move<0xB1>(sacc[e]);  // quad_perm [1,0,3,2]: lane pairs
+    if (G == 4) {
+#pragma unroll
+        for (int e = 0; e < 36; ++e) sacc[e] += dpp_move<0x4E>(sacc[e]);  // quad_perm [2,3,0,1]: pairs of pairs
     }
 }
 
@@ -198,7 +229,7 @@ __device__ __forceinline__ void flush_slots(double* acc, const double* stage, co
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT* __restrict__ cam,
+__global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const CamRT* __restrict__ cam,
                                                       const D4* __restrict__ X, const T* __restrict__ w,
                                                       const double* __restrict__ Hl, const LMDev* lm,
                                                       double* __restrict__ spart) {
@@ -239,24 +270,27 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
         const int4 rows = P.rnd_rows[rnd];
         const int nrow = wid == 0 ? rows.x : wid == 1 ? rows.y : wid == 2 ? rows.z : rows.w;
         const int rofs = wid == 0 ? 0 : wid == 1 ? rows.x : wid == 2 ? rows.x + rows.y : rows.x + rows.y + rows.z;
-        const int G = P.rnd_group[rnd];
+        const int G = __builtin_amdgcn_readfirstlane(P.rnd_group[rnd]);
         const bool busy = sl != 255 && nrow > 0;
         const int2* pl = P.pairs + (size_t)P.rnd_pair0[rnd] + (size_t)rofs * kWave + lane;
         int2 e0 = int2{-1, 0}, e1 = int2{-1, 0};
-        double w0 = 1.0, w1 = 1.0;
+        double w0 = 1.0;
         double Rb[9], tb[3];
         if (busy) {
             load_cam(cam, P.free_cam[P.colcam[col0 + sl]], Rb, tb);
             e0 = pl[0];
             if (nrow > 1) e1 = pl[kWave];
-            if (P.robust) {
-                if (e0.x >= 0) w0 = (double)w[e0.x];
-                if (e1.x >= 0) w1 = (double)w[e1.x];
-            }
+            if (P.robust && e0.x >= 0) w0 = (double)w[e0.x];
         }
         // ---- phase A: one lane per observation of camera f
         const int k = a0 + r * kBlock + tid;
-        if (k < a1) schur_phase_a<T>(P, cam, X, Hl, cf, k, lambda, g0, wbuf + tid * kWbuf, coef, dsum);
+        if (k < a1) {
+            double Rf[9], tf[3];
+            load_cam(cam, cf, Rf, tf);
+            PaLazy in;
+            pa_load_lazy(P, X, Hl, k, P.c_pt[k], in);
+            schur_phase_a<T>(P, Rf, tf, in, lambda, g0, wbuf + tid * kWbuf, coef, dsum);
+        }
         lslot[tid] = sl;
         for (int s = tid; s < ns; s += kBlock) lcount[s] = 0;
         __syncthreads();
@@ -270,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
 #pragma unroll
         for (int e = 0; e < 36; ++e) sacc[e] = 0.0;
         if (busy)
-            schur_pairs<T>(P, w, wbuf, pl, nrow, e0, e1, w0, w1, Rb, tb, sacc);
+            schur_pairs<T>(P, w, wbuf, pl, nrow, e0, e1, w0, Rb, tb, sacc);
         group_sum(sacc, G);
         __syncthreads();  // the buffer's reads done; lane ranges set
         if (tid < ns && lcount[tid] > 0) lcount[tid] -= lfirst[tid];
@@ -317,22 +351,207 @@ __global__ __launch_bounds__(kBlock) void k_schur_row(DevProblem P, const CamRT*
     for (int i = tid; i < ns * 36; i += kBlock) out[i] = acc[i];
 }
 
-size_t schur_row_smem(const DevProblem& P) { return SchurLayout(P.max_ns).total; }
+// ---- Producer/consumer variant: the item's lanes split by role, slots bound per item.
+// Waves 0-1 (producers) run phase A on rounds of kPcRound observations, one
+// per lane, into one of two W buffers; waves 2-3 (consumers) are bound to one
+// slot each for the whole item and sum the pairs of the previous round from
+// the other buffer, so the two phases overlap and one barrier per round
+// separates them.  Each consumer lane keeps its slot's sum in registers
+// across the rounds; the item ends with one flush: every consumer lane's
+// row is staged in LDS and each (slot, entry) adds its slot's lanes in lane
+// order.  The diagonal block and the coefficient are reduced from the
+// producers' registers the same way (staged rows, then fixed partial sums).
+// Host tables: rnd_lane_slot is [items][kPcCons], rnd_rows.x / .y are the
+// pair rows of the two consumer waves.
+struct PcLayout {
+    size_t wbuf, lslot, lrange, red, total;
+    __host__ __device__ PcLayout(int ns) {
+        size_t o = 0;
+        wbuf = o;   o += sizeof(double) * kWbuf * 2 * kPcRound;       // two W buffers; staging at the end
+        lslot = o;  o += sizeof(int) * kPcCons;
+        lrange = o; o += sizeof(int) * 2 * (size_t)ns;                // first consumer lane, end lane per slot
+        red = o;    o += sizeof(double) * 4 * 27;                     // diagonal + coefficient, 4 partial sums
+        total = (o + 15) & ~(size_t)15;
+    }
+};
+static_assert(kPcCons * kStageStride <= kWbuf * 2 * kPcRound, "consumer rows fit in the W buffers");
+static_assert(kPcRound * 27 <= kWbuf * 2 * kPcRound, "producer rows fit in the W buffers");
+static_assert(kPcRound + kPcCons == kBlock && kPcRound == 2 * kWave, "two producer and two consumer waves");
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const CamRT* __restrict__ cam,
+                                                         const D4* __restrict__ X, const T* __restrict__ w,
+                                                         const double* __restrict__ Hl, const LMDev* lm,
+                                                         double* __restrict__ spart) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int it = P.sitem_order[blockIdx.x];
+    const int f = P.sitem_f[it], a0 = P.sitem_a0[it], a1 = P.sitem_a1[it];
+    const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
+    const int cf = P.free_cam[f];
+    const int win = P.cam_win[cf];
+    if (!running(lm, win)) return;
+    const double lambda = lm[win].lambda;
+    const bool g0 = (g == 0);
+    const PcLayout L(ns);
+    const int tid = threadIdx.x, lane = tid % kWave;
+    double* wbuf = reinterpret_cast<double*>(smem + L.wbuf);
+    int* lslot = reinterpret_cast<int*>(smem + L.lslot);
+    int* lfirst = reinterpret_cast<int*>(smem + L.lrange);
+    int* lend = lfirst + ns;
+    double* red = reinterpret_cast<double*>(smem + L.red);
+    const int col0 = P.colptr[f] + g * kMaxSlots;
+    const int rnd0 = P.sitem_rnd0[it];
+    const int nrounds = (a1 - a0 + kPcRound - 1) / kPcRound;
+    for (int s = tid; s < ns; s += kBlock) { lfirst[s] = 0; lend[s] = 0; }
+
+    // The roles split at wave boundaries and the branch is wave-uniform, so each
+    // wave runs only its own loop; both loops pass the same barriers in the same
+    // order (1 + nrounds, then 2 if g0, then 1).
+    if (__builtin_amdgcn_readfirstlane(tid) < kPcRound) {
+        // ---- producers: phase A of round r + 1 while the consumers sum round r
+        double coef[6], dsum[21], Rf[9], tf[3];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) coef[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 21; ++k) dsum[k] = 0.0;
+        load_cam(cam, cf, Rf, tf);
+        // loads run ahead: the point index two rounds ahead, the gathers of the
+        // next round, each issued before the barrier that precedes its use
+        auto point_of = [&](int r) { const int k = a0 + r * kPcRound + tid; return k < a1 ? P.c_pt[k] : -1; };
+        PaIn in;
+        pa_load(P, X, Hl, a0 + tid, point_of(0), in);
+        int p_next = point_of(1);
+        schur_phase_a<T>(P, Rf, tf, in, lambda, g0, wbuf + tid * kWbuf, coef, dsum);
+        pa_load(P, X, Hl, a0 + kPcRound + tid, p_next, in);
+        p_next = point_of(2);
+        __syncthreads();
+        for (int r = 0; r < nrounds; ++r) {
+            // in: round r + 1; p_next: round r + 2
+            schur_phase_a<T>(P, Rf, tf, in, lambda, g0, wbuf + (((r + 1) & 1) * kPcRound + tid) * kWbuf, coef, dsum);
+            pa_load(P, X, Hl, a0 + (r + 2) * kPcRound + tid, p_next, in);
+            p_next = point_of(r + 3);
+            __syncthreads();  // round r + 1's buffer written, round r's buffer read
+        }
+        if (g0) {
+            // diagonal block and coefficient: rows staged, 4 partial sums of 32 lanes each
+            double* row = wbuf + tid * 27;
+#pragma unroll
+            for (int q = 0; q < 21; ++q) row[q] = dsum[q];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) row[21 + q] = coef[q];
+            __syncthreads();
+            if (tid < 4 * 27) {
+                const int q = tid % 27, part = tid / 27;
+                const double* sp = wbuf + (part * (kPcRound / 4)) * 27 + q;
+                double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+#pragma unroll
+                for (int u = 0; u < kPcRound / 4; u += 4) {
+                    v0 += sp[u * 27];
+                    v1 += sp[(u + 1) * 27];
+                    v2 += sp[(u + 2) * 27];
+                    v3 += sp[(u + 3) * 27];
+                }
+                red[part * 27 + q] = (v0 + v1) + (v2 + v3);
+            }
+            __syncthreads();  // partial sums written, producer rows read
+        }
+        __syncthreads();      // consumer rows staged
+    } else {
+        // ---- consumers: bound to one slot for the item; pairs of round r
+        const int c = tid - kPcRound, cw = c / kWave;
+        const int sl = P.rnd_lane_slot[(size_t)it * kPcCons + c];
+        lslot[c] = sl;
+        double sacc[36], Rb[9], tb[3];
+#pragma unroll
+        for (int e = 0; e < 36; ++e) sacc[e] = 0.0;
+        if (sl != 255) load_cam(cam, P.free_cam[P.colcam[col0 + sl]], Rb, tb);
+        int nrow = 0;
+        const int2* pl = nullptr;
+        int2 e0 = int2{-1, 0}, e1 = int2{-1, 0};
+        double w0 = 1.0;
+        // the lane's pair rows of round rnd, first two entries and the first weight
+        auto fetch = [&](int rnd) {
+            const int4 rows = P.rnd_rows[rnd];
+            nrow = cw == 0 ? rows.x : rows.y;
+            pl = P.pairs + (size_t)P.rnd_pair0[rnd] + (size_t)(cw == 0 ? 0 : rows.x) * kWave + lane;
+            e0 = int2{-1, 0}; e1 = int2{-1, 0};
+            w0 = 1.0;
+            if (sl != 255 && nrow > 0) {
+                e0 = pl[0];
+                if (nrow > 1) e1 = pl[kWave];
+                if (P.robust && e0.x >= 0) w0 = (double)w[e0.x];
+            }
+        };
+        fetch(rnd0);
+        __syncthreads();
+        // slots' consumer lanes are contiguous: each slot's first and end lane
+        if (sl != 255) {
+            if (c == 0 || lslot[c - 1] != sl) lfirst[sl] = c;
+            if (c == kPcCons - 1 || lslot[c + 1] != sl) lend[sl] = c + 1;
+        }
+        for (int r = 0; r < nrounds; ++r) {
+            if (sl != 255)
+                schur_pairs<T>(P, w, wbuf + (size_t)(r & 1) * kPcRound * kWbuf, pl, nrow, e0, e1, w0, Rb, tb, sacc);
+            if (r + 1 < nrounds) fetch(rnd0 + r + 1);
+            __syncthreads();
+        }
+        if (g0) {
+            __syncthreads();
+            __syncthreads();
+        }
+#pragma unroll
+        for (int e = 0; e < 36; ++e) wbuf[c * kStageStride + e] = sacc[e];
+        __syncthreads();
+    }
+    // ---- each (slot, entry) adds its slot's staged consumer rows in lane order
+    double* out = spart + P.sitem_pofs[it];
+    for (int task = tid; task < ns * 36; task += kBlock) {
+        const int s = task / 36, e = task % 36;
+        const int l0 = lfirst[s], n = lend[s] - l0;
+        const double* sp = wbuf + l0 * kStageStride + e;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+        int u = 0;
+        for (; u + 4 <= n; u += 4) {
+            v0 += sp[u * kStageStride];
+            v1 += sp[(u + 1) * kStageStride];
+            v2 += sp[(u + 2) * kStageStride];
+            v3 += sp[(u + 3) * kStageStride];
+        }
+        for (; u < n; ++u) v0 += sp[u * kStageStride];
+        double v = (v0 + v1) + (v2 + v3);
+        if (g0 && s == 0) {
+            // packed upper-triangle index of (min, max) of the entry's row and column
+            const int i = e / 6, j = e % 6, lo = i < j ? i : j, hi = i < j ? j : i;
+            const int q = lo * (11 - lo) / 2 + hi;
+            v += (red[q] + red[27 + q]) + (red[54 + q] + red[81 + q]);
+        }
+        out[task] = v;
+    }
+    if (g0 && tid < 6) out[36 * ns + tid] = (red[21 + tid] + red[48 + tid]) + (red[75 + tid] + red[102 + tid]);
+}
+
+size_t schur_row_smem(const DevProblem& P) {
+    return P.schur_pc ? PcLayout(P.max_ns).total : SchurLayout(P.max_ns).total;
+}
 
 template <typename T>
 int prepare_schur_row(const DevProblem& P) {
     const size_t bytes = schur_row_smem(P);
     if (bytes > 160 * 1024) return -1;
-    const hipError_t e =
-        hipFuncSetAttribute((const void*)k_schur_row<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const void* fn = P.schur_pc ? (const void*)k_schur_row_pc<T> : (const void*)k_schur_row<T>;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     return e == hipSuccess ? 0 : -2;
 }
 
 template <typename T>
 void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st) {
     if (P.n_sitems == 0) return;
-    hipLaunchKernelGGL((k_schur_row<T>), dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam, s.X,
-                       (const T*)W.w, (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
+    if (P.schur_pc)
+        hipLaunchKernelGGL((k_schur_row_pc<T>), dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam,
+                           s.X, (const T*)W.w, (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
+    else
+        hipLaunchKernelGGL((k_schur_row<T>), dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam, s.X,
+                           (const T*)W.w, (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
 }
 
 template void launch_schur_row<double>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);

@@ -22,12 +22,18 @@ def windows(seeds):
     return [problems.workload(3, seed=s) for s in seeds]
 
 
-def test_batched_fp64_windows_match_oracle(solver, oracle_mod):
-    say("batched fp64 workload3 seeds 0..7")
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_batched_fp64_windows_match_oracle(solver, oracle_mod, kernel):
+    say(f"batched fp64 workload3 seeds 0..7, Schur-row kernel {kernel}")
     wins = windows(range(8))
     opts = workload_options(3)
     opts.precision = NAV_BA_FP64
-    got = solver.solve_batched(wins, opts)
+    solver.set_schur_kernel(kernel)
+    try:
+        got = solver.solve_batched(wins, opts)
+        assert solver.schur_kernel == kernel
+    finally:
+        solver.set_schur_kernel(0)
     for w, (prob, g) in enumerate(zip(wins, got)):
         ref = oracle_mod.solve(prob, opts)
         check_fp64_against_oracle(oracle_mod, prob, opts, g, ref, label=f"window {w} ({prob.n_cam} kf, {prob.n_pt} pts)")
