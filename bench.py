@@ -792,9 +792,16 @@ def batched_profile(n_windows, steps, warmup, precision="mixed"):
     s.set_timing(True)
     s.run()
     tm = s.timing()
+    s.set_timing(False)
+    # last: the headline's roofline timing of the Schur rows (1 warm-up and 10
+    # timed launches back to back, every window running)
+    schur_us = s.time_schur_rows(1e-4, 10)
+    kernel, chunk = s.schur_kernel, s.schur_chunk
     s.close()
     print(json.dumps({"batched_windows": n_windows, "ms_per_batch": round(dt * 1e3, 3),
-                      "stages_ms": {k: round(v[0], 4) for k, v in tm.items() if v[1] > 0}}), flush=True)
+                      "stages_ms": {k: round(v[0], 4) for k, v in tm.items() if v[1] > 0},
+                      "schur_kernel": SCHUR_KERNEL.get(kernel), "schur_chunk": chunk,
+                      "schur_rows_avg_us": round(schur_us, 2)}), flush=True)
 
 
 def _cpu_model():
@@ -827,6 +834,8 @@ def main():
                     help="two-view RANSAC scoring leg: number of matches, 200 H and 200 F hypotheses (0 = skip)")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling: run only the batched solve (K timed runs) and print its timing")
+    ap.add_argument("--single-only", action="store_true",
+                    help="profiling: run only the single-problem leg of --workload and print it")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="profiling (--batched-only): precision of the batched solve")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -843,6 +852,9 @@ def main():
         return
     if args.batched_only:
         batched_profile(args.windows, args.steps, args.warmup, args.precision)
+        return
+    if args.single_only:
+        print(json.dumps(single_leg(args.workload, args.steps, args.warmup, not args.no_cpu)), flush=True)
         return
     with_cpu = not args.no_cpu
     head = headline_leg(args.windows, args.steps, args.warmup, with_cpu)

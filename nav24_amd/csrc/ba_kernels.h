@@ -28,6 +28,7 @@ constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Ch
 constexpr int kBlocksAutoFc = 16;  // ... and chosen automatically up to this size (one CU's latency chain
                                    // grows with the block count; 64x64 tiles over many CUs win above)
 constexpr int kCostObs = 1024;     // observations per cost / flag block
+constexpr int kXcds = 8;           // XCDs of one MI355X: workgroup i of a launch runs on XCD i % 8
 
 // LM phase of one window
 enum { kPhaseLin = 0, kPhaseTrial = 1, kPhaseDone = 2 };

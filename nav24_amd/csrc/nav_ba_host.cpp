@@ -429,6 +429,63 @@ int assign_lanes(const std::vector<std::vector<int2>>& bucket, int ns, int lanes
     return worst;
 }
 
+// Dispatch order of work items over the XCDs: workgroup i of a launch runs
+// on XCD i % kXcds, each with its own L2.  The items of one window (a batch)
+// or of one band of consecutive rows (one problem) gather the same points,
+// weights and cameras, so they go to one XCD: windows are dealt to the XCDs
+// largest first onto the least loaded, rows in kXcds bands of equal cost.
+// Each XCD takes its items heaviest first; where its list has run out, a slot
+// takes the next item of the XCD with the most left.
+std::vector<int> xcd_order(const SchurGraph& G, const std::vector<int>& item_row, const std::vector<long long>& cost) {
+    const int n = (int)item_row.size();
+    std::vector<int> order;
+    order.reserve(n);
+    if (n == 0) return order;
+    std::vector<int> xcd(n, 0);
+    int n_win = 0;
+    for (int f = 0; f < G.nfc; ++f) n_win = std::max(n_win, G.cam_win[G.free_cam[f]] + 1);
+    if (n_win > 1) {
+        std::vector<long long> wcost(n_win, 0);
+        for (int i = 0; i < n; ++i) wcost[G.cam_win[G.free_cam[item_row[i]]]] += cost[i];
+        std::vector<int> wo(n_win);
+        for (int w = 0; w < n_win; ++w) wo[w] = w;
+        std::stable_sort(wo.begin(), wo.end(), [&](int a, int b) { return wcost[a] > wcost[b]; });
+        std::vector<long long> load(kXcds, 0);
+        std::vector<int> wx(n_win, 0);
+        for (int w : wo) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            wx[w] = x;
+            load[x] += wcost[w];
+        }
+        for (int i = 0; i < n; ++i) xcd[i] = wx[G.cam_win[G.free_cam[item_row[i]]]];
+    } else {
+        std::vector<long long> rcost(G.nfc + 1, 0);
+        long long total = 0;
+        for (int i = 0; i < n; ++i) { rcost[item_row[i]] += cost[i]; total += cost[i]; }
+        std::vector<int> rx(G.nfc, 0);
+        long long run = 0;
+        for (int f = 0; f < G.nfc; ++f) {
+            rx[f] = total > 0 ? (int)std::min<long long>(kXcds - 1, (run + rcost[f] / 2) * kXcds / total) : 0;
+            run += rcost[f];
+        }
+        for (int i = 0; i < n; ++i) xcd[i] = rx[item_row[i]];
+    }
+    std::vector<std::vector<int>> lists(kXcds);
+    for (int i = 0; i < n; ++i) lists[xcd[i]].push_back(i);
+    for (auto& l : lists) std::stable_sort(l.begin(), l.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<size_t> head(kXcds, 0);
+    for (int pos = 0; pos < n; ++pos) {
+        int x = pos % kXcds;
+        if (head[x] == lists[x].size()) {
+            size_t best = 0;
+            for (int y = 0; y < kXcds; ++y)
+                if (lists[y].size() - head[y] > best) { best = lists[y].size() - head[y]; x = y; }
+        }
+        order.push_back(lists[x][head[x]++]);
+    }
+    return order;
+}
+
 // Items: per S row, column groups of kMaxSlots slots, chunks of `chunk`
 // observations taken in rounds of kBlock, dispatched heaviest first.
 // Pairs of a round: the pairs (a, b) of each of its observations a with the
@@ -605,9 +662,7 @@ bool build_schur_items(const SchurGraph& G, int chunk, bool pc, SchurItems& S) {
         for (int sl = 0; sl < ns; ++sl) slotmap[G.free_cam[G.colcam[col0 + sl]]] = -1;
         scost[it] = cost;
     }
-    S.sorder.resize(S.sf.size());
-    for (size_t i = 0; i < S.sorder.size(); ++i) S.sorder[i] = (int)i;
-    std::stable_sort(S.sorder.begin(), S.sorder.end(), [&](int x, int y) { return scost[x] > scost[y]; });
+    S.sorder = xcd_order(G, S.sf, scost);
     return true;
 }
 
@@ -1033,6 +1088,14 @@ std::string check_layout(const HostLayout& L, const SchurItems& S) {
     const int lanes = S.pc ? kPcCons : kBlock;
     const int rlen = S.pc ? kPcRound : kBlock;
     if ((int)S.sf.size() != (int)S.srnd0.size()) return bad("item round table");
+    {
+        std::vector<char> hit(S.sf.size(), 0);
+        if (S.sorder.size() != S.sf.size()) return bad("dispatch order size");
+        for (int i : S.sorder) {
+            if (i < 0 || i >= (int)S.sf.size() || hit[i]) return bad("dispatch order is not a permutation");
+            hit[i] = 1;
+        }
+    }
     const size_t nrnd = S.rnd_rows.size();
     if (S.lane_slot.size() != (S.pc ? S.sf.size() : nrnd) * lanes || S.rnd_group.size() != nrnd ||
         S.rnd_pair0.size() != nrnd)

@@ -195,6 +195,67 @@ __device__ __forceinline__ void schur_pairs(const DevProblem& P, const T* __rest
     }
 }
 
+// One pair's factors: W_a (w_b J_X,b)^T = [M0 M1] (6x2) and J_xi,b = [B0; B1]
+// (2x6), with b's Jacobians at the point stored in a's W row wb.
+template <typename T>
+__device__ __forceinline__ void pair_factors(const DevProblem& P, const double* wb, double wgt, const double Rb[9],
+                                             const double tb[3], double M0[6], double M1[6], double B0[6],
+                                             double B1[6]) {
+    EdgeLin<T> Lb;
+    edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
+    double JXb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) JXb[q] = wgt * (double)Lb.JX[q];
+#pragma unroll
+    for (int rr = 0; rr < 6; ++rr) {
+        const double v0 = wb[3 * rr], v1 = wb[3 * rr + 1], v2 = wb[3 * rr + 2];
+        M0[rr] = v0 * JXb[0] + v1 * JXb[1] + v2 * JXb[2];
+        M1[rr] = v0 * JXb[3] + v1 * JXb[4] + v2 * JXb[5];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { B0[c] = (double)Lb.Jxi[c]; B1[c] = (double)Lb.Jxi[6 + c]; }
+}
+
+// sacc += [M0 M1] [B0; B1], skipping the pinhole zeros of J_xi at (0, 4) and (1, 3).
+__device__ __forceinline__ void pair_add(double sacc[36], const double M0[6], const double M1[6], const double B0[6],
+                                         const double B1[6]) {
+#pragma unroll
+    for (int rr = 0; rr < 6; ++rr)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double& a = sacc[6 * rr + c];
+            if (c == 4) a = fma(M1[rr], B1[c], a);
+            else if (c == 3) a = fma(M0[rr], B0[c], a);
+            else a = fma(M1[rr], B1[c], fma(M0[rr], B0[c], a));
+        }
+}
+
+// Phase B two pairs at a time (the same sums, in the same order, as
+// schur_pairs): the two pairs' Jacobians and products are independent, so
+// their latencies overlap.  e[0..3] are the lane's first four entries (or
+// {-1, 0} past its share) and wa, wb the weights of the first two.  Each
+// step issues the weights of the next two entries and the entries two steps
+// ahead.
+template <typename T>
+__device__ __forceinline__ void schur_pairs2(const DevProblem& P, const T* __restrict__ w, const double* wbuf,
+                                             const int2* pl, int nrow, int2 eA, int2 eB, int2 eC, int2 eD,
+                                             double wA, double wB, const double Rb[9], const double tb[3],
+                                             double sacc[36]) {
+    for (int j = 0; j < nrow && eA.x >= 0; j += 2) {
+        const double wC = P.robust ? (double)w[eC.x >= 0 ? eC.x : 0] : 1.0;
+        const double wD = P.robust ? (double)w[eD.x >= 0 ? eD.x : 0] : 1.0;
+        const int2 eE = j + 4 < nrow ? pl[(size_t)(j + 4) * kWave] : int2{-1, 0};
+        const int2 eF = j + 5 < nrow ? pl[(size_t)(j + 5) * kWave] : int2{-1, 0};
+        double MA0[6], MA1[6], BA0[6], BA1[6], MB0[6], MB1[6], BB0[6], BB1[6];
+        pair_factors<T>(P, wbuf + eA.y * kWbuf, wA, Rb, tb, MA0, MA1, BA0, BA1);
+        pair_factors<T>(P, wbuf + eB.y * kWbuf, wB, Rb, tb, MB0, MB1, BB0, BB1);
+        pair_add(sacc, MA0, MA1, BA0, BA1);
+        if (eB.x >= 0) pair_add(sacc, MB0, MB1, BB0, BB1);
+        eA = eC; eB = eD; wA = wC; wB = wD;
+        eC = eE; eD = eF;
+    }
+}
+
 // The sum of an aligned group of G lanes (2 or 4, the same in the whole
 // workgroup), in every lane of the group.
 __device__ __forceinline__ void group_sum(double sacc[36], int G) {
@@ -465,24 +526,36 @@ __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const Cam
 #pragma unroll
         for (int e = 0; e < 36; ++e) sacc[e] = 0.0;
         if (sl != 255) load_cam(cam, P.free_cam[P.colcam[col0 + sl]], Rb, tb);
-        int nrow = 0;
+        // round r's pair rows, first four entries and first two weights; the
+        // next round's entries are loaded at the start of a round and their
+        // weights at its end, so both land while the round runs
+        int nrow = 0, nrow_n = 0;
         const int2* pl = nullptr;
-        int2 e0 = int2{-1, 0}, e1 = int2{-1, 0};
-        double w0 = 1.0;
-        // the lane's pair rows of round rnd, first two entries and the first weight
-        auto fetch = [&](int rnd) {
+        const int2* pl_n = nullptr;
+        int2 e[4], n[4];
+        double wa = 1.0, wb = 1.0, wa_n = 1.0, wb_n = 1.0;
+        auto fetch_entries = [&](int rnd) {
             const int4 rows = P.rnd_rows[rnd];
-            nrow = cw == 0 ? rows.x : rows.y;
-            pl = P.pairs + (size_t)P.rnd_pair0[rnd] + (size_t)(cw == 0 ? 0 : rows.x) * kWave + lane;
-            e0 = int2{-1, 0}; e1 = int2{-1, 0};
-            w0 = 1.0;
-            if (sl != 255 && nrow > 0) {
-                e0 = pl[0];
-                if (nrow > 1) e1 = pl[kWave];
-                if (P.robust && e0.x >= 0) w0 = (double)w[e0.x];
+            nrow_n = cw == 0 ? rows.x : rows.y;
+            pl_n = P.pairs + (size_t)P.rnd_pair0[rnd] + (size_t)(cw == 0 ? 0 : rows.x) * kWave + lane;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) n[q] = (sl != 255 && q < nrow_n) ? pl_n[q * kWave] : int2{-1, 0};
+        };
+        auto fetch_weights = [&]() {
+            wa_n = 1.0; wb_n = 1.0;
+            if (P.robust && sl != 255 && nrow_n > 0) {
+                wa_n = (double)w[n[0].x >= 0 ? n[0].x : 0];
+                wb_n = (double)w[n[1].x >= 0 ? n[1].x : 0];
             }
         };
-        fetch(rnd0);
+        auto advance = [&]() {
+            nrow = nrow_n; pl = pl_n;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) e[q] = n[q];
+            wa = wa_n; wb = wb_n;
+        };
+        fetch_entries(rnd0);
+        fetch_weights();
         __syncthreads();
         // slots' consumer lanes are contiguous: each slot's first and end lane
         if (sl != 255) {
@@ -490,9 +563,12 @@ __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const Cam
             if (c == kPcCons - 1 || lslot[c + 1] != sl) lend[sl] = c + 1;
         }
         for (int r = 0; r < nrounds; ++r) {
+            advance();
+            if (r + 1 < nrounds) fetch_entries(rnd0 + r + 1);
             if (sl != 255)
-                schur_pairs<T>(P, w, wbuf + (size_t)(r & 1) * kPcRound * kWbuf, pl, nrow, e0, e1, w0, Rb, tb, sacc);
-            if (r + 1 < nrounds) fetch(rnd0 + r + 1);
+                schur_pairs2<T>(P, w, wbuf + (size_t)(r & 1) * kPcRound * kWbuf, pl, nrow, e[0], e[1], e[2], e[3], wa,
+                                wb, Rb, tb, sacc);
+            if (r + 1 < nrounds) fetch_weights();
             __syncthreads();
         }
         if (g0) {
