@@ -119,6 +119,7 @@ struct DevProblem {
     const int* win_cblk0;      // [n_win + 1]
     int n_cblk;
     const int* win_tile0;      // [n_win + 1] first linearisation tile of each window
+    const int* win_chol_order; // [n_win] windows by free-camera count, largest first (batched factorisation)
     // dense reduced system of one problem: tiled (n_pad = multiple of 64) or 6x6 blocks
     int n, n_pad, nt;
     int use_blocks;            // 1: single-workgroup block Cholesky

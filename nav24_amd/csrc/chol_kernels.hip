@@ -771,7 +771,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
     extern __shared__ double sm[];
     __shared__ double xk[6];
     __shared__ int bad;
-    const int win = blockIdx.x;
+    const int win = P.win_chol_order[blockIdx.x];
     if (!running(lm, win)) return;
     const int nb = P.win_fc0[win + 1] - P.win_fc0[win];
     if (nb == 0) return;
@@ -784,14 +784,32 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
     const int lda = P.win_lda[win];
     const int g0 = 6 * P.win_fc0[win];
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < nblk * 36; idx += kSmallThreads) {
-        const int b = idx / 36, e = idx - 36 * (idx / 36);
-        int I = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-        while ((I + 1) * (I + 2) / 2 <= b) ++I;
-        while (I * (I + 1) / 2 > b) --I;
-        const int J = b - I * (I + 1) / 2;
-        const int r = e / 6, c = e - 6 * (e / 6);
-        Lb[b * kBS + e] = Sw[(size_t)(6 * I + r) * lda + 6 * J + c];
+    {
+        // the lower block triangle row by row: wave v takes pieces v, v + 8, ...
+        // of 64 consecutive columns of one row (row i has 6 (i / 6 + 1) of them),
+        // four pieces' loads in flight before their stores
+        const int wv = tid >> 6, lane = tid & 63;
+        const int per_row = (n + 63) >> 6;
+        const int npieces = n * per_row;
+        for (int k0 = wv; k0 < npieces; k0 += 4 * (kSmallThreads >> 6)) {
+            double v[4];
+            int dst[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = k0 + u * (kSmallThreads >> 6);
+                const int i = k / per_row, col = (k - i * per_row) * 64 + lane;
+                const int I = i / 6;
+                dst[u] = -1;
+                if (k < npieces && col < 6 * (I + 1)) {
+                    const int J = col / 6;
+                    dst[u] = blk_off(I, J) + 6 * (i - 6 * I) + (col - 6 * J);
+                    v[u] = Sw[(size_t)i * lda + col];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (dst[u] >= 0) Lb[dst[u]] = v[u];
+        }
     }
     for (int i = tid; i < n; i += kSmallThreads) yv[i] = g[g0 + i];
     if (tid == 0) bad = 0;
@@ -903,6 +921,9 @@ size_t cholesky_small_smem(const DevProblem& P) {
     return sizeof(double) * (nb * (nb + 1) / 2 * kBS + nb * kBS + 6 * nb);
 }
 
+// One workgroup per window, largest windows first: with the largest window's
+// LDS a CU may hold one workgroup, and the small windows then fill in behind
+// the large ones (P.win_chol_order).
 void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st) {
     hipLaunchKernelGGL(k_chol_small, dim3(P.n_win), dim3(kSmallThreads), cholesky_small_smem(P), st, P,
                        (const double*)W.S, (const double*)W.g, W.dp, W.lm);

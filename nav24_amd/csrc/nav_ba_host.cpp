@@ -130,6 +130,7 @@ struct HostLayout {
     std::vector<double> c_uv;
     std::vector<int> tile_obs, tile_pt, tile_win, win_tile0;
     std::vector<int> cblk_o0, cblk_win, win_cblk0, win_pt0;
+    std::vector<int> win_chol_order;
     std::vector<int> hf, ha0, ha1, cam_hitem_ptr;
     std::vector<int> colptr, colcam, zcolptr, zcolcam;
     std::vector<long long> win_S;
@@ -1022,6 +1023,13 @@ int build_layout(const nav_ba_problem* probs, int n_win, const nav_ba_options* o
         }
     }
 
+    // ---- windows by free-camera count, largest first (the batched factorisation's order)
+    L.win_chol_order.resize(n_win);
+    for (int w = 0; w < n_win; ++w) L.win_chol_order[w] = w;
+    std::stable_sort(L.win_chol_order.begin(), L.win_chol_order.end(), [&](int a, int b) {
+        return L.win_fc0[a + 1] - L.win_fc0[a] > L.win_fc0[b + 1] - L.win_fc0[b];
+    });
+
     // ---- dense reduced systems: one padded tiled system, or one small system per window
     L.win_S.assign(n_win, 0);
     L.win_lda.assign(n_win, 1);
@@ -1072,6 +1080,19 @@ int build_layout(const nav_ba_problem* probs, int n_win, const nav_ba_options* o
 std::string check_layout(const HostLayout& L, const SchurItems& S) {
     auto bad = [](const std::string& m) { return m; };
     const int no = L.no, np = L.np, nc = L.nc, nfc = L.nfc;
+    {
+        std::vector<char> hit(L.n_win, 0);
+        if ((int)L.win_chol_order.size() != L.n_win) return bad("window order size");
+        for (int i = 0; i < L.n_win; ++i) {
+            const int w = L.win_chol_order[i];
+            if (w < 0 || w >= L.n_win || hit[w]) return bad("window order is not a permutation");
+            hit[w] = 1;
+            if (i > 0) {
+                const int p = L.win_chol_order[i - 1];
+                if (L.win_fc0[p + 1] - L.win_fc0[p] < L.win_fc0[w + 1] - L.win_fc0[w]) return bad("window order");
+            }
+        }
+    }
     for (int k = 0; k < no; ++k) {
         if (L.o_cam[k] < 0 || L.o_cam[k] >= nc || L.o_pt[k] < 0 || L.o_pt[k] >= np) return bad("observation index");
         if (k < L.pt_ptr[L.o_pt[k]] || k >= L.pt_ptr[L.o_pt[k] + 1]) return bad("pt_ptr range");
@@ -1224,7 +1245,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     UP(P.cam_win, L.cam_win); UP(P.pt_win, L.pt_win); UP(P.win_cam0, L.win_cam0); UP(P.win_fc0, L.win_fc0);
     UP(P.win_pt0, L.win_pt0); UP(P.win_S, L.win_S); UP(P.win_lda, L.win_lda);
     UP(P.cblk_o0, L.cblk_o0); UP(P.cblk_win, L.cblk_win); UP(P.win_cblk0, L.win_cblk0);
-    UP(P.win_tile0, L.win_tile0);
+    UP(P.win_tile0, L.win_tile0); UP(P.win_chol_order, L.win_chol_order);
 #undef UP
     if ((st = pk.flush(h)) != NAV_BA_OK) return st;
     P.n_tiles = (int)L.tile_win.size();
