@@ -396,6 +396,56 @@ def single_leg(workload, steps, warmup, with_cpu):
     return out
 
 
+def kb8_leg(steps, warmup, with_cpu):
+    """The 60-keyframe random walk seen through the Kannala-Brandt fisheye
+    camera (the same scene, observations re-projected with KannalaBrandt8's
+    model, pixel noise and outliers): FP64 solve resident in HBM, the Schur
+    rows' rate, and the FP64 oracle on one core with the final-cost agreement."""
+    import torch
+    from nav24_amd import problems
+    from nav24_amd.abi import NAV_BA_FP64, workload_options
+    from nav24_amd.ba import GraphOptimHip
+
+    prob = problems.with_kb8(problems.workload(1, seed=0), seed=0)
+    opts = workload_options(1)
+    opts.precision = NAV_BA_FP64
+    st = problem_stats(prob)
+    solver = GraphOptimHip(0)
+    t0 = time.perf_counter()
+    solver.prepare(prob, opts)
+    prep = time.perf_counter() - t0
+    for _ in range(warmup):
+        solver.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = solver.fetch()
+    schur_us = solver.time_schur_rows(1e-4, 10)
+    flops = stage_work(st)["schur_row"][1]
+    out = {"workload": "randomwalk-60kf-20000pt, Kannala-Brandt camera k1..k4 = " + str(list(problems.KB8_COEFFS)),
+           "n_obs": prob.n_obs, "lm_iterations": res.iterations, "trials": res.trials,
+           "value": round(res.iterations * steps / dt, 2), "unit": "LM iterations/s",
+           "ms_per_solve": round(dt / steps * 1e3, 3), "prepare_ms": round(prep * 1e3, 2),
+           "final_cost": res.final_cost,
+           "schur_rows": {"kernel": SCHUR_KERNEL.get(solver.schur_kernel), "avg_launch_us": round(schur_us, 2),
+                          "achieved_TFLOPs": round(flops / (schur_us * 1e-6) / 1e12, 3)}}
+    solver.close()
+    if with_cpu:
+        from oracle import oracle  # checker and CPU baseline only
+        t0 = time.perf_counter()
+        ref = oracle.solve(prob, opts)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(ref.iterations / tc, 3), "unit": "LM iterations/s", "cores": 1,
+                               "kind": "port", "sample": f"the full solve ({ref.iterations} LM iterations), FP64 "
+                                                         f"oracle, 1 thread, one run ({tc:.2f} s)"}
+        out["final_cost_rel_vs_oracle"] = abs(res.final_cost - ref.final_cost) / ref.final_cost
+        out["outlier_flags_identical"] = bool(np.array_equal(res.outlier, ref.outlier))
+    return out
+
+
 def loop_leg(steps, warmup, with_cpu):
     """300 keyframes on a 30 m loop, 150 000 points, 30 LM iterations: FP64
     and mixed on the GPU, each against the FP64 oracle's solve of the same
@@ -824,6 +874,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--windows", type=int, default=512, help="headline: number of uneven windows in the batch")
     ap.add_argument("--loop", type=int, default=1, help="loop leg (300 keyframes, 150 000 points): 0 = skip")
+    ap.add_argument("--kb8", type=int, default=1, help="fisheye-camera leg (the random walk through a "
+                                                        "Kannala-Brandt camera): 0 = skip")
     ap.add_argument("--diffcost", type=int, default=64,
                     help="differentiable-cost leg: number of distorted-camera problems (0 = skip)")
     ap.add_argument("--undistort", type=int, default=3_000_000,
@@ -871,6 +923,8 @@ def main():
     if cpu is not None:
         line["cpu_baseline"] = cpu
     line["single_problem"] = single_leg(args.workload, args.steps, args.warmup, with_cpu)
+    if args.kb8:
+        line["kb8"] = kb8_leg(args.steps, args.warmup, with_cpu)
     if args.loop:
         line["loop"] = loop_leg(max(1, min(args.steps, 3)), 1, with_cpu)
     if args.diffcost > 0:

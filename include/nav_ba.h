@@ -13,6 +13,9 @@
  * HIP stream and its device buffers; handles are independent, so one handle
  * per host thread is safe (the reference solve is synchronous on the caller's
  * thread, FE_SlamMonoV.cpp:164-165).
+ *
+ * ABI version 2 added the camera model and k1..k4 at the end of
+ * nav_ba_problem; a zero-initialised problem is the version-1 pinhole.
  */
 #ifndef NAV_BA_H
 #define NAV_BA_H
@@ -23,7 +26,7 @@
 extern "C" {
 #endif
 
-#define NAV_BA_ABI_VERSION 1
+#define NAV_BA_ABI_VERSION 2
 
 typedef enum nav_ba_status {
     NAV_BA_OK = 0,
@@ -40,6 +43,17 @@ typedef enum nav_ba_precision {
     NAV_BA_FP64 = 0,   /* every stage in FP64 */
     NAV_BA_MIXED = 1   /* FP32 residuals/Jacobians/Hessian blocks; FP64 Hll^-1, reduced system, update */
 } nav_ba_precision;
+
+/* Camera model of the reprojection edge: the model the reference's
+ * Calibration loads (Calibration.cpp:78-87) and the edge reaches through
+ * Calibration::project / projectJac (Calibration.cpp:235-247). */
+typedef enum nav_ba_camera {
+    NAV_BA_CAM_PINHOLE = 0,  /* Pinhole::project / projectJac (Pinhole.cpp:30-36,70-80); also the
+                                radial-tangential camera, whose edge sees undistorted key points */
+    NAV_BA_CAM_KB8 = 1       /* KannalaBrandt8::project / projectJac (KannalaBrandt8.cpp:80-98,195-227):
+                                theta = atan2f of float arguments in the residual, double in the
+                                Jacobian; undefined on the optical axis (x = y = 0), as there */
+} nav_ba_camera;
 
 typedef enum nav_ba_stop {
     NAV_BA_STOP_MAX_ITER = 0,    /* ran all requested outer iterations */
@@ -66,8 +80,10 @@ typedef struct nav_ba_problem {
     const uint8_t* pt_fixed;  /* [n_pt] (BE_GraphOptim.cpp:150-153); may be NULL = none */
     const int32_t* obs_cam;   /* [n_obs] camera index of each observation */
     const int32_t* obs_pt;    /* [n_obs] point index of each observation */
-    const double*  obs_uv;    /* [n_obs][2] undistorted pixel measurement */
-    double fx, fy, cx, cy;    /* pinhole intrinsics (Pinhole.cpp:30-36) */
+    const double*  obs_uv;    /* [n_obs][2] the key point the edge measures (Point2D.hpp:23) */
+    double fx, fy, cx, cy;    /* intrinsics (Pinhole.cpp:30-36; mvParameters[0..3] of either model) */
+    int32_t camera_model;     /* nav_ba_camera; 0 (zero-initialised) = pinhole */
+    double k1, k2, k3, k4;    /* NAV_BA_CAM_KB8: mvParameters[4..7]; ignored for the pinhole */
 } nav_ba_problem;
 
 /* Solver settings.  The reference hard-codes these (its defaults:

@@ -23,6 +23,9 @@ NAV_BA_NOT_PREPARED = 7
 NAV_BA_FP64 = 0
 NAV_BA_MIXED = 1
 
+NAV_BA_CAM_PINHOLE = 0
+NAV_BA_CAM_KB8 = 1
+
 NAV_BA_STOP_MAX_ITER = 0
 NAV_BA_STOP_TERMINATE = 1
 NAV_BA_STOP_EMPTY = 2
@@ -43,6 +46,8 @@ class NavBaProblem(C.Structure):
         ("T_cw", _dp), ("cam_fixed", _u8p), ("X", _dp), ("pt_fixed", _u8p),
         ("obs_cam", _i32p), ("obs_pt", _i32p), ("obs_uv", _dp),
         ("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+        ("camera_model", C.c_int32),
+        ("k1", C.c_double), ("k2", C.c_double), ("k3", C.c_double), ("k4", C.c_double),
     ]
 
 
@@ -84,6 +89,8 @@ class Problem:
     cy: float = 240.0
     cam_fixed: Optional[np.ndarray] = None   # (n_cam,) uint8
     pt_fixed: Optional[np.ndarray] = None    # (n_pt,) uint8
+    camera_model: int = 0                    # NAV_BA_CAM_PINHOLE or NAV_BA_CAM_KB8
+    kb: tuple = (0.0, 0.0, 0.0, 0.0)         # Kannala-Brandt k1..k4
     # generator extras (not part of the ABI)
     T_gt: Optional[np.ndarray] = None
     X_gt: Optional[np.ndarray] = None
@@ -125,6 +132,8 @@ class Problem:
         p.obs_pt = ptr(self.obs_pt, C.c_int32)
         p.obs_uv = ptr(self.obs_uv, C.c_double)
         p.fx, p.fy, p.cx, p.cy = self.fx, self.fy, self.cx, self.cy
+        p.camera_model = int(self.camera_model)
+        p.k1, p.k2, p.k3, p.k4 = (float(v) for v in self.kb)
         return p
 
 

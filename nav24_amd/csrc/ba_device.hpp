@@ -122,6 +122,119 @@ __device__ __forceinline__ double edge_linearize(const double R[9], const double
     return L.e0 * L.e0 + L.e1 * L.e1;
 }
 
+// ---- camera models of the edge (Calibration::project / projectJac,
+// Calibration.cpp:235-247, dispatching to the model the calibration loaded,
+// Calibration.cpp:78-87)
+enum CameraModel { kCamPinhole = 0, kCamKB8 = 1 };
+
+// J_xi has the pinhole's structural zeros at (0, 4) and (1, 3) (d e_u / d
+// upsilon_y and d e_v / d upsilon_x); the fisheye Jacobian has none.
+template <int CM>
+struct CamTraits { static constexpr bool xi_zeros = (CM == kCamPinhole); };
+
+// Kannala-Brandt projection of a camera-frame point as the fisheye model's
+// Eigen overload computes it (KannalaBrandt8.cpp:80-98): theta and psi from
+// the float atan2 of float arguments, the odd polynomial and the image
+// coordinates in double.  The float atan2 is evaluated as the double atan2
+// rounded to float, here and in the oracle.  No contraction into fused
+// multiply-adds: the same double operations as the oracle's.
+template <typename PP>
+__device__ __forceinline__ void kb8_project(const PP& K, double xc, double yc, double zc, double& pu, double& pv) {
+#pragma clang fp contract(off)
+    const double x2y2 = xc * xc + yc * yc;
+    const float rf = sqrtf((float)x2y2);
+    const double th = (double)(float)atan2((double)rf, (double)(float)zc);
+    const double psi = (double)(float)atan2((double)(float)yc, (double)(float)xc);
+    const double t2 = th * th, t3 = th * t2, t5 = t3 * t2, t7 = t5 * t2, t9 = t7 * t2;
+    const double r = th + K.k1 * t3 + K.k2 * t5 + K.k3 * t7 + K.k4 * t9;
+    pu = K.fx * r * cos(psi) + K.cx;
+    pv = K.fy * r * sin(psi) + K.cy;
+}
+
+// d pi / d Xc of the fisheye model (KannalaBrandt8.cpp:195-227), in double,
+// with the same expressions (undefined on the optical axis, r = 0, as there).
+template <typename PP>
+__device__ __forceinline__ void kb8_jac(const PP& K, double x, double y, double z, double J[6]) {
+#pragma clang fp contract(off)
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double r2 = x2 + y2;
+    const double r = sqrt(r2);
+    const double r3 = r2 * r;
+    const double th = atan2(r, z);
+    const double t2 = th * th, t3 = t2 * th, t4 = t2 * t2, t5 = t4 * th;
+    const double t6 = t2 * t4, t7 = t6 * th, t8 = t4 * t4, t9 = t8 * th;
+    const double f = th + t3 * K.k1 + t5 * K.k2 + t7 * K.k3 + t9 * K.k4;
+    const double fd = 1 + 3 * K.k1 * t2 + 5 * K.k2 * t4 + 7 * K.k3 * t6 + 9 * K.k4 * t8;
+    J[0] = K.fx * (fd * z * x2 / (r2 * (r2 + z2)) + f * y2 / r3);
+    J[1] = K.fx * (fd * z * y * x / (r2 * (r2 + z2)) - f * y * x / r3);
+    J[2] = -K.fx * fd * x / (r2 + z2);
+    J[3] = K.fy * (fd * z * y * x / (r2 * (r2 + z2)) - f * y * x / r3);
+    J[4] = K.fy * (fd * z * y2 / (r2 * (r2 + z2)) + f * x2 / r3);
+    J[5] = -K.fy * fd * y / (r2 + z2);
+}
+
+// Residual of camera model CM; K carries fx fy cx cy and (fisheye) k1..k4.
+template <int CM, typename PP>
+__device__ __forceinline__ double edge_residual_m(const PP& K, const double R[9], const double t[3], double x,
+                                                  double y, double z, double u, double v, double& e0, double& e1,
+                                                  double& zc) {
+    if constexpr (CM == kCamPinhole) {
+        return edge_residual(R, t, x, y, z, K.fx, K.fy, K.cx, K.cy, u, v, e0, e1, zc);
+    } else {
+        double xc, yc;
+        transform(R, t, x, y, z, xc, yc, zc);
+        double pu, pv;
+        kb8_project(K, xc, yc, zc, pu, pv);
+        e0 = u - pu;
+        e1 = v - pv;
+        return e0 * e0 + e1 * e1;
+    }
+}
+
+// Linearisation of camera model CM: P = -d pi / d Xc, J_X = P R, J_xi =
+// P [[0, z, -y, 1, 0, 0], [-z, 0, x, 0, 1, 0], [y, -x, 0, 0, 0, 1]]
+// (OptimizableTypes.cpp:140-161).  The fisheye Jacobian and its products are
+// formed in double and rounded to T.
+template <int CM, typename T, typename PP>
+__device__ __forceinline__ double edge_linearize_m(const PP& K, const double R[9], const double t[3], double x,
+                                                   double y, double z, double u, double v, EdgeLin<T>& L) {
+    if constexpr (CM == kCamPinhole) {
+        return edge_linearize(R, t, x, y, z, K.fx, K.fy, K.cx, K.cy, u, v, L);
+    } else {
+        double xcd, ycd, zcd;
+        transform(R, t, x, y, z, xcd, ycd, zcd);
+        double pu, pv;
+        kb8_project(K, xcd, ycd, zcd, pu, pv);
+        L.e0 = u - pu;
+        L.e1 = v - pv;
+        L.z = zcd;
+        double J[6];
+        kb8_jac(K, xcd, ycd, zcd, J);
+        // the entries are formed in double and rounded to T: unlike the
+        // pinhole's, the fisheye J_xi entries are sums of two products that can
+        // cancel (in FP32 the mixed path's first step moved by 2e-8 relative)
+        const double P00 = -J[0], P01 = -J[1], P02 = -J[2], P10 = -J[3], P11 = -J[4], P12 = -J[5];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            L.JX[c] = (T)(P00 * R[c] + P01 * R[3 + c] + P02 * R[6 + c]);
+            L.JX[3 + c] = (T)(P10 * R[c] + P11 * R[3 + c] + P12 * R[6 + c]);
+        }
+        L.Jxi[0] = (T)(-(P01 * zcd) + P02 * ycd);
+        L.Jxi[1] = (T)(P00 * zcd - P02 * xcd);
+        L.Jxi[2] = (T)(-(P00 * ycd) + P01 * xcd);
+        L.Jxi[3] = (T)P00;
+        L.Jxi[4] = (T)P01;
+        L.Jxi[5] = (T)P02;
+        L.Jxi[6] = (T)(-(P11 * zcd) + P12 * ycd);
+        L.Jxi[7] = (T)(P10 * zcd - P12 * xcd);
+        L.Jxi[8] = (T)(-(P10 * ycd) + P11 * xcd);
+        L.Jxi[9] = (T)P10;
+        L.Jxi[10] = (T)P11;
+        L.Jxi[11] = (T)P12;
+        return L.e0 * L.e0 + L.e1 * L.e1;
+    }
+}
+
 // Huber: returns rho(s); w = rho'(s).  Without the kernel rho = s, w = 1.
 template <typename T>
 __device__ __forceinline__ T robustify(T s, int robust, T delta, T& w) {

@@ -47,6 +47,8 @@ struct DevProblem {
     int robust;
     double delta;
     double fx, fy, cx, cy;
+    int cam_model;             // CameraModel: pinhole, or Kannala-Brandt with k1..k4
+    double k1, k2, k3, k4;
     // LM settings (same for every window)
     int n_iter, max_trials, auto_lambda;
     double lambda0;

@@ -61,10 +61,11 @@ void launch_lm_reset(const DevProblem& P, const DevWork& W, hipStream_t st) {
 // Jacobian in T, Huber weight (stored as T), and the point's Hll / bl terms
 // formed and summed in FP64 (fixed order from LDS).  A point with more than
 // kTile observations has a tile of its own and is summed by a block reduction.
-// Registers capped at 72 (7 waves per SIMD instead of 6): the kernel waits on
-// its gathers, and the extra wave gave 6 % at 3 M observations.
-template <typename T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_lin_pt(DevProblem P, const CamRT* __restrict__ cam,
+// Registers capped at 72 for the pinhole (7 waves per SIMD instead of 6): the
+// kernel waits on its gathers, and the extra wave gave 6 % at 3 M
+// observations.  The fisheye model's transcendental functions need more.
+template <typename T, int CM>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CM == kCamPinhole ? 7 : 1, 8))) void k_lin_pt(DevProblem P, const CamRT* __restrict__ cam,
                                                    const D4* __restrict__ X, const LMDev* lm,
                                                    T* __restrict__ w_out, double* __restrict__ Hl) {
     // rows padded by one double: the 9 sums of one point read one column, which
@@ -84,8 +85,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) 
         load_cam(cam, ci, R, tt);
         const D4 xp = X[p];
         EdgeLin<T> L;
-        const double s = edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
-                                        P.o_uv[2 * (size_t)a], P.o_uv[2 * (size_t)a + 1], L);
+        const double s = edge_linearize_m<CM>(P, R, tt, xp.x, xp.y, xp.z, P.o_uv[2 * (size_t)a],
+                                              P.o_uv[2 * (size_t)a + 1], L);
         const T wt = huber_weight<T>(s, P);
         w_out[a] = wt;
         if (P.pt_free[p]) {
@@ -148,8 +149,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) 
 template <typename T>
 void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st) {
     if (P.n_tiles == 0) return;
-    hipLaunchKernelGGL(k_lin_pt<T>, dim3(P.n_tiles), dim3(kBlock), 0, st, P, s.cam, s.X, (const LMDev*)W.lm,
-                       (T*)W.w, W.Hl);
+    const auto fn = P.cam_model == kCamKB8 ? k_lin_pt<T, kCamKB8> : k_lin_pt<T, kCamPinhole>;
+    hipLaunchKernelGGL(fn, dim3(P.n_tiles),
+                       dim3(kBlock), 0, st, P, s.cam, s.X, (const LMDev*)W.lm, (T*)W.w, W.Hl);
 }
 
 // ---------------------------------------------------------------- lin_cam
@@ -157,7 +159,7 @@ void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hip
 // and bp (6) from T Jacobians, formed and block-reduced in FP64; then one
 // workgroup per camera sums the chunks.  Inputs are read in camera-major
 // order (c_pt, c_uv), so the per-observation reads are contiguous.
-template <typename T>
+template <typename T, int CM>
 __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* __restrict__ cam,
                                                     const D4* __restrict__ X, const T* __restrict__ w,
                                                     const LMDev* lm, double* __restrict__ hpart) {
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
         if (a0 + (int)threadIdx.x + kBlock * u >= a1) break;
         const D4 xp = xs[u];
         EdgeLin<T> L;
-        const double s = edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, uk[u], vk[u], L);
+        const double s = edge_linearize_m<CM>(P, R, tt, xp.x, xp.y, xp.z, uk[u], vk[u], L);
         const double wa = (double)huber_weight<T>(s, P);
         const double r0 = -L.e0 * wa, r1 = -L.e1 * wa;
         double J[12];
@@ -203,8 +205,9 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
         for (int i = 0; i < 6; ++i) {
             const double A0 = J[i] * wa, A1 = J[6 + i] * wa;
 #pragma unroll
-            for (int j = i; j < 6; ++j, ++q) {  // J_xi is zero at (0, 4) and (1, 3)
-                const bool z0 = (i == 4 || j == 4), z1 = (i == 3 || j == 3);
+            for (int j = i; j < 6; ++j, ++q) {  // a pinhole J_xi is zero at (0, 4) and (1, 3)
+                constexpr bool zx = CamTraits<CM>::xi_zeros;
+                const bool z0 = zx && (i == 4 || j == 4), z1 = zx && (i == 3 || j == 3);
                 if (!z0 && !z1) acc[q] = fma(A1, J[6 + j], fma(A0, J[j], acc[q]));
                 else if (!z0) acc[q] = fma(A0, J[j], acc[q]);
                 else if (!z1) acc[q] = fma(A1, J[6 + j], acc[q]);
@@ -212,8 +215,9 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
         }
 #pragma unroll
         for (int i = 0; i < 6; ++i)
-            acc[21 + i] = i == 4 ? fma(J[6 + i], r1, acc[21 + i])
-                                 : i == 3 ? fma(J[i], r0, acc[21 + i]) : fma(J[6 + i], r1, fma(J[i], r0, acc[21 + i]));
+            acc[21 + i] = (CamTraits<CM>::xi_zeros && i == 4) ? fma(J[6 + i], r1, acc[21 + i])
+                          : (CamTraits<CM>::xi_zeros && i == 3) ? fma(J[i], r0, acc[21 + i])
+                                                                 : fma(J[6 + i], r1, fma(J[i], r0, acc[21 + i]));
     }
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
@@ -254,9 +258,11 @@ __global__ void k_cam_reduce(DevProblem P, const double* __restrict__ hpart, con
 template <typename T>
 void launch_lin_cam(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st) {
     if (P.n_fc == 0) return;
-    if (P.n_hitems > 0)
-        hipLaunchKernelGGL(k_lin_cam<T>, dim3(P.n_hitems), dim3(kBlock), 0, st, P, s.cam, s.X, (const T*)W.w,
+    if (P.n_hitems > 0) {
+        const auto fn = P.cam_model == kCamKB8 ? k_lin_cam<T, kCamKB8> : k_lin_cam<T, kCamPinhole>;
+        hipLaunchKernelGGL(fn, dim3(P.n_hitems), dim3(kBlock), 0, st, P, s.cam, s.X, (const T*)W.w,
                            (const LMDev*)W.lm, W.hpart);
+    }
     hipLaunchKernelGGL(k_cam_reduce, dim3(P.n_fc), dim3(64), 0, st, P, (const double*)W.hpart, (const LMDev*)W.lm, W.Hc);
 }
 
@@ -470,16 +476,17 @@ void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) 
 // per tile.  A point with more than kTile observations has a tile of its own
 // and is summed by a block reduction.
 // c_b of one observation with its camera, weight and camera step d (6) loaded.
-template <typename T>
+template <typename T, int CM>
 __device__ __forceinline__ void backsub_obs_at(const DevProblem& P, const double R[9], const double tt[3], const D4& xp,
                                                double wb, const double* __restrict__ d, double c[3]) {
     EdgeLin<T> L;
-    edge_linearize(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, L);
+    edge_linearize_m<CM>(P, R, tt, xp.x, xp.y, xp.z, 0.0, 0.0, L);
     double t0 = 0.0, t1 = 0.0;
+    constexpr bool zx = CamTraits<CM>::xi_zeros;  // a pinhole J_xi is zero at (0, 4) and (1, 3)
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {  // J_xi is zero at (0, 4) and (1, 3)
-        if (i != 4) t0 += (double)L.Jxi[i] * d[i];
-        if (i != 3) t1 += (double)L.Jxi[6 + i] * d[i];
+    for (int i = 0; i < 6; ++i) {
+        if (!zx || i != 4) t0 += (double)L.Jxi[i] * d[i];
+        if (!zx || i != 3) t1 += (double)L.Jxi[6 + i] * d[i];
     }
     t0 *= wb; t1 *= wb;
     c[0] = (double)L.JX[0] * t0 + (double)L.JX[3] * t1;
@@ -487,7 +494,7 @@ __device__ __forceinline__ void backsub_obs_at(const DevProblem& P, const double
     c[2] = (double)L.JX[2] * t0 + (double)L.JX[5] * t1;
 }
 
-template <typename T>
+template <typename T, int CM>
 __device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __restrict__ cam, const D4& xp,
                                             int ob, const T* __restrict__ w, const double* __restrict__ dp,
                                             double c[3]) {
@@ -497,7 +504,7 @@ __device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __
     if (fb < 0) return;
     double R[9], tt[3];
     load_cam(cam, cb, R, tt);
-    backsub_obs_at<T>(P, R, tt, xp, (double)w[ob], dp + 6 * (size_t)fb, c);
+    backsub_obs_at<T, CM>(P, R, tt, xp, (double)w[ob], dp + 6 * (size_t)fb, c);
 }
 
 // Point p's update from bl (3) and Dinv (6, packed) already loaded.
@@ -520,7 +527,7 @@ __device__ __forceinline__ double backsub_point(const DevProblem& P, int p, doub
     return backsub_point_at(p, lambda, h + 6, D, s, xp, Xn);
 }
 
-template <typename T>
+template <typename T, int CM>
 __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* __restrict__ cam,
                                                     const D4* __restrict__ X, const T* __restrict__ w,
                                                     const double* __restrict__ Hl,
@@ -570,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
             load_cam(cam, cb, R, tt);
         }
         double c[3] = {0.0, 0.0, 0.0};
-        if (va && !fail && fp && fb >= 0) backsub_obs_at<T>(P, R, tt, xp, wb, dp + 6 * (size_t)fb, c);
+        if (va && !fail && fp && fb >= 0) backsub_obs_at<T, CM>(P, R, tt, xp, wb, dp + 6 * (size_t)fb, c);
         sh[0][threadIdx.x] = c[0]; sh[1][threadIdx.x] = c[1]; sh[2][threadIdx.x] = c[2];
         __syncthreads();
         for (int pp = q; pp < p1; pp += kBlock) {
@@ -606,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
         if (active)
             for (int a = o0 + (int)threadIdx.x; a < o1; a += kBlock) {
                 double ca[3];
-                backsub_obs<T>(P, cam, xp, a, w, dp, ca);
+                backsub_obs<T, CM>(P, cam, xp, a, w, dp, ca);
                 c[0] += ca[0]; c[1] += ca[1]; c[2] += ca[2];
             }
         const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -639,8 +646,10 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
 template <typename T>
 void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st) {
     if (P.n_tiles == 0) return;
-    hipLaunchKernelGGL(k_backsub<T>, dim3(P.n_tiles), dim3(kBlock), 0, st, P, cur.cam, cur.X, (const T*)W.w,
-                       (const double*)W.Hl, W.dp, (const LMDev*)W.lm, cand.X, W.red_scale);
+    const auto fn = P.cam_model == kCamKB8 ? k_backsub<T, kCamKB8> : k_backsub<T, kCamPinhole>;
+    hipLaunchKernelGGL(fn, dim3(P.n_tiles),
+                       dim3(kBlock), 0, st, P, cur.cam, cur.X, (const T*)W.w, (const double*)W.Hl, W.dp,
+                       (const LMDev*)W.lm, cand.X, W.red_scale);
 }
 
 // ---------------------------------------------------------------- camera update
@@ -669,7 +678,7 @@ void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState&
 }
 
 // ---------------------------------------------------------------- cost
-template <typename T>
+template <typename T, int CM>
 __global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restrict__ cam, const D4* __restrict__ X,
                                               const LMDev* lm, int trial, double* __restrict__ red_cost) {
     __shared__ double sh[4];
@@ -704,7 +713,7 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restr
         load_cam(cam, ci[q], R, tt);
         const D4 xp = xs[q];
         double e0, e1, zc, w;
-        const double s = edge_residual(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy, uu[q], vv[q], e0, e1, zc);
+        const double s = edge_residual_m<CM>(P, R, tt, xp.x, xp.y, xp.z, uu[q], vv[q], e0, e1, zc);
         const double rho = robustify(s, P.robust, P.delta, w);
         if (a0 + (int)threadIdx.x + 256 * q < a1) acc += rho;
     }
@@ -715,8 +724,9 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restr
 template <typename T>
 void launch_cost(const DevProblem& P, const DevState& s, const DevWork& W, int trial, hipStream_t st) {
     if (P.n_cblk == 0) return;
-    hipLaunchKernelGGL(k_cost<T>, dim3(P.n_cblk), dim3(256), 0, st, P, s.cam, s.X, (const LMDev*)W.lm, trial,
-                       W.red_cost);
+    const auto fn = P.cam_model == kCamKB8 ? k_cost<T, kCamKB8> : k_cost<T, kCamPinhole>;
+    hipLaunchKernelGGL(fn, dim3(P.n_cblk), dim3(256),
+                       0, st, P, s.cam, s.X, (const LMDev*)W.lm, trial, W.red_cost);
 }
 
 // ---------------------------------------------------------------- LM decision
@@ -843,6 +853,7 @@ void launch_commit(const DevProblem& P, const DevState& cur, const DevState& can
 // ---------------------------------------------------------------- outlier flags
 // FP64 at the given state: flag = chi2 > thr or depth <= 0; written in the
 // caller's observation order.  Also the FP64 robust cost and the flag count.
+template <int CM>
 __global__ __launch_bounds__(256) void k_flags(DevProblem P, const CamRT* __restrict__ cam, const D4* __restrict__ X,
                                                double thr, uint8_t* __restrict__ out, double* __restrict__ red_cost,
                                                double* __restrict__ red_cnt) {
@@ -854,8 +865,8 @@ __global__ __launch_bounds__(256) void k_flags(DevProblem P, const CamRT* __rest
         load_cam(cam, P.o_cam[a], R, tt);
         const D4 xp = X[P.o_pt[a]];
         double e0, e1, zc, w;
-        const double s = edge_residual(R, tt, xp.x, xp.y, xp.z, P.fx, P.fy, P.cx, P.cy,
-                                       P.o_uv[2 * (size_t)a], P.o_uv[2 * (size_t)a + 1], e0, e1, zc);
+        const double s = edge_residual_m<CM>(P, R, tt, xp.x, xp.y, xp.z, P.o_uv[2 * (size_t)a],
+                                             P.o_uv[2 * (size_t)a + 1], e0, e1, zc);
         acc += robustify(s, P.robust, P.delta, w);
         const uint8_t f = (s > thr || !(zc > 0.0)) ? 1 : 0;
         out[P.o_orig[a]] = f;
@@ -869,7 +880,9 @@ __global__ __launch_bounds__(256) void k_flags(DevProblem P, const CamRT* __rest
 
 void launch_flags(const DevProblem& P, const DevState& s, const DevWork& W, double thr, hipStream_t st) {
     if (P.n_cblk == 0) return;
-    hipLaunchKernelGGL(k_flags, dim3(P.n_cblk), dim3(256), 0, st, P, s.cam, s.X, thr, W.outlier, W.red_cost, W.red_aux);
+    const auto fn = P.cam_model == kCamKB8 ? k_flags<kCamKB8> : k_flags<kCamPinhole>;
+    hipLaunchKernelGGL(fn, dim3(P.n_cblk), dim3(256), 0,
+                       st, P, s.cam, s.X, thr, W.outlier, W.red_cost, W.red_aux);
 }
 
 __global__ __launch_bounds__(256) void k_final(DevProblem P, const double* __restrict__ red_cost,

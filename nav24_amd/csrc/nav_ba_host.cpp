@@ -140,6 +140,8 @@ struct HostLayout {
     std::vector<PoseQT> pose0;
     std::vector<D4> X0;
     double fx = 0, fy = 0, cx = 0, cy = 0;
+    int cam_model = NAV_BA_CAM_PINHOLE;
+    double kb[4] = {0, 0, 0, 0};
 
     SchurGraph graph() const {
         return SchurGraph{nfc, nc, colptr.data(), colcam.data(), cam_ptr.data(), cam_obs.data(), cam_obs_b0.data(),
@@ -693,10 +695,10 @@ int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
 
 // Process-wide memory of tuned chunk sizes, per device, precision and problem shape.
 struct TuneKey {
-    int device, mixed, n_win, nc, np, no, nfc, kernel;
+    int device, mixed, n_win, nc, np, no, nfc, kernel, cam;
     bool operator<(const TuneKey& o) const {
-        return std::tie(device, mixed, n_win, nc, np, no, nfc, kernel) <
-               std::tie(o.device, o.mixed, o.n_win, o.nc, o.np, o.no, o.nfc, o.kernel);
+        return std::tie(device, mixed, n_win, nc, np, no, nfc, kernel, cam) <
+               std::tie(o.device, o.mixed, o.n_win, o.nc, o.np, o.no, o.nfc, o.kernel, o.cam);
     }
 };
 struct TuneChoice {
@@ -808,9 +810,19 @@ int check_args(const nav_ba_problem* probs, int n_win, const nav_ba_options* o, 
         }
     if (o->precision != NAV_BA_FP64 && o->precision != NAV_BA_MIXED) { err = "unknown precision"; return NAV_BA_BAD_ARG; }
     if (o->n_iter < 0) { err = "negative iteration count"; return NAV_BA_BAD_ARG; }
+    for (int w = 0; w < n_win; ++w)
+        if (probs[w].camera_model != NAV_BA_CAM_PINHOLE && probs[w].camera_model != NAV_BA_CAM_KB8) {
+            err = "unknown camera model in problem " + std::to_string(w);
+            return NAV_BA_BAD_ARG;
+        }
+    auto kb_differs = [&](const nav_ba_problem& a, const nav_ba_problem& b) {
+        return a.camera_model == NAV_BA_CAM_KB8 && (a.k1 != b.k1 || a.k2 != b.k2 || a.k3 != b.k3 || a.k4 != b.k4);
+    };
     for (int w = 1; w < n_win; ++w)
-        if (probs[w].fx != probs[0].fx || probs[w].fy != probs[0].fy || probs[w].cx != probs[0].cx || probs[w].cy != probs[0].cy) {
-            err = "batched windows must share the camera intrinsics";
+        if (probs[w].fx != probs[0].fx || probs[w].fy != probs[0].fy || probs[w].cx != probs[0].cx ||
+            probs[w].cy != probs[0].cy || probs[w].camera_model != probs[0].camera_model ||
+            kb_differs(probs[w], probs[0])) {
+            err = "batched windows must share the camera model and its parameters";
             return NAV_BA_BAD_ARG;
         }
     return NAV_BA_OK;
@@ -1071,6 +1083,11 @@ int build_layout(const nav_ba_problem* probs, int n_win, const nav_ba_options* o
     // intrinsics are float in the reference's calibration (GeometricCamera.h:117)
     L.fx = compat ? rf32(probs[0].fx) : probs[0].fx; L.fy = compat ? rf32(probs[0].fy) : probs[0].fy;
     L.cx = compat ? rf32(probs[0].cx) : probs[0].cx; L.cy = compat ? rf32(probs[0].cy) : probs[0].cy;
+    // the fisheye coefficients are float parameters too (mvParameters[4..7])
+    L.cam_model = probs[0].camera_model;
+    const bool kb = L.cam_model == NAV_BA_CAM_KB8;
+    const double kin[4] = {probs[0].k1, probs[0].k2, probs[0].k3, probs[0].k4};
+    for (int i = 0; i < 4; ++i) L.kb[i] = kb ? (compat ? rf32(kin[i]) : kin[i]) : 0.0;
     return NAV_BA_OK;
 }
 
@@ -1212,9 +1229,12 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
 
     // ---- Schur work items and pair lists at the default chunk size (tuned below)
     const SchurGraph G = L.graph();
-    const TuneKey key{h->device, mixed(h) ? 1 : 0, n_win, nc, np, no, nfc, h->schur_kernel};
+    const TuneKey key{h->device, mixed(h) ? 1 : 0, n_win, nc, np, no, nfc, h->schur_kernel, L.cam_model};
+    // producer/consumer by default for batches and for the fisheye camera (whose
+    // Jacobians leave the rounds kernel short of registers)
     TuneChoice choice{h->schur_chunk > 0 ? h->schur_chunk : kSchurChunk,
-                      h->schur_kernel == 2 || (h->schur_kernel == 0 && batched)};
+                      h->schur_kernel == 2 ||
+                          (h->schur_kernel == 0 && (batched || L.cam_model == NAV_BA_CAM_KB8))};
     bool tune = (h->schur_chunk == 0 || h->schur_kernel == 0) && nfc > 0 && no >= kTuneMinObs;
     if (tune && cached_choice(key, &choice)) tune = false;  // this shape was tuned before on this device
     SchurItems SI;
@@ -1229,6 +1249,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     P.robust = o->robust ? 1 : 0;
     P.delta = o->huber_delta;
     P.fx = L.fx; P.fy = L.fy; P.cx = L.cx; P.cy = L.cy;
+    P.cam_model = L.cam_model == NAV_BA_CAM_KB8 ? kCamKB8 : kCamPinhole;
+    P.k1 = L.kb[0]; P.k2 = L.kb[1]; P.k3 = L.kb[2]; P.k4 = L.kb[3];
     P.n_iter = o->n_iter;
     P.max_trials = h->opt.max_trials;
     P.auto_lambda = !(o->lambda_init > 0);

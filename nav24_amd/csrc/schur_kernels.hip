@@ -99,13 +99,13 @@ __device__ __forceinline__ void pa_load(const DevProblem& P, const D4* __restric
 
 // Phase A for one observation of camera (Rf, tf): W and the point into row
 // wb, diagonal pair and coefficient into the registers.
-template <typename T, typename In>
+template <typename T, int CM, typename In>
 __device__ __forceinline__ void schur_phase_a(const DevProblem& P, const double Rf[9], const double tf[3],
                                               const In& in, double lambda, bool g0, double* wb, double coef[6],
                                               double dsum[21]) {
     if (!in.live) return;
     EdgeLin<T> La;
-    const double sa = edge_linearize(Rf, tf, in.x, in.y, in.z, P.fx, P.fy, P.cx, P.cy, in.u, in.v, La);
+    const double sa = edge_linearize_m<CM>(P, Rf, tf, in.x, in.y, in.z, in.u, in.v, La);
     const double wa = (double)huber_weight<T>(sa, P);  // as k_lin_pt, without a gather of w
     double JXa[6];
 #pragma unroll
@@ -115,8 +115,9 @@ __device__ __forceinline__ void schur_phase_a(const DevProblem& P, const double 
     for (int i = 0; i < 6; ++i) {
         const double A0 = (double)La.Jxi[i] * wa, A1 = (double)La.Jxi[6 + i] * wa;
 #pragma unroll
-        for (int j = 0; j < 3; ++j)  // J_xi is zero at (0, 4) and (1, 3)
-            Ha[3 * i + j] = i == 4 ? A1 * JXa[3 + j] : i == 3 ? A0 * JXa[j] : A0 * JXa[j] + A1 * JXa[3 + j];
+        for (int j = 0; j < 3; ++j)  // a pinhole J_xi is zero at (0, 4) and (1, 3)
+            Ha[3 * i + j] = (CamTraits<CM>::xi_zeros && i == 4) ? A1 * JXa[3 + j]
+                          : (CamTraits<CM>::xi_zeros && i == 3) ? A0 * JXa[j] : A0 * JXa[j] + A1 * JXa[3 + j];
     }
     double Dp[6];
     const double* hp = in.hl();
@@ -151,11 +152,27 @@ __device__ __forceinline__ void schur_phase_a(const DevProblem& P, const double 
     }
 }
 
+// sacc += [M0 M1] [B0; B1], skipping the pinhole zeros of J_xi at (0, 4) and (1, 3).
+template <int CM>
+__device__ __forceinline__ void pair_add(double sacc[36], const double M0[6], const double M1[6], const double B0[6],
+                                         const double B1[6]) {
+    constexpr bool zx = CamTraits<CM>::xi_zeros;
+#pragma unroll
+    for (int rr = 0; rr < 6; ++rr)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double& a = sacc[6 * rr + c];
+            if (zx && c == 4) a = fma(M1[rr], B1[c], a);
+            else if (zx && c == 3) a = fma(M0[rr], B0[c], a);
+            else a = fma(M1[rr], B1[c], fma(M0[rr], B0[c], a));
+        }
+}
+
 // Phase B: the lane's pairs (entry rows of stride kWave from pl, nrow rows; e0,
 // e1 and the weight w0 of e0 already loaded) into sacc, with the camera (R, t).
 // Each load has a pair's work to land: at pair j the weight of entry j + 1
 // and the entry j + 2 are issued, and used one pair later.
-template <typename T>
+template <typename T, int CM>
 __device__ __forceinline__ void schur_pairs(const DevProblem& P, const T* __restrict__ w, const double* wbuf,
                                             const int2* pl, int nrow, int2 e0, int2 e1, double w0,
                                             const double Rb[9], const double tb[3], double sacc[36]) {
@@ -164,7 +181,7 @@ __device__ __forceinline__ void schur_pairs(const DevProblem& P, const T* __rest
         const int2 e2 = j + 2 < nrow ? pl[(size_t)(j + 2) * kWave] : int2{-1, 0};
         const double* wb = wbuf + e0.y * kWbuf;
         EdgeLin<T> Lb;
-        edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
+        edge_linearize_m<CM>(P, Rb, tb, wb[18], wb[19], wb[20], 0.0, 0.0, Lb);
         // the Huber weight of b folded into J_X,b: W_a (w_b J_X,b)^T J_xi,b
         double JXb[6];
 #pragma unroll
@@ -179,17 +196,7 @@ __device__ __forceinline__ void schur_pairs(const DevProblem& P, const T* __rest
         double B0[6], B1[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) { B0[c] = (double)Lb.Jxi[c]; B1[c] = (double)Lb.Jxi[6 + c]; }
-        // J_xi,b is zero at (0, 4) and (1, 3) (edge_linearize: d e_u / d upsilon_y and
-        // d e_v / d upsilon_x of a pinhole); those products are skipped
-#pragma unroll
-        for (int rr = 0; rr < 6; ++rr)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                double& a = sacc[6 * rr + c];
-                if (c == 4) a = fma(M1[rr], B1[c], a);
-                else if (c == 3) a = fma(M0[rr], B0[c], a);
-                else a = fma(M1[rr], B1[c], fma(M0[rr], B0[c], a));
-            }
+        pair_add<CM>(sacc, M0, M1, B0, B1);
         e0 = e1; w0 = w1;
         e1 = e2;
     }
@@ -197,12 +204,12 @@ __device__ __forceinline__ void schur_pairs(const DevProblem& P, const T* __rest
 
 // One pair's factors: W_a (w_b J_X,b)^T = [M0 M1] (6x2) and J_xi,b = [B0; B1]
 // (2x6), with b's Jacobians at the point stored in a's W row wb.
-template <typename T>
+template <typename T, int CM>
 __device__ __forceinline__ void pair_factors(const DevProblem& P, const double* wb, double wgt, const double Rb[9],
                                              const double tb[3], double M0[6], double M1[6], double B0[6],
                                              double B1[6]) {
     EdgeLin<T> Lb;
-    edge_linearize(Rb, tb, wb[18], wb[19], wb[20], P.fx, P.fy, P.cx, P.cy, 0.0, 0.0, Lb);
+    edge_linearize_m<CM>(P, Rb, tb, wb[18], wb[19], wb[20], 0.0, 0.0, Lb);
     double JXb[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) JXb[q] = wgt * (double)Lb.JX[q];
@@ -216,27 +223,13 @@ __device__ __forceinline__ void pair_factors(const DevProblem& P, const double* 
     for (int c = 0; c < 6; ++c) { B0[c] = (double)Lb.Jxi[c]; B1[c] = (double)Lb.Jxi[6 + c]; }
 }
 
-// sacc += [M0 M1] [B0; B1], skipping the pinhole zeros of J_xi at (0, 4) and (1, 3).
-__device__ __forceinline__ void pair_add(double sacc[36], const double M0[6], const double M1[6], const double B0[6],
-                                         const double B1[6]) {
-#pragma unroll
-    for (int rr = 0; rr < 6; ++rr)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            double& a = sacc[6 * rr + c];
-            if (c == 4) a = fma(M1[rr], B1[c], a);
-            else if (c == 3) a = fma(M0[rr], B0[c], a);
-            else a = fma(M1[rr], B1[c], fma(M0[rr], B0[c], a));
-        }
-}
-
 // Phase B two pairs at a time (the same sums, in the same order, as
 // schur_pairs): the two pairs' Jacobians and products are independent, so
 // their latencies overlap.  e[0..3] are the lane's first four entries (or
 // {-1, 0} past its share) and wa, wb the weights of the first two.  Each
 // step issues the weights of the next two entries and the entries two steps
 // ahead.
-template <typename T>
+template <typename T, int CM>
 __device__ __forceinline__ void schur_pairs2(const DevProblem& P, const T* __restrict__ w, const double* wbuf,
                                              const int2* pl, int nrow, int2 eA, int2 eB, int2 eC, int2 eD,
                                              double wA, double wB, const double Rb[9], const double tb[3],
@@ -247,10 +240,10 @@ __device__ __forceinline__ void schur_pairs2(const DevProblem& P, const T* __res
         const int2 eE = j + 4 < nrow ? pl[(size_t)(j + 4) * kWave] : int2{-1, 0};
         const int2 eF = j + 5 < nrow ? pl[(size_t)(j + 5) * kWave] : int2{-1, 0};
         double MA0[6], MA1[6], BA0[6], BA1[6], MB0[6], MB1[6], BB0[6], BB1[6];
-        pair_factors<T>(P, wbuf + eA.y * kWbuf, wA, Rb, tb, MA0, MA1, BA0, BA1);
-        pair_factors<T>(P, wbuf + eB.y * kWbuf, wB, Rb, tb, MB0, MB1, BB0, BB1);
-        pair_add(sacc, MA0, MA1, BA0, BA1);
-        if (eB.x >= 0) pair_add(sacc, MB0, MB1, BB0, BB1);
+        pair_factors<T, CM>(P, wbuf + eA.y * kWbuf, wA, Rb, tb, MA0, MA1, BA0, BA1);
+        pair_factors<T, CM>(P, wbuf + eB.y * kWbuf, wB, Rb, tb, MB0, MB1, BB0, BB1);
+        pair_add<CM>(sacc, MA0, MA1, BA0, BA1);
+        if (eB.x >= 0) pair_add<CM>(sacc, MB0, MB1, BB0, BB1);
         eA = eC; eB = eD; wA = wC; wB = wD;
         eC = eE; eD = eF;
     }
@@ -289,7 +282,7 @@ __device__ __forceinline__ void flush_slots(double* acc, const double* stage, co
     }
 }
 
-template <typename T>
+template <typename T, int CM>
 __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const CamRT* __restrict__ cam,
                                                       const D4* __restrict__ X, const T* __restrict__ w,
                                                       const double* __restrict__ Hl, const LMDev* lm,
@@ -350,7 +343,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const Cam
             load_cam(cam, cf, Rf, tf);
             PaLazy in;
             pa_load_lazy(P, X, Hl, k, P.c_pt[k], in);
-            schur_phase_a<T>(P, Rf, tf, in, lambda, g0, wbuf + tid * kWbuf, coef, dsum);
+            schur_phase_a<T, CM>(P, Rf, tf, in, lambda, g0, wbuf + tid * kWbuf, coef, dsum);
         }
         lslot[tid] = sl;
         for (int s = tid; s < ns; s += kBlock) lcount[s] = 0;
@@ -365,7 +358,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const Cam
 #pragma unroll
         for (int e = 0; e < 36; ++e) sacc[e] = 0.0;
         if (busy)
-            schur_pairs<T>(P, w, wbuf, pl, nrow, e0, e1, w0, Rb, tb, sacc);
+            schur_pairs<T, CM>(P, w, wbuf, pl, nrow, e0, e1, w0, Rb, tb, sacc);
         group_sum(sacc, G);
         __syncthreads();  // the buffer's reads done; lane ranges set
         if (tid < ns && lcount[tid] > 0) lcount[tid] -= lfirst[tid];
@@ -439,7 +432,7 @@ static_assert(kPcCons * kStageStride <= kWbuf * 2 * kPcRound, "consumer rows fit
 static_assert(kPcRound * 27 <= kWbuf * 2 * kPcRound, "producer rows fit in the W buffers");
 static_assert(kPcRound + kPcCons == kBlock && kPcRound == 2 * kWave, "two producer and two consumer waves");
 
-template <typename T>
+template <typename T, int CM>
 __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const CamRT* __restrict__ cam,
                                                          const D4* __restrict__ X, const T* __restrict__ w,
                                                          const double* __restrict__ Hl, const LMDev* lm,
@@ -482,13 +475,14 @@ __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const Cam
         PaIn in;
         pa_load(P, X, Hl, a0 + tid, point_of(0), in);
         int p_next = point_of(1);
-        schur_phase_a<T>(P, Rf, tf, in, lambda, g0, wbuf + tid * kWbuf, coef, dsum);
+        schur_phase_a<T, CM>(P, Rf, tf, in, lambda, g0, wbuf + tid * kWbuf, coef, dsum);
         pa_load(P, X, Hl, a0 + kPcRound + tid, p_next, in);
         p_next = point_of(2);
         __syncthreads();
         for (int r = 0; r < nrounds; ++r) {
             // in: round r + 1; p_next: round r + 2
-            schur_phase_a<T>(P, Rf, tf, in, lambda, g0, wbuf + (((r + 1) & 1) * kPcRound + tid) * kWbuf, coef, dsum);
+            schur_phase_a<T, CM>(P, Rf, tf, in, lambda, g0, wbuf + (((r + 1) & 1) * kPcRound + tid) * kWbuf, coef,
+                                 dsum);
             pa_load(P, X, Hl, a0 + (r + 2) * kPcRound + tid, p_next, in);
             p_next = point_of(r + 3);
             __syncthreads();  // round r + 1's buffer written, round r's buffer read
@@ -565,9 +559,16 @@ __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const Cam
         for (int r = 0; r < nrounds; ++r) {
             advance();
             if (r + 1 < nrounds) fetch_entries(rnd0 + r + 1);
-            if (sl != 255)
-                schur_pairs2<T>(P, w, wbuf + (size_t)(r & 1) * kPcRound * kWbuf, pl, nrow, e[0], e[1], e[2], e[3], wa,
-                                wb, Rb, tb, sacc);
+            // two pairs per step for the pinhole; the fisheye Jacobians leave no
+            // registers for a second pair in flight
+            if (sl != 255) {
+                if constexpr (CamTraits<CM>::xi_zeros)
+                    schur_pairs2<T, CM>(P, w, wbuf + (size_t)(r & 1) * kPcRound * kWbuf, pl, nrow, e[0], e[1], e[2],
+                                        e[3], wa, wb, Rb, tb, sacc);
+                else
+                    schur_pairs<T, CM>(P, w, wbuf + (size_t)(r & 1) * kPcRound * kWbuf, pl, nrow, e[0], e[1], wa, Rb,
+                                       tb, sacc);
+            }
             if (r + 1 < nrounds) fetch_weights();
             __syncthreads();
         }
@@ -614,7 +615,9 @@ template <typename T>
 int prepare_schur_row(const DevProblem& P) {
     const size_t bytes = schur_row_smem(P);
     if (bytes > 160 * 1024) return -1;
-    const void* fn = P.schur_pc ? (const void*)k_schur_row_pc<T> : (const void*)k_schur_row<T>;
+    const bool kb = P.cam_model == kCamKB8;
+    const void* fn = P.schur_pc ? (kb ? (const void*)k_schur_row_pc<T, kCamKB8> : (const void*)k_schur_row_pc<T, kCamPinhole>)
+                                : (kb ? (const void*)k_schur_row<T, kCamKB8> : (const void*)k_schur_row<T, kCamPinhole>);
     const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     return e == hipSuccess ? 0 : -2;
 }
@@ -622,12 +625,11 @@ int prepare_schur_row(const DevProblem& P) {
 template <typename T>
 void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st) {
     if (P.n_sitems == 0) return;
-    if (P.schur_pc)
-        hipLaunchKernelGGL((k_schur_row_pc<T>), dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam,
-                           s.X, (const T*)W.w, (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
-    else
-        hipLaunchKernelGGL((k_schur_row<T>), dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam, s.X,
-                           (const T*)W.w, (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
+    const bool kb = P.cam_model == kCamKB8;
+    auto fn = P.schur_pc ? (kb ? k_schur_row_pc<T, kCamKB8> : k_schur_row_pc<T, kCamPinhole>)
+                         : (kb ? k_schur_row<T, kCamKB8> : k_schur_row<T, kCamPinhole>);
+    hipLaunchKernelGGL(fn, dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam, s.X, (const T*)W.w,
+                       (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
 }
 
 template void launch_schur_row<double>(const DevProblem&, const DevState&, const DevWork&, hipStream_t);

@@ -120,3 +120,38 @@ def normals(seed: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.float64)
     _gen().nav_gen_normals(seed, n, out.ctypes.data_as(_dp))
     return out
+
+
+# Kannala-Brandt coefficients of the fisheye test camera (a mild wide-angle
+# lens; the reference's Calibration reads them as mvParameters[4..7]).
+KB8_COEFFS = (0.02, -0.005, 0.001, -0.0002)
+
+
+def kb8_project(Xc: np.ndarray, K) -> np.ndarray:
+    """Kannala-Brandt image coordinates of camera-frame points (n, 3), in
+    double throughout (the generator's model; the edge's residual rounds its
+    angles to float)."""
+    fx, fy, cx, cy, k1, k2, k3, k4 = K
+    x, y, z = Xc[:, 0], Xc[:, 1], Xc[:, 2]
+    th = np.arctan2(np.hypot(x, y), z)
+    psi = np.arctan2(y, x)
+    r = th + k1 * th ** 3 + k2 * th ** 5 + k3 * th ** 7 + k4 * th ** 9
+    return np.stack([fx * r * np.cos(psi) + cx, fy * r * np.sin(psi) + cy], 1)
+
+
+def with_kb8(prob: Problem, coeffs=KB8_COEFFS, noise_px: float = 1.0, seed: int = 0) -> Problem:
+    """The same scene seen through a Kannala-Brandt camera: every inlier
+    observation re-projected from the generator's ground truth with the
+    fisheye model plus N(0, noise_px) pixel noise, every outlier kept at a
+    uniform random pixel; the initial state is unchanged."""
+    rng = np.random.default_rng(seed)
+    K = (prob.fx, prob.fy, prob.cx, prob.cy) + tuple(coeffs)
+    T = prob.T_gt[prob.obs_cam]
+    R = T[:, :9].reshape(-1, 3, 3)
+    Xc = np.einsum("nij,nj->ni", R, prob.X_gt[prob.obs_pt]) + T[:, 9:]
+    uv = kb8_project(Xc, K) + rng.normal(0.0, noise_px, (prob.n_obs, 2))
+    out = prob.obs_outlier.astype(bool) if prob.obs_outlier is not None else np.zeros(prob.n_obs, bool)
+    uv[out] = rng.uniform([0.0, 0.0], [2 * prob.cx, 2 * prob.cy], (int(out.sum()), 2))
+    return Problem(T_cw=prob.T_cw, X=prob.X, obs_cam=prob.obs_cam, obs_pt=prob.obs_pt, obs_uv=uv,
+                   fx=prob.fx, fy=prob.fy, cx=prob.cx, cy=prob.cy, cam_fixed=prob.cam_fixed, pt_fixed=prob.pt_fixed,
+                   camera_model=1, kb=tuple(coeffs), T_gt=prob.T_gt, X_gt=prob.X_gt, obs_outlier=prob.obs_outlier)

@@ -176,26 +176,85 @@ inline void inv3(const double A[9], double B[9]) {
     B[8] = (A[0] * A[4] - A[1] * A[3]) * id;
 }
 
-struct Intr { double fx, fy, cx, cy; };
+// Camera of the edge: the pinhole (model 0) or the Kannala-Brandt fisheye
+// (model 1, k1..k4), the model Calibration::loadParams picks
+// (Calibration.cpp:78-87) and the edge reaches through Calibration::project /
+// projectJac (Calibration.cpp:235-247).
+struct Intr {
+    double fx, fy, cx, cy;
+    int model;
+    double k1, k2, k3, k4;
+};
 
-// EdgeSE3ProjectXYZ::computeError (OptimizableTypes.hpp:102-107) with
-// Pinhole::project (Pinhole.cpp:30-36): e = z - (fx*x/z + cx, fy*y/z + cy).
+// KannalaBrandt8::project, the Eigen overload the edge calls
+// (KannalaBrandt8.cpp:80-98): theta = atan2f(sqrtf(float(x^2 + y^2)), float(z))
+// and psi = atan2f(float(y), float(x)) in float, the odd polynomial and the
+// image coordinates in double.  The float atan2 is restated as the double
+// atan2 of the same float arguments rounded to float (the math library's
+// atan2f may differ from that by a float ulp: parity unpinned to that ulp).
+inline void kb8_project(const Intr& K, const double Xc[3], double& pu, double& pv) {
+    const double x2y2 = Xc[0] * Xc[0] + Xc[1] * Xc[1];
+    const float rf = std::sqrt((float)x2y2);
+    const double th = (double)(float)std::atan2((double)rf, (double)(float)Xc[2]);
+    const double psi = (double)(float)std::atan2((double)(float)Xc[1], (double)(float)Xc[0]);
+    const double t2 = th * th, t3 = th * t2, t5 = t3 * t2, t7 = t5 * t2, t9 = t7 * t2;
+    const double r = th + K.k1 * t3 + K.k2 * t5 + K.k3 * t7 + K.k4 * t9;
+    pu = K.fx * r * std::cos(psi) + K.cx;
+    pv = K.fy * r * std::sin(psi) + K.cy;
+}
+
+// KannalaBrandt8::projectJac, the Eigen overload (KannalaBrandt8.cpp:195-227),
+// all in double.
+inline void kb8_jac(const Intr& K, const double Xc[3], double J[6]) {
+    const double x = Xc[0], y = Xc[1], z = Xc[2];
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double r2 = x2 + y2;
+    const double r = std::sqrt(r2);
+    const double r3 = r2 * r;
+    const double th = std::atan2(r, z);
+    const double t2 = th * th, t3 = t2 * th, t4 = t2 * t2, t5 = t4 * th;
+    const double t6 = t2 * t4, t7 = t6 * th, t8 = t4 * t4, t9 = t8 * th;
+    const double f = th + t3 * K.k1 + t5 * K.k2 + t7 * K.k3 + t9 * K.k4;
+    const double fd = 1 + 3 * K.k1 * t2 + 5 * K.k2 * t4 + 7 * K.k3 * t6 + 9 * K.k4 * t8;
+    J[0] = K.fx * (fd * z * x2 / (r2 * (r2 + z2)) + f * y2 / r3);
+    J[1] = K.fx * (fd * z * y * x / (r2 * (r2 + z2)) - f * y * x / r3);
+    J[2] = -K.fx * fd * x / (r2 + z2);
+    J[3] = K.fy * (fd * z * y * x / (r2 * (r2 + z2)) - f * y * x / r3);
+    J[4] = K.fy * (fd * z * y2 / (r2 * (r2 + z2)) + f * x2 / r3);
+    J[5] = -K.fy * fd * y / (r2 + z2);
+}
+
+// EdgeSE3ProjectXYZ::computeError (OptimizableTypes.hpp:102-107): e = z -
+// pi(T X); pi = Pinhole::project (Pinhole.cpp:30-36, (fx x/z + cx, fy y/z +
+// cy)) or kb8_project.
 inline void edge_error(const Pose& T, const double X[3], const Intr& K, const double uv[2],
                        double e[2], double Xc[3]) {
     double r[3];
     quat_rotate(T.q, X, r);
     Xc[0] = r[0] + T.t[0]; Xc[1] = r[1] + T.t[1]; Xc[2] = r[2] + T.t[2];
+    if (K.model == 1) {
+        double pu, pv;
+        kb8_project(K, Xc, pu, pv);
+        e[0] = uv[0] - pu;
+        e[1] = uv[1] - pv;
+        return;
+    }
     e[0] = uv[0] - (K.fx * Xc[0] / Xc[2] + K.cx);
     e[1] = uv[1] - (K.fy * Xc[1] / Xc[2] + K.cy);
 }
 
 // EdgeSE3ProjectXYZ::linearizeOplus (OptimizableTypes.cpp:140-161) with
-// Pinhole::projectJac (Pinhole.cpp:70-80):
+// Pinhole::projectJac (Pinhole.cpp:70-80) or kb8_jac:
 //   P = -Jpi(Xc);  J_X = P * R;  J_xi = P * [-[Xc]x | I].
 inline void edge_jac(const Pose& T, const double Xc[3], const Intr& K, double JX[6], double Jxi[12]) {
     const double x = Xc[0], y = Xc[1], z = Xc[2];
-    const double P[6] = {-(K.fx / z), -0.0, -(-K.fx * x / (z * z)),
-                         -0.0, -(K.fy / z), -(-K.fy * y / (z * z))};
+    double P[6] = {-(K.fx / z), -0.0, -(-K.fx * x / (z * z)),
+                   -0.0, -(K.fy / z), -(-K.fy * y / (z * z))};
+    if (K.model == 1) {
+        double J[6];
+        kb8_jac(K, Xc, J);
+        for (int i = 0; i < 6; ++i) P[i] = -J[i];
+    }
     double R[9];
     R_from_quat(T.q, R);
     for (int r = 0; r < 2; ++r)
@@ -236,8 +295,13 @@ int build_graph(const nav_ba_problem* p, const nav_ba_options* o, Graph& g) {
     if (!p || !o || p->n_cam < 0 || p->n_pt < 0 || p->n_obs < 0) return NAV_BA_BAD_ARG;
     const bool compat = o->compat_fp32_io != 0;
     g.n_cam = p->n_cam; g.n_pt = p->n_pt;
-    g.K = {p->fx, p->fy, p->cx, p->cy};
-    if (compat) g.K = {round_f32(p->fx), round_f32(p->fy), round_f32(p->cx), round_f32(p->cy)};
+    const bool kb = p->camera_model == 1;
+    if (p->camera_model != 0 && !kb) return NAV_BA_BAD_ARG;
+    g.K = {p->fx, p->fy, p->cx, p->cy, kb ? 1 : 0, kb ? p->k1 : 0.0, kb ? p->k2 : 0.0, kb ? p->k3 : 0.0,
+           kb ? p->k4 : 0.0};
+    if (compat)  // every calibration parameter is a float (GeometricCamera.h:117)
+        g.K = {round_f32(p->fx), round_f32(p->fy), round_f32(p->cx), round_f32(p->cy), g.K.model,
+               round_f32(g.K.k1), round_f32(g.K.k2), round_f32(g.K.k3), round_f32(g.K.k4)};
     g.robust = o->robust != 0;
     g.delta = o->huber_delta;
     g.pose.resize(g.n_cam);
@@ -557,7 +621,15 @@ void oracle_edge(const double q[4], const double t[3], const double X[3],
                  double fx, double fy, double cx, double cy, const double uv[2],
                  double e[2], double J_X[6], double J_xi[12], double Xc[3]) {
     Pose T{Quat{q[0], q[1], q[2], q[3]}, {t[0], t[1], t[2]}};
-    Intr K{fx, fy, cx, cy};
+    Intr K{fx, fy, cx, cy, 0, 0, 0, 0, 0};
+    edge_error(T, X, K, uv, e, Xc);
+    edge_jac(T, Xc, K, J_X, J_xi);
+}
+
+void oracle_edge_kb8(const double q[4], const double t[3], const double X[3], const double K8[8],
+                     const double uv[2], double e[2], double J_X[6], double J_xi[12], double Xc[3]) {
+    Pose T{Quat{q[0], q[1], q[2], q[3]}, {t[0], t[1], t[2]}};
+    Intr K{K8[0], K8[1], K8[2], K8[3], 1, K8[4], K8[5], K8[6], K8[7]};
     edge_error(T, X, K, uv, e, Xc);
     edge_jac(T, Xc, K, J_X, J_xi);
 }
@@ -571,7 +643,8 @@ double oracle_cost(const nav_ba_problem* p, const nav_ba_options* o, const doubl
     oo.compat_fp32_io = 0;  // evaluate exactly at the given state
     if (build_graph(&q, &oo, g) == NAV_BA_BAD_ARG) return std::numeric_limits<double>::quiet_NaN();
     if (o->compat_fp32_io) {
-        g.K = {round_f32(p->fx), round_f32(p->fy), round_f32(p->cx), round_f32(p->cy)};
+        g.K = {round_f32(g.K.fx), round_f32(g.K.fy), round_f32(g.K.cx), round_f32(g.K.cy), g.K.model,
+               round_f32(g.K.k1), round_f32(g.K.k2), round_f32(g.K.k3), round_f32(g.K.k4)};
         for (size_t k = 0; k < g.e_uv.size(); ++k) g.e_uv[k] = round_f32(g.e_uv[k]);
     }
     // Evaluate with the given rotation matrices' quaternions.

@@ -10,6 +10,7 @@
  *   reprojection edge + Jacobians   core/dataTypes/optimization/OptimizableTypes.hpp:102-113,
  *                                   OptimizableTypes.cpp:140-161
  *   pinhole projection + Jacobian   core/sensor/camera/models/Pinhole.cpp:30-36,70-80
+ *   fisheye projection + Jacobian   core/sensor/camera/models/KannalaBrandt8.cpp:80-98,195-227
  *   FP32 boundary roundings         BE_GraphOptim.cpp:147,239-240; DataConversion.cpp:122-128
  * and the semantics of the un-vendored g2o library the reference calls
  * (BE_GraphOptim.cpp:40-49,66-68): Levenberg-Marquardt schedule, BlockSolver
@@ -82,6 +83,9 @@ void oracle_inv3(const double A[9], double Ainv[9]);
 void oracle_edge(const double q[4], const double t[3], const double X[3],
                  double fx, double fy, double cx, double cy, const double uv[2],
                  double e[2], double J_X[6], double J_xi[12], double Xc[3]);
+/* The same edge with the Kannala-Brandt camera, K8 = fx fy cx cy k1 k2 k3 k4. */
+void oracle_edge_kb8(const double q[4], const double t[3], const double X[3], const double K8[8],
+                     const double uv[2], double e[2], double J_X[6], double J_xi[12], double Xc[3]);
 
 #ifdef __cplusplus
 }

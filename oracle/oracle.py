@@ -54,6 +54,8 @@ def lib():
         L.oracle_se3_oplus.argtypes = [_dp, _dp, _dp, _dp, _dp]
         L.oracle_huber.argtypes = [C.c_double, C.c_double, _dp]
         L.oracle_inv3.argtypes = [_dp, _dp]
+        L.oracle_edge_kb8.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_edge_kb8.restype = None
         L.oracle_edge.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                   _dp, _dp, _dp, _dp, _dp]
         _fp = C.POINTER(C.c_float)
@@ -206,10 +208,17 @@ def inv3(A):
 
 
 def edge(q, t, X, K, uv):
+    """Residual, J_X (2x3), J_xi (2x6) and the camera-frame point of one edge;
+    K = (fx, fy, cx, cy) for the pinhole, or the eight Kannala-Brandt
+    parameters (fx, fy, cx, cy, k1, k2, k3, k4) for the fisheye camera."""
     q = np.ascontiguousarray(q, np.float64); t = np.ascontiguousarray(t, np.float64)
     X = np.ascontiguousarray(X, np.float64); uv = np.ascontiguousarray(uv, np.float64)
     e = np.zeros(2); JX = np.zeros(6); Jxi = np.zeros(12); Xc = np.zeros(3)
-    lib().oracle_edge(_d(q), _d(t), _d(X), K[0], K[1], K[2], K[3], _d(uv), _d(e), _d(JX), _d(Jxi), _d(Xc))
+    if len(K) == 8:
+        k8 = np.ascontiguousarray(K, np.float64)
+        lib().oracle_edge_kb8(_d(q), _d(t), _d(X), _d(k8), _d(uv), _d(e), _d(JX), _d(Jxi), _d(Xc))
+    else:
+        lib().oracle_edge(_d(q), _d(t), _d(X), K[0], K[1], K[2], K[3], _d(uv), _d(e), _d(JX), _d(Jxi), _d(Xc))
     return e, JX.reshape(2, 3), Jxi.reshape(2, 6), Xc
 
 

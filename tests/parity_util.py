@@ -20,7 +20,7 @@ def hist_rel(got, ref):
 def _clone(prob):
     return Problem(**{k: (getattr(prob, k).copy() if isinstance(getattr(prob, k), np.ndarray) else getattr(prob, k))
                       for k in ("T_cw", "X", "obs_cam", "obs_pt", "obs_uv", "fx", "fy", "cx", "cy",
-                                "cam_fixed", "pt_fixed")})
+                                "cam_fixed", "pt_fixed", "camera_model", "kb")})
 
 
 def sensitivity(oracle_mod, prob, opts):

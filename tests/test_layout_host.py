@@ -82,3 +82,20 @@ def test_bad_chunk_rejected():
     with pytest.raises(BAError) as e:
         check_layout([p], workload_options(0), kernel=3)
     assert e.value.status == NAV_BA_BAD_ARG
+
+
+def test_camera_model_checked_on_the_host():
+    a = problems.workload(3, seed=0)
+    b = problems.with_kb8(problems.workload(3, seed=1))
+    check_layout([b], workload_options(3), batched=True)
+    with pytest.raises(BAError) as e:
+        check_layout([a, b], workload_options(3), batched=True)
+    assert e.value.status == NAV_BA_BAD_ARG
+    c = problems.with_kb8(problems.workload(3, seed=2), coeffs=(0.0, 0.0, 0.0, 0.1))
+    with pytest.raises(BAError):
+        check_layout([b, c], workload_options(3), batched=True)
+    bad = problems.workload(0, seed=0, n_pt=100)
+    bad.camera_model = 7
+    with pytest.raises(BAError) as e:
+        check_layout([bad], workload_options(0))
+    assert e.value.status == NAV_BA_BAD_ARG
