@@ -455,7 +455,10 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
     }
 }
 
+// (Batches have no dense system in memory: the per-window factorisation sums
+// the partials into LDS itself.)
 void launch_fill_system(const DevProblem& P, const DevWork& W, hipStream_t st) {
+    if (P.batched) return;
     const size_t tiles = ((size_t)P.n_pad * P.n_pad + 255) / 256;
     const int chunks = P.batched ? 4 : (int)(tiles < 65535 ? (tiles > 0 ? tiles : 1) : 65535);
     hipLaunchKernelGGL(k_fill_system, dim3(chunks, P.n_win), dim3(256), 0, st, P, W.S, W.g);
@@ -463,7 +466,7 @@ void launch_fill_system(const DevProblem& P, const DevWork& W, hipStream_t st) {
 
 template <typename T>
 void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) {
-    if (P.n_fc == 0) return;
+    if (P.n_fc == 0 || P.batched) return;  // batches: summed by the per-window factorisation
     hipLaunchKernelGGL(k_schur_reduce, dim3(P.n_fc, kReduceParts), dim3(kBlock), 0, st, P, (const double*)W.spart,
                        W.Hc, (const LMDev*)W.lm, W.S, W.g);
 }

@@ -765,54 +765,102 @@ __device__ __forceinline__ bool diag_factor(double* a, double* m, double* y) {
     return ok_all;
 }
 
-__global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, const double* __restrict__ S,
-                                                              const double* __restrict__ g,
+__global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, const double* __restrict__ spart,
+                                                              const double* __restrict__ Hc,
                                                               double* __restrict__ dp, LMDev* lm) {
     extern __shared__ double sm[];
     __shared__ double xk[6];
     __shared__ int bad;
+    __shared__ int rstart[kSmallMaxN / 6 + 1];
     const int win = P.win_chol_order[blockIdx.x];
     if (!running(lm, win)) return;
-    const int nb = P.win_fc0[win + 1] - P.win_fc0[win];
+    const int fc0 = P.win_fc0[win];
+    const int nb = P.win_fc0[win + 1] - fc0;
     if (nb == 0) return;
     const int n = 6 * nb;
     const int nblk = nb * (nb + 1) / 2;
     double* Lb = sm;                      // nblk * kBS
     double* Mb = sm + (size_t)nblk * kBS;  // nb * kBS: M_K = L_KK^-1
     double* yv = Mb + (size_t)nb * kBS;    // n
-    const double* Sw = S + P.win_S[win];
-    const int lda = P.win_lda[win];
-    const int g0 = 6 * P.win_fc0[win];
+    const int g0 = 6 * fc0;
     const int tid = threadIdx.x;
+    const double lambda = lm[win].lambda;
+    // ---- the window's reduced system, summed from the Schur rows' partials
+    // straight into LDS: per S row f, S_ff = Hpp + lambda I - sum and S_gf =
+    // -sum^T over the row's chunk items, the same sums in the same order as
+    // the one-problem reduction (k_schur_reduce), so the system has the same
+    // bits; blocks without co-visibility stay zero.  Batched rows have one
+    // column group (at most 30 slots).
+    for (int i = tid; i < nblk * kBS; i += kSmallThreads) Lb[i] = 0.0;
+    if (tid == 0) {
+        int acc = 0;
+        for (int lf = 0; lf < nb; ++lf) {
+            rstart[lf] = acc;
+            acc += 36 * (P.colptr[fc0 + lf + 1] - P.colptr[fc0 + lf]);
+        }
+        rstart[nb] = acc;
+        bad = 0;
+    }
+    __syncthreads();
     {
-        // the lower block triangle row by row: wave v takes pieces v, v + 8, ...
-        // of 64 consecutive columns of one row (row i has 6 (i / 6 + 1) of them),
-        // four pieces' loads in flight before their stores
-        const int wv = tid >> 6, lane = tid & 63;
-        const int per_row = (n + 63) >> 6;
-        const int npieces = n * per_row;
-        for (int k0 = wv; k0 < npieces; k0 += 4 * (kSmallThreads >> 6)) {
+        const int total = rstart[nb];
+        for (int t0 = tid; t0 < total; t0 += 4 * kSmallThreads) {
             double v[4];
             int dst[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int k = k0 + u * (kSmallThreads >> 6);
-                const int i = k / per_row, col = (k - i * per_row) * 64 + lane;
-                const int I = i / 6;
+                const int t = t0 + u * kSmallThreads;
                 dst[u] = -1;
-                if (k < npieces && col < 6 * (I + 1)) {
-                    const int J = col / 6;
-                    dst[u] = blk_off(I, J) + 6 * (i - 6 * I) + (col - 6 * J);
-                    v[u] = Sw[(size_t)i * lda + col];
+                if (t >= total) continue;
+                int lf = 0;
+                while (rstart[lf + 1] <= t) ++lf;
+                const int f = fc0 + lf;
+                const int s = (t - rstart[lf]) / 36, e = (t - rstart[lf]) % 36;
+                const int nch = P.row_nchunks[f], base = P.row_sitem_ptr[f];
+                const double* sp = spart + 36 * s + e;
+                double a[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) a[q] = 0.0;
+                int ch = 0;
+                for (; ch + 8 <= nch; ch += 8) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) a[q] += sp[P.sitem_pofs[base + ch + q]];
+                }
+                for (; ch < nch; ++ch) a[0] += sp[P.sitem_pofs[base + ch]];
+                const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+                const int i = e / 6, c = e % 6;
+                if (s == 0) {
+                    v[u] = Hc[28 * (size_t)f + sym6(i, c)] + (i == c ? lambda : 0.0) - sum;
+                    dst[u] = blk_off(lf, lf) + 6 * i + c;
+                } else {
+                    const int lg = P.colcam[P.colptr[f] + s] - fc0;
+                    v[u] = -sum;
+                    dst[u] = blk_off(lg, lf) + 6 * c + i;
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (dst[u] >= 0) Lb[dst[u]] = v[u];
         }
+        // right-hand side g = bp - coefficient, the coefficient of group 0's
+        // chunks in four partial sums (chunks congruent mod 4), as the reduction
+        for (int t = tid; t < n; t += kSmallThreads) {
+            const int lf = t / 6, e = t % 6;
+            const int f = fc0 + lf;
+            const int nch = P.row_nchunks[f], base = P.row_sitem_ptr[f];
+            double cp[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double c = 0.0;
+                for (int ch = q; ch < nch; ch += 4) {
+                    const int it = base + ch;
+                    c += spart[P.sitem_pofs[it] + 36 * P.sitem_nslots[it] + e];
+                }
+                cp[q] = c;
+            }
+            yv[t] = Hc[28 * (size_t)f + 21 + e] - ((cp[0] + cp[1]) + (cp[2] + cp[3]));
+        }
     }
-    for (int i = tid; i < n; i += kSmallThreads) yv[i] = g[g0 + i];
-    if (tid == 0) bad = 0;
     __syncthreads();
     if (tid == 0) {
         if (!diag_factor(Lb + blk_off(0, 0), Mb, yv)) bad = 1;
@@ -926,7 +974,7 @@ size_t cholesky_small_smem(const DevProblem& P) {
 // the large ones (P.win_chol_order).
 void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st) {
     hipLaunchKernelGGL(k_chol_small, dim3(P.n_win), dim3(kSmallThreads), cholesky_small_smem(P), st, P,
-                       (const double*)W.S, (const double*)W.g, W.dp, W.lm);
+                       (const double*)W.spart, (const double*)W.Hc, W.dp, W.lm);
 }
 
 int prepare_cholesky_small(const DevProblem& P) {

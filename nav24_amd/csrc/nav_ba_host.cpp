@@ -332,7 +332,8 @@ template <typename T>
 void enqueue_reduced_system(nav_ba_handle* h) {
     if (h->P.n_fc > 0) {
         stage(h, NAV_BA_ST_SCHUR_ROW, [&] { launch_schur_row<T>(h->P, h->cur, h->W, h->stream); });
-        stage(h, NAV_BA_ST_SCHUR_REDUCE, [&] { launch_schur_reduce<T>(h->P, h->W, h->stream); });
+        if (!h->P.batched)  // a batch's windows sum their rows in the factorisation kernel
+            stage(h, NAV_BA_ST_SCHUR_REDUCE, [&] { launch_schur_reduce<T>(h->P, h->W, h->stream); });
     }
 }
 
@@ -1295,7 +1296,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.Hl, sizeof(double) * 9 * (size_t)np);
     AL(W.hpart, sizeof(double) * 27 * std::max<size_t>(L.hf.size(), 1));
     AL(W.Hc, sizeof(double) * 28 * std::max(nfc, 1));
-    AL(W.S, sizeof(double) * (size_t)std::max<long long>(L.S_size, 1));
+    AL(W.S, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(L.S_size, 1)));
     AL(W.g, sizeof(double) * nvec);
     AL(W.y, sizeof(double) * nvec);
     AL(W.dp, sizeof(double) * nvec);
