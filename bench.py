@@ -648,43 +648,74 @@ def window_leg(n_windows, reps, with_cpu):
     """One local window per call, as the reference's front end solves them
     (FE_SlamMonoV.cpp:154-165): nav_ba_solve end to end (upload and ordering,
     the LM solve, results back to the host) for windows 0..n_windows-1 of the
-    uneven-window generator, each solved `reps` times after one warm-up call;
-    the prepare / run / fetch split of the same calls; the FP64 oracle on
-    window 0 as the CPU baseline."""
+    uneven-window generator, `reps` calls each, with the prepare / run / fetch
+    split of the same calls.  Two cases: a new structure every call (another
+    window solved on the handle in between: host ordering, work items and
+    graph capture each time), and the same structure with new poses, points
+    and measurements (the cached index: only the state is uploaded).  The FP64
+    oracle on window 0 is the CPU baseline."""
     import torch
     from nav24_amd import problems
-    from nav24_amd.abi import workload_options
+    from nav24_amd.abi import Problem, workload_options
     from nav24_amd.ba import GraphOptimHip
+
+    def moved(p, seed):
+        rng = np.random.default_rng(seed)
+        T = p.T_cw.copy()
+        T[:, 9:] += rng.normal(0, 0.01, (p.n_cam, 3))
+        return Problem(T_cw=T, X=p.X + rng.normal(0, 0.01, p.X.shape), obs_cam=p.obs_cam, obs_pt=p.obs_pt,
+                       obs_uv=p.obs_uv + rng.normal(0, 0.2, p.obs_uv.shape), cam_fixed=p.cam_fixed,
+                       pt_fixed=p.pt_fixed)
 
     s = GraphOptimHip(0)
     o = workload_options(3)
-    per = []
-    for wi in range(n_windows):
-        w = problems.workload(3, seed=wi)
-        s.solve(w, o)
+    wins = [problems.workload(3, seed=wi) for wi in range(n_windows)]
+    other = problems.workload(3, seed=n_windows + 1)
+
+    def timed(prob, before):
+        """Seconds of one solve, then of prepare / run / fetch on a second call."""
+        if before is not None:
+            s.solve(before, o)
         t0 = time.perf_counter()
-        for _ in range(reps):
-            r = s.solve(w, o)
-        t_solve = (time.perf_counter() - t0) / reps
-        tp = tr = tf = 0.0
-        for _ in range(reps):
-            a = time.perf_counter()
-            s.prepare(w, o)
-            b = time.perf_counter()
-            s.run()
-            torch.cuda.synchronize()
-            c = time.perf_counter()
-            s.fetch()
-            d = time.perf_counter()
-            tp, tr, tf = tp + b - a, tr + c - b, tf + d - c
-        per.append({"n_cam": w.n_cam, "n_obs": w.n_obs, "lm_iterations": r.iterations,
-                    "ms_per_solve": round(t_solve * 1e3, 3), "prepare_ms": round(tp / reps * 1e3, 3),
-                    "run_ms": round(tr / reps * 1e3, 3), "fetch_ms": round(tf / reps * 1e3, 3)})
+        r = s.solve(prob, o)
+        t_solve = time.perf_counter() - t0
+        cached = s.prepare_cached
+        if before is not None:
+            s.solve(before, o)
+        a = time.perf_counter()
+        s.prepare(prob, o)
+        b = time.perf_counter()
+        s.run()
+        torch.cuda.synchronize()
+        c = time.perf_counter()
+        s.fetch()
+        d = time.perf_counter()
+        return r, cached, t_solve, b - a, c - b, d - c
+
+    per = []
+    for wi, w in enumerate(wins):
+        s.solve(w, o)  # warm-up
+        row = {"n_cam": w.n_cam, "n_obs": w.n_obs}
+        for case, runs in (("new_structure", [(w, other if k % 2 == 0 else wins[(wi + 1) % n_windows])
+                                              for k in range(reps)]),
+                           ("cached_structure", [(moved(w, 100 * wi + k), None) for k in range(reps)])):
+            if case == "cached_structure":
+                s.solve(w, o)
+            res = [timed(p, before) for p, before in runs]
+            row[case] = {"ms_per_solve": round(float(np.median([x[2] for x in res])) * 1e3, 3),
+                         "prepare_ms": round(float(np.median([x[3] for x in res])) * 1e3, 3),
+                         "run_ms": round(float(np.median([x[4] for x in res])) * 1e3, 3),
+                         "fetch_ms": round(float(np.median([x[5] for x in res])) * 1e3, 3),
+                         "cached_prepares": int(sum(x[1] for x in res)), "lm_iterations": res[-1][0].iterations}
+        per.append(row)
     s.close()
-    ms = np.array([x["ms_per_solve"] for x in per])
+    new_ms = np.array([x["new_structure"]["ms_per_solve"] for x in per])
+    cached_ms = np.array([x["cached_structure"]["ms_per_solve"] for x in per])
     out = {"workload": f"windows 0..{n_windows - 1} of the uneven-window generator, one solve call each",
-           "precision": "mixed (the window options)", "median_ms_per_solve": round(float(np.median(ms)), 3),
-           "max_ms_per_solve": round(float(ms.max()), 3), "windows": per}
+           "precision": "mixed (the window options)",
+           "median_ms_per_solve": round(float(np.median(new_ms)), 3), "max_ms_per_solve": round(float(new_ms.max()), 3),
+           "median_ms_per_solve_cached": round(float(np.median(cached_ms)), 3),
+           "max_ms_per_solve_cached": round(float(cached_ms.max()), 3), "windows": per}
     if with_cpu:
         from oracle import oracle
         w = problems.workload(3, seed=0)

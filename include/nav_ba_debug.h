@@ -71,6 +71,12 @@ int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk);
  * the problem had nothing to optimise). */
 int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk);
 
+/* Whether the last nav_ba_prepare (or the prepare inside nav_ba_solve) found
+ * the same structure as the one before it on this handle (sizes, observation
+ * indices, fixed flags, camera, options and debug settings all equal) and so
+ * only uploaded the new poses, points and measurements: *cached = 1, else 0. */
+int32_t nav_ba_debug_prepare_cached(const nav_ba_handle* h, int32_t* cached);
+
 /* Schur-row kernel for the next nav_ba_prepare on this handle: 0 = automatic
  * (timed with the chunk sizes above 131072 observations; below, producer/
  * consumer for batches and rounds for one problem), 1 = rounds of 256

@@ -29,6 +29,7 @@ EXPORTED = [
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
     "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_check_layout",
     "nav_ba_debug_time_schur_rows", "nav_ba_debug_set_schur_kernel", "nav_ba_debug_get_schur_kernel",
+    "nav_ba_debug_prepare_cached",
 ]
 
 STAGES = ["lin_pt", "lin_cam", "lambda_init", "hinv", "schur_row", "schur_reduce", "cholesky",
@@ -83,6 +84,8 @@ def lib() -> C.CDLL:
         L.nav_ba_debug_set_schur_chunk.restype = C.c_int32
         L.nav_ba_debug_get_schur_chunk.argtypes = [H, C.POINTER(C.c_int32)]
         L.nav_ba_debug_get_schur_chunk.restype = C.c_int32
+        L.nav_ba_debug_prepare_cached.argtypes = [H, C.POINTER(C.c_int32)]
+        L.nav_ba_debug_prepare_cached.restype = C.c_int32
         L.nav_ba_debug_set_schur_kernel.argtypes = [H, C.c_int32]
         L.nav_ba_debug_set_schur_kernel.restype = C.c_int32
         L.nav_ba_debug_get_schur_kernel.argtypes = [H, C.POINTER(C.c_int32)]
@@ -230,6 +233,14 @@ class GraphOptimHip:
         v = C.c_int32(0)
         self._check(lib().nav_ba_debug_get_schur_kernel(self._h, C.byref(v)))
         return v.value
+
+    @property
+    def prepare_cached(self) -> bool:
+        """Whether the last prepare reused the cached index (the same structure
+        as the prepare before it; only the new state was uploaded)."""
+        v = C.c_int32(0)
+        self._check(lib().nav_ba_debug_prepare_cached(self._h, C.byref(v)))
+        return bool(v.value)
 
     def time_schur_rows(self, lam: float = 1e-4, reps: int = 10) -> float:
         """Average device time (us) of the Schur-row kernel over all windows of

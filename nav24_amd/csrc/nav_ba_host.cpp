@@ -182,6 +182,10 @@ struct nav_ba_handle {
     size_t stage_cap = 0;
     char* stage_s = nullptr;   // pinned staging of the Schur work items (the tuner re-uploads them)
     size_t stage_s_cap = 0;
+    // the cached index: the structure and settings of the last successful
+    // prepare; a prepare with the same key only uploads the new state
+    std::vector<char> prep_key;
+    bool prep_cached = false;  // the last prepare took the cached index
     // results of the last run
     bool ran = false;
     std::vector<LMDev> lm_out;
@@ -1203,15 +1207,117 @@ std::string check_layout(const HostLayout& L, const SchurItems& S) {
 }
 
 // Build the device structure of n_win windows laid end to end (see ba_kernels.h).
+// Everything a prepare's structure depends on: sizes, observation indices,
+// fixed flags, the camera, the options (they are baked into the captured step
+// graphs) and the handle's forced settings.  Poses, points and measurements
+// are not part of it.
+void make_prep_key(const nav_ba_handle* h, const nav_ba_problem* probs, int n_win, const nav_ba_options* o,
+                   bool batched, std::vector<char>& key) {
+    key.clear();
+    auto put = [&](const void* p, size_t n) {
+        const char* c = (const char*)p;
+        key.insert(key.end(), c, c + n);
+    };
+    auto put_i = [&](long long v) { put(&v, sizeof v); };
+    auto put_d = [&](double v) { put(&v, sizeof v); };
+    put_i(n_win); put_i(batched); put_i(h->solver_mode); put_i(h->schur_chunk); put_i(h->schur_kernel);
+    put_i(o->n_iter); put_i(o->robust); put_d(o->huber_delta); put_d(o->lambda_init); put_i(o->precision);
+    put_i(o->compat_fp32_io); put_d(o->outlier_chi2); put_i(o->max_trials);
+    for (int w = 0; w < n_win; ++w) {
+        const nav_ba_problem& p = probs[w];
+        put_i(p.n_cam); put_i(p.n_pt); put_i(p.n_obs); put_i(p.camera_model);
+        put_d(p.fx); put_d(p.fy); put_d(p.cx); put_d(p.cy); put_d(p.k1); put_d(p.k2); put_d(p.k3); put_d(p.k4);
+        put_i(p.cam_fixed != nullptr);
+        if (p.cam_fixed) put(p.cam_fixed, (size_t)p.n_cam);
+        put_i(p.pt_fixed != nullptr);
+        if (p.pt_fixed) put(p.pt_fixed, (size_t)p.n_pt);
+        put(p.obs_cam, sizeof(int32_t) * (size_t)p.n_obs);
+        put(p.obs_pt, sizeof(int32_t) * (size_t)p.n_obs);
+    }
+}
+
+// A prepare whose key equals the last one's: the host layout, work items,
+// tuned choices and step graphs stay; the new initial poses, points and
+// measurements (in the orders prepare chose) go to the device.
+int refresh_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, const nav_ba_options* o) {
+    HostLayout& L = h->layout;
+    const bool compat = o->compat_fp32_io != 0;
+    for (int w = 0; w < n_win; ++w) {
+        const nav_ba_problem& p = probs[w];
+        L.win[w].T_in.assign(p.T_cw, p.T_cw + 12 * (size_t)p.n_cam);
+        L.win[w].X_in.assign(p.X, p.X + 3 * (size_t)p.n_pt);
+        // a window's observations are contiguous in point-major order
+        for (int k = L.pt_ptr[L.pt_first[w]]; k < L.pt_ptr[L.pt_first[w + 1]]; ++k) {
+            const double* uv = p.obs_uv + 2 * (size_t)(L.o_orig[k] - L.obs_off[w]);
+            L.o_uv[2 * (size_t)k] = compat ? rf32(uv[0]) : uv[0];
+            L.o_uv[2 * (size_t)k + 1] = compat ? rf32(uv[1]) : uv[1];
+        }
+        for (int c = 0; c < p.n_cam; ++c) {
+            const double* T = p.T_cw + 12 * (size_t)c;
+            PoseQT& P0 = L.pose0[L.cam_off[w] + c];
+            host_quat_from_R(T, P0.q);
+            P0.t[0] = T[9]; P0.t[1] = T[10]; P0.t[2] = T[11];
+            P0.pad = 0;
+        }
+        for (int q = L.pt_first[w]; q < L.pt_first[w + 1]; ++q) {
+            const double* x = p.X + 3 * (size_t)L.win[w].pt_orig[q - L.win[w].pt_local0];
+            L.X0[q] = compat ? D4{rf32(x[0]), rf32(x[1]), rf32(x[2]), 0.0} : D4{x[0], x[1], x[2], 0.0};
+        }
+    }
+    for (size_t k = 0; k < L.cam_obs.size(); ++k) {
+        L.c_uv[2 * k] = L.o_uv[2 * (size_t)L.cam_obs[k]];
+        L.c_uv[2 * k + 1] = L.o_uv[2 * (size_t)L.cam_obs[k] + 1];
+    }
+    // through the pinned staging of the first prepare (it held these arrays and more)
+    const size_t b_ouv = sizeof(double) * L.o_uv.size(), b_cuv = sizeof(double) * L.c_uv.size();
+    const size_t b_pose = sizeof(PoseQT) * L.pose0.size(), b_x = sizeof(D4) * L.X0.size();
+    const size_t need = b_ouv + b_cuv + b_pose + b_x;
+    if (h->stage_cap < need) {
+        if (h->stage) (void)hipHostFree(h->stage);
+        h->stage = nullptr;
+        h->stage_cap = 0;
+        HIPTRY(h, hipHostMalloc((void**)&h->stage, need, hipHostMallocDefault));
+        h->stage_cap = need;
+    }
+    char* sp = h->stage;
+    std::memcpy(sp, L.o_uv.data(), b_ouv);
+    std::memcpy(sp + b_ouv, L.c_uv.data(), b_cuv);
+    std::memcpy(sp + b_ouv + b_cuv, L.pose0.data(), b_pose);
+    std::memcpy(sp + b_ouv + b_cuv + b_pose, L.X0.data(), b_x);
+    hipStream_t s = h->stream;
+    HIPTRY(h, hipMemcpyAsync((void*)h->P.o_uv, sp, b_ouv, hipMemcpyHostToDevice, s));
+    if (b_cuv) HIPTRY(h, hipMemcpyAsync((void*)h->P.c_uv, sp + b_ouv, b_cuv, hipMemcpyHostToDevice, s));
+    HIPTRY(h, hipMemcpyAsync(h->init.pose, sp + b_ouv + b_cuv, b_pose, hipMemcpyHostToDevice, s));
+    HIPTRY(h, hipMemcpyAsync(h->init.X, sp + b_ouv + b_cuv + b_pose, b_x, hipMemcpyHostToDevice, s));
+    HIPTRY(h, hipStreamSynchronize(s));
+    h->ran = false;
+    h->prepared = true;
+    return NAV_BA_OK;
+}
+
 int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, const nav_ba_options* o, bool batched) {
     if (!h) return NAV_BA_BAD_ARG;
     h->err.clear();
+    std::string msg;
+    int st = check_args(probs, n_win, o, batched, msg);
+    if (st != NAV_BA_OK) {
+        h->prepared = false;
+        h->prep_key.clear();
+        return fail(h, st, msg);
+    }
+    std::vector<char> pkey;
+    make_prep_key(h, probs, n_win, o, batched, pkey);
+    h->prep_cached = false;
+    if (h->prepared && !h->all_empty && pkey == h->prep_key) {
+        h->prep_cached = true;
+        HIPTRY(h, hipSetDevice(h->device));
+        HIPTRY(h, hipStreamSynchronize(h->stream));
+        return refresh_windows(h, probs, n_win, o);
+    }
+    h->prep_key.clear();
     h->prepared = false;
     h->ran = false;
     h->schur_chunk_used = 0;
-    std::string msg;
-    int st = check_args(probs, n_win, o, batched, msg);
-    if (st != NAV_BA_OK) return fail(h, st, msg);
     HIPTRY(h, hipSetDevice(h->device));
     HIPTRY(h, hipStreamSynchronize(h->stream));
     if (h->step) { (void)hipGraphExecDestroy(h->step); h->step = nullptr; }
@@ -1329,6 +1435,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     }
     if ((st = capture_step(h)) != NAV_BA_OK) return st;
     h->prepared = true;
+    h->prep_key.swap(pkey);
     return NAV_BA_OK;
 }
 
@@ -1580,6 +1687,12 @@ int32_t nav_ba_debug_set_schur_chunk(nav_ba_handle* h, int32_t chunk) {
 int32_t nav_ba_debug_get_schur_chunk(const nav_ba_handle* h, int32_t* chunk) {
     if (!h || !chunk) return NAV_BA_BAD_ARG;
     *chunk = h->prepared ? h->schur_chunk_used : 0;
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_prepare_cached(const nav_ba_handle* h, int32_t* cached) {
+    if (!h || !cached) return NAV_BA_BAD_ARG;
+    *cached = h->prep_cached ? 1 : 0;
     return NAV_BA_OK;
 }
 
