@@ -52,10 +52,13 @@ int32_t nav_ba_set_timing(nav_ba_handle* h, int32_t enable);
 int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out);
 
 /* Factorisation of a single problem's reduced system, for the next
- * nav_ba_prepare on this handle: 0 = automatic (6x6 camera blocks in one
- * workgroup up to 16 free cameras, 64x64 tiles above), 1 = tiles, 2 = blocks
- * (falls back to tiles above 128 free cameras).  Both give the same step up
- * to rounding; the switch lets tests cover both on the same problem. */
+ * nav_ba_prepare on this handle: 0 = automatic (3 up to 30 free cameras,
+ * 1 above), 1 = 64x64 tiles over many workgroups, 2 = 6x6 camera blocks in
+ * one workgroup with the system in global memory (falls back to tiles above
+ * 128 free cameras), 3 = 6x6 camera blocks in the LDS of one workgroup,
+ * summed there from the Schur partials (the batched windows' kernel; falls
+ * back to tiles above 30 free cameras).  All give the same step up to
+ * rounding; the switch lets tests cover each on the same problem. */
 int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode);
 
 /* Observations per Schur work item for the next nav_ba_prepare on this

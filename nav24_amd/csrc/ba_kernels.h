@@ -23,12 +23,14 @@ constexpr int kMaxSlots = 112;     // S-row column blocks per work item: 78 KiB 
 constexpr int kPcRound = 128;      // producer/consumer Schur rows: observations per round (two producer waves)
 constexpr int kPcCons = 128;       // ... and consumer lanes (two waves), each bound to one slot per item
 constexpr int kNB = 64;            // tiled dense Cholesky tile
-constexpr int kSmallMaxN = 180;    // per-window in-LDS Cholesky: at most 30 free cameras
-constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Cholesky is possible up to this size
-constexpr int kBlocksAutoFc = 16;  // ... and chosen automatically up to this size (one CU's latency chain
-                                   // grows with the block count; 64x64 tiles over many CUs win above)
+constexpr int kSmallMaxN = 180;    // per-window in-LDS Cholesky (every window of a batch, and a single
+                                   // problem by default): at most 30 free cameras
+constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Cholesky in global memory is
+                                   // possible up to this size (chosen only on request)
 constexpr int kCostObs = 1024;     // observations per cost / flag block
 constexpr int kXcds = 8;           // XCDs of one MI355X: workgroup i of a launch runs on XCD i % 8
+constexpr int kBurstWindows = 64;  // up to this many windows the host reads their LM state between bursts
+constexpr int kCommitInAcceptPts = 8192;  // windows up to this many points commit inside the LM decision
 
 // LM phase of one window
 enum { kPhaseLin = 0, kPhaseTrial = 1, kPhaseDone = 2 };
@@ -89,6 +91,8 @@ struct DevProblem {
     const int* sitem_rnd0;     // [n_sitems] first round of the item
     const int* row_sitem_ptr;  // [n_fc + 1]
     const int* row_nchunks;    // [n_fc]
+    const long long* row_pofs0; // [n_fc] partial-sum offset of each row's first item (the items of a row
+                                //   of one column group lie back to back)
     const int* sitem_order;    // [n_sitems] dispatch order: estimated work, heaviest first
     int schur_pc;              // 0: rounds of kBlock, lanes bound per round; 1: producer/consumer rounds
     const uint8_t* rnd_lane_slot;  // slot of each lane (slots' lanes contiguous), 255 idle: [rounds][kBlock],
@@ -124,7 +128,9 @@ struct DevProblem {
     const int* win_chol_order; // [n_win] windows by free-camera count, largest first (batched factorisation)
     // dense reduced system of one problem: tiled (n_pad = multiple of 64) or 6x6 blocks
     int n, n_pad, nt;
-    int use_blocks;            // 1: single-workgroup block Cholesky
+    int use_blocks;            // 1: single-workgroup block Cholesky in global memory
+    int use_win;               // 1: per-window in-LDS Cholesky summed from the Schur partials (every batch)
+    int commit_in_accept;      // 1: k_lm_accept commits accepted trials (small windows), no k_commit
 };
 
 struct DevState {
@@ -237,15 +243,19 @@ template <typename T> void launch_lambda_init(const DevProblem& P, const DevWork
 template <typename T> void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, hipStream_t st);
 void launch_fill_system(const DevProblem& P, const DevWork& W, hipStream_t st);  // once per prepare
 template <typename T> void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st);
-// One problem: the tiled path's one-workgroup backward launch also forms the
-// candidate poses (cholesky_updates_cameras); otherwise launch_update_cam does.
+// The per-window factorisation and the tiled path's one-workgroup backward launch
+// also form the candidate poses (cholesky_updates_cameras); otherwise launch_update_cam does.
 void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
                            hipStream_t st);
 constexpr int kBackSingleMax = 10;  // tiles up to which the backward step runs in one workgroup
 inline bool cholesky_updates_cameras(const DevProblem& P) {
-    return !P.batched && !P.use_blocks && P.n_fc > 0 && P.nt <= kBackSingleMax;
+    return P.n_fc > 0 && (P.use_win || (!P.batched && !P.use_blocks && P.nt <= kBackSingleMax));
 }
-void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st);
+// Every window of a batch, or a single problem of at most 30 free cameras (use_win):
+// the reduced system is summed from the Schur partials inside the factorisation,
+// which also forms the candidate poses.
+void launch_cholesky_small(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
+                           hipStream_t st);
 int prepare_cholesky_small(const DevProblem& P);  // LDS attribute of the per-window factorisation
 int prepare_cholesky(const DevProblem& P);  // LDS attributes of the one-problem factorisation kernels
 template <typename T> void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand,
@@ -254,7 +264,10 @@ void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState&
 // trial = 1: candidate state of running windows; trial = 0: current state, all windows
 template <typename T> void launch_cost(const DevProblem& P, const DevState& s, const DevWork& W, int trial,
                                        hipStream_t st);
-void launch_lm_accept(const DevProblem& P, const DevWork& W, hipStream_t st);
+// The LM decision; with commit_in_accept (every window at most kCommitInAcceptPts
+// points) it also commits an accepted trial, and launch_commit does nothing.
+void launch_lm_accept(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
+                      hipStream_t st);
 void launch_lm_set_chi(const DevProblem& P, const DevWork& W, hipStream_t st);
 void launch_commit(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);
 void launch_flags(const DevProblem& P, const DevState& s, const DevWork& W, double thr, hipStream_t st);

@@ -761,8 +761,12 @@ __device__ double sum_range(const double* v, int i0, int i1, double* sh) {
 // the trial limit was reached, rho == 0, or the iteration count is done.
 __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* __restrict__ red_cost,
                                                    const double* __restrict__ red_scale, LMDev* lm,
-                                                   double* __restrict__ hist, int* done_count) {
+                                                   double* __restrict__ hist, int* done_count,
+                                                   D4* __restrict__ X, const D4* __restrict__ Xn,
+                                                   PoseQT* __restrict__ pose, const PoseQT* __restrict__ pn,
+                                                   CamRT* __restrict__ cam, const CamRT* __restrict__ cn) {
     __shared__ double sh[4];
+    __shared__ int acc;
     const int win = blockIdx.x;
     if (!running(lm, win)) {
         if (threadIdx.x == 0) lm[win].accepted = 0;
@@ -815,12 +819,24 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
         }
         L.fail = 0;
         lm[win] = L;
+        acc = L.accepted;
+    }
+    if (!P.commit_in_accept) return;
+    // small windows: the commit of an accepted trial (k_commit) by the same workgroup
+    __syncthreads();
+    if (!acc) return;
+    for (int i = P.win_pt0[win] + threadIdx.x; i < P.win_pt0[win + 1]; i += 256) X[i] = Xn[i];
+    for (int c = P.win_cam0[win] + threadIdx.x; c < P.win_cam0[win + 1]; c += 256) {
+        pose[c] = pn[c];
+        cam[c] = cn[c];
     }
 }
 
-void launch_lm_accept(const DevProblem& P, const DevWork& W, hipStream_t st) {
+void launch_lm_accept(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
+                      hipStream_t st) {
     hipLaunchKernelGGL(k_lm_accept, dim3(P.n_win), dim3(256), 0, st, P, (const double*)W.red_cost,
-                       (const double*)W.red_scale, W.lm, W.hist, W.done_count);
+                       (const double*)W.red_scale, W.lm, W.hist, W.done_count, cur.X, (const D4*)cand.X, cur.pose,
+                       (const PoseQT*)cand.pose, cur.cam, (const CamRT*)cand.cam);
 }
 
 __global__ __launch_bounds__(256) void k_lm_set_chi(DevProblem P, const double* __restrict__ red_cost, LMDev* lm,
@@ -848,6 +864,7 @@ __global__ void k_commit(DevProblem P, const LMDev* lm, D4* __restrict__ X, cons
 }
 
 void launch_commit(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st) {
+    if (P.commit_in_accept) return;  // done by k_lm_accept
     const int n = P.n_pt > P.n_cam ? P.n_pt : P.n_cam;
     hipLaunchKernelGGL(k_commit, dim3((n + 255) / 256), dim3(256), 0, st, P, (const LMDev*)W.lm, cur.X,
                        (const D4*)cand.X, cur.pose, (const PoseQT*)cand.pose, cur.cam, (const CamRT*)cand.cam);

@@ -684,20 +684,25 @@ __global__ __launch_bounds__(256) void k_chol_back_tile(const double* __restrict
 }
 
 // ---------------------------------------------------------------- per-window systems
-// One workgroup (512 threads) per running window.  The window's reduced
-// system (n = 6 nb unknowns, nb <= 30 free cameras) is held in LDS as a packed
-// lower triangle of 6x6 camera blocks (block (I, J), J <= I, at
-// (I(I+1)/2 + J) * kBS doubles, row-major inside; the odd stride kBS = 37
-// spreads neighbouring blocks over LDS banks).  Right-looking block Cholesky,
-// per block column K:
-//   diag   L_KK = chol(A_KK), M_K = L_KK^-1 and y_K = M_K y_K, in registers of
-//          one thread (1/sqrt by rsq + two Newton steps) - done by the thread
-//          that applied the last update to A_KK, right after it;
-//   panel  L_IK = A_IK M_K^T and y_I -= L_IK y_K, one thread per row of a block;
-//   update A_IJ -= L_IK L_JK^T for K < J <= I, one thread per 6x6 block (36
-//          register accumulators).
-// Two barriers per block column.  Backward substitution x_K = M_K^T (y_K -
-// sum_{I>K} L_IK^T x_I) runs block by block.  A pivot that is not positive and
+// One workgroup (512 threads) per running window, and for a single problem of
+// at most 30 free cameras.  The window's reduced system (n = 6 nb unknowns)
+// is summed from the Schur rows' partials straight into LDS, as a packed lower
+// triangle of 6x6 camera blocks (block (I, J), J <= I, at (I(I+1)/2 + J) * kBS
+// doubles, row-major inside; the odd stride kBS = 37 spreads neighbouring
+// blocks over LDS banks).  Right-looking block Cholesky, per block column K:
+//   panel  L_IK = A_IK M_K^T and y_I -= L_IK y_K, one thread per row of a
+//          block (block row K+1 in the first lanes of wave 0);
+//   next   wave 0: A_{K+1,K+1} -= L_{K+1,K} L_{K+1,K}^T, one entry per lane,
+//          then L = chol(A_{K+1,K+1}), M_{K+1} = L^-1 and y_{K+1} = M_{K+1}
+//          y_{K+1} in registers of lane 0 (1/sqrt by rsq + two Newton steps;
+//          six lanes with a row each measured no faster);
+//   update meanwhile, waves 1..7: A_IJ -= L_IK L_JK^T for the other blocks,
+//          K < J <= I, one thread per 6x6 block (36 register accumulators).
+// Two barriers per block column; the chain through the columns is wave 0's
+// panel rows, its diagonal update and the 6x6 factor.  Backward substitution
+// x_K = M_K^T (y_K - sum_{I>K} L_IK^T x_I) runs block by block in wave 0, with
+// no workgroup barrier; then the window's candidate poses are formed from the
+// step in LDS (the camera update, fused).  A pivot that is not positive and
 // finite marks the window failed (dp = 0, lm.fail = 1), as the tiled path does.
 constexpr int kBS = 37;
 constexpr int kSmallThreads = 512;
@@ -765,18 +770,120 @@ __device__ __forceinline__ bool diag_factor(double* a, double* m, double* y) {
     return ok_all;
 }
 
+// Part of one (row, slot) block of a window's reduced system from the Schur
+// rows' partials: NE = 36 entries (the whole block, 16-byte loads chunk by
+// chunk) or NE = 6 (one row of the block, the loads of up to seven chunks in
+// flight at once).  Task index q: block q / (36 / NE), part q % (36 / NE).
+// The chunks of an entry are added in the one-problem reduction's order
+// (k_schur_reduce): in groups of eight into eight partial sums, the rest into
+// the first, combined pairwise.
+template <int NE>
+__device__ __forceinline__ void sum_block_part(const DevProblem& P, const double* __restrict__ spart,
+                                               const double* __restrict__ Hc, double* Lb, const int* srow,
+                                               const int* rcol, const int* rns, const int* rnch,
+                                               const long long* rp0, int nb, int fc0, double lambda, int q) {
+    constexpr int kParts = 36 / NE;
+    const int b = q / kParts, part = q - kParts * (q / kParts);
+    int lo = 0, hi = nb - 1;  // the row: largest lf with srow[lf] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (srow[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const int lf = lo, f = fc0 + lf, s = b - srow[lf];
+    const int nch = rnch[lf];
+    const long long stride = 36LL * rns[lf] + 6;  // doubles; even, so every item is 16-byte aligned
+    const int e0 = NE * part;
+    const double* sp = spart + rp0[lf] + 36 * s + e0;
+    const int lg = s == 0 ? lf : P.colcam[rcol[lf] + s] - fc0;
+    double a[NE];
+    double h[NE];  // the diagonal block's Hpp + lambda I entries, loaded ahead
+    if (nch < 8) {
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            const int i = (e0 + k) / 6, c = (e0 + k) % 6;
+            h[k] = s == 0 ? Hc[28 * (size_t)f + sym6(i, c)] + (i == c ? lambda : 0.0) : 0.0;
+        }
+        // fewer than eight chunks: one partial sum in chunk order, then the
+        // seven empty ones added pairwise
+        const double z = 0.0;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) a[e] = 0.0;
+        if constexpr (NE == 36) {
+            for (int ch = 0; ch < nch; ++ch) {
+                const double2* p2 = reinterpret_cast<const double2*>(sp + ch * stride);
+                double2 v[18];
+#pragma unroll
+                for (int k = 0; k < 18; ++k) v[k] = p2[k];
+#pragma unroll
+                for (int k = 0; k < 18; ++k) {
+                    a[2 * k] += v[k].x;
+                    a[2 * k + 1] += v[k].y;
+                }
+            }
+        } else {
+            double2 v[7][3];
+#pragma unroll
+            for (int ch = 0; ch < 7; ++ch) {
+                const double2* p2 = reinterpret_cast<const double2*>(sp + min(ch, nch - 1) * stride);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v[ch][k] = p2[k];
+            }
+#pragma unroll
+            for (int ch = 0; ch < 7; ++ch)
+                if (ch < nch)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        a[2 * k] += v[ch][k].x;
+                        a[2 * k + 1] += v[ch][k].y;
+                    }
+        }
+#pragma unroll
+        for (int e = 0; e < NE; ++e) a[e] = ((a[e] + z) + (z + z)) + ((z + z) + (z + z));
+    } else {
+        // eight chunks or more (small chunks of long camera rows): entry by
+        // entry, written as it is summed
+        for (int k = 0; k < NE; ++k) {
+            double w[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) w[u] = 0.0;
+            int ch = 0;
+            for (; ch + 8 <= nch; ch += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) w[u] += sp[(ch + u) * stride + k];
+            }
+            for (; ch < nch; ++ch) w[0] += sp[ch * stride + k];
+            const double sum = ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+            const int i = (e0 + k) / 6, c = (e0 + k) % 6;
+            if (s == 0) Lb[blk_off(lf, lf) + 6 * i + c] = Hc[28 * (size_t)f + sym6(i, c)] + (i == c ? lambda : 0.0) - sum;
+            else Lb[blk_off(lg, lf) + 6 * c + i] = -sum;
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int i = (e0 + k) / 6, c = (e0 + k) % 6;
+        if (s == 0) Lb[blk_off(lf, lf) + 6 * i + c] = h[k] - a[k];
+        else Lb[blk_off(lg, lf) + 6 * c + i] = -a[k];
+    }
+}
+
 __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, const double* __restrict__ spart,
                                                               const double* __restrict__ Hc,
-                                                              double* __restrict__ dp, LMDev* lm) {
+                                                              double* __restrict__ dp, LMDev* lm,
+                                                              const PoseQT* __restrict__ pose, PoseQT* __restrict__ pn,
+                                                              CamRT* __restrict__ cn, double* __restrict__ red_scale) {
+    constexpr int kRows = kSmallMaxN / 6;
     extern __shared__ double sm[];
     __shared__ double xk[6];
+    __shared__ double ssum[kSmallThreads / 64];
     __shared__ int bad;
-    __shared__ int rstart[kSmallMaxN / 6 + 1];
+    __shared__ int srow[kRows + 1];
+    __shared__ int rcol[kRows], rns[kRows], rnch[kRows];
+    __shared__ long long rp0[kRows];
     const int win = P.win_chol_order[blockIdx.x];
     if (!running(lm, win)) return;
     const int fc0 = P.win_fc0[win];
     const int nb = P.win_fc0[win + 1] - fc0;
-    if (nb == 0) return;
     const int n = 6 * nb;
     const int nblk = nb * (nb + 1) / 2;
     double* Lb = sm;                      // nblk * kBS
@@ -784,90 +891,88 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
     double* yv = Mb + (size_t)nb * kBS;    // n
     const int g0 = 6 * fc0;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid = wave_id();
     const double lambda = lm[win].lambda;
+    const int fail0 = lm[win].fail;
+    if (nb > 0) {
     // ---- the window's reduced system, summed from the Schur rows' partials
     // straight into LDS: per S row f, S_ff = Hpp + lambda I - sum and S_gf =
-    // -sum^T over the row's chunk items, the same sums in the same order as
-    // the one-problem reduction (k_schur_reduce), so the system has the same
-    // bits; blocks without co-visibility stay zero.  Batched rows have one
-    // column group (at most 30 slots).
+    // -sum^T over the row's chunk items, and g_f = bp_f - coefficient, the
+    // same sums in the same order as the one-problem reduction
+    // (k_schur_reduce), so both give the same bits; blocks without
+    // co-visibility stay zero.  A row of at most 30 slots is one column group,
+    // whose chunk items lie back to back (36 doubles per slot, then 6 of the
+    // coefficient).  One thread per block, or per row of a block, sums its
+    // entries (sum_block_part), and one more per row the row's coefficient.
     for (int i = tid; i < nblk * kBS; i += kSmallThreads) Lb[i] = 0.0;
-    if (tid == 0) {
-        int acc = 0;
-        for (int lf = 0; lf < nb; ++lf) {
-            rstart[lf] = acc;
-            acc += 36 * (P.colptr[fc0 + lf + 1] - P.colptr[fc0 + lf]);
+    if (wid == 0) {
+        int v = 0;
+        if (lane < nb) {
+            const int f = fc0 + lane;
+            const int c0 = P.colptr[f];
+            v = P.colptr[f + 1] - c0;
+            rcol[lane] = c0;
+            rns[lane] = v;
+            rnch[lane] = P.row_nchunks[f];
+            rp0[lane] = P.row_pofs0[f];
         }
-        rstart[nb] = acc;
-        bad = 0;
+        int incl = v;  // slots before each row: a wave prefix sum (nb <= 30 lanes)
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            const int o = __shfl_up(incl, d);
+            if (lane >= d) incl += o;
+        }
+        if (lane < nb) srow[lane] = incl - v;
+        if (lane == nb - 1) srow[nb] = incl;
+        if (lane == 0) bad = 0;
     }
     __syncthreads();
-    {
-        const int total = rstart[nb];
-        for (int t0 = tid; t0 < total; t0 += 4 * kSmallThreads) {
-            double v[4];
-            int dst[4];
+    const int nslots = srow[nb];
+    // blocks in six row tasks (all chunk loads of a task in flight at once)
+    // where they fit one pass, whole blocks otherwise
+    const bool rows6 = 6 * nslots + nb <= kSmallThreads;
+    const int ntask = rows6 ? 6 * nslots : nslots;
+    if (tid < ntask) {
+        if (rows6) sum_block_part<6>(P, spart, Hc, Lb, srow, rcol, rns, rnch, rp0, nb, fc0, lambda, tid);
+        else sum_block_part<36>(P, spart, Hc, Lb, srow, rcol, rns, rnch, rp0, nb, fc0, lambda, tid);
+    } else if (tid < ntask + nb) {
+        // right-hand side of row lf: g = bp - coefficient, the coefficient of the
+        // row's chunks in four partial sums (chunks congruent mod 4), as the reduction
+        const int lf = tid - ntask, f = fc0 + lf;
+        const int nch = rnch[lf];
+        const long long stride = 36LL * rns[lf] + 6;
+        const double* cq = spart + rp0[lf] + 36 * rns[lf];
+        double bp[6];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int t = t0 + u * kSmallThreads;
-                dst[u] = -1;
-                if (t >= total) continue;
-                int lf = 0;
-                while (rstart[lf + 1] <= t) ++lf;
-                const int f = fc0 + lf;
-                const int s = (t - rstart[lf]) / 36, e = (t - rstart[lf]) % 36;
-                const int nch = P.row_nchunks[f], base = P.row_sitem_ptr[f];
-                const double* sp = spart + 36 * s + e;
-                double a[8];
+        for (int e = 0; e < 6; ++e) bp[e] = Hc[28 * (size_t)f + 21 + e];
+        double cp[4][6];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) a[q] = 0.0;
-                int ch = 0;
-                for (; ch + 8 <= nch; ch += 8) {
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) a[q] += sp[P.sitem_pofs[base + ch + q]];
-                }
-                for (; ch < nch; ++ch) a[0] += sp[P.sitem_pofs[base + ch]];
-                const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-                const int i = e / 6, c = e % 6;
-                if (s == 0) {
-                    v[u] = Hc[28 * (size_t)f + sym6(i, c)] + (i == c ? lambda : 0.0) - sum;
-                    dst[u] = blk_off(lf, lf) + 6 * i + c;
-                } else {
-                    const int lg = P.colcam[P.colptr[f] + s] - fc0;
-                    v[u] = -sum;
-                    dst[u] = blk_off(lg, lf) + 6 * c + i;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (dst[u] >= 0) Lb[dst[u]] = v[u];
-        }
-        // right-hand side g = bp - coefficient, the coefficient of group 0's
-        // chunks in four partial sums (chunks congruent mod 4), as the reduction
-        for (int t = tid; t < n; t += kSmallThreads) {
-            const int lf = t / 6, e = t % 6;
-            const int f = fc0 + lf;
-            const int nch = P.row_nchunks[f], base = P.row_sitem_ptr[f];
-            double cp[4];
+            for (int e = 0; e < 6; ++e) cp[q][e] = 0.0;
+        for (int c0 = 0; c0 < nch; c0 += 4) {
+            double w[4][6];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                double c = 0.0;
-                for (int ch = q; ch < nch; ch += 4) {
-                    const int it = base + ch;
-                    c += spart[P.sitem_pofs[it] + 36 * P.sitem_nslots[it] + e];
-                }
-                cp[q] = c;
+                const double* p = cq + min(c0 + q, nch - 1) * stride;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) w[q][e] = p[e];
             }
-            yv[t] = Hc[28 * (size_t)f + 21 + e] - ((cp[0] + cp[1]) + (cp[2] + cp[3]));
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c0 + q < nch)
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) cp[q][e] += w[q][e];
         }
+#pragma unroll
+        for (int e = 0; e < 6; ++e) yv[6 * lf + e] = bp[e] - ((cp[0][e] + cp[1][e]) + (cp[2][e] + cp[3][e]));
     }
     __syncthreads();
-    if (tid == 0) {
-        if (!diag_factor(Lb + blk_off(0, 0), Mb, yv)) bad = 1;
-    }
+    if (tid == 0 && !diag_factor(Lb + blk_off(0, 0), Mb, yv)) bad = 1;
     __syncthreads();
     for (int K = 0; K < nb; ++K) {
-        // panel: L_IK = A_IK M_K^T, y_I -= L_IK y_K
+        // panel: L_IK = A_IK M_K^T, y_I -= L_IK y_K; block row K+1 in lanes 0..5 of wave 0
         const int rows = 6 * (nb - K - 1);
         const double* M = Mb + K * kBS;
         for (int t = tid; t < rows; t += kSmallThreads) {
@@ -890,46 +995,59 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
             yv[6 * I + r] -= yr;
         }
         __syncthreads();
-        // trailing update A_IJ -= L_IK L_JK^T; block (K+1, K+1) is item 0 and its
-        // thread factors it right away
         const int rem = nb - K - 1;
-        const int nupd = rem * (rem + 1) / 2;
-        for (int q = tid; q < nupd; q += kSmallThreads) {
-            // q -> (I, J) with I, J in K+1.., J <= I, column-major from the diagonal
-            int jj = 0, qq = q, len = rem;
-            while (qq >= len) { qq -= len; ++jj; --len; }
-            const int J = K + 1 + jj, I = J + qq;
-            const double* A = Lb + blk_off(I, K);
-            const double* B = Lb + blk_off(J, K);
-            double* C = Lb + blk_off(I, J);
-            double acc[36];
+        if (rem > 0) {
+            if (wid == 0) {
+                // the next diagonal block, A_{K+1,K+1} -= L_{K+1,K} L_{K+1,K}^T, one
+                // lower entry per lane, then its factor and inverse in lane 0
+                if (lane < 36) {
+                    const int r = lane / 6, c = lane - 6 * (lane / 6);
+                    if (c <= r) {
+                        const double* A = Lb + blk_off(K + 1, K);
+                        double* C = Lb + blk_off(K + 1, K + 1);
+                        double v = C[6 * r + c];
 #pragma unroll
-            for (int e = 0; e < 36; ++e) acc[e] = C[e];
-            double bm[36];
-#pragma unroll
-            for (int e = 0; e < 36; ++e) bm[e] = B[e];
-#pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                double ar[6];
-#pragma unroll
-                for (int m = 0; m < 6; ++m) ar[m] = A[6 * r + m];
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    double v = acc[6 * r + c];
-#pragma unroll
-                    for (int m = 0; m < 6; ++m) v -= ar[m] * bm[6 * c + m];
-                    acc[6 * r + c] = v;
+                        for (int m = 0; m < 6; ++m) v -= A[6 * r + m] * A[6 * c + m];
+                        C[6 * r + c] = v;
+                    }
                 }
-            }
+                wave_sync();
+                if (lane == 0 && !diag_factor(Lb + blk_off(K + 1, K + 1), Mb + (K + 1) * kBS, yv + 6 * (K + 1)))
+                    bad = 1;
+            } else {
+                // the rest of the trailing update, A_IJ -= L_IK L_JK^T for K < J <= I,
+                // one 6x6 block per thread of waves 1..7 (36 register accumulators)
+                const int nupd = rem * (rem + 1) / 2;
+                for (int q = 1 + tid - 64; q < nupd; q += kSmallThreads - 64) {
+                    // q -> (I, J), column-major from the diagonal
+                    int jj = 0, qq = q, len = rem;
+                    while (qq >= len) { qq -= len; ++jj; --len; }
+                    const int J = K + 1 + jj, I = J + qq;
+                    const double* A = Lb + blk_off(I, K);
+                    const double* B = Lb + blk_off(J, K);
+                    double* C = Lb + blk_off(I, J);
+                    double acc[36];
 #pragma unroll
-            for (int e = 0; e < 36; ++e) C[e] = acc[e];
-            if (q == 0) {  // block (K+1, K+1): this thread's own writes are visible to it
-                double yk[6];
+                    for (int e = 0; e < 36; ++e) acc[e] = C[e];
+                    double bm[36];
 #pragma unroll
-                for (int i = 0; i < 6; ++i) yk[i] = yv[6 * J + i];
-                if (!diag_factor(C, Mb + J * kBS, yk)) bad = 1;
+                    for (int e = 0; e < 36; ++e) bm[e] = B[e];
 #pragma unroll
-                for (int i = 0; i < 6; ++i) yv[6 * J + i] = yk[i];
+                    for (int r = 0; r < 6; ++r) {
+                        double ar[6];
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) ar[m] = A[6 * r + m];
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) {
+                            double v = acc[6 * r + c];
+#pragma unroll
+                            for (int m = 0; m < 6; ++m) v -= ar[m] * bm[6 * c + m];
+                            acc[6 * r + c] = v;
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 36; ++e) C[e] = acc[e];
+                }
             }
         }
         __syncthreads();
@@ -937,31 +1055,43 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
     if (bad) {
         for (int i = tid; i < n; i += kSmallThreads) dp[g0 + i] = 0.0;
         if (tid == 0) lm[win].fail = 1;
-        return;
-    }
-    // backward: x_K = M_K^T (y_K - sum_{I>K} L_IK^T x_I); y holds the updated right-hand sides
-    for (int K = nb - 1; K >= 0; --K) {
-        if (tid < 6) {
-            const double* M = Mb + K * kBS;
-            double v = 0.0;
-            for (int m = tid; m < 6; ++m) v += M[6 * m + tid] * yv[6 * K + m];
-            xk[tid] = v;
-        }
-        __syncthreads();
-        if (tid < 6) {
-            yv[6 * K + tid] = xk[tid];
-            dp[g0 + 6 * K + tid] = xk[tid];
-        }
-        for (int t = tid; t < 6 * K; t += kSmallThreads) {
-            const int J = t / 6, c = t - 6 * (t / 6);
-            const double* L = Lb + blk_off(K, J);
-            double v = 0.0;
+    } else if (wid == 0) {
+        // backward, by wave 0 alone: x_K = M_K^T (y_K - sum_{I>K} L_IK^T x_I);
+        // y holds the updated right-hand sides, then the step
+        for (int K = nb - 1; K >= 0; --K) {
+            if (lane < 6) {
+                const double* M = Mb + K * kBS;
+                double v = 0.0;
+                for (int m = lane; m < 6; ++m) v += M[6 * m + lane] * yv[6 * K + m];
+                xk[lane] = v;
+                yv[6 * K + lane] = v;
+                dp[g0 + 6 * K + lane] = v;
+            }
+            wave_sync();
+            for (int t = lane; t < 6 * K; t += 64) {
+                const int J = t / 6, c = t - 6 * (t / 6);
+                const double* L = Lb + blk_off(K, J);
+                double v = 0.0;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) v += L[6 * r + c] * xk[r];
-            yv[t] -= v;
+                for (int r = 0; r < 6; ++r) v += L[6 * r + c] * xk[r];
+                yv[t] -= v;
+            }
+            wave_sync();
         }
-        __syncthreads();
     }
+    }  // nb > 0
+    __syncthreads();
+    // candidate poses of the window's cameras from the step in LDS, and their
+    // part of x'(lambda x + b): the camera update (k_update_cam), with the same
+    // threads and the same sum
+    const int fail = (nb > 0 && bad) ? 1 : fail0;
+    double sc = 0.0;
+    if (tid < 256)
+        for (int c = P.win_cam0[win] + tid; c < P.win_cam0[win + 1]; c += 256)
+            sc += camera_candidate(camera_inputs(P, pose, Hc, c), lambda, fail, c,
+                                   [&](int f, int i) { return yv[6 * (f - fc0) + i]; }, pn, cn);
+    const double s = block_sum<double, 256>(sc, ssum);
+    if (tid == 0) red_scale[win] = s;
 }
 
 size_t cholesky_small_smem(const DevProblem& P) {
@@ -972,9 +1102,11 @@ size_t cholesky_small_smem(const DevProblem& P) {
 // One workgroup per window, largest windows first: with the largest window's
 // LDS a CU may hold one workgroup, and the small windows then fill in behind
 // the large ones (P.win_chol_order).
-void launch_cholesky_small(const DevProblem& P, const DevWork& W, hipStream_t st) {
+void launch_cholesky_small(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
+                           hipStream_t st) {
     hipLaunchKernelGGL(k_chol_small, dim3(P.n_win), dim3(kSmallThreads), cholesky_small_smem(P), st, P,
-                       (const double*)W.spart, (const double*)W.Hc, W.dp, W.lm);
+                       (const double*)W.spart, (const double*)W.Hc, W.dp, W.lm, (const PoseQT*)cur.pose, cand.pose,
+                       cand.cam, W.red_scale);
 }
 
 int prepare_cholesky_small(const DevProblem& P) {
@@ -984,17 +1116,19 @@ int prepare_cholesky_small(const DevProblem& P) {
                    hipSuccess ? 0 : -1;
 }
 
-// ---------------------------------------------------------------- one mid-size problem
+// ---------------------------------------------------------------- one mid-size problem, on request
 // One workgroup (512 threads) factors the whole system in 6x6 camera blocks,
-// the per-window algorithm above with the matrix in global memory (it stays
+// the per-window algorithm with the matrix in global memory (it stays
 // in the L2 of the CU's XCD: 490 KB at 58 free cameras).  The lower block
 // triangle is packed column by column (block (I, J), J <= I, at
 // J nb - J(J-1)/2 + (I - J)) and entry-major (entry e of block b at
 // e * nblk + b), so the threads of a trailing update, which own consecutive
 // blocks, read and write consecutive addresses.  The panel column of the
 // current step is staged in LDS.  One CU runs the whole chain of nb block
-// steps, so this wins only for small systems (chosen up to kBlocksAutoFc free
-// cameras; at 58 it measured 530 us against 349 us for the tiles).
+// steps through L2 (3.5 us per step at 11 free cameras, against 530 us in all
+// at 58 and 349 us for the tiles there), so the per-window kernel in LDS
+// replaced it for small systems; it stays as a switch for tests and sizes
+// between 31 and 128 free cameras.
 __device__ __forceinline__ int colstart(int J, int nb) { return J * nb - J * (J - 1) / 2; }
 
 // (I, J) of packed block b; J found from the column starts

@@ -171,13 +171,14 @@ struct nav_ba_handle {
     hipGraphExec_t step0 = nullptr;  // the first step of a solve
     bool timing = false;
     bool timing_active = false;
-    int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks
+    int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks, 3 in-LDS per window
     int schur_chunk = 0;  // observations per Schur work item: 0 = default or tuned
     int schur_chunk_used = 0;
     int schur_kernel = 0;  // Schur row kernel: 0 = default or tuned, 1 = rounds of kBlock, 2 = producer/consumer
     bool schur_pc_used = false;
     StageTimer timer;
     int* hdone = nullptr;      // pinned: windows finished
+    LMDev* hlm = nullptr;      // pinned: the LM state of up to kBurstWindows windows, between bursts
     char* stage = nullptr;     // pinned staging of a prepare's uploads (grown on demand)
     size_t stage_cap = 0;
     char* stage_s = nullptr;   // pinned staging of the Schur work items (the tuner re-uploads them)
@@ -336,7 +337,7 @@ template <typename T>
 void enqueue_reduced_system(nav_ba_handle* h) {
     if (h->P.n_fc > 0) {
         stage(h, NAV_BA_ST_SCHUR_ROW, [&] { launch_schur_row<T>(h->P, h->cur, h->W, h->stream); });
-        if (!h->P.batched)  // a batch's windows sum their rows in the factorisation kernel
+        if (!h->P.use_win)  // the per-window factorisation sums the rows itself
             stage(h, NAV_BA_ST_SCHUR_REDUCE, [&] { launch_schur_reduce<T>(h->P, h->W, h->stream); });
     }
 }
@@ -348,7 +349,7 @@ void enqueue_step(nav_ba_handle* h, bool first) {
     enqueue_reduced_system<T>(h);
     if (h->P.n_fc > 0)
         stage(h, NAV_BA_ST_CHOLESKY, [&] {
-            if (h->P.batched) launch_cholesky_small(h->P, h->W, h->stream);
+            if (h->P.use_win) launch_cholesky_small(h->P, h->cur, h->cand, h->W, h->stream);
             else launch_cholesky_solve(h->P, h->cur, h->cand, h->W, h->stream);
         });
     stage(h, NAV_BA_ST_BACKSUB, [&] { launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream); });
@@ -356,7 +357,7 @@ void enqueue_step(nav_ba_handle* h, bool first) {
         stage(h, NAV_BA_ST_UPDATE_CAM, [&] { launch_update_cam(h->P, h->cur, h->cand, h->W, h->stream); });
     stage(h, NAV_BA_ST_COST, [&] { launch_cost<T>(h->P, h->cand, h->W, 1, h->stream); });
     stage(h, NAV_BA_ST_ACCEPT, [&] {
-        launch_lm_accept(h->P, h->W, h->stream);
+        launch_lm_accept(h->P, h->cur, h->cand, h->W, h->stream);
         launch_commit(h->P, h->cur, h->cand, h->W, h->stream);
     });
 }
@@ -403,7 +404,7 @@ struct SchurItems {
     bool pc = false;                 // producer/consumer kernel (lanes bound per item)
     long long pofs = 0;
     std::vector<int> sf, sa0, sa1, sg, sns, srnd0, row_sitem_ptr, row_nch, sorder, rnd_pair0;
-    std::vector<long long> spofs;
+    std::vector<long long> spofs, row_pofs0;
     std::vector<uint8_t> lane_slot;  // [rounds][kBlock], or [items][kPcCons] for the producer/consumer kernel
     std::vector<uint8_t> rnd_group;  // [rounds] lanes per group (rounds of kBlock only)
     std::vector<int4> rnd_rows;      // [rounds]
@@ -518,6 +519,7 @@ bool build_schur_items(const SchurGraph& G, int chunk, bool pc, SchurItems& S) {
         const int nobs = G.cam_ptr[f + 1] - G.cam_ptr[f];
         const int nch = std::max(1, (nobs + chunk - 1) / chunk);
         S.row_nch.push_back(nch);
+        S.row_pofs0.push_back(S.pofs);
         for (int gi = 0; gi < ngroups; ++gi) {
             const int ns = std::min(sg_n, ncols - gi * sg_n);
             S.max_ns = std::max(S.max_ns, ns);
@@ -684,6 +686,7 @@ int upload_schur_items(nav_ba_handle* h, const SchurItems& S) {
 #define UP(dst, vec) pk.add(&(dst), (vec))
     UP(P.sitem_f, S.sf); UP(P.sitem_a0, S.sa0); UP(P.sitem_a1, S.sa1); UP(P.sitem_g, S.sg); UP(P.sitem_nslots, S.sns);
     UP(P.sitem_pofs, S.spofs); UP(P.row_sitem_ptr, S.row_sitem_ptr); UP(P.row_nchunks, S.row_nch);
+    UP(P.row_pofs0, S.row_pofs0);
     UP(P.sitem_order, S.sorder); UP(P.sitem_rnd0, S.srnd0);
     UP(P.rnd_lane_slot, S.lane_slot); UP(P.rnd_pair0, S.rnd_pair0); UP(P.rnd_rows, S.rnd_rows);
     UP(P.rnd_group, S.rnd_group); UP(P.pairs, S.pairs);
@@ -1387,8 +1390,16 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     P.n = L.n;
     P.n_pad = L.n_pad;
     P.nt = L.n_pad / kNB;
-    P.use_blocks = (!batched && nfc > 0 && nfc <= kBlocksMaxFc &&
-                    (h->solver_mode == 2 || (h->solver_mode == 0 && nfc <= kBlocksAutoFc))) ? 1 : 0;
+    // factorisation: in LDS per window (every batch; one problem of at most 30
+    // free cameras unless another is asked for), 64x64 tiles, or on request
+    // 6x6 blocks in global memory
+    P.use_win = (batched || (nfc > 0 && 6 * nfc <= kSmallMaxN && (h->solver_mode == 0 || h->solver_mode == 3))) ? 1 : 0;
+    P.use_blocks = (!P.use_win && nfc > 0 && nfc <= kBlocksMaxFc && h->solver_mode == 2) ? 1 : 0;
+    {
+        int maxpt = 0;
+        for (int w = 0; w < n_win; ++w) maxpt = std::max(maxpt, L.pt_first[w + 1] - L.pt_first[w]);
+        P.commit_in_accept = maxpt <= kCommitInAcceptPts ? 1 : 0;
+    }
 
     const size_t tsz = mixed(h) ? sizeof(float) : sizeof(double);
     const size_t nvec = batched ? std::max<size_t>(6 * (size_t)nfc, 1) : std::max<size_t>(L.n_pad, 1);
@@ -1427,7 +1438,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
         return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
     if (prepare_cholesky(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the factorisation kernels");
-    if (batched && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
+    if (P.use_win && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
     launch_fill_system(P, h->W, h->stream);
     if (tune) {
         if ((st = tune_schur_items(h, G, schur_mark, h->schur_kernel, h->schur_chunk, SI)) != NAV_BA_OK) return st;
@@ -1539,7 +1550,8 @@ int32_t nav_ba_create(nav_ba_handle** out, int32_t device) {
     h->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
-        hipHostMalloc((void**)&h->hdone, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&h->hdone, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h->hlm, sizeof(LMDev) * kBurstWindows, hipHostMallocDefault) != hipSuccess) {
         nav_ba_destroy(h);
         return NAV_BA_HIP_ERROR;
     }
@@ -1556,6 +1568,7 @@ void nav_ba_destroy(nav_ba_handle* h) {
     for (auto& b : h->bufs)
         if (b.p) (void)hipFree(b.p);
     if (h->hdone) (void)hipHostFree(h->hdone);
+    if (h->hlm) (void)hipHostFree(h->hlm);
     if (h->stage) (void)hipHostFree(h->stage);
     if (h->stage_s) (void)hipHostFree(h->stage_s);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -1596,20 +1609,31 @@ int32_t nav_ba_run(nav_ba_handle* h) {
     if (m) launch_cost<float>(P, h->cur, h->W, 0, s); else launch_cost<double>(P, h->cur, h->W, 0, s);
     launch_lm_set_chi(P, h->W, s);
     // every window needs at most n_iter * max_trials steps.  Steps are
-    // launched in bursts (n_iter first: one per iteration when every trial is
-    // accepted, then 2), the host checks the count of finished windows after
-    // each burst; steps past a window's end return at once from every kernel.
+    // launched in bursts: n_iter first (one per iteration when every trial is
+    // accepted); then, for up to kBurstWindows windows, half as many again as
+    // the most iterations a running window still has (a step past a window's
+    // end returns at once from every kernel, and costs about as much GPU time
+    // as the host's check costs idle time), else 2.  The host checks the
+    // count of finished windows after each burst.
     const long long max_steps = (long long)P.n_iter * P.max_trials;
     long long steps = 0;
     int st;
     long long burst = std::max(1, P.n_iter);
+    const bool few = P.n_win <= kBurstWindows;
     while (steps < max_steps) {
         for (long long i = 0; i < burst && steps < max_steps; ++i, ++steps)
             if ((st = launch_step(h, steps == 0)) != NAV_BA_OK) return st;
         HIPTRY(h, hipMemcpyAsync(h->hdone, h->W.done_count, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (few) HIPTRY(h, hipMemcpyAsync(h->hlm, h->W.lm, sizeof(LMDev) * P.n_win, hipMemcpyDeviceToHost, s));
         HIPTRY(h, hipStreamSynchronize(s));
         if (*h->hdone >= P.n_win) break;
         burst = 2;
+        if (few) {
+            int rem = 0;
+            for (int w = 0; w < P.n_win; ++w)
+                if (h->hlm[w].phase != kPhaseDone) rem = std::max(rem, P.n_iter - h->hlm[w].iter);
+            burst = std::max<long long>(2, rem + (rem + 1) / 2);
+        }
     }
     // outlier flags and the FP64 cost at the final state
     h->timing_active = h->timing;
@@ -1672,7 +1696,7 @@ int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out) {
 }
 
 int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode) {
-    if (!h || mode < 0 || mode > 2) return NAV_BA_BAD_ARG;
+    if (!h || mode < 0 || mode > 3) return NAV_BA_BAD_ARG;
     h->solver_mode = mode;
     return NAV_BA_OK;
 }
@@ -1791,6 +1815,9 @@ int32_t nav_ba_debug_reduced_system(nav_ba_handle* h, double lambda, double* S, 
     launch_cam_rt(P, h->cur, s);
     if (mixed(h)) { enqueue_lin<float>(h, false); enqueue_reduced_system<float>(h); }
     else { enqueue_lin<double>(h, false); enqueue_reduced_system<double>(h); }
+    if (P.use_win && P.n_fc > 0) {  // the step sums the system inside its factorisation: form S here
+        if (mixed(h)) launch_schur_reduce<float>(P, h->W, s); else launch_schur_reduce<double>(P, h->W, s);
+    }
     HIPTRY(h, hipStreamSynchronize(s));
     if (P.n > max_n) return NAV_BA_OK;
     std::vector<double> Sd((size_t)P.n_pad * P.n_pad), gd(P.n_pad);

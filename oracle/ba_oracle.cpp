@@ -372,11 +372,20 @@ int build_graph(const nav_ba_problem* p, const nav_ba_options* o, Graph& g) {
 
 // activeRobustChi2 over all edges (edges with all vertices fixed included: they
 // only add a constant).
+// Order of the oracle's long sums: 0 as written (edges and points ascending);
+// 1 reversed (the cost over edges and the Schur complement over points in
+// descending order).  The reversed order is a test twin: the same mathematics
+// with other roundings, which measures how far a problem's result moves with
+// the order of its sums alone.
+static int g_sum_order = 0;
+
 double robust_chi2(const Graph& g, const std::vector<Pose>& pose, const std::vector<double>& X,
                    double* chi2_out = nullptr) {
     double chi = 0.0;
     double e[2], Xc[3], rho[3];
-    for (size_t k = 0; k < g.e_cam.size(); ++k) {
+    const size_t ne = g.e_cam.size();
+    for (size_t kk = 0; kk < ne; ++kk) {
+        const size_t k = g_sum_order ? ne - 1 - kk : kk;
         edge_error(pose[g.e_cam[k]], &X[3 * (size_t)g.e_pt[k]], g.K, &g.e_uv[2 * k], e, Xc);
         const double c2 = e[0] * e[0] + e[1] * e[1];
         if (chi2_out) chi2_out[k] = c2;
@@ -496,7 +505,8 @@ bool schur_solve(const Graph& g, const System& s, double lambda, std::vector<dou
             for (int j = 0; j < 6; ++j)
                 S[(size_t)(6 * c + i) * n + 6 * c + j] = s.Hpp[36 * (size_t)c + 6 * i + j] + (i == j ? lambda : 0.0);
     std::vector<double> Dinv(9 * (size_t)g.n_fp);
-    for (int l = 0; l < g.n_fp; ++l) {
+    for (int ll = 0; ll < g.n_fp; ++ll) {
+        const int l = g_sum_order ? g.n_fp - 1 - ll : ll;
         double D[9];
         for (int t = 0; t < 9; ++t) D[t] = s.Hll[9 * (size_t)l + t] + ((t % 4 == 0) ? lambda : 0.0);
         double* Di = &Dinv[9 * (size_t)l];
@@ -591,6 +601,8 @@ double max_abs_diag(const Graph& g, const System& s) {
 }  // namespace
 
 extern "C" {
+
+void oracle_set_sum_order(int32_t order) { g_sum_order = order ? 1 : 0; }
 
 void oracle_quat_from_R(const double R[9], double q[4]) {
     Quat r = quat_from_R(R);

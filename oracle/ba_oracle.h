@@ -30,6 +30,12 @@
 extern "C" {
 #endif
 
+/* Order of the oracle's long sums for the calls that follow: 0 as written
+ * (edges and points ascending, the default), 1 reversed (the cost over edges
+ * and the Schur complement over points descending).  The reversed order is a
+ * test twin: the same algorithm with other roundings. */
+void oracle_set_sum_order(int32_t order);
+
 /* Full solve with g2o semantics.  Same contract as nav_ba_solve. */
 int32_t oracle_ba_solve(const nav_ba_problem* p, const nav_ba_options* o, nav_ba_result* r,
                         double* lambda_history /* [n_iter] lambda at start of each iteration, or NULL */,

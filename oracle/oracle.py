@@ -35,6 +35,8 @@ def lib():
         L = C.CDLL(LIB)
         P, O = C.POINTER(NavBaProblem), C.POINTER(NavBaOptions)
         L.oracle_ba_solve.argtypes = [P, O, C.c_void_p, _dp, _i32p]
+        L.oracle_set_sum_order.argtypes = [C.c_int32]
+        L.oracle_set_sum_order.restype = None
         L.oracle_ba_solve.restype = C.c_int32
         L.oracle_ba_solve_trace.argtypes = [P, O, C.c_void_p, _dp, C.c_int32, _i32p]
         L.oracle_ba_solve_trace.restype = C.c_int32
@@ -101,13 +103,20 @@ def _d(a):
     return a.ctypes.data_as(_dp)
 
 
-def solve(prob: Problem, opts: Options, with_history: bool = False):
+def solve(prob: Problem, opts: Options, with_history: bool = False, reversed_sums: bool = False):
+    """The oracle's solve (ba_oracle.cpp solve_impl).  reversed_sums: the
+    cost over edges and the Schur complement over points summed in descending
+    order, a twin of the same algorithm with other roundings."""
     ps, os_ = prob.c_struct(), opts.c_struct()
     rb = ResultBuffers(prob, opts.n_iter)
     lam = np.zeros(max(opts.n_iter, 1), np.float64)
     tri = np.zeros(max(opts.n_iter, 1), np.int32)
-    st = lib().oracle_ba_solve(C.byref(ps), C.byref(os_), C.addressof(rb.c), _d(lam),
-                               tri.ctypes.data_as(_i32p))
+    L = lib()
+    L.oracle_set_sum_order(1 if reversed_sums else 0)
+    try:
+        st = L.oracle_ba_solve(C.byref(ps), C.byref(os_), C.addressof(rb.c), _d(lam), tri.ctypes.data_as(_i32p))
+    finally:
+        L.oracle_set_sum_order(0)
     res = rb.result(st)
     if with_history:
         return res, lam[: res.iterations], tri[: res.iterations]

@@ -32,7 +32,7 @@ def _solve_window(args):
         p.T_cw[:, 9:] = np.nextafter(p.T_cw[:, 9:], up)
     if perturb in ("t-", "X+t-"):
         p.T_cw[:, 9:] = np.nextafter(p.T_cw[:, 9:], down)
-    r = oracle.solve(p, Options(**opt_fields))
+    r = oracle.solve(p, Options(**opt_fields), reversed_sums=perturb == "order")
     return {"seed": seed, "final_cost": r.final_cost, "cost_history": r.cost_history.tolist(),
             "iterations": r.iterations, "trials": r.trials, "stop_reason": r.stop_reason,
             "n_outliers": r.n_outliers, "outlier": r.outlier.tobytes(),
@@ -48,7 +48,8 @@ def solve_windows(seeds, opts, perturb=None):
     """The oracle's solve of window `seed` of the uneven-window generator, for
     each seed, in worker processes started fresh (spawn).  perturb: None, or
     one ulp on every point coordinate ("X+", "X-"), on every camera
-    translation ("t+", "t-"), or both ("X+t-")."""
+    translation ("t+", "t-"), or both ("X+t-"); or the unperturbed window with
+    the oracle's long sums in reversed order ("order")."""
     fields = {k: getattr(opts, k) for k in ("n_iter", "robust", "huber_delta", "lambda_init", "precision",
                                             "compat_fp32_io", "outlier_chi2", "max_trials")}
     with ProcessPoolExecutor(max_workers=workers(), mp_context=mp.get_context("spawn")) as ex:

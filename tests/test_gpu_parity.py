@@ -50,6 +50,49 @@ def test_fp64_both_factorisations_match_oracle(solver, oracle_mod, mode):
     check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_fp64_every_factorisation_of_a_small_problem(solver, oracle_mod, mode):
+    """22 free cameras: automatic (the per-window kernel in LDS), 64x64 tiles,
+    6x6 blocks in global memory and the per-window kernel by request each
+    reach the oracle's result."""
+    _say(f"fp64 solve workload=1 n_cam=24 n_pt=3000 factorisation mode {mode}")
+    prob = problems.workload(1, seed=5, n_cam=24, n_pt=3000)
+    opts = workload_options(1)
+    opts.precision = NAV_BA_FP64
+    opts.n_iter = 8
+    ref = oracle_mod.solve(prob, opts)
+    solver.set_solver(mode)
+    try:
+        got = solver.solve(prob, opts)
+    finally:
+        solver.set_solver(0)
+    check_fp64_against_oracle(oracle_mod, prob, opts, got, ref)
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_window_factorisation_equals_a_batch_of_one(solver, kernel):
+    """A single window through the per-window factorisation and the same
+    window as a batch of one run the same kernels on the same sums (given the
+    same Schur-row kernel and chunk): the results agree to the last bit."""
+    prob = problems.workload(3, seed=6)
+    opts = workload_options(3)
+    opts.precision = NAV_BA_FP64
+    _say(f"window 6 single vs batch of one, Schur-row kernel {kernel}")
+    solver.set_schur_kernel(kernel)
+    solver.set_schur_chunk(256)
+    try:
+        a = solver.solve(prob, opts)
+        (b,) = solver.solve_batched([prob], opts)
+    finally:
+        solver.set_schur_kernel(0)
+        solver.set_schur_chunk(0)
+    _say(f"  single {a.final_cost!r} batch {b.final_cost!r}")
+    assert a.iterations == b.iterations and a.trials == b.trials
+    assert a.final_cost == b.final_cost
+    assert np.array_equal(a.T_cw, b.T_cw) and np.array_equal(a.X, b.X)
+    assert np.array_equal(a.outlier, b.outlier)
+
+
 @pytest.mark.parametrize("n_cam", [24, 19, 20])
 def test_fp64_tiled_partial_last_tile(solver, oracle_mod, n_cam):
     """Tiled factorisation whose last 64-row tile holds 4, 38 and 44 rows of

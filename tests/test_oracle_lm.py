@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from nav24_amd import problems
-from nav24_amd.abi import NAV_BA_EMPTY, Options, Problem, workload_options
+from nav24_amd.abi import NAV_BA_EMPTY, NAV_BA_FP64, Options, Problem, workload_options
 from oracle import oracle
 
 
@@ -230,3 +230,19 @@ def test_lm_trace_pin_catches_planted_slip(tmp_path, name, old, new):
     res, tr = oracle.solve_trace(prob, o, library=ctypes.CDLL(str(so)))
     with pytest.raises(AssertionError):
         check_lm_trace(tr, res, o.lambda_init)
+
+
+def test_reversed_sums_twin_moves_only_roundings():
+    """The reversed-order twin (cost over edges and Schur complement over
+    points summed descending) is the same algorithm: on a well-conditioned
+    window the schedule is the same and the cost moves by rounding only, and
+    the setting does not outlive the call."""
+    w = problems.workload(3, seed=0)
+    o = workload_options(3)
+    o.precision = NAV_BA_FP64
+    a = oracle.solve(w, o)
+    b = oracle.solve(w, o, reversed_sums=True)
+    c = oracle.solve(w, o)
+    assert (a.iterations, a.trials) == (b.iterations, b.trials)
+    assert abs(a.final_cost - b.final_cost) <= 1e-11 * a.final_cost
+    assert a.final_cost == c.final_cost and np.array_equal(a.X, c.X)
