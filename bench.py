@@ -919,6 +919,8 @@ def main():
                     help="profiling: run only the batched solve (K timed runs) and print its timing")
     ap.add_argument("--single-only", action="store_true",
                     help="profiling: run only the single-problem leg of --workload and print it")
+    ap.add_argument("--window-only", action="store_true",
+                    help="profiling: run only the per-call window leg (--window-solves windows) and print it")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="profiling (--batched-only): precision of the batched solve")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -938,6 +940,9 @@ def main():
         return
     if args.single_only:
         print(json.dumps(single_leg(args.workload, args.steps, args.warmup, not args.no_cpu)), flush=True)
+        return
+    if args.window_only:
+        print(json.dumps(window_leg(max(args.window_solves, 1), 10, not args.no_cpu)), flush=True)
         return
     with_cpu = not args.no_cpu
     head = headline_leg(args.windows, args.steps, args.warmup, with_cpu)

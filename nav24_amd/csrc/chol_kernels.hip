@@ -770,6 +770,100 @@ __device__ __forceinline__ bool diag_factor(double* a, double* m, double* y) {
     return ok_all;
 }
 
+// Block column C of a window's system in LDS, by one wave: first the update
+// by panel K (A_IC -= L_IK L_CK^T for I >= C; none when K < 0), then its
+// factorisation in registers, one row per lane and slot (rows 6C + lane +
+// 64 q): for each of its six columns j, the pivot l_jj = sqrt(d_j) (1/sqrt by
+// rsq and two Newton steps, kept in dinv), the rows below scaled by 1/l_jj,
+// the forward step y_j = y_j / l_jj and y_r -= l_rj y_j below, and the rank-1
+// update of the column block's later columns; the pivots and multipliers come
+// from lanes 0..5 as wave-uniform values.  The factor L_IC overwrites the
+// column block and y its rows.  Returns false on a pivot that is not
+// positive and finite (replaced by 1), the same in every lane.
+__device__ __forceinline__ bool win_panel(double* Lb, double* yv, double* dinv, int n, int C, int K, int lane) {
+    constexpr int kQ = (kSmallMaxN + 63) / 64;
+    const int r0 = 6 * C;
+    double v[kQ][6], yq[kQ];
+    double u[6][6];  // L_CK, wave-uniform
+    if (K >= 0) {
+#pragma unroll
+        for (int e = 0; e < 36; ++e) u[e / 6][e % 6] = Lb[blk_off(C, K) + e];
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int r = r0 + lane + 64 * q;
+        yq[q] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[q][c] = 0.0;
+        if (r < n) {
+            const int I = r / 6, i = r - 6 * (r / 6);
+            const double* a = Lb + blk_off(I, C) + 6 * i;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v[q][c] = a[c];
+            yq[q] = yv[r];
+            if (K >= 0) {
+                const double* l = Lb + blk_off(I, K) + 6 * i;
+                double lk[6];
+#pragma unroll
+                for (int m = 0; m < 6; ++m) lk[m] = l[m];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    double w = v[q][c];
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) w -= lk[m] * u[c][m];
+                    v[q][c] = w;
+                }
+            }
+        }
+    }
+    bool ok_all = true;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double d = lane_value(v[0][j], j);
+        const bool ok = (d > 0.0) && isfinite(d);
+        ok_all = ok_all && ok;
+        d = ok ? d : 1.0;
+        double rs = __builtin_amdgcn_rsq(d);
+        rs = rs * (1.5 - 0.5 * d * rs * rs);
+        rs = rs * (1.5 - 0.5 * d * rs * rs);
+        const double yj = lane_value(yq[0], j) * rs;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int lr = lane + 64 * q;
+            if (lr > j && r0 + lr < n) {
+                v[q][j] *= rs;
+                yq[q] -= v[q][j] * yj;
+            }
+        }
+        if (lane == j) {
+            v[0][j] = d * rs;
+            yq[0] = yj;
+        }
+#pragma unroll
+        for (int c = j + 1; c < 6; ++c) {
+            const double lcj = lane_value(v[0][j], c);
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const int lr = lane + 64 * q;
+                if (lr >= c && r0 + lr < n) v[q][c] -= v[q][j] * lcj;
+            }
+        }
+        if (lane == 0) dinv[r0 + j] = rs;
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int r = r0 + lane + 64 * q;
+        if (r < n) {
+            const int I = r / 6, i = r - 6 * (r / 6);
+            double* a = Lb + blk_off(I, C) + 6 * i;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a[c] = v[q][c];
+            yv[r] = yq[q];
+        }
+    }
+    return ok_all;
+}
+
 // Part of one (row, slot) block of a window's reduced system from the Schur
 // rows' partials: NE = 36 entries (the whole block, 16-byte loads chunk by
 // chunk) or NE = 6 (one row of the block, the loads of up to seven chunks in
@@ -887,8 +981,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
     const int n = 6 * nb;
     const int nblk = nb * (nb + 1) / 2;
     double* Lb = sm;                      // nblk * kBS
-    double* Mb = sm + (size_t)nblk * kBS;  // nb * kBS: M_K = L_KK^-1
-    double* yv = Mb + (size_t)nb * kBS;    // n
+    double* yv = sm + (size_t)nblk * kBS;  // n: right-hand side, then the step
+    double* dinv = yv + n;                 // n: 1 / l_jj
     const int g0 = 6 * fc0;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -969,85 +1063,49 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
         for (int e = 0; e < 6; ++e) yv[6 * lf + e] = bp[e] - ((cp[0][e] + cp[1][e]) + (cp[2][e] + cp[3][e]));
     }
     __syncthreads();
-    if (tid == 0 && !diag_factor(Lb + blk_off(0, 0), Mb, yv)) bad = 1;
+    // ---- factorisation, right-looking by block columns of six.  Wave 0 keeps
+    // the chain: it updates block column C = K+1 with panel K and factors it
+    // in registers (win_panel), while waves 1..7 apply panel K to the block
+    // columns beyond C; one barrier per block column.
+    if (wid == 0 && !win_panel(Lb, yv, dinv, n, 0, -1, lane) && lane == 0) bad = 1;
     __syncthreads();
-    for (int K = 0; K < nb; ++K) {
-        // panel: L_IK = A_IK M_K^T, y_I -= L_IK y_K; block row K+1 in lanes 0..5 of wave 0
-        const int rows = 6 * (nb - K - 1);
-        const double* M = Mb + K * kBS;
-        for (int t = tid; t < rows; t += kSmallThreads) {
-            const int I = K + 1 + t / 6, r = t - 6 * (t / 6);
-            double* row = Lb + blk_off(I, K) + 6 * r;
-            double a[6], o[6];
+    for (int K = 0; K + 1 < nb; ++K) {
+        if (wid == 0) {
+            if (!win_panel(Lb, yv, dinv, n, K + 1, K, lane) && lane == 0) bad = 1;
+        } else {
+            // A_IJ -= L_IK L_JK^T for K+2 <= J <= I, one 6x6 block per thread of
+            // waves 1..7 (36 register accumulators)
+            const int rem = nb - K - 2;
+            const int nupd = rem * (rem + 1) / 2;
+            for (int q = tid - 64; q < nupd; q += kSmallThreads - 64) {
+                // q -> (I, J), column-major from the diagonal
+                int jj = 0, qq = q, len = rem;
+                while (qq >= len) { qq -= len; ++jj; --len; }
+                const int J = K + 2 + jj, I = J + qq;
+                const double* A = Lb + blk_off(I, K);
+                const double* B = Lb + blk_off(J, K);
+                double* C = Lb + blk_off(I, J);
+                double acc[36];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) a[c] = row[c];
-            double yr = 0.0;
+                for (int e = 0; e < 36; ++e) acc[e] = C[e];
+                double bm[36];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                double v = 0.0;
+                for (int e = 0; e < 36; ++e) bm[e] = B[e];
 #pragma unroll
-                for (int m = 0; m <= c; ++m) v += a[m] * M[6 * c + m];
-                o[c] = v;
-                yr += v * yv[6 * K + c];
-            }
+                for (int r = 0; r < 6; ++r) {
+                    double ar[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) row[c] = o[c];
-            yv[6 * I + r] -= yr;
-        }
-        __syncthreads();
-        const int rem = nb - K - 1;
-        if (rem > 0) {
-            if (wid == 0) {
-                // the next diagonal block, A_{K+1,K+1} -= L_{K+1,K} L_{K+1,K}^T, one
-                // lower entry per lane, then its factor and inverse in lane 0
-                if (lane < 36) {
-                    const int r = lane / 6, c = lane - 6 * (lane / 6);
-                    if (c <= r) {
-                        const double* A = Lb + blk_off(K + 1, K);
-                        double* C = Lb + blk_off(K + 1, K + 1);
-                        double v = C[6 * r + c];
+                    for (int m = 0; m < 6; ++m) ar[m] = A[6 * r + m];
 #pragma unroll
-                        for (int m = 0; m < 6; ++m) v -= A[6 * r + m] * A[6 * c + m];
-                        C[6 * r + c] = v;
+                    for (int c = 0; c < 6; ++c) {
+                        double v = acc[6 * r + c];
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) v -= ar[m] * bm[6 * c + m];
+                        acc[6 * r + c] = v;
                     }
                 }
-                wave_sync();
-                if (lane == 0 && !diag_factor(Lb + blk_off(K + 1, K + 1), Mb + (K + 1) * kBS, yv + 6 * (K + 1)))
-                    bad = 1;
-            } else {
-                // the rest of the trailing update, A_IJ -= L_IK L_JK^T for K < J <= I,
-                // one 6x6 block per thread of waves 1..7 (36 register accumulators)
-                const int nupd = rem * (rem + 1) / 2;
-                for (int q = 1 + tid - 64; q < nupd; q += kSmallThreads - 64) {
-                    // q -> (I, J), column-major from the diagonal
-                    int jj = 0, qq = q, len = rem;
-                    while (qq >= len) { qq -= len; ++jj; --len; }
-                    const int J = K + 1 + jj, I = J + qq;
-                    const double* A = Lb + blk_off(I, K);
-                    const double* B = Lb + blk_off(J, K);
-                    double* C = Lb + blk_off(I, J);
-                    double acc[36];
 #pragma unroll
-                    for (int e = 0; e < 36; ++e) acc[e] = C[e];
-                    double bm[36];
-#pragma unroll
-                    for (int e = 0; e < 36; ++e) bm[e] = B[e];
-#pragma unroll
-                    for (int r = 0; r < 6; ++r) {
-                        double ar[6];
-#pragma unroll
-                        for (int m = 0; m < 6; ++m) ar[m] = A[6 * r + m];
-#pragma unroll
-                        for (int c = 0; c < 6; ++c) {
-                            double v = acc[6 * r + c];
-#pragma unroll
-                            for (int m = 0; m < 6; ++m) v -= ar[m] * bm[6 * c + m];
-                            acc[6 * r + c] = v;
-                        }
-                    }
-#pragma unroll
-                    for (int e = 0; e < 36; ++e) C[e] = acc[e];
-                }
+                for (int e = 0; e < 36; ++e) C[e] = acc[e];
             }
         }
         __syncthreads();
@@ -1056,16 +1114,25 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
         for (int i = tid; i < n; i += kSmallThreads) dp[g0 + i] = 0.0;
         if (tid == 0) lm[win].fail = 1;
     } else if (wid == 0) {
-        // backward, by wave 0 alone: x_K = M_K^T (y_K - sum_{I>K} L_IK^T x_I);
-        // y holds the updated right-hand sides, then the step
+        // backward, by wave 0 alone: x_K = L_KK^-T (y_K - sum_{I>K} L_IK^T x_I),
+        // the 6x6 solve in lane 0; y holds the updated right-hand sides, then the step
         for (int K = nb - 1; K >= 0; --K) {
-            if (lane < 6) {
-                const double* M = Mb + K * kBS;
-                double v = 0.0;
-                for (int m = lane; m < 6; ++m) v += M[6 * m + lane] * yv[6 * K + m];
-                xk[lane] = v;
-                yv[6 * K + lane] = v;
-                dp[g0 + 6 * K + lane] = v;
+            if (lane == 0) {
+                const double* L = Lb + blk_off(K, K);
+                double x[6];
+#pragma unroll
+                for (int i = 5; i >= 0; --i) {
+                    double v = yv[6 * K + i];
+#pragma unroll
+                    for (int k = i + 1; k < 6; ++k) v -= L[6 * k + i] * x[k];
+                    x[i] = v * dinv[6 * K + i];
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    xk[i] = x[i];
+                    yv[6 * K + i] = x[i];
+                    dp[g0 + 6 * K + i] = x[i];
+                }
             }
             wave_sync();
             for (int t = lane; t < 6 * K; t += 64) {
@@ -1096,7 +1163,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
 
 size_t cholesky_small_smem(const DevProblem& P) {
     const size_t nb = (size_t)P.max_win_fc;
-    return sizeof(double) * (nb * (nb + 1) / 2 * kBS + nb * kBS + 6 * nb);
+    return sizeof(double) * (nb * (nb + 1) / 2 * kBS + 12 * nb);
 }
 
 // One workgroup per window, largest windows first: with the largest window's

@@ -36,12 +36,16 @@ namespace {
 constexpr const char* kVersion = "nav24_amd-ba 0.2 (gfx950)";
 constexpr int kMaxDenseN = 16384;  // tiled dense reduced system of one window (2 GiB FP64)
 // Schur chunk tuner: problems with at least kTuneMinObs observations time each
-// candidate chunk size once at prepare (best of kTuneReps launches) and keep a
-// candidate only if it beats the current best by the margin.
+// candidate chunk size once at prepare (best of kTuneReps launches) and take
+// the largest size within kTuneNear of the fastest; the other row kernel is
+// taken only if it beats that by kTuneMargin.  Sizes whose times differ by
+// less than the timing noise give different roundings, so the rule prefers a
+// stable pick (the largest near-best size) to the fastest of a noisy few.
 constexpr int kTuneMinObs = 1 << 17;
 constexpr int kTuneChunks[] = {128, 256, 512, 1024, 2048, 4096};
 constexpr int kTuneReps = 3;
-constexpr float kTuneMargin = 0.97f;
+constexpr float kTuneNear = 1.05f;
+constexpr float kTuneMargin = 0.95f;
 
 struct DevBuf {
     void* p = nullptr;
@@ -734,12 +738,13 @@ int time_schur(nav_ba_handle* h, float* best_ms) {
     *best_ms = 1e30f;
     for (int r = 0; r <= kTuneReps; ++r) {
         HIPTRY(h, hipEventRecord(h->ev0, h->stream));
+        const bool red = !h->P.use_win;  // the per-window factorisation sums the rows itself
         if (mixed(h)) {
             launch_schur_row<float>(h->P, h->cur, h->W, h->stream);
-            launch_schur_reduce<float>(h->P, h->W, h->stream);
+            if (red) launch_schur_reduce<float>(h->P, h->W, h->stream);
         } else {
             launch_schur_row<double>(h->P, h->cur, h->W, h->stream);
-            launch_schur_reduce<double>(h->P, h->W, h->stream);
+            if (red) launch_schur_reduce<double>(h->P, h->W, h->stream);
         }
         HIPTRY(h, hipEventRecord(h->ev1, h->stream));
         HIPTRY(h, hipEventSynchronize(h->ev1));
@@ -752,9 +757,10 @@ int time_schur(nav_ba_handle* h, float* best_ms) {
 
 // Pick the kernel and chunk size of the Schur items by timing candidates on
 // the problem itself, linearised at its initial state: every chunk size with
-// the default kernel, then the other kernel at the best size.  `S` holds the
-// default items, uploaded from allocation slot `mark` on; it returns holding
-// (and with the device holding) the fastest candidate's items.  Chunk sizes
+// the default kernel (the largest within kTuneNear of the fastest is kept),
+// then the other kernel at that size.  `S` holds the default items, uploaded
+// from allocation slot `mark` on; it returns holding (and with the device
+// holding) the chosen candidate's items.  Chunk sizes
 // above the longest camera row give the same items as the longest row's size
 // and are skipped.  `kernel` (1 or 2, also the default kernel then) and
 // `chunk_only` (> 0) limit the candidates to one kernel or one size; 0
@@ -777,28 +783,41 @@ int tune_schur_items(nav_ba_handle* h, const SchurGraph& G, int mark, int kernel
     int longest = 0;
     for (int f = 0; f < G.nfc; ++f) longest = std::max(longest, G.cam_ptr[f + 1] - G.cam_ptr[f]);
     SchurItems C;
-    auto trial = [&](int chunk, bool pc) -> int {
+    // time one candidate; ms < 0 when it cannot run (pair entries beyond int32, LDS)
+    auto trial = [&](int chunk, bool pc, float* ms) -> int {
+        *ms = -1.f;
         if (!build_schur_items(G, chunk, pc, C)) return NAV_BA_OK;
         h->nbuf = mark;
         int st2;
         if ((st2 = upload_schur_items(h, C)) != NAV_BA_OK) return st2;
         if ((mixed(h) ? prepare_schur_row<float>(h->P) : prepare_schur_row<double>(h->P)) != 0) return NAV_BA_OK;
-        float ms;
-        if ((st2 = time_schur(h, &ms)) != NAV_BA_OK) return st2;
-        if (ms < kTuneMargin * best_ms) { best_ms = ms; std::swap(S, C); }
-        return NAV_BA_OK;
+        return time_schur(h, ms);
     };
-    // the sizes with the default kernel, then the other kernel at the best size
+    // the sizes with the default kernel: the largest within kTuneNear of the fastest
     if (kernel == 0 || (kernel == 2) == def_pc) {
+        std::vector<std::pair<int, float>> timed{{def, best_ms}};
         int prev = 0;
         for (int chunk : kTuneChunks) {
             if (prev >= longest) break;  // the same items as the previous size
             prev = chunk;
             if (chunk == def || (chunk_only > 0 && chunk != chunk_only)) continue;
-            if ((st = trial(chunk, def_pc)) != NAV_BA_OK) return st;
+            float ms;
+            if ((st = trial(chunk, def_pc, &ms)) != NAV_BA_OK) return st;
+            if (ms > 0.f) timed.push_back({chunk, ms});
         }
+        float tmin = best_ms;
+        for (const auto& c : timed) tmin = std::min(tmin, c.second);
+        int pick = 0;
+        for (const auto& c : timed)
+            if (c.second <= kTuneNear * tmin && c.first > pick) { pick = c.first; best_ms = c.second; }
+        if (pick != def && build_schur_items(G, pick, def_pc, C)) std::swap(S, C);
     }
-    if (kernel == 0 && (st = trial(S.chunk, !def_pc)) != NAV_BA_OK) return st;
+    // the other kernel at that size
+    if (kernel == 0) {
+        float ms;
+        if ((st = trial(S.chunk, !def_pc, &ms)) != NAV_BA_OK) return st;
+        if (ms > 0.f && ms < kTuneMargin * best_ms) std::swap(S, C);
+    }
     h->nbuf = mark;
     if ((st = upload_schur_items(h, S)) != NAV_BA_OK) return st;
     if ((mixed(h) ? prepare_schur_row<float>(h->P) : prepare_schur_row<double>(h->P)) != 0)

@@ -147,8 +147,11 @@ def test_loop_full_size_fp64_against_oracle(solver, oracle_mod):
     from the start, and a point crossing a camera plane during the large
     steps of iterations 7-9 makes the path depend on the last bits (one ulp of
     input moves the cost by at most 3e-9 up to iteration 7, by 1e-6 at
-    iteration 8, by up to 2e-4 at the end).  So each iteration's cost is held to
-    max(1e-9, 4 x that iteration's one-ulp sensitivity), the final cost
+    iteration 8, by up to 2e-4 at the end).  The order of the GPU's own sums
+    moves it too, so the same solve runs again with two other Schur chunk
+    sizes and row kernels (valid orders of the same sums) and their spread is
+    printed.  Each iteration's cost is held to max(1e-9, 4 s), s the larger of that
+    iteration's one-ulp sensitivity and the GPU's order spread, the final cost
     likewise; the flags must be the oracle's flags at the GPU's final state."""
     import json
     import os
@@ -162,15 +165,27 @@ def test_loop_full_size_fp64_against_oracle(solver, oracle_mod):
     o.precision = NAV_BA_FP64
     say(f"loop at full size: {prob.n_obs} observations, FP64, 30 iterations")
     got = solver.solve(prob, o)
+    alts = []
+    for kernel, chunk in ((1, 256), (2, 1024)):
+        solver.set_schur_kernel(kernel)
+        solver.set_schur_chunk(chunk)
+        try:
+            alts.append(solver.solve(prob, o))
+        finally:
+            solver.set_schur_kernel(0)
+            solver.set_schur_chunk(0)
     ref_h = np.array(fx["cost_history"])
-    sens_h = np.array(fx["one_ulp_sensitivity"]["history"])
-    n = min(len(got.cost_history), len(ref_h))
+    n = min([len(got.cost_history), len(ref_h)] + [len(a.cost_history) for a in alts])
+    order_h = np.max([np.abs(a.cost_history[:n] - got.cost_history[:n]) / ref_h[:n] for a in alts], axis=0)
+    sens_h = np.maximum(np.array(fx["one_ulp_sensitivity"]["history"])[:n], order_h)
     r = np.abs(got.cost_history[:n] - ref_h[:n]) / ref_h[:n]
-    bar = np.maximum(1e-9, 4 * sens_h[:n])
+    bar = np.maximum(1e-9, 4 * sens_h)
     say("  per iteration rel " + " ".join(f"{x:.0e}" for x in r))
+    say("  GPU order spread  " + " ".join(f"{x:.0e}" for x in order_h))
     say("  per iteration bar " + " ".join(f"{x:.0e}" for x in bar))
     assert np.all(r <= bar)
-    fb = max(1e-9, 4 * fx["one_ulp_sensitivity"]["final"])
+    order_f = max(rel(a.final_cost, got.final_cost) for a in alts)
+    fb = max(1e-9, 4 * max(fx["one_ulp_sensitivity"]["final"], order_f))
     say(f"  final gpu {got.final_cost!r} oracle {fx['final_cost']!r} rel {rel(got.final_cost, fx['final_cost']):.2e} "
         f"(bar {fb:.1e}); trials {got.trials}/{fx['trials']}")
     assert rel(got.final_cost, fx["final_cost"]) <= fb
