@@ -100,6 +100,13 @@ int32_t nav_ba_debug_get_schur_kernel(const nav_ba_handle* h, int32_t* kernel);
  * figures of that kernel with every window running. */
 int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t reps, double* avg_us);
 
+/* The same for one stage's launches (NAV_BA_ST_LIN_PT, NAV_BA_ST_LIN_CAM with
+ * its chunk reduction, NAV_BA_ST_COST at the initial state, or
+ * NAV_BA_ST_SCHUR_ROW): every window running and doing the stage's work, so
+ * the average is the time of a launch that does it (a solve's stage timing
+ * also counts the launches that a window in its trial phase skips). */
+int32_t nav_ba_debug_time_stage(nav_ba_handle* h, int32_t stage, double lambda, int32_t reps, double* avg_us);
+
 /* Host-only check of the structure a prepare builds (no device needed): the
  * flat graph arrays and the Schur work items of `kernel` (1 or 2, as
  * nav_ba_debug_set_schur_kernel) at `chunk` observations per item

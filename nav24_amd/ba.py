@@ -28,7 +28,7 @@ EXPORTED = [
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
     "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_check_layout",
-    "nav_ba_debug_time_schur_rows", "nav_ba_debug_set_schur_kernel", "nav_ba_debug_get_schur_kernel",
+    "nav_ba_debug_time_schur_rows", "nav_ba_debug_time_stage", "nav_ba_debug_set_schur_kernel", "nav_ba_debug_get_schur_kernel",
     "nav_ba_debug_prepare_cached",
 ]
 
@@ -96,6 +96,8 @@ def lib() -> C.CDLL:
         L.nav_ba_debug_check_layout.restype = C.c_int32
         L.nav_ba_debug_time_schur_rows.argtypes = [H, C.c_double, C.c_int32, C.POINTER(C.c_double)]
         L.nav_ba_debug_time_schur_rows.restype = C.c_int32
+        L.nav_ba_debug_time_stage.argtypes = [H, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double)]
+        L.nav_ba_debug_time_stage.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -247,6 +249,15 @@ class GraphOptimHip:
         the prepared problem at its initial linearisation (every window running)."""
         v = C.c_double(0.0)
         self._check(lib().nav_ba_debug_time_schur_rows(self._h, lam, int(reps), C.byref(v)), allow_empty=True)
+        return v.value
+
+    def time_stage(self, stage: str, lam: float = 1e-4, reps: int = 10) -> float:
+        """Average device time (us) of one stage's launch ("lin_pt", "lin_cam",
+        "cost" or "schur_row") over all windows at the initial linearisation,
+        every window running and doing the stage's work."""
+        v = C.c_double(0.0)
+        self._check(lib().nav_ba_debug_time_stage(self._h, STAGES.index(stage), lam, int(reps), C.byref(v)),
+                    allow_empty=True)
         return v.value
 
     @property

@@ -1781,6 +1781,51 @@ int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t re
     return NAV_BA_OK;
 }
 
+int32_t nav_ba_debug_time_stage(nav_ba_handle* h, int32_t stage, double lambda, int32_t reps, double* avg_us) {
+    if (!h || !avg_us || reps <= 0) return NAV_BA_BAD_ARG;
+    if (stage != NAV_BA_ST_LIN_PT && stage != NAV_BA_ST_LIN_CAM && stage != NAV_BA_ST_COST &&
+        stage != NAV_BA_ST_SCHUR_ROW)
+        return NAV_BA_BAD_ARG;
+    if (!h->prepared) return fail(h, NAV_BA_NOT_PREPARED, "debug call before nav_ba_prepare");
+    *avg_us = 0.0;
+    if (h->all_empty || (stage == NAV_BA_ST_SCHUR_ROW && h->P.n_fc == 0)) return NAV_BA_EMPTY;
+    HIPTRY(h, hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const DevProblem& P = h->P;
+    HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
+    HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
+    std::vector<LMDev> L(P.n_win);
+    for (auto& l : L) { l = LMDev{}; l.lambda = lambda; l.nu = 2.0; l.phase = kPhaseLin; l.iter = 1; }
+    HIPTRY(h, hipMemcpyAsync(h->W.lm, L.data(), sizeof(LMDev) * L.size(), hipMemcpyHostToDevice, s));
+    launch_cam_rt(P, h->cur, s);
+    const bool m = mixed(h);
+    if (m) enqueue_lin<float>(h, false); else enqueue_lin<double>(h, false);
+    auto one = [&] {
+        switch (stage) {
+        case NAV_BA_ST_LIN_PT:
+            if (m) launch_lin_pt<float>(P, h->cur, h->W, s); else launch_lin_pt<double>(P, h->cur, h->W, s);
+            break;
+        case NAV_BA_ST_LIN_CAM:
+            if (m) launch_lin_cam<float>(P, h->cur, h->W, s); else launch_lin_cam<double>(P, h->cur, h->W, s);
+            break;
+        case NAV_BA_ST_COST:
+            if (m) launch_cost<float>(P, h->cur, h->W, 1, s); else launch_cost<double>(P, h->cur, h->W, 1, s);
+            break;
+        default:
+            if (m) launch_schur_row<float>(P, h->cur, h->W, s); else launch_schur_row<double>(P, h->cur, h->W, s);
+        }
+    };
+    one();  // warm-up
+    HIPTRY(h, hipEventRecord(h->ev0, s));
+    for (int r = 0; r < reps; ++r) one();
+    HIPTRY(h, hipEventRecord(h->ev1, s));
+    HIPTRY(h, hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPTRY(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *avg_us = 1e3 * (double)ms / reps;
+    return NAV_BA_OK;
+}
+
 int32_t nav_ba_debug_check_layout(const nav_ba_problem* problems, int32_t n_windows, const nav_ba_options* o,
                                   int32_t batched, int32_t chunk, int32_t kernel, int64_t* n_pairs,
                                   int32_t* n_rounds, char* msg, int32_t msg_len) {
