@@ -57,9 +57,18 @@ int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out);
  * one workgroup with the system in global memory (falls back to tiles above
  * 128 free cameras), 3 = 6x6 camera blocks in the LDS of one workgroup,
  * summed there from the Schur partials (the batched windows' kernel; falls
- * back to tiles above 30 free cameras).  All give the same step up to
- * rounding; the switch lets tests cover each on the same problem. */
+ * back to tiles above 30 free cameras), 4 = block-Jacobi preconditioned
+ * conjugate gradients on the dense reduced system (stop at a relative
+ * residual of 1e-10 or 2000 iterations; chosen automatically above 16384
+ * unknowns, up to 49152).  The factorisations give the same step up to
+ * rounding, conjugate gradients up to its tolerance; the switch lets tests
+ * cover each on the same problem. */
 int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode);
+
+/* The conjugate-gradient solve of the last trial of the last run (solver mode
+ * 4, or a system above 16384 unknowns): its iteration count and relative
+ * residual ||r|| / ||g||; 0 and 0 when the problem is factored instead. */
+int32_t nav_ba_debug_pcg_stats(const nav_ba_handle* h, int32_t* iterations, double* rel_residual);
 
 /* Observations per Schur work item for the next nav_ba_prepare on this
  * handle: 0 = automatic (problems of at least 131072 observations time the

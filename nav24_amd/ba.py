@@ -28,7 +28,7 @@ EXPORTED = [
     "nav_ba_prepare_batched", "nav_ba_fetch_batched", "nav_ba_solve_batched",
     "nav_ba_debug_reduced_system", "nav_ba_set_timing", "nav_ba_get_timing", "nav_ba_debug_set_solver",
     "nav_ba_debug_set_schur_chunk", "nav_ba_debug_get_schur_chunk", "nav_ba_debug_check_layout",
-    "nav_ba_debug_time_schur_rows", "nav_ba_debug_time_stage", "nav_ba_debug_set_schur_kernel", "nav_ba_debug_get_schur_kernel",
+    "nav_ba_debug_time_schur_rows", "nav_ba_debug_time_stage", "nav_ba_debug_pcg_stats", "nav_ba_debug_set_schur_kernel", "nav_ba_debug_get_schur_kernel",
     "nav_ba_debug_prepare_cached",
 ]
 
@@ -98,6 +98,8 @@ def lib() -> C.CDLL:
         L.nav_ba_debug_time_schur_rows.restype = C.c_int32
         L.nav_ba_debug_time_stage.argtypes = [H, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double)]
         L.nav_ba_debug_time_stage.restype = C.c_int32
+        L.nav_ba_debug_pcg_stats.argtypes = [H, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        L.nav_ba_debug_pcg_stats.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -215,7 +217,8 @@ class GraphOptimHip:
         return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(STAGES)}
 
     def set_solver(self, mode: int) -> None:
-        """Factorisation for the next prepare: 0 auto, 1 64x64 tiles, 2 6x6 camera blocks."""
+        """Reduced-system solver for the next prepare: 0 auto, 1 64x64 tiles, 2 6x6 camera
+        blocks in one workgroup, 3 the per-window kernel in LDS, 4 conjugate gradients."""
         self._check(lib().nav_ba_debug_set_solver(self._h, int(mode)))
 
     def set_schur_chunk(self, chunk: int) -> None:
@@ -250,6 +253,13 @@ class GraphOptimHip:
         v = C.c_double(0.0)
         self._check(lib().nav_ba_debug_time_schur_rows(self._h, lam, int(reps), C.byref(v)), allow_empty=True)
         return v.value
+
+    def pcg_stats(self):
+        """(iterations, relative residual) of the last trial's conjugate-gradient
+        solve; (0, 0.0) when the problem is factored instead."""
+        it, res = C.c_int32(0), C.c_double(0.0)
+        self._check(lib().nav_ba_debug_pcg_stats(self._h, C.byref(it), C.byref(res)))
+        return it.value, res.value
 
     def time_stage(self, stage: str, lam: float = 1e-4, reps: int = 10) -> float:
         """Average device time (us) of one stage's launch ("lin_pt", "lin_cam",

@@ -25,7 +25,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 
-HIP_SOURCES = ["ba_kernels.hip", "schur_kernels.hip", "chol_kernels.hip", "nav_ba_host.cpp", "cost_kernels.hip",
+HIP_SOURCES = ["ba_kernels.hip", "schur_kernels.hip", "chol_kernels.hip", "pcg_kernels.hip", "nav_ba_host.cpp",
+               "cost_kernels.hip",
                "undistort_kernels.hip", "twoview_kernels.hip"]
 
 

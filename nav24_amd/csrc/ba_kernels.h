@@ -44,6 +44,15 @@ struct LMDev {
     int iter, q, stop, pad;
 };
 
+// State of the conjugate-gradient solve of one problem's reduced system
+struct PcgState {
+    double rz;     // r.z of the current iterate
+    double rr0;    // ||g||^2
+    double rr;     // ||r||^2 of the current iterate
+    double alpha, beta;
+    int iter, done, fail, pad;
+};
+
 struct DevProblem {
     int n_cam, n_pt, n_obs, n_fc;
     int robust;
@@ -131,6 +140,7 @@ struct DevProblem {
     int use_blocks;            // 1: single-workgroup block Cholesky in global memory
     int use_win;               // 1: per-window in-LDS Cholesky summed from the Schur partials (every batch)
     int commit_in_accept;      // 1: k_lm_accept commits accepted trials (small windows), no k_commit
+    int use_pcg;               // 1: block-Jacobi conjugate gradients on the reduced system (one problem)
 };
 
 struct DevState {
@@ -161,6 +171,8 @@ struct DevWork {
     double* final;      // [n_win * 2] FP64 final cost, outlier count
     int* done_count;    // windows finished
     LMDev* lm;          // [n_win]
+    double* pcg;        // conjugate gradients (one problem): 36 per camera block, x r z p q, partial sums
+    PcgState* pcg_st;
 };
 
 __device__ __forceinline__ bool needs_lin(const LMDev* lm, int w) { return lm[w].phase == kPhaseLin; }
@@ -249,7 +261,7 @@ void launch_cholesky_solve(const DevProblem& P, const DevState& cur, const DevSt
                            hipStream_t st);
 constexpr int kBackSingleMax = 10;  // tiles up to which the backward step runs in one workgroup
 inline bool cholesky_updates_cameras(const DevProblem& P) {
-    return P.n_fc > 0 && (P.use_win || (!P.batched && !P.use_blocks && P.nt <= kBackSingleMax));
+    return P.n_fc > 0 && (P.use_win || (!P.batched && !P.use_blocks && !P.use_pcg && P.nt <= kBackSingleMax));
 }
 // Every window of a batch, or a single problem of at most 30 free cameras (use_win):
 // the reduced system is summed from the Schur partials inside the factorisation,
@@ -258,6 +270,14 @@ void launch_cholesky_small(const DevProblem& P, const DevState& cur, const DevSt
                            hipStream_t st);
 int prepare_cholesky_small(const DevProblem& P);  // LDS attribute of the per-window factorisation
 int prepare_cholesky(const DevProblem& P);  // LDS attributes of the one-problem factorisation kernels
+// Block-Jacobi conjugate gradients on one problem's reduced system (use_pcg):
+// the start, `count` iterations (each stops at once after the stop test), and
+// the step into dp.  The host reads W.pcg_st between chunks of iterations.
+size_t pcg_work_doubles(int n);
+void launch_pcg_start(const DevProblem& P, const DevWork& W, hipStream_t st);
+void launch_pcg_iterations(const DevProblem& P, const DevWork& W, int count, double tol, int max_iter,
+                           hipStream_t st);
+void launch_pcg_finish(const DevProblem& P, const DevWork& W, hipStream_t st);
 template <typename T> void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand,
                                           const DevWork& W, hipStream_t st);
 void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);

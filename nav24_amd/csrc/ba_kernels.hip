@@ -424,6 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
         } else {
             const int lg = P.colcam[cbase + s] - fc0;
             Sw[(size_t)(6 * lg + c) * lda + 6 * lf + i] = -sum;
+            if (P.use_pcg) Sw[(size_t)(6 * lf + i) * lda + 6 * lg + c] = -sum;  // both triangles for the matvec
         }
     }
     {
@@ -433,6 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_reduce(DevProblem P, const dou
         for (int idx = q0 + threadIdx.x; idx < q1; idx += kBlock) {
             const int lg = P.zcolcam[z0 + idx / 36] - fc0, e = idx % 36;
             Sw[(size_t)(6 * lg + e % 6) * lda + 6 * lf + e / 6] = 0.0;
+            if (P.use_pcg) Sw[(size_t)(6 * lf + e / 6) * lda + 6 * lg + e % 6] = 0.0;
         }
     }
     if (blockIdx.y == 0) {

@@ -35,6 +35,13 @@ namespace {
 
 constexpr const char* kVersion = "nav24_amd-ba 0.2 (gfx950)";
 constexpr int kMaxDenseN = 16384;  // tiled dense reduced system of one window (2 GiB FP64)
+// Conjugate gradients on the reduced system (above kMaxDenseN, or on request):
+// dense S up to kPcgMaxN unknowns (19 GiB FP64), stop at ||r|| <= kPcgTol ||g||
+// or kPcgMaxIter iterations; the host reads the stop flag every kPcgChunk.
+constexpr int kPcgMaxN = 49152;
+constexpr double kPcgTol = 1e-10;
+constexpr int kPcgMaxIter = 2000;
+constexpr int kPcgChunk = 8;
 // Schur chunk tuner: problems with at least kTuneMinObs observations time each
 // candidate chunk size once at prepare (best of kTuneReps launches) and take
 // the largest size within kTuneNear of the fastest; the other row kernel is
@@ -175,7 +182,10 @@ struct nav_ba_handle {
     hipGraphExec_t step0 = nullptr;  // the first step of a solve
     bool timing = false;
     bool timing_active = false;
-    int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks, 3 in-LDS per window
+    int solver_mode = 0;  // 0 auto, 1 tiles, 2 blocks, 3 in-LDS per window, 4 conjugate gradients
+    PcgState* hpcg = nullptr;  // pinned: the conjugate-gradient state, read between chunks
+    PcgState pcg_last{};       // ... and after the last trial's solve
+    int pcg_err = NAV_BA_OK;
     int schur_chunk = 0;  // observations per Schur work item: 0 = default or tuned
     int schur_chunk_used = 0;
     int schur_kernel = 0;  // Schur row kernel: 0 = default or tuned, 1 = rounds of kBlock, 2 = producer/consumer
@@ -346,6 +356,30 @@ void enqueue_reduced_system(nav_ba_handle* h) {
     }
 }
 
+// Conjugate gradients on the reduced system of one problem, driven from the
+// host: the start, then chunks of kPcgChunk iterations with the stop flag read
+// after each, then the step.  Not capturable (it reads device state), so a
+// problem that uses it runs its steps eagerly.
+void run_pcg(nav_ba_handle* h) {
+    hipStream_t s = h->stream;
+    auto chk = [&](hipError_t e) {
+        if (e != hipSuccess && h->pcg_err == NAV_BA_OK) {
+            h->pcg_err = NAV_BA_HIP_ERROR;
+            h->err = std::string("conjugate gradients: ") + hipGetErrorString(e);
+        }
+        return e == hipSuccess;
+    };
+    launch_pcg_start(h->P, h->W, s);
+    for (int it = 0; it < kPcgMaxIter; it += kPcgChunk) {
+        launch_pcg_iterations(h->P, h->W, kPcgChunk, kPcgTol, kPcgMaxIter, s);
+        if (!chk(hipMemcpyAsync(h->hpcg, h->W.pcg_st, sizeof(PcgState), hipMemcpyDeviceToHost, s))) return;
+        if (!chk(hipStreamSynchronize(s))) return;
+        if (h->hpcg->done) break;
+    }
+    if (h->hpcg->iter >= 0) h->pcg_last = *h->hpcg;  // a solve ran (the window had not finished)
+    launch_pcg_finish(h->P, h->W, s);
+}
+
 // One LM step of every window: gated linearisation, one trial, decision.
 template <typename T>
 void enqueue_step(nav_ba_handle* h, bool first) {
@@ -354,6 +388,7 @@ void enqueue_step(nav_ba_handle* h, bool first) {
     if (h->P.n_fc > 0)
         stage(h, NAV_BA_ST_CHOLESKY, [&] {
             if (h->P.use_win) launch_cholesky_small(h->P, h->cur, h->cand, h->W, h->stream);
+            else if (h->P.use_pcg) run_pcg(h);
             else launch_cholesky_solve(h->P, h->cur, h->cand, h->W, h->stream);
         });
     stage(h, NAV_BA_ST_BACKSUB, [&] { launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream); });
@@ -367,13 +402,16 @@ void enqueue_step(nav_ba_handle* h, bool first) {
 }
 
 int launch_step(nav_ba_handle* h, bool first) {
-    if (!h->timing) {
+    if (!h->timing && !h->P.use_pcg) {
         HIPTRY(h, hipGraphLaunch(first ? h->step0 : h->step, h->stream));
         return NAV_BA_OK;
     }
-    h->timing_active = true;
+    // stage timing, or conjugate gradients: the step's launches one by one
+    h->timing_active = h->timing;
+    h->pcg_err = NAV_BA_OK;
     if (mixed(h)) enqueue_step<float>(h, first); else enqueue_step<double>(h, first);
     h->timing_active = false;
+    if (h->pcg_err != NAV_BA_OK) return fail(h, h->pcg_err, h->err);
     return NAV_BA_OK;
 }
 
@@ -1075,7 +1113,7 @@ int build_layout(const nav_ba_problem* probs, int n_win, const nav_ba_options* o
     L.n = 6 * nfc;
     L.n_pad = ((L.n + kNB - 1) / kNB) * kNB;
     if (!batched) {
-        if (L.n > kMaxDenseN) { err = "reduced camera system larger than 16384"; return NAV_BA_TOO_LARGE; }
+        if (L.n > kPcgMaxN) { err = "reduced camera system larger than 49152"; return NAV_BA_TOO_LARGE; }
         L.win_lda[0] = std::max(L.n_pad, 1);
         L.S_size = std::max<long long>((long long)L.n_pad * L.n_pad, 1);
     } else {
@@ -1413,7 +1451,9 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     // free cameras unless another is asked for), 64x64 tiles, or on request
     // 6x6 blocks in global memory
     P.use_win = (batched || (nfc > 0 && 6 * nfc <= kSmallMaxN && (h->solver_mode == 0 || h->solver_mode == 3))) ? 1 : 0;
-    P.use_blocks = (!P.use_win && nfc > 0 && nfc <= kBlocksMaxFc && h->solver_mode == 2) ? 1 : 0;
+    P.use_pcg = (!batched && nfc > 0 && (h->solver_mode == 4 || L.n > kMaxDenseN)) ? 1 : 0;
+    if (P.use_pcg) P.use_win = 0;
+    P.use_blocks = (!P.use_win && !P.use_pcg && nfc > 0 && nfc <= kBlocksMaxFc && h->solver_mode == 2) ? 1 : 0;
     {
         int maxpt = 0;
         for (int w = 0; w < n_win; ++w) maxpt = std::max(maxpt, L.pt_first[w + 1] - L.pt_first[w]);
@@ -1436,7 +1476,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.g, sizeof(double) * nvec);
     AL(W.y, sizeof(double) * nvec);
     AL(W.dp, sizeof(double) * nvec);
-    AL(W.Lm, sizeof(double) * (batched ? (size_t)1 : (size_t)std::max<long long>(L.S_size, 1)));
+    AL(W.Lm, sizeof(double) * ((batched || P.use_pcg) ? (size_t)1 : (size_t)std::max<long long>(L.S_size, 1)));
     AL(W.Li, sizeof(double) * 4096 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1)));
     AL(W.fpart, sizeof(double) * 64 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1) * std::max(P.nt, 1)));
     AL(W.red_cost, sizeof(double) * std::max(P.n_cblk, 1));
@@ -1447,6 +1487,8 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.final, sizeof(double) * 2 * (size_t)n_win);
     AL(W.done_count, sizeof(int));
     AL(W.lm, sizeof(LMDev) * n_win);
+    AL(W.pcg, sizeof(double) * (P.use_pcg ? pcg_work_doubles(P.n) : (size_t)1));
+    AL(W.pcg_st, sizeof(PcgState));
 #undef AL
     const int schur_mark = h->nbuf;
     if ((st = upload_schur_items(h, SI)) != NAV_BA_OK) return st;
@@ -1456,14 +1498,14 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     // Schur rows and the per-window factorisation may use more than 64 KiB of LDS
     if ((mixed(h) ? prepare_schur_row<float>(P) : prepare_schur_row<double>(P)) != 0)
         return fail(h, NAV_BA_TOO_LARGE, "Schur row workspace exceeds the LDS of one workgroup");
-    if (prepare_cholesky(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the factorisation kernels");
+    if (!P.use_pcg && prepare_cholesky(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the factorisation kernels");
     if (P.use_win && prepare_cholesky_small(P) != 0) return fail(h, NAV_BA_HIP_ERROR, "LDS attribute of the window factorisation");
     launch_fill_system(P, h->W, h->stream);
     if (tune) {
         if ((st = tune_schur_items(h, G, schur_mark, h->schur_kernel, h->schur_chunk, SI)) != NAV_BA_OK) return st;
         remember_choice(key, TuneChoice{SI.chunk, SI.pc});
     }
-    if ((st = capture_step(h)) != NAV_BA_OK) return st;
+    if (!P.use_pcg && (st = capture_step(h)) != NAV_BA_OK) return st;
     h->prepared = true;
     h->prep_key.swap(pkey);
     return NAV_BA_OK;
@@ -1570,7 +1612,8 @@ int32_t nav_ba_create(nav_ba_handle** out, int32_t device) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
         hipHostMalloc((void**)&h->hdone, sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h->hlm, sizeof(LMDev) * kBurstWindows, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&h->hlm, sizeof(LMDev) * kBurstWindows, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h->hpcg, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
         nav_ba_destroy(h);
         return NAV_BA_HIP_ERROR;
     }
@@ -1588,6 +1631,7 @@ void nav_ba_destroy(nav_ba_handle* h) {
         if (b.p) (void)hipFree(b.p);
     if (h->hdone) (void)hipHostFree(h->hdone);
     if (h->hlm) (void)hipHostFree(h->hlm);
+    if (h->hpcg) (void)hipHostFree(h->hpcg);
     if (h->stage) (void)hipHostFree(h->stage);
     if (h->stage_s) (void)hipHostFree(h->stage_s);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -1715,7 +1759,7 @@ int32_t nav_ba_get_timing(const nav_ba_handle* h, nav_ba_timing* out) {
 }
 
 int32_t nav_ba_debug_set_solver(nav_ba_handle* h, int32_t mode) {
-    if (!h || mode < 0 || mode > 3) return NAV_BA_BAD_ARG;
+    if (!h || mode < 0 || mode > 4) return NAV_BA_BAD_ARG;
     h->solver_mode = mode;
     return NAV_BA_OK;
 }
@@ -1778,6 +1822,15 @@ int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t re
     float ms = 0.f;
     HIPTRY(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     *avg_us = 1e3 * (double)ms / reps;
+    return NAV_BA_OK;
+}
+
+int32_t nav_ba_debug_pcg_stats(const nav_ba_handle* h, int32_t* iterations, double* rel_residual) {
+    if (!h || !iterations || !rel_residual) return NAV_BA_BAD_ARG;
+    if (!h->prepared) return NAV_BA_NOT_PREPARED;
+    const PcgState& s = h->pcg_last;
+    *iterations = h->P.use_pcg ? s.iter : 0;
+    *rel_residual = (h->P.use_pcg && s.rr0 > 0.0) ? std::sqrt(s.rr / s.rr0) : 0.0;
     return NAV_BA_OK;
 }
 
