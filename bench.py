@@ -572,6 +572,13 @@ def diffcost_leg(n_problems, steps, with_cpu=False):
                                       "pose": rel(res["fp32"].g_pose, res["fp64"].g_pose),
                                       "point": rel(res["fp32"].g_X, res["fp64"].g_X),
                                       "intrinsics": rel(res["fp32"].g_intr, res["fp64"].g_intr)}
+    # the same against FP64 at the FP32-rounded inputs: the arithmetic's part of
+    # the error, without the rounding of the poses, points and intrinsics
+    r32in = b.eval(*[torch.tensor(a, device="cuda", dtype=torch.float32).double() for a in (T, X, intr)], grad=True)
+    out["fp32"]["rel_err_vs_fp64_same_inputs"] = {"cost": rel(res["fp32"].cost, r32in.cost),
+                                                  "pose": rel(res["fp32"].g_pose, r32in.g_pose),
+                                                  "point": rel(res["fp32"].g_X, r32in.g_X),
+                                                  "intrinsics": rel(res["fp32"].g_intr, r32in.g_intr)}
     b.close()
     return out
 

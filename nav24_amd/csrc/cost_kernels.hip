@@ -12,7 +12,8 @@
 //               camera-major order (fixed order);
 //   k_cg_final  per-problem costs and the intrinsic gradient over the tiles.
 // The forward (cost only) skips the adjoint and the second kernel.
-// T = double or float; tile partials and the final sums are FP64.
+// T = double or float: the projection and residual are FP64 either way, the
+// adjoint runs in T; tile partials and the final sums are FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,8 +39,7 @@ struct CostProblem {
     double delta;
     const int* o_cam;      // point-major, caller (global) camera index
     const int* o_pt;       // point-major, caller (global) point index
-    const double* uv_d;    // 2 per observation
-    const float* uv_f;
+    const double* uv_d;    // 2 per observation (FP64 for both compute types: the residual is FP64)
     const int* pt_list;    // unique points with observations, ascending
     const int* pt_ptr;     // [n_pts_used + 1]
     const int* tile_obs;   // [n_tiles + 1]
@@ -52,7 +52,7 @@ struct CostProblem {
 
 template <typename T>
 struct ObsGrad {
-    T cost;
+    double cost;  // the robust term, from the FP64 residual
     T gX[3];
     T gpose[6];
     T gintr[8];
@@ -61,75 +61,77 @@ struct ObsGrad {
 template <typename T>
 __device__ __forceinline__ T sqrt_t(T v) { return sqrt(v); }
 
-// Cost term of one observation and, if kGrad, its adjoint.
+// Cost term of one observation and, if kGrad, its adjoint.  The projection
+// and residual are formed in FP64 whatever T is: the residual is a difference
+// of two pixel coordinates of a few hundred pixels, and in FP32 that
+// cancellation alone left the cost of an FP32 problem 6.6e-4 off its FP64
+// value (FP64 issues at the FP32 rate on gfx950 without packed math, so the
+// forward costs the same).  The adjoint then runs in T.
 template <typename T, bool kGrad>
-__device__ __forceinline__ void obs_term(const T* __restrict__ Tcw, const T* __restrict__ X, const T in[8], T u,
-                                         T v, int robust, T delta, ObsGrad<T>& g) {
-    const T R0 = Tcw[0], R1 = Tcw[1], R2 = Tcw[2], R3 = Tcw[3], R4 = Tcw[4], R5 = Tcw[5];
-    const T R6 = Tcw[6], R7 = Tcw[7], R8 = Tcw[8];
-    const T x0 = X[0], x1 = X[1], x2 = X[2];
-    const T Xc0 = R0 * x0 + R1 * x1 + R2 * x2 + Tcw[9];
-    const T Xc1 = R3 * x0 + R4 * x1 + R5 * x2 + Tcw[10];
-    const T Xc2 = R6 * x0 + R7 * x1 + R8 * x2 + Tcw[11];
-    const T iz = T(1) / Xc2;
-    const T x = Xc0 * iz, y = Xc1 * iz;
-    const T fx = in[0], fy = in[1], cx = in[2], cy = in[3], k1 = in[4], k2 = in[5], p1 = in[6], p2 = in[7];
-    const T r2 = x * x + y * y;
-    const T rad = T(1) + k1 * r2 + k2 * r2 * r2;
-    const T xy = x * y;
-    const T xd = x * rad + T(2) * p1 * xy + p2 * (r2 + T(2) * x * x);
-    const T yd = y * rad + p1 * (r2 + T(2) * y * y) + T(2) * p2 * xy;
-    const T e0 = u - (fx * xd + cx);
-    const T e1 = v - (fy * yd + cy);
-    const T s = e0 * e0 + e1 * e1;
-    T w;
-    const T d2 = delta * delta;
+__device__ __forceinline__ void obs_term(const T* __restrict__ Tcw, const T* __restrict__ X, const T in[8], double u,
+                                         double v, int robust, double delta, ObsGrad<T>& g) {
+    const double R0 = Tcw[0], R1 = Tcw[1], R2 = Tcw[2], R3 = Tcw[3], R4 = Tcw[4], R5 = Tcw[5];
+    const double R6 = Tcw[6], R7 = Tcw[7], R8 = Tcw[8];
+    const double x0 = X[0], x1 = X[1], x2 = X[2];
+    const double Xc0 = R0 * x0 + R1 * x1 + R2 * x2 + (double)Tcw[9];
+    const double Xc1 = R3 * x0 + R4 * x1 + R5 * x2 + (double)Tcw[10];
+    const double Xc2 = R6 * x0 + R7 * x1 + R8 * x2 + (double)Tcw[11];
+    const double iz = 1.0 / Xc2;
+    const double x = Xc0 * iz, y = Xc1 * iz;
+    const double fx = in[0], fy = in[1], cx = in[2], cy = in[3], k1 = in[4], k2 = in[5], p1 = in[6], p2 = in[7];
+    const double r2 = x * x + y * y;
+    const double rad = 1.0 + k1 * r2 + k2 * r2 * r2;
+    const double xy = x * y;
+    const double xd = x * rad + 2.0 * p1 * xy + p2 * (r2 + 2.0 * x * x);
+    const double yd = y * rad + p1 * (r2 + 2.0 * y * y) + 2.0 * p2 * xy;
+    const double e0 = u - (fx * xd + cx);
+    const double e1 = v - (fy * yd + cy);
+    const double s = e0 * e0 + e1 * e1;
+    double w;
+    const double d2 = delta * delta;
     if (!robust || s <= d2) {
         g.cost = s;
-        w = T(1);
+        w = 1.0;
     } else {
-        const T r = sqrt_t(s);
-        g.cost = T(2) * delta * r - d2;
+        const double r = sqrt(s);
+        g.cost = 2.0 * delta * r - d2;
         w = delta / r;
     }
     if (!kGrad) return;
-    const T gpu = T(-2) * w * e0, gpv = T(-2) * w * e1;  // d C / d (projected u, v)
-    g.gintr[0] = gpu * xd;
-    g.gintr[1] = gpv * yd;
+    // the adjoint in T from the FP64 forward values
+    const T tx = (T)x, ty = (T)y, tr2 = (T)r2, trad = (T)rad, txy = (T)xy, tiz = (T)iz;
+    const T tk1 = in[4], tk2 = in[5], tp1 = in[6], tp2 = in[7], tfx = in[0], tfy = in[1];
+    const T gpu = (T)(-2.0 * w * e0), gpv = (T)(-2.0 * w * e1);  // d C / d (projected u, v)
+    g.gintr[0] = gpu * (T)xd;
+    g.gintr[1] = gpv * (T)yd;
     g.gintr[2] = gpu;
     g.gintr[3] = gpv;
-    const T gxd = gpu * fx, gyd = gpv * fy;
-    g.gintr[4] = (gxd * x + gyd * y) * r2;
-    g.gintr[5] = (gxd * x + gyd * y) * r2 * r2;
-    g.gintr[6] = gxd * T(2) * xy + gyd * (r2 + T(2) * y * y);
-    g.gintr[7] = gxd * (r2 + T(2) * x * x) + gyd * T(2) * xy;
-    const T kr = k1 + T(2) * k2 * r2;
-    const T dxdx = rad + T(2) * x * x * kr + T(2) * p1 * y + T(6) * p2 * x;
-    const T dxdy = T(2) * xy * kr + T(2) * p1 * x + T(2) * p2 * y;
+    const T gxd = gpu * tfx, gyd = gpv * tfy;
+    g.gintr[4] = (gxd * tx + gyd * ty) * tr2;
+    g.gintr[5] = (gxd * tx + gyd * ty) * tr2 * tr2;
+    g.gintr[6] = gxd * T(2) * txy + gyd * (tr2 + T(2) * ty * ty);
+    g.gintr[7] = gxd * (tr2 + T(2) * tx * tx) + gyd * T(2) * txy;
+    const T kr = tk1 + T(2) * tk2 * tr2;
+    const T dxdx = trad + T(2) * tx * tx * kr + T(2) * tp1 * ty + T(6) * tp2 * tx;
+    const T dxdy = T(2) * txy * kr + T(2) * tp1 * tx + T(2) * tp2 * ty;
     const T dydx = dxdy;
-    const T dydy = rad + T(2) * y * y * kr + T(6) * p1 * y + T(2) * p2 * x;
+    const T dydy = trad + T(2) * ty * ty * kr + T(6) * tp1 * ty + T(2) * tp2 * tx;
     const T gx = gxd * dxdx + gyd * dydx;
     const T gy = gxd * dxdy + gyd * dydy;
-    const T gc0 = gx * iz, gc1 = gy * iz, gc2 = -(gx * x + gy * y) * iz;  // d C / d Xc
+    const T gc0 = gx * tiz, gc1 = gy * tiz, gc2 = -(gx * tx + gy * ty) * tiz;  // d C / d Xc
     // world point: R^T gc
-    g.gX[0] = R0 * gc0 + R3 * gc1 + R6 * gc2;
-    g.gX[1] = R1 * gc0 + R4 * gc1 + R7 * gc2;
-    g.gX[2] = R2 * gc0 + R5 * gc1 + R8 * gc2;
+    g.gX[0] = Tcw[0] * gc0 + Tcw[3] * gc1 + Tcw[6] * gc2;
+    g.gX[1] = Tcw[1] * gc0 + Tcw[4] * gc1 + Tcw[7] * gc2;
+    g.gX[2] = Tcw[2] * gc0 + Tcw[5] * gc1 + Tcw[8] * gc2;
     // pose, left perturbation: d Xc / d omega = -[Xc]x, d Xc / d upsilon = I
-    g.gpose[0] = Xc1 * gc2 - Xc2 * gc1;
-    g.gpose[1] = Xc2 * gc0 - Xc0 * gc2;
-    g.gpose[2] = Xc0 * gc1 - Xc1 * gc0;
+    const T tc0 = (T)Xc0, tc1 = (T)Xc1, tc2 = (T)Xc2;
+    g.gpose[0] = tc1 * gc2 - tc2 * gc1;
+    g.gpose[1] = tc2 * gc0 - tc0 * gc2;
+    g.gpose[2] = tc0 * gc1 - tc1 * gc0;
     g.gpose[3] = gc0;
     g.gpose[4] = gc1;
     g.gpose[5] = gc2;
 }
-
-template <typename T>
-__device__ __forceinline__ const T* uv_of(const CostProblem& P, int a);
-template <>
-__device__ __forceinline__ const double* uv_of<double>(const CostProblem& P, int a) { return P.uv_d + 2 * (size_t)a; }
-template <>
-__device__ __forceinline__ const float* uv_of<float>(const CostProblem& P, int a) { return P.uv_f + 2 * (size_t)a; }
 
 // One workgroup per tile.  part[t] = {cost, d/d intrinsics (8)} of the tile (FP64).
 template <typename T, bool kGrad>
@@ -146,15 +148,14 @@ __global__ __launch_bounds__(kThreads) void k_cg_obs(CostProblem P, const T* __r
     T in[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) in[k] = intr[k];
-    const T delta = (T)P.delta;
     double cost = 0.0, gi[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     T gp[3] = {T(0), T(0), T(0)};
     for (int a = o0 + (int)threadIdx.x; a < o1; a += kThreads) {
         const int c = P.o_cam[a], q = P.o_pt[a];
-        const T* uv = uv_of<T>(P, a);
+        const double* uv = P.uv_d + 2 * (size_t)a;
         ObsGrad<T> g;
-        obs_term<T, kGrad>(Tcw + 12 * (size_t)c, X + 3 * (size_t)q, in, uv[0], uv[1], P.robust, delta, g);
-        cost += (double)g.cost;
+        obs_term<T, kGrad>(Tcw + 12 * (size_t)c, X + 3 * (size_t)q, in, uv[0], uv[1], P.robust, P.delta, g);
+        cost += g.cost;
         if (kGrad) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) gi[k] += (double)g.gintr[k];
@@ -399,12 +400,10 @@ int32_t nav_ba_cost_prepare(nav_ba_cost_handle* h, const nav_ba_problem* probs, 
     }
     std::vector<int> o_cam(no), o_pt(no);
     std::vector<double> uv_d(2 * no);
-    std::vector<float> uv_f(2 * no);
     for (size_t k = 0; k < no; ++k) {
         const int i = ord[k];
         o_cam[k] = gcam[i]; o_pt[k] = gpt[i];
         uv_d[2 * k] = guv[2 * (size_t)i]; uv_d[2 * k + 1] = guv[2 * (size_t)i + 1];
-        uv_f[2 * k] = (float)uv_d[2 * k]; uv_f[2 * k + 1] = (float)uv_d[2 * k + 1];
     }
     // unique points, tiles of whole points per problem
     std::vector<int> pt_list, pt_ptr{0}, tile_obs{0}, tile_pt{0}, tile_prob, prob_tile0{0};
@@ -449,7 +448,7 @@ int32_t nav_ba_cost_prepare(nav_ba_cost_handle* h, const nav_ba_problem* probs, 
     P.delta = huber_delta;
     int st;
     if ((st = up(h, o_cam, &P.o_cam)) || (st = up(h, o_pt, &P.o_pt)) || (st = up(h, uv_d, &P.uv_d)) ||
-        (st = up(h, uv_f, &P.uv_f)) || (st = up(h, pt_list, &P.pt_list)) || (st = up(h, pt_ptr, &P.pt_ptr)) ||
+        (st = up(h, pt_list, &P.pt_list)) || (st = up(h, pt_ptr, &P.pt_ptr)) ||
         (st = up(h, tile_obs, &P.tile_obs)) || (st = up(h, tile_pt, &P.tile_pt)) ||
         (st = up(h, tile_prob, &P.tile_prob)) || (st = up(h, prob_tile0, &P.prob_tile0)) ||
         (st = up(h, cam_ptr, &P.cam_ptr)) || (st = up(h, cam_obs, &P.cam_obs)))

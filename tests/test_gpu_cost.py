@@ -78,10 +78,21 @@ def test_fp32_close_to_fp64_and_cost_only_path(dev):
     b = CostBatch(probs)
     r64 = b.eval(*(torch.tensor(a, device=dev) for a in (T, X, intr)))
     r32 = b.eval(*(torch.tensor(a, device=dev, dtype=torch.float32) for a in (T, X, intr)))
+    # against FP64 at the same, float-rounded inputs: the FP32 path forms the
+    # projection and residual in FP64 and only the adjoint in FP32, so what is
+    # left is FP32 rounding of the gradient terms (about 1e-7 relative)
+    r64f = b.eval(*(torch.tensor(a, device=dev, dtype=torch.float32).double() for a in (T, X, intr)))
+    for name, a, ref in (("cost", r32.cost, r64f.cost), ("pose", r32.g_pose, r64f.g_pose),
+                         ("point", r32.g_X, r64f.g_X), ("intr", r32.g_intr, r64f.g_intr)):
+        err = (a.double() - ref).abs().max().item() / ref.abs().max().item()
+        say(f"  fp32 {name}: max rel err {err:.2e} (FP64 at the same inputs)")
+        assert err <= 1e-5, name
+    # against FP64 at the original inputs the rounding of the inputs to float
+    # adds its own part (poses and points of ~10 m to 6e-8 relative)
     for name, a, ref in (("cost", r32.cost, r64.cost), ("pose", r32.g_pose, r64.g_pose),
                          ("point", r32.g_X, r64.g_X), ("intr", r32.g_intr, r64.g_intr)):
         err = (a.double() - ref).abs().max().item() / ref.abs().max().item()
-        say(f"  fp32 {name}: max rel err {err:.2e}")
+        say(f"  fp32 {name}: max rel err {err:.2e} (FP64 at the original inputs)")
         assert err <= 2e-3, name
     fwd = b.eval(*(torch.tensor(a, device=dev) for a in (T, X, intr)), grad=False)
     assert fwd.g_pose is None and torch.equal(fwd.cost, r64.cost)

@@ -120,6 +120,7 @@ def test_differentiable_cost_64_problems_full_size():
     dev = torch.device("cuda:0")
     r64 = b.eval(*(torch.tensor(a, device=dev) for a in (T, X, intr)))
     r32 = b.eval(*(torch.tensor(a, device=dev, dtype=torch.float32) for a in (T, X, intr)))
+    r64f = b.eval(*(torch.tensor(a, device=dev, dtype=torch.float32).double() for a in (T, X, intr)))
     costs, gps, gxs, gi = [], [], [], np.zeros(8)
     c0 = p0 = 0
     for p in probs:
@@ -128,6 +129,11 @@ def test_differentiable_cost_64_problems_full_size():
         costs.append(c); gps.append(gp); gxs.append(gx); gi += g8
         c0 += p.n_cam; p0 += p.n_pt
     ref = (np.array(costs), np.concatenate(gps), np.concatenate(gxs), gi)
+    for name, g32, gf in zip(("cost", "pose", "point", "intr"), (r32.cost, r32.g_pose, r32.g_X, r32.g_intr),
+                             (r64f.cost, r64f.g_pose, r64f.g_X, r64f.g_intr)):
+        ea = (g32.double() - gf).abs().max().item() / gf.abs().max().item()
+        say(f"  {name}: fp32 against FP64 at the same float inputs {ea:.2e}")
+        assert ea <= 1e-5, name
     for name, g, g32, r in zip(("cost", "pose", "point", "intr"),
                                (r64.cost, r64.g_pose, r64.g_X, r64.g_intr),
                                (r32.cost, r32.g_pose, r32.g_X, r32.g_intr), ref):
@@ -135,7 +141,7 @@ def test_differentiable_cost_64_problems_full_size():
         e32 = np.abs(g32.double().cpu().numpy() - r).max() / np.abs(r).max()
         say(f"  {name}: fp64 {e64:.2e}, fp32 {e32:.2e}")
         assert e64 <= 1e-11, name
-        assert e32 <= 2e-3, name
+        assert e32 <= 2e-3, name  # mostly the inputs' rounding to float
     b.close()
 
 
