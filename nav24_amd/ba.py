@@ -164,6 +164,8 @@ class GraphOptimHip:
         return self._check(lib().nav_ba_run(self._h), allow_empty=True)
 
     def fetch(self) -> Result:
+        if self._prob is None:  # no prepare on this handle: the library says so
+            self._check(lib().nav_ba_fetch(self._h, C.byref(NavBaResult())))
         rb = ResultBuffers(self._prob, self._opts.n_iter)
         st = lib().nav_ba_fetch(self._h, C.byref(rb.c))
         self._check(st, allow_empty=True)
