@@ -163,7 +163,8 @@ struct DevWork {
     double* Lm;         // one problem: Cholesky factor (tiled: n_pad^2; blocks: packed 6x6 blocks)
     double* Li;         // tiled path: inverses of the diagonal factor tiles [nt][64][64]
     double* fpart;      // tiled path: forward-step products L_ij y_j [nt][nt][64]
-    double* red_cost;   // [n_cblk]
+    double* red_cost;   // [n_cblk] the cost at the current state (start of a solve, flags)
+    double* red_tcost;  // [n_tiles] the trial cost at the candidate state, per linearisation tile
     double* red_scale;  // [n_win + n_tiles]: per-window camera part, then per-tile point parts
     double* red_aux;    // [n_cblk] outlier counts
     uint8_t* outlier;   // [n_obs] caller order

@@ -479,7 +479,10 @@ void launch_schur_reduce(const DevProblem& P, const DevWork& W, hipStream_t st) 
 // observation); per point, in fixed order from LDS: cl = bl - sum_b c_b,
 // dX = Dinv cl, X' = X + dX, and the point part of x'(lambda x + b), summed
 // per tile.  A point with more than kTile observations has a tile of its own
-// and is summed by a block reduction.
+// and is summed by a block reduction.  The same workgroup then forms the
+// trial cost of the tile's observations at the candidate state (the candidate
+// cameras are formed before it), summed per tile in a fixed order, so that the
+// trial needs no separate cost pass over the observations.
 // c_b of one observation with its camera, weight and camera step d (6) loaded.
 template <typename T, int CM>
 __device__ __forceinline__ void backsub_obs_at(const DevProblem& P, const double R[9], const double tt[3], const D4& xp,
@@ -514,22 +517,36 @@ __device__ __forceinline__ void backsub_obs(const DevProblem& P, const CamRT* __
 
 // Point p's update from bl (3) and Dinv (6, packed) already loaded.
 __device__ __forceinline__ double backsub_point_at(int p, double lambda, const double bl[3], const double D[6],
-                                                   const double s[3], const D4& xp, D4* __restrict__ Xn) {
+                                                   const double s[3], const D4& xp, D4* __restrict__ Xn, D4& xn) {
     const double bl0 = bl[0], bl1 = bl[1], bl2 = bl[2];
     const double c0 = bl0 - s[0], c1 = bl1 - s[1], c2 = bl2 - s[2];
     const double d0 = D[0] * c0 + D[1] * c1 + D[2] * c2;
     const double d1 = D[1] * c0 + D[3] * c1 + D[4] * c2;
     const double d2 = D[2] * c0 + D[4] * c1 + D[5] * c2;
-    Xn[p] = D4{xp.x + d0, xp.y + d1, xp.z + d2, xp.w};
+    xn = D4{xp.x + d0, xp.y + d1, xp.z + d2, xp.w};
+    Xn[p] = xn;
     return d0 * (lambda * d0 + bl0) + d1 * (lambda * d1 + bl1) + d2 * (lambda * d2 + bl2);
 }
 
 __device__ __forceinline__ double backsub_point(const DevProblem& P, int p, double lambda, const double* __restrict__ Hl,
-                                                const double s[3], const D4& xp, D4* __restrict__ Xn) {
+                                                const double s[3], const D4& xp, D4* __restrict__ Xn, D4& xn) {
     const double* h = Hl + 9 * (size_t)p;
     double D[6];
     point_dinv(h, lambda, D);
-    return backsub_point_at(p, lambda, h + 6, D, s, xp, Xn);
+    return backsub_point_at(p, lambda, h + 6, D, s, xp, Xn, xn);
+}
+
+// Robust cost of observation a at the candidate state (its camera's candidate
+// pose and its point's candidate position xn), as k_cost forms it.
+template <int CM>
+__device__ __forceinline__ double trial_term(const DevProblem& P, const CamRT* __restrict__ cn, int cb, const D4& xn,
+                                             int a) {
+    double R[9], tt[3];
+    load_cam(cn, cb, R, tt);
+    double e0, e1, zc, w;
+    const double s = edge_residual_m<CM>(P, R, tt, xn.x, xn.y, xn.z, P.o_uv[2 * (size_t)a],
+                                         P.o_uv[2 * (size_t)a + 1], e0, e1, zc);
+    return robustify(s, P.robust, P.delta, w);
 }
 
 template <typename T, int CM>
@@ -537,9 +554,11 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
                                                     const D4* __restrict__ X, const T* __restrict__ w,
                                                     const double* __restrict__ Hl,
                                                     const double* __restrict__ dp, const LMDev* lm,
-                                                    D4* __restrict__ Xn, double* __restrict__ red_scale) {
+                                                    D4* __restrict__ Xn, double* __restrict__ red_scale,
+                                                    const CamRT* __restrict__ cn, double* __restrict__ red_tcost) {
     __shared__ double sh[3][kTile];
     __shared__ double red[kBlock / kWave][3];
+    __shared__ D4 xs_new[kTile];  // the tile's candidate points (a point has an observation: at most kTile)
     const int t = blockIdx.x;
     const int win = P.tile_win[t];
     if (!running(lm, win)) return;
@@ -547,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
     const int p0 = P.tile_pt[t], p1 = P.tile_pt[t + 1];
     const bool fail = lm[win].fail != 0;
     const double lambda = lm[win].lambda;
-    double sc = 0.0;
+    double sc = 0.0, cost = 0.0;
     if (o1 - o0 <= kTile) {
         // loads in three waves, each issued together: the observation's indices
         // and weight with this thread's first point (X, pt_free, bl, Dinv); the
@@ -599,17 +618,22 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
                 }
                 for (; b < b1; ++b) { s[0] += sh[0][b]; s[1] += sh[1][b]; s[2] += sh[2][b]; }
                 s[0] += t[0]; s[1] += t[1]; s[2] += t[2];
+                D4 xn;
                 if (first) {
                     double D[6];
                     point_dinv(hq, lambda, D);
-                    sc += backsub_point_at(pp, lambda, hq + 6, D, s, xpp, Xn);
+                    sc += backsub_point_at(pp, lambda, hq + 6, D, s, xpp, Xn, xn);
                 } else {
-                    sc += backsub_point(P, pp, lambda, Hl, s, xpp, Xn);
+                    sc += backsub_point(P, pp, lambda, Hl, s, xpp, Xn, xn);
                 }
+                if (pp - p0 < kTile) xs_new[pp - p0] = xn;
             } else {
                 Xn[pp] = xpp;
+                if (pp - p0 < kTile) xs_new[pp - p0] = xpp;
             }
         }
+        __syncthreads();
+        if (va && !fail) cost = trial_term<CM>(P, cn, cb, p - p0 < kTile ? xs_new[p - p0] : Xn[p], a);
     } else {
         // one point (p0) with more than kTile observations
         const D4 xp = X[p0];
@@ -636,16 +660,28 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
                     s[k] = 0.0;
                     for (int i = 0; i < kBlock / kWave; ++i) s[k] += red[i][k];
                 }
-                sc = backsub_point(P, p0, lambda, Hl, s, xp, Xn);
+                D4 xn;
+                sc = backsub_point(P, p0, lambda, Hl, s, xp, Xn, xn);
+                xs_new[0] = xn;
             } else {
                 Xn[p0] = xp;
+                xs_new[0] = xp;
             }
         }
         __syncthreads();
+        if (!fail) {
+            const D4 xn = xs_new[0];
+            for (int a = o0 + (int)threadIdx.x; a < o1; a += kBlock) cost += trial_term<CM>(P, cn, P.o_cam[a], xn, a);
+        }
     }
     __shared__ double shs[kBlock / kWave];
     const double s = block_sum<double, kBlock>(sc, shs);
-    if (threadIdx.x == 0) red_scale[P.n_win + t] = s;
+    __syncthreads();
+    const double cs = block_sum<double, kBlock>(cost, shs);
+    if (threadIdx.x == 0) {
+        red_scale[P.n_win + t] = s;
+        red_tcost[t] = cs;
+    }
 }
 
 template <typename T>
@@ -654,7 +690,7 @@ void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& ca
     const auto fn = P.cam_model == kCamKB8 ? k_backsub<T, kCamKB8> : k_backsub<T, kCamPinhole>;
     hipLaunchKernelGGL(fn, dim3(P.n_tiles),
                        dim3(kBlock), 0, st, P, cur.cam, cur.X, (const T*)W.w, (const double*)W.Hl, W.dp,
-                       (const LMDev*)W.lm, cand.X, W.red_scale);
+                       (const LMDev*)W.lm, cand.X, W.red_scale, (const CamRT*)cand.cam, W.red_tcost);
 }
 
 // ---------------------------------------------------------------- camera update
@@ -761,7 +797,7 @@ __device__ double sum_range(const double* v, int i0, int i1, double* sh) {
 // then the trial loop of solve(): another trial while rho < 0 and fewer than
 // max_trials trials; otherwise the iteration ends, and the window stops when
 // the trial limit was reached, rho == 0, or the iteration count is done.
-__global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* __restrict__ red_cost,
+__global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* __restrict__ red_tcost,
                                                    const double* __restrict__ red_scale, LMDev* lm,
                                                    double* __restrict__ hist, int* done_count,
                                                    D4* __restrict__ X, const D4* __restrict__ Xn,
@@ -774,8 +810,8 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
         if (threadIdx.x == 0) lm[win].accepted = 0;
         return;
     }
-    // both ranges' loads in flight together
-    const double pc = range_part(red_cost, P.win_cblk0[win], P.win_cblk0[win + 1]);
+    // both ranges' loads in flight together (the trial cost per linearisation tile, from k_backsub)
+    const double pc = range_part(red_tcost, P.win_tile0[win], P.win_tile0[win + 1]);
     const double ps = range_part(red_scale, P.n_win + P.win_tile0[win], P.n_win + P.win_tile0[win + 1]);
     const double cost = block_sum<double, 256>(pc, sh);
     __syncthreads();
@@ -836,7 +872,7 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
 
 void launch_lm_accept(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
                       hipStream_t st) {
-    hipLaunchKernelGGL(k_lm_accept, dim3(P.n_win), dim3(256), 0, st, P, (const double*)W.red_cost,
+    hipLaunchKernelGGL(k_lm_accept, dim3(P.n_win), dim3(256), 0, st, P, (const double*)W.red_tcost,
                        (const double*)W.red_scale, W.lm, W.hist, W.done_count, cur.X, (const D4*)cand.X, cur.pose,
                        (const PoseQT*)cand.pose, cur.cam, (const CamRT*)cand.cam);
 }

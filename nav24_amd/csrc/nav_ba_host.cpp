@@ -391,10 +391,10 @@ void enqueue_step(nav_ba_handle* h, bool first) {
             else if (h->P.use_pcg) run_pcg(h);
             else launch_cholesky_solve(h->P, h->cur, h->cand, h->W, h->stream);
         });
-    stage(h, NAV_BA_ST_BACKSUB, [&] { launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream); });
+    // the candidate cameras first: the back-substitution also forms the trial cost
     if (!cholesky_updates_cameras(h->P))
         stage(h, NAV_BA_ST_UPDATE_CAM, [&] { launch_update_cam(h->P, h->cur, h->cand, h->W, h->stream); });
-    stage(h, NAV_BA_ST_COST, [&] { launch_cost<T>(h->P, h->cand, h->W, 1, h->stream); });
+    stage(h, NAV_BA_ST_BACKSUB, [&] { launch_backsub<T>(h->P, h->cur, h->cand, h->W, h->stream); });
     stage(h, NAV_BA_ST_ACCEPT, [&] {
         launch_lm_accept(h->P, h->cur, h->cand, h->W, h->stream);
         launch_commit(h->P, h->cur, h->cand, h->W, h->stream);
@@ -1480,6 +1480,7 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.Li, sizeof(double) * 4096 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1)));
     AL(W.fpart, sizeof(double) * 64 * (batched ? (size_t)1 : (size_t)std::max(P.nt, 1) * std::max(P.nt, 1)));
     AL(W.red_cost, sizeof(double) * std::max(P.n_cblk, 1));
+    AL(W.red_tcost, sizeof(double) * std::max(P.n_tiles, 1));
     AL(W.red_scale, sizeof(double) * (n_win + std::max(P.n_tiles, 1)));
     AL(W.red_aux, sizeof(double) * std::max(P.n_cblk, 1));
     AL(W.outlier, std::max<size_t>(no, 1));
