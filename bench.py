@@ -70,9 +70,10 @@ def stage_work(st, fp64=True):
     }
 
 
-STAGE_KERNEL = {"schur_row": "k_schur_row<double>", "lin_pt": "k_lin_pt<double>",
-                "lin_cam": "k_lin_cam<double>", "cost": "k_cost<double>"}
-SCHUR_KERNEL = {1: "k_schur_row<double>", 2: "k_schur_row_pc<double>"}
+# kernel names as the profiler reports them: <scalar type, camera model 0 = pinhole>
+STAGE_KERNEL = {"schur_row": "k_schur_row<double, 0>", "lin_pt": "k_lin_pt<double, 0>",
+                "lin_cam": "k_lin_cam<double, 0>", "cost": "k_cost<double, 0>"}
+SCHUR_KERNEL = {1: "k_schur_row<double, 0>", 2: "k_schur_row_pc<double, 0>"}
 
 
 def measured_traffic(workload_name, stage):
@@ -193,7 +194,7 @@ def headline_leg(n_windows, steps, warmup, with_cpu):
     schur_us = s.time_schur_rows(1e-4, 10)
     flops = stage_work(st)["schur_row"][1]
     ach = flops / (schur_us * 1e-6) / 1e12
-    kname = SCHUR_KERNEL.get(s.schur_kernel, "k_schur_row<double>")
+    kname = SCHUR_KERNEL.get(s.schur_kernel, STAGE_KERNEL["schur_row"])
     roof = {"kernel": kname, "bound": "fp64-valu", "achieved": round(ach, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 5),
             "traffic": measured_traffic(WORKLOAD_NAMES[3], kname), "avg_launch_us": round(schur_us, 2),
