@@ -780,8 +780,8 @@ __device__ __forceinline__ bool diag_factor(double* a, double* m, double* y) {
 // from lanes 0..5 as wave-uniform values.  The factor L_IC overwrites the
 // column block and y its rows.  Returns false on a pivot that is not
 // positive and finite (replaced by 1), the same in every lane.
-__device__ __forceinline__ bool win_panel(double* Lb, double* yv, double* dinv, int n, int C, int K, int lane) {
-    constexpr int kQ = (kSmallMaxN + 63) / 64;
+template <int kQ>
+__device__ __forceinline__ bool win_panel_q(double* Lb, double* yv, double* dinv, int n, int C, int K, int lane) {
     const int r0 = 6 * C;
     double v[kQ][6], yq[kQ];
     double u[6][6];  // L_CK, wave-uniform
@@ -827,13 +827,14 @@ __device__ __forceinline__ bool win_panel(double* Lb, double* yv, double* dinv, 
         rs = rs * (1.5 - 0.5 * d * rs * rs);
         rs = rs * (1.5 - 0.5 * d * rs * rs);
         const double yj = lane_value(yq[0], j) * rs;
+        // unmasked: a row past n (zero) is never written back, and the diagonal
+        // block's rows (lanes 0..5 of slot 0) take extra operations only above
+        // the diagonal, which nothing reads; only their right-hand side, final
+        // once their own pivot is passed, is masked
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const int lr = lane + 64 * q;
-            if (lr > j && r0 + lr < n) {
-                v[q][j] *= rs;
-                yq[q] -= v[q][j] * yj;
-            }
+            v[q][j] *= rs;
+            if (q > 0 || lane > j) yq[q] -= v[q][j] * yj;
         }
         if (lane == j) {
             v[0][j] = d * rs;
@@ -843,10 +844,7 @@ __device__ __forceinline__ bool win_panel(double* Lb, double* yv, double* dinv, 
         for (int c = j + 1; c < 6; ++c) {
             const double lcj = lane_value(v[0][j], c);
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                const int lr = lane + 64 * q;
-                if (lr >= c && r0 + lr < n) v[q][c] -= v[q][j] * lcj;
-            }
+            for (int q = 0; q < kQ; ++q) v[q][c] -= v[q][j] * lcj;
         }
         if (lane == 0) dinv[r0 + j] = rs;
     }
@@ -862,6 +860,15 @@ __device__ __forceinline__ bool win_panel(double* Lb, double* yv, double* dinv, 
         }
     }
     return ok_all;
+}
+
+// win_panel_q with as many row slots as block column C has rows (n - 6C).
+__device__ __forceinline__ bool win_panel(double* Lb, double* yv, double* dinv, int n, int C, int K, int lane) {
+    static_assert(kSmallMaxN <= 192, "three row slots of 64 lanes");
+    const int rows = n - 6 * C;
+    if (rows > 128) return win_panel_q<3>(Lb, yv, dinv, n, C, K, lane);
+    if (rows > 64) return win_panel_q<2>(Lb, yv, dinv, n, C, K, lane);
+    return win_panel_q<1>(Lb, yv, dinv, n, C, K, lane);
 }
 
 // Part of one (row, slot) block of a window's reduced system from the Schur
@@ -1073,39 +1080,67 @@ __global__ __launch_bounds__(kSmallThreads) void k_chol_small(DevProblem P, cons
         if (wid == 0) {
             if (!win_panel(Lb, yv, dinv, n, K + 1, K, lane) && lane == 0) bad = 1;
         } else {
-            // A_IJ -= L_IK L_JK^T for K+2 <= J <= I, one 6x6 block per thread of
-            // waves 1..7 (36 register accumulators)
+            // A_IJ -= L_IK L_JK^T for K+2 <= J <= I by waves 1..7: one 6x6 block
+            // per thread (36 register accumulators) while the blocks outnumber
+            // the threads' sixths, else one block row per thread, so that the
+            // last columns' few blocks do not hold the barrier for a whole
+            // block's 216 products.  Either way each entry subtracts its six
+            // products in the same order.
             const int rem = nb - K - 2;
             const int nupd = rem * (rem + 1) / 2;
-            for (int q = tid - 64; q < nupd; q += kSmallThreads - 64) {
-                // q -> (I, J), column-major from the diagonal
+            if (6 * nupd > kSmallThreads - 64) {
+                for (int q = tid - 64; q < nupd; q += kSmallThreads - 64) {
+                    // q -> (I, J), column-major from the diagonal
+                    int jj = 0, qq = q, len = rem;
+                    while (qq >= len) { qq -= len; ++jj; --len; }
+                    const int J = K + 2 + jj, I = J + qq;
+                    const double* A = Lb + blk_off(I, K);
+                    const double* B = Lb + blk_off(J, K);
+                    double* C = Lb + blk_off(I, J);
+                    double acc[36];
+#pragma unroll
+                    for (int e = 0; e < 36; ++e) acc[e] = C[e];
+                    double bm[36];
+#pragma unroll
+                    for (int e = 0; e < 36; ++e) bm[e] = B[e];
+#pragma unroll
+                    for (int r = 0; r < 6; ++r) {
+                        double ar[6];
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) ar[m] = A[6 * r + m];
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) {
+                            double v = acc[6 * r + c];
+#pragma unroll
+                            for (int m = 0; m < 6; ++m) v -= ar[m] * bm[6 * c + m];
+                            acc[6 * r + c] = v;
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 36; ++e) C[e] = acc[e];
+                }
+            } else if (tid - 64 < 6 * nupd) {
+                const int q = (tid - 64) / 6, r = (tid - 64) - 6 * ((tid - 64) / 6);
                 int jj = 0, qq = q, len = rem;
                 while (qq >= len) { qq -= len; ++jj; --len; }
                 const int J = K + 2 + jj, I = J + qq;
-                const double* A = Lb + blk_off(I, K);
+                const double* A = Lb + blk_off(I, K) + 6 * r;
                 const double* B = Lb + blk_off(J, K);
-                double* C = Lb + blk_off(I, J);
-                double acc[36];
+                double* C = Lb + blk_off(I, J) + 6 * r;
+                double ar[6], acc[6];
 #pragma unroll
-                for (int e = 0; e < 36; ++e) acc[e] = C[e];
-                double bm[36];
+                for (int m = 0; m < 6; ++m) ar[m] = A[m];
 #pragma unroll
-                for (int e = 0; e < 36; ++e) bm[e] = B[e];
+                for (int c = 0; c < 6; ++c) acc[c] = C[c];
 #pragma unroll
-                for (int r = 0; r < 6; ++r) {
-                    double ar[6];
+                for (int c = 0; c < 6; ++c) {
+                    double v = acc[c];
 #pragma unroll
-                    for (int m = 0; m < 6; ++m) ar[m] = A[6 * r + m];
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        double v = acc[6 * r + c];
-#pragma unroll
-                        for (int m = 0; m < 6; ++m) v -= ar[m] * bm[6 * c + m];
-                        acc[6 * r + c] = v;
-                    }
+                    for (int m = 0; m < 6; ++m) v -= ar[m] * B[6 * c + m];
+                    acc[c] = v;
                 }
 #pragma unroll
-                for (int e = 0; e < 36; ++e) C[e] = acc[e];
+                for (int c = 0; c < 6; ++c) C[c] = acc[c];
             }
         }
         __syncthreads();
