@@ -408,6 +408,45 @@ __device__ __forceinline__ T wave_sum(T v) {
     return (readlane(v, 15) + readlane(v, 31)) + (readlane(v, 47) + readlane(v, 63));
 }
 
+// The wave's sums of a[0..27) by a reduce-scatter: at distances 32, 16, 8,
+// 4 and 2 each lane pair splits its current values in halves (28 -> 14 -> 7
+// -> 4 -> 2 -> 1, zero padded), keeps one half and adds the partner's copy
+// of it; then lanes 2i and 2i+1 add their values.  Lane l ends with the sum
+// of index 14 b5 + 7 b4 + (4 b3 + 2 b2 + b1) (b_k = bit k of l; none when the
+// last term is 7 or the index is 27).  A fixed order: the same bits every
+// run.  About a third of the instructions of 27 separate wave sums.
+__device__ __forceinline__ double wave_reduce_scatter27(const double (&a)[27], int lane, int& idx) {
+    double v14[14], v7[7], v4[4], v2[2];
+    const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0, h3 = (lane & 8) != 0, h2 = (lane & 4) != 0,
+               h1 = (lane & 2) != 0;
+#pragma unroll
+    for (int j = 0; j < 14; ++j) {
+        const double lo = a[j], hi = 14 + j < 27 ? a[14 + j] : 0.0;
+        v14[j] = (h5 ? hi : lo) + __shfl_xor(h5 ? lo : hi, 32);
+    }
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const double lo = v14[j], hi = v14[7 + j];
+        v7[j] = (h4 ? hi : lo) + __shfl_xor(h4 ? lo : hi, 16);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double lo = v7[j], hi = 4 + j < 7 ? v7[4 + j] : 0.0;
+        v4[j] = (h3 ? hi : lo) + __shfl_xor(h3 ? lo : hi, 8);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double lo = v4[j], hi = v4[2 + j];
+        v2[j] = (h2 ? hi : lo) + __shfl_xor(h2 ? lo : hi, 4);
+    }
+    double v = (h1 ? v2[1] : v2[0]) + __shfl_xor(h1 ? v2[0] : v2[1], 2);
+    v += __shfl_xor(v, 1);
+    const int local = 4 * (int)h3 + 2 * (int)h2 + (int)h1;
+    const int i = 14 * (int)h5 + 7 * (int)h4 + local;
+    idx = (local < 7 && i < 27) ? i : -1;
+    return v;
+}
+
 // Sum over a block of NT threads; result valid in thread 0.  `sh` needs NT/64 slots.
 template <typename T, int NT>
 __device__ __forceinline__ T block_sum(T v, T* sh) {

@@ -220,11 +220,9 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
                                                                  : fma(J[6 + i], r1, fma(J[i], r0, acc[21 + i]));
     }
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        const double v = wave_sum(acc[k]);
-        if (lane == 0) red[wid][k] = v;
-    }
+    int k;
+    const double v = wave_reduce_scatter27(acc, lane, k);
+    if ((lane & 1) == 0 && k >= 0) red[wid][k] = v;
     __syncthreads();
     if (threadIdx.x < 27) {
         double s = 0.0;

@@ -374,16 +374,14 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const Cam
     // ---- diagonal block (slot 0 of group 0) and coefficient: wave sums, then the
     // waves in order; S-row slots out (slot-major)
     if (g0) {
+        double a27[27];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const double v = wave_sum(coef[k]);
-            if (lane == 0) red[wid * 27 + 21 + k] = v;
-        }
+        for (int q = 0; q < 21; ++q) a27[q] = dsum[q];
 #pragma unroll
-        for (int q = 0; q < 21; ++q) {
-            const double v = wave_sum(dsum[q]);
-            if (lane == 0) red[wid * 27 + q] = v;
-        }
+        for (int k = 0; k < 6; ++k) a27[21 + k] = coef[k];
+        int k27;
+        const double v = wave_reduce_scatter27(a27, lane, k27);
+        if ((lane & 1) == 0 && k27 >= 0) red[wid * 27 + k27] = v;
     }
     __syncthreads();
     double* out = spart + P.sitem_pofs[it];
