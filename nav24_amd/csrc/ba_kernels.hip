@@ -672,11 +672,24 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
             for (int a = o0 + (int)threadIdx.x; a < o1; a += kBlock) cost += trial_term<CM>(P, cn, P.o_cam[a], xn, a);
         }
     }
-    __shared__ double shs[kBlock / kWave];
-    const double s = block_sum<double, kBlock>(sc, shs);
+    // both tile sums in one pass: wave sums, then the waves in order (the
+    // additions of block_sum, with one barrier for the two; shs is used
+    // nowhere else)
+    __shared__ double shs[2][kBlock / kWave];
+    const double ws = wave_sum(sc), wc = wave_sum(cost);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0) {
+        shs[0][wid] = ws;
+        shs[1][wid] = wc;
+    }
     __syncthreads();
-    const double cs = block_sum<double, kBlock>(cost, shs);
     if (threadIdx.x == 0) {
+        double s = 0.0, cs = 0.0;
+#pragma unroll
+        for (int i = 0; i < kBlock / kWave; ++i) {
+            s += shs[0][i];
+            cs += shs[1][i];
+        }
         red_scale[P.n_win + t] = s;
         red_tcost[t] = cs;
     }
