@@ -156,7 +156,8 @@ void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hip
 
 // ---------------------------------------------------------------- lin_cam
 // One workgroup per (free camera, chunk of its observations): Hpp (21 packed)
-// and bp (6) from T Jacobians, formed and block-reduced in FP64; then one
+// and bp (6) from T Jacobians, formed in FP64 and reduced per wave by a
+// reduce-scatter of the 27 sums, then over the waves in order; then one
 // workgroup per camera sums the chunks.  Inputs are read in camera-major
 // order (c_pt, c_uv), so the per-observation reads are contiguous.
 template <typename T, int CM>

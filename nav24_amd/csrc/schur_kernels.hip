@@ -371,8 +371,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const Cam
         flush_slots(acc, wbuf, lfirst, lcount, ns, G, tid, kBlock);
         __syncthreads();
     }
-    // ---- diagonal block (slot 0 of group 0) and coefficient: wave sums, then the
-    // waves in order; S-row slots out (slot-major)
+    // ---- diagonal block (slot 0 of group 0) and coefficient: the 27 sums of
+    // each wave by a reduce-scatter, then the waves in order; S-row slots out
+    // (slot-major)
     if (g0) {
         double a27[27];
 #pragma unroll
