@@ -141,12 +141,23 @@ struct DevProblem {
     int use_win;               // 1: per-window in-LDS Cholesky summed from the Schur partials (every batch)
     int commit_in_accept;      // 1: k_lm_accept commits accepted trials (small windows), no k_commit
     int use_pcg;               // 1: block-Jacobi conjugate gradients on the reduced system (one problem)
+    // the points' two state buffers: window w's current points are in
+    // xb[xsel[w]], its candidate (a trial's back-substitution) in the other;
+    // an accepted trial flips xsel[w] instead of copying its points
+    D4* xb0;
+    D4* xb1;
+    uint8_t* xsel;             // [n_win]
 };
+
+// Window win's current and candidate points.
+__device__ __forceinline__ const D4* x_cur(const DevProblem& P, int win) { return P.xsel[win] ? P.xb1 : P.xb0; }
+__device__ __forceinline__ D4* x_cand(const DevProblem& P, int win) { return P.xsel[win] ? P.xb0 : P.xb1; }
 
 struct DevState {
     PoseQT* pose;  // [n_cam]
     CamRT* cam;    // [n_cam]
-    D4* X;         // [n_pt]
+    D4* X;         // [n_pt]: state buffer 0 (cur) or 1 (cand); which one holds a window's
+                   // current points is its xsel (DevProblem)
 };
 
 // Work buffers.

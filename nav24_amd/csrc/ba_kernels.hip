@@ -12,11 +12,14 @@
 //              the damped point inverse (Hll + lambda I)^-1 itself, FP64
 //   schur_red  partials -> dense S (FP64) and g per window
 //   cholesky   tiled (one large window) or in-LDS per window (chol_kernels.hip)
-//   backsub    dX = (Hll + lambda I)^-1 (bl - Hpl^T dxi), candidate points, x'(lambda x + b)
 //   update_cam T <- exp(dxi) T (FP64 SE3 exp-map), candidate cameras
-//   cost       robust chi2 at the candidate, window-aligned partial sums
-//   accept     rho, lambda / nu schedule, accept or reject, next phase
-//   commit     candidate -> current for windows whose trial was accepted
+//   backsub    dX = (Hll + lambda I)^-1 (bl - Hpl^T dxi), candidate points into
+//              the window's other state buffer, x'(lambda x + b), and the
+//              robust chi2 at the candidate per tile
+//   accept     rho, lambda / nu schedule, accept or reject, next phase; an
+//              accepted trial flips the window's point buffer (no copy)
+//   commit     candidate cameras -> current for windows whose trial was accepted
+//   cost       robust chi2 at the current state (a solve's start)
 // Every reduction has a fixed order: results are deterministic.
 #include <float.h>
 
@@ -66,13 +69,14 @@ void launch_lm_reset(const DevProblem& P, const DevWork& W, hipStream_t st) {
 // observations.  The fisheye model's transcendental functions need more.
 template <typename T, int CM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CM == kCamPinhole ? 7 : 1, 8))) void k_lin_pt(DevProblem P, const CamRT* __restrict__ cam,
-                                                   const D4* __restrict__ X, const LMDev* lm,
+                                                   const LMDev* lm,
                                                    T* __restrict__ w_out, double* __restrict__ Hl) {
     // rows padded by one double: the 9 sums of one point read one column, which
     // an unpadded 2 KB row stride would put in a single LDS bank
     __shared__ double sh[9][kTile + 1];
     __shared__ double red[kBlock / kWave][9];
     const int t = blockIdx.x;
+    const D4* __restrict__ X = x_cur(P, P.tile_win[t]);  // its selector loaded beside the LM state
     if (!needs_lin(lm, P.tile_win[t])) return;
     const int o0 = P.tile_obs[t], o1 = P.tile_obs[t + 1];
     const int p0 = P.tile_pt[t], p1 = P.tile_pt[t + 1];
@@ -151,7 +155,7 @@ void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hip
     if (P.n_tiles == 0) return;
     const auto fn = P.cam_model == kCamKB8 ? k_lin_pt<T, kCamKB8> : k_lin_pt<T, kCamPinhole>;
     hipLaunchKernelGGL(fn, dim3(P.n_tiles),
-                       dim3(kBlock), 0, st, P, s.cam, s.X, (const LMDev*)W.lm, (T*)W.w, W.Hl);
+                       dim3(kBlock), 0, st, P, s.cam, (const LMDev*)W.lm, (T*)W.w, W.Hl);
 }
 
 // ---------------------------------------------------------------- lin_cam
@@ -162,12 +166,12 @@ void launch_lin_pt(const DevProblem& P, const DevState& s, const DevWork& W, hip
 // order (c_pt, c_uv), so the per-observation reads are contiguous.
 template <typename T, int CM>
 __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* __restrict__ cam,
-                                                    const D4* __restrict__ X, const T* __restrict__ w,
                                                     const LMDev* lm, double* __restrict__ hpart) {
     __shared__ double red[kBlock / kWave][27];
     const int it = blockIdx.x;
     const int f = P.hitem_f[it], a0 = P.hitem_a0[it], a1 = P.hitem_a1[it];
     const int cf = P.free_cam[f];
+    const D4* __restrict__ X = x_cur(P, P.cam_win[cf]);  // its selector loaded beside the LM state
     if (!needs_lin(lm, P.cam_win[cf])) return;
     double R[9], tt[3];
     load_cam(cam, cf, R, tt);
@@ -259,8 +263,7 @@ void launch_lin_cam(const DevProblem& P, const DevState& s, const DevWork& W, hi
     if (P.n_fc == 0) return;
     if (P.n_hitems > 0) {
         const auto fn = P.cam_model == kCamKB8 ? k_lin_cam<T, kCamKB8> : k_lin_cam<T, kCamPinhole>;
-        hipLaunchKernelGGL(fn, dim3(P.n_hitems), dim3(kBlock), 0, st, P, s.cam, s.X, (const T*)W.w,
-                           (const LMDev*)W.lm, W.hpart);
+        hipLaunchKernelGGL(fn, dim3(P.n_hitems), dim3(kBlock), 0, st, P, s.cam, (const LMDev*)W.lm, W.hpart);
     }
     hipLaunchKernelGGL(k_cam_reduce, dim3(P.n_fc), dim3(64), 0, st, P, (const double*)W.hpart, (const LMDev*)W.lm, W.Hc);
 }
@@ -550,16 +553,17 @@ __device__ __forceinline__ double trial_term(const DevProblem& P, const CamRT* _
 
 template <typename T, int CM>
 __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* __restrict__ cam,
-                                                    const D4* __restrict__ X, const T* __restrict__ w,
-                                                    const double* __restrict__ Hl,
+                                                    const T* __restrict__ w, const double* __restrict__ Hl,
                                                     const double* __restrict__ dp, const LMDev* lm,
-                                                    D4* __restrict__ Xn, double* __restrict__ red_scale,
+                                                    double* __restrict__ red_scale,
                                                     const CamRT* __restrict__ cn, double* __restrict__ red_tcost) {
     __shared__ double sh[3][kTile];
     __shared__ double red[kBlock / kWave][3];
     __shared__ D4 xs_new[kTile];  // the tile's candidate points (a point has an observation: at most kTile)
     const int t = blockIdx.x;
     const int win = P.tile_win[t];
+    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
+    D4* __restrict__ Xn = x_cand(P, win);      // and the buffer of its candidate
     if (!running(lm, win)) return;
     const int o0 = P.tile_obs[t], o1 = P.tile_obs[t + 1];
     const int p0 = P.tile_pt[t], p1 = P.tile_pt[t + 1];
@@ -701,8 +705,8 @@ void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& ca
     if (P.n_tiles == 0) return;
     const auto fn = P.cam_model == kCamKB8 ? k_backsub<T, kCamKB8> : k_backsub<T, kCamPinhole>;
     hipLaunchKernelGGL(fn, dim3(P.n_tiles),
-                       dim3(kBlock), 0, st, P, cur.cam, cur.X, (const T*)W.w, (const double*)W.Hl, W.dp,
-                       (const LMDev*)W.lm, cand.X, W.red_scale, (const CamRT*)cand.cam, W.red_tcost);
+                       dim3(kBlock), 0, st, P, cur.cam, (const T*)W.w, (const double*)W.Hl, W.dp,
+                       (const LMDev*)W.lm, W.red_scale, (const CamRT*)cand.cam, W.red_tcost);
 }
 
 // ---------------------------------------------------------------- camera update
@@ -732,11 +736,12 @@ void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState&
 
 // ---------------------------------------------------------------- cost
 template <typename T, int CM>
-__global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restrict__ cam, const D4* __restrict__ X,
+__global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restrict__ cam,
                                               const LMDev* lm, int trial, double* __restrict__ red_cost) {
     __shared__ double sh[4];
     const int b = blockIdx.x;
     const int win = P.cblk_win[b];
+    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
     if (trial && (!running(lm, win) || lm[win].fail)) {
         if (threadIdx.x == 0) red_cost[b] = 0.0;
         return;
@@ -779,7 +784,7 @@ void launch_cost(const DevProblem& P, const DevState& s, const DevWork& W, int t
     if (P.n_cblk == 0) return;
     const auto fn = P.cam_model == kCamKB8 ? k_cost<T, kCamKB8> : k_cost<T, kCamPinhole>;
     hipLaunchKernelGGL(fn, dim3(P.n_cblk), dim3(256),
-                       0, st, P, s.cam, s.X, (const LMDev*)W.lm, trial, W.red_cost);
+                       0, st, P, s.cam, (const LMDev*)W.lm, trial, W.red_cost);
 }
 
 // ---------------------------------------------------------------- LM decision
@@ -812,7 +817,6 @@ __device__ double sum_range(const double* v, int i0, int i1, double* sh) {
 __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* __restrict__ red_tcost,
                                                    const double* __restrict__ red_scale, LMDev* lm,
                                                    double* __restrict__ hist, int* done_count,
-                                                   D4* __restrict__ X, const D4* __restrict__ Xn,
                                                    PoseQT* __restrict__ pose, const PoseQT* __restrict__ pn,
                                                    CamRT* __restrict__ cam, const CamRT* __restrict__ cn) {
     __shared__ double sh[4];
@@ -870,12 +874,14 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
         L.fail = 0;
         lm[win] = L;
         acc = L.accepted;
+        // an accepted trial's points become current by flipping the window's
+        // state buffer (the back-substitution wrote them to the other one)
+        if (L.accepted) P.xsel[win] ^= 1;
     }
     if (!P.commit_in_accept) return;
-    // small windows: the commit of an accepted trial (k_commit) by the same workgroup
+    // small windows: the commit of an accepted trial's cameras (k_commit) by the same workgroup
     __syncthreads();
     if (!acc) return;
-    for (int i = P.win_pt0[win] + threadIdx.x; i < P.win_pt0[win + 1]; i += 256) X[i] = Xn[i];
     for (int c = P.win_cam0[win] + threadIdx.x; c < P.win_cam0[win + 1]; c += 256) {
         pose[c] = pn[c];
         cam[c] = cn[c];
@@ -885,7 +891,7 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
 void launch_lm_accept(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
                       hipStream_t st) {
     hipLaunchKernelGGL(k_lm_accept, dim3(P.n_win), dim3(256), 0, st, P, (const double*)W.red_tcost,
-                       (const double*)W.red_scale, W.lm, W.hist, W.done_count, cur.X, (const D4*)cand.X, cur.pose,
+                       (const double*)W.red_scale, W.lm, W.hist, W.done_count, cur.pose,
                        (const PoseQT*)cand.pose, cur.cam, (const CamRT*)cand.cam);
 }
 
@@ -905,30 +911,28 @@ void launch_lm_set_chi(const DevProblem& P, const DevWork& W, hipStream_t st) {
     hipLaunchKernelGGL(k_lm_set_chi, dim3(P.n_win), dim3(256), 0, st, P, (const double*)W.red_cost, W.lm, W.hist);
 }
 
-__global__ void k_commit(DevProblem P, const LMDev* lm, D4* __restrict__ X, const D4* __restrict__ Xn,
-                         PoseQT* __restrict__ pose, const PoseQT* __restrict__ pn, CamRT* __restrict__ cam,
-                         const CamRT* __restrict__ cn) {
+__global__ void k_commit(DevProblem P, const LMDev* lm, PoseQT* __restrict__ pose, const PoseQT* __restrict__ pn,
+                         CamRT* __restrict__ cam, const CamRT* __restrict__ cn) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < P.n_pt && lm[P.pt_win[i]].accepted) X[i] = Xn[i];
     if (i < P.n_cam && lm[P.cam_win[i]].accepted) { pose[i] = pn[i]; cam[i] = cn[i]; }
 }
 
 void launch_commit(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st) {
-    if (P.commit_in_accept) return;  // done by k_lm_accept
-    const int n = P.n_pt > P.n_cam ? P.n_pt : P.n_cam;
-    hipLaunchKernelGGL(k_commit, dim3((n + 255) / 256), dim3(256), 0, st, P, (const LMDev*)W.lm, cur.X,
-                       (const D4*)cand.X, cur.pose, (const PoseQT*)cand.pose, cur.cam, (const CamRT*)cand.cam);
+    if (P.commit_in_accept || P.n_cam == 0) return;  // done by k_lm_accept
+    hipLaunchKernelGGL(k_commit, dim3((P.n_cam + 255) / 256), dim3(256), 0, st, P, (const LMDev*)W.lm, cur.pose,
+                       (const PoseQT*)cand.pose, cur.cam, (const CamRT*)cand.cam);
 }
 
 // ---------------------------------------------------------------- outlier flags
 // FP64 at the given state: flag = chi2 > thr or depth <= 0; written in the
 // caller's observation order.  Also the FP64 robust cost and the flag count.
 template <int CM>
-__global__ __launch_bounds__(256) void k_flags(DevProblem P, const CamRT* __restrict__ cam, const D4* __restrict__ X,
+__global__ __launch_bounds__(256) void k_flags(DevProblem P, const CamRT* __restrict__ cam,
                                                double thr, uint8_t* __restrict__ out, double* __restrict__ red_cost,
                                                double* __restrict__ red_cnt) {
     __shared__ double sh[4];
     const int b = blockIdx.x;
+    const D4* __restrict__ X = x_cur(P, P.cblk_win[b]);  // the window's final points
     double acc = 0.0, cnt = 0.0;
     for (int a = P.cblk_o0[b] + threadIdx.x; a < P.cblk_o0[b + 1]; a += 256) {
         double R[9], tt[3];
@@ -952,7 +956,7 @@ void launch_flags(const DevProblem& P, const DevState& s, const DevWork& W, doub
     if (P.n_cblk == 0) return;
     const auto fn = P.cam_model == kCamKB8 ? k_flags<kCamKB8> : k_flags<kCamPinhole>;
     hipLaunchKernelGGL(fn, dim3(P.n_cblk), dim3(256), 0,
-                       st, P, s.cam, s.X, thr, W.outlier, W.red_cost, W.red_aux);
+                       st, P, s.cam, thr, W.outlier, W.red_cost, W.red_aux);
 }
 
 __global__ __launch_bounds__(256) void k_final(DevProblem P, const double* __restrict__ red_cost,

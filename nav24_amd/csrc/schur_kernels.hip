@@ -284,7 +284,7 @@ __device__ __forceinline__ void flush_slots(double* acc, const double* stage, co
 
 template <typename T, int CM>
 __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const CamRT* __restrict__ cam,
-                                                      const D4* __restrict__ X, const T* __restrict__ w,
+                                                      const T* __restrict__ w,
                                                       const double* __restrict__ Hl, const LMDev* lm,
                                                       double* __restrict__ spart) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -293,6 +293,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const Cam
     const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
     const int cf = P.free_cam[f];
     const int win = P.cam_win[cf];
+    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
     if (!running(lm, win)) return;
     const double lambda = lm[win].lambda;
     const bool g0 = (g == 0);
@@ -433,7 +434,7 @@ static_assert(kPcRound + kPcCons == kBlock && kPcRound == 2 * kWave, "two produc
 
 template <typename T, int CM>
 __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const CamRT* __restrict__ cam,
-                                                         const D4* __restrict__ X, const T* __restrict__ w,
+                                                         const T* __restrict__ w,
                                                          const double* __restrict__ Hl, const LMDev* lm,
                                                          double* __restrict__ spart) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -442,6 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const Cam
     const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
     const int cf = P.free_cam[f];
     const int win = P.cam_win[cf];
+    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
     if (!running(lm, win)) return;
     const double lambda = lm[win].lambda;
     const bool g0 = (g == 0);
@@ -627,7 +629,7 @@ void launch_schur_row(const DevProblem& P, const DevState& s, const DevWork& W, 
     const bool kb = P.cam_model == kCamKB8;
     auto fn = P.schur_pc ? (kb ? k_schur_row_pc<T, kCamKB8> : k_schur_row_pc<T, kCamPinhole>)
                          : (kb ? k_schur_row<T, kCamKB8> : k_schur_row<T, kCamPinhole>);
-    hipLaunchKernelGGL(fn, dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam, s.X, (const T*)W.w,
+    hipLaunchKernelGGL(fn, dim3(P.n_sitems), dim3(kBlock), schur_row_smem(P), st, P, s.cam, (const T*)W.w,
                        (const double*)W.Hl, (const LMDev*)W.lm, W.spart);
 }
 
