@@ -551,20 +551,19 @@ __device__ __forceinline__ double trial_term(const DevProblem& P, const CamRT* _
     return robustify(s, P.robust, P.delta, w);
 }
 
+// Tile t of window win: X the window's current points, Xn the buffer of its
+// candidate (restrict parameters: the compiler may keep point loads ahead of
+// the candidate stores).
 template <typename T, int CM>
-__global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* __restrict__ cam,
-                                                    const T* __restrict__ w, const double* __restrict__ Hl,
-                                                    const double* __restrict__ dp, const LMDev* lm,
-                                                    double* __restrict__ red_scale,
-                                                    const CamRT* __restrict__ cn, double* __restrict__ red_tcost) {
+__device__ __forceinline__ void backsub_tile(const DevProblem& P, const CamRT* __restrict__ cam,
+                                             const D4* __restrict__ X, const T* __restrict__ w,
+                                             const double* __restrict__ Hl, const double* __restrict__ dp,
+                                             const LMDev* lm, D4* __restrict__ Xn, double* __restrict__ red_scale,
+                                             const CamRT* __restrict__ cn, double* __restrict__ red_tcost, int t,
+                                             int win) {
     __shared__ double sh[3][kTile];
     __shared__ double red[kBlock / kWave][3];
     __shared__ D4 xs_new[kTile];  // the tile's candidate points (a point has an observation: at most kTile)
-    const int t = blockIdx.x;
-    const int win = P.tile_win[t];
-    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
-    D4* __restrict__ Xn = x_cand(P, win);      // and the buffer of its candidate
-    if (!running(lm, win)) return;
     const int o0 = P.tile_obs[t], o1 = P.tile_obs[t + 1];
     const int p0 = P.tile_pt[t], p1 = P.tile_pt[t + 1];
     const bool fail = lm[win].fail != 0;
@@ -698,6 +697,18 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
         red_scale[P.n_win + t] = s;
         red_tcost[t] = cs;
     }
+}
+
+template <typename T, int CM>
+__global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* __restrict__ cam,
+                                                    const T* __restrict__ w, const double* __restrict__ Hl,
+                                                    const double* __restrict__ dp, const LMDev* lm,
+                                                    double* __restrict__ red_scale,
+                                                    const CamRT* __restrict__ cn, double* __restrict__ red_tcost) {
+    const int t = blockIdx.x;
+    const int win = P.tile_win[t];
+    if (!running(lm, win)) return;
+    backsub_tile<T, CM>(P, cam, x_cur(P, win), w, Hl, dp, lm, x_cand(P, win), red_scale, cn, red_tcost, t, win);
 }
 
 template <typename T>
