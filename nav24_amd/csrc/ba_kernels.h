@@ -139,11 +139,12 @@ struct DevProblem {
     int n, n_pad, nt;
     int use_blocks;            // 1: single-workgroup block Cholesky in global memory
     int use_win;               // 1: per-window in-LDS Cholesky summed from the Schur partials (every batch)
-    int commit_in_accept;      // 1: k_lm_accept commits accepted trials (small windows), no k_commit
+    int commit_in_accept;      // 1: k_lm_accept commits accepted trials' cameras (small windows), no k_commit
     int use_pcg;               // 1: block-Jacobi conjugate gradients on the reduced system (one problem)
-    // the points' two state buffers: window w's current points are in
-    // xb[xsel[w]], its candidate (a trial's back-substitution) in the other;
-    // an accepted trial flips xsel[w] instead of copying its points
+    // the points' two state buffers: window w's current points are in xb1
+    // when xsel[w] is set, else in xb0, and its candidate (a trial's
+    // back-substitution) in the other; an accepted trial flips xsel[w]
+    // instead of copying its points
     D4* xb0;
     D4* xb1;
     uint8_t* xsel;             // [n_win]
