@@ -30,7 +30,7 @@ constexpr int kBlocksMaxFc = 128;  // one problem: the single-workgroup block Ch
 constexpr int kCostObs = 1024;     // observations per cost / flag block
 constexpr int kXcds = 8;           // XCDs of one MI355X: workgroup i of a launch runs on XCD i % 8
 constexpr int kBurstWindows = 64;  // up to this many windows the host reads their LM state between bursts
-constexpr int kCommitInAcceptPts = 8192;  // windows up to this many points commit inside the LM decision
+constexpr int kCommitInAcceptCams = 1024;  // windows up to this many cameras commit them inside the LM decision
 
 // LM phase of one window
 enum { kPhaseLin = 0, kPhaseTrial = 1, kPhaseDone = 2 };
@@ -294,11 +294,12 @@ void launch_pcg_finish(const DevProblem& P, const DevWork& W, hipStream_t st);
 template <typename T> void launch_backsub(const DevProblem& P, const DevState& cur, const DevState& cand,
                                           const DevWork& W, hipStream_t st);
 void launch_update_cam(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W, hipStream_t st);
-// trial = 1: candidate state of running windows; trial = 0: current state, all windows
+// The cost at the current state; trial = 1 leaves out finished and failed windows
 template <typename T> void launch_cost(const DevProblem& P, const DevState& s, const DevWork& W, int trial,
                                        hipStream_t st);
-// The LM decision; with commit_in_accept (every window at most kCommitInAcceptPts
-// points) it also commits an accepted trial, and launch_commit does nothing.
+// The LM decision (an accepted trial flips the window's point buffer); with
+// commit_in_accept (every window at most kCommitInAcceptCams cameras) it also
+// commits the accepted cameras, and launch_commit does nothing.
 void launch_lm_accept(const DevProblem& P, const DevState& cur, const DevState& cand, const DevWork& W,
                       hipStream_t st);
 void launch_lm_set_chi(const DevProblem& P, const DevWork& W, hipStream_t st);

@@ -1456,9 +1456,9 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     if (P.use_pcg) P.use_win = 0;
     P.use_blocks = (!P.use_win && !P.use_pcg && nfc > 0 && nfc <= kBlocksMaxFc && h->solver_mode == 2) ? 1 : 0;
     {
-        int maxpt = 0;
-        for (int w = 0; w < n_win; ++w) maxpt = std::max(maxpt, L.pt_first[w + 1] - L.pt_first[w]);
-        P.commit_in_accept = maxpt <= kCommitInAcceptPts ? 1 : 0;
+        int maxcam = 0;
+        for (int w = 0; w < n_win; ++w) maxcam = std::max(maxcam, L.cam_off[w + 1] - L.cam_off[w]);
+        P.commit_in_accept = maxcam <= kCommitInAcceptCams ? 1 : 0;
     }
 
     const size_t tsz = mixed(h) ? sizeof(float) : sizeof(double);
