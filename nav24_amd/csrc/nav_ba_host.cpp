@@ -608,12 +608,20 @@ bool build_schur_items(const SchurGraph& G, int chunk, bool pc, SchurItems& S) {
         if (S.pairs.size() + tot > (size_t)0x7fffffff) return false;
         S.rnd_pair0.push_back((int)S.pairs.size());
         S.rnd_rows.push_back(int4{rows[0], rows[1], rows[2], rows[3]});
-        for (int wv = 0; wv < nw; ++wv)
+        const size_t base = S.pairs.size();
+        S.pairs.resize(base + tot);
+        int2* out = S.pairs.data() + base;
+        const int2* src[kWave];
+        int len[kWave];
+        for (int wv = 0; wv < nw; ++wv) {
+            for (int l = 0; l < kWave; ++l) {
+                const int t = l0 + wv * kWave + l;
+                len[l] = lane_len[t];
+                src[l] = len[l] > 0 ? bucket[lane_s[t]].data() + lane_beg[t] : nullptr;
+            }
             for (int j = 0; j < rows[wv]; ++j)
-                for (int l = 0; l < kWave; ++l) {
-                    const int t = l0 + wv * kWave + l;
-                    S.pairs.push_back(j < lane_len[t] ? bucket[lane_s[t]][lane_beg[t] + j] : int2{-1, 0});
-                }
+                for (int l = 0; l < kWave; ++l) *out++ = j < len[l] ? src[l][j] : int2{-1, 0};
+        }
         return true;
     };
     // a slot's pairs of one round split into contiguous, nearly equal shares
