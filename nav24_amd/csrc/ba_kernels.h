@@ -41,7 +41,8 @@ struct alignas(32) D4 { double x, y, z, w; };
 struct LMDev {
     double lambda, nu, chi, temp_chi, scale, rho;
     int fail, accepted, trials, phase;
-    int iter, q, stop, pad;
+    int iter, q, stop;
+    int xsel;  // the point buffer (DevProblem xb0 / xb1) that holds the window's current points
 };
 
 // State of the conjugate-gradient solve of one problem's reduced system
@@ -142,23 +143,26 @@ struct DevProblem {
     int commit_in_accept;      // 1: k_lm_accept commits accepted trials' cameras (small windows), no k_commit
     int use_pcg;               // 1: block-Jacobi conjugate gradients on the reduced system (one problem)
     // the points' two state buffers: window w's current points are in xb1
-    // when xsel[w] is set, else in xb0, and its candidate (a trial's
-    // back-substitution) in the other; an accepted trial flips xsel[w]
+    // when its LM state's xsel is set, else in xb0, and its candidate (a
+    // trial's back-substitution) in the other; an accepted trial flips xsel
     // instead of copying its points
     D4* xb0;
     D4* xb1;
-    uint8_t* xsel;             // [n_win]
 };
 
 // Window win's current and candidate points.
-__device__ __forceinline__ const D4* x_cur(const DevProblem& P, int win) { return P.xsel[win] ? P.xb1 : P.xb0; }
-__device__ __forceinline__ D4* x_cand(const DevProblem& P, int win) { return P.xsel[win] ? P.xb0 : P.xb1; }
+__device__ __forceinline__ const D4* x_cur(const DevProblem& P, const LMDev* lm, int win) {
+    return lm[win].xsel ? P.xb1 : P.xb0;
+}
+__device__ __forceinline__ D4* x_cand(const DevProblem& P, const LMDev* lm, int win) {
+    return lm[win].xsel ? P.xb0 : P.xb1;
+}
 
 struct DevState {
     PoseQT* pose;  // [n_cam]
     CamRT* cam;    // [n_cam]
     D4* X;         // [n_pt]: state buffer 0 (cur) or 1 (cand); which one holds a window's
-                   // current points is its xsel (DevProblem)
+                   // current points is its LM state's xsel
 };
 
 // Work buffers.

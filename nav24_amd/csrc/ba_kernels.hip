@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CM == kC
     __shared__ double sh[9][kTile + 1];
     __shared__ double red[kBlock / kWave][9];
     const int t = blockIdx.x;
-    const D4* __restrict__ X = x_cur(P, P.tile_win[t]);  // its selector loaded beside the LM state
+    const D4* __restrict__ X = x_cur(P, lm, P.tile_win[t]);  // its selector loaded beside the LM state
     if (!needs_lin(lm, P.tile_win[t])) return;
     const int o0 = P.tile_obs[t], o1 = P.tile_obs[t + 1];
     const int p0 = P.tile_pt[t], p1 = P.tile_pt[t + 1];
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cam(DevProblem P, const CamRT* _
     const int it = blockIdx.x;
     const int f = P.hitem_f[it], a0 = P.hitem_a0[it], a1 = P.hitem_a1[it];
     const int cf = P.free_cam[f];
-    const D4* __restrict__ X = x_cur(P, P.cam_win[cf]);  // its selector loaded beside the LM state
+    const D4* __restrict__ X = x_cur(P, lm, P.cam_win[cf]);  // its selector loaded beside the LM state
     if (!needs_lin(lm, P.cam_win[cf])) return;
     double R[9], tt[3];
     load_cam(cam, cf, R, tt);
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P, const CamRT* _
     const int t = blockIdx.x;
     const int win = P.tile_win[t];
     if (!running(lm, win)) return;
-    backsub_tile<T, CM>(P, cam, x_cur(P, win), w, Hl, dp, lm, x_cand(P, win), red_scale, cn, red_tcost, t, win);
+    backsub_tile<T, CM>(P, cam, x_cur(P, lm, win), w, Hl, dp, lm, x_cand(P, lm, win), red_scale, cn, red_tcost, t, win);
 }
 
 template <typename T>
@@ -752,7 +752,7 @@ __global__ __launch_bounds__(256) void k_cost(DevProblem P, const CamRT* __restr
     __shared__ double sh[4];
     const int b = blockIdx.x;
     const int win = P.cblk_win[b];
-    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
+    const D4* __restrict__ X = x_cur(P, lm, win);  // the window's current points
     if (trial && (!running(lm, win) || lm[win].fail)) {
         if (threadIdx.x == 0) red_cost[b] = 0.0;
         return;
@@ -855,6 +855,9 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
             L.nu = 2;
             L.chi = temp;
             L.accepted = 1;
+            // the trial's points become current: the back-substitution wrote
+            // them to the other state buffer
+            L.xsel ^= 1;
         } else {
             L.lambda *= L.nu;
             L.nu *= 2;
@@ -885,9 +888,6 @@ __global__ __launch_bounds__(256) void k_lm_accept(DevProblem P, const double* _
         L.fail = 0;
         lm[win] = L;
         acc = L.accepted;
-        // an accepted trial's points become current by flipping the window's
-        // state buffer (the back-substitution wrote them to the other one)
-        if (L.accepted) P.xsel[win] ^= 1;
     }
     if (!P.commit_in_accept) return;
     // small windows: the commit of an accepted trial's cameras (k_commit) by the same workgroup
@@ -938,12 +938,12 @@ void launch_commit(const DevProblem& P, const DevState& cur, const DevState& can
 // FP64 at the given state: flag = chi2 > thr or depth <= 0; written in the
 // caller's observation order.  Also the FP64 robust cost and the flag count.
 template <int CM>
-__global__ __launch_bounds__(256) void k_flags(DevProblem P, const CamRT* __restrict__ cam,
+__global__ __launch_bounds__(256) void k_flags(DevProblem P, const CamRT* __restrict__ cam, const LMDev* lm,
                                                double thr, uint8_t* __restrict__ out, double* __restrict__ red_cost,
                                                double* __restrict__ red_cnt) {
     __shared__ double sh[4];
     const int b = blockIdx.x;
-    const D4* __restrict__ X = x_cur(P, P.cblk_win[b]);  // the window's final points
+    const D4* __restrict__ X = x_cur(P, lm, P.cblk_win[b]);  // the window's final points
     double acc = 0.0, cnt = 0.0;
     for (int a = P.cblk_o0[b] + threadIdx.x; a < P.cblk_o0[b + 1]; a += 256) {
         double R[9], tt[3];
@@ -967,7 +967,7 @@ void launch_flags(const DevProblem& P, const DevState& s, const DevWork& W, doub
     if (P.n_cblk == 0) return;
     const auto fn = P.cam_model == kCamKB8 ? k_flags<kCamKB8> : k_flags<kCamPinhole>;
     hipLaunchKernelGGL(fn, dim3(P.n_cblk), dim3(256), 0,
-                       st, P, s.cam, thr, W.outlier, W.red_cost, W.red_aux);
+                       st, P, s.cam, (const LMDev*)W.lm, thr, W.outlier, W.red_cost, W.red_aux);
 }
 
 __global__ __launch_bounds__(256) void k_final(DevProblem P, const double* __restrict__ red_cost,

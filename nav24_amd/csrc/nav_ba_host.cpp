@@ -816,7 +816,6 @@ int tune_schur_items(nav_ba_handle* h, const SchurGraph& G, int mark, int kernel
     const DevProblem& P = h->P;
     HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
     HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
-    HIPTRY(h, hipMemsetAsync(P.xsel, 0, P.n_win, s));  // every window's current points in buffer 0
     std::vector<LMDev> L(P.n_win);
     for (auto& l : L) { l = LMDev{}; l.lambda = 1e-4; l.nu = 2.0; l.phase = kPhaseLin; l.iter = 1; }
     HIPTRY(h, hipMemcpyAsync(h->W.lm, L.data(), sizeof(LMDev) * L.size(), hipMemcpyHostToDevice, s));
@@ -1499,7 +1498,6 @@ int prepare_windows(nav_ba_handle* h, const nav_ba_problem* probs, int n_win, co
     AL(W.lm, sizeof(LMDev) * n_win);
     AL(W.pcg, sizeof(double) * (P.use_pcg ? pcg_work_doubles(P.n) : (size_t)1));
     AL(W.pcg_st, sizeof(PcgState));
-    AL(P.xsel, (size_t)n_win);
 #undef AL
     P.xb0 = h->cur.X;
     P.xb1 = h->cand.X;
@@ -1574,24 +1572,22 @@ int fetch_all(nav_ba_handle* h, nav_ba_result* results, int n) {
     if (n != (int)h->layout.win.size()) return fail(h, NAV_BA_BAD_ARG, "result count differs from the prepared window count");
     std::vector<PoseQT> pose;
     std::vector<D4> X, X1;
-    std::vector<uint8_t> flags, sel;
+    std::vector<uint8_t> flags;
     if (!h->all_empty) {
         HIPTRY(h, hipSetDevice(h->device));
         pose.resize(h->P.n_cam);
         X.resize(h->P.n_pt);
         X1.resize(h->P.n_pt);
         flags.resize(h->P.n_obs);
-        sel.resize(h->P.n_win);
         HIPTRY(h, hipMemcpyAsync(pose.data(), h->cur.pose, sizeof(PoseQT) * pose.size(), hipMemcpyDeviceToHost, h->stream));
         HIPTRY(h, hipMemcpyAsync(X.data(), h->P.xb0, sizeof(D4) * X.size(), hipMemcpyDeviceToHost, h->stream));
         HIPTRY(h, hipMemcpyAsync(X1.data(), h->P.xb1, sizeof(D4) * X1.size(), hipMemcpyDeviceToHost, h->stream));
-        HIPTRY(h, hipMemcpyAsync(sel.data(), h->P.xsel, sel.size(), hipMemcpyDeviceToHost, h->stream));
         if (!flags.empty())
             HIPTRY(h, hipMemcpyAsync(flags.data(), h->W.outlier, flags.size(), hipMemcpyDeviceToHost, h->stream));
         HIPTRY(h, hipStreamSynchronize(h->stream));
-        // each window's current points from the state buffer its selector names
+        // each window's current points from the state buffer its LM state names
         for (int q = 0; q < h->P.n_pt; ++q)
-            if (sel[h->layout.pt_win[q]]) X[q] = X1[q];
+            if (h->lm_out[h->layout.pt_win[q]].xsel) X[q] = X1[q];
     }
     int worst = NAV_BA_OK;
     for (int w = 0; w < n; ++w) {
@@ -1687,7 +1683,6 @@ int32_t nav_ba_run(nav_ba_handle* h) {
     HIPTRY(h, hipEventRecord(h->ev0, s));
     HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
     HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
-    HIPTRY(h, hipMemsetAsync(P.xsel, 0, P.n_win, s));  // every window's current points in buffer 0
     launch_lm_reset(P, h->W, s);
     launch_cam_rt(P, h->cur, s);
     if (m) launch_cost<float>(P, h->cur, h->W, 0, s); else launch_cost<double>(P, h->cur, h->W, 0, s);
@@ -1826,7 +1821,6 @@ int32_t nav_ba_debug_time_schur_rows(nav_ba_handle* h, double lambda, int32_t re
     const DevProblem& P = h->P;
     HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
     HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
-    HIPTRY(h, hipMemsetAsync(P.xsel, 0, P.n_win, s));  // every window's current points in buffer 0
     std::vector<LMDev> L(P.n_win);
     for (auto& l : L) { l = LMDev{}; l.lambda = lambda; l.nu = 2.0; l.phase = kPhaseLin; l.iter = 1; }
     HIPTRY(h, hipMemcpyAsync(h->W.lm, L.data(), sizeof(LMDev) * L.size(), hipMemcpyHostToDevice, s));
@@ -1869,7 +1863,6 @@ int32_t nav_ba_debug_time_stage(nav_ba_handle* h, int32_t stage, double lambda, 
     const DevProblem& P = h->P;
     HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
     HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
-    HIPTRY(h, hipMemsetAsync(P.xsel, 0, P.n_win, s));  // every window's current points in buffer 0
     std::vector<LMDev> L(P.n_win);
     for (auto& l : L) { l = LMDev{}; l.lambda = lambda; l.nu = 2.0; l.phase = kPhaseLin; l.iter = 1; }
     HIPTRY(h, hipMemcpyAsync(h->W.lm, L.data(), sizeof(LMDev) * L.size(), hipMemcpyHostToDevice, s));
@@ -1946,7 +1939,6 @@ int32_t nav_ba_debug_reduced_system(nav_ba_handle* h, double lambda, double* S, 
     *n_out = P.n;
     HIPTRY(h, hipMemcpyAsync(h->cur.pose, h->init.pose, sizeof(PoseQT) * P.n_cam, hipMemcpyDeviceToDevice, s));
     HIPTRY(h, hipMemcpyAsync(h->cur.X, h->init.X, sizeof(D4) * P.n_pt, hipMemcpyDeviceToDevice, s));
-    HIPTRY(h, hipMemsetAsync(P.xsel, 0, P.n_win, s));  // every window's current points in buffer 0
     LMDev L{};
     L.lambda = lambda;
     L.nu = 2.0;

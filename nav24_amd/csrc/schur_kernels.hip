@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_schur_row(DevProblem P, const Cam
     const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
     const int cf = P.free_cam[f];
     const int win = P.cam_win[cf];
-    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
+    const D4* __restrict__ X = x_cur(P, lm, win);  // the window's current points
     if (!running(lm, win)) return;
     const double lambda = lm[win].lambda;
     const bool g0 = (g == 0);
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_row_pc(DevProblem P, const Cam
     const int g = P.sitem_g[it], ns = P.sitem_nslots[it];
     const int cf = P.free_cam[f];
     const int win = P.cam_win[cf];
-    const D4* __restrict__ X = x_cur(P, win);  // the window's current points
+    const D4* __restrict__ X = x_cur(P, lm, win);  // the window's current points
     if (!running(lm, win)) return;
     const double lambda = lm[win].lambda;
     const bool g0 = (g == 0);
