@@ -40,7 +40,7 @@ def _solve_window(args):
 
 
 def workers() -> int:
-    """Pool size: the host's cores, at most 16 (one GPU box's share)."""
+    """Pool size: the host's cores, at most 16 (the CPU share of a one-GPU host)."""
     return max(1, min(16, os.cpu_count() or 1))
 
 
