@@ -951,35 +951,37 @@ int build_layout(const nav_ba_problem* probs, int n_win, const nav_ba_options* o
     L.all_empty = !any;
     if (L.all_empty) return NAV_BA_OK;
 
-    // ---- observations ordered by (point, camera, input order): two stable counting sorts
-    std::vector<int> by_cam(no), ord(no);
+    // ---- observations ordered by (point, camera, input order): a stable counting
+    // sort by point, then a stable insertion sort by camera inside each point's
+    // observations (a few per point)
+    std::vector<int> ord(no);
+    L.pt_ptr.assign(np + 1, 0);
     {
-        std::vector<int> off(nc + 1, 0);
-        for (int i = 0; i < no; ++i) off[g_cam[i] + 1]++;
-        for (int c = 0; c < nc; ++c) off[c + 1] += off[c];
-        for (int i = 0; i < no; ++i) by_cam[off[g_cam[i]]++] = i;
-        std::vector<int> poff(np + 1, 0);
-        for (int i = 0; i < no; ++i) poff[g_pt[i] + 1]++;
-        for (int q = 0; q < np; ++q) poff[q + 1] += poff[q];
-        for (int k = 0; k < no; ++k) {
-            const int i = by_cam[k];
-            ord[poff[g_pt[i]]++] = i;
-        }
+        for (int i = 0; i < no; ++i) L.pt_ptr[g_pt[i] + 1]++;
+        for (int q = 0; q < np; ++q) L.pt_ptr[q + 1] += L.pt_ptr[q];
+        std::vector<int> pos(L.pt_ptr.begin(), L.pt_ptr.end() - 1);
+        for (int i = 0; i < no; ++i) ord[pos[g_pt[i]]++] = i;
+        for (int q = 0; q < np; ++q)
+            for (int k = L.pt_ptr[q] + 1; k < L.pt_ptr[q + 1]; ++k) {
+                const int i = ord[k], c = g_cam[i];
+                int j = k;
+                for (; j > L.pt_ptr[q] && g_cam[ord[j - 1]] > c; --j) ord[j] = ord[j - 1];
+                ord[j] = i;
+            }
     }
-    L.o_cam.resize(no); L.o_pt.resize(no); L.o_orig.resize(no); L.pt_ptr.assign(np + 1, 0);
+    L.o_cam.resize(no); L.o_pt.resize(no); L.o_orig.resize(no);
     L.o_uv.resize(2 * (size_t)no);
-    for (int k = 0; k < no; ++k) {
+    // windows hold contiguous point ranges, so the window of ord[k] never decreases in k
+    for (int k = 0, w = 0; k < no; ++k) {
         const int i = ord[k];
-        const int w = (int)(std::upper_bound(L.obs_off.begin(), L.obs_off.end(), i) - L.obs_off.begin()) - 1;
+        while (i >= L.obs_off[w + 1]) ++w;
         const double* uv = probs[w].obs_uv + 2 * (size_t)(i - L.obs_off[w]);
         L.o_cam[k] = g_cam[i];
         L.o_pt[k] = g_pt[i];
         L.o_orig[k] = i;
         L.o_uv[2 * (size_t)k] = compat ? rf32(uv[0]) : uv[0];  // cv::Point2f measurement (Point2D.hpp:33)
         L.o_uv[2 * (size_t)k + 1] = compat ? rf32(uv[1]) : uv[1];
-        L.pt_ptr[L.o_pt[k] + 1]++;
     }
-    for (int q = 0; q < np; ++q) L.pt_ptr[q + 1] += L.pt_ptr[q];
     L.pt_free.resize(np);
     for (int q = 0; q < np; ++q) {
         const int w = L.pt_win[q];
@@ -1187,6 +1189,16 @@ std::string check_layout(const HostLayout& L, const SchurItems& S) {
         if (L.o_cam[k] < 0 || L.o_cam[k] >= nc || L.o_pt[k] < 0 || L.o_pt[k] >= np) return bad("observation index");
         if (k < L.pt_ptr[L.o_pt[k]] || k >= L.pt_ptr[L.o_pt[k] + 1]) return bad("pt_ptr range");
         if (k > 0 && L.o_pt[k] == L.o_pt[k - 1] && L.o_cam[k] < L.o_cam[k - 1]) return bad("camera order in a point");
+        if (k > 0 && L.o_pt[k] == L.o_pt[k - 1] && L.o_cam[k] == L.o_cam[k - 1] && L.o_orig[k] < L.o_orig[k - 1])
+            return bad("input order among a point's observations on one camera");
+    }
+    {
+        std::vector<char> hit(no, 0);
+        for (int k = 0; k < no; ++k) {
+            const int i = L.o_orig[k];
+            if (i < 0 || i >= no || hit[i]) return bad("observation order is not a permutation");
+            hit[i] = 1;
+        }
     }
     for (size_t k = 0; k < L.cam_obs.size(); ++k) {
         const int a = L.cam_obs[k];
