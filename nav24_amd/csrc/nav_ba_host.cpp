@@ -459,24 +459,29 @@ struct SchurItems {
 int assign_lanes(const std::vector<std::vector<int2>>& bucket, int ns, int lanes, int grp, std::vector<int>& nlane) {
     nlane.assign(ns, 0);
     int used = 0, worst = 0;
-    std::priority_queue<std::pair<int, int>> pq;
+    // max-heap of (pairs per lane, -slot), kept in storage reused across calls
+    // (the same heap operations, hence the same ties, as a priority queue)
+    thread_local std::vector<std::pair<int, int>> pq;
+    pq.clear();
+    auto push = [&](std::pair<int, int> e) { pq.push_back(e); std::push_heap(pq.begin(), pq.end()); };
     for (int sl = 0; sl < ns; ++sl)
         if (!bucket[sl].empty()) {
             nlane[sl] = grp;
             used += grp;
             const int c = (int)bucket[sl].size();
-            pq.push({(c + grp - 1) / grp, -sl});
+            push({(c + grp - 1) / grp, -sl});
         }
     if (used > lanes) return 1 << 30;
-    while (used + grp <= lanes && !pq.empty() && pq.top().first > 1) {
-        const int sl = -pq.top().second;
-        pq.pop();
+    while (used + grp <= lanes && !pq.empty() && pq.front().first > 1) {
+        const int sl = -pq.front().second;
+        std::pop_heap(pq.begin(), pq.end());
+        pq.pop_back();
         nlane[sl] += grp;
         used += grp;
         const int c = (int)bucket[sl].size();
-        pq.push({(c + nlane[sl] - 1) / nlane[sl], -sl});
+        push({(c + nlane[sl] - 1) / nlane[sl], -sl});
     }
-    if (!pq.empty()) worst = pq.top().first;
+    if (!pq.empty()) worst = pq.front().first;
     return worst;
 }
 
